@@ -1,0 +1,164 @@
+/*
+ * lare3d_dev.h -- C-ABI of the MI355X device implementation of the Lare3d
+ * per-step hot path (Lagrangian predictor-corrector with edge shock
+ * viscosity, Van Leer x/y/z remap with constrained-transport face-B remap,
+ * RKL2 Braginskii conduction, boundary/halo fill, CFL timestep reduction,
+ * per-step energy diagnostics).
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and
+ * returns an int status: 0 = ok, < 0 = -(hipError_t) or a library error
+ * (see LARE_E_*), > 0 = a physics fault detected on the device.  The text
+ * of the last error is available from lare_dev_last_error().
+ *
+ * Array convention (the data contract of the reference): host arrays are
+ * Fortran column-major (x fastest) and are passed as a pointer to their
+ * first element, i.e. C_LOC(a) of the allocatable with its ghost bounds as
+ * allocated in mpi_routines.F90:295-334:
+ *   cell  (rho, energy, eta, visc_dep, ohmic_dep): (-1:nx+2, -1:ny+2, -1:nz+2)
+ *   node  (vx, vy, vz, vx1.., visc3)            : (-2:nx+2, -2:ny+2, -2:nz+2)
+ *   bx (-2:nx+2, -1:ny+2, -1:nz+2), by (-1:nx+2, -2:ny+2, -1:nz+2),
+ *   bz (-1:nx+2, -1:ny+2, -2:nz+2)
+ *   1-D metrics dxb(-1:nx+2), dxc(-1:nx+1) (same for y, z), grav(-1:nz+2).
+ *
+ * Threading: the caller is single threaded (one process per GPU).  The
+ * library is not re-entrant; all work is queued on one HIP stream.
+ */
+#ifndef LARE3D_DEV_H
+#define LARE3D_DEV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Boundary codes, as constants.f90 of the reference (shared_data.F90:65-66). */
+#define LARE_BC_PERIODIC 1
+#define LARE_BC_USER 2
+
+/* Precision modes of the bulk fields. */
+#define LARE_PREC_FP64 0
+#define LARE_PREC_FP32 1
+
+/* Library error codes (negative, disjoint from -hipError_t which is > -1000). */
+#define LARE_E_STATE   -1001 /* call order: not initialised / no fields     */
+#define LARE_E_ARG     -1002 /* invalid argument / unsupported option       */
+#define LARE_E_NOMEM   -1003 /* device allocation failed                    */
+/* Physics faults (positive). */
+#define LARE_F_NONFINITE 1   /* non-finite dt or energy after a step         */
+
+/* Grid: sizes, boundary codes and the 1-D metrics of setup.F90:92-247.
+ * Replaces the module state nx/ny/nz, x/y/zbc_*, dxb/dxc/dyb/dyc/dzb/dzc
+ * and grav of shared_data.F90:96-116,135-137. */
+typedef struct lare_grid {
+  int32_t nx, ny, nz;
+  int32_t xbc_min, xbc_max, ybc_min, ybc_max, zbc_min, zbc_max;
+  const double *dxb, *dxc; /* dxb(-1:nx+2), dxc(-1:nx+1) */
+  const double *dyb, *dyc;
+  const double *dzb, *dzc;
+  const double *grav;      /* grav(-1:nz+2); NULL means zero gravity */
+} lare_grid;
+
+/* Run parameters of control.f90 (control_variables) and normalise.f90.
+ * Only EOS_IDEAL is supported: xi_n is uniform (lare3d.f90:72) and the
+ * ionisation potential is 0 (normalise.f90:48). */
+typedef struct lare_params {
+  double gamma;            /* setup.F90:46 default 5/3                    */
+  double visc1, visc2;     /* shock viscosity coefficients               */
+  double eta_background, eta0, j_max; /* resistivity (lagran.F90:1176)  */
+  double kappa_0;          /* parallel conductivity (normalise.f90:54-55) */
+  double flux_limiter;     /* Larsen flux limiter (conduct.f90:90)        */
+  double xi_n;             /* neutral fraction, uniform (0 or 1)          */
+  double dt_from_restart;  /* used by the first set_dt when restart != 0  */
+  int32_t resistive_mhd;
+  int32_t conduction;
+  int32_t heat_flux_limiter;
+  int32_t use_viscous_damping;
+  int32_t restart;
+  int32_t precision;       /* LARE_PREC_FP64 or LARE_PREC_FP32            */
+} lare_params;
+
+/* Field pointers (Fortran arrays, see the array convention above).
+ * visc3 may be NULL when use_viscous_damping is 0; visc_dep/ohmic_dep/eta
+ * may be NULL on upload (then they start at zero). */
+typedef struct lare_fields {
+  double *rho, *energy;
+  double *vx, *vy, *vz;
+  double *bx, *by, *bz;
+  double *visc3;
+  double *eta, *visc_dep, *ohmic_dep;
+} lare_fields;
+
+/* Scalar state of shared_data.F90:126-139 that the step reads and writes. */
+typedef struct lare_scalars {
+  double dt, dtr, time;
+  double total_visc_heating, total_ohmic_heating;
+  int32_t n_s_stages;      /* last RKL2 stage count                       */
+  int32_t peak_substeps;
+  int32_t last_substeps;   /* resistive substeps of the last step         */
+  int32_t first_set_dt;    /* 1 until set_dt has run once (restart logic) */
+  int32_t step;            /* step counter of lare3d.f90:82 (Strang order) */
+} lare_scalars;
+
+/* Initialise the device state for one grid (allocates all device memory).
+ * Replaces the allocation part of mpi_initialise (mpi_routines.F90:295-334)
+ * and grid() for the device copy.  Re-initialising frees the old state. */
+int lare_dev_init(const lare_grid *grid, const lare_params *params);
+
+/* Copy host fields to the device (after set_initial_conditions). */
+int lare_dev_upload(const lare_fields *fields);
+
+/* Copy device fields to host arrays (dump steps only). */
+int lare_dev_download(lare_fields *fields);
+
+/* Scalar state access (dt, time, heating totals ...). */
+int lare_dev_get_scalars(lare_scalars *out);
+int lare_dev_set_scalars(const lare_scalars *in);
+
+/* lagran.F90:50 lagrangian_step (incl. set_dt :1070, conduct_heat, resistive
+ * substeps and the predictor/corrector). */
+int lare_dev_lagrangian_step(void);
+
+/* remap.f90:33 eulerian_remap(i) -- Strang-ordered x/y/z Van Leer remap. */
+int lare_dev_eulerian_remap(int32_t step);
+
+/* lagran.F90:1176 eta_calc. */
+int lare_dev_eta_calc(void);
+
+/* conduct.f90:39 conduct_heat -- one RKL2 super-step of length dt. */
+int lare_dev_conduct_heat(void);
+
+/* boundary.f90:48 boundary_conditions (B, energy, density, velocity). */
+int lare_dev_boundary_conditions(void);
+
+/* diagnostics.F90:517 energy_account: en[0] = magnetic, en[1] = kinetic,
+ * en[2] = internal (FP64 sums).  divb (may be NULL) receives max |div B|
+ * over interior cells, scaled by the cell width (a device diagnostic with no
+ * reference counterpart). */
+int lare_dev_energy_account(double en[3], double *divb);
+
+/* One full iteration of the lare3d.f90:82-88 loop body:
+ * lagrangian_step, eulerian_remap(step), eta_calc.  Returns dt and time. */
+int lare_dev_step(int32_t step, double *dt, double *time);
+
+/* Per-kernel timing with HIP events on the library stream (for roofline
+ * measurement).  enable != 0 starts recording; lare_dev_kernel_stats fills
+ * up to cap entries of (name, count, total_ms) and returns the number. */
+int lare_dev_profile(int32_t enable);
+int lare_dev_kernel_stats(char *names, int32_t name_len, int32_t *counts,
+                          double *total_ms, int32_t cap);
+
+/* Block until all queued device work is done. */
+int lare_dev_synchronize(void);
+
+/* Free all device memory. */
+int lare_dev_finalize(void);
+
+/* Text of the last error (static storage, never NULL). */
+const char *lare_dev_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LARE3D_DEV_H */

@@ -1,0 +1,111 @@
+"""ctypes view of the C-ABI declared in include/lare3d_dev.h.
+
+The structs mirror the header field for field.  The same structs describe
+the inputs of the device library and of the CPU oracle used by the tests.
+"""
+import ctypes as C
+import os
+
+BC_PERIODIC = 1
+BC_USER = 2
+PREC_FP64 = 0
+PREC_FP32 = 1
+
+_dp = C.POINTER(C.c_double)
+
+
+class LareGrid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("xbc_min", C.c_int32), ("xbc_max", C.c_int32),
+                ("ybc_min", C.c_int32), ("ybc_max", C.c_int32),
+                ("zbc_min", C.c_int32), ("zbc_max", C.c_int32),
+                ("dxb", _dp), ("dxc", _dp), ("dyb", _dp), ("dyc", _dp),
+                ("dzb", _dp), ("dzc", _dp), ("grav", _dp)]
+
+
+class LareParams(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("visc1", C.c_double), ("visc2", C.c_double),
+                ("eta_background", C.c_double), ("eta0", C.c_double), ("j_max", C.c_double),
+                ("kappa_0", C.c_double), ("flux_limiter", C.c_double), ("xi_n", C.c_double),
+                ("dt_from_restart", C.c_double),
+                ("resistive_mhd", C.c_int32), ("conduction", C.c_int32),
+                ("heat_flux_limiter", C.c_int32), ("use_viscous_damping", C.c_int32),
+                ("restart", C.c_int32), ("precision", C.c_int32)]
+
+
+class LareFields(C.Structure):
+    _fields_ = [(n, _dp) for n in ("rho", "energy", "vx", "vy", "vz", "bx", "by", "bz",
+                                   "visc3", "eta", "visc_dep", "ohmic_dep")]
+
+
+class LareScalars(C.Structure):
+    _fields_ = [("dt", C.c_double), ("dtr", C.c_double), ("time", C.c_double),
+                ("total_visc_heating", C.c_double), ("total_ohmic_heating", C.c_double),
+                ("n_s_stages", C.c_int32), ("peak_substeps", C.c_int32),
+                ("last_substeps", C.c_int32), ("first_set_dt", C.c_int32),
+                ("step", C.c_int32)]
+
+
+def dptr(a):
+    """Pointer to the first element of a float64 numpy array (or NULL)."""
+    if a is None:
+        return C.cast(None, _dp)
+    assert a.dtype.name == "float64" and (a.flags.f_contiguous or a.ndim == 1)
+    return a.ctypes.data_as(_dp)
+
+
+# Exported symbols of the device library, with argtypes.
+DEVICE_SYMBOLS = {
+    "lare_dev_init": [C.POINTER(LareGrid), C.POINTER(LareParams)],
+    "lare_dev_upload": [C.POINTER(LareFields)],
+    "lare_dev_download": [C.POINTER(LareFields)],
+    "lare_dev_get_scalars": [C.POINTER(LareScalars)],
+    "lare_dev_set_scalars": [C.POINTER(LareScalars)],
+    "lare_dev_lagrangian_step": [],
+    "lare_dev_eulerian_remap": [C.c_int32],
+    "lare_dev_eta_calc": [],
+    "lare_dev_conduct_heat": [],
+    "lare_dev_boundary_conditions": [],
+    "lare_dev_energy_account": [_dp, _dp],
+    "lare_dev_step": [C.c_int32, _dp, _dp],
+    "lare_dev_profile": [C.c_int32],
+    "lare_dev_kernel_stats": [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), _dp, C.c_int32],
+    "lare_dev_synchronize": [],
+    "lare_dev_finalize": [],
+    "lare_dev_last_error": [],
+}
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "liblare3d_dev.so")
+
+
+class DeviceError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path=None):
+    """Load the device library built in-tree.  Raises if it is missing:
+    there is no CPU fallback for the product path."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise DeviceError(f"device library not built: {p} (run __graft_entry__.build())")
+    lib = C.CDLL(p)
+    for name, args in DEVICE_SYMBOLS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_char_p if name == "lare_dev_last_error" else C.c_int
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(lib, rc, what):
+    if rc != 0:
+        msg = lib.lare_dev_last_error().decode(errors="replace")
+        raise DeviceError(f"{what} failed with status {rc}: {msg}")

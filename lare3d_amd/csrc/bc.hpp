@@ -1,0 +1,101 @@
+// bc.hpp -- boundary / halo fill on one GPU.
+//
+// Single-rank semantics of boundary.f90 + mpiboundary.f90: on a periodic
+// axis every MPI_SENDRECV is a self-exchange of 2- or 3-layer slabs that
+// span the full extent of the other two axes (so corners wrap); on a
+// BC_USER axis the MPI exchange is a no-op and the user mirrors / zero
+// walls of boundary.f90 apply.  The reference runs all exchanges (x, y, z)
+// before all user fills (x, y, z); the host launches one kernel per axis in
+// that order.  One launch handles up to 8 fields so that a whole
+// boundary_conditions call is at most three launches.
+#pragma once
+#include "common.hpp"
+
+enum BcOp {
+  BC_NONE = 0,
+  BC_PER_CELL,   // (n+1:n+2) <- (1:2), (-1:0) <- (n-1:n)      cell_xface
+  BC_PER_FACE,   // (n+1:n+2) <- (1:2), (-2:0) <- (n-2:n)      bx_xface / bx_xface1
+  BC_PER_NODE,   // (n:n+2) <- (0:2), (-2:-1) <- (n-2:n-1)     node_xface1 / node_xface
+  BC_MIR_CELL,   // 0 <- 1, -1 <- 2, n+1 <- n, n+2 <- n-1     boundary.f90:134-160 etc.
+  BC_MIR_FACE,   // -1 <- 1, -2 <- 2, n+1 <- n-1, n+2 <- n-2  boundary.f90:68-69,77-78
+  BC_ZERO_NODE,  // -2..0 = 0, n..n+2 = 0                      boundary.f90:260-292
+  BC_DM_CLAMP    // dm(n+1) <- dm(n), dm(-1) <- dm(0) over transverse 0..n+1 (xremap.f90:906-915)
+};
+
+#define BC_MAXF 8
+struct BcSet {
+  double *f[BC_MAXF];
+  int op[BC_MAXF];
+  int nf;
+};
+
+__global__ void k_bc_axis(Geo g, BcSet s, int axis) {
+  // plane coordinates: u runs along the contiguous axis where possible
+  const int u = axis == 0 ? 1 : 0;
+  const int w = axis == 2 ? 1 : 2;
+  const int iu = -2 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int iw = -2 + (int)blockIdx.y;
+  if (iu > g.n(u) + 2) return;
+  const int n = g.n(axis);
+  const int st = g.stride(axis);
+  int c[3];
+  c[axis] = 0;
+  c[u] = iu;
+  c[w] = iw;
+  const int base = g.idx(c[0], c[1], c[2]);  // index of layer 0 on this line
+  for (int q = 0; q < s.nf; ++q) {
+    double *a = s.f[q];
+    switch (s.op[q]) {
+      case BC_PER_CELL:
+        a[base + (n + 1) * st] = a[base + 1 * st];
+        a[base + (n + 2) * st] = a[base + 2 * st];
+        a[base - 1 * st] = a[base + (n - 1) * st];
+        a[base + 0 * st] = a[base + n * st];
+        break;
+      case BC_PER_FACE:
+        a[base + (n + 1) * st] = a[base + 1 * st];
+        a[base + (n + 2) * st] = a[base + 2 * st];
+        a[base - 2 * st] = a[base + (n - 2) * st];
+        a[base - 1 * st] = a[base + (n - 1) * st];
+        a[base + 0 * st] = a[base + n * st];
+        break;
+      case BC_PER_NODE: {
+        const double v0 = a[base], v1 = a[base + st], v2 = a[base + 2 * st];
+        a[base + n * st] = v0;
+        a[base + (n + 1) * st] = v1;
+        a[base + (n + 2) * st] = v2;
+        a[base - 2 * st] = a[base + (n - 2) * st];
+        a[base - 1 * st] = a[base + (n - 1) * st];
+        break;
+      }
+      case BC_MIR_CELL:
+        a[base + 0 * st] = a[base + 1 * st];
+        a[base - 1 * st] = a[base + 2 * st];
+        a[base + (n + 1) * st] = a[base + n * st];
+        a[base + (n + 2) * st] = a[base + (n - 1) * st];
+        break;
+      case BC_MIR_FACE:
+        a[base - 1 * st] = a[base + 1 * st];
+        a[base - 2 * st] = a[base + 2 * st];
+        a[base + (n + 1) * st] = a[base + (n - 1) * st];
+        a[base + (n + 2) * st] = a[base + (n - 2) * st];
+        break;
+      case BC_ZERO_NODE:
+        a[base - 2 * st] = 0.0;
+        a[base - 1 * st] = 0.0;
+        a[base + 0 * st] = 0.0;
+        a[base + n * st] = 0.0;
+        a[base + (n + 1) * st] = 0.0;
+        a[base + (n + 2) * st] = 0.0;
+        break;
+      case BC_DM_CLAMP:
+        if (iu >= 0 && iu <= g.n(u) + 1 && iw >= 0 && iw <= g.n(w) + 1) {
+          a[base + (n + 1) * st] = a[base + n * st];
+          a[base - 1 * st] = a[base + 0 * st];
+        }
+        break;
+      default:
+        break;
+    }
+  }
+}

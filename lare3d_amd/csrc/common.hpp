@@ -1,0 +1,99 @@
+// common.hpp -- device layout and launch helpers shared by all kernels.
+//
+// Every 3-D field lives in one padded box covering Fortran indices
+// -2..n+2 on each axis (the union of the cell, node and face bounds of
+// mpi_routines.F90:295-334), x fastest.  Index (i, j, k) maps to
+//     (k + 2) * sz + (j + 2) * sy + (i + OX)
+// with OX = 16 so that i = 0 sits on a 128-byte boundary for FP64 rows and
+// the row pitch sy is a multiple of 16 elements.  All fields share the
+// same index arithmetic, which lets one kernel address cell, node and face
+// arrays with one offset.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define OX 16
+
+struct Geo {
+  int nx, ny, nz;
+  int sy, sz;
+  // 1-D metrics, indexable by Fortran index (pointer already offset)
+  const double *dxb, *dxc, *dyb, *dyc, *dzb, *dzc, *grav;
+  int per[3];     // axis periodic (both sides)
+  int user[3];    // axis BC_USER (both sides)
+  __host__ __device__ __forceinline__ int idx(int i, int j, int k) const {
+    return (k + 2) * sz + (j + 2) * sy + (i + OX);
+  }
+  __host__ __device__ __forceinline__ int n(int a) const { return a == 0 ? nx : (a == 1 ? ny : nz); }
+  __host__ __device__ __forceinline__ int stride(int a) const { return a == 0 ? 1 : (a == 1 ? sy : sz); }
+  __device__ __forceinline__ double db(int a, int i) const { return a == 0 ? dxb[i] : (a == 1 ? dyb[i] : dzb[i]); }
+  __device__ __forceinline__ double dc(int a, int i) const { return a == 0 ? dxc[i] : (a == 1 ? dyc[i] : dzc[i]); }
+  // cell volume of setup.F90:242-246, recomputed from the metrics
+  __device__ __forceinline__ double cv(int i, int j, int k) const { return dxb[i] * dyb[j] * dzb[k]; }
+};
+
+// Index box [lo, hi] inclusive per axis.
+struct Box {
+  int x0, x1, y0, y1, z0, z1;
+};
+
+#define BX 64
+#define BY 4
+
+// Thread (i, j, k) of a box launch; returns false outside the box.
+#define BOX_INDEX(b, i, j, k)                              \
+  const int i = (b).x0 + (int)(blockIdx.x * BX + threadIdx.x); \
+  const int j = (b).y0 + (int)(blockIdx.y * BY + threadIdx.y); \
+  const int k = (b).z0 + (int)blockIdx.z;                     \
+  if (i > (b).x1 || j > (b).y1 || k > (b).z1) return;
+
+static inline dim3 box_grid(const Box &b) {
+  return dim3((unsigned)((b.x1 - b.x0 + 1 + BX - 1) / BX), (unsigned)((b.y1 - b.y0 + 1 + BY - 1) / BY),
+              (unsigned)(b.z1 - b.z0 + 1));
+}
+static inline dim3 box_block() { return dim3(BX, BY, 1); }
+static inline Box mkbox(int x0, int x1, int y0, int y1, int z0, int z1) {
+  Box b = {x0, x1, y0, y1, z0, z1};
+  return b;
+}
+
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+
+// Fortran MAX(x, 0.0_num) as used on energy and visc_heat.
+__device__ __forceinline__ double max0(double x) { return x > 0.0 ? x : 0.0; }
+
+// Wavefront (64-lane) reductions.
+__device__ __forceinline__ double wave_min(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = dmin(v, __shfl_down(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Block-level reduction of NV values into partial[blockid * NV + q].
+// op 0 = min, 1 = sum.  Block size BX*BY = 256 (4 waves).
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NV], const int *ops, double *partial) {
+  __shared__ double red[4][NV];
+  const int lane = threadIdx.x & 63;
+  const int wave = (threadIdx.y * BX + threadIdx.x) >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = ops[q] == 0 ? wave_min(v[q]) : wave_sum(v[q]);
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) red[wave][q] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const int bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double r = red[0][q];
+      for (int w = 1; w < 4; ++w) r = ops[q] == 0 ? dmin(r, red[w][q]) : r + red[w][q];
+      partial[(size_t)bid * NV + q] = r;
+    }
+  }
+}

@@ -1,0 +1,254 @@
+// conduct.hpp -- RKL2 super-stepped Braginskii conduction (conduct.f90).
+//
+// Device form of heat_conduct_sts2 (conduct.f90:299-416):
+//  * heat_flux (conduct.f90:152-291) scatters each face flux into both
+//    neighbouring cells; here every thread gathers the six face fluxes of
+//    its own cell and sums them in the reference's accumulation order
+//    ((((fz_in + fy_in) + fx_in) - fx_out) - fy_out) - fz_out, so no
+//    atomics are needed and the result is the scatter's sum.
+//  * temperature is never materialised: each stage derives T from the
+//    stage input on the fly.  Ghost temperatures follow the reference's
+//    boundary behaviour without a halo pass: on BC_USER axes the mirror of
+//    temperature_bcs is an index map; on periodic axes stage 2 reads the
+//    pre-conduction energy ghosts (temperature_bcs exchanges rho, not T,
+//    boundary.f90:214, and only the interior of Y(2) was refreshed,
+//    conduct.f90:366), while stages >= 3 read the periodic image (the
+//    energy_bcs at conduct.f90:403).
+//  * the Y(0:3) buffer copies of conduct.f90:400-406 become a rotation of
+//    three device buffers.
+#pragma once
+#include "common.hpp"
+
+#define MIN_B 1.0e-10
+
+// Ghost-read modes of the stage kernels.
+#define TMODE_DIRECT 0   // read the array as stored (stage 1: Y0 = energy)
+#define TMODE_STALE 1    // stage 2: user axes mirrored, periodic ghosts from E0
+#define TMODE_WRAP 2     // stages >= 3: user axes mirrored, periodic axes wrapped
+
+__device__ __forceinline__ int mirror_idx(int c, int n) {
+  return c < 1 ? 1 - c : 2 * n + 1 - c;  // 0->1, -1->2, n+1->n, n+2->n-1
+}
+
+// Energy-like value at (i,j,k) under a ghost-read mode.
+__device__ __forceinline__ double ghost_read(const Geo &g, const double *__restrict__ Y,
+                                             const double *__restrict__ E0, int i, int j, int k, int mode) {
+  if (mode == TMODE_DIRECT) return Y[g.idx(i, j, k)];
+  int c[3] = {i, j, k};
+  bool stale = false;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int n = g.n(a);
+    if (c[a] < 1 || c[a] > n) {
+      if (g.user[a]) c[a] = mirror_idx(c[a], n);
+      else if (mode == TMODE_STALE) stale = true;
+      else c[a] = c[a] < 1 ? c[a] + n : c[a] - n;
+    }
+  }
+  return (stale ? E0 : Y)[g.idx(c[0], c[1], c[2])];
+}
+
+__device__ __forceinline__ double pow25(double t) { return pow(t, 2.5); }
+
+struct CondArgs {
+  double kappa_0, tfac, dt;
+};
+
+// Larsen flux-limiter factor over cells 0..n and the explicit parabolic
+// time-step minimum over 1..n (calc_s_stages, conduct.f90:69-129).
+// Temperatures from energy as stored (ghosts valid at call time).
+__global__ void k_larsen(Geo g, const double *__restrict__ energy, const double *__restrict__ rho,
+                         double *__restrict__ larsen, double *__restrict__ partial, double flux_limiter,
+                         double kappa_0, double tfac, int limiter_on) {
+  const int i = (int)(blockIdx.x * BX + threadIdx.x);
+  const int j = (int)(blockIdx.y * BY + threadIdx.y);
+  const int k = (int)blockIdx.z;
+  double v[1] = {1.0e300};
+  const bool in = i <= g.nx && j <= g.ny && k <= g.nz;
+  if (in) {
+    const int c = g.idx(i, j, k);
+    const double T = tfac * energy[c];
+    double L = 1.0;
+    if (limiter_on) {
+      const double temp = pow(T, 2.5);
+      const double q_fs = flux_limiter * 42.85 * rho[c] * pow(T, 1.5);
+      const double q_fs2 = q_fs * q_fs;
+      const double q_spx = -kappa_0 * temp * (tfac * energy[c + 1] - tfac * energy[c - 1]) * 0.5 / g.dxb[i];
+      const double q_spy = -kappa_0 * temp * (tfac * energy[c + g.sy] - tfac * energy[c - g.sy]) * 0.5 / g.dyb[j];
+      const double q_spz = -kappa_0 * temp * (tfac * energy[c + g.sz] - tfac * energy[c - g.sz]) * 0.5 / g.dzb[k];
+      const double q_sp2 = q_spx * q_spx + q_spy * q_spy + q_spz * q_spz;
+      L = q_fs / sqrt(q_fs2 + q_sp2);
+    }
+    larsen[c] = L;
+    if (i >= 1 && j >= 1 && k >= 1) {
+      const double kappa1 = kappa_0 * L;
+      const double tp = pow(T, 2.5);
+      const double r = rho[c];
+      const double dx = g.dxb[i], dy = g.dyb[j], dz = g.dzb[k];
+      const double dt1 = r * (dx * dx) / (2.0 * kappa1 * tp);
+      const double dt2 = r * (dy * dy) / (2.0 * kappa1 * tp);
+      const double dt3 = r * (dz * dz) / (2.0 * kappa1 * tp);
+      v[0] = dmin(dt1, dmin(dt2, dt3));
+    }
+  }
+  const int ops[1] = {0};
+  block_reduce_store<1>(v, ops, partial);
+}
+
+// Face flux fc_sp of conduct.f90:176-207 (x face between cells ix, ix+1),
+// given the temperature neighbourhood T[a][b][c] with a,b,c in {0,1,2}
+// meaning offsets -1,0,+1 around the *lower* cell... see callers.
+// The three face kinds are written out separately, in the reference's
+// operation order.
+
+struct Tn {  // 3x3x3 temperature neighbourhood of a cell (corners unused)
+  double t[3][3][3];
+};
+
+// x face between (ox,oy,oz)+(0,0,0) and +(1,0,0) relative to centre index
+// (1,1,1) of the neighbourhood, lower cell at neighbourhood x index lx.
+__device__ __forceinline__ double flux_x_face(const Geo &g, const Tn &T, int lx, int ix, int iy, int iz,
+                                              const double *__restrict__ bx, const double *__restrict__ by,
+                                              const double *__restrict__ bz, double L, double kappa_0) {
+  const int c = g.idx(ix, iy, iz);
+  const double byf = 0.25 * (by[c] + by[c + 1] + by[c - g.sy] + by[c + 1 - g.sy]);
+  const double bzf = 0.25 * (bz[c] + bz[c + 1] + bz[c - g.sz] + bz[c + 1 - g.sz]);
+  const double bxc = bx[c];
+  const double modb = bxc * bxc + byf * byf + bzf * bzf + MIN_B;
+  double tb = (T.t[lx + 1][1][1] + T.t[lx][1][1]) / 2.0;
+  double tb_p = (T.t[lx + 1][2][1] + T.t[lx][2][1]) / 2.0;
+  double tb_m = (T.t[lx + 1][0][1] + T.t[lx][0][1]) / 2.0;
+  const double tg_a1 = (tb_p - tb_m) / (g.dyc[iy] + g.dyc[iy - 1]);
+  tb_p = (T.t[lx + 1][1][2] + T.t[lx][1][2]) / 2.0;
+  tb_m = (T.t[lx + 1][1][0] + T.t[lx][1][0]) / 2.0;
+  const double tg_a2 = (tb_p - tb_m) / (g.dzc[iz] + g.dzc[iz - 1]);
+  const double tg = (T.t[lx + 1][1][1] - T.t[lx][1][1]) / g.dxc[ix];
+  tb = dmax(tb, 0.0);
+  return -L * kappa_0 * pow25(tb) * (bxc * (tg * bxc + tg_a1 * byf + tg_a2 * bzf) + tg * MIN_B) / modb;
+}
+
+__device__ __forceinline__ double flux_y_face(const Geo &g, const Tn &T, int ly, int ix, int iy, int iz,
+                                              const double *__restrict__ bx, const double *__restrict__ by,
+                                              const double *__restrict__ bz, double L, double kappa_0) {
+  const int c = g.idx(ix, iy, iz);
+  const double bxf = 0.25 * (bx[c] + bx[c + g.sy] + bx[c - 1] + bx[c - 1 + g.sy]);
+  const double bzf = 0.25 * (bz[c] + bz[c + g.sy] + bz[c - g.sz] + bz[c + g.sy - g.sz]);
+  const double byc = by[c];
+  const double modb = bxf * bxf + byc * byc + bzf * bzf + MIN_B;
+  double tb = (T.t[1][ly + 1][1] + T.t[1][ly][1]) / 2.0;
+  double tb_p = (T.t[2][ly][1] + T.t[2][ly + 1][1]) / 2.0;
+  double tb_m = (T.t[0][ly + 1][1] + T.t[0][ly][1]) / 2.0;
+  const double tg_a1 = (tb_p - tb_m) / (g.dxc[ix] + g.dxc[ix - 1]);
+  tb_p = (T.t[1][ly + 1][2] + T.t[1][ly][2]) / 2.0;
+  tb_m = (T.t[1][ly + 1][0] + T.t[1][ly][0]) / 2.0;
+  const double tg_a2 = (tb_p - tb_m) / (g.dzc[iz] + g.dzc[iz - 1]);
+  const double tg = (T.t[1][ly + 1][1] - T.t[1][ly][1]) / g.dyc[iy];
+  tb = dmax(tb, 0.0);
+  return -L * kappa_0 * pow25(tb) * (byc * (tg * byc + tg_a1 * bxf + tg_a2 * bzf) + tg * MIN_B) / modb;
+}
+
+__device__ __forceinline__ double flux_z_face(const Geo &g, const Tn &T, int lz, int ix, int iy, int iz,
+                                              const double *__restrict__ bx, const double *__restrict__ by,
+                                              const double *__restrict__ bz, double L, double kappa_0) {
+  const int c = g.idx(ix, iy, iz);
+  const double bxf = 0.25 * (bx[c] + bx[c + g.sz] + bx[c - 1] + bx[c - 1 + g.sz]);
+  const double byf = 0.25 * (by[c] + by[c + g.sz] + by[c - g.sy] + by[c - g.sy + g.sz]);
+  const double bzc = bz[c];
+  const double modb = bxf * bxf + byf * byf + bzc * bzc + MIN_B;
+  double tb = (T.t[1][1][lz + 1] + T.t[1][1][lz]) / 2.0;
+  double tb_p = (T.t[2][1][lz + 1] + T.t[2][1][lz]) / 2.0;
+  double tb_m = (T.t[0][1][lz + 1] + T.t[0][1][lz]) / 2.0;
+  const double tg_a1 = (tb_p - tb_m) / (g.dxc[ix] + g.dxc[ix - 1]);
+  tb_p = (T.t[1][2][lz + 1] + T.t[1][2][lz]) / 2.0;
+  tb_m = (T.t[1][0][lz + 1] + T.t[1][0][lz]) / 2.0;
+  const double tg_a2 = (tb_p - tb_m) / (g.dyc[iy] + g.dyc[iy - 1]);
+  const double tg = (T.t[1][1][lz + 1] - T.t[1][1][lz]) / g.dzc[iz];
+  tb = dmax(tb, 0.0);
+  return -L * kappa_0 * pow25(tb) * (bzc * (tg * bzc + tg_a1 * bxf + tg_a2 * byf) + tg * MIN_B) / modb;
+}
+
+// Net heat flux into cell (i,j,k) (the value heat_flux leaves in flux(i,j,k)).
+__device__ __forceinline__ double cell_heat_flux(const Geo &g, const double *__restrict__ Y,
+                                                 const double *__restrict__ E0, int mode, double tfac,
+                                                 const double *__restrict__ bx, const double *__restrict__ by,
+                                                 const double *__restrict__ bz, const double *__restrict__ larsen,
+                                                 double kappa_0, int i, int j, int k) {
+  Tn T;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int far = (a != 1) + (b != 1) + (c != 1);
+        T.t[a][b][c] = far == 3 ? 0.0 : tfac * ghost_read(g, Y, E0, i + a - 1, j + b - 1, k + c - 1, mode);
+      }
+  const int c0 = g.idx(i, j, k);
+  // faces in: from (i,j,k-1) z, (i,j-1,k) y, (i-1,j,k) x; faces out: own x, y, z
+  // For the z face of the lower neighbour we need T around (i,j,k-1); the
+  // face lies between neighbourhood z indices 0 and 1 (lz = 0).
+  const double fz_in = flux_z_face(g, T, 0, i, j, k - 1, bx, by, bz, larsen[c0 - g.sz], kappa_0);
+  const double fy_in = flux_y_face(g, T, 0, i, j - 1, k, bx, by, bz, larsen[c0 - g.sy], kappa_0);
+  const double fx_in = flux_x_face(g, T, 0, i - 1, j, k, bx, by, bz, larsen[c0 - 1], kappa_0);
+  const double L0 = larsen[c0];
+  const double fx_out = flux_x_face(g, T, 1, i, j, k, bx, by, bz, L0, kappa_0);
+  const double fy_out = flux_y_face(g, T, 1, i, j, k, bx, by, bz, L0, kappa_0);
+  const double fz_out = flux_z_face(g, T, 1, i, j, k, bx, by, bz, L0, kappa_0);
+  double f = 0.0 + fz_in / g.dzb[k - 1];
+  f = f + fy_in / g.dyb[j - 1];
+  f = f + fx_in / g.dxb[i - 1];
+  f = f - fx_out / g.dxb[i];
+  f = f - fy_out / g.dyb[j];
+  f = f - fz_out / g.dzb[k];
+  return f;
+}
+
+// Stage 1 (conduct.f90:345-367): Lc = flux/rho, Y1 = Y0 + mu~1 dt Lc.
+__global__ void k_rkl2_stage1(Geo g, Box b, const double *__restrict__ E0, const double *__restrict__ rho,
+                              const double *__restrict__ bx, const double *__restrict__ by,
+                              const double *__restrict__ bz, const double *__restrict__ larsen,
+                              double *__restrict__ Lc, double *__restrict__ Yout, CondArgs a, double mu_t1) {
+  BOX_INDEX(b, i, j, k);
+  const double f = cell_heat_flux(g, E0, E0, TMODE_DIRECT, a.tfac, bx, by, bz, larsen, a.kappa_0, i, j, k);
+  const int c = g.idx(i, j, k);
+  const double lc = (1.0 / rho[c]) * f;
+  Lc[c] = lc;
+  const double c0 = mu_t1 * a.dt * lc;
+  Yout[c] = E0[c] + c0;
+}
+
+// Stage j >= 2 (conduct.f90:369-396).
+__global__ void k_rkl2_stage(Geo g, Box b, const double *__restrict__ Y2, const double *__restrict__ Y1,
+                             const double *__restrict__ E0, const double *__restrict__ rho,
+                             const double *__restrict__ bx, const double *__restrict__ by,
+                             const double *__restrict__ bz, const double *__restrict__ larsen,
+                             const double *__restrict__ Lc, double *__restrict__ Yout, CondArgs a, int mode,
+                             double mu, double nu, double one_m_mu_nu, double mu_t, double gam_t) {
+  BOX_INDEX(b, i, j, k);
+  const double f = cell_heat_flux(g, Y2, E0, mode, a.tfac, bx, by, bz, larsen, a.kappa_0, i, j, k);
+  const int c = g.idx(i, j, k);
+  const double c0 = gam_t * a.dt * Lc[c];
+  const double lj = (1.0 / rho[c]) * f;
+  const double c1 = mu_t * a.dt * lj;
+  double y3 = mu * Y2[c] + nu * Y1[c] + one_m_mu_nu * E0[c];
+  y3 = y3 + c1 + c0;
+  Yout[c] = y3;
+}
+
+// Final write of conduct_heat (conduct.f90:409-410): energy interior from
+// the last stage, ghosts by energy_bcs (index map).  delta_mode != 0 applies
+// lagrangian_step's energy0 + (energy - energy0) and MAX(., 0)
+// (lagran.F90:168-181) over the whole cell box.
+__global__ void k_rkl2_final(Geo g, Box b, const double *__restrict__ Ys, double *__restrict__ energy,
+                             int delta_mode) {
+  BOX_INDEX(b, i, j, k);
+  const double v = ghost_read(g, Ys, Ys, i, j, k, TMODE_WRAP);
+  const int c = g.idx(i, j, k);
+  if (delta_mode) {
+    const double e0 = energy[c];
+    const double d = v - e0;
+    energy[c] = max0(e0 + d);
+  } else {
+    energy[c] = v;
+  }
+}

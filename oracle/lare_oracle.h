@@ -1,0 +1,50 @@
+/*
+ * lare_oracle.h -- CPU restatement of the Lare3d per-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the device
+ * implementation: only tests/, __graft_entry__.smoke() and the CPU-baseline
+ * leg of bench.py may load it.  It is never part of the product path.
+ *
+ * Every routine restates the reference Fortran (file:line cited at each
+ * function in lare_oracle.c) with the reference's loop ranges, loop order
+ * and operation order, in FP64.  Arrays use the reference's index bounds;
+ * storage is a common (-2:n+2)^3 box per array, and every element outside
+ * the reference's allocated bounds of that array holds NaN, so that a read
+ * outside the reference's bounds shows up as a non-finite result.
+ *
+ * The structs of include/lare3d_dev.h describe grid, parameters, fields and
+ * scalars, so that a test hands the same inputs to the oracle and the device.
+ */
+#ifndef LARE_ORACLE_H
+#define LARE_ORACLE_H
+
+#include "../include/lare3d_dev.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_state orc_state;
+
+orc_state *orc_create(const lare_grid *grid, const lare_params *params);
+void orc_destroy(orc_state *o);
+int orc_upload(orc_state *o, const lare_fields *f);
+int orc_download(orc_state *o, lare_fields *f);
+int orc_get_scalars(orc_state *o, lare_scalars *s);
+int orc_set_scalars(orc_state *o, const lare_scalars *s);
+
+void orc_lagrangian_step(orc_state *o);
+void orc_eulerian_remap(orc_state *o, int step);
+void orc_eta_calc(orc_state *o);
+void orc_conduct_heat(orc_state *o);
+void orc_boundary_conditions(orc_state *o);
+void orc_energy_account(orc_state *o, double en[3]);
+/* max over interior cells of |div B| * min(dxb,dyb,dzb) (device diagnostic) */
+double orc_divb(orc_state *o);
+void orc_step(orc_state *o, int step, double *dt, double *time);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,122 @@
+"""ctypes wrapper of the CPU restatement (liblare_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the device implementation.  Only
+tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+from lare3d_amd.abi import LareFields, LareGrid, LareParams, LareScalars
+from lare3d_amd.fields import Fields
+from lare3d_amd.grid import grid_struct
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liblare_oracle.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def _load():
+    if not os.path.exists(LIB):
+        build()
+    lib = C.CDLL(LIB)
+    vp = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    sig = {
+        "orc_create": ([C.POINTER(LareGrid), C.POINTER(LareParams)], vp),
+        "orc_destroy": ([vp], None),
+        "orc_upload": ([vp, C.POINTER(LareFields)], C.c_int),
+        "orc_download": ([vp, C.POINTER(LareFields)], C.c_int),
+        "orc_get_scalars": ([vp, C.POINTER(LareScalars)], C.c_int),
+        "orc_set_scalars": ([vp, C.POINTER(LareScalars)], C.c_int),
+        "orc_lagrangian_step": ([vp], None),
+        "orc_eulerian_remap": ([vp, C.c_int], None),
+        "orc_eta_calc": ([vp], None),
+        "orc_conduct_heat": ([vp], None),
+        "orc_boundary_conditions": ([vp], None),
+        "orc_energy_account": ([vp, dp], None),
+        "orc_divb": ([vp], C.c_double),
+        "orc_step": ([vp, C.c_int, dp, dp], None),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+class Oracle:
+    """Same call surface as lare3d_amd.device.Device, on the CPU."""
+
+    def __init__(self, grid, params, fields):
+        self.grid = grid
+        self.n = grid.n
+        self._g = grid_struct(grid)
+        self._p = params
+        self.h = lib().orc_create(C.byref(self._g), C.byref(self._p))
+        self._f = fields.copy()
+        lib().orc_upload(self.h, C.byref(self._f.to_struct()))
+
+    def close(self):
+        if self.h:
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def fields(self):
+        f = Fields(*self.n)
+        lib().orc_download(self.h, C.byref(f.to_struct(with_visc3=False)))
+        return f
+
+    def scalars(self):
+        s = LareScalars()
+        lib().orc_get_scalars(self.h, C.byref(s))
+        return s
+
+    def set_scalars(self, **kw):
+        s = self.scalars()
+        for k, v in kw.items():
+            setattr(s, k, v)
+        lib().orc_set_scalars(self.h, C.byref(s))
+
+    def lagrangian_step(self):
+        lib().orc_lagrangian_step(self.h)
+
+    def eulerian_remap(self, step):
+        lib().orc_eulerian_remap(self.h, step)
+
+    def eta_calc(self):
+        lib().orc_eta_calc(self.h)
+
+    def conduct_heat(self):
+        lib().orc_conduct_heat(self.h)
+
+    def boundary_conditions(self):
+        lib().orc_boundary_conditions(self.h)
+
+    def energy_account(self):
+        en = (C.c_double * 3)()
+        lib().orc_energy_account(self.h, en)
+        return tuple(en)
+
+    def divb(self):
+        return lib().orc_divb(self.h)
+
+    def step(self, step):
+        dt, t = C.c_double(), C.c_double()
+        lib().orc_step(self.h, step, C.byref(dt), C.byref(t))
+        return dt.value, t.value

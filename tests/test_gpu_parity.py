@@ -1,0 +1,92 @@
+"""Device (HIP, through the C-ABI) versus the CPU restatement, FP64.
+
+Bar: relative L2 <= 1e-10 per field per step (the FP64 parity bar), and
+bit-exact for pure copies (boundary fills).
+"""
+import numpy as np
+import pytest
+
+from lare3d_amd import BC_PERIODIC, BC_USER, configs
+from tests.util import STATE, field_errors, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _pair(cfg):
+    from lare3d_amd.device import Device
+    from oracle.oracle import Oracle
+    grid, params, fields = cfg[:3]
+    print(f"\n[case] n={grid.n} bcs={grid.bcs}", flush=True)
+    return Device(grid, params, fields), Oracle(grid, params, fields)
+
+
+def _init(dev, orc):
+    for d in (dev, orc):
+        d.boundary_conditions()
+        d.eta_calc()
+
+
+@pytest.mark.parametrize("bcs", [(BC_PERIODIC,) * 6, (BC_USER,) * 6,
+                                 (BC_PERIODIC, BC_PERIODIC, BC_USER, BC_USER, BC_PERIODIC, BC_PERIODIC)])
+def test_boundary_conditions_bit_exact(bcs):
+    grid, params, f = configs.uniform(6, bcs=bcs)
+    rng = np.random.default_rng(7)
+    for name in STATE:
+        a = getattr(f, name)
+        a[...] = rng.standard_normal(a.shape)
+    dev, orc = _pair((grid, params, f))
+    dev.boundary_conditions()
+    orc.boundary_conditions()
+    fd, fo = dev.fields(), orc.fields()
+    for name in STATE:
+        assert np.array_equal(getattr(fd, name), getattr(fo, name)), name
+    dev.close()
+
+
+def test_conduct_heat_arcade():
+    grid, params, f, dt = configs.arcade_conduction(16, 16, 32, s_target=9)
+    dev, orc = _pair((grid, params, f))
+    _init(dev, orc)
+    for d in (dev, orc):
+        d.set_scalars(dt=dt)
+    for it in range(3):
+        dev.conduct_heat()
+        orc.conduct_heat()
+        sd, so = dev.scalars(), orc.scalars()
+        assert sd.n_s_stages == so.n_s_stages and sd.n_s_stages > 1
+        e = rel_l2(dev.fields().energy, orc.fields().energy)
+        print(f"super-step {it}: s={sd.n_s_stages} rel L2(energy incl. ghosts) = {e:.3e}")
+        assert e <= TOL
+    dev.close()
+
+
+CASES = {
+    "orszag_tang": lambda: configs.orszag_tang(16),
+    "harris": lambda: configs.harris(16),
+    "arcade": lambda: configs.arcade_conduction(16, 16, 24, s_target=7),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_full_steps(case):
+    dev, orc = _pair(CASES[case]())
+    _init(dev, orc)
+    for step in range(1, 8):
+        dtd, td = dev.step(step)
+        dto, to = orc.step(step)
+        errs = field_errors(dev.fields(), orc.fields())
+        worst = max(errs.values())
+        print(f"{case} step {step}: dt dev {dtd:.16e} orc {dto:.16e} worst rel L2 {worst:.3e}", flush=True)
+        assert abs(dtd - dto) <= 1e-12 * abs(dto)
+        assert worst <= TOL, errs
+    en_d = dev.energy_account()
+    en_o = orc.energy_account()
+    for a, b in zip(en_d, en_o):
+        assert abs(a - b) <= 1e-12 * abs(b) + 1e-300
+    sd, so = dev.scalars(), orc.scalars()
+    assert abs(sd.total_visc_heating - so.total_visc_heating) <= 1e-10 * abs(so.total_visc_heating) + 1e-300
+    assert abs(sd.total_ohmic_heating - so.total_ohmic_heating) <= 1e-10 * abs(so.total_ohmic_heating) + 1e-300
+    assert sd.last_substeps == so.last_substeps and sd.n_s_stages == so.n_s_stages
+    dev.close()
