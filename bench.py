@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Throughput of the Lare3d hot path on one MI355X.
+
+Workload (one "step" = one iteration of the lare3d.f90:82-88 loop body:
+lagrangian_step + eulerian_remap + eta_calc): the resistive Harris-sheet
+configuration at 256^3 in FP64 -- Bx = tanh(y/0.5), rho = 1 + sech^2(y/0.5),
+pressure balance, eta = 1e-3 plus 1e-2 above |J| = 1, seeded 1% velocity
+noise, shock viscosity on, periodic x/z and BC_USER walls in y.
+
+Prints one JSON line: cell-updates/s over the timed steps (inputs resident
+in HBM), the roofline of the dominant kernel from HIP events on the library
+stream, and the CPU restatement timed on a bounded sample of the same
+workload as a reported baseline.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+# Algorithmic bytes per processed point (FP64 = 8 B per field access) for
+# the kernels that can dominate; a point is one index of the kernel's launch
+# box.  Counting: each distinct field read once, each written field once.
+KERNEL_BYTES_PER_POINT = {
+    "mom_flux": 8 * (3 + 1 + 1 + 1 + 1 + 3),        # v x3, v1, db1, rho_v, dmv -> Fx,Fy,Fz
+    "ct_flux": 8 * (1 + 1 + 1 + 1),                 # B_t, v_sweep, db1 -> F
+    "geom": 8 * (3 + 3),                            # v1 x3 -> cv1, cv2, db1
+    "vertex": 8 * (4 + 4),                          # rho1, cv1, cv2, dm -> rho_v, cv1v, cv2v, dmv
+    "alpha": 8 * (3 + 2 + 3),                       # v x3, rho_v, cs_v -> alpha1..3
+    "visc_dt": 8 * (3 + 2 + 3 + 1 + 3 + 2),         # v x3, rho, e, B x3, eta, alpha x3 -> visc_heat, p0
+    "vertex_prep": 8 * (5 + 2),                     # rho, e, B x3 -> rho_v, cs_v
+    "force": 8 * (3 + 1 + 3 + 1 + 3 + 3),           # v x3, pp, Bpred x3, rho_v, alpha x3 -> v1 x3
+    "shock_corr": 8 * (6 + 3 + 1 + 3 + 3),          # v, v1, alpha, pp, e, rho, visc_dep (rw)
+    "energy_flux": 8 * (1 + 1 + 1 + 1 + 1 + 1),     # e, rho1, v_sweep, db1, dm -> F
+    "mass_flux": 8 * (1 + 1 + 1 + 1),               # rho, v_sweep, db1 -> dm
+    "mom_update": 8 * (4 + 3 + 3 + 3),              # rho_v,cv1v,cv2v,dmv, F x3, v x3 (rw)
+    "rho_energy_update": 8 * (5 + 1 + 2 + 2),       # rho1,cv1,cv2,dm,F, e(rw), rho_new
+    "ct_update": 8 * (1 + 2 + 2),                   # F, t (rw), a (rw)
+    "rkl2_stage": 8 * (9 + 1),                      # Y2,Y1,E0,rho,B x3,larsen,Lc -> Y3
+}
+
+
+def box_points(name, n):
+    """Index points in one launch box of the named kernel (n^3 grid)."""
+    sizes = {
+        "mom_flux": (n + 2) * (n + 1) ** 2, "ct_flux": (n + 3) ** 3, "geom": (n + 4) ** 3,
+        "vertex": (n + 3) * (n + 1) ** 2, "alpha": (n + 3) * (n + 4) * (n + 3),
+        "visc_dt": (n + 2) ** 3, "vertex_prep": (n + 3) ** 3, "force": (n + 1) ** 3,
+        "shock_corr": n ** 3, "energy_flux": (n + 1) ** 3, "mass_flux": (n + 1) * (n + 2) ** 2,
+        "mom_update": (n + 1) ** 3, "rho_energy_update": n ** 3, "ct_update": (n + 2) ** 3,
+        "rkl2_stage": n ** 3,
+    }
+    return sizes.get(name)
+
+
+def step_bytes_per_cell(n_sub, s_stages, conduction):
+    """Whole-step algorithmic bytes per cell-update (FP64): 113 field accesses
+    for ideal MHD, 20 per resistive substep, 8s + 2 for conduction."""
+    acc = 113 + 20 * n_sub + ((8 * s_stages + 2) if conduction else 0)
+    return 8.0 * acc
+
+
+def cpu_baseline(n_sample, steps):
+    """The CPU restatement (single thread) on a bounded sample of the same
+    workload.  Reported baseline only."""
+    from lare3d_amd import configs
+    from oracle.oracle import Oracle, build
+    build()
+    grid, params, fields = configs.harris(n_sample)
+    orc = Oracle(grid, params, fields)
+    orc.boundary_conditions()
+    orc.eta_calc()
+    orc.step(1)  # untimed first step (allocation warm-up)
+    t0 = time.perf_counter()
+    for s in range(2, 2 + steps):
+        orc.step(s)
+    dt = time.perf_counter() - t0
+    orc.close()
+    return n_sample ** 3 * steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--cpu-n", type=int, default=96)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
+                                   "(multi-GPU decomposition is a design, not a built path)"}))
+        sys.exit(2)
+
+    from lare3d_amd import configs
+    from lare3d_amd.device import Device
+
+    n = args.n
+    grid, params, fields = configs.harris(n)
+    dev = Device(grid, params, fields)
+    dev.boundary_conditions()
+    dev.eta_calc()
+    step = 0
+    for _ in range(args.warmup):
+        step += 1
+        dev.step(step)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step += 1
+        dev.step(step)
+    dev.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    cups = n ** 3 * args.steps / elapsed
+    sc = dev.scalars()
+    en, divb = dev.energy_account(with_divb=True)
+
+    # per-kernel durations from HIP events on the library stream
+    dev.profile(True)
+    for _ in range(args.profile_steps):
+        step += 1
+        dev.step(step)
+    stats = dev.kernel_stats()
+    dev.profile(False)
+    total_ms = sum(v[1] for v in stats.values())
+    ranked = sorted(stats.items(), key=lambda kv: -kv[1][1])
+    dom = next((k for k, _ in ranked if k in KERNEL_BYTES_PER_POINT), ranked[0][0])
+    cnt, ms = stats[dom]
+    avg_s = ms / cnt / 1e3
+    bytes_launch = KERNEL_BYTES_PER_POINT[dom] * box_points(dom, n)
+    achieved = bytes_launch / avg_s / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_s * 1e6, 2),
+                "share_of_step": round(ms / total_ms, 3)}
+    bpc = step_bytes_per_cell(max(sc.last_substeps, 1), sc.n_s_stages, bool(params.conduction))
+    step_roof = {"bytes_per_cell": bpc, "achieved": round(cups * bpc / 1e9, 1), "peak": HBM_PEAK_GBS,
+                 "unit": "GB/s", "frac": round(cups * bpc / 1e9 / HBM_PEAK_GBS, 4)}
+    top = {k: {"launches": v[0], "ms": round(v[1], 3)} for k, v in ranked[:12]}
+    dev.close()
+
+    cpu = None
+    if not args.no_cpu:
+        rate, secs = cpu_baseline(args.cpu_n, args.cpu_steps)
+        cpu = {"value": rate, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"CPU restatement (oracle/), Harris sheet {args.cpu_n}^3 FP64, "
+                         f"{args.cpu_steps} steps after one untimed step, {secs:.1f} s"}
+
+    out = {
+        "metric": "cell-updates/s at 256^3 (FP64 plain vs FP32+surrogate), % of HBM peak",
+        "value": cups,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Harris-sheet initial condition)",
+        "config": {"workload": f"resistive Harris-sheet reconnection {n}^3 FP64, shock viscosity on, "
+                               "eta = 1e-3 + 1e-2 above |J| = 1 (lagrangian_step + eulerian_remap + eta_calc)",
+                   "n": [n, n, n], "resistive_substeps": sc.last_substeps, "rkl2_stages": sc.n_s_stages,
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "step_roofline": step_roof,
+        "cpu_baseline": cpu,
+        "kernels": top,
+        "diagnostics": {"energy": en, "max_divb_dx": divb, "dt": sc.dt},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

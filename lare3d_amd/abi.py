@@ -92,7 +92,7 @@ def load_library(path=None):
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = path or LIB_PATH
+    p = path or os.environ.get("LARE3D_DEV_LIB") or LIB_PATH
     if not os.path.exists(p):
         raise DeviceError(f"device library not built: {p} (run __graft_entry__.build())")
     lib = C.CDLL(p)
