@@ -1,0 +1,113 @@
+!  lare_gpu_bridge -- ISO_C_BINDING interface to liblare3d_dev.so
+!  (include/lare3d_dev.h).  A Fortran driver USEs this module and calls the
+!  device library with its own allocatable arrays (C_LOC of the first
+!  element, i.e. with the ghost bounds of mpi_routines.F90:295-334).
+
+MODULE lare_gpu_bridge
+
+  USE, INTRINSIC :: ISO_C_BINDING
+
+  IMPLICIT NONE
+
+  INTEGER(C_INT32_T), PARAMETER :: LARE_BC_PERIODIC = 1, LARE_BC_USER = 2
+  INTEGER(C_INT32_T), PARAMETER :: LARE_PREC_FP64 = 0, LARE_PREC_FP32 = 1
+
+  TYPE, BIND(C) :: lare_grid
+    INTEGER(C_INT32_T) :: nx, ny, nz
+    INTEGER(C_INT32_T) :: xbc_min, xbc_max, ybc_min, ybc_max, zbc_min, zbc_max
+    TYPE(C_PTR) :: dxb, dxc, dyb, dyc, dzb, dzc, grav
+  END TYPE lare_grid
+
+  TYPE, BIND(C) :: lare_params
+    REAL(C_DOUBLE) :: gamma, visc1, visc2
+    REAL(C_DOUBLE) :: eta_background, eta0, j_max
+    REAL(C_DOUBLE) :: kappa_0, flux_limiter, xi_n, dt_from_restart
+    INTEGER(C_INT32_T) :: resistive_mhd, conduction, heat_flux_limiter
+    INTEGER(C_INT32_T) :: use_viscous_damping, restart, precision
+  END TYPE lare_params
+
+  TYPE, BIND(C) :: lare_fields
+    TYPE(C_PTR) :: rho, energy, vx, vy, vz, bx, by, bz
+    TYPE(C_PTR) :: visc3, eta, visc_dep, ohmic_dep
+  END TYPE lare_fields
+
+  TYPE, BIND(C) :: lare_scalars
+    REAL(C_DOUBLE) :: dt, dtr, time
+    REAL(C_DOUBLE) :: total_visc_heating, total_ohmic_heating
+    INTEGER(C_INT32_T) :: n_s_stages, peak_substeps, last_substeps, first_set_dt, step
+  END TYPE lare_scalars
+
+  INTERFACE
+    INTEGER(C_INT) FUNCTION lare_dev_init(grid, params) BIND(C, NAME='lare_dev_init')
+      IMPORT :: C_INT, lare_grid, lare_params
+      TYPE(lare_grid), INTENT(IN) :: grid
+      TYPE(lare_params), INTENT(IN) :: params
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_upload(fields) BIND(C, NAME='lare_dev_upload')
+      IMPORT :: C_INT, lare_fields
+      TYPE(lare_fields), INTENT(IN) :: fields
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_download(fields) BIND(C, NAME='lare_dev_download')
+      IMPORT :: C_INT, lare_fields
+      TYPE(lare_fields), INTENT(INOUT) :: fields
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_get_scalars(s) BIND(C, NAME='lare_dev_get_scalars')
+      IMPORT :: C_INT, lare_scalars
+      TYPE(lare_scalars), INTENT(OUT) :: s
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_set_scalars(s) BIND(C, NAME='lare_dev_set_scalars')
+      IMPORT :: C_INT, lare_scalars
+      TYPE(lare_scalars), INTENT(IN) :: s
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_lagrangian_step() BIND(C, NAME='lare_dev_lagrangian_step')
+      IMPORT :: C_INT
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_eulerian_remap(step) BIND(C, NAME='lare_dev_eulerian_remap')
+      IMPORT :: C_INT, C_INT32_T
+      INTEGER(C_INT32_T), VALUE :: step
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_eta_calc() BIND(C, NAME='lare_dev_eta_calc')
+      IMPORT :: C_INT
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_conduct_heat() BIND(C, NAME='lare_dev_conduct_heat')
+      IMPORT :: C_INT
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_boundary_conditions() BIND(C, NAME='lare_dev_boundary_conditions')
+      IMPORT :: C_INT
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_energy_account(en, divb) BIND(C, NAME='lare_dev_energy_account')
+      IMPORT :: C_INT, C_DOUBLE
+      REAL(C_DOUBLE), INTENT(OUT) :: en(3), divb
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_step(step, dt, time) BIND(C, NAME='lare_dev_step')
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE
+      INTEGER(C_INT32_T), VALUE :: step
+      REAL(C_DOUBLE), INTENT(OUT) :: dt, time
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_synchronize() BIND(C, NAME='lare_dev_synchronize')
+      IMPORT :: C_INT
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_finalize() BIND(C, NAME='lare_dev_finalize')
+      IMPORT :: C_INT
+    END FUNCTION
+    TYPE(C_PTR) FUNCTION lare_dev_last_error() BIND(C, NAME='lare_dev_last_error')
+      IMPORT :: C_PTR
+    END FUNCTION
+  END INTERFACE
+
+CONTAINS
+
+  ! Text of the last library error as a Fortran string.
+  FUNCTION lare_dev_error_text() RESULT(msg)
+    CHARACTER(LEN=512) :: msg
+    CHARACTER(KIND=C_CHAR), DIMENSION(:), POINTER :: cs
+    INTEGER :: i
+    CALL C_F_POINTER(lare_dev_last_error(), cs, [512])
+    msg = ' '
+    DO i = 1, 512
+      IF (cs(i) == C_NULL_CHAR) EXIT
+      msg(i:i) = cs(i)
+    END DO
+  END FUNCTION lare_dev_error_text
+
+END MODULE lare_gpu_bridge
