@@ -29,7 +29,7 @@ struct BcSet {
   int nf;
 };
 
-__global__ void k_bc_axis(Geo g, BcSet s, int axis) {
+__global__ __launch_bounds__(64) void k_bc_axis(Geo g, BcSet s, int axis) {
   // plane coordinates: u runs along the contiguous axis where possible
   const int u = axis == 0 ? 1 : 0;
   const int w = axis == 2 ? 1 : 2;

@@ -57,7 +57,7 @@ struct CondArgs {
 // Larsen flux-limiter factor over cells 0..n and the explicit parabolic
 // time-step minimum over 1..n (calc_s_stages, conduct.f90:69-129).
 // Temperatures from energy as stored (ghosts valid at call time).
-__global__ void k_larsen(Geo g, const double *__restrict__ energy, const double *__restrict__ rho,
+__global__ __launch_bounds__(BX * BY) void k_larsen(Geo g, const double *__restrict__ energy, const double *__restrict__ rho,
                          double *__restrict__ larsen, double *__restrict__ partial, double flux_limiter,
                          double kappa_0, double tfac, int limiter_on) {
   const int i = (int)(blockIdx.x * BX + threadIdx.x);
@@ -204,7 +204,7 @@ __device__ __forceinline__ double cell_heat_flux(const Geo &g, const double *__r
 }
 
 // Stage 1 (conduct.f90:345-367): Lc = flux/rho, Y1 = Y0 + mu~1 dt Lc.
-__global__ void k_rkl2_stage1(Geo g, Box b, const double *__restrict__ E0, const double *__restrict__ rho,
+__global__ __launch_bounds__(BX * BY) void k_rkl2_stage1(Geo g, Box b, const double *__restrict__ E0, const double *__restrict__ rho,
                               const double *__restrict__ bx, const double *__restrict__ by,
                               const double *__restrict__ bz, const double *__restrict__ larsen,
                               double *__restrict__ Lc, double *__restrict__ Yout, CondArgs a, double mu_t1) {
@@ -218,7 +218,7 @@ __global__ void k_rkl2_stage1(Geo g, Box b, const double *__restrict__ E0, const
 }
 
 // Stage j >= 2 (conduct.f90:369-396).
-__global__ void k_rkl2_stage(Geo g, Box b, const double *__restrict__ Y2, const double *__restrict__ Y1,
+__global__ __launch_bounds__(BX * BY) void k_rkl2_stage(Geo g, Box b, const double *__restrict__ Y2, const double *__restrict__ Y1,
                              const double *__restrict__ E0, const double *__restrict__ rho,
                              const double *__restrict__ bx, const double *__restrict__ by,
                              const double *__restrict__ bz, const double *__restrict__ larsen,
@@ -239,7 +239,7 @@ __global__ void k_rkl2_stage(Geo g, Box b, const double *__restrict__ Y2, const 
 // the last stage, ghosts by energy_bcs (index map).  delta_mode != 0 applies
 // lagrangian_step's energy0 + (energy - energy0) and MAX(., 0)
 // (lagran.F90:168-181) over the whole cell box.
-__global__ void k_rkl2_final(Geo g, Box b, const double *__restrict__ Ys, double *__restrict__ energy,
+__global__ __launch_bounds__(BX * BY) void k_rkl2_final(Geo g, Box b, const double *__restrict__ Ys, double *__restrict__ energy,
                              int delta_mode) {
   BOX_INDEX(b, i, j, k);
   const double v = ghost_read(g, Ys, Ys, i, j, k, TMODE_WRAP);

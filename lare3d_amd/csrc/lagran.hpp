@@ -55,7 +55,7 @@ __device__ __forceinline__ double cell_cs(const Geo &g, const double *__restrict
   return sqrt((gamma * p + 1.0 * b2) / rmin);
 }
 
-__global__ void k_vertex_prep(Geo g, Box b, const double *__restrict__ rho, const double *__restrict__ energy,
+__global__ __launch_bounds__(BX * BY) void k_vertex_prep(Geo g, Box b, const double *__restrict__ rho, const double *__restrict__ energy,
                               const double *__restrict__ bx, const double *__restrict__ by,
                               const double *__restrict__ bz, double *__restrict__ rho_v, double *__restrict__ cs_v,
                               double gamma) {
@@ -104,7 +104,7 @@ __device__ __forceinline__ double edge_visc(const double *__restrict__ vx, const
   return q_k_bar * (1.0 - psi) * dvdots;
 }
 
-__global__ void k_alpha(Geo g, Box b, const double *__restrict__ vx, const double *__restrict__ vy,
+__global__ __launch_bounds__(BX * BY) void k_alpha(Geo g, Box b, const double *__restrict__ vx, const double *__restrict__ vy,
                         const double *__restrict__ vz, const double *__restrict__ rho_v,
                         const double *__restrict__ cs_v, double *__restrict__ al1, double *__restrict__ al2,
                         double *__restrict__ al3, LagArgs a) {
@@ -169,7 +169,7 @@ __device__ __forceinline__ double edge_heat(const Geo &g, int i, int j, int k, c
 
 // p_visc + visc_heat over cells 0..n+1; pressure p0 of lagran.F90:93 stored
 // for the predictor; set_dt partial minima over 0..n.
-__global__ void k_visc_dt(Geo g, const double *__restrict__ vx, const double *__restrict__ vy,
+__global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, const double *__restrict__ vx, const double *__restrict__ vy,
                           const double *__restrict__ vz, const double *__restrict__ rho,
                           const double *__restrict__ energy, const double *__restrict__ bx,
                           const double *__restrict__ by, const double *__restrict__ bz,
@@ -223,7 +223,7 @@ __global__ void k_visc_dt(Geo g, const double *__restrict__ vx, const double *__
 }
 
 // viscous_damping (lagran.F90:483-524), two passes: q then v += dt q
-__global__ void k_damp_q(Geo g, Box b, const double *__restrict__ vx, const double *__restrict__ vy,
+__global__ __launch_bounds__(BX * BY) void k_damp_q(Geo g, Box b, const double *__restrict__ vx, const double *__restrict__ vy,
                          const double *__restrict__ vz, const double *__restrict__ visc3, double *__restrict__ qx,
                          double *__restrict__ qy, double *__restrict__ qz) {
   BOX_INDEX(b, i, j, k);
@@ -242,7 +242,7 @@ __global__ void k_damp_q(Geo g, Box b, const double *__restrict__ vx, const doub
                           ((u[c + g.sz] - u0) / dzbp - (u0 - u[c - g.sz]) / dzb) / dzc);
   }
 }
-__global__ void k_damp_apply(Geo g, Box b, double *__restrict__ vx, double *__restrict__ vy, double *__restrict__ vz,
+__global__ __launch_bounds__(BX * BY) void k_damp_apply(Geo g, Box b, double *__restrict__ vx, double *__restrict__ vy, double *__restrict__ vz,
                              const double *__restrict__ qx, const double *__restrict__ qy,
                              const double *__restrict__ qz, double dt) {
   BOX_INDEX(b, i, j, k);
@@ -271,7 +271,7 @@ __device__ __forceinline__ void vertex_current(const Geo &g, const double *__res
 }
 
 // eta_calc (lagran.F90:1176-1230) over -1..n+1
-__global__ void k_eta(Geo g, Box b, const double *__restrict__ bx, const double *__restrict__ by,
+__global__ __launch_bounds__(BX * BY) void k_eta(Geo g, Box b, const double *__restrict__ bx, const double *__restrict__ by,
                       const double *__restrict__ bz, double *__restrict__ eta, double eta_bg, double eta0,
                       double j_max) {
   BOX_INDEX(b, i, j, k);
@@ -280,14 +280,14 @@ __global__ void k_eta(Geo g, Box b, const double *__restrict__ bx, const double 
   eta[g.idx(i, j, k)] = sqrt(jx * jx + jy * jy + jz * jz) > j_max ? eta_bg + eta0 : eta_bg;
 }
 
-__global__ void k_fill(Geo g, Box b, double *__restrict__ a, double v) {
+__global__ __launch_bounds__(BX * BY) void k_fill(Geo g, Box b, double *__restrict__ a, double v) {
   BOX_INDEX(b, i, j, k);
   a[g.idx(i, j, k)] = v;
 }
 
 // rkstep (lagran.F90:1472-1503) over nodes 0..n, with the
 // total_ohmic_heating increments of :1309-1328 reduced in the same pass.
-__global__ void k_rkstep(Geo g, const double *__restrict__ bx, const double *__restrict__ by,
+__global__ __launch_bounds__(BX * BY) void k_rkstep(Geo g, const double *__restrict__ bx, const double *__restrict__ by,
                          const double *__restrict__ bz, const double *__restrict__ eta, double *__restrict__ fxr,
                          double *__restrict__ fyr, double *__restrict__ fzr, double *__restrict__ curlb,
                          double *__restrict__ partial, double dt) {
@@ -317,7 +317,7 @@ __global__ void k_rkstep(Geo g, const double *__restrict__ bx, const double *__r
 
 // bstep (lagran.F90:1582-1631); B updated in place (bx1 = bx at the same
 // point, :1259-1261)
-__global__ void k_bstep(Geo g, Box b, double *__restrict__ bx, double *__restrict__ by, double *__restrict__ bz,
+__global__ __launch_bounds__(BX * BY) void k_bstep(Geo g, Box b, double *__restrict__ bx, double *__restrict__ by, double *__restrict__ bz,
                         const double *__restrict__ kx, const double *__restrict__ ky, const double *__restrict__ kz,
                         double dt) {
   BOX_INDEX(b, i, j, k);
@@ -342,7 +342,7 @@ __global__ void k_bstep(Geo g, Box b, double *__restrict__ bx, double *__restric
 
 // Ohmic heating (lagran.F90:1270-1288) on cells 1..n, then
 // energy = MAX(energy, 0) over the whole cell box (:1305)
-__global__ void k_ohmic(Geo g, Box b, double *__restrict__ energy, double *__restrict__ ohmic_dep,
+__global__ __launch_bounds__(BX * BY) void k_ohmic(Geo g, Box b, double *__restrict__ energy, double *__restrict__ ohmic_dep,
                         const double *__restrict__ rho, const double *__restrict__ curlb, double dt) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
@@ -358,7 +358,7 @@ __global__ void k_ohmic(Geo g, Box b, double *__restrict__ energy, double *__res
   energy[c] = max0(e);
 }
 
-__global__ void k_energy_max(Geo g, Box b, double *__restrict__ energy) {
+__global__ __launch_bounds__(BX * BY) void k_energy_max(Geo g, Box b, double *__restrict__ energy) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
   energy[c] = max0(energy[c]);
@@ -366,7 +366,7 @@ __global__ void k_energy_max(Geo g, Box b, double *__restrict__ energy) {
 
 // predictor energy and pressure (lagran.F90:234-249) over cells 0..n+1;
 // p0 is the start-of-step pressure (:93), cv1 the predictor-remap volume
-__global__ void k_pred_pressure(Geo g, Box b, const double *__restrict__ energy, const double *__restrict__ rho,
+__global__ __launch_bounds__(BX * BY) void k_pred_pressure(Geo g, Box b, const double *__restrict__ energy, const double *__restrict__ rho,
                                 const double *__restrict__ cv1, const double *__restrict__ visc_heat,
                                 const double *__restrict__ p0, double *__restrict__ pp, double gamma, double dt2) {
   BOX_INDEX(b, i, j, k);
@@ -382,7 +382,7 @@ __global__ void k_pred_pressure(Geo g, Box b, const double *__restrict__ energy,
 // predictor momentum (lagran.F90:251-387) with fx/fy/fz_visc (:745-789)
 // recomputed here from alpha and v; B is the half-step (predictor-remapped)
 // field.
-__global__ void k_force(Geo g, Box b, const double *__restrict__ vx, const double *__restrict__ vy,
+__global__ __launch_bounds__(BX * BY) void k_force(Geo g, Box b, const double *__restrict__ vx, const double *__restrict__ vy,
                         const double *__restrict__ vz, const double *__restrict__ pp,
                         const double *__restrict__ bx, const double *__restrict__ by, const double *__restrict__ bz,
                         const double *__restrict__ rho_v, const double *__restrict__ al1,
@@ -438,7 +438,7 @@ __global__ void k_force(Geo g, Box b, const double *__restrict__ vx, const doubl
 
 // shock_heating (lagran.F90:861-937) + corrector (:415-462) over cells
 // 1..n; total_visc_heating partial sums.
-__global__ void k_shock_corr(Geo g, const double *__restrict__ vx, const double *__restrict__ vy,
+__global__ __launch_bounds__(BX * BY) void k_shock_corr(Geo g, const double *__restrict__ vx, const double *__restrict__ vy,
                              const double *__restrict__ vz, const double *__restrict__ vx1,
                              const double *__restrict__ vy1, const double *__restrict__ vz1,
                              const double *__restrict__ al1, const double *__restrict__ al2,
@@ -479,7 +479,7 @@ __global__ void k_shock_corr(Geo g, const double *__restrict__ vx, const double 
   block_reduce_store<1>(v, ops, partial);
 }
 
-__global__ void k_vupdate(Geo g, Box b, double *__restrict__ vx, double *__restrict__ vy, double *__restrict__ vz,
+__global__ __launch_bounds__(BX * BY) void k_vupdate(Geo g, Box b, double *__restrict__ vx, double *__restrict__ vy, double *__restrict__ vz,
                           const double *__restrict__ vx1, const double *__restrict__ vy1,
                           const double *__restrict__ vz1) {
   BOX_INDEX(b, i, j, k);
@@ -491,7 +491,7 @@ __global__ void k_vupdate(Geo g, Box b, double *__restrict__ vx, double *__restr
 
 // energy_account (diagnostics.F90:517-609) + max |div B| * min(dx):
 // partial[0..3] = (en_b, en_ke, en_int, -max divb) per block over box 0..n
-__global__ void k_energy(Geo g, const double *__restrict__ rho, const double *__restrict__ energy,
+__global__ __launch_bounds__(BX * BY) void k_energy(Geo g, const double *__restrict__ rho, const double *__restrict__ energy,
                          const double *__restrict__ vx, const double *__restrict__ vy, const double *__restrict__ vz,
                          const double *__restrict__ bx, const double *__restrict__ by,
                          const double *__restrict__ bz, double *__restrict__ partial) {

@@ -166,30 +166,59 @@ Box nodes0n() { return mkbox(0, nx(), 0, ny(), 0, nz()); }
 // reduction over partials of a grid launched from index 0 on every axis
 dim3 grid_from0(int x1, int y1, int z1) { return box_grid(mkbox(0, x1, 0, y1, 0, z1)); }
 
-__global__ void k_final_reduce(const double *__restrict__ partial, int nblocks, int nv, int minmask, int accumulate,
-                               double *__restrict__ out) {
-  __shared__ double red[256];
+// Two-level reduction of the per-block partials (rows of nv values; bit q of
+// minmask selects min, else sum).  Level 1: RED_ROWS blocks each fold one
+// contiguous chunk of rows; level 2: one block folds the RED_ROWS results and
+// stores (or adds) the nv scalars.
+#define RED_ROWS 512
+#define RED_MAXV 4
+
+__device__ __forceinline__ void fold_rows(const double *__restrict__ rows, int r0, int r1, int nv, int minmask,
+                                          double *__restrict__ out, int accumulate) {
+  __shared__ double red[4][RED_MAXV];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int q = 0; q < nv; ++q) {
     const bool isMin = (minmask >> q) & 1;
     double v = isMin ? 1.7976931348623157e+308 : 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
-      const double x = partial[(size_t)b * nv + q];
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += 256) {
+      const double x = rows[(size_t)r * nv + q];
       v = isMin ? dmin(v, x) : v + x;
     }
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) red[threadIdx.x] = isMin ? dmin(red[threadIdx.x], red[threadIdx.x + s]) : red[threadIdx.x] + red[threadIdx.x + s];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[q] = accumulate ? out[q] + red[0] : red[0];
-    __syncthreads();
+    v = isMin ? wave_min(v) : wave_sum(v);
+    if (lane == 0) red[wave][q] = v;
   }
+  __syncthreads();
+  if ((int)threadIdx.x < nv) {
+    const int q = threadIdx.x;
+    const bool isMin = (minmask >> q) & 1;
+    double r = red[0][q];
+    for (int w = 1; w < 4; ++w) r = isMin ? dmin(r, red[w][q]) : r + red[w][q];
+    if (accumulate) r = out[q] + r;
+    out[q] = r;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_rows(const double *__restrict__ partial, int nrows, int nv, int minmask,
+                                                     int chunk, double *__restrict__ level1) {
+  const int r0 = (int)blockIdx.x * chunk;
+  const int r1 = r0 + chunk < nrows ? r0 + chunk : nrows;
+  fold_rows(partial, r0, r1, nv, minmask, level1 + (size_t)blockIdx.x * nv, 0);
+}
+
+__global__ __launch_bounds__(256) void k_final_reduce(const double *__restrict__ level1, int nrows, int nv, int minmask,
+                                                      int accumulate, double *__restrict__ out) {
+  fold_rows(level1, 0, nrows, nv, minmask, out, accumulate);
 }
 
 int final_reduce(dim3 grid, int nv, int minmask, int slot, int accumulate) {
   const int nb = (int)(grid.x * grid.y * grid.z);
-  LAUNCH("final_reduce", k_final_reduce, dim3(1), dim3(256), S.partial, nb, nv, minmask, accumulate, S.dscal + slot);
+  if (nv > RED_MAXV || (size_t)nb * nv > S.partial_cap)
+    return fail(LARE_E_ARG, "reduction of %d x %d values exceeds the partial buffer", nb, nv);
+  const int chunk = (nb + RED_ROWS - 1) / RED_ROWS;
+  const int rows = (nb + chunk - 1) / chunk;  // every level-1 block has >= 1 row
+  double *level1 = S.partial + S.partial_cap;
+  LAUNCH("reduce_rows", k_reduce_rows, dim3(rows), dim3(256), S.partial, nb, nv, minmask, chunk, level1);
+  LAUNCH("final_reduce", k_final_reduce, dim3(1), dim3(256), level1, rows, nv, minmask, accumulate, S.dscal + slot);
   return 0;
 }
 
@@ -726,7 +755,8 @@ int lare_dev_init(const lare_grid *grid, const lare_params *params) {
   // partial sums: largest reduction grid (0..n+1 per axis) x 4 values
   dim3 gr = grid_from0(n[0] + 2, n[1] + 2, n[2] + 2);
   S.partial_cap = (size_t)gr.x * gr.y * gr.z * 4;
-  HIPCHK(hipMalloc((void **)&S.partial, sizeof(double) * S.partial_cap));
+  // + RED_ROWS x RED_MAXV level-1 results after the per-block partials
+  HIPCHK(hipMalloc((void **)&S.partial, sizeof(double) * (S.partial_cap + RED_ROWS * RED_MAXV)));
   HIPCHK(hipMalloc((void **)&S.dscal, sizeof(double) * 16));
   HIPCHK(hipMemset(S.dscal, 0, sizeof(double) * 16));
   HIPCHK(hipHostMalloc((void **)&S.hscal, sizeof(double) * 16));

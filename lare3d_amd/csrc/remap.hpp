@@ -42,7 +42,7 @@ __device__ __forceinline__ double vl_di(double dfi, double dfu, double sign_v, d
 // Lagrangian geometry of a sweep (xremap.f90:57-105 and analogues).
 // Velocities are the time-centred ones (v in predictor mode).
 template <int AX>
-__global__ void k_geom(Geo g, Box b, const double *__restrict__ ux, const double *__restrict__ uy,
+__global__ __launch_bounds__(BX * BY) void k_geom(Geo g, Box b, const double *__restrict__ ux, const double *__restrict__ uy,
                        const double *__restrict__ uz, double *__restrict__ cv1, double *__restrict__ cv2,
                        double *__restrict__ db1, double dt, double xpass, double ypass, double zpass, int write_cv1,
                        int write_cv2) {
@@ -93,7 +93,7 @@ __device__ __forceinline__ double dc1(const Geo &g, const double *__restrict__ d
 // VS: stride of the axis the velocity is averaged along (v(c) + v(c-VS));
 // GS: stride of the axis d?b1 is averaged along (d(c) + d(c+GS)).
 template <int AX>
-__global__ void k_ct_flux(Geo g, Box b, const double *__restrict__ fld, const double *__restrict__ vel,
+__global__ __launch_bounds__(BX * BY) void k_ct_flux(Geo g, Box b, const double *__restrict__ fld, const double *__restrict__ vel,
                           const double *__restrict__ db1, double *__restrict__ F, int VS, int GS, double dt) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
@@ -128,7 +128,7 @@ __global__ void k_ct_flux(Geo g, Box b, const double *__restrict__ fld, const do
 }
 
 // CT update pair: t = t - F + F(c - TS) on box bt; a = a + F - F(c - AS) on ba.
-__global__ void k_ct_update(Geo g, Box b, const double *__restrict__ F, double *__restrict__ t,
+__global__ __launch_bounds__(BX * BY) void k_ct_update(Geo g, Box b, const double *__restrict__ F, double *__restrict__ t,
                             double *__restrict__ a, int TS, int AS, Box bt, Box ba) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
@@ -155,7 +155,7 @@ __device__ __forceinline__ double face_area(const Geo &g, int i, int j, int k) {
 
 // x_mass_flux (xremap.f90:474-533): sweep 0..n, transverse 0..n+1
 template <int AX>
-__global__ void k_mass_flux(Geo g, Box b, const double *__restrict__ rho, const double *__restrict__ vel,
+__global__ __launch_bounds__(BX * BY) void k_mass_flux(Geo g, Box b, const double *__restrict__ rho, const double *__restrict__ vel,
                             const double *__restrict__ db1, double *__restrict__ dm, double dt) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
@@ -180,7 +180,7 @@ __global__ void k_mass_flux(Geo g, Box b, const double *__restrict__ rho, const 
 
 // x_energy_flux (xremap.f90:539-601): box 0..n
 template <int AX>
-__global__ void k_energy_flux(Geo g, Box b, const double *__restrict__ energy, const double *__restrict__ rho1,
+__global__ __launch_bounds__(BX * BY) void k_energy_flux(Geo g, Box b, const double *__restrict__ energy, const double *__restrict__ rho1,
                               const double *__restrict__ vel, const double *__restrict__ db1,
                               const double *__restrict__ dm, double *__restrict__ F, double dt) {
   BOX_INDEX(b, i, j, k);
@@ -209,7 +209,7 @@ __global__ void k_energy_flux(Geo g, Box b, const double *__restrict__ energy, c
 
 // density and energy updates of xremap.f90:174-196 on cells 1..n
 template <int AX>
-__global__ void k_rho_energy_update(Geo g, Box b, const double *__restrict__ rho1, double *__restrict__ rho_new,
+__global__ __launch_bounds__(BX * BY) void k_rho_energy_update(Geo g, Box b, const double *__restrict__ rho1, double *__restrict__ rho_new,
                                     double *__restrict__ energy, const double *__restrict__ cv1,
                                     const double *__restrict__ cv2, const double *__restrict__ dm,
                                     const double *__restrict__ F) {
@@ -225,7 +225,7 @@ __global__ void k_rho_energy_update(Geo g, Box b, const double *__restrict__ rho
 // vertex quantities of xremap.f90:201-280: rho_v over sweep -1..n+1 and
 // transverse 0..n; cv1v, cv2v over 0..n; dm_v over sweep -1..n.
 template <int AX>
-__global__ void k_vertex(Geo g, Box b, const double *__restrict__ rho1, const double *__restrict__ cv1,
+__global__ __launch_bounds__(BX * BY) void k_vertex(Geo g, Box b, const double *__restrict__ rho1, const double *__restrict__ cv1,
                          const double *__restrict__ cv2, const double *__restrict__ dm, double *__restrict__ rho_v,
                          double *__restrict__ cv1v, double *__restrict__ cv2v, double *__restrict__ dmv) {
   BOX_INDEX(b, i, j, k);
@@ -266,7 +266,7 @@ __device__ __forceinline__ double mom_area(const Geo &g, int i, int j, int k) {
 // x_momx/y/z_flux (xremap.f90:605-895, rke off): sweep -1..n, transverse
 // 0..n; the three velocity components share the upwind switch and geometry.
 template <int AX>
-__global__ void k_mom_flux(Geo g, Box b, const double *__restrict__ vx, const double *__restrict__ vy,
+__global__ __launch_bounds__(BX * BY) void k_mom_flux(Geo g, Box b, const double *__restrict__ vx, const double *__restrict__ vy,
                            const double *__restrict__ vz, const double *__restrict__ vel1,
                            const double *__restrict__ db1, const double *__restrict__ rho_v,
                            const double *__restrict__ dmv, double *__restrict__ Fx, double *__restrict__ Fy,
@@ -304,7 +304,7 @@ __global__ void k_mom_flux(Geo g, Box b, const double *__restrict__ vx, const do
 
 // momentum updates (xremap.f90:282-329) on nodes 0..n
 template <int AX>
-__global__ void k_mom_update(Geo g, Box b, double *__restrict__ vx, double *__restrict__ vy, double *__restrict__ vz,
+__global__ __launch_bounds__(BX * BY) void k_mom_update(Geo g, Box b, double *__restrict__ vx, double *__restrict__ vy, double *__restrict__ vz,
                              const double *__restrict__ rho_v, const double *__restrict__ cv1v,
                              const double *__restrict__ cv2v, const double *__restrict__ dmv,
                              const double *__restrict__ Fx, const double *__restrict__ Fy,
@@ -322,7 +322,7 @@ __global__ void k_mom_update(Geo g, Box b, double *__restrict__ vx, double *__re
 // B <-> face flux scaling of remap.f90:43-69 / :101-127 over the whole
 // face arrays.  dir > 0 multiplies (out may differ from in: the predictor
 // remap works on a copy); dir < 0 divides in the reference's order.
-__global__ void k_bscale(Geo g, Box b, const double *__restrict__ bxi, const double *__restrict__ byi,
+__global__ __launch_bounds__(BX * BY) void k_bscale(Geo g, Box b, const double *__restrict__ bxi, const double *__restrict__ byi,
                          const double *__restrict__ bzi, double *__restrict__ bxo, double *__restrict__ byo,
                          double *__restrict__ bzo, int dir) {
   BOX_INDEX(b, i, j, k);
