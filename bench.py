@@ -62,6 +62,36 @@ def box_points(name, n):
     return sizes.get(name)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "harris256_fp64_hbm_traffic.json")
+
+
+def csrc_sha1():
+    """Hash of the kernel sources, to match counter data to the code it measured."""
+    import glob
+    import hashlib
+    h = hashlib.sha1()
+    d = os.path.join(ROOT, "lare3d_amd", "csrc")
+    for p in sorted(glob.glob(os.path.join(d, "*.hpp")) + glob.glob(os.path.join(d, "*.hip"))):
+        with open(p, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the FETCH_SIZE / WRITE_SIZE counter
+    runs recorded under profiles/ (scaled by a plain copy of known size), or
+    None when they were taken on different kernel sources."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)
+    except OSError:
+        return None
+    if rec.get("csrc_sha1") != csrc_sha1() or kernel not in rec["kernels"]:
+        return None
+    k = rec["kernels"][kernel]
+    return k["fetch_bytes"] + k["write_bytes"]
+
+
 def step_bytes_per_cell(n_sub, s_stages, conduction):
     """Whole-step algorithmic bytes per cell-update (FP64): 113 field accesses
     for ideal MHD, 20 per resistive substep, 8s + 2 for conduction."""
@@ -88,6 +118,18 @@ def cpu_baseline(n_sample, steps):
     return n_sample ** 3 * steps / dt, dt
 
 
+def copy_probe(nbytes=1 << 30, reps=3):
+    """A plain device-to-device copy of known size (torch's copy kernel), so
+    that a hardware-counter run can scale FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,12 +140,18 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=96)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy and exit (counter calibration for HBM traffic; "
+                         "run it as its own process under the same counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
                                    "(multi-GPU decomposition is a design, not a built path)"}))
         sys.exit(2)
 
+    if args.copy_probe:
+        copy_probe()
+        return
     from lare3d_amd import configs
     from lare3d_amd.device import Device
 
@@ -143,7 +191,7 @@ def main():
     bytes_launch = KERNEL_BYTES_PER_POINT[dom] * box_points(dom, n)
     achieved = bytes_launch / avg_s / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
                 "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_s * 1e6, 2),
                 "share_of_step": round(ms / total_ms, 3)}
     bpc = step_bytes_per_cell(max(sc.last_substeps, 1), sc.n_s_stages, bool(params.conduction))
