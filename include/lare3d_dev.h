@@ -148,6 +148,62 @@ int lare_dev_profile(int32_t enable);
 int lare_dev_kernel_stats(char *names, int32_t name_len, int32_t *counts,
                           double *total_ms, int32_t cap);
 
+/* ------------------------------------------------------------------------
+ * Learned conduction surrogate.  No counterpart in the reference: it stands
+ * in for whole conduct_heat super-steps (conduct.f90:39) and falls back to
+ * them.  A 3x3x3-stencil MLP (27 cells x 10 channels -> 64 -> 64 -> 1, GELU)
+ * predicts the temperature increment of one RKL2 super-step; BF16 MFMA with
+ * FP32 accumulation for inference and training.  Each use is checked by the
+ * trapezoidal residual |E1 - E0 - dt/2 (L(E0) + L(E1))| / |E0| (L2 over the
+ * interior, L = heat-flux divergence / rho); above the tolerance the step is
+ * redone by the plain RKL2 solver.
+ *
+ * Parameter vector (float, LARE_SUR_NPARAM): W1[336][64] (rows 324..335 and
+ * the rows of channels 10, 11 of each cell meet zero inputs), b1[64],
+ * W2[64][64], b2[64], W3[64], b3.  Inputs per row: 27 neighbours (dz, dy, dx
+ * in -1..1, x fastest) x 12 channels (T, unit B x3, rho, log dt, |B|,
+ * dT/dx, dT/dy, dT/dz, 0, 0), standardised, then 12 zeros (336 floats). */
+#define LARE_SUR_K 336
+#define LARE_SUR_NCH 10
+#define LARE_SUR_NPARAM (336 * 64 + 64 + 64 * 64 + 64 + 64 + 1)
+
+/* Load / read the FP32 master parameters (n == LARE_SUR_NPARAM).  Loading
+ * resets the Adam moments. */
+int lare_dev_sur_set_params(const float *params, int32_t n);
+int lare_dev_sur_get_params(float *params, int32_t n);
+/* Channel standardisation (mean and std of the 10 channels) and the scale of
+ * the network output (dT = y * y_scale). */
+int lare_dev_sur_set_norm(const float *mu, const float *sd, float y_scale);
+int lare_dev_sur_get_norm(float *mu, float *sd, float *y_scale);
+/* Training data from the current device state and dt (set_scalars): count
+ * random interior cells (hash of seed), their raw input rows, and the
+ * temperature increment one plain RKL2 super-step gives them.  The state is
+ * left unchanged.  *total receives the number of rows held. */
+int lare_dev_sur_collect(int32_t count, uint64_t seed, int32_t *total);
+/* Copy the first `rows` held rows (LARE_SUR_K floats each) and targets to
+ * the host (raw before lare_dev_sur_fit_norm, standardised after). */
+int lare_dev_sur_rows(float *X, float *y, int32_t rows);
+/* Standardisation from the collected rows (centre-cell channel statistics,
+ * RMS target); standardises the held rows. */
+int lare_dev_sur_fit_norm(void);
+/* Adam (lr, beta 0.9 / 0.999) for `steps` mini-batches of `batch` rows
+ * (a multiple of 16) drawn by hash of seed; loss_out[steps] receives the
+ * mean squared error of each mini-batch (may be NULL). */
+int lare_dev_sur_train(int32_t steps, int32_t batch, float lr, uint64_t seed, float *loss_out);
+/* Network output for `rows` standardised input rows (LARE_SUR_K floats each). */
+int lare_dev_sur_forward(const float *X, int32_t rows, float *y);
+/* Gradient of the mean squared error against targets t over the rows. */
+int lare_dev_sur_grad(const float *X, const float *t, int32_t rows, float *grad, float *loss);
+/* Use the surrogate for the conduction of lagrangian_step (on != 0), with the
+ * relative residual tolerance tol. */
+int lare_dev_sur_enable(int32_t on, double tol);
+/* One checked surrogate super-step of length dt on the current state (the
+ * surrogate form of lare_dev_conduct_heat).  *accepted = 0 means the
+ * residual exceeded the tolerance and RKL2 did the step. */
+int lare_dev_sur_conduct(double *residual, int32_t *accepted);
+/* Counts of accepted and rejected surrogate steps, and the last residual. */
+int lare_dev_sur_stats(int32_t *accepted, int32_t *rejected, double *last_residual);
+
 /* Block until all queued device work is done. */
 int lare_dev_synchronize(void);
 

@@ -12,6 +12,7 @@ PREC_FP64 = 0
 PREC_FP32 = 1
 
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 
 
 class LareGrid(C.Structure):
@@ -70,6 +71,19 @@ DEVICE_SYMBOLS = {
     "lare_dev_step": [C.c_int32, _dp, _dp],
     "lare_dev_profile": [C.c_int32],
     "lare_dev_kernel_stats": [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), _dp, C.c_int32],
+    "lare_dev_sur_set_params": [_fp, C.c_int32],
+    "lare_dev_sur_get_params": [_fp, C.c_int32],
+    "lare_dev_sur_set_norm": [_fp, _fp, C.c_float],
+    "lare_dev_sur_get_norm": [_fp, _fp, _fp],
+    "lare_dev_sur_collect": [C.c_int32, C.c_uint64, C.POINTER(C.c_int32)],
+    "lare_dev_sur_rows": [_fp, _fp, C.c_int32],
+    "lare_dev_sur_fit_norm": [],
+    "lare_dev_sur_train": [C.c_int32, C.c_int32, C.c_float, C.c_uint64, _fp],
+    "lare_dev_sur_forward": [_fp, C.c_int32, _fp],
+    "lare_dev_sur_grad": [_fp, _fp, C.c_int32, _fp, _fp],
+    "lare_dev_sur_enable": [C.c_int32, C.c_double],
+    "lare_dev_sur_conduct": [_dp, C.POINTER(C.c_int32)],
+    "lare_dev_sur_stats": [C.POINTER(C.c_int32), C.POINTER(C.c_int32), _dp],
     "lare_dev_synchronize": [],
     "lare_dev_finalize": [],
     "lare_dev_last_error": [],

@@ -170,6 +170,39 @@ def arcade_conduction(nx=64, ny=64, nz=128, s_target=21, b0=0.1, shear=0.5):
     return grid, params, f, dt
 
 
+def harris_conduction(n=32, seed=0, s_target=9, kappa_0=1.0e-2, blobs=12):
+    """Training and evaluation states of the conduction surrogate: the
+    resistive Harris sheet (seeded velocity noise) with Braginskii conduction
+    on and a seeded temperature structure filling the box,
+    T = T_h (1 + sum_m A_m exp(-|x - x_m|^2 / w_m^2)) over `blobs` Gaussians
+    with A_m in [0.5, 3], w_m in [0.5, 2] and x_m uniform in the box.
+    Returns also the dt giving about s_target RKL2 stages."""
+    grid, params, f = harris(n, seed=seed)
+    params.conduction = 1
+    params.kappa_0 = kappa_0
+    params.flux_limiter = 0.1
+    params.heat_flux_limiter = 1
+    rng = np.random.default_rng(1000 + seed)
+    gam = params.gamma
+    tfac = (gam - 1.0) / (2.0 - params.xi_n)
+    xs = _coords(grid, "ccc", (-1, -1, -1))
+    X, Y, Z = np.meshgrid(xs[0], xs[1], xs[2], indexing="ij")
+    bump = np.ones_like(X)
+    for _ in range(blobs):
+        amp = rng.uniform(0.5, 3.0)
+        w = rng.uniform(0.5, 2.0)
+        x0 = [rng.uniform(grid.extent[a][0], grid.extent[a][1]) for a in range(3)]
+        bump += amp * np.exp(-((X - x0[0]) ** 2 + (Y - x0[1]) ** 2 + (Z - x0[2]) ** 2) / w ** 2)
+    f.energy[...] = f.energy * bump
+    tmax = float((tfac * f.energy).max())
+    dmin = min(min(grid.dxb[a]) for a in range(3))
+    rmin = float(f.rho.min())
+    dtp = 0.9 / math.sqrt(3.0) * rmin * dmin ** 2 / (2.0 * kappa_0 * tmax ** 2.5)
+    r = ((2 * s_target - 1 + 1.0) ** 2 - 9.0) / 16.0
+    dt = 0.98 * r * dtp
+    return grid, params, f, dt
+
+
 def uniform(n=8, bcs=(BC_PERIODIC,) * 6, **kw):
     """A uniform state, for invariance tests."""
     grid = Grid(n, n, n, ((0.0, 1.0),) * 3, bcs)

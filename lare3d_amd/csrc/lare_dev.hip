@@ -21,6 +21,7 @@
 #include "conduct.hpp"
 #include "lagran.hpp"
 #include "remap.hpp"
+#include "surrogate.hpp"
 
 namespace {
 
@@ -520,6 +521,10 @@ int set_dt_and_stages(double dt_local, double dtr_local) {
   return 0;
 }
 
+// learned conduction surrogate (surrogate_rt.hpp)
+int sur_enabled();
+int sur_conduct_step(int delta_mode, double *res_out, int *acc_out);
+
 int lagrangian_step() {
   const int N0 = nx(), N1 = ny(), N2 = nz();
   LagArgs a = lag_args();  // dt = previous step's dt (Q1)
@@ -570,10 +575,15 @@ int lagrangian_step() {
   }
 
   if (S.p.conduction) {
-    rc = calc_s_stages();
-    if (rc) return rc;
-    rc = heat_conduct_sts2(1);
-    if (rc) return rc;
+    if (sur_enabled()) {
+      rc = sur_conduct_step(1, nullptr, nullptr);
+      if (rc) return rc;
+    } else {
+      rc = calc_s_stages();
+      if (rc) return rc;
+      rc = heat_conduct_sts2(1);
+      if (rc) return rc;
+    }
   } else {
     LAUNCH_BOX("energy_max", k_energy_max, cell_box(), S.energy);
   }
@@ -610,7 +620,9 @@ int lagrangian_step() {
 // ----------------------------------------------------------------------------
 // memory
 // ----------------------------------------------------------------------------
+void sur_free();
 void free_all() {
+  sur_free();
   if (S.stream) (void)hipStreamSynchronize(S.stream);
   for (double *p : S.allocs) (void)hipFree(p);
   S.allocs.clear();
@@ -951,3 +963,5 @@ int lare_dev_finalize(void) {
 }
 
 }  // extern "C"
+
+#include "surrogate_rt.hpp"

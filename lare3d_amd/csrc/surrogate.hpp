@@ -1,0 +1,398 @@
+// surrogate.hpp -- learned conduction surrogate: a 3x3x3-stencil MLP that
+// predicts the temperature increment of one whole RKL2 super-step
+// (conduct.f90:39 conduct_heat), with BF16 MFMA inference and training
+// kernels (FP32 accumulation) and a trapezoidal residual check that sends a
+// step back to the plain RKL2 solver when it fails.  No reference
+// counterpart.
+//
+// Network: x (27 neighbours x 12 channels = 324, channels 10 and 11 zero,
+//   padded with zeros to K = 336 = 21 MFMA steps)
+//   -> W1 (336 x 64) + b1 -> GELU -> W2 (64 x 64) + b2 -> GELU -> W3 (64) + b3
+//   -> y, with dT = y * y_scale.
+// Channels of a cell (standardised with the stored mean / std):
+//   0 T, 1-3 unit vector of the cell-centred B, 4 rho, 5 log dt, 6 |B|,
+//   7-9 centred differences of T along x, y, z.
+//
+// MFMA layout used throughout (v_mfma_f32_16x16x16_bf16, 64 lanes, l = lane):
+//   A (16 x 16, rows m, cols k): lane l holds A[l % 16][4 (l / 16) + r], r = 0..3
+//   B (16 x 16, rows k, cols n): lane l holds B[4 (l / 16) + r][l % 16]
+//   D (16 x 16):                 lane l holds D[4 (l / 16) + r][l % 16]
+// D has the layout of B, so a layer computed as (W^T)(H^T) feeds the next
+// layer's B operand from registers, with no transpose through LDS.
+#pragma once
+#include "common.hpp"
+
+#define SUR_CH 12            // channels stored per cell (10 used)
+#define SUR_NCH 10
+#define SUR_KREAL 324        // 27 * SUR_CH
+#define SUR_K 336            // SUR_KREAL padded to a multiple of 16
+#define SUR_H 64
+#define SUR_KP 344           // padded LDS row of W1^T (bf16 elements)
+#define SUR_HP 72            // padded LDS row of W2^T
+// parameter vector (float): W1[k][f], b1[f], W2[k][f], b2[f], W3[f], b3
+#define SUR_OFF_W1 0
+#define SUR_OFF_B1 (SUR_K * SUR_H)
+#define SUR_OFF_W2 (SUR_OFF_B1 + SUR_H)
+#define SUR_OFF_B2 (SUR_OFF_W2 + SUR_H * SUR_H)
+#define SUR_OFF_W3 (SUR_OFF_B2 + SUR_H)
+#define SUR_OFF_B3 (SUR_OFF_W3 + SUR_H)
+#define SUR_NPARAM (SUR_OFF_B3 + 1)
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16, round to nearest even (finite inputs).
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
+  bf16x4 v;
+  v[0] = (short)f2bf(a);
+  v[1] = (short)f2bf(b);
+  v[2] = (short)f2bf(c);
+  v[3] = (short)f2bf(d);
+  return v;
+}
+
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
+  const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
+  return cdf + x * pdf;
+}
+
+struct SurNorm {
+  float mu[SUR_CH], sd[SUR_CH];
+  float y_scale;
+};
+
+// Cell channels over cells 0..n+1 (box b).  Temperature T = tfac * e
+// (conduct.f90:72-73 with xi_n = 0 and zero ionisation energy).
+__global__ __launch_bounds__(BX * BY) void k_sur_features(Geo g, Box b, const double *__restrict__ energy,
+                                                          const double *__restrict__ rho,
+                                                          const double *__restrict__ bx,
+                                                          const double *__restrict__ by,
+                                                          const double *__restrict__ bz, float *__restrict__ feat,
+                                                          SurNorm nm, double tfac, double logdt, int normalise) {
+  BOX_INDEX(b, i, j, k);
+  const int c = g.idx(i, j, k);
+  const int X = 1, Y = g.sy, Z = g.sz;
+  const double T = tfac * energy[c];
+  const double b1x = (bx[c] + bx[c - X]) * 0.5;
+  const double b1y = (by[c] + by[c - Y]) * 0.5;
+  const double b1z = (bz[c] + bz[c - Z]) * 0.5;
+  const double bm = sqrt(b1x * b1x + b1y * b1y + b1z * b1z);
+  const double inv = 1.0 / (bm > 1.0e-30 ? bm : 1.0e-30);
+  const double gx = tfac * (energy[c + X] - energy[c - X]) / (g.dxc[i] + g.dxc[i - 1]);
+  const double gy = tfac * (energy[c + Y] - energy[c - Y]) / (g.dyc[j] + g.dyc[j - 1]);
+  const double gz = tfac * (energy[c + Z] - energy[c - Z]) / (g.dzc[k] + g.dzc[k - 1]);
+  float v[SUR_CH] = {(float)T, (float)(b1x * inv), (float)(b1y * inv), (float)(b1z * inv), (float)rho[c],
+                     (float)logdt, (float)bm, (float)gx, (float)gy, (float)gz, 0.0f, 0.0f};
+  float *o = feat + (size_t)c * SUR_CH;
+#pragma unroll
+  for (int q = 0; q < SUR_CH; ++q) o[q] = (normalise && q < SUR_NCH) ? (v[q] - nm.mu[q]) / nm.sd[q] : v[q];
+}
+
+// Neighbour q (0..26) of a stencil as a linear offset: q = 9 (dz+1) + 3 (dy+1) + (dx+1).
+__device__ __forceinline__ int nb_offset(const Geo &g, int q) {
+  return (q / 9 - 1) * g.sz + ((q / 3) % 3 - 1) * g.sy + (q % 3 - 1);
+}
+
+// splitmix64 hash (public-domain mixing function of Steele, Lea and Flood).
+__host__ __device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// Random interior cells for rows [0, rows) of a sample; cells[r] = linear index.
+__global__ void k_sur_sample_cells(Geo g, unsigned long long seed, int rows, int *__restrict__ cells) {
+  const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r >= rows) return;
+  const unsigned long long h = mix64(seed * 0x100000001B3ull + (unsigned long long)r);
+  const int i = 1 + (int)((h & 0xFFFFFull) % (unsigned long long)g.nx);
+  const int j = 1 + (int)(((h >> 20) & 0xFFFFFull) % (unsigned long long)g.ny);
+  const int k = 1 + (int)(((h >> 40) & 0xFFFFFull) % (unsigned long long)g.nz);
+  cells[r] = g.idx(i, j, k);
+}
+
+// X[base + r][q * 12 + ch] = feat[cells[r] + off(q)][ch]
+__global__ void k_sur_gather(Geo g, const float *__restrict__ feat, const int *__restrict__ cells, int rows,
+                             float *__restrict__ Xout) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= rows * SUR_K) return;
+  const int r = t / SUR_K, kk = t % SUR_K;
+  const int q = kk / SUR_CH, ch = kk % SUR_CH;
+  Xout[t] = kk < SUR_KREAL ? feat[(size_t)(cells[r] + nb_offset(g, q)) * SUR_CH + ch] : 0.0f;
+}
+
+// target dT of the sampled cells after one plain super-step
+__global__ void k_sur_target(const double *__restrict__ E0, const double *__restrict__ E1,
+                             const int *__restrict__ cells, int rows, double tfac, float *__restrict__ y) {
+  const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r >= rows) return;
+  const int c = cells[r];
+  y[r] = (float)(tfac * (E1[c] - E0[c]));
+}
+
+// standardise stored raw rows in place: x = (x - mu) / sd per channel; y /= y_scale
+__global__ void k_sur_standardise(float *__restrict__ X, float *__restrict__ y, int rows, SurNorm nm) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t < rows * SUR_K) {
+    const int kk = t % SUR_K, ch = kk % SUR_CH;
+    X[t] = (kk < SUR_KREAL && ch < SUR_NCH) ? (X[t] - nm.mu[ch]) / nm.sd[ch] : 0.0f;
+  }
+  if (t < rows) y[t] = y[t] / nm.y_scale;
+}
+
+// ----------------------------------------------------------------------------
+// training
+// ----------------------------------------------------------------------------
+// C(m, n) = sum_k A(m, k) B(k, n) with BF16 operands (rounded from FP32) and
+// FP32 accumulation; element (m, k) of A at A[m * sam + k * sak], and so on.
+// M, N, K multiples of 16; one wave per 16 x 16 tile of C.
+__global__ __launch_bounds__(64) void k_gemm_bf16(int M, int N, int K, const float *__restrict__ A, int sam, int sak,
+                                                  const float *__restrict__ B, int sbk, int sbn,
+                                                  float *__restrict__ C, int scm, int scn) {
+  const int l = threadIdx.x;
+  const int n0 = blockIdx.x * 16, m0 = blockIdx.y * 16;
+  if (m0 >= M || n0 >= N) return;
+  const int am = m0 + (l & 15), bn = n0 + (l & 15), kq = 4 * (l >> 4);
+  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    const int kb = k0 + kq;
+    const float *pa = A + (size_t)am * sam + (size_t)kb * sak;
+    const float *pb = B + (size_t)kb * sbk + (size_t)bn * sbn;
+    const bf16x4 a = pack4(pa[0], pa[sak], pa[2 * sak], pa[3 * sak]);
+    const bf16x4 b = pack4(pb[0], pb[sbk], pb[2 * sbk], pb[3 * sbk]);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+  const int cn = n0 + (l & 15);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) C[(size_t)(m0 + kq + r) * scm + (size_t)cn * scn] = acc[r];
+}
+
+// Z[f][b] += bias[f]; H = GELU(Z)   (feature-major, rows of length nb)
+__global__ void k_bias_gelu(float *__restrict__ Z, float *__restrict__ H, const float *__restrict__ bias, int nf,
+                            int nb) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= nf * nb) return;
+  const float z = Z[t] + bias[t / nb];
+  Z[t] = z;
+  H[t] = gelu(z);
+}
+
+// output layer, loss and its gradient over columns b < nb (row length nb):
+// y[b] = sum_f W3[f] H2[f][b] + b3; for the first nreal columns with a
+// target, dy[b] = 2 (y - t) / nreal and lossacc[block] = sum (y - t)^2;
+// dy = 0 on padding columns.
+__global__ __launch_bounds__(256) void k_out_layer(const float *__restrict__ H2, const float *__restrict__ w,
+                                                   const float *__restrict__ tgt, int nb, int nreal,
+                                                   float *__restrict__ yout, float *__restrict__ dy,
+                                                   float *__restrict__ lossacc) {
+  __shared__ float red[4];
+  const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  float e2 = 0.0f;
+  if (b < nb) {
+    float y = w[SUR_OFF_B3];
+    for (int f = 0; f < SUR_H; ++f) y += w[SUR_OFF_W3 + f] * H2[(size_t)f * nb + b];
+    yout[b] = y;
+    if (tgt) {
+      float d = 0.0f;
+      if (b < nreal) {
+        const float e = y - tgt[b];
+        e2 = e * e;
+        d = 2.0f * e / (float)nreal;
+      }
+      dy[b] = d;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) e2 += __shfl_down(e2, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e2;
+  __syncthreads();
+  if (threadIdx.x == 0) lossacc[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// dZ2[f][b] = W3[f] dy[b] * gelu'(Z2[f][b])
+__global__ void k_back_out(const float *__restrict__ Z2, const float *__restrict__ w, const float *__restrict__ dy,
+                           int nb, float *__restrict__ dZ2) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= SUR_H * nb) return;
+  const int f = t / nb, b = t % nb;
+  dZ2[t] = w[SUR_OFF_W3 + f] * dy[b] * gelu_grad(Z2[t]);
+}
+
+// dZ[t] = dH[t] * gelu'(Z[t])
+__global__ void k_gelu_back(const float *__restrict__ Z, const float *__restrict__ dH, int n, float *__restrict__ dZ) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= n) return;
+  dZ[t] = dH[t] * gelu_grad(Z[t]);
+}
+
+// out[r] = sum_b M[r][b] * (v ? v[b] : 1)  (one block per row)
+__global__ __launch_bounds__(256) void k_rowdot(const float *__restrict__ Mx, const float *__restrict__ v, int nb,
+                                                float *__restrict__ out) {
+  __shared__ float red[4];
+  const int r = blockIdx.x;
+  float s = 0.0f;
+  for (int b = threadIdx.x; b < nb; b += 256) s += Mx[(size_t)r * nb + b] * (v ? v[b] : 1.0f);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[r] = red[0] + red[1] + red[2] + red[3];
+}
+
+// mini-batch of a training step: rows picked by hash (with replacement)
+__global__ void k_sur_batch(const float *__restrict__ X, const float *__restrict__ y, int rows,
+                            unsigned long long seed, int nb, float *__restrict__ Xb, float *__restrict__ yb) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= nb * SUR_K) return;
+  const int b = t / SUR_K, kk = t % SUR_K;
+  const int r = (int)(mix64(seed + (unsigned long long)b) % (unsigned long long)rows);
+  Xb[t] = X[(size_t)r * SUR_K + kk];
+  if (kk == 0) yb[b] = y[r];
+}
+
+// *out = scale * sum of n partials (one thread)
+__global__ void k_sum_into(const float *__restrict__ part, int n, float scale, float *__restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  float s = 0.0f;
+  for (int q = 0; q < n; ++q) s += part[q];
+  *out = s * scale;
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master parameters.
+__global__ void k_adam(float *__restrict__ w, const float *__restrict__ gr, float *__restrict__ m,
+                       float *__restrict__ v, int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= n) return;
+  const float gg = gr[t];
+  const float mm = b1 * m[t] + (1.0f - b1) * gg;
+  const float vv = b2 * v[t] + (1.0f - b2) * gg * gg;
+  m[t] = mm;
+  v[t] = vv;
+  w[t] = w[t] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+// ----------------------------------------------------------------------------
+// inference over the interior
+// ----------------------------------------------------------------------------
+// BF16 copies of W1^T (64 x SUR_KP) and W2^T (64 x SUR_HP) for the LDS of the
+// inference kernel.
+__global__ void k_sur_pack(const float *__restrict__ w, unsigned short *__restrict__ w1t,
+                           unsigned short *__restrict__ w2t) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t < SUR_H * SUR_KP) {
+    const int f = t / SUR_KP, kk = t % SUR_KP;
+    w1t[t] = kk < SUR_K ? f2bf(w[SUR_OFF_W1 + kk * SUR_H + f]) : 0;
+  }
+  if (t < SUR_H * SUR_HP) {
+    const int f = t / SUR_HP, kk = t % SUR_HP;
+    w2t[t] = kk < SUR_H ? f2bf(w[SUR_OFF_W2 + kk * SUR_H + f]) : 0;
+  }
+}
+
+#define SUR_INFER_WAVES 4
+// E1 = E0 + dT / tfac over the interior, 16 cells (consecutive in x) per
+// wave-group.  Features must be valid on cells 0..n+1.
+__global__ __launch_bounds__(64 * SUR_INFER_WAVES) void k_sur_infer(
+    Geo g, const float *__restrict__ feat, const unsigned short *__restrict__ w1t,
+    const unsigned short *__restrict__ w2t, const float *__restrict__ w, const double *__restrict__ E0,
+    double *__restrict__ E1, double tfac, float y_scale, int ngroups) {
+  __shared__ __attribute__((aligned(16))) unsigned short sW1[SUR_H * SUR_KP];
+  __shared__ __attribute__((aligned(16))) unsigned short sW2[SUR_H * SUR_HP];
+  __shared__ float sB1[SUR_H], sB2[SUR_H], sW3[SUR_H];
+  for (int t = threadIdx.x; t < SUR_H * SUR_KP; t += blockDim.x) sW1[t] = w1t[t];
+  for (int t = threadIdx.x; t < SUR_H * SUR_HP; t += blockDim.x) sW2[t] = w2t[t];
+  for (int t = threadIdx.x; t < SUR_H; t += blockDim.x) {
+    sB1[t] = w[SUR_OFF_B1 + t];
+    sB2[t] = w[SUR_OFF_B2 + t];
+    sW3[t] = w[SUR_OFF_W3 + t];
+  }
+  __syncthreads();
+  const float b3 = w[SUR_OFF_B3];
+  const int l = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int col = l & 15, kq = 4 * (l >> 4);
+  const int gpr = (g.nx + 15) / 16;  // groups per x-row
+  for (int gi = blockIdx.x * SUR_INFER_WAVES + wave; gi < ngroups; gi += gridDim.x * SUR_INFER_WAVES) {
+    const int gx = gi % gpr, row = gi / gpr;
+    const int j = 1 + row % g.ny, k = 1 + row / g.ny;
+    const int i = 1 + gx * 16 + col;
+    const int ic = i <= g.nx ? i : g.nx;  // clamp loads of the masked lanes
+    const int c = g.idx(ic, j, k);
+    // layer 1: Z1^T (64 x 16 cells) = W1^T (64 x 336) . X^T (336 x 16)
+    f32x4 acc[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int kk = 0; kk < SUR_K / 16; ++kk) {
+      const int kb = kk * 16 + kq;
+      const int q = kb / SUR_CH, ch = kb % SUR_CH;
+      float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (kb < SUR_KREAL) x = *(const float4 *)(feat + (size_t)(c + nb_offset(g, q)) * SUR_CH + ch);
+      const bf16x4 bop = pack4(x.x, x.y, x.z, x.w);
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        const bf16x4 aop = *(const bf16x4 *)(sW1 + (mt * 16 + col) * SUR_KP + kb);
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aop, bop, acc[mt], 0, 0, 0);
+      }
+    }
+    bf16x4 h1[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      float hv[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) hv[r] = gelu(acc[mt][r] + sB1[mt * 16 + kq + r]);
+      h1[mt] = pack4(hv[0], hv[1], hv[2], hv[3]);
+    }
+    // layer 2: Z2^T = W2^T (64 x 64) . H1^T (64 x 16); H1^T fragments are h1[]
+    f32x4 acc2[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      acc2[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const bf16x4 aop = *(const bf16x4 *)(sW2 + (mt * 16 + col) * SUR_HP + kk * 16 + kq);
+        acc2[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aop, h1[kk], acc2[mt], 0, 0, 0);
+      }
+    }
+    // layer 3 (FP32): y = W3 . GELU(Z2 + b2) + b3, summed over the 4 lane groups
+    float part = 0.0f;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int f = mt * 16 + kq + r;
+        part += sW3[f] * gelu(acc2[mt][r] + sB2[f]);
+      }
+    part += __shfl_xor(part, 16, 64);
+    part += __shfl_xor(part, 32, 64);
+    if (l < 16 && i <= g.nx) {
+      const double dT = (double)((part + b3) * y_scale);
+      E1[c] = E0[c] + dT / tfac;
+    }
+  }
+}
+
+// Trapezoidal residual of a predicted super-step over the interior:
+// partial sums of (E1 - E0 - trap)^2 and E0^2, with trap = dt/2 (L(E0) + L(E1)).
+__global__ __launch_bounds__(BX * BY) void k_sur_residual(Geo g, const double *__restrict__ E0,
+                                                          const double *__restrict__ E1,
+                                                          const double *__restrict__ trap,
+                                                          double *__restrict__ partial) {
+  const int i = 1 + (int)(blockIdx.x * BX + threadIdx.x);
+  const int j = 1 + (int)(blockIdx.y * BY + threadIdx.y);
+  const int k = 1 + (int)blockIdx.z;
+  double v[2] = {0.0, 0.0};
+  if (i <= g.nx && j <= g.ny && k <= g.nz) {
+    const int c = g.idx(i, j, k);
+    const double r = (E1[c] - E0[c]) - trap[c];
+    v[0] = r * r;
+    v[1] = E0[c] * E0[c];
+  }
+  const int ops[2] = {1, 1};
+  block_reduce_store<2>(v, ops, partial);
+}

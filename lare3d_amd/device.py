@@ -36,6 +36,13 @@ class Device:
         st = self._f.to_struct(with_visc3=bool(params.use_viscous_damping))
         check(self.lib, self.lib.lare_dev_upload(C.byref(st)), "lare_dev_upload")
 
+    def upload(self, fields):
+        """Replace the device state with new host fields (same grid and
+        parameters; the surrogate's parameters and rows are kept)."""
+        self._f = fields.copy()
+        st = self._f.to_struct(with_visc3=bool(self._p.use_viscous_damping))
+        check(self.lib, self.lib.lare_dev_upload(C.byref(st)), "lare_dev_upload")
+
     def close(self):
         if self.lib is not None:
             self.lib.lare_dev_finalize()
