@@ -7,10 +7,12 @@ configuration at 256^3 in FP64 -- Bx = tanh(y/0.5), rho = 1 + sech^2(y/0.5),
 pressure balance, eta = 1e-3 plus 1e-2 above |J| = 1, seeded 1% velocity
 noise, shock viscosity on, periodic x/z and BC_USER walls in y.
 
-Prints one JSON line: cell-updates/s over the timed steps (inputs resident
-in HBM), the roofline of the dominant kernel from HIP events on the library
-stream, and the CPU restatement timed on a bounded sample of the same
-workload as a reported baseline.
+Prints one JSON line: cell-updates/s of the FP64 path over the timed steps
+(inputs resident in HBM), the roofline of the dominant kernel from HIP events
+on the library stream, the same workload with FP32 bulk storage ("fp32":
+throughput and the relative L2 error of every field against the FP64 run
+after the same steps), and the CPU restatement timed on a bounded sample of
+the same workload as a reported baseline.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
 """
@@ -130,6 +132,50 @@ def copy_probe(nbytes=1 << 30, reps=3):
     del a, b
 
 
+def timed_run(dev, warmup, steps):
+    """warmup untimed steps, then `steps` timed ones; (CU/s, ms/step, last step)."""
+    n = dev.n
+    dev.boundary_conditions()
+    dev.eta_calc()
+    step = 0
+    for _ in range(warmup):
+        step += 1
+        dev.step(step)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step += 1
+        dev.step(step)
+    dev.synchronize()
+    elapsed = time.perf_counter() - t0
+    return n[0] * n[1] * n[2] * steps / elapsed, 1e3 * elapsed / steps, step
+
+
+def fp32_run(n, warmup, steps, f64_end):
+    """The same workload with FP32 bulk storage: throughput, and the relative
+    L2 error of every field against the FP64 run after the same steps."""
+    from lare3d_amd import configs
+    from lare3d_amd.abi import PREC_FP32
+    from lare3d_amd.device import Device
+    grid, params, fields = configs.harris(n)
+    params.precision = PREC_FP32
+    dev = Device(grid, params, fields)
+    cups, ms, _ = timed_run(dev, warmup, steps)
+    f32_end = dev.fields()
+    _, divb = dev.energy_account(with_divb=True)
+    dev.close()
+    errs = {}
+    # scalar fields, and the velocity and magnetic field as vectors
+    for name, comps in (("rho", ("rho",)), ("energy", ("energy",)), ("v", ("vx", "vy", "vz")),
+                        ("B", ("bx", "by", "bz"))):
+        a = np.concatenate([f32_end.interior(c).ravel() for c in comps])
+        b = np.concatenate([f64_end.interior(c).ravel() for c in comps])
+        errs[name] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    return {"value": cups, "unit": "cell-updates/s", "ms_per_step": ms,
+            "storage": "FP32 bulk fields; FP64 face B, CT fluxes, metrics, reductions and arithmetic",
+            "field_rel_l2_vs_fp64": errs, "steps_compared": warmup + steps, "max_divb_dx": divb}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +186,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=96)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy and exit (counter calibration for HBM traffic; "
                          "run it as its own process under the same counters)")
@@ -158,23 +205,10 @@ def main():
     n = args.n
     grid, params, fields = configs.harris(n)
     dev = Device(grid, params, fields)
-    dev.boundary_conditions()
-    dev.eta_calc()
-    step = 0
-    for _ in range(args.warmup):
-        step += 1
-        dev.step(step)
-    dev.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step += 1
-        dev.step(step)
-    dev.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1e3 * elapsed / args.steps
-    cups = n ** 3 * args.steps / elapsed
+    cups, ms_per_step, step = timed_run(dev, args.warmup, args.steps)
     sc = dev.scalars()
     en, divb = dev.energy_account(with_divb=True)
+    f64_end = dev.fields() if not args.no_fp32 else None
 
     # per-kernel durations from HIP events on the library stream
     dev.profile(True)
@@ -199,6 +233,10 @@ def main():
                  "unit": "GB/s", "frac": round(cups * bpc / 1e9 / HBM_PEAK_GBS, 4)}
     top = {k: {"launches": v[0], "ms": round(v[1], 3)} for k, v in ranked[:12]}
     dev.close()
+    fp32 = None
+    if not args.no_fp32:
+        fp32 = fp32_run(n, args.warmup, args.steps, f64_end)
+        fp32["speedup_vs_fp64"] = fp32["value"] / cups
 
     cpu = None
     if not args.no_cpu:
@@ -227,6 +265,7 @@ def main():
         "roofline": roofline,
         "step_roofline": step_roof,
         "cpu_baseline": cpu,
+        "fp32": fp32,
         "kernels": top,
         "diagnostics": {"energy": en, "max_divb_dx": divb, "dt": sc.dt},
     }

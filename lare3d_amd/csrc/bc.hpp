@@ -8,8 +8,6 @@
 // before all user fills (x, y, z); the host launches one kernel per axis in
 // that order.  One launch handles up to 8 fields so that a whole
 // boundary_conditions call is at most three launches.
-#pragma once
-#include "common.hpp"
 
 enum BcOp {
   BC_NONE = 0,
@@ -24,10 +22,67 @@ enum BcOp {
 
 #define BC_MAXF 8
 struct BcSet {
-  double *f[BC_MAXF];
+  void *f[BC_MAXF];
   int op[BC_MAXF];
+  int wide[BC_MAXF];  // 1: FP64 array, 0: FP32 array
   int nf;
 };
+
+template <class T>
+__device__ __forceinline__ void bc_line(T *a, int op, int base, int st, int n, bool dm_ok) {
+  switch (op) {
+    case BC_PER_CELL:
+      a[base + (n + 1) * st] = a[base + 1 * st];
+      a[base + (n + 2) * st] = a[base + 2 * st];
+      a[base - 1 * st] = a[base + (n - 1) * st];
+      a[base + 0 * st] = a[base + n * st];
+      break;
+    case BC_PER_FACE:
+      a[base + (n + 1) * st] = a[base + 1 * st];
+      a[base + (n + 2) * st] = a[base + 2 * st];
+      a[base - 2 * st] = a[base + (n - 2) * st];
+      a[base - 1 * st] = a[base + (n - 1) * st];
+      a[base + 0 * st] = a[base + n * st];
+      break;
+    case BC_PER_NODE: {
+      const T v0 = a[base], v1 = a[base + st], v2 = a[base + 2 * st];
+      a[base + n * st] = v0;
+      a[base + (n + 1) * st] = v1;
+      a[base + (n + 2) * st] = v2;
+      a[base - 2 * st] = a[base + (n - 2) * st];
+      a[base - 1 * st] = a[base + (n - 1) * st];
+      break;
+    }
+    case BC_MIR_CELL:
+      a[base + 0 * st] = a[base + 1 * st];
+      a[base - 1 * st] = a[base + 2 * st];
+      a[base + (n + 1) * st] = a[base + n * st];
+      a[base + (n + 2) * st] = a[base + (n - 1) * st];
+      break;
+    case BC_MIR_FACE:
+      a[base - 1 * st] = a[base + 1 * st];
+      a[base - 2 * st] = a[base + 2 * st];
+      a[base + (n + 1) * st] = a[base + (n - 1) * st];
+      a[base + (n + 2) * st] = a[base + (n - 2) * st];
+      break;
+    case BC_ZERO_NODE:
+      a[base - 2 * st] = (T)0;
+      a[base - 1 * st] = (T)0;
+      a[base + 0 * st] = (T)0;
+      a[base + n * st] = (T)0;
+      a[base + (n + 1) * st] = (T)0;
+      a[base + (n + 2) * st] = (T)0;
+      break;
+    case BC_DM_CLAMP:
+      if (dm_ok) {
+        a[base + (n + 1) * st] = a[base + n * st];
+        a[base - 1 * st] = a[base + 0 * st];
+      }
+      break;
+    default:
+      break;
+  }
+}
 
 __global__ __launch_bounds__(64) void k_bc_axis(Geo g, BcSet s, int axis) {
   // plane coordinates: u runs along the contiguous axis where possible
@@ -43,59 +98,9 @@ __global__ __launch_bounds__(64) void k_bc_axis(Geo g, BcSet s, int axis) {
   c[u] = iu;
   c[w] = iw;
   const int base = g.idx(c[0], c[1], c[2]);  // index of layer 0 on this line
+  const bool dm_ok = iu >= 0 && iu <= g.n(u) + 1 && iw >= 0 && iw <= g.n(w) + 1;
   for (int q = 0; q < s.nf; ++q) {
-    double *a = s.f[q];
-    switch (s.op[q]) {
-      case BC_PER_CELL:
-        a[base + (n + 1) * st] = a[base + 1 * st];
-        a[base + (n + 2) * st] = a[base + 2 * st];
-        a[base - 1 * st] = a[base + (n - 1) * st];
-        a[base + 0 * st] = a[base + n * st];
-        break;
-      case BC_PER_FACE:
-        a[base + (n + 1) * st] = a[base + 1 * st];
-        a[base + (n + 2) * st] = a[base + 2 * st];
-        a[base - 2 * st] = a[base + (n - 2) * st];
-        a[base - 1 * st] = a[base + (n - 1) * st];
-        a[base + 0 * st] = a[base + n * st];
-        break;
-      case BC_PER_NODE: {
-        const double v0 = a[base], v1 = a[base + st], v2 = a[base + 2 * st];
-        a[base + n * st] = v0;
-        a[base + (n + 1) * st] = v1;
-        a[base + (n + 2) * st] = v2;
-        a[base - 2 * st] = a[base + (n - 2) * st];
-        a[base - 1 * st] = a[base + (n - 1) * st];
-        break;
-      }
-      case BC_MIR_CELL:
-        a[base + 0 * st] = a[base + 1 * st];
-        a[base - 1 * st] = a[base + 2 * st];
-        a[base + (n + 1) * st] = a[base + n * st];
-        a[base + (n + 2) * st] = a[base + (n - 1) * st];
-        break;
-      case BC_MIR_FACE:
-        a[base - 1 * st] = a[base + 1 * st];
-        a[base - 2 * st] = a[base + 2 * st];
-        a[base + (n + 1) * st] = a[base + (n - 1) * st];
-        a[base + (n + 2) * st] = a[base + (n - 2) * st];
-        break;
-      case BC_ZERO_NODE:
-        a[base - 2 * st] = 0.0;
-        a[base - 1 * st] = 0.0;
-        a[base + 0 * st] = 0.0;
-        a[base + n * st] = 0.0;
-        a[base + (n + 1) * st] = 0.0;
-        a[base + (n + 2) * st] = 0.0;
-        break;
-      case BC_DM_CLAMP:
-        if (iu >= 0 && iu <= g.n(u) + 1 && iw >= 0 && iw <= g.n(w) + 1) {
-          a[base + (n + 1) * st] = a[base + n * st];
-          a[base - 1 * st] = a[base + 0 * st];
-        }
-        break;
-      default:
-        break;
-    }
+    if (s.wide[q]) bc_line((double *)s.f[q], s.op[q], base, st, n, dm_ok);
+    else bc_line((float *)s.f[q], s.op[q], base, st, n, dm_ok);
   }
 }

@@ -8,10 +8,7 @@
 // the row pitch sy is a multiple of 16 elements.  All fields share the
 // same index arithmetic, which lets one kernel address cell, node and face
 // arrays with one offset.
-#pragma once
 
-#include <hip/hip_runtime.h>
-#include <stdint.h>
 
 #define OX 16
 

@@ -16,8 +16,6 @@
 //    energy_bcs at conduct.f90:403).
 //  * the Y(0:3) buffer copies of conduct.f90:400-406 become a rotation of
 //    three device buffers.
-#pragma once
-#include "common.hpp"
 
 #define MIN_B 1.0e-10
 
@@ -31,8 +29,8 @@ __device__ __forceinline__ int mirror_idx(int c, int n) {
 }
 
 // Energy-like value at (i,j,k) under a ghost-read mode.
-__device__ __forceinline__ double ghost_read(const Geo &g, const double *__restrict__ Y,
-                                             const double *__restrict__ E0, int i, int j, int k, int mode) {
+__device__ __forceinline__ double ghost_read(const Geo &g, const real *__restrict__ Y,
+                                             const real *__restrict__ E0, int i, int j, int k, int mode) {
   if (mode == TMODE_DIRECT) return Y[g.idx(i, j, k)];
   int c[3] = {i, j, k};
   bool stale = false;
@@ -57,8 +55,8 @@ struct CondArgs {
 // Larsen flux-limiter factor over cells 0..n and the explicit parabolic
 // time-step minimum over 1..n (calc_s_stages, conduct.f90:69-129).
 // Temperatures from energy as stored (ghosts valid at call time).
-__global__ __launch_bounds__(BX * BY) void k_larsen(Geo g, const double *__restrict__ energy, const double *__restrict__ rho,
-                         double *__restrict__ larsen, double *__restrict__ partial, double flux_limiter,
+__global__ __launch_bounds__(BX * BY) void k_larsen(Geo g, const real *__restrict__ energy, const real *__restrict__ rho,
+                         real *__restrict__ larsen, double *__restrict__ partial, double flux_limiter,
                          double kappa_0, double tfac, int limiter_on) {
   const int i = (int)(blockIdx.x * BX + threadIdx.x);
   const int j = (int)(blockIdx.y * BY + threadIdx.y);
@@ -168,10 +166,10 @@ __device__ __forceinline__ double flux_z_face(const Geo &g, const Tn &T, int lz,
 }
 
 // Net heat flux into cell (i,j,k) (the value heat_flux leaves in flux(i,j,k)).
-__device__ __forceinline__ double cell_heat_flux(const Geo &g, const double *__restrict__ Y,
-                                                 const double *__restrict__ E0, int mode, double tfac,
+__device__ __forceinline__ double cell_heat_flux(const Geo &g, const real *__restrict__ Y,
+                                                 const real *__restrict__ E0, int mode, double tfac,
                                                  const double *__restrict__ bx, const double *__restrict__ by,
-                                                 const double *__restrict__ bz, const double *__restrict__ larsen,
+                                                 const double *__restrict__ bz, const real *__restrict__ larsen,
                                                  double kappa_0, int i, int j, int k) {
   Tn T;
 #pragma unroll
@@ -204,10 +202,10 @@ __device__ __forceinline__ double cell_heat_flux(const Geo &g, const double *__r
 }
 
 // Stage 1 (conduct.f90:345-367): Lc = flux/rho, Y1 = Y0 + mu~1 dt Lc.
-__global__ __launch_bounds__(BX * BY) void k_rkl2_stage1(Geo g, Box b, const double *__restrict__ E0, const double *__restrict__ rho,
+__global__ __launch_bounds__(BX * BY) void k_rkl2_stage1(Geo g, Box b, const real *__restrict__ E0, const real *__restrict__ rho,
                               const double *__restrict__ bx, const double *__restrict__ by,
-                              const double *__restrict__ bz, const double *__restrict__ larsen,
-                              double *__restrict__ Lc, double *__restrict__ Yout, CondArgs a, double mu_t1) {
+                              const double *__restrict__ bz, const real *__restrict__ larsen,
+                              real *__restrict__ Lc, real *__restrict__ Yout, CondArgs a, double mu_t1) {
   BOX_INDEX(b, i, j, k);
   const double f = cell_heat_flux(g, E0, E0, TMODE_DIRECT, a.tfac, bx, by, bz, larsen, a.kappa_0, i, j, k);
   const int c = g.idx(i, j, k);
@@ -218,11 +216,11 @@ __global__ __launch_bounds__(BX * BY) void k_rkl2_stage1(Geo g, Box b, const dou
 }
 
 // Stage j >= 2 (conduct.f90:369-396).
-__global__ __launch_bounds__(BX * BY) void k_rkl2_stage(Geo g, Box b, const double *__restrict__ Y2, const double *__restrict__ Y1,
-                             const double *__restrict__ E0, const double *__restrict__ rho,
+__global__ __launch_bounds__(BX * BY) void k_rkl2_stage(Geo g, Box b, const real *__restrict__ Y2, const real *__restrict__ Y1,
+                             const real *__restrict__ E0, const real *__restrict__ rho,
                              const double *__restrict__ bx, const double *__restrict__ by,
-                             const double *__restrict__ bz, const double *__restrict__ larsen,
-                             const double *__restrict__ Lc, double *__restrict__ Yout, CondArgs a, int mode,
+                             const double *__restrict__ bz, const real *__restrict__ larsen,
+                             const real *__restrict__ Lc, real *__restrict__ Yout, CondArgs a, int mode,
                              double mu, double nu, double one_m_mu_nu, double mu_t, double gam_t) {
   BOX_INDEX(b, i, j, k);
   const double f = cell_heat_flux(g, Y2, E0, mode, a.tfac, bx, by, bz, larsen, a.kappa_0, i, j, k);
@@ -239,7 +237,7 @@ __global__ __launch_bounds__(BX * BY) void k_rkl2_stage(Geo g, Box b, const doub
 // the last stage, ghosts by energy_bcs (index map).  delta_mode != 0 applies
 // lagrangian_step's energy0 + (energy - energy0) and MAX(., 0)
 // (lagran.F90:168-181) over the whole cell box.
-__global__ __launch_bounds__(BX * BY) void k_rkl2_final(Geo g, Box b, const double *__restrict__ Ys, double *__restrict__ energy,
+__global__ __launch_bounds__(BX * BY) void k_rkl2_final(Geo g, Box b, const real *__restrict__ Ys, real *__restrict__ energy,
                              int delta_mode) {
   BOX_INDEX(b, i, j, k);
   const double v = ghost_read(g, Ys, Ys, i, j, k, TMODE_WRAP);

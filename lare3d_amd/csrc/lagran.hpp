@@ -15,8 +15,6 @@
 //   k_shock_corr      shock_heating (:861-937) fused with the corrector
 //                     (:415-462) over cells 1..n
 //   k_vupdate         v = 2 v1 - v (:399-407)
-#pragma once
-#include "common.hpp"
 
 #define NONE_ZERO 2.2250738585072014e-308
 #define LARGEST 1.7976931348623157e+308
@@ -41,8 +39,8 @@ __device__ __forceinline__ double cv_sum8(const Geo &g, int i, int j, int k) {
 }
 
 // fast magnetosonic speed of shock_viscosity (:554-556) in cell c
-__device__ __forceinline__ double cell_cs(const Geo &g, const double *__restrict__ rho,
-                                          const double *__restrict__ energy, const double *__restrict__ bx,
+__device__ __forceinline__ double cell_cs(const Geo &g, const real *__restrict__ rho,
+                                          const real *__restrict__ energy, const double *__restrict__ bx,
                                           const double *__restrict__ by, const double *__restrict__ bz, int c,
                                           double gamma) {
   const double r = rho[c];
@@ -55,9 +53,9 @@ __device__ __forceinline__ double cell_cs(const Geo &g, const double *__restrict
   return sqrt((gamma * p + 1.0 * b2) / rmin);
 }
 
-__global__ __launch_bounds__(BX * BY) void k_vertex_prep(Geo g, Box b, const double *__restrict__ rho, const double *__restrict__ energy,
+__global__ __launch_bounds__(BX * BY) void k_vertex_prep(Geo g, Box b, const real *__restrict__ rho, const real *__restrict__ energy,
                               const double *__restrict__ bx, const double *__restrict__ by,
-                              const double *__restrict__ bz, double *__restrict__ rho_v, double *__restrict__ cs_v,
+                              const double *__restrict__ bz, real *__restrict__ rho_v, real *__restrict__ cs_v,
                               double gamma) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
@@ -82,9 +80,9 @@ __global__ __launch_bounds__(BX * BY) void k_vertex_prep(Geo g, Box b, const dou
 }
 
 // edge_viscosity (lagran.F90:795-855) between nodes n1 and n2 (linear idx)
-__device__ __forceinline__ double edge_visc(const double *__restrict__ vx, const double *__restrict__ vy,
-                                            const double *__restrict__ vz, const double *__restrict__ rho_v,
-                                            const double *__restrict__ cs_v, int n1, int n2, double dvdots,
+__device__ __forceinline__ double edge_visc(const real *__restrict__ vx, const real *__restrict__ vy,
+                                            const real *__restrict__ vz, const real *__restrict__ rho_v,
+                                            const real *__restrict__ cs_v, int n1, int n2, double dvdots,
                                             double dx, const LagArgs &a) {
   dvdots = -fabs(dvdots);
   const double r1 = rho_v[n1], r2 = rho_v[n2];
@@ -104,10 +102,10 @@ __device__ __forceinline__ double edge_visc(const double *__restrict__ vx, const
   return q_k_bar * (1.0 - psi) * dvdots;
 }
 
-__global__ __launch_bounds__(BX * BY) void k_alpha(Geo g, Box b, const double *__restrict__ vx, const double *__restrict__ vy,
-                        const double *__restrict__ vz, const double *__restrict__ rho_v,
-                        const double *__restrict__ cs_v, double *__restrict__ al1, double *__restrict__ al2,
-                        double *__restrict__ al3, LagArgs a) {
+__global__ __launch_bounds__(BX * BY) void k_alpha(Geo g, Box b, const real *__restrict__ vx, const real *__restrict__ vy,
+                        const real *__restrict__ vz, const real *__restrict__ rho_v,
+                        const real *__restrict__ cs_v, real *__restrict__ al1, real *__restrict__ al2,
+                        real *__restrict__ al3, LagArgs a) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -146,8 +144,8 @@ __device__ __forceinline__ void edge_nodes(int c, int sy, int sz, int e[12][2]) 
 }
 
 // 12-edge heating sum of lagran.F90:722-736 (also :916-928), divided by cv
-__device__ __forceinline__ double edge_heat(const Geo &g, int i, int j, int k, const double *__restrict__ al1,
-                                            const double *__restrict__ al2, const double *__restrict__ al3,
+__device__ __forceinline__ double edge_heat(const Geo &g, int i, int j, int k, const real *__restrict__ al1,
+                                            const real *__restrict__ al2, const real *__restrict__ al3,
                                             const double aa[12]) {
   const int c = g.idx(i, j, k);
   const int sy = g.sy, sz = g.sz;
@@ -169,13 +167,13 @@ __device__ __forceinline__ double edge_heat(const Geo &g, int i, int j, int k, c
 
 // p_visc + visc_heat over cells 0..n+1; pressure p0 of lagran.F90:93 stored
 // for the predictor; set_dt partial minima over 0..n.
-__global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, const double *__restrict__ vx, const double *__restrict__ vy,
-                          const double *__restrict__ vz, const double *__restrict__ rho,
-                          const double *__restrict__ energy, const double *__restrict__ bx,
+__global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, const real *__restrict__ vx, const real *__restrict__ vy,
+                          const real *__restrict__ vz, const real *__restrict__ rho,
+                          const real *__restrict__ energy, const double *__restrict__ bx,
                           const double *__restrict__ by, const double *__restrict__ bz,
-                          const double *__restrict__ eta, const double *__restrict__ al1,
-                          const double *__restrict__ al2, const double *__restrict__ al3,
-                          double *__restrict__ visc_heat, double *__restrict__ p0, double *__restrict__ partial,
+                          const real *__restrict__ eta, const real *__restrict__ al1,
+                          const real *__restrict__ al2, const real *__restrict__ al3,
+                          real *__restrict__ visc_heat, real *__restrict__ p0, double *__restrict__ partial,
                           LagArgs a) {
   const int i = (int)(blockIdx.x * BX + threadIdx.x);
   const int j = (int)(blockIdx.y * BY + threadIdx.y);
@@ -223,28 +221,28 @@ __global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, const double *__rest
 }
 
 // viscous_damping (lagran.F90:483-524), two passes: q then v += dt q
-__global__ __launch_bounds__(BX * BY) void k_damp_q(Geo g, Box b, const double *__restrict__ vx, const double *__restrict__ vy,
-                         const double *__restrict__ vz, const double *__restrict__ visc3, double *__restrict__ qx,
-                         double *__restrict__ qy, double *__restrict__ qz) {
+__global__ __launch_bounds__(BX * BY) void k_damp_q(Geo g, Box b, const real *__restrict__ vx, const real *__restrict__ vy,
+                         const real *__restrict__ vz, const real *__restrict__ visc3, real *__restrict__ qx,
+                         real *__restrict__ qy, real *__restrict__ qz) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
-  const double *V[3] = {vx, vy, vz};
-  double *Q[3] = {qx, qy, qz};
+  const real *V[3] = {vx, vy, vz};
+  real *Q[3] = {qx, qy, qz};
   const double dxbp = g.dxb[i + 1], dxb = g.dxb[i], dxc = g.dxc[i];
   const double dybp = g.dyb[j + 1], dyb = g.dyb[j], dyc = g.dyc[j];
   const double dzbp = g.dzb[k + 1], dzb = g.dzb[k], dzc = g.dzc[k];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    const double *u = V[q];
+    const real *u = V[q];
     const double u0 = u[c];
     Q[q][c] = visc3[c] * (((u[c + 1] - u0) / dxbp - (u0 - u[c - 1]) / dxb) / dxc +
                           ((u[c + g.sy] - u0) / dybp - (u0 - u[c - g.sy]) / dyb) / dyc +
                           ((u[c + g.sz] - u0) / dzbp - (u0 - u[c - g.sz]) / dzb) / dzc);
   }
 }
-__global__ __launch_bounds__(BX * BY) void k_damp_apply(Geo g, Box b, double *__restrict__ vx, double *__restrict__ vy, double *__restrict__ vz,
-                             const double *__restrict__ qx, const double *__restrict__ qy,
-                             const double *__restrict__ qz, double dt) {
+__global__ __launch_bounds__(BX * BY) void k_damp_apply(Geo g, Box b, real *__restrict__ vx, real *__restrict__ vy, real *__restrict__ vz,
+                             const real *__restrict__ qx, const real *__restrict__ qy,
+                             const real *__restrict__ qz, double dt) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
   vx[c] = vx[c] + dt * qx[c];
@@ -272,7 +270,7 @@ __device__ __forceinline__ void vertex_current(const Geo &g, const double *__res
 
 // eta_calc (lagran.F90:1176-1230) over -1..n+1
 __global__ __launch_bounds__(BX * BY) void k_eta(Geo g, Box b, const double *__restrict__ bx, const double *__restrict__ by,
-                      const double *__restrict__ bz, double *__restrict__ eta, double eta_bg, double eta0,
+                      const double *__restrict__ bz, real *__restrict__ eta, double eta_bg, double eta0,
                       double j_max) {
   BOX_INDEX(b, i, j, k);
   double jx, jy, jz;
@@ -280,7 +278,7 @@ __global__ __launch_bounds__(BX * BY) void k_eta(Geo g, Box b, const double *__r
   eta[g.idx(i, j, k)] = sqrt(jx * jx + jy * jy + jz * jz) > j_max ? eta_bg + eta0 : eta_bg;
 }
 
-__global__ __launch_bounds__(BX * BY) void k_fill(Geo g, Box b, double *__restrict__ a, double v) {
+__global__ __launch_bounds__(BX * BY) void k_fill(Geo g, Box b, real *__restrict__ a, double v) {
   BOX_INDEX(b, i, j, k);
   a[g.idx(i, j, k)] = v;
 }
@@ -288,8 +286,8 @@ __global__ __launch_bounds__(BX * BY) void k_fill(Geo g, Box b, double *__restri
 // rkstep (lagran.F90:1472-1503) over nodes 0..n, with the
 // total_ohmic_heating increments of :1309-1328 reduced in the same pass.
 __global__ __launch_bounds__(BX * BY) void k_rkstep(Geo g, const double *__restrict__ bx, const double *__restrict__ by,
-                         const double *__restrict__ bz, const double *__restrict__ eta, double *__restrict__ fxr,
-                         double *__restrict__ fyr, double *__restrict__ fzr, double *__restrict__ curlb,
+                         const double *__restrict__ bz, const real *__restrict__ eta, double *__restrict__ fxr,
+                         double *__restrict__ fyr, double *__restrict__ fzr, real *__restrict__ curlb,
                          double *__restrict__ partial, double dt) {
   const int i = (int)(blockIdx.x * BX + threadIdx.x);
   const int j = (int)(blockIdx.y * BY + threadIdx.y);
@@ -342,8 +340,8 @@ __global__ __launch_bounds__(BX * BY) void k_bstep(Geo g, Box b, double *__restr
 
 // Ohmic heating (lagran.F90:1270-1288) on cells 1..n, then
 // energy = MAX(energy, 0) over the whole cell box (:1305)
-__global__ __launch_bounds__(BX * BY) void k_ohmic(Geo g, Box b, double *__restrict__ energy, double *__restrict__ ohmic_dep,
-                        const double *__restrict__ rho, const double *__restrict__ curlb, double dt) {
+__global__ __launch_bounds__(BX * BY) void k_ohmic(Geo g, Box b, real *__restrict__ energy, real *__restrict__ ohmic_dep,
+                        const real *__restrict__ rho, const real *__restrict__ curlb, double dt) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
   double e = energy[c];
@@ -358,7 +356,7 @@ __global__ __launch_bounds__(BX * BY) void k_ohmic(Geo g, Box b, double *__restr
   energy[c] = max0(e);
 }
 
-__global__ __launch_bounds__(BX * BY) void k_energy_max(Geo g, Box b, double *__restrict__ energy) {
+__global__ __launch_bounds__(BX * BY) void k_energy_max(Geo g, Box b, real *__restrict__ energy) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
   energy[c] = max0(energy[c]);
@@ -366,9 +364,9 @@ __global__ __launch_bounds__(BX * BY) void k_energy_max(Geo g, Box b, double *__
 
 // predictor energy and pressure (lagran.F90:234-249) over cells 0..n+1;
 // p0 is the start-of-step pressure (:93), cv1 the predictor-remap volume
-__global__ __launch_bounds__(BX * BY) void k_pred_pressure(Geo g, Box b, const double *__restrict__ energy, const double *__restrict__ rho,
-                                const double *__restrict__ cv1, const double *__restrict__ visc_heat,
-                                const double *__restrict__ p0, double *__restrict__ pp, double gamma, double dt2) {
+__global__ __launch_bounds__(BX * BY) void k_pred_pressure(Geo g, Box b, const real *__restrict__ energy, const real *__restrict__ rho,
+                                const real *__restrict__ cv1, const real *__restrict__ visc_heat,
+                                const real *__restrict__ p0, real *__restrict__ pp, double gamma, double dt2) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
   const double cv = g.cv(i, j, k);
@@ -382,12 +380,12 @@ __global__ __launch_bounds__(BX * BY) void k_pred_pressure(Geo g, Box b, const d
 // predictor momentum (lagran.F90:251-387) with fx/fy/fz_visc (:745-789)
 // recomputed here from alpha and v; B is the half-step (predictor-remapped)
 // field.
-__global__ __launch_bounds__(BX * BY) void k_force(Geo g, Box b, const double *__restrict__ vx, const double *__restrict__ vy,
-                        const double *__restrict__ vz, const double *__restrict__ pp,
+__global__ __launch_bounds__(BX * BY) void k_force(Geo g, Box b, const real *__restrict__ vx, const real *__restrict__ vy,
+                        const real *__restrict__ vz, const real *__restrict__ pp,
                         const double *__restrict__ bx, const double *__restrict__ by, const double *__restrict__ bz,
-                        const double *__restrict__ rho_v, const double *__restrict__ al1,
-                        const double *__restrict__ al2, const double *__restrict__ al3, double *__restrict__ vx1,
-                        double *__restrict__ vy1, double *__restrict__ vz1, double dt2) {
+                        const real *__restrict__ rho_v, const real *__restrict__ al1,
+                        const real *__restrict__ al2, const real *__restrict__ al3, real *__restrict__ vx1,
+                        real *__restrict__ vy1, real *__restrict__ vz1, double dt2) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
   const int X = 1, Y = g.sy, Z = g.sz;
@@ -400,11 +398,11 @@ __global__ __launch_bounds__(BX * BY) void k_force(Geo g, Box b, const double *_
   const double a5 = al3[c] * dxc * dyc;
   const double a6 = al3[c + Z] * dxc * dyc;
   const double cvv = 0.125 * cv_sum8(g, i, j, k);
-  const double *V[3] = {vx, vy, vz};
+  const real *V[3] = {vx, vy, vz};
   double fv[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    const double *u = V[q];
+    const real *u = V[q];
     const double u0 = u[c];
     fv[q] = (a1 * (u0 - u[c - X]) + a2 * (u0 - u[c + X]) + a3 * (u0 - u[c - Y]) + a4 * (u0 - u[c + Y]) +
              a5 * (u0 - u[c - Z]) + a6 * (u0 - u[c + Z])) /
@@ -438,12 +436,12 @@ __global__ __launch_bounds__(BX * BY) void k_force(Geo g, Box b, const double *_
 
 // shock_heating (lagran.F90:861-937) + corrector (:415-462) over cells
 // 1..n; total_visc_heating partial sums.
-__global__ __launch_bounds__(BX * BY) void k_shock_corr(Geo g, const double *__restrict__ vx, const double *__restrict__ vy,
-                             const double *__restrict__ vz, const double *__restrict__ vx1,
-                             const double *__restrict__ vy1, const double *__restrict__ vz1,
-                             const double *__restrict__ al1, const double *__restrict__ al2,
-                             const double *__restrict__ al3, const double *__restrict__ pp,
-                             double *__restrict__ energy, double *__restrict__ rho, double *__restrict__ visc_dep,
+__global__ __launch_bounds__(BX * BY) void k_shock_corr(Geo g, const real *__restrict__ vx, const real *__restrict__ vy,
+                             const real *__restrict__ vz, const real *__restrict__ vx1,
+                             const real *__restrict__ vy1, const real *__restrict__ vz1,
+                             const real *__restrict__ al1, const real *__restrict__ al2,
+                             const real *__restrict__ al3, const real *__restrict__ pp,
+                             real *__restrict__ energy, real *__restrict__ rho, real *__restrict__ visc_dep,
                              double *__restrict__ partial, double dt) {
   const int i = 1 + (int)(blockIdx.x * BX + threadIdx.x);
   const int j = 1 + (int)(blockIdx.y * BY + threadIdx.y);
@@ -479,9 +477,9 @@ __global__ __launch_bounds__(BX * BY) void k_shock_corr(Geo g, const double *__r
   block_reduce_store<1>(v, ops, partial);
 }
 
-__global__ __launch_bounds__(BX * BY) void k_vupdate(Geo g, Box b, double *__restrict__ vx, double *__restrict__ vy, double *__restrict__ vz,
-                          const double *__restrict__ vx1, const double *__restrict__ vy1,
-                          const double *__restrict__ vz1) {
+__global__ __launch_bounds__(BX * BY) void k_vupdate(Geo g, Box b, real *__restrict__ vx, real *__restrict__ vy, real *__restrict__ vz,
+                          const real *__restrict__ vx1, const real *__restrict__ vy1,
+                          const real *__restrict__ vz1) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
   vx[c] = 2.0 * vx1[c] - vx[c];
@@ -491,8 +489,8 @@ __global__ __launch_bounds__(BX * BY) void k_vupdate(Geo g, Box b, double *__res
 
 // energy_account (diagnostics.F90:517-609) + max |div B| * min(dx):
 // partial[0..3] = (en_b, en_ke, en_int, -max divb) per block over box 0..n
-__global__ __launch_bounds__(BX * BY) void k_energy(Geo g, const double *__restrict__ rho, const double *__restrict__ energy,
-                         const double *__restrict__ vx, const double *__restrict__ vy, const double *__restrict__ vz,
+__global__ __launch_bounds__(BX * BY) void k_energy(Geo g, const real *__restrict__ rho, const real *__restrict__ energy,
+                         const real *__restrict__ vx, const real *__restrict__ vy, const real *__restrict__ vz,
                          const double *__restrict__ bx, const double *__restrict__ by,
                          const double *__restrict__ bz, double *__restrict__ partial) {
   const int i = (int)(blockIdx.x * BX + threadIdx.x);

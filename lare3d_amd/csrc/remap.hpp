@@ -16,8 +16,6 @@
 //  * vertex-averaged cv1/cv2/dm go to separate arrays instead of being
 //    written over the cell values (xremap.f90:233-280).
 //  * rho is double-buffered (rho1 = the old buffer) instead of copied.
-#pragma once
-#include "common.hpp"
 
 #define SIXTH (1.0 / 6.0)
 
@@ -39,62 +37,115 @@ __device__ __forceinline__ double vl_di(double dfi, double dfu, double sign_v, d
   return sign_v * ss * dmin(dmin(fabs(Da) * dbu, fabs(dfi)), fabs(dfu));
 }
 
+// Columns of a z-marching launch: thread (i, j) of tile (blockIdx.x,
+// blockIdx.y) walks planes k0 .. k1 of its chunk (ZCHUNK planes per block).
+#define ZCHUNK 8
+#define ZCOLUMN(b, i, j, k0, k1)                                      \
+  const int i = (b).x0 + (int)(blockIdx.x * BX + threadIdx.x);         \
+  const int j = (b).y0 + (int)(blockIdx.y * BY + threadIdx.y);         \
+  const int k0 = (b).z0 + (int)blockIdx.z * ZCHUNK;                    \
+  const int k1 = k0 + ZCHUNK - 1 < (b).z1 ? k0 + ZCHUNK - 1 : (b).z1;  \
+  if (i > (b).x1 || j > (b).y1) return;
+
+// A field at the four in-plane points (i, j), (i-1, j), (i, j-1), (i-1, j-1).
+struct Quad {
+  real p0, px, py, pxy;
+};
+__device__ __forceinline__ Quad quad_back(const real *__restrict__ u, int c, int Y) {
+  Quad q = {u[c], u[c - 1], u[c - Y], u[c - 1 - Y]};
+  return q;
+}
+
 // Lagrangian geometry of a sweep (xremap.f90:57-105 and analogues).
-// Velocities are the time-centred ones (v in predictor mode).
+// Velocities are the time-centred ones (v in predictor mode).  The node
+// velocities of planes k and k-1 are kept in registers while marching in z.
 template <int AX>
-__global__ __launch_bounds__(BX * BY) void k_geom(Geo g, Box b, const double *__restrict__ ux, const double *__restrict__ uy,
-                       const double *__restrict__ uz, double *__restrict__ cv1, double *__restrict__ cv2,
-                       double *__restrict__ db1, double dt, double xpass, double ypass, double zpass, int write_cv1,
+__global__ __launch_bounds__(BX * BY) void k_geom(Geo g, Box b, const real *__restrict__ ux, const real *__restrict__ uy,
+                       const real *__restrict__ uz, real *__restrict__ cv1, real *__restrict__ cv2,
+                       real *__restrict__ db1, double dt, double xpass, double ypass, double zpass, int write_cv1,
                        int write_cv2) {
-  BOX_INDEX(b, i, j, k);
-  const int c = g.idx(i, j, k);
-  const int X = 1, Y = g.sy, Z = g.sz;
-  const double vxb = (ux[c] + ux[c - Y] + ux[c - Z] + ux[c - Y - Z]) * 0.25;
-  const double vxbm = (ux[c - X] + ux[c - X - Y] + ux[c - X - Z] + ux[c - X - Y - Z]) * 0.25;
-  const double vyb = (uy[c] + uy[c - X] + uy[c - Z] + uy[c - X - Z]) * 0.25;
-  const double vybm = (uy[c - Y] + uy[c - X - Y] + uy[c - Y - Z] + uy[c - X - Y - Z]) * 0.25;
-  const double vzb = (uz[c] + uz[c - X] + uz[c - Y] + uz[c - X - Y]) * 0.25;
-  const double vzbm = (uz[c - Z] + uz[c - X - Z] + uz[c - Y - Z] + uz[c - X - Y - Z]) * 0.25;
-  const double cv = g.cv(i, j, k);
-  double dv1, dv2, d1;
-  if (AX == 0) {
-    const double tA = ypass * (vyb - vybm) / g.dyb[j];
-    const double tB = zpass * (vzb - vzbm) / g.dzb[k];
-    dv1 = (tA + tB + (vxb - vxbm) / g.dxb[i]) * dt;
-    dv2 = (tA + tB) * dt;
-    d1 = g.dxb[i] + (vxb - vxbm) * dt;
-  } else if (AX == 1) {
-    const double tA = xpass * (vxb - vxbm) / g.dxb[i];
-    const double tB = zpass * (vzb - vzbm) / g.dzb[k];
-    dv1 = (tA + tB + (vyb - vybm) / g.dyb[j]) * dt;
-    dv2 = (tA + tB) * dt;
-    d1 = g.dyb[j] + (vyb - vybm) * dt;
-  } else {
-    const double tA = xpass * (vxb - vxbm) / g.dxb[i];
-    const double tB = ypass * (vyb - vybm) / g.dyb[j];
-    dv1 = (tA + tB + (vzb - vzbm) / g.dzb[k]) * dt;
-    dv2 = (tA + tB) * dt;
-    d1 = g.dzb[k] + (vzb - vzbm) * dt;
+  ZCOLUMN(b, i, j, k0, k1);
+  const int Y = g.sy;
+  const int cb = g.idx(i, j, k0 - 1);
+  Quad ax = quad_back(ux, cb, Y), ay = quad_back(uy, cb, Y), az = quad_back(uz, cb, Y);  // plane k-1
+  for (int k = k0; k <= k1; ++k) {
+    const int c = g.idx(i, j, k);
+    const Quad bx_ = quad_back(ux, c, Y), by_ = quad_back(uy, c, Y), bz_ = quad_back(uz, c, Y);  // plane k
+    // sums in the order of u(c) + u(c-Y) + u(c-Z) + u(c-Y-Z) etc.
+    const double vxb = (bx_.p0 + bx_.py + ax.p0 + ax.py) * 0.25;
+    const double vxbm = (bx_.px + bx_.pxy + ax.px + ax.pxy) * 0.25;
+    const double vyb = (by_.p0 + by_.px + ay.p0 + ay.px) * 0.25;
+    const double vybm = (by_.py + by_.pxy + ay.py + ay.pxy) * 0.25;
+    const double vzb = (bz_.p0 + bz_.px + bz_.py + bz_.pxy) * 0.25;
+    const double vzbm = (az.p0 + az.px + az.py + az.pxy) * 0.25;
+    const double cv = g.cv(i, j, k);
+    double dv1, dv2, d1;
+    if (AX == 0) {
+      const double tA = ypass * (vyb - vybm) / g.dyb[j];
+      const double tB = zpass * (vzb - vzbm) / g.dzb[k];
+      dv1 = (tA + tB + (vxb - vxbm) / g.dxb[i]) * dt;
+      dv2 = (tA + tB) * dt;
+      d1 = g.dxb[i] + (vxb - vxbm) * dt;
+    } else if (AX == 1) {
+      const double tA = xpass * (vxb - vxbm) / g.dxb[i];
+      const double tB = zpass * (vzb - vzbm) / g.dzb[k];
+      dv1 = (tA + tB + (vyb - vybm) / g.dyb[j]) * dt;
+      dv2 = (tA + tB) * dt;
+      d1 = g.dyb[j] + (vyb - vybm) * dt;
+    } else {
+      const double tA = xpass * (vxb - vxbm) / g.dxb[i];
+      const double tB = ypass * (vyb - vybm) / g.dyb[j];
+      dv1 = (tA + tB + (vzb - vzbm) / g.dzb[k]) * dt;
+      dv2 = (tA + tB) * dt;
+      d1 = g.dzb[k] + (vzb - vzbm) * dt;
+    }
+    if (write_cv1) cv1[c] = cv * (1.0 + dv1);
+    if (write_cv2) cv2[c] = cv * (1.0 + dv2);
+    db1[c] = d1;
+    ax = bx_;
+    ay = by_;
+    az = bz_;
   }
-  if (write_cv1) cv1[c] = cv * (1.0 + dv1);
-  if (write_cv2) cv2[c] = cv * (1.0 + dv2);
-  db1[c] = d1;
 }
 
 // dxc1 at sweep coordinate s (index c), clamped as xremap.f90:110
 template <int AX>
-__device__ __forceinline__ double dc1(const Geo &g, const double *__restrict__ db1, int c, int s) {
+__device__ __forceinline__ double dc1(const Geo &g, const real *__restrict__ db1, int c, int s) {
   const int S = sstride<AX>(g);
   const int n = g.n(AX);
   return 0.5 * (db1[c] + db1[s + 1 <= n + 2 ? c + S : c]);
+}
+
+// Van Leer CT flux from its stencil values along the sweep: field fr* at
+// s-1 (clamped), s, s+1, s+2 (clamped), the widths w* averaged along GS at
+// the same points, db1 at s and s+1, and dxc1 at s-1, s, s+1.
+__device__ __forceinline__ double ct_flux_core(double v_advect, double frm, double fri, double frp, double frp2,
+                                               double dbm, double db, double dbp, double dbp2, double db1c,
+                                               double db1p, double dcm, double dci, double dcp, double dt) {
+  const double fm = frm / dbm;
+  const double fi = fri / db;
+  const double fp = frp / dbp;
+  const double fp2 = frp2 / dbp2;
+  const double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
+  const double sign_v = copysign(1.0, v_advect);
+  const double vad_p = (sign_v + 1.0) * 0.5;
+  const double vad_m = 1.0 - vad_p;
+  const double fu = fi * vad_p + fp * vad_m;
+  const double dfu = dfm * vad_p + dfp * vad_m;
+  const double dcu = dcm * vad_p + dcp * vad_m;
+  const double dbu = db1c * vad_p + db1p * vad_m;
+  const double du = db * vad_p + dbp * vad_m;
+  const double phi = fabs(v_advect) * dt / du;
+  const double Di = vl_di(dfi, dfu, sign_v, dci, dcu, dbu, phi);
+  return (fu + Di * (1.0 - phi)) * v_advect * dt;
 }
 
 // CT flux of a transverse face field along AX (vx_by_flux etc.).
 // VS: stride of the axis the velocity is averaged along (v(c) + v(c-VS));
 // GS: stride of the axis d?b1 is averaged along (d(c) + d(c+GS)).
 template <int AX>
-__global__ __launch_bounds__(BX * BY) void k_ct_flux(Geo g, Box b, const double *__restrict__ fld, const double *__restrict__ vel,
-                          const double *__restrict__ db1, double *__restrict__ F, int VS, int GS, double dt) {
+__global__ __launch_bounds__(BX * BY) void k_ct_flux(Geo g, Box b, const double *__restrict__ fld, const real *__restrict__ vel,
+                          const real *__restrict__ db1, double *__restrict__ F, int VS, int GS, double dt) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
   const int S = sstride<AX>(g);
@@ -108,23 +159,56 @@ __global__ __launch_bounds__(BX * BY) void k_ct_flux(Geo g, Box b, const double 
   const double dbp = (db1[cp] + db1[cp + GS]) * 0.5;
   const double dbp2 = (db1[cp2] + db1[cp2 + GS]) * 0.5;
   const double dbm = (db1[cm] + db1[cm + GS]) * 0.5;
-  const double fm = fld[cm] / dbm;
-  const double fi = fld[c] / db;
-  const double fp = fld[cp] / dbp;
-  const double fp2 = fld[cp2] / dbp2;
-  const double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
-  const double sign_v = copysign(1.0, v_advect);
-  const double vad_p = (sign_v + 1.0) * 0.5;
-  const double vad_m = 1.0 - vad_p;
-  const double fu = fi * vad_p + fp * vad_m;
-  const double dfu = dfm * vad_p + dfp * vad_m;
-  const double dci = dc1<AX>(g, db1, c, s);
-  const double dcu = dc1<AX>(g, db1, cm, sm) * vad_p + dc1<AX>(g, db1, cp, s + 1) * vad_m;
-  const double dbu = db1[c] * vad_p + db1[cp] * vad_m;
-  const double du = db * vad_p + dbp * vad_m;
-  const double phi = fabs(v_advect) * dt / du;
-  const double Di = vl_di(dfi, dfu, sign_v, dci, dcu, dbu, phi);
-  F[c] = (fu + Di * (1.0 - phi)) * v_advect * dt;
+  F[c] = ct_flux_core(v_advect, fld[cm], fld[c], fld[cp], fld[cp2], dbm, db, dbp, dbp2, db1[c], db1[cp],
+                      dc1<AX>(g, db1, cm, sm), dc1<AX>(g, db1, c, s), dc1<AX>(g, db1, cp, s + 1), dt);
+}
+
+// The same flux for the z sweep, each thread marching up its column for
+// ZCHUNK planes with the four-plane stencil of fld and db1 in registers, so
+// every plane is read from memory once per column chunk.
+__global__ __launch_bounds__(BX * BY) void k_ct_flux_z(Geo g, Box b, const double *__restrict__ fld,
+                                                       const real *__restrict__ vel, const real *__restrict__ db1,
+                                                       double *__restrict__ F, int VS, int GS, double dt) {
+  ZCOLUMN(b, i, j, k0, k1);
+  const int n = g.nz;
+  // window over planes k-1, k, k+1, k+2, each clamped to [-1, n+2]
+  double f[4];
+  real d[4], w[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    int kk = k0 - 1 + q;
+    kk = kk < -1 ? -1 : (kk > n + 2 ? n + 2 : kk);
+    const int cq = g.idx(i, j, kk);
+    f[q] = fld[cq];
+    d[q] = db1[cq];
+    w[q] = db1[cq + GS];
+  }
+  for (int k = k0; k <= k1; ++k) {
+    const int c = g.idx(i, j, k);
+    const double v_advect = (vel[c] + vel[c - VS]) * 0.5;
+    const double dbm = (d[0] + w[0]) * 0.5;
+    const double db = (d[1] + w[1]) * 0.5;
+    const double dbp = (d[2] + w[2]) * 0.5;
+    const double dbp2 = (d[3] + w[3]) * 0.5;
+    // dxc1 at the clamped s-1 (its upper point is s, or s+1 when s = -1),
+    // at s, and at s+1 (upper point clamped to n+2, as d[3] already is)
+    const double dcm = 0.5 * (d[0] + (k > -1 ? d[1] : d[2]));
+    const double dci = 0.5 * (d[1] + d[2]);
+    const double dcp = 0.5 * (d[2] + d[3]);
+    F[c] = ct_flux_core(v_advect, f[0], f[1], f[2], f[3], dbm, db, dbp, dbp2, d[1], d[2], dcm, dci, dcp, dt);
+    int kn = k + 3;
+    kn = kn > n + 2 ? n + 2 : kn;
+    const int cn = g.idx(i, j, kn);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      f[q] = f[q + 1];
+      d[q] = d[q + 1];
+      w[q] = w[q + 1];
+    }
+    f[3] = fld[cn];
+    d[3] = db1[cn];
+    w[3] = db1[cn + GS];
+  }
 }
 
 // CT update pair: t = t - F + F(c - TS) on box bt; a = a + F - F(c - AS) on ba.
@@ -140,7 +224,7 @@ __global__ __launch_bounds__(BX * BY) void k_ct_update(Geo g, Box b, const doubl
 
 // Face-averaged sweep velocity of the mass/energy fluxes (xremap.f90:492-493)
 template <int AX>
-__device__ __forceinline__ double face_velocity(const Geo &g, const double *__restrict__ vel, int c) {
+__device__ __forceinline__ double face_velocity(const Geo &g, const real *__restrict__ vel, int c) {
   const int X = 1, Y = g.sy, Z = g.sz;
   if (AX == 0) return (vel[c] + vel[c - Y] + vel[c - Z] + vel[c - Y - Z]) * 0.25;
   if (AX == 1) return (vel[c] + vel[c - X] + vel[c - Z] + vel[c - X - Z]) * 0.25;
@@ -153,66 +237,146 @@ __device__ __forceinline__ double face_area(const Geo &g, int i, int j, int k) {
   return g.dxb[i] * g.dyb[j];
 }
 
+// Van Leer upwind value fu and limited slope Di of a cell-centred quantity
+// along the sweep (xremap.f90:474-533), from its values at s-1..s+2, dxc1 at
+// s-1, s, s+1 and db1 at s, s+1; also the upwind weights, dbu and phi.
+__device__ __forceinline__ void vl_cell(double v_advect, double fm, double fi, double fp, double fp2, double dcm,
+                                        double dci, double dcp, double db1c, double db1p, double dt, double &vad_p,
+                                        double &vad_m, double &dbu, double &fu, double &Di, double &phi) {
+  const double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
+  const double sign_v = copysign(1.0, v_advect);
+  vad_p = (sign_v + 1.0) * 0.5;
+  vad_m = 1.0 - vad_p;
+  fu = fi * vad_p + fp * vad_m;
+  const double dfu = dfm * vad_p + dfp * vad_m;
+  const double dcu = dcm * vad_p + dcp * vad_m;
+  dbu = db1c * vad_p + db1p * vad_m;
+  phi = fabs(v_advect) * dt / dbu;
+  Di = vl_di(dfi, dfu, sign_v, dci, dcu, dbu, phi);
+}
+
 // x_mass_flux (xremap.f90:474-533): sweep 0..n, transverse 0..n+1
 template <int AX>
-__global__ __launch_bounds__(BX * BY) void k_mass_flux(Geo g, Box b, const double *__restrict__ rho, const double *__restrict__ vel,
-                            const double *__restrict__ db1, double *__restrict__ dm, double dt) {
+__global__ __launch_bounds__(BX * BY) void k_mass_flux(Geo g, Box b, const real *__restrict__ rho, const real *__restrict__ vel,
+                            const real *__restrict__ db1, real *__restrict__ dm, double dt) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
   const int S = sstride<AX>(g);
   const int s = coord<AX>(i, j, k);
   const double area = face_area<AX>(g, i, j, k);
   const double v_advect = face_velocity<AX>(g, vel, c);
-  const double fm = rho[c - S], fi = rho[c], fp = rho[c + S], fp2 = rho[c + 2 * S];
-  const double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
-  const double sign_v = copysign(1.0, v_advect);
-  const double vad_p = (sign_v + 1.0) * 0.5;
-  const double vad_m = 1.0 - vad_p;
-  const double fu = fi * vad_p + fp * vad_m;
-  const double dfu = dfm * vad_p + dfp * vad_m;
-  const double dci = dc1<AX>(g, db1, c, s);
-  const double dcu = dc1<AX>(g, db1, c - S, s - 1) * vad_p + dc1<AX>(g, db1, c + S, s + 1) * vad_m;
-  const double dbu = db1[c] * vad_p + db1[c + S] * vad_m;
-  const double phi = fabs(v_advect) * dt / dbu;
-  const double Di = vl_di(dfi, dfu, sign_v, dci, dcu, dbu, phi);
+  double vad_p, vad_m, dbu, fu, Di, phi;
+  vl_cell(v_advect, rho[c - S], rho[c], rho[c + S], rho[c + 2 * S], dc1<AX>(g, db1, c - S, s - 1),
+          dc1<AX>(g, db1, c, s), dc1<AX>(g, db1, c + S, s + 1), db1[c], db1[c + S], dt, vad_p, vad_m, dbu, fu, Di,
+          phi);
   dm[c] = (fu + Di * (1.0 - phi)) * v_advect * dt * area;
+}
+
+// The z-sweep mass flux, marching in z with rho and db1 of planes k-1..k+2
+// in registers (box planes 0..n, so no clamping is needed).
+__global__ __launch_bounds__(BX * BY) void k_mass_flux_z(Geo g, Box b, const real *__restrict__ rho,
+                                                         const real *__restrict__ vel, const real *__restrict__ db1,
+                                                         real *__restrict__ dm, double dt) {
+  ZCOLUMN(b, i, j, k0, k1);
+  const int Z = g.sz;
+  real r[4], d[4];
+  const int cb = g.idx(i, j, k0 - 1);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    r[q] = rho[cb + q * Z];
+    d[q] = db1[cb + q * Z];
+  }
+  for (int k = k0; k <= k1; ++k) {
+    const int c = g.idx(i, j, k);
+    const double area = face_area<2>(g, i, j, k);
+    const double v_advect = face_velocity<2>(g, vel, c);
+    double vad_p, vad_m, dbu, fu, Di, phi;
+    vl_cell(v_advect, r[0], r[1], r[2], r[3], 0.5 * (d[0] + d[1]), 0.5 * (d[1] + d[2]), 0.5 * (d[2] + d[3]), d[1],
+            d[2], dt, vad_p, vad_m, dbu, fu, Di, phi);
+    dm[c] = (fu + Di * (1.0 - phi)) * v_advect * dt * area;
+    if (k < k1) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        r[q] = r[q + 1];
+        d[q] = d[q + 1];
+      }
+      r[3] = rho[c + 3 * Z];
+      d[3] = db1[c + 3 * Z];
+    }
+  }
 }
 
 // x_energy_flux (xremap.f90:539-601): box 0..n
 template <int AX>
-__global__ __launch_bounds__(BX * BY) void k_energy_flux(Geo g, Box b, const double *__restrict__ energy, const double *__restrict__ rho1,
-                              const double *__restrict__ vel, const double *__restrict__ db1,
-                              const double *__restrict__ dm, double *__restrict__ F, double dt) {
+__global__ __launch_bounds__(BX * BY) void k_energy_flux(Geo g, Box b, const real *__restrict__ energy, const real *__restrict__ rho1,
+                              const real *__restrict__ vel, const real *__restrict__ db1,
+                              const real *__restrict__ dm, real *__restrict__ F, double dt) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
   const int S = sstride<AX>(g);
   const int s = coord<AX>(i, j, k);
   const double area = face_area<AX>(g, i, j, k);
   const double v_advect = face_velocity<AX>(g, vel, c);
-  const double fm = energy[c - S], fi = energy[c], fp = energy[c + S], fp2 = energy[c + 2 * S];
-  const double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
-  const double sign_v = copysign(1.0, v_advect);
-  const double vad_p = (sign_v + 1.0) * 0.5;
-  const double vad_m = 1.0 - vad_p;
-  const double fu = fi * vad_p + fp * vad_m;
-  const double dfu = dfm * vad_p + dfp * vad_m;
-  const double dci = dc1<AX>(g, db1, c, s);
-  const double dcu = dc1<AX>(g, db1, c - S, s - 1) * vad_p + dc1<AX>(g, db1, c + S, s + 1) * vad_m;
-  const double dbu = db1[c] * vad_p + db1[c + S] * vad_m;
-  const double phi = fabs(v_advect) * dt / dbu;
-  const double Di = vl_di(dfi, dfu, sign_v, dci, dcu, dbu, phi);
+  double vad_p, vad_m, dbu, fu, Di, phi;
+  vl_cell(v_advect, energy[c - S], energy[c], energy[c + S], energy[c + 2 * S], dc1<AX>(g, db1, c - S, s - 1),
+          dc1<AX>(g, db1, c, s), dc1<AX>(g, db1, c + S, s + 1), db1[c], db1[c + S], dt, vad_p, vad_m, dbu, fu, Di,
+          phi);
   const double rhou = rho1[c] * vad_p + rho1[c + S] * vad_m;
   const double d = dm[c];
   const double dmu = fabs(d) / area / dbu / rhou;
   F[c] = (fu + Di * (1.0 - dmu)) * d;
 }
 
+// The z-sweep energy flux, marching in z (energy and db1 of planes k-1..k+2,
+// rho1 of k and k+1 in registers).
+__global__ __launch_bounds__(BX * BY) void k_energy_flux_z(Geo g, Box b, const real *__restrict__ energy,
+                                                           const real *__restrict__ rho1,
+                                                           const real *__restrict__ vel,
+                                                           const real *__restrict__ db1,
+                                                           const real *__restrict__ dm, real *__restrict__ F,
+                                                           double dt) {
+  ZCOLUMN(b, i, j, k0, k1);
+  const int Z = g.sz;
+  real e[4], d[4], r0, r1;
+  const int cb = g.idx(i, j, k0 - 1);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    e[q] = energy[cb + q * Z];
+    d[q] = db1[cb + q * Z];
+  }
+  r0 = rho1[cb + Z];
+  r1 = rho1[cb + 2 * Z];
+  for (int k = k0; k <= k1; ++k) {
+    const int c = g.idx(i, j, k);
+    const double area = face_area<2>(g, i, j, k);
+    const double v_advect = face_velocity<2>(g, vel, c);
+    double vad_p, vad_m, dbu, fu, Di, phi;
+    vl_cell(v_advect, e[0], e[1], e[2], e[3], 0.5 * (d[0] + d[1]), 0.5 * (d[1] + d[2]), 0.5 * (d[2] + d[3]), d[1],
+            d[2], dt, vad_p, vad_m, dbu, fu, Di, phi);
+    const double rhou = r0 * vad_p + r1 * vad_m;
+    const double dd = dm[c];
+    const double dmu = fabs(dd) / area / dbu / rhou;
+    F[c] = (fu + Di * (1.0 - dmu)) * dd;
+    if (k < k1) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        e[q] = e[q + 1];
+        d[q] = d[q + 1];
+      }
+      e[3] = energy[c + 3 * Z];
+      d[3] = db1[c + 3 * Z];
+      r0 = r1;
+      r1 = rho1[c + 2 * Z];
+    }
+  }
+}
+
 // density and energy updates of xremap.f90:174-196 on cells 1..n
 template <int AX>
-__global__ __launch_bounds__(BX * BY) void k_rho_energy_update(Geo g, Box b, const double *__restrict__ rho1, double *__restrict__ rho_new,
-                                    double *__restrict__ energy, const double *__restrict__ cv1,
-                                    const double *__restrict__ cv2, const double *__restrict__ dm,
-                                    const double *__restrict__ F) {
+__global__ __launch_bounds__(BX * BY) void k_rho_energy_update(Geo g, Box b, const real *__restrict__ rho1, real *__restrict__ rho_new,
+                                    real *__restrict__ energy, const real *__restrict__ cv1,
+                                    const real *__restrict__ cv2, const real *__restrict__ dm,
+                                    const real *__restrict__ F) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
   const int S = sstride<AX>(g);
@@ -222,37 +386,76 @@ __global__ __launch_bounds__(BX * BY) void k_rho_energy_update(Geo g, Box b, con
   energy[c] = (energy[c] * cv1[c] * r1 + F[c - S] - F[c]) / (cv2[c] * rn);
 }
 
+// A field at the four in-plane points (i, j), (i+1, j), (i, j+1), (i+1, j+1).
+__device__ __forceinline__ Quad quad_fwd(const real *__restrict__ u, int c, int Y) {
+  Quad q = {u[c], u[c + 1], u[c + Y], u[c + 1 + Y]};
+  return q;
+}
+
 // vertex quantities of xremap.f90:201-280: rho_v over sweep -1..n+1 and
-// transverse 0..n; cv1v, cv2v over 0..n; dm_v over sweep -1..n.
+// transverse 0..n; cv1v, cv2v over 0..n; dm_v over sweep -1..n.  The cell
+// values of planes k and k+1 are kept in registers while marching in z;
+// every sum runs over the cells in the order (i,j,k), (i+1,j,k), (i,j+1,k),
+// (i+1,j+1,k), then the same four at k+1.
 template <int AX>
-__global__ __launch_bounds__(BX * BY) void k_vertex(Geo g, Box b, const double *__restrict__ rho1, const double *__restrict__ cv1,
-                         const double *__restrict__ cv2, const double *__restrict__ dm, double *__restrict__ rho_v,
-                         double *__restrict__ cv1v, double *__restrict__ cv2v, double *__restrict__ dmv) {
-  BOX_INDEX(b, i, j, k);
-  const int c = g.idx(i, j, k);
-  const int X = 1, Y = g.sy, Z = g.sz;
-  const int cc[8] = {c, c + X, c + Y, c + X + Y, c + Z, c + X + Z, c + Y + Z, c + X + Y + Z};
-  double s = rho1[cc[0]] * cv1[cc[0]];
-#pragma unroll
-  for (int q = 1; q < 8; ++q) s = s + rho1[cc[q]] * cv1[cc[q]];
-  double s1 = cv1[cc[0]];
-#pragma unroll
-  for (int q = 1; q < 8; ++q) s1 = s1 + cv1[cc[q]];
-  rho_v[c] = s / s1;
-  const int sc = coord<AX>(i, j, k);
+__global__ __launch_bounds__(BX * BY) void k_vertex(Geo g, Box b, const real *__restrict__ rho1, const real *__restrict__ cv1,
+                         const real *__restrict__ cv2, const real *__restrict__ dm, real *__restrict__ rho_v,
+                         real *__restrict__ cv1v, real *__restrict__ cv2v, real *__restrict__ dmv) {
+  ZCOLUMN(b, i, j, k0, k1);
+  const int Y = g.sy;
   const int n = g.n(AX);
-  if (sc >= 0 && sc <= n) {
-    cv1v[c] = s1 * 0.125;
-    double s2 = cv2[cc[0]];
-#pragma unroll
-    for (int q = 1; q < 8; ++q) s2 = s2 + cv2[cc[q]];
-    cv2v[c] = s2 * 0.125;
-  }
-  if (sc <= n) {
-    double sd = dm[cc[0]];
-#pragma unroll
-    for (int q = 1; q < 8; ++q) sd = sd + dm[cc[q]];
-    dmv[c] = sd * 0.125;
+  const int cl = g.idx(i, j, k0);
+  Quad lr = quad_fwd(rho1, cl, Y), l1 = quad_fwd(cv1, cl, Y), l2 = quad_fwd(cv2, cl, Y), ld = quad_fwd(dm, cl, Y);
+  for (int k = k0; k <= k1; ++k) {
+    const int c = g.idx(i, j, k);
+    const int ch = c + g.sz;
+    const Quad hr = quad_fwd(rho1, ch, Y), h1 = quad_fwd(cv1, ch, Y), h2 = quad_fwd(cv2, ch, Y),
+               hd = quad_fwd(dm, ch, Y);
+    double s = (double)lr.p0 * l1.p0;
+    s = s + (double)lr.px * l1.px;
+    s = s + (double)lr.py * l1.py;
+    s = s + (double)lr.pxy * l1.pxy;
+    s = s + (double)hr.p0 * h1.p0;
+    s = s + (double)hr.px * h1.px;
+    s = s + (double)hr.py * h1.py;
+    s = s + (double)hr.pxy * h1.pxy;
+    double s1 = l1.p0;
+    s1 = s1 + l1.px;
+    s1 = s1 + l1.py;
+    s1 = s1 + l1.pxy;
+    s1 = s1 + h1.p0;
+    s1 = s1 + h1.px;
+    s1 = s1 + h1.py;
+    s1 = s1 + h1.pxy;
+    rho_v[c] = s / s1;
+    const int sc = coord<AX>(i, j, k);
+    if (sc >= 0 && sc <= n) {
+      cv1v[c] = s1 * 0.125;
+      double s2 = l2.p0;
+      s2 = s2 + l2.px;
+      s2 = s2 + l2.py;
+      s2 = s2 + l2.pxy;
+      s2 = s2 + h2.p0;
+      s2 = s2 + h2.px;
+      s2 = s2 + h2.py;
+      s2 = s2 + h2.pxy;
+      cv2v[c] = s2 * 0.125;
+    }
+    if (sc <= n) {
+      double sd = ld.p0;
+      sd = sd + ld.px;
+      sd = sd + ld.py;
+      sd = sd + ld.pxy;
+      sd = sd + hd.p0;
+      sd = sd + hd.px;
+      sd = sd + hd.py;
+      sd = sd + hd.pxy;
+      dmv[c] = sd * 0.125;
+    }
+    lr = hr;
+    l1 = h1;
+    l2 = h2;
+    ld = hd;
   }
 }
 
@@ -266,11 +469,11 @@ __device__ __forceinline__ double mom_area(const Geo &g, int i, int j, int k) {
 // x_momx/y/z_flux (xremap.f90:605-895, rke off): sweep -1..n, transverse
 // 0..n; the three velocity components share the upwind switch and geometry.
 template <int AX>
-__global__ __launch_bounds__(BX * BY) void k_mom_flux(Geo g, Box b, const double *__restrict__ vx, const double *__restrict__ vy,
-                           const double *__restrict__ vz, const double *__restrict__ vel1,
-                           const double *__restrict__ db1, const double *__restrict__ rho_v,
-                           const double *__restrict__ dmv, double *__restrict__ Fx, double *__restrict__ Fy,
-                           double *__restrict__ Fz, double dt) {
+__global__ __launch_bounds__(BX * BY) void k_mom_flux(Geo g, Box b, const real *__restrict__ vx, const real *__restrict__ vy,
+                           const real *__restrict__ vz, const real *__restrict__ vel1,
+                           const real *__restrict__ db1, const real *__restrict__ rho_v,
+                           const real *__restrict__ dmv, real *__restrict__ Fx, real *__restrict__ Fy,
+                           real *__restrict__ Fz, double dt) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
   const int S = sstride<AX>(g);
@@ -287,11 +490,11 @@ __global__ __launch_bounds__(BX * BY) void k_mom_flux(Geo g, Box b, const double
   const double rhou = rho_v[c] * vad_p + rho_v[c + S] * vad_m;
   const double d = dmv[c];
   const double dmu = fabs(d) / area / dbu / rhou;
-  const double *V[3] = {vx, vy, vz};
-  double *FF[3] = {Fx, Fy, Fz};
+  const real *V[3] = {vx, vy, vz};
+  real *FF[3] = {Fx, Fy, Fz};
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    const double *u = V[q];
+    const real *u = V[q];
     const double fm = u[c - S], fi = u[c], fp = u[c + S], fp2 = u[c + 2 * S];
     const double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
     const double fu = fi * vad_p + fp * vad_m;
@@ -302,13 +505,82 @@ __global__ __launch_bounds__(BX * BY) void k_mom_flux(Geo g, Box b, const double
   }
 }
 
+// The z-sweep momentum fluxes, marching in z: the three velocity components
+// of planes k-1..k+2, db1 of k..k+2, the sweep velocity and rho_v of k and
+// k+1 in registers.
+__global__ __launch_bounds__(BX * BY) void k_mom_flux_z(Geo g, Box b, const real *__restrict__ vx,
+                                                        const real *__restrict__ vy, const real *__restrict__ vz,
+                                                        const real *__restrict__ vel1,
+                                                        const real *__restrict__ db1,
+                                                        const real *__restrict__ rho_v,
+                                                        const real *__restrict__ dmv, real *__restrict__ Fx,
+                                                        real *__restrict__ Fy, real *__restrict__ Fz, double dt) {
+  ZCOLUMN(b, i, j, k0, k1);
+  const int Z = g.sz;
+  real u[3][4], d[3], v0, v1, rv0, rv1;
+  const real *V[3] = {vx, vy, vz};
+  real *FF[3] = {Fx, Fy, Fz};
+  const int cb = g.idx(i, j, k0);
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int p = 0; p < 4; ++p) u[q][p] = V[q][cb + (p - 1) * Z];
+#pragma unroll
+  for (int p = 0; p < 3; ++p) d[p] = db1[cb + p * Z];
+  v0 = vel1[cb];
+  v1 = vel1[cb + Z];
+  rv0 = rho_v[cb];
+  rv1 = rho_v[cb + Z];
+  for (int k = k0; k <= k1; ++k) {
+    const int c = g.idx(i, j, k);
+    const double area = mom_area<2>(g, i, j, k);
+    const double v_advect = (v0 + v1) * 0.5;
+    const double sign_v = copysign(1.0, v_advect);
+    const double vad_p = (sign_v + 1.0) * 0.5;
+    const double vad_m = 1.0 - vad_p;
+    const double dci = d[1];
+    const double dcu = d[0] * vad_p + d[2] * vad_m;
+    const double dbu = 0.5 * (d[0] + d[1]) * vad_p + 0.5 * (d[1] + d[2]) * vad_m;
+    const double phi = fabs(v_advect) * dt / dbu;
+    const double rhou = rv0 * vad_p + rv1 * vad_m;
+    const double dd = dmv[c];
+    const double dmu = fabs(dd) / area / dbu / rhou;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double fm = u[q][0], fi = u[q][1], fp = u[q][2], fp2 = u[q][3];
+      const double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
+      const double fu = fi * vad_p + fp * vad_m;
+      const double dfu = dfm * vad_p + dfp * vad_m;
+      const double Di = vl_di(dfi, dfu, sign_v, dci, dcu, dbu, phi);
+      const double f = fu + Di * (1.0 - dmu);
+      FF[q][c] = f * dd;
+    }
+    if (k < k1) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        u[q][0] = u[q][1];
+        u[q][1] = u[q][2];
+        u[q][2] = u[q][3];
+        u[q][3] = V[q][c + 3 * Z];
+      }
+      d[0] = d[1];
+      d[1] = d[2];
+      d[2] = db1[c + 3 * Z];
+      v0 = v1;
+      v1 = vel1[c + 2 * Z];
+      rv0 = rv1;
+      rv1 = rho_v[c + 2 * Z];
+    }
+  }
+}
+
 // momentum updates (xremap.f90:282-329) on nodes 0..n
 template <int AX>
-__global__ __launch_bounds__(BX * BY) void k_mom_update(Geo g, Box b, double *__restrict__ vx, double *__restrict__ vy, double *__restrict__ vz,
-                             const double *__restrict__ rho_v, const double *__restrict__ cv1v,
-                             const double *__restrict__ cv2v, const double *__restrict__ dmv,
-                             const double *__restrict__ Fx, const double *__restrict__ Fy,
-                             const double *__restrict__ Fz) {
+__global__ __launch_bounds__(BX * BY) void k_mom_update(Geo g, Box b, real *__restrict__ vx, real *__restrict__ vy, real *__restrict__ vz,
+                             const real *__restrict__ rho_v, const real *__restrict__ cv1v,
+                             const real *__restrict__ cv2v, const real *__restrict__ dmv,
+                             const real *__restrict__ Fx, const real *__restrict__ Fy,
+                             const real *__restrict__ Fz) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
   const int S = sstride<AX>(g);

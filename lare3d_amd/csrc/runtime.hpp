@@ -1,0 +1,1440 @@
+// runtime.hpp -- device state, launch sequencing and the entry points of one
+// precision mode.  lare_dev.hip includes this file twice: in namespace fp64
+// with LARE_REAL = double and in namespace fp32 with LARE_REAL = float.
+// `real` is the storage type of the bulk fields (density, energy, velocity,
+// their Lagrangian and remap work arrays, conduction stages); the face
+// magnetic field, its constrained-transport fluxes and resistive EMFs, the
+// metrics and every reduction stay FP64, and kernels do their arithmetic in
+// FP64 registers.  No include guard: one inclusion per namespace.
+typedef LARE_REAL real;
+
+#include "common.hpp"
+#include "bc.hpp"
+#include "conduct.hpp"
+#include "lagran.hpp"
+#include "remap.hpp"
+#include "surrogate.hpp"
+
+// ----------------------------------------------------------------------------
+// kernel timing with HIP events on the library stream
+// ----------------------------------------------------------------------------
+struct ProfRec {
+  const char *name;
+  hipEvent_t a, b;
+};
+struct ProfAcc {
+  std::string name;
+  int count;
+  double ms;
+};
+
+struct Dev {
+  bool ready = false, loaded = false;
+  int n[3];
+  int sy, sz;
+  size_t ntot;
+  int bc[6];
+  int per[3], user[3];
+  lare_params p;
+  double tfac, visc2_norm;
+  hipStream_t stream = nullptr;
+  double *metric = nullptr;  // 7 arrays of length nmax+5, index i+2
+  Geo g;
+  // state (bulk fields in the storage precision, face B in FP64)
+  real *rho, *rho_alt, *energy, *vx, *vy, *vz, *vx1, *vy1, *vz1;
+  double *bx, *by, *bz;
+  real *eta, *visc3, *visc_dep, *ohmic_dep;
+  // lagrangian scratch
+  real *rho_v, *cs_v, *al1, *al2, *al3, *visc_heat, *p0, *pp;
+  double *bpx, *bpy, *bpz;       // predictor B
+  double *fxr, *fyr, *fzr;       // resistive EMFs (CT update of B)
+  real *curlb, *qx, *qy, *qz;
+  // remap scratch: F is the CT face flux (FP64), Fe the energy flux
+  real *cv1, *cv2, *db1, *Fe, *dm, *Fx, *Fy, *Fz, *rho_vr, *cv1v, *cv2v, *dmv;
+  double *F;
+  // conduction scratch
+  real *larsen, *Lc, *Yb[3];
+  // reductions
+  double *partial = nullptr;
+  size_t partial_cap = 0;
+  double *dscal = nullptr;   // device scalars: [0..3] last reduce, [8] tvh acc, [9] toh acc
+  double *hscal = nullptr;   // pinned host mirror
+  std::vector<void *> allocs;
+  // host scalars
+  double dt = 0.0, dt2 = 0.0, dtr = 0.0, time = 0.0;
+  int n_s_stages = 0, peak_substeps = 0, last_substeps = 0, first_set_dt = 1, step = 0;
+  // profiling
+  bool prof = false;
+  std::vector<ProfRec> pending;
+  std::vector<hipEvent_t> pool;
+  std::vector<ProfAcc> acc;
+};
+
+Dev S;
+
+hipEvent_t ev_get() {
+  if (!S.pool.empty()) {
+    hipEvent_t e = S.pool.back();
+    S.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+void prof_flush() {
+  if (S.pending.empty()) return;
+  (void)hipStreamSynchronize(S.stream);
+  for (auto &r : S.pending) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, r.a, r.b);
+    bool found = false;
+    for (auto &a : S.acc)
+      if (a.name == r.name) {
+        a.count++;
+        a.ms += ms;
+        found = true;
+        break;
+      }
+    if (!found) S.acc.push_back({r.name, 1, (double)ms});
+    S.pool.push_back(r.a);
+    S.pool.push_back(r.b);
+  }
+  S.pending.clear();
+}
+
+struct ProfScope {
+  const char *name;
+  hipEvent_t a = nullptr;
+  explicit ProfScope(const char *n) : name(n) {
+    if (S.prof) {
+      a = ev_get();
+      (void)hipEventRecord(a, S.stream);
+    }
+  }
+  ~ProfScope() {
+    if (S.prof) {
+      hipEvent_t b = ev_get();
+      (void)hipEventRecord(b, S.stream);
+      S.pending.push_back({name, a, b});
+      if (S.pending.size() > 4096) prof_flush();
+    }
+  }
+};
+
+#define LAUNCH(name, kern, grid, block, ...)                                   \
+  do {                                                                         \
+    ProfScope ps_(name);                                                       \
+    hipLaunchKernelGGL(kern, grid, block, 0, S.stream, __VA_ARGS__);           \
+    hipError_t e_ = hipGetLastError();                                         \
+    if (e_ != hipSuccess) return fail(-(int)e_, "launch %s: %s", name, hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCH_BOX(name, kern, box, ...) LAUNCH(name, kern, box_grid(box), box_block(), S.g, box, __VA_ARGS__)
+
+int nx() { return S.n[0]; }
+int ny() { return S.n[1]; }
+int nz() { return S.n[2]; }
+Box full_box() { return mkbox(-2, nx() + 2, -2, ny() + 2, -2, nz() + 2); }
+Box cell_box() { return mkbox(-1, nx() + 2, -1, ny() + 2, -1, nz() + 2); }
+Box interior() { return mkbox(1, nx(), 1, ny(), 1, nz()); }
+Box nodes0n() { return mkbox(0, nx(), 0, ny(), 0, nz()); }
+
+// grid of a z-marching launch: BX x BY columns, ZCHUNK planes per block
+dim3 zchunk_grid(const Box &b) {
+  return dim3((unsigned)((b.x1 - b.x0 + 1 + BX - 1) / BX), (unsigned)((b.y1 - b.y0 + 1 + BY - 1) / BY),
+              (unsigned)((b.z1 - b.z0 + 1 + ZCHUNK - 1) / ZCHUNK));
+}
+
+// reduction over partials of a grid launched from index 0 on every axis
+dim3 grid_from0(int x1, int y1, int z1) { return box_grid(mkbox(0, x1, 0, y1, 0, z1)); }
+
+// Two-level reduction of the per-block partials (rows of nv values; bit q of
+// minmask selects min, else sum).  Level 1: RED_ROWS blocks each fold one
+// contiguous chunk of rows; level 2: one block folds the RED_ROWS results and
+// stores (or adds) the nv scalars.
+#define RED_ROWS 512
+#define RED_MAXV 4
+
+__device__ __forceinline__ void fold_rows(const double *__restrict__ rows, int r0, int r1, int nv, int minmask,
+                                          double *__restrict__ out, int accumulate) {
+  __shared__ double red[4][RED_MAXV];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int q = 0; q < nv; ++q) {
+    const bool isMin = (minmask >> q) & 1;
+    double v = isMin ? 1.7976931348623157e+308 : 0.0;
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += 256) {
+      const double x = rows[(size_t)r * nv + q];
+      v = isMin ? dmin(v, x) : v + x;
+    }
+    v = isMin ? wave_min(v) : wave_sum(v);
+    if (lane == 0) red[wave][q] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nv) {
+    const int q = threadIdx.x;
+    const bool isMin = (minmask >> q) & 1;
+    double r = red[0][q];
+    for (int w = 1; w < 4; ++w) r = isMin ? dmin(r, red[w][q]) : r + red[w][q];
+    if (accumulate) r = out[q] + r;
+    out[q] = r;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_rows(const double *__restrict__ partial, int nrows, int nv, int minmask,
+                                                     int chunk, double *__restrict__ level1) {
+  const int r0 = (int)blockIdx.x * chunk;
+  const int r1 = r0 + chunk < nrows ? r0 + chunk : nrows;
+  fold_rows(partial, r0, r1, nv, minmask, level1 + (size_t)blockIdx.x * nv, 0);
+}
+
+__global__ __launch_bounds__(256) void k_final_reduce(const double *__restrict__ level1, int nrows, int nv, int minmask,
+                                                      int accumulate, double *__restrict__ out) {
+  fold_rows(level1, 0, nrows, nv, minmask, out, accumulate);
+}
+
+int final_reduce(dim3 grid, int nv, int minmask, int slot, int accumulate) {
+  const int nb = (int)(grid.x * grid.y * grid.z);
+  if (nv > RED_MAXV || (size_t)nb * nv > S.partial_cap)
+    return fail(LARE_E_ARG, "reduction of %d x %d values exceeds the partial buffer", nb, nv);
+  const int chunk = (nb + RED_ROWS - 1) / RED_ROWS;
+  const int rows = (nb + chunk - 1) / chunk;  // every level-1 block has >= 1 row
+  double *level1 = S.partial + S.partial_cap;
+  LAUNCH("reduce_rows", k_reduce_rows, dim3(rows), dim3(256), S.partial, nb, nv, minmask, chunk, level1);
+  LAUNCH("final_reduce", k_final_reduce, dim3(1), dim3(256), level1, rows, nv, minmask, accumulate, S.dscal + slot);
+  return 0;
+}
+
+int read_scalars(int slot, int count, double *out) {
+  HIPCHK(hipMemcpyAsync(S.hscal, S.dscal + slot, sizeof(double) * count, hipMemcpyDeviceToHost, S.stream));
+  HIPCHK(hipStreamSynchronize(S.stream));
+  for (int q = 0; q < count; ++q) out[q] = S.hscal[q];
+  return 0;
+}
+
+// ----------------------------------------------------------------------------
+// boundary groups
+// ----------------------------------------------------------------------------
+enum FieldKind { FK_CELL, FK_NODE, FK_BX, FK_BY, FK_BZ };
+
+struct BcField {
+  void *a;
+  FieldKind kind;
+};
+
+// FP64 storage: face B (and predictor B); everything else in `real`
+bool wide_kind(FieldKind k) { return k == FK_BX || k == FK_BY || k == FK_BZ || sizeof(real) == sizeof(double); }
+
+int op_for(FieldKind k, int axis, bool periodic) {
+  switch (k) {
+    case FK_CELL: return periodic ? BC_PER_CELL : BC_MIR_CELL;
+    case FK_NODE: return periodic ? BC_PER_NODE : BC_ZERO_NODE;
+    case FK_BX: return axis == 0 ? (periodic ? BC_PER_FACE : BC_MIR_FACE) : (periodic ? BC_PER_CELL : BC_MIR_CELL);
+    case FK_BY: return axis == 1 ? (periodic ? BC_PER_FACE : BC_MIR_FACE) : (periodic ? BC_PER_CELL : BC_MIR_CELL);
+    case FK_BZ: return axis == 2 ? (periodic ? BC_PER_FACE : BC_MIR_FACE) : (periodic ? BC_PER_CELL : BC_MIR_CELL);
+  }
+  return BC_NONE;
+}
+
+int launch_bc_axis(const BcSet &s, int axis, const char *name) {
+  const int u = axis == 0 ? 1 : 0, w = axis == 2 ? 1 : 2;
+  dim3 grid((unsigned)((S.n[u] + 5 + 63) / 64), (unsigned)(S.n[w] + 5));
+  LAUNCH(name, k_bc_axis, grid, dim3(64), S.g, s, axis);
+  return 0;
+}
+
+// exchanges on periodic axes (x, y, z), then user fills (x, y, z)
+int apply_bcs(const BcField *f, int nf, const char *name) {
+  for (int pass = 0; pass < 2; ++pass)
+    for (int ax = 0; ax < 3; ++ax) {
+      const bool periodic = S.per[ax];
+      if ((pass == 0) != periodic) continue;
+      BcSet s = {};
+      s.nf = nf;
+      for (int q = 0; q < nf; ++q) {
+        s.f[q] = f[q].a;
+        s.op[q] = op_for(f[q].kind, ax, periodic);
+        s.wide[q] = wide_kind(f[q].kind) ? 1 : 0;
+      }
+      int rc = launch_bc_axis(s, ax, name);
+      if (rc) return rc;
+    }
+  return 0;
+}
+
+int bfield_bcs(double *bx, double *by, double *bz) {
+  BcField f[3] = {{bx, FK_BX}, {by, FK_BY}, {bz, FK_BZ}};
+  return apply_bcs(f, 3, "bc_bfield");
+}
+int energy_bcs() {
+  BcField f[1] = {{S.energy, FK_CELL}};
+  return apply_bcs(f, 1, "bc_energy");
+}
+int boundary_conditions() {
+  BcField f[8] = {{S.bx, FK_BX},     {S.by, FK_BY},   {S.bz, FK_BZ},   {S.energy, FK_CELL},
+                  {S.rho, FK_CELL},  {S.vx, FK_NODE}, {S.vy, FK_NODE}, {S.vz, FK_NODE}};
+  return apply_bcs(f, 8, "bc_all");
+}
+
+// ----------------------------------------------------------------------------
+// conduction
+// ----------------------------------------------------------------------------
+int calc_s_stages() {
+  dim3 gr = grid_from0(nx(), ny(), nz());
+  LAUNCH("larsen", k_larsen, gr, box_block(), S.g, S.energy, S.rho, S.larsen, S.partial, S.p.flux_limiter,
+         S.p.kappa_0, S.tfac, S.p.heat_flux_limiter);
+  int rc = final_reduce(gr, 1, 1, 0, 0);
+  if (rc) return rc;
+  double dtp;
+  rc = read_scalars(0, 1, &dtp);
+  if (rc) return rc;
+  // the reference starts its minimum at 1e10 (conduct.f90:110)
+  double dt_parab = dtp < 1.0e10 ? dtp : 1.0e10;
+  dt_parab = 0.9 * dt_parab / sqrt(3.0);
+  const double stages = 0.5 * (sqrt(9.0 + 16.0 * (S.dt / dt_parab)) - 1.0);
+  int ns = (int)ceil(stages);
+  if (ns % 2 == 0) ns = ns + 1;
+  S.n_s_stages = ns;
+  return 0;
+}
+
+// RKL2 super-step; delta_mode = 1 inside lagrangian_step
+int heat_conduct_sts2(int delta_mode) {
+  const int s = S.n_s_stages;
+  Box in = interior();
+  if (s <= 1) {  // the reference's Y(3) is never written: energy unchanged
+    if (delta_mode) LAUNCH_BOX("energy_max", k_energy_max, cell_box(), S.energy);
+    return 0;
+  }
+  std::vector<double> b(s + 1), a(s + 1), mu_t(s + 1), mu(s + 1), nu(s + 1), gt(s + 1);
+  const double ds = (double)s;
+  const double omega_1 = 4.0 / (ds * ds + ds - 2.0);
+  for (int j = 0; j <= 2; ++j) b[j] = 1.0 / 3.0;
+  for (int j = 3; j <= s; ++j) {
+    const double dj = (double)j;
+    b[j] = (dj * dj + dj - 2.0) / (2.0 * dj * (dj + 1.0));
+  }
+  for (int j = 0; j <= s; ++j) a[j] = 1.0 - b[j];
+  mu_t[1] = omega_1 / 3.0;
+  for (int j = 2; j <= s; ++j) {
+    const double dj = (double)j;
+    mu_t[j] = ((2.0 * dj - 1.0) / dj) * omega_1 * (b[j] / b[j - 1]);
+    mu[j] = ((2.0 * dj - 1.0) / dj) * (b[j] / b[j - 1]);
+    nu[j] = -1.0 * ((dj - 1.0) / dj) * (b[j] / b[j - 2]);
+    gt[j] = -1.0 * a[j - 1] * mu_t[j];
+  }
+  CondArgs ca = {S.p.kappa_0, S.tfac, S.dt};
+  real *E0 = S.energy;
+  real *outp = S.Yb[0];
+  LAUNCH_BOX("rkl2_stage1", k_rkl2_stage1, in, E0, S.rho, S.bx, S.by, S.bz, S.larsen, S.Lc, outp, ca, mu_t[1]);
+  real *y1 = E0, *y2 = outp;
+  int next = 1;
+  for (int j = 2; j <= s; ++j) {
+    real *yo = S.Yb[next];
+    next = (next + 1) % 3;
+    const int mode = j == 2 ? TMODE_STALE : TMODE_WRAP;
+    LAUNCH_BOX("rkl2_stage", k_rkl2_stage, in, y2, y1, E0, S.rho, S.bx, S.by, S.bz, S.larsen, S.Lc, yo, ca, mode,
+               mu[j], nu[j], (1.0 - mu[j]) - nu[j], mu_t[j], gt[j]);
+    y1 = y2;
+    y2 = yo;
+  }
+  LAUNCH_BOX("rkl2_final", k_rkl2_final, cell_box(), y2, S.energy, delta_mode);
+  return 0;
+}
+
+// ----------------------------------------------------------------------------
+// remap
+// ----------------------------------------------------------------------------
+int strides(int ax) { return ax == 0 ? 1 : (ax == 1 ? S.sy : S.sz); }
+
+template <int AX>
+int ct_pair(double *bxa, double *bya, double *bza, const real *vel, double dt) {
+  const int N0 = nx(), N1 = ny(), N2 = nz();
+  Box fb = mkbox(-1, N0 + 1, -1, N1 + 1, -1, N2 + 1);
+  Box ub = mkbox(0, N0 + 1, 0, N1 + 1, 0, N2 + 1);
+  double *B[3] = {bxa, bya, bza};
+  // (transverse field t, velocity-average axis, geometry-average axis, box_t, box_a) x 2
+  int t1, va1, ga1, t2, va2, ga2;
+  Box bt1 = ub, ba1 = ub, bt2 = ub, ba2 = ub;
+  if (AX == 0) {
+    t1 = 1; va1 = 2; ga1 = 1;  // vx_by_flux
+    t2 = 2; va2 = 1; ga2 = 2;  // vx_bz_flux
+    bt2 = mkbox(1, N0, 1, N1, 0, N2);
+    ba2 = mkbox(0, N0, 0, N1 + 1, 0, N2 + 1);
+  } else if (AX == 1) {
+    t1 = 0; va1 = 2; ga1 = 0;  // vy_bx_flux
+    t2 = 2; va2 = 0; ga2 = 2;  // vy_bz_flux
+    ba2 = mkbox(1, N0, 0, N1, 1, N2);
+  } else {
+    t1 = 0; va1 = 1; ga1 = 0;  // vz_bx_flux
+    t2 = 1; va2 = 0; ga2 = 1;  // vz_by_flux
+  }
+  for (int pair = 0; pair < 2; ++pair) {
+    const int t = pair ? t2 : t1, va = pair ? va2 : va1, ga = pair ? ga2 : ga1;
+    if (AX == 2) {
+      LAUNCH("ct_flux", k_ct_flux_z, zchunk_grid(fb), box_block(), S.g, fb, B[t], vel, S.db1, S.F, strides(va),
+             strides(ga), dt);
+    } else {
+      LAUNCH_BOX("ct_flux", k_ct_flux<AX>, fb, B[t], vel, S.db1, S.F, strides(va), strides(ga), dt);
+    }
+    LAUNCH_BOX("ct_update", k_ct_update, ub, S.F, B[t], B[AX], strides(AX), strides(t), pair ? bt2 : bt1,
+               pair ? ba2 : ba1);
+  }
+  return 0;
+}
+
+// one predictor-mode sweep (B only, no flag reset, no halo refresh)
+template <int AX>
+int sweep_predictor(double dt, bool last) {
+  LAUNCH("geom", k_geom<AX>, zchunk_grid(cell_box()), box_block(), S.g, cell_box(), S.vx, S.vy, S.vz, S.cv1, S.cv2, S.db1, dt, 1.0, 1.0, 1.0,
+             last ? 1 : 0, 0);
+  const real *vel = AX == 0 ? S.vx : (AX == 1 ? S.vy : S.vz);
+  return ct_pair<AX>(S.bpx, S.bpy, S.bpz, vel, dt);
+}
+
+template <int AX>
+int sweep_full(double dt, double pass[3]) {
+  const int N[3] = {nx(), ny(), nz()};
+  const real *vel = AX == 0 ? S.vx1 : (AX == 1 ? S.vy1 : S.vz1);
+  LAUNCH("geom", k_geom<AX>, zchunk_grid(cell_box()), box_block(), S.g, cell_box(), S.vx1, S.vy1, S.vz1, S.cv1, S.cv2, S.db1, dt, pass[0], pass[1],
+             pass[2], 1, 1);
+  int rc = ct_pair<AX>(S.bx, S.by, S.bz, vel, dt);
+  if (rc) return rc;
+  int hi[3] = {N[0] + 1, N[1] + 1, N[2] + 1};
+  hi[AX] = N[AX];
+  Box mb = mkbox(0, hi[0], 0, hi[1], 0, hi[2]);
+  if (AX == 2) LAUNCH("mass_flux", k_mass_flux_z, zchunk_grid(mb), box_block(), S.g, mb, S.rho, vel, S.db1, S.dm, dt);
+  else LAUNCH_BOX("mass_flux", k_mass_flux<AX>, mb, S.rho, vel, S.db1, S.dm, dt);
+  {
+    BcSet s = {};
+    s.nf = 1;
+    s.f[0] = S.dm;
+    s.op[0] = S.per[AX] ? BC_PER_CELL : BC_DM_CLAMP;
+    s.wide[0] = wide_kind(FK_CELL) ? 1 : 0;
+    rc = launch_bc_axis(s, AX, "bc_dm");
+    if (rc) return rc;
+  }
+  if (AX == 2)
+    LAUNCH("energy_flux", k_energy_flux_z, zchunk_grid(nodes0n()), box_block(), S.g, nodes0n(), S.energy, S.rho, vel,
+           S.db1, S.dm, S.Fe, dt);
+  else LAUNCH_BOX("energy_flux", k_energy_flux<AX>, nodes0n(), S.energy, S.rho, vel, S.db1, S.dm, S.Fe, dt);
+  LAUNCH_BOX("rho_energy_update", k_rho_energy_update<AX>, interior(), S.rho, S.rho_alt, S.energy, S.cv1, S.cv2,
+             S.dm, S.Fe);
+  {
+    int lo[3] = {0, 0, 0}, h2[3] = {N[0], N[1], N[2]};
+    lo[AX] = -1;
+    h2[AX] = N[AX] + 1;
+    Box vb = mkbox(lo[0], h2[0], lo[1], h2[1], lo[2], h2[2]);
+    LAUNCH("vertex", k_vertex<AX>, zchunk_grid(vb), box_block(), S.g, vb, S.rho, S.cv1, S.cv2, S.dm, S.rho_vr, S.cv1v,
+           S.cv2v, S.dmv);
+    h2[AX] = N[AX];
+    Box fb = mkbox(lo[0], h2[0], lo[1], h2[1], lo[2], h2[2]);
+    if (AX == 2)
+      LAUNCH("mom_flux", k_mom_flux_z, zchunk_grid(fb), box_block(), S.g, fb, S.vx, S.vy, S.vz, vel, S.db1, S.rho_vr,
+             S.dmv, S.Fx, S.Fy, S.Fz, dt);
+    else LAUNCH_BOX("mom_flux", k_mom_flux<AX>, fb, S.vx, S.vy, S.vz, vel, S.db1, S.rho_vr, S.dmv, S.Fx, S.Fy, S.Fz, dt);
+  }
+  LAUNCH_BOX("mom_update", k_mom_update<AX>, nodes0n(), S.vx, S.vy, S.vz, S.rho_vr, S.cv1v, S.cv2v, S.dmv, S.Fx,
+             S.Fy, S.Fz);
+  real *t = S.rho;
+  S.rho = S.rho_alt;
+  S.rho_alt = t;
+  rc = boundary_conditions();
+  if (rc) return rc;
+  pass[AX] = 0.0;
+  return 0;
+}
+
+const int kStrang[6][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}, {0, 2, 1}, {2, 1, 0}, {1, 0, 2}};
+
+int eulerian_remap(int step, bool predictor) {
+  const double dt = S.dt;
+  const int c = ((step % 6) + 6) % 6;
+  if (predictor) {
+    LAUNCH_BOX("bscale", k_bscale, full_box(), S.bx, S.by, S.bz, S.bpx, S.bpy, S.bpz, 1);
+    for (int q = 0; q < 3; ++q) {
+      const int ax = kStrang[c][q];
+      const bool last = q == 2;
+      int rc = ax == 0 ? sweep_predictor<0>(dt, last) : (ax == 1 ? sweep_predictor<1>(dt, last) : sweep_predictor<2>(dt, last));
+      if (rc) return rc;
+    }
+    LAUNCH_BOX("bscale", k_bscale, full_box(), S.bpx, S.bpy, S.bpz, S.bpx, S.bpy, S.bpz, -1);
+    return bfield_bcs(S.bpx, S.bpy, S.bpz);
+  }
+  double pass[3] = {1.0, 1.0, 1.0};
+  LAUNCH_BOX("bscale", k_bscale, full_box(), S.bx, S.by, S.bz, S.bx, S.by, S.bz, 1);
+  for (int q = 0; q < 3; ++q) {
+    const int ax = kStrang[c][q];
+    int rc = ax == 0 ? sweep_full<0>(dt, pass) : (ax == 1 ? sweep_full<1>(dt, pass) : sweep_full<2>(dt, pass));
+    if (rc) return rc;
+  }
+  LAUNCH_BOX("bscale", k_bscale, full_box(), S.bx, S.by, S.bz, S.bx, S.by, S.bz, -1);
+  return bfield_bcs(S.bx, S.by, S.bz);
+}
+
+// ----------------------------------------------------------------------------
+// lagrangian step
+// ----------------------------------------------------------------------------
+int eta_calc() {
+  if (S.p.resistive_mhd) {
+    LAUNCH_BOX("eta", k_eta, mkbox(-1, nx() + 1, -1, ny() + 1, -1, nz() + 1), S.bx, S.by, S.bz, S.eta,
+               S.p.eta_background, S.p.eta0, S.p.j_max);
+  } else {
+    LAUNCH_BOX("fill", k_fill, cell_box(), S.eta, 0.0);
+  }
+  return 0;
+}
+
+LagArgs lag_args() {
+  LagArgs a;
+  a.gamma = S.p.gamma;
+  a.visc1 = S.p.visc1;
+  a.visc2_norm = S.visc2_norm;
+  a.dt = S.dt;
+  a.dt2 = S.dt2;
+  a.eta_bg = S.p.eta_background;
+  a.eta0 = S.p.eta0;
+  a.j_max = S.p.j_max;
+  a.resistive = S.p.resistive_mhd;
+  return a;
+}
+
+int set_dt_and_stages(double dt_local, double dtr_local) {
+  if (S.first_set_dt) {
+    S.first_set_dt = 0;
+    if (S.p.restart) {
+      S.dt = S.p.dt_from_restart;
+      return 0;
+    }
+  }
+  S.dt = 0.9 * dt_local;
+  S.dtr = 0.9 * dtr_local;
+  if (S.p.conduction) {
+    int rc = calc_s_stages();
+    if (rc) return rc;
+    const int ss_limit = 60;
+    if (S.n_s_stages >= ss_limit) {
+      const double f = (double)(2 * ss_limit * ss_limit - 9) / (double)(2 * S.n_s_stages * S.n_s_stages - 9);
+      S.dt = S.dt * f;
+      S.dtr = S.dtr * f;
+    }
+  }
+  S.time = S.time + S.dt;
+  return 0;
+}
+
+// learned conduction surrogate (surrogate_rt.hpp)
+int sur_enabled();
+int sur_conduct_step(int delta_mode, double *res_out, int *acc_out);
+
+int lagrangian_step() {
+  const int N0 = nx(), N1 = ny(), N2 = nz();
+  LagArgs a = lag_args();  // dt = previous step's dt (Q1)
+  LAUNCH_BOX("vertex_prep", k_vertex_prep, mkbox(-1, N0 + 1, -1, N1 + 1, -1, N2 + 1), S.rho, S.energy, S.bx, S.by,
+             S.bz, S.rho_v, S.cs_v, S.p.gamma);
+  if (S.p.use_viscous_damping) {
+    LAUNCH_BOX("damp_q", k_damp_q, nodes0n(), S.vx, S.vy, S.vz, S.visc3, S.qx, S.qy, S.qz);
+    LAUNCH_BOX("damp_apply", k_damp_apply, nodes0n(), S.vx, S.vy, S.vz, S.qx, S.qy, S.qz, S.dt);
+  }
+  LAUNCH_BOX("alpha", k_alpha, mkbox(-1, N0 + 1, -1, N1 + 2, 0, N2 + 2), S.vx, S.vy, S.vz, S.rho_v, S.cs_v, S.al1,
+             S.al2, S.al3, a);
+  dim3 gr = grid_from0(N0 + 1, N1 + 1, N2 + 1);
+  LAUNCH("visc_dt", k_visc_dt, gr, box_block(), S.g, S.vx, S.vy, S.vz, S.rho, S.energy, S.bx, S.by, S.bz, S.eta,
+         S.al1, S.al2, S.al3, S.visc_heat, S.p0, S.partial, a);
+  int rc = final_reduce(gr, 2, 3, 0, 0);
+  if (rc) return rc;
+  double mins[2];
+  rc = read_scalars(0, 2, mins);
+  if (rc) return rc;
+  rc = set_dt_and_stages(mins[0], mins[1]);
+  if (rc) return rc;
+  S.dt2 = 0.5 * S.dt;
+  S.last_substeps = 0;
+
+  if (S.p.resistive_mhd) {
+    const double dt_sub = S.dtr;
+    const int substeps = (int)(S.dt / dt_sub) + 1;
+    if (substeps > S.peak_substeps) S.peak_substeps = substeps;
+    S.last_substeps = substeps;
+    const double actual_dt = S.dt;
+    S.dt = S.dt / (double)substeps;
+    for (int sc = 1; sc <= substeps; ++sc) {
+      rc = eta_calc();
+      if (rc) return rc;
+      dim3 g2 = grid_from0(N0, N1, N2);
+      LAUNCH("rkstep", k_rkstep, g2, box_block(), S.g, S.bx, S.by, S.bz, S.eta, S.fxr, S.fyr, S.fzr, S.curlb,
+             S.partial, S.dt);
+      rc = final_reduce(g2, 1, 0, 9, 1);
+      if (rc) return rc;
+      LAUNCH_BOX("bstep", k_bstep, nodes0n(), S.bx, S.by, S.bz, S.fxr, S.fyr, S.fzr, S.dt);
+      rc = bfield_bcs(S.bx, S.by, S.bz);
+      if (rc) return rc;
+      LAUNCH_BOX("ohmic", k_ohmic, cell_box(), S.energy, S.ohmic_dep, S.rho, S.curlb, S.dt);
+      rc = energy_bcs();
+      if (rc) return rc;
+    }
+    S.dt = actual_dt;
+  }
+
+  if (S.p.conduction) {
+    if (sur_enabled()) {
+      rc = sur_conduct_step(1, nullptr, nullptr);
+      if (rc) return rc;
+    } else {
+      rc = calc_s_stages();
+      if (rc) return rc;
+      rc = heat_conduct_sts2(1);
+      if (rc) return rc;
+    }
+  } else {
+    LAUNCH_BOX("energy_max", k_energy_max, cell_box(), S.energy);
+  }
+
+  // predictor-corrector (lagran.F90:208-478)
+  const double dt_full = S.dt;
+  S.dt = 0.5 * dt_full;
+  rc = eulerian_remap(S.step, true);
+  S.dt = dt_full;
+  if (rc) return rc;
+  LAUNCH_BOX("pred_pressure", k_pred_pressure, mkbox(0, N0 + 1, 0, N1 + 1, 0, N2 + 1), S.energy, S.rho, S.cv1,
+             S.visc_heat, S.p0, S.pp, S.p.gamma, S.dt2);
+  LAUNCH_BOX("force", k_force, nodes0n(), S.vx, S.vy, S.vz, S.pp, S.bpx, S.bpy, S.bpz, S.rho_v, S.al1, S.al2, S.al3,
+             S.vx1, S.vy1, S.vz1, S.dt2);
+  {
+    BcField f[3] = {{S.vx1, FK_NODE}, {S.vy1, FK_NODE}, {S.vz1, FK_NODE}};
+    rc = apply_bcs(f, 3, "bc_v1");
+    if (rc) return rc;
+  }
+  dim3 g3 = box_grid(interior());
+  LAUNCH("shock_corr", k_shock_corr, g3, box_block(), S.g, S.vx, S.vy, S.vz, S.vx1, S.vy1, S.vz1, S.al1, S.al2,
+         S.al3, S.pp, S.energy, S.rho, S.visc_dep, S.partial, S.dt);
+  rc = final_reduce(g3, 1, 0, 8, 1);
+  if (rc) return rc;
+  LAUNCH_BOX("vupdate", k_vupdate, nodes0n(), S.vx, S.vy, S.vz, S.vx1, S.vy1, S.vz1);
+  {
+    BcField f[5] = {{S.energy, FK_CELL}, {S.rho, FK_CELL}, {S.vx, FK_NODE}, {S.vy, FK_NODE}, {S.vz, FK_NODE}};
+    rc = apply_bcs(f, 5, "bc_lagrangian");
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+// ----------------------------------------------------------------------------
+// memory
+// ----------------------------------------------------------------------------
+void sur_free();
+void free_all() {
+  sur_free();
+  if (S.stream) (void)hipStreamSynchronize(S.stream);
+  for (void *p : S.allocs) (void)hipFree(p);
+  S.allocs.clear();
+  if (S.metric) (void)hipFree(S.metric);
+  S.metric = nullptr;
+  if (S.partial) (void)hipFree(S.partial);
+  S.partial = nullptr;
+  if (S.dscal) (void)hipFree(S.dscal);
+  S.dscal = nullptr;
+  if (S.hscal) (void)hipHostFree(S.hscal);
+  S.hscal = nullptr;
+  for (auto &r : S.pending) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  S.pending.clear();
+  for (auto e : S.pool) (void)hipEventDestroy(e);
+  S.pool.clear();
+  S.acc.clear();
+  S.ready = S.loaded = false;
+}
+
+template <class T>
+int dalloc(T **p) {
+  hipError_t e = hipMalloc((void **)p, sizeof(T) * S.ntot);
+  if (e != hipSuccess) return fail(LARE_E_NOMEM, "hipMalloc of %zu elements failed: %s", S.ntot, hipGetErrorString(e));
+  S.allocs.push_back((void *)*p);
+  e = hipMemsetAsync(*p, 0, sizeof(T) * S.ntot, S.stream);
+  if (e != hipSuccess) return fail(-(int)e, "hipMemset failed");
+  return 0;
+}
+
+// Fortran array bounds (lower) per field kind
+void lower_of(FieldKind k, int lo[3]) {
+  lo[0] = lo[1] = lo[2] = (k == FK_NODE) ? -2 : -1;
+  if (k == FK_BX) lo[0] = -2;
+  if (k == FK_BY) lo[1] = -2;
+  if (k == FK_BZ) lo[2] = -2;
+}
+
+// host staging buffer of one element type
+template <class T>
+std::vector<T> &stage_buf() {
+  static std::vector<T> v;
+  return v;
+}
+
+// copy one field between a host Fortran array (FP64, reference bounds) and
+// its padded device box, converting to the device storage type
+template <class T>
+int xfer(T *dev, double *host, FieldKind k, bool to_dev) {
+  int lo[3];
+  lower_of(k, lo);
+  const long ex = nx() + 2 - lo[0] + 1, ey = ny() + 2 - lo[1] + 1;
+  std::vector<T> &st = stage_buf<T>();
+  if (st.size() != S.ntot) st.assign(S.ntot, (T)0);
+  HIPCHK(hipMemcpy(st.data(), dev, sizeof(T) * S.ntot, hipMemcpyDeviceToHost));
+  for (int kk = lo[2]; kk <= nz() + 2; ++kk)
+    for (int jj = lo[1]; jj <= ny() + 2; ++jj)
+      for (int ii = lo[0]; ii <= nx() + 2; ++ii) {
+        const long h = ((long)(kk - lo[2]) * ey + (jj - lo[1])) * ex + (ii - lo[0]);
+        if (to_dev) st[S.g.idx(ii, jj, kk)] = (T)host[h];
+        else host[h] = (double)st[S.g.idx(ii, jj, kk)];
+      }
+  if (to_dev) HIPCHK(hipMemcpy(dev, st.data(), sizeof(T) * S.ntot, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// ============================================================================
+// entry points of this precision (wrapped by the C-ABI in lare_dev.hip)
+// ============================================================================
+
+
+int api_init(const lare_grid *grid, const lare_params *params) {
+  if (!grid || !params) return fail(LARE_E_ARG, "lare_dev_init: NULL argument");
+  free_all();
+  const int n[3] = {grid->nx, grid->ny, grid->nz};
+  const int bc[6] = {grid->xbc_min, grid->xbc_max, grid->ybc_min, grid->ybc_max, grid->zbc_min, grid->zbc_max};
+  for (int a = 0; a < 3; ++a) {
+    if (n[a] < 4) return fail(LARE_E_ARG, "grid size %d on axis %d is below 4", n[a], a);
+    const int lo = bc[2 * a], hi = bc[2 * a + 1];
+    if (lo != hi || (lo != LARE_BC_PERIODIC && lo != LARE_BC_USER))
+      return fail(LARE_E_ARG, "axis %d: boundary codes (%d, %d) unsupported (need both periodic or both BC_USER)", a, lo, hi);
+  }
+  if (!grid->dxb || !grid->dxc || !grid->dyb || !grid->dyc || !grid->dzb || !grid->dzc)
+    return fail(LARE_E_ARG, "lare_dev_init: metric arrays missing");
+  memcpy(S.n, n, sizeof(n));
+  memcpy(S.bc, bc, sizeof(bc));
+  for (int a = 0; a < 3; ++a) {
+    S.per[a] = bc[2 * a] == LARE_BC_PERIODIC;
+    S.user[a] = bc[2 * a] == LARE_BC_USER;
+  }
+  S.p = *params;
+  S.tfac = (params->gamma - 1.0) / (2.0 - params->xi_n);
+  S.visc2_norm = 0.25 * (params->gamma + 1.0) * params->visc2;
+  S.sy = ((n[0] + 19 + 15) / 16) * 16;
+  S.sz = S.sy * (n[1] + 5);
+  S.ntot = (size_t)S.sz * (size_t)(n[2] + 5);
+  if (S.ntot > (size_t)2147483647) return fail(LARE_E_ARG, "grid too large for 32-bit indexing");
+  if (!S.stream) HIPCHK(hipStreamCreate(&S.stream));  // default priority
+  // metrics: 7 arrays, each with index offset 2, length nmax + 6
+  const int nmax = n[0] > n[1] ? (n[0] > n[2] ? n[0] : n[2]) : (n[1] > n[2] ? n[1] : n[2]);
+  const int L = nmax + 6;
+  std::vector<double> hm(7 * (size_t)L, 0.0);
+  const double *src[7] = {grid->dxb, grid->dxc, grid->dyb, grid->dyc, grid->dzb, grid->dzc, grid->grav};
+  for (int q = 0; q < 7; ++q) {
+    const int ax = q < 6 ? q / 2 : 2;
+    const int hi = (q < 6 && (q % 2) == 1) ? n[ax] + 1 : n[ax] + 2;
+    for (int i = -1; i <= hi; ++i) hm[(size_t)q * L + (i + 2)] = src[q] ? src[q][i + 1] : 0.0;
+  }
+  HIPCHK(hipMalloc((void **)&S.metric, sizeof(double) * hm.size()));
+  HIPCHK(hipMemcpy(S.metric, hm.data(), sizeof(double) * hm.size(), hipMemcpyHostToDevice));
+  S.g.nx = n[0];
+  S.g.ny = n[1];
+  S.g.nz = n[2];
+  S.g.sy = S.sy;
+  S.g.sz = S.sz;
+  const double *m[7];
+  for (int q = 0; q < 7; ++q) m[q] = S.metric + (size_t)q * L + 2;
+  S.g.dxb = m[0]; S.g.dxc = m[1]; S.g.dyb = m[2]; S.g.dyc = m[3]; S.g.dzb = m[4]; S.g.dzc = m[5]; S.g.grav = m[6];
+  for (int a = 0; a < 3; ++a) {
+    S.g.per[a] = S.per[a];
+    S.g.user[a] = S.user[a];
+  }
+  real **bulk[] = {&S.rho, &S.rho_alt, &S.energy, &S.vx, &S.vy, &S.vz, &S.vx1, &S.vy1, &S.vz1, &S.eta,
+                    &S.visc3, &S.visc_dep, &S.ohmic_dep, &S.rho_v, &S.cs_v, &S.al1, &S.al2, &S.al3,
+                    &S.visc_heat, &S.p0, &S.pp, &S.curlb, &S.cv1, &S.cv2, &S.db1, &S.Fe, &S.dm, &S.Fx, &S.Fy,
+                    &S.Fz, &S.rho_vr, &S.cv1v, &S.cv2v, &S.dmv, &S.larsen, &S.Lc, &S.Yb[0], &S.Yb[1], &S.Yb[2]};
+  for (real **pp : bulk) {
+    int rc = dalloc(pp);
+    if (rc) return rc;
+  }
+  double **wide[] = {&S.bx, &S.by, &S.bz, &S.bpx, &S.bpy, &S.bpz, &S.fxr, &S.fyr, &S.fzr, &S.F};
+  for (double **pp : wide) {
+    int rc = dalloc(pp);
+    if (rc) return rc;
+  }
+  if (params->use_viscous_damping) {
+    int rc = dalloc(&S.qx);
+    if (!rc) rc = dalloc(&S.qy);
+    if (!rc) rc = dalloc(&S.qz);
+    if (rc) return rc;
+  } else {
+    S.qx = S.qy = S.qz = nullptr;
+  }
+  // partial sums: largest reduction grid (0..n+1 per axis) x 4 values
+  dim3 gr = grid_from0(n[0] + 2, n[1] + 2, n[2] + 2);
+  S.partial_cap = (size_t)gr.x * gr.y * gr.z * 4;
+  // + RED_ROWS x RED_MAXV level-1 results after the per-block partials
+  HIPCHK(hipMalloc((void **)&S.partial, sizeof(double) * (S.partial_cap + RED_ROWS * RED_MAXV)));
+  HIPCHK(hipMalloc((void **)&S.dscal, sizeof(double) * 16));
+  HIPCHK(hipMemset(S.dscal, 0, sizeof(double) * 16));
+  HIPCHK(hipHostMalloc((void **)&S.hscal, sizeof(double) * 16));
+  HIPCHK(hipStreamSynchronize(S.stream));
+  S.dt = S.dt2 = S.dtr = S.time = 0.0;
+  S.n_s_stages = S.peak_substeps = S.last_substeps = 0;
+  S.first_set_dt = 1;
+  S.step = 0;
+  S.ready = true;
+  return 0;
+}
+
+int api_upload(const lare_fields *f) {
+  if (!S.ready) return fail(LARE_E_STATE, "lare_dev_upload before lare_dev_init");
+  if (!f || !f->rho || !f->energy || !f->vx || !f->vy || !f->vz || !f->bx || !f->by || !f->bz)
+    return fail(LARE_E_ARG, "lare_dev_upload: missing field");
+  if (S.p.use_viscous_damping && !f->visc3) return fail(LARE_E_ARG, "visc3 required with viscous damping");
+  HIPCHK(hipStreamSynchronize(S.stream));
+  int rc = 0;
+  rc |= xfer(S.rho, f->rho, FK_CELL, true);
+  rc |= xfer(S.energy, f->energy, FK_CELL, true);
+  rc |= xfer(S.vx, f->vx, FK_NODE, true);
+  rc |= xfer(S.vy, f->vy, FK_NODE, true);
+  rc |= xfer(S.vz, f->vz, FK_NODE, true);
+  rc |= xfer(S.bx, f->bx, FK_BX, true);
+  rc |= xfer(S.by, f->by, FK_BY, true);
+  rc |= xfer(S.bz, f->bz, FK_BZ, true);
+  if (f->visc3) rc |= xfer(S.visc3, f->visc3, FK_NODE, true);
+  if (f->eta) rc |= xfer(S.eta, f->eta, FK_CELL, true);
+  if (f->visc_dep) rc |= xfer(S.visc_dep, f->visc_dep, FK_CELL, true);
+  if (f->ohmic_dep) rc |= xfer(S.ohmic_dep, f->ohmic_dep, FK_CELL, true);
+  if (rc) return fail(LARE_E_STATE, "upload failed: %s", g_err);
+  S.loaded = true;
+  return 0;
+}
+
+int api_download(lare_fields *f) {
+  if (!S.loaded) return fail(LARE_E_STATE, "lare_dev_download before lare_dev_upload");
+  if (!f) return fail(LARE_E_ARG, "lare_dev_download: NULL");
+  HIPCHK(hipStreamSynchronize(S.stream));
+  int rc = 0;
+  if (f->rho) rc |= xfer(S.rho, f->rho, FK_CELL, false);
+  if (f->energy) rc |= xfer(S.energy, f->energy, FK_CELL, false);
+  if (f->vx) rc |= xfer(S.vx, f->vx, FK_NODE, false);
+  if (f->vy) rc |= xfer(S.vy, f->vy, FK_NODE, false);
+  if (f->vz) rc |= xfer(S.vz, f->vz, FK_NODE, false);
+  if (f->bx) rc |= xfer(S.bx, f->bx, FK_BX, false);
+  if (f->by) rc |= xfer(S.by, f->by, FK_BY, false);
+  if (f->bz) rc |= xfer(S.bz, f->bz, FK_BZ, false);
+  if (f->eta) rc |= xfer(S.eta, f->eta, FK_CELL, false);
+  if (f->visc_dep) rc |= xfer(S.visc_dep, f->visc_dep, FK_CELL, false);
+  if (f->ohmic_dep) rc |= xfer(S.ohmic_dep, f->ohmic_dep, FK_CELL, false);
+  if (rc) return fail(LARE_E_STATE, "download failed: %s", g_err);
+  return 0;
+}
+
+int api_get_scalars(lare_scalars *o) {
+  if (!S.ready) return fail(LARE_E_STATE, "not initialised");
+  double acc[2];
+  int rc = read_scalars(8, 2, acc);
+  if (rc) return rc;
+  o->dt = S.dt;
+  o->dtr = S.dtr;
+  o->time = S.time;
+  o->total_visc_heating = acc[0];
+  o->total_ohmic_heating = acc[1];
+  o->n_s_stages = S.n_s_stages;
+  o->peak_substeps = S.peak_substeps;
+  o->last_substeps = S.last_substeps;
+  o->first_set_dt = S.first_set_dt;
+  o->step = S.step;
+  return 0;
+}
+
+int api_set_scalars(const lare_scalars *in) {
+  if (!S.ready) return fail(LARE_E_STATE, "not initialised");
+  S.dt = in->dt;
+  S.dtr = in->dtr;
+  S.time = in->time;
+  S.n_s_stages = in->n_s_stages;
+  S.peak_substeps = in->peak_substeps;
+  S.last_substeps = in->last_substeps;
+  S.first_set_dt = in->first_set_dt;
+  S.step = in->step;
+  double acc[2] = {in->total_visc_heating, in->total_ohmic_heating};
+  HIPCHK(hipMemcpy(S.dscal + 8, acc, sizeof(acc), hipMemcpyHostToDevice));
+  return 0;
+}
+
+#define REQUIRE_LOADED() \
+  if (!S.loaded) return fail(LARE_E_STATE, "%s before lare_dev_upload", __func__)
+
+int api_lagrangian_step(void) {
+  REQUIRE_LOADED();
+  return lagrangian_step();
+}
+
+int api_eulerian_remap(int32_t step) {
+  REQUIRE_LOADED();
+  return eulerian_remap(step, false);
+}
+
+int api_eta_calc(void) {
+  REQUIRE_LOADED();
+  return eta_calc();
+}
+
+int api_conduct_heat(void) {
+  REQUIRE_LOADED();
+  if (S.p.kappa_0 <= 0.0) return fail(LARE_E_ARG, "conduct_heat needs kappa_0 > 0");
+  int rc = calc_s_stages();
+  if (rc) return rc;
+  return heat_conduct_sts2(0);
+}
+
+int api_boundary_conditions(void) {
+  REQUIRE_LOADED();
+  return boundary_conditions();
+}
+
+int api_energy_account(double en[3], double *divb) {
+  REQUIRE_LOADED();
+  if (!en) return fail(LARE_E_ARG, "energy_account: NULL output");
+  dim3 gr = grid_from0(nx(), ny(), nz());
+  LAUNCH("energy", k_energy, gr, box_block(), S.g, S.rho, S.energy, S.vx, S.vy, S.vz, S.bx, S.by, S.bz, S.partial);
+  int rc = final_reduce(gr, 4, 8, 0, 0);
+  if (rc) return rc;
+  double v[4];
+  rc = read_scalars(0, 4, v);
+  if (rc) return rc;
+  en[0] = v[0];
+  en[1] = v[1];
+  en[2] = v[2];
+  if (divb) *divb = -v[3];
+  return 0;
+}
+
+int api_step(int32_t step, double *dt, double *time) {
+  REQUIRE_LOADED();
+  S.step = step;
+  int rc = lagrangian_step();
+  if (rc) return rc;
+  rc = eulerian_remap(step, false);
+  if (rc) return rc;
+  rc = eta_calc();
+  if (rc) return rc;
+  if (!(S.dt == S.dt) || (S.dt <= 0.0 && !S.p.restart))
+    return fail(LARE_F_NONFINITE, "step %d: invalid dt %g", step, S.dt);
+  if (dt) *dt = S.dt;
+  if (time) *time = S.time;
+  return 0;
+}
+
+int api_profile(int32_t enable) {
+  if (!S.ready) return fail(LARE_E_STATE, "not initialised");
+  prof_flush();
+  if (enable && !S.prof) S.acc.clear();
+  S.prof = enable != 0;
+  return 0;
+}
+
+int api_kernel_stats(char *names, int32_t name_len, int32_t *counts, double *total_ms, int32_t cap) {
+  if (!S.ready) return fail(LARE_E_STATE, "not initialised");
+  prof_flush();
+  int q = 0;
+  for (auto &a : S.acc) {
+    if (q >= cap) break;
+    if (names && name_len > 0) {
+      strncpy(names + (size_t)q * name_len, a.name.c_str(), (size_t)name_len - 1);
+      names[(size_t)q * name_len + name_len - 1] = 0;
+    }
+    if (counts) counts[q] = a.count;
+    if (total_ms) total_ms[q] = a.ms;
+    ++q;
+  }
+  return q;
+}
+
+int api_synchronize(void) {
+  if (!S.stream) return 0;
+  HIPCHK(hipStreamSynchronize(S.stream));
+  return 0;
+}
+
+int api_finalize(void) {
+  free_all();
+  if (S.stream) {
+    (void)hipStreamDestroy(S.stream);
+    S.stream = nullptr;
+  }
+  return 0;
+}
+
+
+// ============================================================================
+// learned conduction surrogate
+// ============================================================================
+// Runtime of the learned conduction surrogate (kernels in surrogate.hpp).
+//
+// Data flow:
+//   collect: features of the current state -> random interior cells -> rows
+//            of 27 x 12 raw channels; one plain RKL2 super-step on the same
+//            state gives the target dT of each row; the state is restored.
+//   fit_norm: channel mean / std and the target scale from the rows; the
+//            stored rows are standardised in place.
+//   train:   Adam on mini-batches drawn from the rows; forward and backward
+//            GEMMs in BF16 MFMA with FP32 accumulation.
+//   conduct: features -> MFMA inference over the interior -> E1; residual
+//            r = |E1 - E0 - dt/2 (L(E0) + L(E1))| / |E0| (L2 norms over the
+//            interior, L = heat-flux divergence / rho of conduct.f90:152-291);
+//            r <= tol accepts E1, otherwise the super-step is redone by RKL2.
+
+
+
+struct Sur {
+  float *w = nullptr, *gr = nullptr, *m = nullptr, *v = nullptr;  // SUR_NPARAM each
+  unsigned short *w1t = nullptr, *w2t = nullptr;                   // packed BF16 for inference
+  bool have_params = false, have_norm = false;
+  SurNorm nm;
+  int adam_t = 0;
+  float *feat = nullptr;      // ntot * SUR_CH
+  real *E1 = nullptr, *trap = nullptr, *E0save = nullptr;
+  // dataset
+  float *X = nullptr, *y = nullptr;
+  int *cells = nullptr;
+  int rows = 0, cap = 0;
+  bool standardised = false;
+  // mini-batch work arrays (capacity bcap columns)
+  int bcap = 0;
+  float *Xb = nullptr, *yb = nullptr, *Z1 = nullptr, *H1 = nullptr, *Z2 = nullptr, *H2 = nullptr, *yo = nullptr,
+        *dy = nullptr, *dZ2 = nullptr, *dH1 = nullptr, *dZ1 = nullptr, *lossacc = nullptr, *losshist = nullptr;
+  int losscap = 0;
+  // use inside lagrangian_step
+  int enabled = 0;
+  double tol = 1.0e-4;
+  int accepted = 0, rejected = 0;
+  double last_res = 0.0;
+};
+Sur SU;
+
+template <class T>
+void sfree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+void sur_free_batch() {
+  sfree(SU.Xb); sfree(SU.yb); sfree(SU.Z1); sfree(SU.H1); sfree(SU.Z2); sfree(SU.H2); sfree(SU.yo);
+  sfree(SU.dy); sfree(SU.dZ2); sfree(SU.dH1); sfree(SU.dZ1); sfree(SU.lossacc);
+  SU.bcap = 0;
+}
+
+void sur_free() {
+  sfree(SU.w); sfree(SU.gr); sfree(SU.m); sfree(SU.v); sfree(SU.w1t); sfree(SU.w2t);
+  sfree(SU.feat); sfree(SU.E1); sfree(SU.trap); sfree(SU.E0save);
+  sfree(SU.X); sfree(SU.y); sfree(SU.cells); sfree(SU.losshist);
+  sur_free_batch();
+  const int en = SU.enabled;
+  const double tol = SU.tol;
+  SU = Sur();
+  SU.enabled = en;  // the mode survives re-initialisation; parameters do not
+  SU.tol = tol;
+}
+
+template <class T>
+int salloc(T **p, size_t count) {
+  hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
+  if (e != hipSuccess) return fail(LARE_E_NOMEM, "hipMalloc of %zu bytes failed: %s", sizeof(T) * count,
+                                   hipGetErrorString(e));
+  return 0;
+}
+
+int sur_alloc_params() {
+  if (SU.w) return 0;
+  int rc = salloc(&SU.w, SUR_NPARAM);
+  if (!rc) rc = salloc(&SU.gr, SUR_NPARAM);
+  if (!rc) rc = salloc(&SU.m, SUR_NPARAM);
+  if (!rc) rc = salloc(&SU.v, SUR_NPARAM);
+  if (!rc) rc = salloc(&SU.w1t, SUR_H * SUR_KP);
+  if (!rc) rc = salloc(&SU.w2t, SUR_H * SUR_HP);
+  if (rc) return rc;
+  HIPCHK(hipMemsetAsync(SU.m, 0, sizeof(float) * SUR_NPARAM, S.stream));
+  HIPCHK(hipMemsetAsync(SU.v, 0, sizeof(float) * SUR_NPARAM, S.stream));
+  return 0;
+}
+
+int sur_alloc_fields() {
+  if (SU.feat) return 0;
+  int rc = salloc(&SU.feat, S.ntot * SUR_CH);
+  if (!rc) rc = salloc(&SU.E1, S.ntot);
+  if (!rc) rc = salloc(&SU.trap, S.ntot);
+  if (!rc) rc = salloc(&SU.E0save, S.ntot);
+  if (rc) return rc;
+  // cells never written by the feature kernel are never read by it either;
+  // zero them once so no stale bits reach a gather
+  HIPCHK(hipMemsetAsync(SU.feat, 0, sizeof(float) * S.ntot * SUR_CH, S.stream));
+  return 0;
+}
+
+int sur_ensure_batch(int cols) {
+  if (cols <= SU.bcap) return 0;
+  sur_free_batch();
+  const size_t c = (size_t)cols;
+  int rc = salloc(&SU.Xb, c * SUR_K);
+  if (!rc) rc = salloc(&SU.yb, c);
+  if (!rc) rc = salloc(&SU.Z1, c * SUR_H);
+  if (!rc) rc = salloc(&SU.H1, c * SUR_H);
+  if (!rc) rc = salloc(&SU.Z2, c * SUR_H);
+  if (!rc) rc = salloc(&SU.H2, c * SUR_H);
+  if (!rc) rc = salloc(&SU.yo, c);
+  if (!rc) rc = salloc(&SU.dy, c);
+  if (!rc) rc = salloc(&SU.dZ2, c * SUR_H);
+  if (!rc) rc = salloc(&SU.dH1, c * SUR_H);
+  if (!rc) rc = salloc(&SU.dZ1, c * SUR_H);
+  if (!rc) rc = salloc(&SU.lossacc, (c + 255) / 256);
+  if (rc) return rc;
+  SU.bcap = cols;
+  return 0;
+}
+
+unsigned blocks_for(size_t n, unsigned bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+int gemm(int M, int N, int K, const float *A, int sam, int sak, const float *B, int sbk, int sbn, float *C, int scm,
+         int scn) {
+  LAUNCH("sur_gemm", k_gemm_bf16, dim3((unsigned)(N / 16), (unsigned)(M / 16)), dim3(64), M, N, K, A, sam, sak, B, sbk,
+         sbn, C, scm, scn);
+  return 0;
+}
+
+// Forward pass on SU.Xb (nb columns, a multiple of 16; the first nreal are
+// real rows).  With tgt: loss partials and dy as well.
+int sur_forward(int nb, int nreal, const float *tgt) {
+  int rc = gemm(SUR_H, nb, SUR_K, SU.w + SUR_OFF_W1, 1, SUR_H, SU.Xb, 1, SUR_K, SU.Z1, nb, 1);
+  if (rc) return rc;
+  LAUNCH("sur_bias_gelu", k_bias_gelu, dim3(blocks_for((size_t)SUR_H * nb)), dim3(256), SU.Z1, SU.H1,
+         SU.w + SUR_OFF_B1, SUR_H, nb);
+  rc = gemm(SUR_H, nb, SUR_H, SU.w + SUR_OFF_W2, 1, SUR_H, SU.H1, nb, 1, SU.Z2, nb, 1);
+  if (rc) return rc;
+  LAUNCH("sur_bias_gelu", k_bias_gelu, dim3(blocks_for((size_t)SUR_H * nb)), dim3(256), SU.Z2, SU.H2,
+         SU.w + SUR_OFF_B2, SUR_H, nb);
+  LAUNCH("sur_out", k_out_layer, dim3(blocks_for(nb)), dim3(256), SU.H2, SU.w, tgt, nb, nreal, SU.yo, SU.dy,
+         SU.lossacc);
+  return 0;
+}
+
+// Backward pass after sur_forward with a target: gradient into SU.gr.
+int sur_backward(int nb) {
+  LAUNCH("sur_rowdot", k_rowdot, dim3(SUR_H), dim3(256), SU.H2, SU.dy, nb, SU.gr + SUR_OFF_W3);
+  LAUNCH("sur_rowdot", k_rowdot, dim3(1), dim3(256), SU.dy, (const float *)nullptr, nb, SU.gr + SUR_OFF_B3);
+  LAUNCH("sur_back_out", k_back_out, dim3(blocks_for((size_t)SUR_H * nb)), dim3(256), SU.Z2, SU.w, SU.dy, nb,
+         SU.dZ2);
+  LAUNCH("sur_rowdot", k_rowdot, dim3(SUR_H), dim3(256), SU.dZ2, (const float *)nullptr, nb, SU.gr + SUR_OFF_B2);
+  // dW2[k][f] = sum_b H1[k][b] dZ2[f][b]
+  int rc = gemm(SUR_H, SUR_H, nb, SU.H1, nb, 1, SU.dZ2, 1, nb, SU.gr + SUR_OFF_W2, SUR_H, 1);
+  if (rc) return rc;
+  // dH1[k][b] = sum_f W2[k][f] dZ2[f][b]
+  rc = gemm(SUR_H, nb, SUR_H, SU.w + SUR_OFF_W2, SUR_H, 1, SU.dZ2, nb, 1, SU.dH1, nb, 1);
+  if (rc) return rc;
+  LAUNCH("sur_gelu_back", k_gelu_back, dim3(blocks_for((size_t)SUR_H * nb)), dim3(256), SU.Z1, SU.dH1,
+         SUR_H * nb, SU.dZ1);
+  LAUNCH("sur_rowdot", k_rowdot, dim3(SUR_H), dim3(256), SU.dZ1, (const float *)nullptr, nb, SU.gr + SUR_OFF_B1);
+  // dW1[k][f] = sum_b X[b][k] dZ1[f][b]
+  return gemm(SUR_K, SUR_H, nb, SU.Xb, 1, SUR_K, SU.dZ1, 1, nb, SU.gr + SUR_OFF_W1, SUR_H, 1);
+}
+
+int sur_enabled() { return SU.enabled; }
+
+int sur_pack() {
+  LAUNCH("sur_pack", k_sur_pack, dim3(blocks_for((size_t)SUR_H * SUR_KP)), dim3(256), SU.w, SU.w1t, SU.w2t);
+  return 0;
+}
+
+int sur_features(int normalise) {
+  int rc = sur_alloc_fields();
+  if (rc) return rc;
+  const double logdt = log(S.dt > 0.0 ? S.dt : 1.0e-300);
+  LAUNCH_BOX("sur_features", k_sur_features, mkbox(0, nx() + 1, 0, ny() + 1, 0, nz() + 1), S.energy, S.rho, S.bx,
+             S.by, S.bz, SU.feat, SU.nm, S.tfac, logdt, normalise);
+  return 0;
+}
+
+// One surrogate super-step of length S.dt on S.energy (stages and Larsen
+// factor from calc_s_stages), checked and, if rejected, redone by RKL2.
+int sur_conduct_step(int delta_mode, double *res_out, int *acc_out) {
+  if (!SU.have_params || !SU.have_norm) return fail(LARE_E_STATE, "surrogate: no trained parameters loaded");
+  int rc = calc_s_stages();
+  if (rc) return rc;
+  rc = sur_features(1);
+  if (rc) return rc;
+  const int gpr = (nx() + 15) / 16;
+  const int ngroups = gpr * ny() * nz();
+  const unsigned nblk = (unsigned)((ngroups + SUR_INFER_WAVES - 1) / SUR_INFER_WAVES);
+  LAUNCH("sur_infer", k_sur_infer, dim3(nblk < 2048u ? nblk : 2048u), dim3(64 * SUR_INFER_WAVES), S.g, SU.feat,
+         SU.w1t, SU.w2t, SU.w, S.energy, SU.E1, S.tfac, SU.nm.y_scale, ngroups);
+  // trapezoid dt/2 (L(E0) + L(E1)): stage kernels with all history weights zero
+  CondArgs ca = {S.p.kappa_0, S.tfac, S.dt};
+  Box in = interior();
+  LAUNCH_BOX("rkl2_stage1", k_rkl2_stage1, in, S.energy, S.rho, S.bx, S.by, S.bz, S.larsen, S.Lc, S.Yb[0], ca, 0.0);
+  LAUNCH_BOX("rkl2_stage", k_rkl2_stage, in, SU.E1, SU.E1, S.energy, S.rho, S.bx, S.by, S.bz, S.larsen, S.Lc,
+             SU.trap, ca, TMODE_WRAP, 0.0, 0.0, 0.0, 0.5, 0.5);
+  dim3 gr = box_grid(interior());
+  LAUNCH("sur_residual", k_sur_residual, gr, box_block(), S.g, S.energy, SU.E1, SU.trap, S.partial);
+  rc = final_reduce(gr, 2, 0, 0, 0);
+  if (rc) return rc;
+  double v[2];
+  rc = read_scalars(0, 2, v);
+  if (rc) return rc;
+  const double r = v[1] > 0.0 ? sqrt(v[0] / v[1]) : 0.0;
+  SU.last_res = r;
+  const int ok = (r == r) && r <= SU.tol;
+  if (res_out) *res_out = r;
+  if (acc_out) *acc_out = ok;
+  if (ok) {
+    ++SU.accepted;
+    LAUNCH_BOX("rkl2_final", k_rkl2_final, cell_box(), SU.E1, S.energy, delta_mode);
+    return 0;
+  }
+  ++SU.rejected;
+  return heat_conduct_sts2(delta_mode);
+}
+
+// rows of standardised inputs (SUR_K floats each) from the host into SU.Xb,
+// zero-padded to a multiple of 16 columns
+int sur_stage_rows(const float *X, const float *t, int rows, int *nb_out) {
+  const int nb = (rows + 15) / 16 * 16;
+  int rc = sur_ensure_batch(nb);
+  if (rc) return rc;
+  HIPCHK(hipMemsetAsync(SU.Xb, 0, sizeof(float) * (size_t)nb * SUR_K, S.stream));
+  HIPCHK(hipMemcpyAsync(SU.Xb, X, sizeof(float) * (size_t)rows * SUR_K, hipMemcpyHostToDevice, S.stream));
+  if (t) {
+    HIPCHK(hipMemsetAsync(SU.yb, 0, sizeof(float) * nb, S.stream));
+    HIPCHK(hipMemcpyAsync(SU.yb, t, sizeof(float) * rows, hipMemcpyHostToDevice, S.stream));
+  }
+  *nb_out = nb;
+  return 0;
+}
+
+
+int api_sur_set_params(const float *params, int32_t n) {
+  if (!S.ready) return fail(LARE_E_STATE, "not initialised");
+  if (!params || n != SUR_NPARAM) return fail(LARE_E_ARG, "surrogate parameters: need %d floats, got %d", SUR_NPARAM, n);
+  int rc = sur_alloc_params();
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(SU.w, params, sizeof(float) * SUR_NPARAM, hipMemcpyHostToDevice));
+  HIPCHK(hipMemsetAsync(SU.m, 0, sizeof(float) * SUR_NPARAM, S.stream));
+  HIPCHK(hipMemsetAsync(SU.v, 0, sizeof(float) * SUR_NPARAM, S.stream));
+  SU.adam_t = 0;
+  SU.have_params = true;
+  return sur_pack();
+}
+
+int api_sur_get_params(float *params, int32_t n) {
+  if (!SU.have_params) return fail(LARE_E_STATE, "surrogate: no parameters");
+  if (!params || n != SUR_NPARAM) return fail(LARE_E_ARG, "surrogate parameters: need %d floats, got %d", SUR_NPARAM, n);
+  HIPCHK(hipStreamSynchronize(S.stream));
+  HIPCHK(hipMemcpy(params, SU.w, sizeof(float) * SUR_NPARAM, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int api_sur_set_norm(const float *mu, const float *sd, float y_scale) {
+  if (!mu || !sd || !(y_scale > 0.0f)) return fail(LARE_E_ARG, "surrogate norm: bad arguments");
+  for (int q = 0; q < SUR_CH; ++q) {
+    SU.nm.mu[q] = q < SUR_NCH ? mu[q] : 0.0f;
+    SU.nm.sd[q] = q < SUR_NCH ? sd[q] : 1.0f;
+    if (!(SU.nm.sd[q] > 0.0f)) return fail(LARE_E_ARG, "surrogate norm: std of channel %d is not positive", q);
+  }
+  SU.nm.y_scale = y_scale;
+  SU.have_norm = true;
+  return 0;
+}
+
+int api_sur_get_norm(float *mu, float *sd, float *y_scale) {
+  if (!SU.have_norm) return fail(LARE_E_STATE, "surrogate: no normalisation");
+  for (int q = 0; q < SUR_NCH; ++q) {
+    if (mu) mu[q] = SU.nm.mu[q];
+    if (sd) sd[q] = SU.nm.sd[q];
+  }
+  if (y_scale) *y_scale = SU.nm.y_scale;
+  return 0;
+}
+
+int api_sur_collect(int32_t count, uint64_t seed, int32_t *total) {
+  REQUIRE_LOADED();
+  if (count <= 0) return fail(LARE_E_ARG, "surrogate collect: count %d", count);
+  if (SU.standardised) return fail(LARE_E_STATE, "surrogate collect: rows already standardised");
+  if (S.p.kappa_0 <= 0.0 || !(S.dt > 0.0)) return fail(LARE_E_ARG, "surrogate collect needs kappa_0 > 0 and dt > 0");
+  if (SU.rows + count > SU.cap) {  // grow the row store
+    const int ncap = (SU.rows + count) * 2;
+    float *X = nullptr, *y = nullptr;
+    int *cells = nullptr;
+    int rc = salloc(&X, (size_t)ncap * SUR_K);
+    if (!rc) rc = salloc(&y, (size_t)ncap);
+    if (!rc) rc = salloc(&cells, (size_t)ncap);
+    if (rc) return rc;
+    if (SU.rows) {
+      HIPCHK(hipMemcpyAsync(X, SU.X, sizeof(float) * SU.rows * SUR_K, hipMemcpyDeviceToDevice, S.stream));
+      HIPCHK(hipMemcpyAsync(y, SU.y, sizeof(float) * SU.rows, hipMemcpyDeviceToDevice, S.stream));
+    }
+    HIPCHK(hipStreamSynchronize(S.stream));
+    sfree(SU.X); sfree(SU.y); sfree(SU.cells);
+    SU.X = X; SU.y = y; SU.cells = cells;
+    SU.cap = ncap;
+  }
+  int rc = calc_s_stages();
+  if (rc) return rc;
+  rc = sur_features(0);
+  if (rc) return rc;
+  int *cells = SU.cells + SU.rows;
+  LAUNCH("sur_sample", k_sur_sample_cells, dim3(blocks_for(count)), dim3(256), S.g, (unsigned long long)seed, count,
+         cells);
+  LAUNCH("sur_gather", k_sur_gather, dim3(blocks_for((size_t)count * SUR_K)), dim3(256), S.g, SU.feat, cells, count,
+         SU.X + (size_t)SU.rows * SUR_K);
+  HIPCHK(hipMemcpyAsync(SU.E0save, S.energy, sizeof(real) * S.ntot, hipMemcpyDeviceToDevice, S.stream));
+  rc = heat_conduct_sts2(0);
+  if (rc) return rc;
+  LAUNCH("sur_target", k_sur_target, dim3(blocks_for(count)), dim3(256), SU.E0save, S.energy, cells, count, S.tfac,
+         SU.y + SU.rows);
+  HIPCHK(hipMemcpyAsync(S.energy, SU.E0save, sizeof(real) * S.ntot, hipMemcpyDeviceToDevice, S.stream));
+  HIPCHK(hipStreamSynchronize(S.stream));
+  SU.rows += count;
+  if (total) *total = SU.rows;
+  return 0;
+}
+
+int api_sur_rows(float *X, float *y, int32_t rows) {
+  if (rows < 0 || rows > SU.rows) return fail(LARE_E_ARG, "surrogate rows: %d requested, %d held", rows, SU.rows);
+  HIPCHK(hipStreamSynchronize(S.stream));
+  if (X && rows) HIPCHK(hipMemcpy(X, SU.X, sizeof(float) * (size_t)rows * SUR_K, hipMemcpyDeviceToHost));
+  if (y && rows) HIPCHK(hipMemcpy(y, SU.y, sizeof(float) * (size_t)rows, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int api_sur_fit_norm(void) {
+  if (SU.rows < 2) return fail(LARE_E_STATE, "surrogate fit_norm: %d rows collected", SU.rows);
+  if (SU.standardised) return fail(LARE_E_STATE, "surrogate fit_norm: rows already standardised");
+  std::vector<float> X((size_t)SU.rows * SUR_K), y(SU.rows);
+  HIPCHK(hipStreamSynchronize(S.stream));
+  HIPCHK(hipMemcpy(X.data(), SU.X, sizeof(float) * X.size(), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(y.data(), SU.y, sizeof(float) * y.size(), hipMemcpyDeviceToHost));
+  // channel statistics of the centre cell (neighbour 13) of every row
+  for (int ch = 0; ch < SUR_CH; ++ch) {
+    double s = 0.0, s2 = 0.0;
+    for (int r = 0; r < SU.rows; ++r) {
+      const double x = X[(size_t)r * SUR_K + 13 * SUR_CH + ch];
+      s += x;
+      s2 += x * x;
+    }
+    const double mu = s / SU.rows;
+    double var = s2 / SU.rows - mu * mu;
+    const double sd = var > 1.0e-12 * (1.0 + mu * mu) ? sqrt(var) : 1.0;
+    SU.nm.mu[ch] = ch < SUR_NCH ? (float)mu : 0.0f;
+    SU.nm.sd[ch] = ch < SUR_NCH ? (float)sd : 1.0f;
+  }
+  double sy2 = 0.0;
+  for (int r = 0; r < SU.rows; ++r) sy2 += (double)y[r] * y[r];
+  const double ys = sqrt(sy2 / SU.rows);
+  SU.nm.y_scale = ys > 0.0 ? (float)ys : 1.0f;
+  SU.have_norm = true;
+  LAUNCH("sur_standardise", k_sur_standardise, dim3(blocks_for((size_t)SU.rows * SUR_K)), dim3(256), SU.X, SU.y,
+         SU.rows, SU.nm);
+  SU.standardised = true;
+  return 0;
+}
+
+int api_sur_train(int32_t steps, int32_t batch, float lr, uint64_t seed, float *loss_out) {
+  if (!S.ready) return fail(LARE_E_STATE, "not initialised");
+  if (!SU.have_params) return fail(LARE_E_STATE, "surrogate train: no initial parameters");
+  if (!SU.standardised) return fail(LARE_E_STATE, "surrogate train: call fit_norm after collecting rows");
+  if (steps <= 0 || batch < 16 || batch % 16 != 0) return fail(LARE_E_ARG, "surrogate train: steps %d batch %d", steps, batch);
+  int rc = sur_ensure_batch(batch);
+  if (rc) return rc;
+  if (steps > SU.losscap) {
+    sfree(SU.losshist);
+    rc = salloc(&SU.losshist, (size_t)steps);
+    if (rc) return rc;
+    SU.losscap = steps;
+  }
+  const float b1 = 0.9f, b2 = 0.999f;
+  for (int s = 0; s < steps; ++s) {
+    const unsigned long long bseed = mix64(seed * 0x9E3779B97F4A7C15ull + (unsigned long long)s);
+    LAUNCH("sur_batch", k_sur_batch, dim3(blocks_for((size_t)batch * SUR_K)), dim3(256), SU.X, SU.y, SU.rows, bseed,
+           batch, SU.Xb, SU.yb);
+    rc = sur_forward(batch, batch, SU.yb);
+    if (rc) return rc;
+    LAUNCH("sur_loss", k_sum_into, dim3(1), dim3(64), SU.lossacc, (int)blocks_for(batch), 1.0f / (float)batch,
+           SU.losshist + s);
+    rc = sur_backward(batch);
+    if (rc) return rc;
+    SU.adam_t += 1;
+    const float c1 = 1.0f - powf(b1, (float)SU.adam_t), c2 = 1.0f - powf(b2, (float)SU.adam_t);
+    LAUNCH("sur_adam", k_adam, dim3(blocks_for(SUR_NPARAM)), dim3(256), SU.w, SU.gr, SU.m, SU.v, SUR_NPARAM, lr, b1,
+           b2, 1.0e-8f, c1, c2);
+  }
+  rc = sur_pack();
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(S.stream));
+  if (loss_out) HIPCHK(hipMemcpy(loss_out, SU.losshist, sizeof(float) * steps, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int api_sur_forward(const float *X, int32_t rows, float *y) {
+  if (!S.ready) return fail(LARE_E_STATE, "not initialised");
+  if (!SU.have_params) return fail(LARE_E_STATE, "surrogate forward: no parameters");
+  if (!X || !y || rows <= 0) return fail(LARE_E_ARG, "surrogate forward: bad arguments");
+  int nb;
+  int rc = sur_stage_rows(X, nullptr, rows, &nb);
+  if (rc) return rc;
+  rc = sur_forward(nb, rows, nullptr);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(S.stream));
+  HIPCHK(hipMemcpy(y, SU.yo, sizeof(float) * rows, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int api_sur_grad(const float *X, const float *t, int32_t rows, float *grad, float *loss) {
+  if (!S.ready) return fail(LARE_E_STATE, "not initialised");
+  if (!SU.have_params) return fail(LARE_E_STATE, "surrogate grad: no parameters");
+  if (!X || !t || !grad || rows <= 0) return fail(LARE_E_ARG, "surrogate grad: bad arguments");
+  int nb;
+  int rc = sur_stage_rows(X, t, rows, &nb);
+  if (rc) return rc;
+  rc = sur_forward(nb, rows, SU.yb);
+  if (rc) return rc;
+  rc = sur_backward(nb);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(S.stream));
+  HIPCHK(hipMemcpy(grad, SU.gr, sizeof(float) * SUR_NPARAM, hipMemcpyDeviceToHost));
+  if (loss) {
+    const int nblk = (int)blocks_for(nb);
+    std::vector<float> part(nblk);
+    HIPCHK(hipMemcpy(part.data(), SU.lossacc, sizeof(float) * nblk, hipMemcpyDeviceToHost));
+    double s = 0.0;
+    for (float p : part) s += p;
+    *loss = (float)(s / rows);
+  }
+  return 0;
+}
+
+int api_sur_enable(int32_t on, double tol) {
+  if (!(tol > 0.0)) return fail(LARE_E_ARG, "surrogate tolerance must be positive");
+  if (on && (!SU.have_params || !SU.have_norm)) return fail(LARE_E_STATE, "surrogate enable: no trained parameters");
+  SU.enabled = on != 0;
+  SU.tol = tol;
+  return 0;
+}
+
+int api_sur_conduct(double *residual, int32_t *accepted) {
+  REQUIRE_LOADED();
+  if (S.p.kappa_0 <= 0.0) return fail(LARE_E_ARG, "surrogate conduct needs kappa_0 > 0");
+  int acc = 0;
+  int rc = sur_conduct_step(0, residual, &acc);
+  if (accepted) *accepted = acc;
+  return rc;
+}
+
+int api_sur_stats(int32_t *accepted, int32_t *rejected, double *last_residual) {
+  if (accepted) *accepted = SU.accepted;
+  if (rejected) *rejected = SU.rejected;
+  if (last_residual) *last_residual = SU.last_res;
+  return 0;
+}
+

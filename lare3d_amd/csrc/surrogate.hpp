@@ -19,8 +19,6 @@
 //   D (16 x 16):                 lane l holds D[4 (l / 16) + r][l % 16]
 // D has the layout of B, so a layer computed as (W^T)(H^T) feeds the next
 // layer's B operand from registers, with no transpose through LDS.
-#pragma once
-#include "common.hpp"
 
 #define SUR_CH 12            // channels stored per cell (10 used)
 #define SUR_NCH 10
@@ -70,8 +68,8 @@ struct SurNorm {
 
 // Cell channels over cells 0..n+1 (box b).  Temperature T = tfac * e
 // (conduct.f90:72-73 with xi_n = 0 and zero ionisation energy).
-__global__ __launch_bounds__(BX * BY) void k_sur_features(Geo g, Box b, const double *__restrict__ energy,
-                                                          const double *__restrict__ rho,
+__global__ __launch_bounds__(BX * BY) void k_sur_features(Geo g, Box b, const real *__restrict__ energy,
+                                                          const real *__restrict__ rho,
                                                           const double *__restrict__ bx,
                                                           const double *__restrict__ by,
                                                           const double *__restrict__ bz, float *__restrict__ feat,
@@ -130,7 +128,7 @@ __global__ void k_sur_gather(Geo g, const float *__restrict__ feat, const int *_
 }
 
 // target dT of the sampled cells after one plain super-step
-__global__ void k_sur_target(const double *__restrict__ E0, const double *__restrict__ E1,
+__global__ void k_sur_target(const real *__restrict__ E0, const real *__restrict__ E1,
                              const int *__restrict__ cells, int rows, double tfac, float *__restrict__ y) {
   const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (r >= rows) return;
@@ -300,8 +298,8 @@ __global__ void k_sur_pack(const float *__restrict__ w, unsigned short *__restri
 // wave-group.  Features must be valid on cells 0..n+1.
 __global__ __launch_bounds__(64 * SUR_INFER_WAVES) void k_sur_infer(
     Geo g, const float *__restrict__ feat, const unsigned short *__restrict__ w1t,
-    const unsigned short *__restrict__ w2t, const float *__restrict__ w, const double *__restrict__ E0,
-    double *__restrict__ E1, double tfac, float y_scale, int ngroups) {
+    const unsigned short *__restrict__ w2t, const float *__restrict__ w, const real *__restrict__ E0,
+    real *__restrict__ E1, double tfac, float y_scale, int ngroups) {
   __shared__ __attribute__((aligned(16))) unsigned short sW1[SUR_H * SUR_KP];
   __shared__ __attribute__((aligned(16))) unsigned short sW2[SUR_H * SUR_HP];
   __shared__ float sB1[SUR_H], sB2[SUR_H], sW3[SUR_H];
@@ -379,9 +377,9 @@ __global__ __launch_bounds__(64 * SUR_INFER_WAVES) void k_sur_infer(
 
 // Trapezoidal residual of a predicted super-step over the interior:
 // partial sums of (E1 - E0 - trap)^2 and E0^2, with trap = dt/2 (L(E0) + L(E1)).
-__global__ __launch_bounds__(BX * BY) void k_sur_residual(Geo g, const double *__restrict__ E0,
-                                                          const double *__restrict__ E1,
-                                                          const double *__restrict__ trap,
+__global__ __launch_bounds__(BX * BY) void k_sur_residual(Geo g, const real *__restrict__ E0,
+                                                          const real *__restrict__ E1,
+                                                          const real *__restrict__ trap,
                                                           double *__restrict__ partial) {
   const int i = 1 + (int)(blockIdx.x * BX + threadIdx.x);
   const int j = 1 + (int)(blockIdx.y * BY + threadIdx.y);
