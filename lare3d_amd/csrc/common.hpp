@@ -94,3 +94,28 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], const int *o
     }
   }
 }
+
+// Columns of a z-marching launch: thread (i, j) of tile (blockIdx.x,
+// blockIdx.y) walks planes k0 .. k1 of its chunk (ZCHUNK planes per block).
+#define ZCHUNK 8
+#define ZCOLUMN(b, i, j, k0, k1)                                      \
+  const int i = (b).x0 + (int)(blockIdx.x * BX + threadIdx.x);         \
+  const int j = (b).y0 + (int)(blockIdx.y * BY + threadIdx.y);         \
+  const int k0 = (b).z0 + (int)blockIdx.z * ZCHUNK;                    \
+  const int k1 = k0 + ZCHUNK - 1 < (b).z1 ? k0 + ZCHUNK - 1 : (b).z1;  \
+  if (i > (b).x1 || j > (b).y1) return;
+
+// A field at the four in-plane points (i, j), (i-1, j), (i, j-1), (i-1, j-1).
+struct Quad {
+  real p0, px, py, pxy;
+};
+__device__ __forceinline__ Quad quad_back(const real *__restrict__ u, int c, int Y) {
+  Quad q = {u[c], u[c - 1], u[c - Y], u[c - 1 - Y]};
+  return q;
+}
+
+// A field at the four in-plane points (i, j), (i+1, j), (i, j+1), (i+1, j+1).
+__device__ __forceinline__ Quad quad_fwd(const real *__restrict__ u, int c, int Y) {
+  Quad q = {u[c], u[c + 1], u[c + Y], u[c + 1 + Y]};
+  return q;
+}

@@ -250,32 +250,59 @@ __global__ __launch_bounds__(BX * BY) void k_damp_apply(Geo g, Box b, real *__re
   vz[c] = vz[c] + dt * qz[c];
 }
 
-// vertex current of lagran.F90:1479-1494 (also :358-373, :1424-1439)
-__device__ __forceinline__ void vertex_current(const Geo &g, const double *__restrict__ bx,
-                                               const double *__restrict__ by, const double *__restrict__ bz, int i,
-                                               int j, int k, double &jx, double &jy, double &jz) {
-  const int c = g.idx(i, j, k);
-  const int X = 1, Y = g.sy, Z = g.sz;
-  const double dxc = g.dxc[i], dyc = g.dyc[j], dzc = g.dzc[k];
-  const double jx1 = (bz[c + Y] - bz[c]) / dyc - (by[c + Z] - by[c]) / dzc;
-  const double jx2 = (bz[c + X + Y] - bz[c + X]) / dyc - (by[c + X + Z] - by[c + X]) / dzc;
-  const double jy1 = (bx[c + Z] - bx[c]) / dzc - (bz[c + X] - bz[c]) / dxc;
-  const double jy2 = (bx[c + Y + Z] - bx[c + Y]) / dzc - (bz[c + X + Y] - bz[c + Y]) / dxc;
-  const double jz1 = (by[c + X] - by[c]) / dxc - (bx[c + Y] - bx[c]) / dyc;
-  const double jz2 = (by[c + X + Z] - by[c + Z]) / dxc - (bx[c + Y + Z] - bx[c + Z]) / dyc;
+// Face B of one plane needed by the vertex current: bx at (i, j), (i, j+1)
+// and by at (i, j), (i+1, j).
+struct BPlane {
+  double bx0, bxy, by0, byx;
+};
+__device__ __forceinline__ BPlane bplane(const double *__restrict__ bx, const double *__restrict__ by, int c, int Y) {
+  BPlane p = {bx[c], bx[c + Y], by[c], by[c + 1]};
+  return p;
+}
+
+// vertex current of lagran.F90:1479-1494 (also :358-373, :1424-1439) from
+// bx, by of planes k (lo) and k+1 (hi) and bz at (i,j), (i+1,j), (i,j+1),
+// (i+1,j+1) of plane k
+__device__ __forceinline__ void vertex_current_vals(double dxc, double dyc, double dzc, const BPlane &lo,
+                                                    const BPlane &hi, double bz0, double bzx, double bzy, double bzxy,
+                                                    double &jx, double &jy, double &jz) {
+  const double jx1 = (bzy - bz0) / dyc - (hi.by0 - lo.by0) / dzc;
+  const double jx2 = (bzxy - bzx) / dyc - (hi.byx - lo.byx) / dzc;
+  const double jy1 = (hi.bx0 - lo.bx0) / dzc - (bzx - bz0) / dxc;
+  const double jy2 = (hi.bxy - lo.bxy) / dzc - (bzxy - bzy) / dxc;
+  const double jz1 = (lo.byx - lo.by0) / dxc - (lo.bxy - lo.bx0) / dyc;
+  const double jz2 = (hi.byx - hi.by0) / dxc - (hi.bxy - hi.bx0) / dyc;
   jx = (jx1 + jx2) * 0.5;
   jy = (jy1 + jy2) * 0.5;
   jz = (jz1 + jz2) * 0.5;
 }
 
-// eta_calc (lagran.F90:1176-1230) over -1..n+1
+// the same vertex current, read from memory at node (i, j, k)
+__device__ __forceinline__ void vertex_current(const Geo &g, const double *__restrict__ bx,
+                                               const double *__restrict__ by, const double *__restrict__ bz, int i,
+                                               int j, int k, double &jx, double &jy, double &jz) {
+  const int c = g.idx(i, j, k);
+  const int Y = g.sy;
+  const BPlane lo = bplane(bx, by, c, Y), hi = bplane(bx, by, c + g.sz, Y);
+  vertex_current_vals(g.dxc[i], g.dyc[j], g.dzc[k], lo, hi, bz[c], bz[c + 1], bz[c + Y], bz[c + 1 + Y], jx, jy, jz);
+}
+
+// eta_calc (lagran.F90:1176-1230) over -1..n+1, marching in z with the face
+// B of plane k+1 kept for the next plane
 __global__ __launch_bounds__(BX * BY) void k_eta(Geo g, Box b, const double *__restrict__ bx, const double *__restrict__ by,
                       const double *__restrict__ bz, real *__restrict__ eta, double eta_bg, double eta0,
                       double j_max) {
-  BOX_INDEX(b, i, j, k);
-  double jx, jy, jz;
-  vertex_current(g, bx, by, bz, i, j, k, jx, jy, jz);
-  eta[g.idx(i, j, k)] = sqrt(jx * jx + jy * jy + jz * jz) > j_max ? eta_bg + eta0 : eta_bg;
+  ZCOLUMN(b, i, j, k0, k1);
+  const int Y = g.sy, Z = g.sz;
+  BPlane lo = bplane(bx, by, g.idx(i, j, k0), Y);
+  for (int k = k0; k <= k1; ++k) {
+    const int c = g.idx(i, j, k);
+    const BPlane hi = bplane(bx, by, c + Z, Y);
+    double jx, jy, jz;
+    vertex_current_vals(g.dxc[i], g.dyc[j], g.dzc[k], lo, hi, bz[c], bz[c + 1], bz[c + Y], bz[c + 1 + Y], jx, jy, jz);
+    eta[c] = sqrt(jx * jx + jy * jy + jz * jz) > j_max ? eta_bg + eta0 : eta_bg;
+    lo = hi;
+  }
 }
 
 __global__ __launch_bounds__(BX * BY) void k_fill(Geo g, Box b, real *__restrict__ a, double v) {
@@ -284,76 +311,103 @@ __global__ __launch_bounds__(BX * BY) void k_fill(Geo g, Box b, real *__restrict
 }
 
 // rkstep (lagran.F90:1472-1503) over nodes 0..n, with the
-// total_ohmic_heating increments of :1309-1328 reduced in the same pass.
-__global__ __launch_bounds__(BX * BY) void k_rkstep(Geo g, const double *__restrict__ bx, const double *__restrict__ by,
+// total_ohmic_heating increments of :1309-1328 reduced in the same pass;
+// marches in z (box nodes 0..n).
+__global__ __launch_bounds__(BX * BY) void k_rkstep(Geo g, Box b, const double *__restrict__ bx, const double *__restrict__ by,
                          const double *__restrict__ bz, const real *__restrict__ eta, double *__restrict__ fxr,
                          double *__restrict__ fyr, double *__restrict__ fzr, real *__restrict__ curlb,
                          double *__restrict__ partial, double dt) {
-  const int i = (int)(blockIdx.x * BX + threadIdx.x);
-  const int j = (int)(blockIdx.y * BY + threadIdx.y);
-  const int k = (int)blockIdx.z;
+  const int i = b.x0 + (int)(blockIdx.x * BX + threadIdx.x);
+  const int j = b.y0 + (int)(blockIdx.y * BY + threadIdx.y);
+  const int k0 = b.z0 + (int)blockIdx.z * ZCHUNK;
+  const int k1 = k0 + ZCHUNK - 1 < b.z1 ? k0 + ZCHUNK - 1 : b.z1;
   double v[1] = {0.0};
-  if (i <= g.nx && j <= g.ny && k <= g.nz) {
-    double jx, jy, jz;
-    vertex_current(g, bx, by, bz, i, j, k, jx, jy, jz);
-    const int c = g.idx(i, j, k);
-    const double e = eta[c];
-    fxr[c] = -jx * e * g.dxc[i] * 0.5;
-    fyr[c] = -jy * e * g.dyc[j] * 0.5;
-    fzr[c] = -jz * e * g.dzc[k] * 0.5;
-    const double cb = e * (jx * jx + jy * jy + jz * jz);
-    curlb[c] = cb;
-    double w1 = dt * g.dxc[i] * g.dyc[j] * g.dzc[k] * cb;
-    if (i == 0 || i == g.nx) w1 = w1 * 0.5;
-    if (j == 0 || j == g.ny) w1 = w1 * 0.5;
-    if (k == 0 || k == g.nz) w1 = w1 * 0.5;
-    v[0] = w1;
+  if (i <= b.x1 && j <= b.y1) {
+    const int Y = g.sy, Z = g.sz;
+    BPlane lo = bplane(bx, by, g.idx(i, j, k0), Y);
+    for (int k = k0; k <= k1; ++k) {
+      const int c = g.idx(i, j, k);
+      const BPlane hi = bplane(bx, by, c + Z, Y);
+      double jx, jy, jz;
+      vertex_current_vals(g.dxc[i], g.dyc[j], g.dzc[k], lo, hi, bz[c], bz[c + 1], bz[c + Y], bz[c + 1 + Y], jx, jy,
+                          jz);
+      lo = hi;
+      const double e = eta[c];
+      fxr[c] = -jx * e * g.dxc[i] * 0.5;
+      fyr[c] = -jy * e * g.dyc[j] * 0.5;
+      fzr[c] = -jz * e * g.dzc[k] * 0.5;
+      const double cb = e * (jx * jx + jy * jy + jz * jz);
+      curlb[c] = cb;
+      double w1 = dt * g.dxc[i] * g.dyc[j] * g.dzc[k] * cb;
+      if (i == 0 || i == g.nx) w1 = w1 * 0.5;
+      if (j == 0 || j == g.ny) w1 = w1 * 0.5;
+      if (k == 0 || k == g.nz) w1 = w1 * 0.5;
+      v[0] += w1;
+    }
   }
   const int ops[1] = {1};
   block_reduce_store<1>(v, ops, partial);
 }
 
 // bstep (lagran.F90:1582-1631); B updated in place (bx1 = bx at the same
-// point, :1259-1261)
+// point, :1259-1261).  Marches in z with the EMFs of plane k-1 in registers.
+struct EPlane {
+  double kx0, kxm, ky0, kym, kz0, kzm, kzx;  // kx at (i,j),(i-1,j); ky at (i,j),(i,j-1); kz at (i,j),(i,j-1),(i-1,j)
+};
+__device__ __forceinline__ EPlane eplane(const double *__restrict__ kx, const double *__restrict__ ky,
+                                         const double *__restrict__ kz, int c, int Y) {
+  EPlane p = {kx[c], kx[c - 1], ky[c], ky[c - Y], kz[c], kz[c - Y], kz[c - 1]};
+  return p;
+}
 __global__ __launch_bounds__(BX * BY) void k_bstep(Geo g, Box b, double *__restrict__ bx, double *__restrict__ by, double *__restrict__ bz,
                         const double *__restrict__ kx, const double *__restrict__ ky, const double *__restrict__ kz,
                         double dt) {
-  BOX_INDEX(b, i, j, k);
-  const int c = g.idx(i, j, k);
+  ZCOLUMN(b, i, j, k0, k1);
   const int Y = g.sy, Z = g.sz;
-  if (j >= 1 && k >= 1) {
-    const double area = g.dyb[j] * g.dzb[k];
-    bx[c] = bx[c] + (kz[c] - kz[c - Y] + kz[c - Z] - kz[c - Y - Z] - ky[c] + ky[c - Z] - ky[c - Y] + ky[c - Y - Z]) *
-                        dt / area;
-  }
-  if (i >= 1 && k >= 1) {
-    const double area = g.dxb[i] * g.dzb[k];
-    by[c] = by[c] + (kx[c] - kx[c - Z] + kx[c - 1] - kx[c - 1 - Z] - kz[c] + kz[c - 1] - kz[c - Z] + kz[c - 1 - Z]) *
-                        dt / area;
-  }
-  if (i >= 1 && j >= 1) {
-    const double area = g.dxb[i] * g.dyb[j];
-    bz[c] = bz[c] + (ky[c] - ky[c - 1] + ky[c - Y] - ky[c - 1 - Y] - kx[c] + kx[c - Y] - kx[c - 1] + kx[c - 1 - Y]) *
-                        dt / area;
+  EPlane m = eplane(kx, ky, kz, g.idx(i, j, k0) - Z, Y);  // plane k-1
+  for (int k = k0; k <= k1; ++k) {
+    const int c = g.idx(i, j, k);
+    const EPlane p = eplane(kx, ky, kz, c, Y);
+    if (j >= 1 && k >= 1) {
+      const double area = g.dyb[j] * g.dzb[k];
+      bx[c] = bx[c] + (p.kz0 - p.kzm + m.kz0 - m.kzm - p.ky0 + m.ky0 - p.kym + m.kym) * dt / area;
+    }
+    if (i >= 1 && k >= 1) {
+      const double area = g.dxb[i] * g.dzb[k];
+      by[c] = by[c] + (p.kx0 - m.kx0 + p.kxm - m.kxm - p.kz0 + p.kzx - m.kz0 + m.kzx) * dt / area;
+    }
+    if (i >= 1 && j >= 1) {
+      const double area = g.dxb[i] * g.dyb[j];
+      bz[c] = bz[c] + (p.ky0 - ky[c - 1] + p.kym - ky[c - 1 - Y] - p.kx0 + kx[c - Y] - p.kxm + kx[c - 1 - Y]) *
+                          dt / area;
+    }
+    m = p;
   }
 }
 
 // Ohmic heating (lagran.F90:1270-1288) on cells 1..n, then
-// energy = MAX(energy, 0) over the whole cell box (:1305)
+// energy = MAX(energy, 0) over the whole cell box (:1305); marches in z with
+// the vertex heating of plane k-1 in registers.
 __global__ __launch_bounds__(BX * BY) void k_ohmic(Geo g, Box b, real *__restrict__ energy, real *__restrict__ ohmic_dep,
                         const real *__restrict__ rho, const real *__restrict__ curlb, double dt) {
-  BOX_INDEX(b, i, j, k);
-  const int c = g.idx(i, j, k);
-  double e = energy[c];
-  if (i >= 1 && i <= g.nx && j >= 1 && j <= g.ny && k >= 1 && k <= g.nz) {
-    const int X = 1, Y = g.sy, Z = g.sz;
-    const double s = curlb[c] + curlb[c - X] + curlb[c - Y] + curlb[c - X - Y] + curlb[c - Z] + curlb[c - X - Z] +
-                     curlb[c - Y - Z] + curlb[c - X - Y - Z];
-    const double lh = s * dt / (8.0 * rho[c]);
-    ohmic_dep[c] = ohmic_dep[c] + lh;
-    e = e + lh;
+  ZCOLUMN(b, i, j, k0, k1);
+  const int Y = g.sy;
+  const bool inxy = i >= 1 && i <= g.nx && j >= 1 && j <= g.ny;
+  Quad m = quad_back(curlb, g.idx(i, j, k0 - 1), Y);
+  for (int k = k0; k <= k1; ++k) {
+    const int c = g.idx(i, j, k);
+    const Quad p = quad_back(curlb, c, Y);
+    double e = energy[c];
+    if (inxy && k >= 1 && k <= g.nz) {
+      // curlb(c) + (c-X) + (c-Y) + (c-X-Y) + (c-Z) + (c-X-Z) + (c-Y-Z) + (c-X-Y-Z)
+      const double s = (double)p.p0 + p.px + p.py + p.pxy + m.p0 + m.px + m.py + m.pxy;
+      const double lh = s * dt / (8.0 * rho[c]);
+      ohmic_dep[c] = ohmic_dep[c] + lh;
+      e = e + lh;
+    }
+    energy[c] = max0(e);
+    m = p;
   }
-  energy[c] = max0(e);
 }
 
 __global__ __launch_bounds__(BX * BY) void k_energy_max(Geo g, Box b, real *__restrict__ energy) {

@@ -37,25 +37,6 @@ __device__ __forceinline__ double vl_di(double dfi, double dfu, double sign_v, d
   return sign_v * ss * dmin(dmin(fabs(Da) * dbu, fabs(dfi)), fabs(dfu));
 }
 
-// Columns of a z-marching launch: thread (i, j) of tile (blockIdx.x,
-// blockIdx.y) walks planes k0 .. k1 of its chunk (ZCHUNK planes per block).
-#define ZCHUNK 8
-#define ZCOLUMN(b, i, j, k0, k1)                                      \
-  const int i = (b).x0 + (int)(blockIdx.x * BX + threadIdx.x);         \
-  const int j = (b).y0 + (int)(blockIdx.y * BY + threadIdx.y);         \
-  const int k0 = (b).z0 + (int)blockIdx.z * ZCHUNK;                    \
-  const int k1 = k0 + ZCHUNK - 1 < (b).z1 ? k0 + ZCHUNK - 1 : (b).z1;  \
-  if (i > (b).x1 || j > (b).y1) return;
-
-// A field at the four in-plane points (i, j), (i-1, j), (i, j-1), (i-1, j-1).
-struct Quad {
-  real p0, px, py, pxy;
-};
-__device__ __forceinline__ Quad quad_back(const real *__restrict__ u, int c, int Y) {
-  Quad q = {u[c], u[c - 1], u[c - Y], u[c - 1 - Y]};
-  return q;
-}
-
 // Lagrangian geometry of a sweep (xremap.f90:57-105 and analogues).
 // Velocities are the time-centred ones (v in predictor mode).  The node
 // velocities of planes k and k-1 are kept in registers while marching in z.
@@ -384,12 +365,6 @@ __global__ __launch_bounds__(BX * BY) void k_rho_energy_update(Geo g, Box b, con
   const double rn = (r1 * cv1[c] + dm[c - S] - dm[c]) / cv2[c];
   rho_new[c] = rn;
   energy[c] = (energy[c] * cv1[c] * r1 + F[c - S] - F[c]) / (cv2[c] * rn);
-}
-
-// A field at the four in-plane points (i, j), (i+1, j), (i, j+1), (i+1, j+1).
-__device__ __forceinline__ Quad quad_fwd(const real *__restrict__ u, int c, int Y) {
-  Quad q = {u[c], u[c + 1], u[c + Y], u[c + 1 + Y]};
-  return q;
 }
 
 // vertex quantities of xremap.f90:201-280: rho_v over sweep -1..n+1 and

@@ -478,7 +478,8 @@ int eulerian_remap(int step, bool predictor) {
 // ----------------------------------------------------------------------------
 int eta_calc() {
   if (S.p.resistive_mhd) {
-    LAUNCH_BOX("eta", k_eta, mkbox(-1, nx() + 1, -1, ny() + 1, -1, nz() + 1), S.bx, S.by, S.bz, S.eta,
+    const Box eb = mkbox(-1, nx() + 1, -1, ny() + 1, -1, nz() + 1);
+    LAUNCH("eta", k_eta, zchunk_grid(eb), box_block(), S.g, eb, S.bx, S.by, S.bz, S.eta,
                S.p.eta_background, S.p.eta0, S.p.j_max);
   } else {
     LAUNCH_BOX("fill", k_fill, cell_box(), S.eta, 0.0);
@@ -562,15 +563,17 @@ int lagrangian_step() {
     for (int sc = 1; sc <= substeps; ++sc) {
       rc = eta_calc();
       if (rc) return rc;
-      dim3 g2 = grid_from0(N0, N1, N2);
-      LAUNCH("rkstep", k_rkstep, g2, box_block(), S.g, S.bx, S.by, S.bz, S.eta, S.fxr, S.fyr, S.fzr, S.curlb,
-             S.partial, S.dt);
+      dim3 g2 = zchunk_grid(nodes0n());
+      LAUNCH("rkstep", k_rkstep, g2, box_block(), S.g, nodes0n(), S.bx, S.by, S.bz, S.eta, S.fxr, S.fyr, S.fzr,
+             S.curlb, S.partial, S.dt);
       rc = final_reduce(g2, 1, 0, 9, 1);
       if (rc) return rc;
-      LAUNCH_BOX("bstep", k_bstep, nodes0n(), S.bx, S.by, S.bz, S.fxr, S.fyr, S.fzr, S.dt);
+      LAUNCH("bstep", k_bstep, zchunk_grid(nodes0n()), box_block(), S.g, nodes0n(), S.bx, S.by, S.bz, S.fxr, S.fyr,
+             S.fzr, S.dt);
       rc = bfield_bcs(S.bx, S.by, S.bz);
       if (rc) return rc;
-      LAUNCH_BOX("ohmic", k_ohmic, cell_box(), S.energy, S.ohmic_dep, S.rho, S.curlb, S.dt);
+      LAUNCH("ohmic", k_ohmic, zchunk_grid(cell_box()), box_block(), S.g, cell_box(), S.energy, S.ohmic_dep, S.rho,
+             S.curlb, S.dt);
       rc = energy_bcs();
       if (rc) return rc;
     }
