@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # box.  Counting: each distinct field read once, each written field once.
 KERNEL_BYTES_PER_POINT = {
     "mom_flux": 8 * (3 + 1 + 1 + 1 + 1 + 3),        # v x3, v1, db1, rho_v, dmv -> Fx,Fy,Fz
-    "ct_flux": 8 * (1 + 1 + 1 + 1),                 # B_t, v_sweep, db1 -> F
+    "ct_flux": 8 * (2 + 1 + 1 + 2),                 # B_t1, B_t2, v_sweep, db1 -> F1, F2
     "geom": 8 * (3 + 3),                            # v1 x3 -> cv1, cv2, db1
     "vertex": 8 * (4 + 4),                          # rho1, cv1, cv2, dm -> rho_v, cv1v, cv2v, dmv
     "alpha": 8 * (3 + 2 + 3),                       # v x3, rho_v, cs_v -> alpha1..3
@@ -46,7 +46,7 @@ KERNEL_BYTES_PER_POINT = {
     "mass_flux": 8 * (1 + 1 + 1 + 1),               # rho, v_sweep, db1 -> dm
     "mom_update": 8 * (4 + 3 + 3 + 3),              # rho_v,cv1v,cv2v,dmv, F x3, v x3 (rw)
     "rho_energy_update": 8 * (5 + 1 + 2 + 2),       # rho1,cv1,cv2,dm,F, e(rw), rho_new
-    "ct_update": 8 * (1 + 2 + 2),                   # F, t (rw), a (rw)
+    "ct_update": 8 * (2 + 2 + 2 + 2),               # F1, F2, t1 (rw), t2 (rw), a (rw)
     "rkl2_stage": 8 * (9 + 1),                      # Y2,Y1,E0,rho,B x3,larsen,Lc -> Y3
 }
 

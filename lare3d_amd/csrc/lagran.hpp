@@ -53,44 +53,81 @@ __device__ __forceinline__ double cell_cs(const Geo &g, const real *__restrict__
   return sqrt((gamma * p + 1.0 * b2) / rmin);
 }
 
+// rho and the fast speed of the four cells (i,j), (i+1,j), (i,j+1), (i+1,j+1)
+// of one plane, starting at cell index c
+struct CellQuad {
+  double r[4], cs[4];
+};
+__device__ __forceinline__ CellQuad cell_quad(const Geo &g, const real *__restrict__ rho,
+                                              const real *__restrict__ energy, const double *__restrict__ bx,
+                                              const double *__restrict__ by, const double *__restrict__ bz, int c,
+                                              double gamma) {
+  CellQuad q;
+  const int off[4] = {0, 1, g.sy, 1 + g.sy};
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    q.r[p] = rho[c + off[p]];
+    q.cs[p] = cell_cs(g, rho, energy, bx, by, bz, c + off[p], gamma);
+  }
+  return q;
+}
+
+// vertex density rho_v and fast speed cs_v (lagran.F90:105-117, :561-581)
+// over nodes -1..n+1; marches in z, keeping the cell values of plane k+1 for
+// the next node plane, so each cell's fast speed is computed once per
+// column chunk.  Sums run over the 8 cells in the order (i,j,k), (i+1,j,k),
+// (i,j+1,k), (i+1,j+1,k), then the same four at k+1.
 __global__ __launch_bounds__(BX * BY) void k_vertex_prep(Geo g, Box b, const real *__restrict__ rho, const real *__restrict__ energy,
                               const double *__restrict__ bx, const double *__restrict__ by,
                               const double *__restrict__ bz, real *__restrict__ rho_v, real *__restrict__ cs_v,
                               double gamma) {
-  BOX_INDEX(b, i, j, k);
-  const int c = g.idx(i, j, k);
-  const int sy = g.sy, sz = g.sz;
-  const int cc[8] = {c, c + 1, c + sy, c + 1 + sy, c + sz, c + 1 + sz, c + sy + sz, c + 1 + sy + sz};
-  const int di[8] = {0, 1, 0, 1, 0, 1, 0, 1}, dj[8] = {0, 0, 1, 1, 0, 0, 1, 1}, dk[8] = {0, 0, 0, 0, 1, 1, 1, 1};
-  double cvs[8];
+  ZCOLUMN(b, i, j, k0, k1);
+  const int Z = g.sz;
+  CellQuad lo = cell_quad(g, rho, energy, bx, by, bz, g.idx(i, j, k0), gamma);
+  for (int k = k0; k <= k1; ++k) {
+    const int c = g.idx(i, j, k);
+    const CellQuad hi = cell_quad(g, rho, energy, bx, by, bz, c + Z, gamma);
+    double cvs[8];
+    const int di[8] = {0, 1, 0, 1, 0, 1, 0, 1}, dj[8] = {0, 0, 1, 1, 0, 0, 1, 1}, dk[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 #pragma unroll
-  for (int q = 0; q < 8; ++q) cvs[q] = g.cv(i + di[q], j + dj[q], k + dk[q]);
-  double s = rho[cc[0]] * cvs[0];
+    for (int q = 0; q < 8; ++q) cvs[q] = g.cv(i + di[q], j + dj[q], k + dk[q]);
+    double s = lo.r[0] * cvs[0];
 #pragma unroll
-  for (int q = 1; q < 8; ++q) s = s + rho[cc[q]] * cvs[q];
-  double sc = cvs[0];
+    for (int q = 1; q < 8; ++q) s = s + (q < 4 ? lo.r[q] : hi.r[q - 4]) * cvs[q];
+    double sc = cvs[0];
 #pragma unroll
-  for (int q = 1; q < 8; ++q) sc = sc + cvs[q];
-  rho_v[c] = s / sc;
-  const double cvv = 0.125 * sc;
-  double scs = cell_cs(g, rho, energy, bx, by, bz, cc[0], gamma) * cvs[0];
+    for (int q = 1; q < 8; ++q) sc = sc + cvs[q];
+    rho_v[c] = s / sc;
+    const double cvv = 0.125 * sc;
+    double scs = lo.cs[0] * cvs[0];
 #pragma unroll
-  for (int q = 1; q < 8; ++q) scs = scs + cell_cs(g, rho, energy, bx, by, bz, cc[q], gamma) * cvs[q];
-  cs_v[c] = 0.125 * scs / cvv;
+    for (int q = 1; q < 8; ++q) scs = scs + (q < 4 ? lo.cs[q] : hi.cs[q - 4]) * cvs[q];
+    cs_v[c] = 0.125 * scs / cvv;
+    lo = hi;
+  }
 }
 
-// edge_viscosity (lagran.F90:795-855) between nodes n1 and n2 (linear idx)
-__device__ __forceinline__ double edge_visc(const real *__restrict__ vx, const real *__restrict__ vy,
-                                            const real *__restrict__ vz, const real *__restrict__ rho_v,
-                                            const real *__restrict__ cs_v, int n1, int n2, double dvdots,
-                                            double dx, const LagArgs &a) {
+// node values used by the edge viscosity
+struct NodeV {
+  real vx, vy, vz, r, cs;
+};
+__device__ __forceinline__ NodeV node_v(const real *__restrict__ vx, const real *__restrict__ vy,
+                                        const real *__restrict__ vz, const real *__restrict__ rho_v,
+                                        const real *__restrict__ cs_v, int c) {
+  NodeV n = {vx[c], vy[c], vz[c], rho_v[c], cs_v[c]};
+  return n;
+}
+
+// edge_viscosity (lagran.F90:795-855) between nodes 1 and 2
+__device__ __forceinline__ double edge_visc(const NodeV &n1, const NodeV &n2, double dvdots, double dx,
+                                            const LagArgs &a) {
   dvdots = -fabs(dvdots);
-  const double r1 = rho_v[n1], r2 = rho_v[n2];
+  const double r1 = n1.r, r2 = n2.r;
   const double rho_edge = 2.0 * r1 * r2 / (r1 + r2);
-  const double cs_edge = dmin(cs_v[n1], cs_v[n2]);
-  const double dvx = vx[n1] - vx[n2];
-  const double dvy = vy[n1] - vy[n2];
-  const double dvz = vz[n1] - vz[n2];
+  const double cs_edge = dmin(n1.cs, n2.cs);
+  const double dvx = n1.vx - n2.vx;
+  const double dvy = n1.vy - n2.vy;
+  const double dvz = n1.vz - n2.vz;
   const double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
   const double dv = sqrt(dv2);
   const double psi = 0.0;
@@ -102,29 +139,35 @@ __device__ __forceinline__ double edge_visc(const real *__restrict__ vx, const r
   return q_k_bar * (1.0 - psi) * dvdots;
 }
 
+// Caramana edge viscosities alpha1..3 (lagran.F90:583-678) on their edge
+// ranges.  Marches in z: the nodes of plane k become the plane k-1 nodes of
+// the next edge plane.
 __global__ __launch_bounds__(BX * BY) void k_alpha(Geo g, Box b, const real *__restrict__ vx, const real *__restrict__ vy,
                         const real *__restrict__ vz, const real *__restrict__ rho_v,
                         const real *__restrict__ cs_v, real *__restrict__ al1, real *__restrict__ al2,
                         real *__restrict__ al3, LagArgs a) {
-  BOX_INDEX(b, i, j, k);
-  const int c = g.idx(i, j, k);
+  ZCOLUMN(b, i, j, k0, k1);
+  const int Y = g.sy;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  if (i >= 0 && i <= nx + 1 && j >= 0 && j <= ny + 2 && k >= 0 && k <= nz + 2) {
-    const int n1 = g.idx(i - 1, j - 1, k - 1), n2 = g.idx(i, j - 1, k - 1);
-    al1[c] = edge_visc(vx, vy, vz, rho_v, cs_v, n1, n2, -(vx[n1] - vx[n2]), g.dxb[i], a);
-  }
-  if (i >= -1 && i <= nx + 1 && j >= 0 && j <= ny + 1 && k >= 0 && k <= nz + 2) {
-    const int n1 = g.idx(i, j - 1, k - 1), n2 = g.idx(i, j, k - 1);
-    al2[c] = edge_visc(vx, vy, vz, rho_v, cs_v, n1, n2, -(vy[n1] - vy[n2]), g.dyb[j], a);
-  }
-  if (i >= -1 && i <= nx + 1 && j >= -1 && j <= ny + 1 && k >= 0 && k <= nz + 1) {
-    const int n1 = g.idx(i, j, k - 1), n2 = g.idx(i, j, k);
-    al3[c] = edge_visc(vx, vy, vz, rho_v, cs_v, n1, n2, -(vz[n1] - vz[n2]), g.dzb[k], a);
+  const int cb = g.idx(i, j, k0 - 1);
+  // plane k-1 nodes (i-1, j-1), (i, j-1), (i, j)
+  NodeV m1 = node_v(vx, vy, vz, rho_v, cs_v, cb - 1 - Y), m2 = node_v(vx, vy, vz, rho_v, cs_v, cb - Y),
+        m3 = node_v(vx, vy, vz, rho_v, cs_v, cb);
+  for (int k = k0; k <= k1; ++k) {
+    const int c = g.idx(i, j, k);
+    const NodeV p3 = node_v(vx, vy, vz, rho_v, cs_v, c);
+    if (i >= 0 && i <= nx + 1 && j >= 0 && j <= ny + 2 && k >= 0 && k <= nz + 2)
+      al1[c] = edge_visc(m1, m2, -(m1.vx - m2.vx), g.dxb[i], a);
+    if (i >= -1 && i <= nx + 1 && j >= 0 && j <= ny + 1 && k >= 0 && k <= nz + 2)
+      al2[c] = edge_visc(m2, m3, -(m2.vy - m3.vy), g.dyb[j], a);
+    if (i >= -1 && i <= nx + 1 && j >= -1 && j <= ny + 1 && k >= 0 && k <= nz + 1)
+      al3[c] = edge_visc(m3, p3, -(m3.vz - p3.vz), g.dzb[k], a);
+    m1 = node_v(vx, vy, vz, rho_v, cs_v, c - 1 - Y);
+    m2 = node_v(vx, vy, vz, rho_v, cs_v, c - Y);
+    m3 = p3;
   }
 }
 
-// node pairs of the 12 cell edges a1..a12 (lagran.F90:690-715), as linear
-// offsets relative to node (i,j,k) given sy, sz
 __device__ __forceinline__ void edge_nodes(int c, int sy, int sz, int e[12][2]) {
   const int m = c - 1 - sy - sz;  // (ixm, iym, izm)
   const int X = 1, Y = sy, Z = sz;
@@ -165,9 +208,54 @@ __device__ __forceinline__ double edge_heat(const Geo &g, int i, int j, int k, c
   return h / g.cv(i, j, k);
 }
 
+// Node values of one plane at the four nodes of a cell column: A = (i-1,j-1),
+// B = (i,j-1), C = (i,j), D = (i-1,j), for one field.
+struct Node4 {
+  real a, b, c, d;
+};
+__device__ __forceinline__ Node4 node4(const real *__restrict__ u, int c, int Y) {
+  Node4 n = {u[c - 1 - Y], u[c - Y], u[c], u[c - 1]};
+  return n;
+}
+// the 12 edges of cell (i,j,k) in the order of edge_nodes, as (plane, node)
+// of their first and second node: plane l = k-1, h = k; nodes a..d as Node4
+#define EDGES(F)                                                                                       \
+  F(0, l, a, l, b) F(1, l, b, l, c) F(2, l, c, l, d) F(3, l, d, l, a) F(4, h, a, h, b) F(5, h, b, h, c) \
+  F(6, h, c, h, d) F(7, h, d, h, a) F(8, l, c, h, c) F(9, l, d, h, d) F(10, l, a, h, a) F(11, l, b, h, b)
+
+// alpha values around a cell: al1 at (i,j), (i,j+1) and al2 at (i,j), (i-1,j)
+// of one plane
+struct Al12 {
+  real a1, a1y, a2, a2x;
+};
+__device__ __forceinline__ Al12 al12(const real *__restrict__ al1, const real *__restrict__ al2, int c, int Y) {
+  Al12 v = {al1[c], al1[c + Y], al2[c], al2[c - 1]};
+  return v;
+}
+
+// edge_heat from values: al1/al2 of planes k (lo) and k+1 (hi), al3 at
+// (i,j), (i-1,j), (i-1,j-1), (i,j-1) of plane k; same terms and order
+__device__ __forceinline__ double edge_heat_v(double dxb, double dyb, double dzb, double cv, const Al12 &lo,
+                                              const Al12 &hi, double a30, double a3x, double a3xy, double a3y,
+                                              const double aa[12]) {
+  double h = -0.25 * dyb * dzb * lo.a1 * aa[0];
+  h = h - 0.25 * dxb * dzb * lo.a2 * aa[1];
+  h = h - 0.25 * dyb * dzb * lo.a1y * aa[2];
+  h = h - 0.25 * dxb * dzb * lo.a2x * aa[3];
+  h = h - 0.25 * dyb * dzb * hi.a1 * aa[4];
+  h = h - 0.25 * dxb * dzb * hi.a2 * aa[5];
+  h = h - 0.25 * dyb * dzb * hi.a1y * aa[6];
+  h = h - 0.25 * dxb * dzb * hi.a2x * aa[7];
+  h = h - 0.25 * dxb * dyb * a30 * aa[8];
+  h = h - 0.25 * dxb * dyb * a3x * aa[9];
+  h = h - 0.25 * dxb * dyb * a3xy * aa[10];
+  h = h - 0.25 * dxb * dyb * a3y * aa[11];
+  return h / cv;
+}
+
 // p_visc + visc_heat over cells 0..n+1; pressure p0 of lagran.F90:93 stored
 // for the predictor; set_dt partial minima over 0..n.
-__global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, const real *__restrict__ vx, const real *__restrict__ vy,
+__global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, Box bx0, const real *__restrict__ vx, const real *__restrict__ vy,
                           const real *__restrict__ vz, const real *__restrict__ rho,
                           const real *__restrict__ energy, const double *__restrict__ bx,
                           const double *__restrict__ by, const double *__restrict__ bz,
@@ -175,45 +263,60 @@ __global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, const real *__restri
                           const real *__restrict__ al2, const real *__restrict__ al3,
                           real *__restrict__ visc_heat, real *__restrict__ p0, double *__restrict__ partial,
                           LagArgs a) {
-  const int i = (int)(blockIdx.x * BX + threadIdx.x);
-  const int j = (int)(blockIdx.y * BY + threadIdx.y);
-  const int k = (int)blockIdx.z;
+  // cells 0..n+1, marching in z: node velocities and al1/al2 of plane k
+  // become those of plane k-1 / k for the next cell
+  const int i = bx0.x0 + (int)(blockIdx.x * BX + threadIdx.x);
+  const int j = bx0.y0 + (int)(blockIdx.y * BY + threadIdx.y);
+  const int k0 = bx0.z0 + (int)blockIdx.z * ZCHUNK;
+  const int k1 = k0 + ZCHUNK - 1 < bx0.z1 ? k0 + ZCHUNK - 1 : bx0.z1;
   double v[2] = {LARGEST, LARGEST};
-  if (i <= g.nx + 1 && j <= g.ny + 1 && k <= g.nz + 1) {
-    const int c = g.idx(i, j, k);
-    int e[12][2];
-    edge_nodes(c, g.sy, g.sz, e);
-    double aa[12];
-#pragma unroll
-    for (int q = 0; q < 12; ++q) {
-      const double dx = vx[e[q][0]] - vx[e[q][1]];
-      const double dy = vy[e[q][0]] - vy[e[q][1]];
-      const double dz = vz[e[q][0]] - vz[e[q][1]];
-      aa[q] = dx * dx + dy * dy + dz * dz;
-    }
-    double pv = 0.0;
-    pv = dmax(pv, -al1[c] * sqrt(aa[0]));
-    pv = dmax(pv, -al2[c] * sqrt(aa[1]));
-    pv = dmax(pv, -al3[c] * sqrt(aa[8]));
-    visc_heat[c] = edge_heat(g, i, j, k, al1, al2, al3, aa);
-    const double r = rho[c];
-    const double p = (a.gamma - 1.0) * r * (energy[c] - 0.0);
-    p0[c] = p;
-    if (i <= g.nx && j <= g.ny && k <= g.nz) {
-      const double w1 = bx[c] * bx[c] + by[c] * by[c] + bz[c] * bz[c];
-      const double rho0 = dmax(r, NONE_ZERO);
-      const double cs2 = (a.gamma * p + 1.0 * w1) / rho0;
-      const double c_visc2 = pv / rho0;
-      const double dxb = g.dxb[i], dyb = g.dyb[j], dzb = g.dzb[k];
-      const double length = dmin(dmin(dxb, dyb), dzb);
-      const double dt1 = length / (sqrt(c_visc2) + sqrt(cs2 + c_visc2));
-      const double dt2 = dmin(dmin(dxb / dmax(fabs(vx[c]), NONE_ZERO), dyb / dmax(fabs(vy[c]), NONE_ZERO)),
-                              dzb / dmax(fabs(vz[c]), NONE_ZERO));
-      v[0] = dmin(dt1, dt2);
-      if (a.resistive) {
-        const double dxlocal = 1.0 / (1.0 / (dxb * dxb) + 1.0 / (dyb * dyb) + 1.0 / (dzb * dzb));
-        v[1] = 0.2 * dxlocal / dmax(eta[c], NONE_ZERO);
+  if (i <= bx0.x1 && j <= bx0.y1) {
+    const int Y = g.sy, Z = g.sz;
+    const int cb = g.idx(i, j, k0);
+    Node4 lx = node4(vx, cb - Z, Y), ly = node4(vy, cb - Z, Y), lz = node4(vz, cb - Z, Y);
+    Al12 alo = al12(al1, al2, cb, Y);
+    for (int k = k0; k <= k1; ++k) {
+      const int c = g.idx(i, j, k);
+      const Node4 hx = node4(vx, c, Y), hy = node4(vy, c, Y), hz = node4(vz, c, Y);
+      const Al12 ahi = al12(al1, al2, c + Z, Y);
+      double aa[12];
+      // |v(first) - v(second)|^2 per edge
+#define AAV(q, P1, N1, P2, N2)                                                                    \
+  {                                                                                               \
+    const double dx = P1##x.N1 - P2##x.N2, dy = P1##y.N1 - P2##y.N2, dz = P1##z.N1 - P2##z.N2; \
+    aa[q] = dx * dx + dy * dy + dz * dz;                                                          \
+  }
+      EDGES(AAV)
+#undef AAV
+      const double a30 = al3[c], a3x = al3[c - 1], a3xy = al3[c - 1 - Y], a3y = al3[c - Y];
+      double pv = 0.0;
+      pv = dmax(pv, -(double)alo.a1 * sqrt(aa[0]));
+      pv = dmax(pv, -(double)alo.a2 * sqrt(aa[1]));
+      pv = dmax(pv, -a30 * sqrt(aa[8]));
+      visc_heat[c] = edge_heat_v(g.dxb[i], g.dyb[j], g.dzb[k], g.cv(i, j, k), alo, ahi, a30, a3x, a3xy, a3y, aa);
+      const double r = rho[c];
+      const double p = (a.gamma - 1.0) * r * (energy[c] - 0.0);
+      p0[c] = p;
+      if (i <= g.nx && j <= g.ny && k <= g.nz) {
+        const double w1 = bx[c] * bx[c] + by[c] * by[c] + bz[c] * bz[c];
+        const double rho0 = dmax(r, NONE_ZERO);
+        const double cs2 = (a.gamma * p + 1.0 * w1) / rho0;
+        const double c_visc2 = pv / rho0;
+        const double dxb = g.dxb[i], dyb = g.dyb[j], dzb = g.dzb[k];
+        const double length = dmin(dmin(dxb, dyb), dzb);
+        const double dt1 = length / (sqrt(c_visc2) + sqrt(cs2 + c_visc2));
+        const double dt2 = dmin(dmin(dxb / dmax(fabs((double)hx.c), NONE_ZERO), dyb / dmax(fabs((double)hy.c), NONE_ZERO)),
+                                dzb / dmax(fabs((double)hz.c), NONE_ZERO));
+        v[0] = dmin(v[0], dmin(dt1, dt2));
+        if (a.resistive) {
+          const double dxlocal = 1.0 / (1.0 / (dxb * dxb) + 1.0 / (dyb * dyb) + 1.0 / (dzb * dzb));
+          v[1] = dmin(v[1], 0.2 * dxlocal / dmax((double)eta[c], NONE_ZERO));
+        }
       }
+      lx = hx;
+      ly = hy;
+      lz = hz;
+      alo = ahi;
     }
   }
   const int ops[2] = {0, 0};
@@ -440,92 +543,143 @@ __global__ __launch_bounds__(BX * BY) void k_force(Geo g, Box b, const real *__r
                         const real *__restrict__ rho_v, const real *__restrict__ al1,
                         const real *__restrict__ al2, const real *__restrict__ al3, real *__restrict__ vx1,
                         real *__restrict__ vy1, real *__restrict__ vz1, double dt2) {
-  BOX_INDEX(b, i, j, k);
-  const int c = g.idx(i, j, k);
+  ZCOLUMN(b, i, j, k0, k1);
   const int X = 1, Y = g.sy, Z = g.sz;
-  const double dxc = g.dxc[i], dyc = g.dyc[j], dzc = g.dzc[k];
-  // viscous forces
-  const double a1 = al1[c + Y + Z] * dyc * dzc;
-  const double a2 = al1[c + X + Y + Z] * dyc * dzc;
-  const double a3 = al2[c + Z] * dxc * dzc;
-  const double a4 = al2[c + Y + Z] * dxc * dzc;
-  const double a5 = al3[c] * dxc * dyc;
-  const double a6 = al3[c + Z] * dxc * dyc;
-  const double cvv = 0.125 * cv_sum8(g, i, j, k);
   const real *V[3] = {vx, vy, vz};
-  double fv[3];
+  // marching windows: node velocity at (i, j) of planes k-1, k; pressure,
+  // face B and alpha3 of plane k
+  const int cb = g.idx(i, j, k0);
+  real um[3], u0[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    const real *u = V[q];
-    const double u0 = u[c];
-    fv[q] = (a1 * (u0 - u[c - X]) + a2 * (u0 - u[c + X]) + a3 * (u0 - u[c - Y]) + a4 * (u0 - u[c + Y]) +
-             a5 * (u0 - u[c - Z]) + a6 * (u0 - u[c + Z])) /
-            cvv;
+    um[q] = V[q][cb - Z];
+    u0[q] = V[q][cb];
   }
-  const double p = pp[c], ppx = pp[c + X], ppy = pp[c + Y], ppz = pp[c + Z];
-  const double ppxy = pp[c + X + Y], ppxz = pp[c + X + Z], ppyz = pp[c + Y + Z], ppxyz = pp[c + X + Y + Z];
-  double w1 = (p + ppy + ppz + ppyz) * 0.25;
-  double w2 = (ppx + ppxy + ppxz + ppxyz) * 0.25;
-  double fx = -(w2 - w1) / dxc;
-  w1 = (p + ppx + ppz + ppxz) * 0.25;
-  w2 = (ppy + ppxy + ppyz + ppxyz) * 0.25;
-  double fy = -(w2 - w1) / dyc;
-  w1 = (p + ppx + ppy + ppxy) * 0.25;
-  w2 = (ppz + ppxz + ppyz + ppxyz) * 0.25;
-  double fz = -(w2 - w1) / dzc;
-  const double bxv = 0.25 * (bx[c] + bx[c + Y] + bx[c + Z] + bx[c + Y + Z]);
-  const double byv = 0.25 * (by[c] + by[c + X] + by[c + Z] + by[c + X + Z]);
-  const double bzv = 0.25 * (bz[c] + bz[c + Y] + bz[c + X] + bz[c + X + Y]);
-  double jx, jy, jz;
-  vertex_current(g, bx, by, bz, i, j, k, jx, jy, jz);
-  fx = fx + 1.0 * (jy * bzv - jz * byv);
-  fy = fy + 1.0 * (jz * bxv - jx * bzv);
-  fz = fz + 1.0 * (jx * byv - jy * bxv);
-  const double rv = rho_v[c];
-  fz = fz - rv * g.grav[k];
-  vx1[c] = vx[c] + dt2 * (fv[0] + fx) / rv;
-  vy1[c] = vy[c] + dt2 * (fv[1] + fy) / rv;
-  vz1[c] = vz[c] + dt2 * (fv[2] + fz) / rv;
+  Quad plo = quad_fwd(pp, cb, Y);
+  BPlane blo = bplane(bx, by, cb, Y);
+  real a3lo = al3[cb];
+  for (int k = k0; k <= k1; ++k) {
+    const int c = g.idx(i, j, k);
+    const double dxc = g.dxc[i], dyc = g.dyc[j], dzc = g.dzc[k];
+    real up[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) up[q] = V[q][c + Z];
+    const Quad phi = quad_fwd(pp, c + Z, Y);
+    const BPlane bhi = bplane(bx, by, c + Z, Y);
+    const double bz0 = bz[c], bzx = bz[c + X], bzy = bz[c + Y], bzxy = bz[c + X + Y];
+    const real a3hi = al3[c + Z];
+    // viscous forces
+    const double a1 = al1[c + Y + Z] * dyc * dzc;
+    const double a2 = al1[c + X + Y + Z] * dyc * dzc;
+    const double a3 = al2[c + Z] * dxc * dzc;
+    const double a4 = al2[c + Y + Z] * dxc * dzc;
+    const double a5 = a3lo * dxc * dyc;
+    const double a6 = a3hi * dxc * dyc;
+    const double cvv = 0.125 * cv_sum8(g, i, j, k);
+    double fv[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const real *u = V[q];
+      const double uc = u0[q];
+      fv[q] = (a1 * (uc - u[c - X]) + a2 * (uc - u[c + X]) + a3 * (uc - u[c - Y]) + a4 * (uc - u[c + Y]) +
+               a5 * (uc - um[q]) + a6 * (uc - up[q])) /
+              cvv;
+    }
+    const double p = plo.p0, ppx = plo.px, ppy = plo.py, ppz = phi.p0;
+    const double ppxy = plo.pxy, ppxz = phi.px, ppyz = phi.py, ppxyz = phi.pxy;
+    double w1 = (p + ppy + ppz + ppyz) * 0.25;
+    double w2 = (ppx + ppxy + ppxz + ppxyz) * 0.25;
+    double fx = -(w2 - w1) / dxc;
+    w1 = (p + ppx + ppz + ppxz) * 0.25;
+    w2 = (ppy + ppxy + ppyz + ppxyz) * 0.25;
+    double fy = -(w2 - w1) / dyc;
+    w1 = (p + ppx + ppy + ppxy) * 0.25;
+    w2 = (ppz + ppxz + ppyz + ppxyz) * 0.25;
+    double fz = -(w2 - w1) / dzc;
+    // bx(c) + bx(c+Y) + bx(c+Z) + bx(c+Y+Z), and likewise for by and bz
+    const double bxv = 0.25 * (blo.bx0 + blo.bxy + bhi.bx0 + bhi.bxy);
+    const double byv = 0.25 * (blo.by0 + blo.byx + bhi.by0 + bhi.byx);
+    const double bzv = 0.25 * (bz0 + bzy + bzx + bzxy);
+    double jx, jy, jz;
+    vertex_current_vals(dxc, dyc, dzc, blo, bhi, bz0, bzx, bzy, bzxy, jx, jy, jz);
+    fx = fx + 1.0 * (jy * bzv - jz * byv);
+    fy = fy + 1.0 * (jz * bxv - jx * bzv);
+    fz = fz + 1.0 * (jx * byv - jy * bxv);
+    const double rv = rho_v[c];
+    fz = fz - rv * g.grav[k];
+    vx1[c] = u0[0] + dt2 * (fv[0] + fx) / rv;
+    vy1[c] = u0[1] + dt2 * (fv[1] + fy) / rv;
+    vz1[c] = u0[2] + dt2 * (fv[2] + fz) / rv;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      um[q] = u0[q];
+      u0[q] = up[q];
+    }
+    plo = phi;
+    blo = bhi;
+    a3lo = a3hi;
+  }
 }
 
 // shock_heating (lagran.F90:861-937) + corrector (:415-462) over cells
 // 1..n; total_visc_heating partial sums.
-__global__ __launch_bounds__(BX * BY) void k_shock_corr(Geo g, const real *__restrict__ vx, const real *__restrict__ vy,
+__global__ __launch_bounds__(BX * BY) void k_shock_corr(Geo g, Box b, const real *__restrict__ vx, const real *__restrict__ vy,
                              const real *__restrict__ vz, const real *__restrict__ vx1,
                              const real *__restrict__ vy1, const real *__restrict__ vz1,
                              const real *__restrict__ al1, const real *__restrict__ al2,
                              const real *__restrict__ al3, const real *__restrict__ pp,
                              real *__restrict__ energy, real *__restrict__ rho, real *__restrict__ visc_dep,
                              double *__restrict__ partial, double dt) {
-  const int i = 1 + (int)(blockIdx.x * BX + threadIdx.x);
-  const int j = 1 + (int)(blockIdx.y * BY + threadIdx.y);
-  const int k = 1 + (int)blockIdx.z;
+  // cells 1..n, marching in z: the node values (v, v1) and al1/al2 of plane
+  // k are reused as plane k-1 / k values of the next cell
+  const int i = b.x0 + (int)(blockIdx.x * BX + threadIdx.x);
+  const int j = b.y0 + (int)(blockIdx.y * BY + threadIdx.y);
+  const int k0 = b.z0 + (int)blockIdx.z * ZCHUNK;
+  const int k1 = k0 + ZCHUNK - 1 < b.z1 ? k0 + ZCHUNK - 1 : b.z1;
   double v[1] = {0.0};
-  if (i <= g.nx && j <= g.ny && k <= g.nz) {
-    const int c = g.idx(i, j, k);
-    int e[12][2];
-    edge_nodes(c, g.sy, g.sz, e);
-    double aa[12];
-#pragma unroll
-    for (int q = 0; q < 12; ++q) {
-      const int p = e[q][0], r = e[q][1];
-      aa[q] = (vx[p] - vx[r]) * (vx1[p] - vx1[r]) + (vy[p] - vy[r]) * (vy1[p] - vy1[r]) +
-              (vz[p] - vz[r]) * (vz1[p] - vz1[r]);
+  if (i <= b.x1 && j <= b.y1) {
+    const int Y = g.sy, Z = g.sz;
+    const int cb = g.idx(i, j, k0) - Z;
+    Node4 lx = node4(vx, cb, Y), ly = node4(vy, cb, Y), lz = node4(vz, cb, Y);
+    Node4 lx1 = node4(vx1, cb, Y), ly1 = node4(vy1, cb, Y), lz1 = node4(vz1, cb, Y);
+    Al12 alo = al12(al1, al2, cb + Z, Y);
+    for (int k = k0; k <= k1; ++k) {
+      const int c = g.idx(i, j, k);
+      const Node4 hx = node4(vx, c, Y), hy = node4(vy, c, Y), hz = node4(vz, c, Y);
+      const Node4 hx1 = node4(vx1, c, Y), hy1 = node4(vy1, c, Y), hz1 = node4(vz1, c, Y);
+      const Al12 ahi = al12(al1, al2, c + Z, Y);
+      double aa[12];
+      // (v(first) - v(second)) . (v1(first) - v1(second)) per edge
+#define AAS(q, P1, N1, P2, N2)                                                            \
+  aa[q] = (P1##x.N1 - P2##x.N2) * (P1##x1.N1 - P2##x1.N2) + (P1##y.N1 - P2##y.N2) * (P1##y1.N1 - P2##y1.N2) + \
+          (P1##z.N1 - P2##z.N2) * (P1##z1.N1 - P2##z1.N2);
+      EDGES(AAS)
+#undef AAS
+      const double a30 = al3[c], a3x = al3[c - 1], a3xy = al3[c - 1 - Y], a3y = al3[c - Y];
+      const double vh =
+          max0(edge_heat_v(g.dxb[i], g.dyb[j], g.dzb[k], g.cv(i, j, k), alo, ahi, a30, a3x, a3xy, a3y, aa));
+      // face-averaged v1 (quad sums in the order c, c-Y, c-Z, c-Y-Z etc.;
+      // node C = c, B = c-Y, D = c-X, A = c-X-Y)
+      const double vxb = (hx1.c + hx1.b + lx1.c + lx1.b) * 0.25;
+      const double vxbm = (hx1.d + hx1.a + lx1.d + lx1.a) * 0.25;
+      const double vyb = (hy1.c + hy1.d + ly1.c + ly1.d) * 0.25;
+      const double vybm = (hy1.b + hy1.a + ly1.b + ly1.a) * 0.25;
+      const double vzb = (hz1.c + hz1.d + hz1.b + hz1.a) * 0.25;
+      const double vzbm = (lz1.c + lz1.d + lz1.b + lz1.a) * 0.25;
+      const double dv = ((vxb - vxbm) / g.dxb[i] + (vyb - vybm) / g.dyb[j] + (vzb - vzbm) / g.dzb[k]) * dt;
+      const double r = rho[c];
+      energy[c] = energy[c] + (dt * vh - dv * pp[c]) / r;
+      visc_dep[c] = visc_dep[c] + dt * vh;
+      rho[c] = r / (1.0 + dv);
+      v[0] += dt * vh * g.cv(i, j, k);
+      lx = hx;
+      ly = hy;
+      lz = hz;
+      lx1 = hx1;
+      ly1 = hy1;
+      lz1 = hz1;
+      alo = ahi;
     }
-    const double vh = max0(edge_heat(g, i, j, k, al1, al2, al3, aa));
-    const int X = 1, Y = g.sy, Z = g.sz;
-    const double vxb = (vx1[c] + vx1[c - Y] + vx1[c - Z] + vx1[c - Y - Z]) * 0.25;
-    const double vxbm = (vx1[c - X] + vx1[c - X - Y] + vx1[c - X - Z] + vx1[c - X - Y - Z]) * 0.25;
-    const double vyb = (vy1[c] + vy1[c - X] + vy1[c - Z] + vy1[c - X - Z]) * 0.25;
-    const double vybm = (vy1[c - Y] + vy1[c - X - Y] + vy1[c - Y - Z] + vy1[c - X - Y - Z]) * 0.25;
-    const double vzb = (vz1[c] + vz1[c - X] + vz1[c - Y] + vz1[c - X - Y]) * 0.25;
-    const double vzbm = (vz1[c - Z] + vz1[c - X - Z] + vz1[c - Y - Z] + vz1[c - X - Y - Z]) * 0.25;
-    const double dv = ((vxb - vxbm) / g.dxb[i] + (vyb - vybm) / g.dyb[j] + (vzb - vzbm) / g.dzb[k]) * dt;
-    const double r = rho[c];
-    energy[c] = energy[c] + (dt * vh - dv * pp[c]) / r;
-    visc_dep[c] = visc_dep[c] + dt * vh;
-    rho[c] = r / (1.0 + dv);
-    v[0] = dt * vh * g.cv(i, j, k);
   }
   const int ops[1] = {1};
   block_reduce_store<1>(v, ops, partial);

@@ -121,12 +121,16 @@ __device__ __forceinline__ double ct_flux_core(double v_advect, double frm, doub
   return (fu + Di * (1.0 - phi)) * v_advect * dt;
 }
 
-// CT flux of a transverse face field along AX (vx_by_flux etc.).
-// VS: stride of the axis the velocity is averaged along (v(c) + v(c-VS));
-// GS: stride of the axis d?b1 is averaged along (d(c) + d(c+GS)).
+// CT fluxes of the two transverse face fields of a sweep along AX
+// (vx_by_flux and vx_bz_flux etc.) in one pass.  VS: stride of the axis the
+// velocity is averaged along (v(c) + v(c-VS)); GS: stride of the axis d?b1 is
+// averaged along (d(c) + d(c+GS)); one pair of strides per field.
 template <int AX>
-__global__ __launch_bounds__(BX * BY) void k_ct_flux(Geo g, Box b, const double *__restrict__ fld, const real *__restrict__ vel,
-                          const real *__restrict__ db1, double *__restrict__ F, int VS, int GS, double dt) {
+__global__ __launch_bounds__(BX * BY) void k_ct_flux(Geo g, Box b, const double *__restrict__ fld1,
+                                                     const double *__restrict__ fld2, const real *__restrict__ vel,
+                                                     const real *__restrict__ db1, double *__restrict__ F1,
+                                                     double *__restrict__ F2, int VS1, int GS1, int VS2, int GS2,
+                                                     double dt) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
   const int S = sstride<AX>(g);
@@ -135,72 +139,103 @@ __global__ __launch_bounds__(BX * BY) void k_ct_flux(Geo g, Box b, const double 
   const int sm = s - 1 > -1 ? s - 1 : -1;
   const int sp2 = s + 2 < n + 2 ? s + 2 : n + 2;
   const int cm = c + (sm - s) * S, cp = c + S, cp2 = c + (sp2 - s) * S;
-  const double v_advect = (vel[c] + vel[c - VS]) * 0.5;
-  const double db = (db1[c] + db1[c + GS]) * 0.5;
-  const double dbp = (db1[cp] + db1[cp + GS]) * 0.5;
-  const double dbp2 = (db1[cp2] + db1[cp2 + GS]) * 0.5;
-  const double dbm = (db1[cm] + db1[cm + GS]) * 0.5;
-  F[c] = ct_flux_core(v_advect, fld[cm], fld[c], fld[cp], fld[cp2], dbm, db, dbp, dbp2, db1[c], db1[cp],
-                      dc1<AX>(g, db1, cm, sm), dc1<AX>(g, db1, c, s), dc1<AX>(g, db1, cp, s + 1), dt);
+  const double dcm = dc1<AX>(g, db1, cm, sm), dci = dc1<AX>(g, db1, c, s), dcp = dc1<AX>(g, db1, cp, s + 1);
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const double *fld = q ? fld2 : fld1;
+    const int VS = q ? VS2 : VS1, GS = q ? GS2 : GS1;
+    const double v_advect = (vel[c] + vel[c - VS]) * 0.5;
+    const double db = (db1[c] + db1[c + GS]) * 0.5;
+    const double dbp = (db1[cp] + db1[cp + GS]) * 0.5;
+    const double dbp2 = (db1[cp2] + db1[cp2 + GS]) * 0.5;
+    const double dbm = (db1[cm] + db1[cm + GS]) * 0.5;
+    (q ? F2 : F1)[c] = ct_flux_core(v_advect, fld[cm], fld[c], fld[cp], fld[cp2], dbm, db, dbp, dbp2, db1[c],
+                                    db1[cp], dcm, dci, dcp, dt);
+  }
 }
 
-// The same flux for the z sweep, each thread marching up its column for
-// ZCHUNK planes with the four-plane stencil of fld and db1 in registers, so
-// every plane is read from memory once per column chunk.
-__global__ __launch_bounds__(BX * BY) void k_ct_flux_z(Geo g, Box b, const double *__restrict__ fld,
+// The same fluxes for the z sweep, each thread marching up its column for
+// ZCHUNK planes with the four-plane stencils of both fields and of db1 in
+// registers, so every plane is read from memory once per column chunk.
+__global__ __launch_bounds__(BX * BY) void k_ct_flux_z(Geo g, Box b, const double *__restrict__ fld1,
+                                                       const double *__restrict__ fld2,
                                                        const real *__restrict__ vel, const real *__restrict__ db1,
-                                                       double *__restrict__ F, int VS, int GS, double dt) {
+                                                       double *__restrict__ F1, double *__restrict__ F2, int VS1,
+                                                       int GS1, int VS2, int GS2, double dt) {
   ZCOLUMN(b, i, j, k0, k1);
   const int n = g.nz;
   // window over planes k-1, k, k+1, k+2, each clamped to [-1, n+2]
-  double f[4];
-  real d[4], w[4];
+  double f1[4], f2[4];
+  real d[4], w1[4], w2[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     int kk = k0 - 1 + q;
     kk = kk < -1 ? -1 : (kk > n + 2 ? n + 2 : kk);
     const int cq = g.idx(i, j, kk);
-    f[q] = fld[cq];
+    f1[q] = fld1[cq];
+    f2[q] = fld2[cq];
     d[q] = db1[cq];
-    w[q] = db1[cq + GS];
+    w1[q] = db1[cq + GS1];
+    w2[q] = db1[cq + GS2];
   }
   for (int k = k0; k <= k1; ++k) {
     const int c = g.idx(i, j, k);
-    const double v_advect = (vel[c] + vel[c - VS]) * 0.5;
-    const double dbm = (d[0] + w[0]) * 0.5;
-    const double db = (d[1] + w[1]) * 0.5;
-    const double dbp = (d[2] + w[2]) * 0.5;
-    const double dbp2 = (d[3] + w[3]) * 0.5;
     // dxc1 at the clamped s-1 (its upper point is s, or s+1 when s = -1),
     // at s, and at s+1 (upper point clamped to n+2, as d[3] already is)
     const double dcm = 0.5 * (d[0] + (k > -1 ? d[1] : d[2]));
     const double dci = 0.5 * (d[1] + d[2]);
     const double dcp = 0.5 * (d[2] + d[3]);
-    F[c] = ct_flux_core(v_advect, f[0], f[1], f[2], f[3], dbm, db, dbp, dbp2, d[1], d[2], dcm, dci, dcp, dt);
+    {
+      const double v_advect = (vel[c] + vel[c - VS1]) * 0.5;
+      F1[c] = ct_flux_core(v_advect, f1[0], f1[1], f1[2], f1[3], (d[0] + w1[0]) * 0.5, (d[1] + w1[1]) * 0.5,
+                           (d[2] + w1[2]) * 0.5, (d[3] + w1[3]) * 0.5, d[1], d[2], dcm, dci, dcp, dt);
+    }
+    {
+      const double v_advect = (vel[c] + vel[c - VS2]) * 0.5;
+      F2[c] = ct_flux_core(v_advect, f2[0], f2[1], f2[2], f2[3], (d[0] + w2[0]) * 0.5, (d[1] + w2[1]) * 0.5,
+                           (d[2] + w2[2]) * 0.5, (d[3] + w2[3]) * 0.5, d[1], d[2], dcm, dci, dcp, dt);
+    }
     int kn = k + 3;
     kn = kn > n + 2 ? n + 2 : kn;
     const int cn = g.idx(i, j, kn);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      f[q] = f[q + 1];
+      f1[q] = f1[q + 1];
+      f2[q] = f2[q + 1];
       d[q] = d[q + 1];
-      w[q] = w[q + 1];
+      w1[q] = w1[q + 1];
+      w2[q] = w2[q + 1];
     }
-    f[3] = fld[cn];
+    f1[3] = fld1[cn];
+    f2[3] = fld2[cn];
     d[3] = db1[cn];
-    w[3] = db1[cn + GS];
+    w1[3] = db1[cn + GS1];
+    w2[3] = db1[cn + GS2];
   }
 }
 
-// CT update pair: t = t - F + F(c - TS) on box bt; a = a + F - F(c - AS) on ba.
-__global__ __launch_bounds__(BX * BY) void k_ct_update(Geo g, Box b, const double *__restrict__ F, double *__restrict__ t,
-                            double *__restrict__ a, int TS, int AS, Box bt, Box ba) {
+// CT updates of a sweep along AX (xremap.f90:121-157 and analogues), both
+// pairs in the reference's order:
+//   t1 = t1 - F1 + F1(c - TS) on bt1;  a = a + F1 - F1(c - AS1) on ba1;
+//   t2 = t2 - F2 + F2(c - TS) on bt2;  a = a + F2 - F2(c - AS2) on ba2.
+__global__ __launch_bounds__(BX * BY) void k_ct_update(Geo g, Box b, const double *__restrict__ F1,
+                                                       const double *__restrict__ F2, double *__restrict__ t1,
+                                                       double *__restrict__ t2, double *__restrict__ a, int TS,
+                                                       int AS1, int AS2, Box bt1, Box ba1, Box bt2, Box ba2) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
-  if (i >= bt.x0 && i <= bt.x1 && j >= bt.y0 && j <= bt.y1 && k >= bt.z0 && k <= bt.z1)
-    t[c] = t[c] - F[c] + F[c - TS];
-  if (i >= ba.x0 && i <= ba.x1 && j >= ba.y0 && j <= ba.y1 && k >= ba.z0 && k <= ba.z1)
-    a[c] = a[c] + F[c] - F[c - AS];
+  const bool u1 = i >= bt1.x0 && i <= bt1.x1 && j >= bt1.y0 && j <= bt1.y1 && k >= bt1.z0 && k <= bt1.z1;
+  const bool w1 = i >= ba1.x0 && i <= ba1.x1 && j >= ba1.y0 && j <= ba1.y1 && k >= ba1.z0 && k <= ba1.z1;
+  const bool u2 = i >= bt2.x0 && i <= bt2.x1 && j >= bt2.y0 && j <= bt2.y1 && k >= bt2.z0 && k <= bt2.z1;
+  const bool w2 = i >= ba2.x0 && i <= ba2.x1 && j >= ba2.y0 && j <= ba2.y1 && k >= ba2.z0 && k <= ba2.z1;
+  if (u1) t1[c] = t1[c] - F1[c] + F1[c - TS];
+  if (u2) t2[c] = t2[c] - F2[c] + F2[c - TS];
+  if (w1 || w2) {
+    double va = a[c];
+    if (w1) va = va + F1[c] - F1[c - AS1];
+    if (w2) va = va + F2[c] - F2[c - AS2];
+    a[c] = va;
+  }
 }
 
 // Face-averaged sweep velocity of the mass/energy fluxes (xremap.f90:492-493)

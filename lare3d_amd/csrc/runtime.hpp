@@ -51,7 +51,7 @@ struct Dev {
   real *curlb, *qx, *qy, *qz;
   // remap scratch: F is the CT face flux (FP64), Fe the energy flux
   real *cv1, *cv2, *db1, *Fe, *dm, *Fx, *Fy, *Fz, *rho_vr, *cv1v, *cv2v, *dmv;
-  double *F;
+  double *F, *F2;
   // conduction scratch
   real *larsen, *Lc, *Yb[3];
   // reductions
@@ -370,17 +370,15 @@ int ct_pair(double *bxa, double *bya, double *bza, const real *vel, double dt) {
     t1 = 0; va1 = 1; ga1 = 0;  // vz_bx_flux
     t2 = 1; va2 = 0; ga2 = 1;  // vz_by_flux
   }
-  for (int pair = 0; pair < 2; ++pair) {
-    const int t = pair ? t2 : t1, va = pair ? va2 : va1, ga = pair ? ga2 : ga1;
-    if (AX == 2) {
-      LAUNCH("ct_flux", k_ct_flux_z, zchunk_grid(fb), box_block(), S.g, fb, B[t], vel, S.db1, S.F, strides(va),
-             strides(ga), dt);
-    } else {
-      LAUNCH_BOX("ct_flux", k_ct_flux<AX>, fb, B[t], vel, S.db1, S.F, strides(va), strides(ga), dt);
-    }
-    LAUNCH_BOX("ct_update", k_ct_update, ub, S.F, B[t], B[AX], strides(AX), strides(t), pair ? bt2 : bt1,
-               pair ? ba2 : ba1);
+  if (AX == 2) {
+    LAUNCH("ct_flux", k_ct_flux_z, zchunk_grid(fb), box_block(), S.g, fb, B[t1], B[t2], vel, S.db1, S.F, S.F2,
+           strides(va1), strides(ga1), strides(va2), strides(ga2), dt);
+  } else {
+    LAUNCH_BOX("ct_flux", k_ct_flux<AX>, fb, B[t1], B[t2], vel, S.db1, S.F, S.F2, strides(va1), strides(ga1),
+               strides(va2), strides(ga2), dt);
   }
+  LAUNCH_BOX("ct_update", k_ct_update, ub, S.F, S.F2, B[t1], B[t2], B[AX], strides(AX), strides(t1), strides(t2),
+             bt1, ba1, bt2, ba2);
   return 0;
 }
 
@@ -532,16 +530,23 @@ int sur_conduct_step(int delta_mode, double *res_out, int *acc_out);
 int lagrangian_step() {
   const int N0 = nx(), N1 = ny(), N2 = nz();
   LagArgs a = lag_args();  // dt = previous step's dt (Q1)
-  LAUNCH_BOX("vertex_prep", k_vertex_prep, mkbox(-1, N0 + 1, -1, N1 + 1, -1, N2 + 1), S.rho, S.energy, S.bx, S.by,
-             S.bz, S.rho_v, S.cs_v, S.p.gamma);
+  {
+    const Box vb = mkbox(-1, N0 + 1, -1, N1 + 1, -1, N2 + 1);
+    LAUNCH("vertex_prep", k_vertex_prep, zchunk_grid(vb), box_block(), S.g, vb, S.rho, S.energy, S.bx, S.by, S.bz,
+           S.rho_v, S.cs_v, S.p.gamma);
+  }
   if (S.p.use_viscous_damping) {
     LAUNCH_BOX("damp_q", k_damp_q, nodes0n(), S.vx, S.vy, S.vz, S.visc3, S.qx, S.qy, S.qz);
     LAUNCH_BOX("damp_apply", k_damp_apply, nodes0n(), S.vx, S.vy, S.vz, S.qx, S.qy, S.qz, S.dt);
   }
-  LAUNCH_BOX("alpha", k_alpha, mkbox(-1, N0 + 1, -1, N1 + 2, 0, N2 + 2), S.vx, S.vy, S.vz, S.rho_v, S.cs_v, S.al1,
-             S.al2, S.al3, a);
-  dim3 gr = grid_from0(N0 + 1, N1 + 1, N2 + 1);
-  LAUNCH("visc_dt", k_visc_dt, gr, box_block(), S.g, S.vx, S.vy, S.vz, S.rho, S.energy, S.bx, S.by, S.bz, S.eta,
+  {
+    const Box ab = mkbox(-1, N0 + 1, -1, N1 + 2, 0, N2 + 2);
+    LAUNCH("alpha", k_alpha, zchunk_grid(ab), box_block(), S.g, ab, S.vx, S.vy, S.vz, S.rho_v, S.cs_v, S.al1, S.al2,
+           S.al3, a);
+  }
+  const Box vdb = mkbox(0, N0 + 1, 0, N1 + 1, 0, N2 + 1);
+  dim3 gr = zchunk_grid(vdb);
+  LAUNCH("visc_dt", k_visc_dt, gr, box_block(), S.g, vdb, S.vx, S.vy, S.vz, S.rho, S.energy, S.bx, S.by, S.bz, S.eta,
          S.al1, S.al2, S.al3, S.visc_heat, S.p0, S.partial, a);
   int rc = final_reduce(gr, 2, 3, 0, 0);
   if (rc) return rc;
@@ -602,16 +607,16 @@ int lagrangian_step() {
   if (rc) return rc;
   LAUNCH_BOX("pred_pressure", k_pred_pressure, mkbox(0, N0 + 1, 0, N1 + 1, 0, N2 + 1), S.energy, S.rho, S.cv1,
              S.visc_heat, S.p0, S.pp, S.p.gamma, S.dt2);
-  LAUNCH_BOX("force", k_force, nodes0n(), S.vx, S.vy, S.vz, S.pp, S.bpx, S.bpy, S.bpz, S.rho_v, S.al1, S.al2, S.al3,
-             S.vx1, S.vy1, S.vz1, S.dt2);
+  LAUNCH("force", k_force, zchunk_grid(nodes0n()), box_block(), S.g, nodes0n(), S.vx, S.vy, S.vz, S.pp, S.bpx, S.bpy,
+         S.bpz, S.rho_v, S.al1, S.al2, S.al3, S.vx1, S.vy1, S.vz1, S.dt2);
   {
     BcField f[3] = {{S.vx1, FK_NODE}, {S.vy1, FK_NODE}, {S.vz1, FK_NODE}};
     rc = apply_bcs(f, 3, "bc_v1");
     if (rc) return rc;
   }
-  dim3 g3 = box_grid(interior());
-  LAUNCH("shock_corr", k_shock_corr, g3, box_block(), S.g, S.vx, S.vy, S.vz, S.vx1, S.vy1, S.vz1, S.al1, S.al2,
-         S.al3, S.pp, S.energy, S.rho, S.visc_dep, S.partial, S.dt);
+  dim3 g3 = zchunk_grid(interior());
+  LAUNCH("shock_corr", k_shock_corr, g3, box_block(), S.g, interior(), S.vx, S.vy, S.vz, S.vx1, S.vy1, S.vz1, S.al1,
+         S.al2, S.al3, S.pp, S.energy, S.rho, S.visc_dep, S.partial, S.dt);
   rc = final_reduce(g3, 1, 0, 8, 1);
   if (rc) return rc;
   LAUNCH_BOX("vupdate", k_vupdate, nodes0n(), S.vx, S.vy, S.vz, S.vx1, S.vy1, S.vz1);
@@ -761,7 +766,7 @@ int api_init(const lare_grid *grid, const lare_params *params) {
     int rc = dalloc(pp);
     if (rc) return rc;
   }
-  double **wide[] = {&S.bx, &S.by, &S.bz, &S.bpx, &S.bpy, &S.bpz, &S.fxr, &S.fyr, &S.fzr, &S.F};
+  double **wide[] = {&S.bx, &S.by, &S.bz, &S.bpx, &S.bpy, &S.bpz, &S.fxr, &S.fyr, &S.fzr, &S.F, &S.F2};
   for (double **pp : wide) {
     int rc = dalloc(pp);
     if (rc) return rc;
