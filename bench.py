@@ -176,6 +176,99 @@ def fp32_run(n, warmup, steps, f64_end):
             "field_rel_l2_vs_fp64": errs, "steps_compared": warmup + steps, "max_divb_dx": divb}
 
 
+def conduction_leg(n, seed=1, kappa_0=0.5, s_target=7, rows_per_state=17000, train_steps=6000):
+    """Braginskii conduction super-steps on the Harris sheet with seeded hot
+    spots (n^3): the plain RKL2 solver against the learned surrogate, in FP64
+    and FP32 storage.  The surrogate is trained here, on the device, from
+    three seeded states (rows and RKL2 targets collected by the library),
+    and evaluated on a fourth.  Reports super-step times, the temperature
+    error of the surrogate against RKL2, and its residual-check outcome."""
+    from lare3d_amd import configs
+    from lare3d_amd.abi import PREC_FP32
+    from lare3d_amd.device import Device
+    from lare3d_amd.surrogate import Surrogate, initial_params
+
+    def state(sd):
+        return configs.harris_conduction(n, seed=sd, s_target=s_target, kappa_0=kappa_0)
+
+    grid, params, f0, dt0 = state(seed)
+    held = state(seed + 99)
+    out = {"workload": f"resistive Harris sheet {n}^3 with Braginskii conduction (kappa_0 = {kappa_0}, "
+                       f"12 seeded hot spots), one conduction super-step of about {s_target} RKL2 stages"}
+    res = {}
+    for prec in ("fp64", "fp32"):
+        params.precision = PREC_FP32 if prec == "fp32" else 0
+        dev = Device(grid, params, f0)
+        sur = Surrogate(dev)
+        if prec == "fp64":
+            sur.set_params(initial_params(0))
+            t0 = time.perf_counter()
+            for q in range(3):
+                g_, p_, f_, dt_ = state(seed + 1 + q) if q else (grid, params, f0, dt0)
+                dev.upload(f_)
+                dev.boundary_conditions()
+                dev.set_scalars(dt=dt_)
+                sur.collect(rows_per_state, 100 + q)
+            sur.fit_norm()
+            t_collect = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            # Adam with a stepped learning rate; the moments carry over
+            loss = np.concatenate([sur.train(train_steps // 3, batch=512, lr=lr, seed=7 + q)
+                                   for q, lr in enumerate((1e-3, 3e-4, 1e-4))])
+            t_train = time.perf_counter() - t0
+            trained = (sur.params(), sur.norm())
+            out["training"] = {"rows": 3 * rows_per_state, "steps": int(loss.size), "batch": 512,
+                               "learning_rate": "1e-3, 3e-4, 1e-4 (thirds)",
+                               "collect_s": round(t_collect, 2), "train_s": round(t_train, 2),
+                               "loss_first50": float(loss[:50].mean()), "loss_last50": float(loss[-50:].mean())}
+        else:
+            sur.set_params(trained[0])
+            sur.set_norm(*trained[1])
+        _, _, fh, dth = held
+
+        def load():
+            dev.upload(fh)
+            dev.boundary_conditions()
+            dev.set_scalars(dt=dth)
+
+        def plain():
+            load()
+            dev.synchronize()
+            t0 = time.perf_counter()
+            dev.conduct_heat()
+            dev.synchronize()
+            return time.perf_counter() - t0
+
+        def surrogate(tol):
+            load()
+            sur.enable(False, tol)
+            dev.synchronize()
+            t0 = time.perf_counter()
+            r, acc = sur.conduct()
+            dev.synchronize()
+            return time.perf_counter() - t0, r, acc
+
+        tp = min(plain() for _ in range(3)) * 1e3
+        stages = dev.scalars().n_s_stages
+        e_rkl2 = dev.fields().interior("energy")
+        ts = min(surrogate(1e30)[0] for _ in range(3)) * 1e3
+        _, r_forced, _ = surrogate(1e30)
+        e_sur = dev.fields().interior("energy")
+        _, r_tol, acc_tol = surrogate(1e-4)
+        e0 = fh.interior("energy")
+        res[prec] = {"rkl2_ms": round(tp, 3), "rkl2_stages": stages, "surrogate_ms": round(ts, 3),
+                     "surrogate_speedup": round(tp / ts, 2),
+                     "T_rel_l2_vs_rkl2": float(np.linalg.norm(e_sur - e_rkl2) / np.linalg.norm(e_rkl2)),
+                     "dT_rel_l2_vs_rkl2": float(np.linalg.norm((e_sur - e0) - (e_rkl2 - e0)) /
+                                                np.linalg.norm(e_rkl2 - e0)),
+                     "residual": r_forced, "accepted_at_1e-4": bool(acc_tol),
+                     "cell_updates_per_s_rkl2": n ** 3 / (tp * 1e-3),
+                     "cell_updates_per_s_surrogate": n ** 3 / (ts * 1e-3)}
+        dev.close()
+    out.update(res)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +280,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage run")
+    ap.add_argument("--no-conduction", action="store_true",
+                    help="skip the conduction leg (RKL2 vs the learned surrogate)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy and exit (counter calibration for HBM traffic; "
                          "run it as its own process under the same counters)")
@@ -238,6 +333,8 @@ def main():
         fp32 = fp32_run(n, args.warmup, args.steps, f64_end)
         fp32["speedup_vs_fp64"] = fp32["value"] / cups
 
+    cond = None if args.no_conduction else conduction_leg(n)
+
     cpu = None
     if not args.no_cpu:
         rate, secs = cpu_baseline(args.cpu_n, args.cpu_steps)
@@ -266,6 +363,7 @@ def main():
         "step_roofline": step_roof,
         "cpu_baseline": cpu,
         "fp32": fp32,
+        "conduction": cond,
         "kernels": top,
         "diagnostics": {"energy": en, "max_divb_dx": divb, "dt": sc.dt},
     }

@@ -121,36 +121,56 @@ __device__ __forceinline__ double ct_flux_core(double v_advect, double frm, doub
   return (fu + Di * (1.0 - phi)) * v_advect * dt;
 }
 
-// CT fluxes of the two transverse face fields of a sweep along AX
-// (vx_by_flux and vx_bz_flux etc.) in one pass.  VS: stride of the axis the
-// velocity is averaged along (v(c) + v(c-VS)); GS: stride of the axis d?b1 is
-// averaged along (d(c) + d(c+GS)); one pair of strides per field.
+// CT fluxes of the two transverse face fields of an x or y sweep (AX = 0, 1:
+// vx_by_flux and vx_bz_flux, vy_bx_flux and vy_bz_flux) in one pass.  The
+// first field's velocity is averaged along z (VS1 = z stride) and the
+// second's widths along z (GS2 = z stride), so the kernel marches up
+// z-columns keeping v of plane k-1 and db1 of plane k+1 at the four stencil
+// points for the next plane.  GS1 and VS2 are in-plane strides.
 template <int AX>
 __global__ __launch_bounds__(BX * BY) void k_ct_flux(Geo g, Box b, const double *__restrict__ fld1,
                                                      const double *__restrict__ fld2, const real *__restrict__ vel,
                                                      const real *__restrict__ db1, double *__restrict__ F1,
-                                                     double *__restrict__ F2, int VS1, int GS1, int VS2, int GS2,
-                                                     double dt) {
-  BOX_INDEX(b, i, j, k);
-  const int c = g.idx(i, j, k);
+                                                     double *__restrict__ F2, int GS1, int VS2, double dt) {
+  ZCOLUMN(b, i, j, k0, k1);
   const int S = sstride<AX>(g);
+  const int Z = g.sz;
   const int n = g.n(AX);
-  const int s = coord<AX>(i, j, k);
+  const int s = coord<AX>(i, j, 0);
   const int sm = s - 1 > -1 ? s - 1 : -1;
   const int sp2 = s + 2 < n + 2 ? s + 2 : n + 2;
-  const int cm = c + (sm - s) * S, cp = c + S, cp2 = c + (sp2 - s) * S;
-  const double dcm = dc1<AX>(g, db1, cm, sm), dci = dc1<AX>(g, db1, c, s), dcp = dc1<AX>(g, db1, cp, s + 1);
-#pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const double *fld = q ? fld2 : fld1;
-    const int VS = q ? VS2 : VS1, GS = q ? GS2 : GS1;
-    const double v_advect = (vel[c] + vel[c - VS]) * 0.5;
-    const double db = (db1[c] + db1[c + GS]) * 0.5;
-    const double dbp = (db1[cp] + db1[cp + GS]) * 0.5;
-    const double dbp2 = (db1[cp2] + db1[cp2 + GS]) * 0.5;
-    const double dbm = (db1[cm] + db1[cm + GS]) * 0.5;
-    (q ? F2 : F1)[c] = ct_flux_core(v_advect, fld[cm], fld[c], fld[cp], fld[cp2], dbm, db, dbp, dbp2, db1[c],
-                                    db1[cp], dcm, dci, dcp, dt);
+  const int om = (sm - s) * S, op2 = (sp2 - s) * S;  // offsets of the clamped s-1 and s+2 points
+  // stencil points along AX: m (s-1 clamped), i (s), p (s+1), q (s+2 clamped)
+  const int cb = g.idx(i, j, k0);
+  real vm = vel[cb - Z];                          // v(c - Z) for the first field
+  real dz[4] = {db1[cb + om + Z], db1[cb + Z], db1[cb + S + Z], db1[cb + op2 + Z]};  // db1(. + Z) for the second
+  for (int k = k0; k <= k1; ++k) {
+    const int c = g.idx(i, j, k);
+    const int cm = c + om, cp = c + S, cp2 = c + op2;
+    const real d0 = db1[cm], d1 = db1[c], d2 = db1[cp], d3 = db1[cp2];
+    const double dcm = 0.5 * (d0 + db1[sm + 1 <= n + 2 ? cm + S : cm]);
+    const double dci = 0.5 * (d1 + db1[s + 1 <= n + 2 ? c + S : c]);
+    const double dcp = 0.5 * (d2 + db1[s + 2 <= n + 2 ? cp + S : cp]);
+    const real vc = vel[c];
+    {
+      const double v_advect = (vc + vm) * 0.5;
+      F1[c] = ct_flux_core(v_advect, fld1[cm], fld1[c], fld1[cp], fld1[cp2], (d0 + db1[cm + GS1]) * 0.5,
+                           (d1 + db1[c + GS1]) * 0.5, (d2 + db1[cp + GS1]) * 0.5, (d3 + db1[cp2 + GS1]) * 0.5, d1,
+                           d2, dcm, dci, dcp, dt);
+    }
+    {
+      const double v_advect = (vc + vel[c - VS2]) * 0.5;
+      F2[c] = ct_flux_core(v_advect, fld2[cm], fld2[c], fld2[cp], fld2[cp2], (d0 + dz[0]) * 0.5, (d1 + dz[1]) * 0.5,
+                           (d2 + dz[2]) * 0.5, (d3 + dz[3]) * 0.5, d1, d2, dcm, dci, dcp, dt);
+    }
+    vm = vc;
+    if (k < k1) {
+      const int cz = c + 2 * Z;
+      dz[0] = db1[cz + om];
+      dz[1] = db1[cz];
+      dz[2] = db1[cz + S];
+      dz[3] = db1[cz + op2];
+    }
   }
 }
 

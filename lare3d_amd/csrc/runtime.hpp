@@ -370,12 +370,13 @@ int ct_pair(double *bxa, double *bya, double *bza, const real *vel, double dt) {
     t1 = 0; va1 = 1; ga1 = 0;  // vz_bx_flux
     t2 = 1; va2 = 0; ga2 = 1;  // vz_by_flux
   }
-  if (AX == 2) {
+  if constexpr (AX == 2) {
     LAUNCH("ct_flux", k_ct_flux_z, zchunk_grid(fb), box_block(), S.g, fb, B[t1], B[t2], vel, S.db1, S.F, S.F2,
            strides(va1), strides(ga1), strides(va2), strides(ga2), dt);
   } else {
-    LAUNCH_BOX("ct_flux", k_ct_flux<AX>, fb, B[t1], B[t2], vel, S.db1, S.F, S.F2, strides(va1), strides(ga1),
-               strides(va2), strides(ga2), dt);
+    // x and y sweeps: va1 = z and ga2 = z (see the flux table above)
+    LAUNCH("ct_flux", k_ct_flux<AX>, zchunk_grid(fb), box_block(), S.g, fb, B[t1], B[t2], vel, S.db1, S.F, S.F2,
+           strides(ga1), strides(va2), dt);
   }
   LAUNCH_BOX("ct_update", k_ct_update, ub, S.F, S.F2, B[t1], B[t2], B[AX], strides(AX), strides(t1), strides(t2),
              bt1, ba1, bt2, ba2);
@@ -1292,8 +1293,8 @@ int api_sur_collect(int32_t count, uint64_t seed, int32_t *total) {
   rc = sur_features(0);
   if (rc) return rc;
   int *cells = SU.cells + SU.rows;
-  LAUNCH("sur_sample", k_sur_sample_cells, dim3(blocks_for(count)), dim3(256), S.g, (unsigned long long)seed, count,
-         cells);
+  LAUNCH("sur_sample", k_sur_sample_cells, dim3(blocks_for(count)), dim3(256), S.g, SU.feat, (unsigned long long)seed,
+         count, cells);
   LAUNCH("sur_gather", k_sur_gather, dim3(blocks_for((size_t)count * SUR_K)), dim3(256), S.g, SU.feat, cells, count,
          SU.X + (size_t)SU.rows * SUR_K);
   HIPCHK(hipMemcpyAsync(SU.E0save, S.energy, sizeof(real) * S.ntot, hipMemcpyDeviceToDevice, S.stream));

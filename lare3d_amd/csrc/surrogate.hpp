@@ -106,15 +106,39 @@ __host__ __device__ __forceinline__ unsigned long long mix64(unsigned long long 
   return x ^ (x >> 31);
 }
 
-// Random interior cells for rows [0, rows) of a sample; cells[r] = linear index.
-__global__ void k_sur_sample_cells(Geo g, unsigned long long seed, int rows, int *__restrict__ cells) {
-  const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (r >= rows) return;
-  const unsigned long long h = mix64(seed * 0x100000001B3ull + (unsigned long long)r);
+// Interior cells for rows [0, rows) of a sample (cells[r] = linear index),
+// drawn by hash of (seed, r).  Even rows are uniform; odd rows take the
+// most active of 8 uniform candidates, scored by the discrete Laplacian
+// |6 T - sum of the 6 face neighbours| of the raw temperature channel, so
+// that the cells where conduction acts are well represented.
+__device__ __forceinline__ int sample_cell(const Geo &g, unsigned long long h) {
   const int i = 1 + (int)((h & 0xFFFFFull) % (unsigned long long)g.nx);
   const int j = 1 + (int)(((h >> 20) & 0xFFFFFull) % (unsigned long long)g.ny);
   const int k = 1 + (int)(((h >> 40) & 0xFFFFFull) % (unsigned long long)g.nz);
-  cells[r] = g.idx(i, j, k);
+  return g.idx(i, j, k);
+}
+__global__ void k_sur_sample_cells(Geo g, const float *__restrict__ feat, unsigned long long seed, int rows,
+                                   int *__restrict__ cells) {
+  const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r >= rows) return;
+  const unsigned long long h = mix64(seed * 0x100000001B3ull + (unsigned long long)r);
+  int best = sample_cell(g, h);
+  if (r & 1) {
+    float score = -1.0f;
+    for (int t = 0; t < 8; ++t) {
+      const int c = sample_cell(g, mix64(h + (unsigned long long)(t + 1)));
+      const float T = feat[(size_t)c * SUR_CH];
+      const float lap = 6.0f * T - feat[(size_t)(c + 1) * SUR_CH] - feat[(size_t)(c - 1) * SUR_CH] -
+                        feat[(size_t)(c + g.sy) * SUR_CH] - feat[(size_t)(c - g.sy) * SUR_CH] -
+                        feat[(size_t)(c + g.sz) * SUR_CH] - feat[(size_t)(c - g.sz) * SUR_CH];
+      const float sc = fabsf(lap);
+      if (sc > score) {
+        score = sc;
+        best = c;
+      }
+    }
+  }
+  cells[r] = best;
 }
 
 // X[base + r][q * 12 + ch] = feat[cells[r] + off(q)][ch]
@@ -293,10 +317,10 @@ __global__ void k_sur_pack(const float *__restrict__ w, unsigned short *__restri
   }
 }
 
-#define SUR_INFER_WAVES 4
+#define SUR_INFER_WAVES 8
 // E1 = E0 + dT / tfac over the interior, 16 cells (consecutive in x) per
 // wave-group.  Features must be valid on cells 0..n+1.
-__global__ __launch_bounds__(64 * SUR_INFER_WAVES) void k_sur_infer(
+__global__ __launch_bounds__(64 * SUR_INFER_WAVES, 2) void k_sur_infer(
     Geo g, const float *__restrict__ feat, const unsigned short *__restrict__ w1t,
     const unsigned short *__restrict__ w2t, const float *__restrict__ w, const real *__restrict__ E0,
     real *__restrict__ E1, double tfac, float y_scale, int ngroups) {
@@ -322,21 +346,33 @@ __global__ __launch_bounds__(64 * SUR_INFER_WAVES) void k_sur_infer(
     const int i = 1 + gx * 16 + col;
     const int ic = i <= g.nx ? i : g.nx;  // clamp loads of the masked lanes
     const int c = g.idx(ic, j, k);
-    // layer 1: Z1^T (64 x 16 cells) = W1^T (64 x 336) . X^T (336 x 16)
+    // layer 1: Z1^T (64 x 16 cells) = W1^T (64 x 336) . X^T (336 x 16), in
+    // three chunks of 7 MFMA steps; the 7 feature gathers of a chunk are
+    // issued together before its MFMAs, so their latencies overlap.
     f32x4 acc[4];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int kk = 0; kk < SUR_K / 16; ++kk) {
-      const int kb = kk * 16 + kq;
-      const int q = kb / SUR_CH, ch = kb % SUR_CH;
-      float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (kb < SUR_KREAL) x = *(const float4 *)(feat + (size_t)(c + nb_offset(g, q)) * SUR_CH + ch);
-      const bf16x4 bop = pack4(x.x, x.y, x.z, x.w);
 #pragma unroll
-      for (int mt = 0; mt < 4; ++mt) {
-        const bf16x4 aop = *(const bf16x4 *)(sW1 + (mt * 16 + col) * SUR_KP + kb);
-        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aop, bop, acc[mt], 0, 0, 0);
+    for (int ck = 0; ck < 3; ++ck) {
+      bf16x4 bops[7];
+#pragma unroll
+      for (int u = 0; u < 7; ++u) {
+        const int kb = (ck * 7 + u) * 16 + kq;
+        const int q = kb / SUR_CH, ch = kb % SUR_CH;
+        float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (kb < SUR_KREAL) x = *(const float4 *)(feat + (size_t)(c + nb_offset(g, q)) * SUR_CH + ch);
+        bops[u] = pack4(x.x, x.y, x.z, x.w);
       }
+#pragma unroll
+      for (int u = 0; u < 7; ++u) {
+        const int kb = (ck * 7 + u) * 16 + kq;
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+          const bf16x4 aop = *(const bf16x4 *)(sW1 + (mt * 16 + col) * SUR_KP + kb);
+          acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aop, bops[u], acc[mt], 0, 0, 0);
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
     }
     bf16x4 h1[4];
 #pragma unroll

@@ -93,6 +93,33 @@ MODULE lare_gpu_bridge
     TYPE(C_PTR) FUNCTION lare_dev_last_error() BIND(C, NAME='lare_dev_last_error')
       IMPORT :: C_PTR
     END FUNCTION
+    ! learned conduction surrogate (no counterpart in the reference)
+    INTEGER(C_INT) FUNCTION lare_dev_sur_set_params(params, n) BIND(C, NAME='lare_dev_sur_set_params')
+      IMPORT :: C_INT, C_INT32_T, C_FLOAT
+      REAL(C_FLOAT), INTENT(IN) :: params(*)
+      INTEGER(C_INT32_T), VALUE :: n
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_sur_get_params(params, n) BIND(C, NAME='lare_dev_sur_get_params')
+      IMPORT :: C_INT, C_INT32_T, C_FLOAT
+      REAL(C_FLOAT), INTENT(OUT) :: params(*)
+      INTEGER(C_INT32_T), VALUE :: n
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_sur_set_norm(mu, sd, y_scale) BIND(C, NAME='lare_dev_sur_set_norm')
+      IMPORT :: C_INT, C_FLOAT
+      REAL(C_FLOAT), INTENT(IN) :: mu(*), sd(*)
+      REAL(C_FLOAT), VALUE :: y_scale
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_sur_enable(on, tol) BIND(C, NAME='lare_dev_sur_enable')
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE
+      INTEGER(C_INT32_T), VALUE :: on
+      REAL(C_DOUBLE), VALUE :: tol
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_sur_stats(accepted, rejected, last_residual) &
+        BIND(C, NAME='lare_dev_sur_stats')
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE
+      INTEGER(C_INT32_T), INTENT(OUT) :: accepted, rejected
+      REAL(C_DOUBLE), INTENT(OUT) :: last_residual
+    END FUNCTION
   END INTERFACE
 
 CONTAINS

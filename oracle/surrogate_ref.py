@@ -15,7 +15,8 @@ The surrogate has no counterpart in the reference.  What this module pins:
                    error (checked in the tests against central finite
                    differences in float64);
   * sample_cells(): the splitmix64 hash (Steele, Lea and Flood) the device
-                   uses to draw training cells.
+                   uses to draw training cells, and its activity-biased
+                   choice among candidates.
 """
 import math
 
@@ -126,12 +127,35 @@ def mix64(x):
     return x ^ (x >> 31)
 
 
-def sample_cells(seed, rows, n):
-    """Interior cells (i, j, k), 1-based, that the device draws for rows 0..rows-1."""
+def _cell_of(h, n):
+    return (1 + (h & 0xFFFFF) % n[0], 1 + ((h >> 20) & 0xFFFFF) % n[1], 1 + ((h >> 40) & 0xFFFFF) % n[2])
+
+
+def sample_cells(seed, rows, n, temperature=None):
+    """Interior cells (i, j, k), 1-based, that the device draws for rows
+    0..rows-1: even rows uniform; odd rows the candidate of 8 with the
+    largest |6 T - sum of the six face neighbours| (FP32 arithmetic on the
+    FP32 temperature channel; first maximum wins).  temperature(i, j, k)
+    returns that channel; without it only even rows are valid."""
     out = []
     for r in range(rows):
         h = mix64((seed * 0x100000001B3 + r) & _M64)
-        out.append((1 + (h & 0xFFFFF) % n[0], 1 + ((h >> 20) & 0xFFFFF) % n[1], 1 + ((h >> 40) & 0xFFFFF) % n[2]))
+        best = _cell_of(h, n)
+        if r & 1:
+            if temperature is None:
+                best = None
+            else:
+                score = np.float32(-1.0)
+                for t in range(8):
+                    i, j, k = _cell_of(mix64((h + t + 1) & _M64), n)
+                    lap = np.float32(6.0) * temperature(i, j, k)
+                    for (a, b, c) in ((i + 1, j, k), (i - 1, j, k), (i, j + 1, k), (i, j - 1, k), (i, j, k + 1),
+                                      (i, j, k - 1)):
+                        lap = np.float32(lap - temperature(a, b, c))
+                    sc = np.float32(abs(lap))
+                    if sc > score:
+                        score, best = sc, (i, j, k)
+        out.append(best)
     return out
 
 
