@@ -184,7 +184,8 @@ int lare_dev_sur_collect(int32_t count, uint64_t seed, int32_t *total);
  * the host (raw before lare_dev_sur_fit_norm, standardised after). */
 int lare_dev_sur_rows(float *X, float *y, int32_t rows);
 /* Standardisation from the collected rows (centre-cell channel statistics,
- * RMS target); standardises the held rows. */
+ * with the std of the log dt channel floored at 1; RMS target); standardises
+ * the held rows. */
 int lare_dev_sur_fit_norm(void);
 /* Adam (lr, beta 0.9 / 0.999) for `steps` mini-batches of `batch` rows
  * (a multiple of 16) drawn by hash of seed; loss_out[steps] receives the
@@ -201,6 +202,9 @@ int lare_dev_sur_enable(int32_t on, double tol);
  * surrogate form of lare_dev_conduct_heat).  *accepted = 0 means the
  * residual exceeded the tolerance and RKL2 did the step. */
 int lare_dev_sur_conduct(double *residual, int32_t *accepted);
+/* The same residual check applied to the plain RKL2 super-step from the
+ * current state (the state is left unchanged): the floor of the check. */
+int lare_dev_sur_plain_residual(double *residual);
 /* Counts of accepted and rejected surrogate steps, and the last residual. */
 int lare_dev_sur_stats(int32_t *accepted, int32_t *rejected, double *last_residual);
 

@@ -83,6 +83,7 @@ DEVICE_SYMBOLS = {
     "lare_dev_sur_grad": [_fp, _fp, C.c_int32, _fp, _fp],
     "lare_dev_sur_enable": [C.c_int32, C.c_double],
     "lare_dev_sur_conduct": [_dp, C.POINTER(C.c_int32)],
+    "lare_dev_sur_plain_residual": [_dp],
     "lare_dev_sur_stats": [C.POINTER(C.c_int32), C.POINTER(C.c_int32), _dp],
     "lare_dev_synchronize": [],
     "lare_dev_finalize": [],

@@ -119,6 +119,7 @@ int lare_dev_sur_grad(const float *X, const float *t, int32_t rows, float *grad,
 }
 int lare_dev_sur_enable(int32_t on, double tol) { return FWD(api_sur_enable(on, tol)); }
 int lare_dev_sur_conduct(double *residual, int32_t *accepted) { return FWD(api_sur_conduct(residual, accepted)); }
+int lare_dev_sur_plain_residual(double *residual) { return FWD(api_sur_plain_residual(residual)); }
 int lare_dev_sur_stats(int32_t *accepted, int32_t *rejected, double *last_residual) {
   return FWD(api_sur_stats(accepted, rejected, last_residual));
 }

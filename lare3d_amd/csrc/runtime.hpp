@@ -1166,6 +1166,27 @@ int sur_features(int normalise) {
   return 0;
 }
 
+// Trapezoidal residual |E1 - E0 - dt/2 (L(E0) + L(E1))| / |E0| (L2 over the
+// interior) of a candidate super-step E1 from E0 = S.energy, using the
+// Larsen factor of the last calc_s_stages.  The stage kernels with all
+// history weights zero evaluate the trapezoid into SU.trap.
+int sur_residual(const real *E1, double *res) {
+  CondArgs ca = {S.p.kappa_0, S.tfac, S.dt};
+  Box in = interior();
+  LAUNCH_BOX("rkl2_stage1", k_rkl2_stage1, in, S.energy, S.rho, S.bx, S.by, S.bz, S.larsen, S.Lc, S.Yb[0], ca, 0.0);
+  LAUNCH_BOX("rkl2_stage", k_rkl2_stage, in, E1, E1, S.energy, S.rho, S.bx, S.by, S.bz, S.larsen, S.Lc, SU.trap, ca,
+             TMODE_WRAP, 0.0, 0.0, 0.0, 0.5, 0.5);
+  dim3 gr = box_grid(interior());
+  LAUNCH("sur_residual", k_sur_residual, gr, box_block(), S.g, S.energy, E1, SU.trap, S.partial);
+  int rc = final_reduce(gr, 2, 0, 0, 0);
+  if (rc) return rc;
+  double v[2];
+  rc = read_scalars(0, 2, v);
+  if (rc) return rc;
+  *res = v[1] > 0.0 ? sqrt(v[0] / v[1]) : 0.0;
+  return 0;
+}
+
 // One surrogate super-step of length S.dt on S.energy (stages and Larsen
 // factor from calc_s_stages), checked and, if rejected, redone by RKL2.
 int sur_conduct_step(int delta_mode, double *res_out, int *acc_out) {
@@ -1179,20 +1200,9 @@ int sur_conduct_step(int delta_mode, double *res_out, int *acc_out) {
   const unsigned nblk = (unsigned)((ngroups + SUR_INFER_WAVES - 1) / SUR_INFER_WAVES);
   LAUNCH("sur_infer", k_sur_infer, dim3(nblk < 2048u ? nblk : 2048u), dim3(64 * SUR_INFER_WAVES), S.g, SU.feat,
          SU.w1t, SU.w2t, SU.w, S.energy, SU.E1, S.tfac, SU.nm.y_scale, ngroups);
-  // trapezoid dt/2 (L(E0) + L(E1)): stage kernels with all history weights zero
-  CondArgs ca = {S.p.kappa_0, S.tfac, S.dt};
-  Box in = interior();
-  LAUNCH_BOX("rkl2_stage1", k_rkl2_stage1, in, S.energy, S.rho, S.bx, S.by, S.bz, S.larsen, S.Lc, S.Yb[0], ca, 0.0);
-  LAUNCH_BOX("rkl2_stage", k_rkl2_stage, in, SU.E1, SU.E1, S.energy, S.rho, S.bx, S.by, S.bz, S.larsen, S.Lc,
-             SU.trap, ca, TMODE_WRAP, 0.0, 0.0, 0.0, 0.5, 0.5);
-  dim3 gr = box_grid(interior());
-  LAUNCH("sur_residual", k_sur_residual, gr, box_block(), S.g, S.energy, SU.E1, SU.trap, S.partial);
-  rc = final_reduce(gr, 2, 0, 0, 0);
+  double r;
+  rc = sur_residual(SU.E1, &r);
   if (rc) return rc;
-  double v[2];
-  rc = read_scalars(0, 2, v);
-  if (rc) return rc;
-  const double r = v[1] > 0.0 ? sqrt(v[0] / v[1]) : 0.0;
   SU.last_res = r;
   const int ok = (r == r) && r <= SU.tol;
   if (res_out) *res_out = r;
@@ -1334,7 +1344,10 @@ int api_sur_fit_norm(void) {
     }
     const double mu = s / SU.rows;
     double var = s2 / SU.rows - mu * mu;
-    const double sd = var > 1.0e-12 * (1.0 + mu * mu) ? sqrt(var) : 1.0;
+    double sd = var > 1.0e-12 * (1.0 + mu * mu) ? sqrt(var) : 1.0;
+    // log dt (channel 5) is nearly constant within one training run; a floor
+    // of 1 keeps a step with a somewhat different dt inside the input range
+    if (ch == 5 && sd < 1.0) sd = 1.0;
     SU.nm.mu[ch] = ch < SUR_NCH ? (float)mu : 0.0f;
     SU.nm.sd[ch] = ch < SUR_NCH ? (float)sd : 1.0f;
   }
@@ -1438,6 +1451,21 @@ int api_sur_conduct(double *residual, int32_t *accepted) {
   int rc = sur_conduct_step(0, residual, &acc);
   if (accepted) *accepted = acc;
   return rc;
+}
+
+int api_sur_plain_residual(double *residual) {
+  REQUIRE_LOADED();
+  if (S.p.kappa_0 <= 0.0) return fail(LARE_E_ARG, "plain residual needs kappa_0 > 0");
+  if (!residual) return fail(LARE_E_ARG, "plain residual: NULL output");
+  int rc = sur_alloc_fields();
+  if (!rc) rc = calc_s_stages();
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(SU.E0save, S.energy, sizeof(real) * S.ntot, hipMemcpyDeviceToDevice, S.stream));
+  rc = heat_conduct_sts2(0);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(SU.E1, S.energy, sizeof(real) * S.ntot, hipMemcpyDeviceToDevice, S.stream));
+  HIPCHK(hipMemcpyAsync(S.energy, SU.E0save, sizeof(real) * S.ntot, hipMemcpyDeviceToDevice, S.stream));
+  return sur_residual(SU.E1, residual);
 }
 
 int api_sur_stats(int32_t *accepted, int32_t *rejected, double *last_residual) {

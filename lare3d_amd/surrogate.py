@@ -119,6 +119,13 @@ class Surrogate:
         check(self.lib, self.lib.lare_dev_sur_conduct(C.byref(r), C.byref(a)), "sur_conduct")
         return r.value, bool(a.value)
 
+    def plain_residual(self):
+        """The residual check applied to the plain RKL2 super-step from the
+        current state (state unchanged): the floor of the check."""
+        r = C.c_double()
+        check(self.lib, self.lib.lare_dev_sur_plain_residual(C.byref(r)), "sur_plain_residual")
+        return r.value
+
     def stats(self):
         a, rj = C.c_int32(), C.c_int32()
         r = C.c_double()

@@ -285,6 +285,14 @@ def test_training_gain_and_fallback():
         assert err_trained < 0.5 * err_untrained
         assert res_t < res_u
 
+        dev.upload(fh)
+        dev.boundary_conditions()
+        dev.set_scalars(dt=dth)
+        e_before = dev.fields().energy
+        r_plain = sur.plain_residual()
+        print(f"residual check applied to the plain RKL2 step: {r_plain:.3e}")
+        assert r_plain < res_t
+        assert np.array_equal(dev.fields().energy, e_before)  # state unchanged
         # tolerance below the residual: the step falls back to RKL2 exactly
         res_f, acc_f, f_f = sur_dT(0.5 * res_t)
         assert not acc_f
