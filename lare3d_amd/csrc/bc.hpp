@@ -104,3 +104,69 @@ __global__ __launch_bounds__(64) void k_bc_axis(Geo g, BcSet s, int axis) {
     else bc_line((float *)s.f[q], s.op[q], base, st, n, dm_ok);
   }
 }
+
+// The x-axis fill with the work spread over the ghost slots of each line:
+// thread (line, slot) does the slot-th (destination <- source) copy of the
+// operation, slots 0..7, lines over the full (y, z) extent -2..n+2.  The
+// copies of one operation touch disjoint destination and source layers
+// (n >= 4), so doing them in parallel gives the sequential result, and the
+// eight lanes of a line share its two end cache lines.
+template <class T>
+__device__ __forceinline__ void bc_slot(T *a, int op, int base, int n, int slot, bool dm_ok) {
+  int dst = 0, src = 0;
+  bool zero = false, act = false;
+  switch (op) {
+    case BC_PER_CELL: {
+      const int d[4] = {n + 1, n + 2, -1, 0}, r[4] = {1, 2, n - 1, n};
+      if (slot < 4) { dst = d[slot]; src = r[slot]; act = true; }
+      break;
+    }
+    case BC_PER_FACE: {
+      const int d[5] = {n + 1, n + 2, -2, -1, 0}, r[5] = {1, 2, n - 2, n - 1, n};
+      if (slot < 5) { dst = d[slot]; src = r[slot]; act = true; }
+      break;
+    }
+    case BC_PER_NODE: {
+      const int d[5] = {n, n + 1, n + 2, -2, -1}, r[5] = {0, 1, 2, n - 2, n - 1};
+      if (slot < 5) { dst = d[slot]; src = r[slot]; act = true; }
+      break;
+    }
+    case BC_MIR_CELL: {
+      const int d[4] = {0, -1, n + 1, n + 2}, r[4] = {1, 2, n, n - 1};
+      if (slot < 4) { dst = d[slot]; src = r[slot]; act = true; }
+      break;
+    }
+    case BC_MIR_FACE: {
+      const int d[4] = {-1, -2, n + 1, n + 2}, r[4] = {1, 2, n - 1, n - 2};
+      if (slot < 4) { dst = d[slot]; src = r[slot]; act = true; }
+      break;
+    }
+    case BC_ZERO_NODE: {
+      const int d[6] = {-2, -1, 0, n, n + 1, n + 2};
+      if (slot < 6) { dst = d[slot]; zero = true; act = true; }
+      break;
+    }
+    case BC_DM_CLAMP: {
+      const int d[2] = {n + 1, -1}, r[2] = {n, 0};
+      if (slot < 2 && dm_ok) { dst = d[slot]; src = r[slot]; act = true; }
+      break;
+    }
+    default:
+      break;
+  }
+  if (act) a[base + dst] = zero ? (T)0 : a[base + src];
+}
+
+__global__ __launch_bounds__(256) void k_bc_x(Geo g, BcSet s) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int slot = t & 7, line = t >> 3;
+  const int ly = g.ny + 5;
+  if (line >= ly * (g.nz + 5)) return;
+  const int j = -2 + line % ly, k = -2 + line / ly;
+  const int base = g.idx(0, j, k);
+  const bool dm_ok = j >= 0 && j <= g.ny + 1 && k >= 0 && k <= g.nz + 1;
+  for (int q = 0; q < s.nf; ++q) {
+    if (s.wide[q]) bc_slot((double *)s.f[q], s.op[q], base, g.nx, slot, dm_ok);
+    else bc_slot((float *)s.f[q], s.op[q], base, g.nx, slot, dm_ok);
+  }
+}

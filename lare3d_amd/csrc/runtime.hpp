@@ -238,6 +238,11 @@ int op_for(FieldKind k, int axis, bool periodic) {
 }
 
 int launch_bc_axis(const BcSet &s, int axis, const char *name) {
+  if (axis == 0) {
+    const int lines = (S.n[1] + 5) * (S.n[2] + 5);
+    LAUNCH(name, k_bc_x, dim3((unsigned)((lines * 8 + 255) / 256)), dim3(256), S.g, s);
+    return 0;
+  }
   const int u = axis == 0 ? 1 : 0, w = axis == 2 ? 1 : 2;
   dim3 grid((unsigned)((S.n[u] + 5 + 63) / 64), (unsigned)(S.n[w] + 5));
   LAUNCH(name, k_bc_axis, grid, dim3(64), S.g, s, axis);

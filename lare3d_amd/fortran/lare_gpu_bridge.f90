@@ -109,6 +109,22 @@ MODULE lare_gpu_bridge
       REAL(C_FLOAT), INTENT(IN) :: mu(*), sd(*)
       REAL(C_FLOAT), VALUE :: y_scale
     END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_sur_collect(count, seed, total) BIND(C, NAME='lare_dev_sur_collect')
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T
+      INTEGER(C_INT32_T), VALUE :: count
+      INTEGER(C_INT64_T), VALUE :: seed
+      INTEGER(C_INT32_T), INTENT(OUT) :: total
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_sur_fit_norm() BIND(C, NAME='lare_dev_sur_fit_norm')
+      IMPORT :: C_INT
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_sur_train(steps, batch, lr, seed, loss) BIND(C, NAME='lare_dev_sur_train')
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_FLOAT
+      INTEGER(C_INT32_T), VALUE :: steps, batch
+      REAL(C_FLOAT), VALUE :: lr
+      INTEGER(C_INT64_T), VALUE :: seed
+      REAL(C_FLOAT), INTENT(OUT) :: loss(*)
+    END FUNCTION
     INTEGER(C_INT) FUNCTION lare_dev_sur_enable(on, tol) BIND(C, NAME='lare_dev_sur_enable')
       IMPORT :: C_INT, C_INT32_T, C_DOUBLE
       INTEGER(C_INT32_T), VALUE :: on
