@@ -38,17 +38,29 @@ struct Box {
 #define BX 64
 #define BY 4
 
-// Thread (i, j, k) of a box launch; returns false outside the box.
-#define BOX_INDEX(b, i, j, k)                              \
-  const int i = (b).x0 + (int)(blockIdx.x * BX + threadIdx.x); \
-  const int j = (b).y0 + (int)(blockIdx.y * BY + threadIdx.y); \
-  const int k = (b).z0 + (int)blockIdx.z;                     \
-  if (i > (b).x1 || j > (b).y1 || k > (b).z1) return;
+// In-plane position (i, j) of a thread of a box launch.  The (x, y) extent
+// of the box is walked as one flattened index, BX*BY consecutive points per
+// block, so a box a few points wider than a multiple of 64 (n+1 .. n+4 for
+// n = 256) does not launch a column of blocks whose waves are almost empty.
+// i always lies inside [x0, x1]; j may pass y1 in the last block.
+#define PLANE_POS(b, i, j)                                                                  \
+  const unsigned pw_ = (unsigned)((b).x1 - (b).x0 + 1);                                     \
+  const unsigned pp_ = blockIdx.x * (unsigned)(BX * BY) + threadIdx.y * (unsigned)BX + threadIdx.x; \
+  const unsigned pj_ = pp_ / pw_;                                                           \
+  const int i = (b).x0 + (int)(pp_ - pj_ * pw_);                                            \
+  const int j = (b).y0 + (int)pj_;
 
-static inline dim3 box_grid(const Box &b) {
-  return dim3((unsigned)((b.x1 - b.x0 + 1 + BX - 1) / BX), (unsigned)((b.y1 - b.y0 + 1 + BY - 1) / BY),
-              (unsigned)(b.z1 - b.z0 + 1));
+// Thread (i, j, k) of a box launch; returns false outside the box.
+#define BOX_INDEX(b, i, j, k)                  \
+  PLANE_POS(b, i, j)                           \
+  const int k = (b).z0 + (int)blockIdx.z;      \
+  if (j > (b).y1 || k > (b).z1) return;
+
+static inline unsigned plane_blocks(const Box &b) {
+  const long w = b.x1 - b.x0 + 1, h = b.y1 - b.y0 + 1;
+  return (unsigned)((w * h + BX * BY - 1) / (BX * BY));
 }
+static inline dim3 box_grid(const Box &b) { return dim3(plane_blocks(b), 1, (unsigned)(b.z1 - b.z0 + 1)); }
 static inline dim3 box_block() { return dim3(BX, BY, 1); }
 static inline Box mkbox(int x0, int x1, int y0, int y1, int z0, int z1) {
   Box b = {x0, x1, y0, y1, z0, z1};
@@ -95,15 +107,16 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], const int *o
   }
 }
 
-// Columns of a z-marching launch: thread (i, j) of tile (blockIdx.x,
-// blockIdx.y) walks planes k0 .. k1 of its chunk (ZCHUNK planes per block).
+// Columns of a z-marching launch: thread (i, j) of the flattened plane tile
+// blockIdx.x walks planes k0 .. k1 of its chunk (ZCHUNK planes per block).
 #define ZCHUNK 8
-#define ZCOLUMN(b, i, j, k0, k1)                                      \
-  const int i = (b).x0 + (int)(blockIdx.x * BX + threadIdx.x);         \
-  const int j = (b).y0 + (int)(blockIdx.y * BY + threadIdx.y);         \
+#define ZRANGE(b, k0, k1)                                             \
   const int k0 = (b).z0 + (int)blockIdx.z * ZCHUNK;                    \
-  const int k1 = k0 + ZCHUNK - 1 < (b).z1 ? k0 + ZCHUNK - 1 : (b).z1;  \
-  if (i > (b).x1 || j > (b).y1) return;
+  const int k1 = k0 + ZCHUNK - 1 < (b).z1 ? k0 + ZCHUNK - 1 : (b).z1;
+#define ZCOLUMN(b, i, j, k0, k1) \
+  PLANE_POS(b, i, j)             \
+  ZRANGE(b, k0, k1)              \
+  if (j > (b).y1) return;
 
 // A field at the four in-plane points (i, j), (i-1, j), (i, j-1), (i-1, j-1).
 struct Quad {

@@ -58,11 +58,11 @@ struct CondArgs {
 __global__ __launch_bounds__(BX * BY) void k_larsen(Geo g, const real *__restrict__ energy, const real *__restrict__ rho,
                          real *__restrict__ larsen, double *__restrict__ partial, double flux_limiter,
                          double kappa_0, double tfac, int limiter_on) {
-  const int i = (int)(blockIdx.x * BX + threadIdx.x);
-  const int j = (int)(blockIdx.y * BY + threadIdx.y);
+  const Box b = {0, g.nx, 0, g.ny, 0, g.nz};
+  PLANE_POS(b, i, j)
   const int k = (int)blockIdx.z;
   double v[1] = {1.0e300};
-  const bool in = i <= g.nx && j <= g.ny && k <= g.nz;
+  const bool in = j <= g.ny && k <= g.nz;
   if (in) {
     const int c = g.idx(i, j, k);
     const double T = tfac * energy[c];

@@ -265,12 +265,10 @@ __global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, Box bx0, const real 
                           LagArgs a) {
   // cells 0..n+1, marching in z: node velocities and al1/al2 of plane k
   // become those of plane k-1 / k for the next cell
-  const int i = bx0.x0 + (int)(blockIdx.x * BX + threadIdx.x);
-  const int j = bx0.y0 + (int)(blockIdx.y * BY + threadIdx.y);
-  const int k0 = bx0.z0 + (int)blockIdx.z * ZCHUNK;
-  const int k1 = k0 + ZCHUNK - 1 < bx0.z1 ? k0 + ZCHUNK - 1 : bx0.z1;
+  PLANE_POS(bx0, i, j)
+  ZRANGE(bx0, k0, k1)
   double v[2] = {LARGEST, LARGEST};
-  if (i <= bx0.x1 && j <= bx0.y1) {
+  if (j <= bx0.y1) {
     const int Y = g.sy, Z = g.sz;
     const int cb = g.idx(i, j, k0);
     Node4 lx = node4(vx, cb - Z, Y), ly = node4(vy, cb - Z, Y), lz = node4(vz, cb - Z, Y);
@@ -420,12 +418,10 @@ __global__ __launch_bounds__(BX * BY) void k_rkstep(Geo g, Box b, const double *
                          const double *__restrict__ bz, const real *__restrict__ eta, double *__restrict__ fxr,
                          double *__restrict__ fyr, double *__restrict__ fzr, real *__restrict__ curlb,
                          double *__restrict__ partial, double dt) {
-  const int i = b.x0 + (int)(blockIdx.x * BX + threadIdx.x);
-  const int j = b.y0 + (int)(blockIdx.y * BY + threadIdx.y);
-  const int k0 = b.z0 + (int)blockIdx.z * ZCHUNK;
-  const int k1 = k0 + ZCHUNK - 1 < b.z1 ? k0 + ZCHUNK - 1 : b.z1;
+  PLANE_POS(b, i, j)
+  ZRANGE(b, k0, k1)
   double v[1] = {0.0};
-  if (i <= b.x1 && j <= b.y1) {
+  if (j <= b.y1) {
     const int Y = g.sy, Z = g.sz;
     BPlane lo = bplane(bx, by, g.idx(i, j, k0), Y);
     for (int k = k0; k <= k1; ++k) {
@@ -632,12 +628,10 @@ __global__ __launch_bounds__(BX * BY) void k_shock_corr(Geo g, Box b, const real
                              double *__restrict__ partial, double dt) {
   // cells 1..n, marching in z: the node values (v, v1) and al1/al2 of plane
   // k are reused as plane k-1 / k values of the next cell
-  const int i = b.x0 + (int)(blockIdx.x * BX + threadIdx.x);
-  const int j = b.y0 + (int)(blockIdx.y * BY + threadIdx.y);
-  const int k0 = b.z0 + (int)blockIdx.z * ZCHUNK;
-  const int k1 = k0 + ZCHUNK - 1 < b.z1 ? k0 + ZCHUNK - 1 : b.z1;
+  PLANE_POS(b, i, j)
+  ZRANGE(b, k0, k1)
   double v[1] = {0.0};
-  if (i <= b.x1 && j <= b.y1) {
+  if (j <= b.y1) {
     const int Y = g.sy, Z = g.sz;
     const int cb = g.idx(i, j, k0) - Z;
     Node4 lx = node4(vx, cb, Y), ly = node4(vy, cb, Y), lz = node4(vz, cb, Y);
@@ -701,11 +695,11 @@ __global__ __launch_bounds__(BX * BY) void k_energy(Geo g, const real *__restric
                          const real *__restrict__ vx, const real *__restrict__ vy, const real *__restrict__ vz,
                          const double *__restrict__ bx, const double *__restrict__ by,
                          const double *__restrict__ bz, double *__restrict__ partial) {
-  const int i = (int)(blockIdx.x * BX + threadIdx.x);
-  const int j = (int)(blockIdx.y * BY + threadIdx.y);
+  const Box b = {0, g.nx, 0, g.ny, 0, g.nz};
+  PLANE_POS(b, i, j)
   const int k = (int)blockIdx.z;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
-  if (i <= g.nx && j <= g.ny && k <= g.nz) {
+  if (j <= g.ny && k <= g.nz) {
     const int c = g.idx(i, j, k);
     const int X = 1, Y = g.sy, Z = g.sz;
     if (i >= 1 && j >= 1 && k >= 1) {

@@ -141,10 +141,9 @@ Box cell_box() { return mkbox(-1, nx() + 2, -1, ny() + 2, -1, nz() + 2); }
 Box interior() { return mkbox(1, nx(), 1, ny(), 1, nz()); }
 Box nodes0n() { return mkbox(0, nx(), 0, ny(), 0, nz()); }
 
-// grid of a z-marching launch: BX x BY columns, ZCHUNK planes per block
+// grid of a z-marching launch: BX*BY columns, ZCHUNK planes per block
 dim3 zchunk_grid(const Box &b) {
-  return dim3((unsigned)((b.x1 - b.x0 + 1 + BX - 1) / BX), (unsigned)((b.y1 - b.y0 + 1 + BY - 1) / BY),
-              (unsigned)((b.z1 - b.z0 + 1 + ZCHUNK - 1) / ZCHUNK));
+  return dim3(plane_blocks(b), 1, (unsigned)((b.z1 - b.z0 + 1 + ZCHUNK - 1) / ZCHUNK));
 }
 
 // reduction over partials of a grid launched from index 0 on every axis

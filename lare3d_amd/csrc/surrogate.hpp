@@ -417,11 +417,11 @@ __global__ __launch_bounds__(BX * BY) void k_sur_residual(Geo g, const real *__r
                                                           const real *__restrict__ E1,
                                                           const real *__restrict__ trap,
                                                           double *__restrict__ partial) {
-  const int i = 1 + (int)(blockIdx.x * BX + threadIdx.x);
-  const int j = 1 + (int)(blockIdx.y * BY + threadIdx.y);
+  const Box b = {1, g.nx, 1, g.ny, 1, g.nz};
+  PLANE_POS(b, i, j)
   const int k = 1 + (int)blockIdx.z;
   double v[2] = {0.0, 0.0};
-  if (i <= g.nx && j <= g.ny && k <= g.nz) {
+  if (j <= g.ny && k <= g.nz) {
     const int c = g.idx(i, j, k);
     const double r = (E1[c] - E0[c]) - trap[c];
     v[0] = r * r;
