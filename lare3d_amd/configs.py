@@ -92,16 +92,17 @@ def orszag_tang(n=64):
     return grid, params, f
 
 
-def harris(n=64, seed=12345, ny=None, nz=None, noise=0.01):
+def harris(n=64, seed=12345, ny=None, nz=None, noise=0.01, stretch=(None, None, None)):
     """Resistive Harris sheet (config 2): Bx = tanh(y/0.5),
     rho = 1 + sech^2(y/0.5), p = 0.5 rho (total pressure balance,
     isothermal), eta = 1e-3 plus 1e-2 above |J| = 1.0, seeded velocity noise,
-    shock viscosity on.  Periodic in x and z, BC_USER walls in y."""
+    shock viscosity on.  Periodic in x and z, BC_USER walls in y.
+    stretch: stretched length per axis (setup.F90:256-285), or None."""
     ny = ny or n
     nz = nz or n
     L = 4.0
     grid = Grid(n, ny, nz, ((-2 * L, 2 * L), (-L, L), (-2 * L, 2 * L)),
-                (BC_PERIODIC, BC_PERIODIC, BC_USER, BC_USER, BC_PERIODIC, BC_PERIODIC))
+                (BC_PERIODIC, BC_PERIODIC, BC_USER, BC_USER, BC_PERIODIC, BC_PERIODIC), stretch)
     params = default_params(resistive_mhd=1, eta_background=1.0e-3, eta0=1.0e-2, j_max=1.0)
     f = Fields(n, ny, nz)
     gam = params.gamma

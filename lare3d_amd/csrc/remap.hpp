@@ -40,18 +40,26 @@ __device__ __forceinline__ double vl_di(double dfi, double dfu, double sign_v, d
 // Lagrangian geometry of a sweep (xremap.f90:57-105 and analogues).
 // Velocities are the time-centred ones (v in predictor mode).  The node
 // velocities of planes k and k-1 are kept in registers while marching in z.
-template <int AX>
+// CV selects the outputs: 0 = d?b1 only (which needs only the sweep
+// component of the velocity), 1 = cv1 and d?b1, 2 = cv1, cv2 and d?b1.
+template <int AX, int CV>
 __global__ __launch_bounds__(BX * BY) void k_geom(Geo g, Box b, const real *__restrict__ ux, const real *__restrict__ uy,
                        const real *__restrict__ uz, real *__restrict__ cv1, real *__restrict__ cv2,
-                       real *__restrict__ db1, double dt, double xpass, double ypass, double zpass, int write_cv1,
-                       int write_cv2) {
+                       real *__restrict__ db1, double dt, double xpass, double ypass, double zpass) {
   ZCOLUMN(b, i, j, k0, k1);
+  constexpr bool NX = CV > 0 || AX == 0, NY = CV > 0 || AX == 1, NZ = CV > 0 || AX == 2;
   const int Y = g.sy;
   const int cb = g.idx(i, j, k0 - 1);
-  Quad ax = quad_back(ux, cb, Y), ay = quad_back(uy, cb, Y), az = quad_back(uz, cb, Y);  // plane k-1
+  Quad ax = {}, ay = {}, az = {};  // plane k-1
+  if (NX) ax = quad_back(ux, cb, Y);
+  if (NY) ay = quad_back(uy, cb, Y);
+  if (NZ) az = quad_back(uz, cb, Y);
   for (int k = k0; k <= k1; ++k) {
     const int c = g.idx(i, j, k);
-    const Quad bx_ = quad_back(ux, c, Y), by_ = quad_back(uy, c, Y), bz_ = quad_back(uz, c, Y);  // plane k
+    Quad bx_ = {}, by_ = {}, bz_ = {};  // plane k
+    if (NX) bx_ = quad_back(ux, c, Y);
+    if (NY) by_ = quad_back(uy, c, Y);
+    if (NZ) bz_ = quad_back(uz, c, Y);
     // sums in the order of u(c) + u(c-Y) + u(c-Z) + u(c-Y-Z) etc.
     const double vxb = (bx_.p0 + bx_.py + ax.p0 + ax.py) * 0.25;
     const double vxbm = (bx_.px + bx_.pxy + ax.px + ax.pxy) * 0.25;
@@ -59,29 +67,32 @@ __global__ __launch_bounds__(BX * BY) void k_geom(Geo g, Box b, const real *__re
     const double vybm = (by_.py + by_.pxy + ay.py + ay.pxy) * 0.25;
     const double vzb = (bz_.p0 + bz_.px + bz_.py + bz_.pxy) * 0.25;
     const double vzbm = (az.p0 + az.px + az.py + az.pxy) * 0.25;
-    const double cv = g.cv(i, j, k);
-    double dv1, dv2, d1;
-    if (AX == 0) {
-      const double tA = ypass * (vyb - vybm) / g.dyb[j];
-      const double tB = zpass * (vzb - vzbm) / g.dzb[k];
-      dv1 = (tA + tB + (vxb - vxbm) / g.dxb[i]) * dt;
-      dv2 = (tA + tB) * dt;
-      d1 = g.dxb[i] + (vxb - vxbm) * dt;
-    } else if (AX == 1) {
-      const double tA = xpass * (vxb - vxbm) / g.dxb[i];
-      const double tB = zpass * (vzb - vzbm) / g.dzb[k];
-      dv1 = (tA + tB + (vyb - vybm) / g.dyb[j]) * dt;
-      dv2 = (tA + tB) * dt;
-      d1 = g.dyb[j] + (vyb - vybm) * dt;
-    } else {
-      const double tA = xpass * (vxb - vxbm) / g.dxb[i];
-      const double tB = ypass * (vyb - vybm) / g.dyb[j];
-      dv1 = (tA + tB + (vzb - vzbm) / g.dzb[k]) * dt;
-      dv2 = (tA + tB) * dt;
-      d1 = g.dzb[k] + (vzb - vzbm) * dt;
+    double d1;
+    if (AX == 0) d1 = g.dxb[i] + (vxb - vxbm) * dt;
+    else if (AX == 1) d1 = g.dyb[j] + (vyb - vybm) * dt;
+    else d1 = g.dzb[k] + (vzb - vzbm) * dt;
+    if (CV > 0) {
+      const double cv = g.cv(i, j, k);
+      double dv1, dv2;
+      if (AX == 0) {
+        const double tA = ypass * (vyb - vybm) / g.dyb[j];
+        const double tB = zpass * (vzb - vzbm) / g.dzb[k];
+        dv1 = (tA + tB + (vxb - vxbm) / g.dxb[i]) * dt;
+        dv2 = (tA + tB) * dt;
+      } else if (AX == 1) {
+        const double tA = xpass * (vxb - vxbm) / g.dxb[i];
+        const double tB = zpass * (vzb - vzbm) / g.dzb[k];
+        dv1 = (tA + tB + (vyb - vybm) / g.dyb[j]) * dt;
+        dv2 = (tA + tB) * dt;
+      } else {
+        const double tA = xpass * (vxb - vxbm) / g.dxb[i];
+        const double tB = ypass * (vyb - vybm) / g.dyb[j];
+        dv1 = (tA + tB + (vzb - vzbm) / g.dzb[k]) * dt;
+        dv2 = (tA + tB) * dt;
+      }
+      cv1[c] = cv * (1.0 + dv1);
+      if (CV > 1) cv2[c] = cv * (1.0 + dv2);
     }
-    if (write_cv1) cv1[c] = cv * (1.0 + dv1);
-    if (write_cv2) cv2[c] = cv * (1.0 + dv2);
     db1[c] = d1;
     ax = bx_;
     ay = by_;
@@ -121,18 +132,56 @@ __device__ __forceinline__ double ct_flux_core(double v_advect, double frm, doub
   return (fu + Di * (1.0 - phi)) * v_advect * dt;
 }
 
+// Face B component Q at (i, j, k) as the face flux that the remap moves,
+// and back (remap.f90:46-48 and 104-106, same operation order), with the
+// index range each component is defined on (bx(-2:nx+2, -1:ny+2, -1:nz+2)
+// and its analogues, remap.f90:44-67).
+template <int Q>
+__device__ __forceinline__ bool bface_ok(int i, int j, int k) {
+  return Q == 0 ? (j >= -1 && k >= -1) : (Q == 1 ? (i >= -1 && k >= -1) : (i >= -1 && j >= -1));
+}
+template <int Q>
+__device__ __forceinline__ double to_flux(const Geo &g, double v, int i, int j, int k) {
+  if (Q == 0) return v * g.dyb[j] * g.dzb[k];
+  if (Q == 1) return v * g.dxb[i] * g.dzb[k];
+  return v * g.dxb[i] * g.dyb[j];
+}
+template <int Q>
+__device__ __forceinline__ double from_flux(const Geo &g, double v, int i, int j, int k) {
+  if (Q == 0) return v / g.dyb[j] / g.dzb[k];
+  if (Q == 1) return v / g.dxb[i] / g.dzb[k];
+  return v / g.dxb[i] / g.dyb[j];
+}
+// The transverse components (t1, t2) of a sweep along AX and its CT update
+// pairs: x -> (by, bz), y -> (bx, bz), z -> (bx, by).
+template <int AX> struct CtComp {
+  static constexpr int T1 = AX == 0 ? 1 : 0;
+  static constexpr int T2 = AX == 2 ? 1 : 2;
+};
+// Stencil value of component Q at the point whose sweep coordinate is s
+// (other coordinates those of (i, j, k)); SC = 1 reads the field as B and
+// converts it to the face flux on the fly (first sweep of a remap).
+template <int AX, int Q, int SC>
+__device__ __forceinline__ double ct_load(const Geo &g, const double *__restrict__ f, int c, int s, int i, int j,
+                                          int k) {
+  const double v = f[c];
+  if (!SC) return v;
+  return to_flux<Q>(g, v, AX == 0 ? s : i, AX == 1 ? s : j, AX == 2 ? s : k);
+}
+
 // CT fluxes of the two transverse face fields of an x or y sweep (AX = 0, 1:
 // vx_by_flux and vx_bz_flux, vy_bx_flux and vy_bz_flux) in one pass.  The
 // first field's velocity is averaged along z (VS1 = z stride) and the
 // second's widths along z (GS2 = z stride), so the kernel marches up
 // z-columns keeping v of plane k-1 and db1 of plane k+1 at the four stencil
 // points for the next plane.  GS1 and VS2 are in-plane strides.
-template <int AX>
+template <int AX, int SC>
 __global__ __launch_bounds__(BX * BY) void k_ct_flux(Geo g, Box b, const double *__restrict__ fld1,
                                                      const double *__restrict__ fld2, const real *__restrict__ vel,
                                                      const real *__restrict__ db1, double *__restrict__ F1,
                                                      double *__restrict__ F2, int GS1, int VS2, double dt) {
   ZCOLUMN(b, i, j, k0, k1);
+  constexpr int T1 = CtComp<AX>::T1, T2 = CtComp<AX>::T2;
   const int S = sstride<AX>(g);
   const int Z = g.sz;
   const int n = g.n(AX);
@@ -154,13 +203,17 @@ __global__ __launch_bounds__(BX * BY) void k_ct_flux(Geo g, Box b, const double 
     const real vc = vel[c];
     {
       const double v_advect = (vc + vm) * 0.5;
-      F1[c] = ct_flux_core(v_advect, fld1[cm], fld1[c], fld1[cp], fld1[cp2], (d0 + db1[cm + GS1]) * 0.5,
+      F1[c] = ct_flux_core(v_advect, ct_load<AX, T1, SC>(g, fld1, cm, sm, i, j, k),
+                           ct_load<AX, T1, SC>(g, fld1, c, s, i, j, k), ct_load<AX, T1, SC>(g, fld1, cp, s + 1, i, j, k),
+                           ct_load<AX, T1, SC>(g, fld1, cp2, sp2, i, j, k), (d0 + db1[cm + GS1]) * 0.5,
                            (d1 + db1[c + GS1]) * 0.5, (d2 + db1[cp + GS1]) * 0.5, (d3 + db1[cp2 + GS1]) * 0.5, d1,
                            d2, dcm, dci, dcp, dt);
     }
     {
       const double v_advect = (vc + vel[c - VS2]) * 0.5;
-      F2[c] = ct_flux_core(v_advect, fld2[cm], fld2[c], fld2[cp], fld2[cp2], (d0 + dz[0]) * 0.5, (d1 + dz[1]) * 0.5,
+      F2[c] = ct_flux_core(v_advect, ct_load<AX, T2, SC>(g, fld2, cm, sm, i, j, k),
+                           ct_load<AX, T2, SC>(g, fld2, c, s, i, j, k), ct_load<AX, T2, SC>(g, fld2, cp, s + 1, i, j, k),
+                           ct_load<AX, T2, SC>(g, fld2, cp2, sp2, i, j, k), (d0 + dz[0]) * 0.5, (d1 + dz[1]) * 0.5,
                            (d2 + dz[2]) * 0.5, (d3 + dz[3]) * 0.5, d1, d2, dcm, dci, dcp, dt);
     }
     vm = vc;
@@ -177,6 +230,7 @@ __global__ __launch_bounds__(BX * BY) void k_ct_flux(Geo g, Box b, const double 
 // The same fluxes for the z sweep, each thread marching up its column for
 // ZCHUNK planes with the four-plane stencils of both fields and of db1 in
 // registers, so every plane is read from memory once per column chunk.
+template <int SC>
 __global__ __launch_bounds__(BX * BY) void k_ct_flux_z(Geo g, Box b, const double *__restrict__ fld1,
                                                        const double *__restrict__ fld2,
                                                        const real *__restrict__ vel, const real *__restrict__ db1,
@@ -192,8 +246,8 @@ __global__ __launch_bounds__(BX * BY) void k_ct_flux_z(Geo g, Box b, const doubl
     int kk = k0 - 1 + q;
     kk = kk < -1 ? -1 : (kk > n + 2 ? n + 2 : kk);
     const int cq = g.idx(i, j, kk);
-    f1[q] = fld1[cq];
-    f2[q] = fld2[cq];
+    f1[q] = ct_load<2, 0, SC>(g, fld1, cq, kk, i, j, kk);
+    f2[q] = ct_load<2, 1, SC>(g, fld2, cq, kk, i, j, kk);
     d[q] = db1[cq];
     w1[q] = db1[cq + GS1];
     w2[q] = db1[cq + GS2];
@@ -226,8 +280,8 @@ __global__ __launch_bounds__(BX * BY) void k_ct_flux_z(Geo g, Box b, const doubl
       w1[q] = w1[q + 1];
       w2[q] = w2[q + 1];
     }
-    f1[3] = fld1[cn];
-    f2[3] = fld2[cn];
+    f1[3] = ct_load<2, 0, SC>(g, fld1, cn, kn, i, j, kn);
+    f2[3] = ct_load<2, 1, SC>(g, fld2, cn, kn, i, j, kn);
     d[3] = db1[cn];
     w1[3] = db1[cn + GS1];
     w2[3] = db1[cn + GS2];
@@ -238,22 +292,56 @@ __global__ __launch_bounds__(BX * BY) void k_ct_flux_z(Geo g, Box b, const doubl
 // pairs in the reference's order:
 //   t1 = t1 - F1 + F1(c - TS) on bt1;  a = a + F1 - F1(c - AS1) on ba1;
 //   t2 = t2 - F2 + F2(c - TS) on bt2;  a = a + F2 - F2(c - AS2) on ba2.
+// SC = 0: in place on the face fluxes (t1i == t1 etc.), over the box b.
+// SC = +1 (first sweep of a remap): the inputs are B; every point of each
+// component's range in b is converted to the face flux (remap.f90:44-67),
+// updated where its box says so, and stored to the outputs.
+// SC = -1 (last sweep of the predictor remap): the updated face flux of
+// every such point is converted back to B (remap.f90:102-125).
+template <int AX, int SC>
 __global__ __launch_bounds__(BX * BY) void k_ct_update(Geo g, Box b, const double *__restrict__ F1,
-                                                       const double *__restrict__ F2, double *__restrict__ t1,
-                                                       double *__restrict__ t2, double *__restrict__ a, int TS,
-                                                       int AS1, int AS2, Box bt1, Box ba1, Box bt2, Box ba2) {
+                                                       const double *__restrict__ F2, const double *t1i,
+                                                       const double *t2i, const double *ai, double *t1, double *t2,
+                                                       double *a, int TS, int AS1, int AS2, Box bt1, Box ba1,
+                                                       Box bt2, Box ba2) {
   BOX_INDEX(b, i, j, k);
+  constexpr int T1 = CtComp<AX>::T1, T2 = CtComp<AX>::T2;
   const int c = g.idx(i, j, k);
   const bool u1 = i >= bt1.x0 && i <= bt1.x1 && j >= bt1.y0 && j <= bt1.y1 && k >= bt1.z0 && k <= bt1.z1;
   const bool w1 = i >= ba1.x0 && i <= ba1.x1 && j >= ba1.y0 && j <= ba1.y1 && k >= ba1.z0 && k <= ba1.z1;
   const bool u2 = i >= bt2.x0 && i <= bt2.x1 && j >= bt2.y0 && j <= bt2.y1 && k >= bt2.z0 && k <= bt2.z1;
   const bool w2 = i >= ba2.x0 && i <= ba2.x1 && j >= ba2.y0 && j <= ba2.y1 && k >= ba2.z0 && k <= ba2.z1;
-  if (u1) t1[c] = t1[c] - F1[c] + F1[c - TS];
-  if (u2) t2[c] = t2[c] - F2[c] + F2[c - TS];
-  if (w1 || w2) {
-    double va = a[c];
+  if (SC == 0) {
+    if (u1) t1[c] = t1[c] - F1[c] + F1[c - TS];
+    if (u2) t2[c] = t2[c] - F2[c] + F2[c - TS];
+    if (w1 || w2) {
+      double va = a[c];
+      if (w1) va = va + F1[c] - F1[c - AS1];
+      if (w2) va = va + F2[c] - F2[c - AS2];
+      a[c] = va;
+    }
+    return;
+  }
+  if (bface_ok<T1>(i, j, k)) {
+    double v = t1i[c];
+    if (SC > 0) v = to_flux<T1>(g, v, i, j, k);
+    if (u1) v = v - F1[c] + F1[c - TS];
+    if (SC < 0) v = from_flux<T1>(g, v, i, j, k);
+    t1[c] = v;
+  }
+  if (bface_ok<T2>(i, j, k)) {
+    double v = t2i[c];
+    if (SC > 0) v = to_flux<T2>(g, v, i, j, k);
+    if (u2) v = v - F2[c] + F2[c - TS];
+    if (SC < 0) v = from_flux<T2>(g, v, i, j, k);
+    t2[c] = v;
+  }
+  if (bface_ok<AX>(i, j, k)) {
+    double va = ai[c];
+    if (SC > 0) va = to_flux<AX>(g, va, i, j, k);
     if (w1) va = va + F1[c] - F1[c - AS1];
     if (w2) va = va + F2[c] - F2[c - AS2];
+    if (SC < 0) va = from_flux<AX>(g, va, i, j, k);
     a[c] = va;
   }
 }
@@ -629,16 +717,13 @@ __global__ __launch_bounds__(BX * BY) void k_bscale(Geo g, Box b, const double *
                          double *__restrict__ bzo, int dir) {
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
-  const bool okx = j >= -1 && k >= -1;  // bx(-2:nx+2,-1:ny+2,-1:nz+2)
-  const bool oky = i >= -1 && k >= -1;
-  const bool okz = i >= -1 && j >= -1;
   if (dir > 0) {
-    if (okx) bxo[c] = bxi[c] * g.dyb[j] * g.dzb[k];
-    if (oky) byo[c] = byi[c] * g.dxb[i] * g.dzb[k];
-    if (okz) bzo[c] = bzi[c] * g.dxb[i] * g.dyb[j];
+    if (bface_ok<0>(i, j, k)) bxo[c] = to_flux<0>(g, bxi[c], i, j, k);
+    if (bface_ok<1>(i, j, k)) byo[c] = to_flux<1>(g, byi[c], i, j, k);
+    if (bface_ok<2>(i, j, k)) bzo[c] = to_flux<2>(g, bzi[c], i, j, k);
   } else {
-    if (okx) bxo[c] = bxi[c] / g.dyb[j] / g.dzb[k];
-    if (oky) byo[c] = byi[c] / g.dxb[i] / g.dzb[k];
-    if (okz) bzo[c] = bzi[c] / g.dxb[i] / g.dyb[j];
+    if (bface_ok<0>(i, j, k)) bxo[c] = from_flux<0>(g, bxi[c], i, j, k);
+    if (bface_ok<1>(i, j, k)) byo[c] = from_flux<1>(g, byi[c], i, j, k);
+    if (bface_ok<2>(i, j, k)) bzo[c] = from_flux<2>(g, bzi[c], i, j, k);
   }
 }

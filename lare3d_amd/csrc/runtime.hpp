@@ -352,12 +352,18 @@ int heat_conduct_sts2(int delta_mode) {
 // ----------------------------------------------------------------------------
 int strides(int ax) { return ax == 0 ? 1 : (ax == 1 ? S.sy : S.sz); }
 
+// CT fluxes and updates of one sweep on the face arrays B.  sc = 0: B holds
+// face fluxes and is updated in place.  sc = +1 (first sweep of a remap): Bin
+// holds B; the fluxes convert it on the fly and the update writes the face
+// fluxes of every point to B (the scaling pass of remap.f90:44-67 folded in).
+// sc = -1 (last predictor sweep): the update converts every point back to B
+// (remap.f90:102-125 folded in).
 template <int AX>
-int ct_pair(double *bxa, double *bya, double *bza, const real *vel, double dt) {
+int ct_pair(double *const Bin[3], double *const B[3], const real *vel, double dt, int sc) {
   const int N0 = nx(), N1 = ny(), N2 = nz();
   Box fb = mkbox(-1, N0 + 1, -1, N1 + 1, -1, N2 + 1);
   Box ub = mkbox(0, N0 + 1, 0, N1 + 1, 0, N2 + 1);
-  double *B[3] = {bxa, bya, bza};
+  const double *const *Bf = sc > 0 ? Bin : B;  // what the fluxes read
   // (transverse field t, velocity-average axis, geometry-average axis, box_t, box_a) x 2
   int t1, va1, ga1, t2, va2, ga2;
   Box bt1 = ub, ba1 = ub, bt2 = ub, ba2 = ub;
@@ -375,34 +381,57 @@ int ct_pair(double *bxa, double *bya, double *bza, const real *vel, double dt) {
     t2 = 1; va2 = 0; ga2 = 1;  // vz_by_flux
   }
   if constexpr (AX == 2) {
-    LAUNCH("ct_flux", k_ct_flux_z, zchunk_grid(fb), box_block(), S.g, fb, B[t1], B[t2], vel, S.db1, S.F, S.F2,
-           strides(va1), strides(ga1), strides(va2), strides(ga2), dt);
+    if (sc > 0)
+      LAUNCH("ct_flux", k_ct_flux_z<1>, zchunk_grid(fb), box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1, S.F,
+             S.F2, strides(va1), strides(ga1), strides(va2), strides(ga2), dt);
+    else
+      LAUNCH("ct_flux", k_ct_flux_z<0>, zchunk_grid(fb), box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1, S.F,
+             S.F2, strides(va1), strides(ga1), strides(va2), strides(ga2), dt);
   } else {
     // x and y sweeps: va1 = z and ga2 = z (see the flux table above)
-    LAUNCH("ct_flux", k_ct_flux<AX>, zchunk_grid(fb), box_block(), S.g, fb, B[t1], B[t2], vel, S.db1, S.F, S.F2,
-           strides(ga1), strides(va2), dt);
+    if (sc > 0)
+      LAUNCH("ct_flux", (k_ct_flux<AX, 1>), zchunk_grid(fb), box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1, S.F,
+             S.F2, strides(ga1), strides(va2), dt);
+    else
+      LAUNCH("ct_flux", (k_ct_flux<AX, 0>), zchunk_grid(fb), box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1, S.F,
+             S.F2, strides(ga1), strides(va2), dt);
   }
-  LAUNCH_BOX("ct_update", k_ct_update, ub, S.F, S.F2, B[t1], B[t2], B[AX], strides(AX), strides(t1), strides(t2),
-             bt1, ba1, bt2, ba2);
+  if (sc == 0)
+    LAUNCH_BOX("ct_update", (k_ct_update<AX, 0>), ub, S.F, S.F2, B[t1], B[t2], B[AX], B[t1], B[t2], B[AX],
+               strides(AX), strides(t1), strides(t2), bt1, ba1, bt2, ba2);
+  else if (sc > 0)
+    LAUNCH_BOX("ct_update", (k_ct_update<AX, 1>), full_box(), S.F, S.F2, Bin[t1], Bin[t2], Bin[AX], B[t1], B[t2],
+               B[AX], strides(AX), strides(t1), strides(t2), bt1, ba1, bt2, ba2);
+  else
+    LAUNCH_BOX("ct_update", (k_ct_update<AX, -1>), full_box(), S.F, S.F2, B[t1], B[t2], B[AX], B[t1], B[t2], B[AX],
+               strides(AX), strides(t1), strides(t2), bt1, ba1, bt2, ba2);
   return 0;
 }
 
-// one predictor-mode sweep (B only, no flag reset, no halo refresh)
+// one predictor-mode sweep (B only, no flag reset, no halo refresh) of the
+// copy bp of B; the first sweep reads B itself, the last converts back
 template <int AX>
-int sweep_predictor(double dt, bool last) {
-  LAUNCH("geom", k_geom<AX>, zchunk_grid(cell_box()), box_block(), S.g, cell_box(), S.vx, S.vy, S.vz, S.cv1, S.cv2, S.db1, dt, 1.0, 1.0, 1.0,
-             last ? 1 : 0, 0);
+int sweep_predictor(double dt, bool first, bool last) {
+  if (last)
+    LAUNCH("geom", (k_geom<AX, 1>), zchunk_grid(cell_box()), box_block(), S.g, cell_box(), S.vx, S.vy, S.vz, S.cv1,
+           S.cv2, S.db1, dt, 1.0, 1.0, 1.0);
+  else
+    LAUNCH("geom", (k_geom<AX, 0>), zchunk_grid(cell_box()), box_block(), S.g, cell_box(), S.vx, S.vy, S.vz, S.cv1,
+           S.cv2, S.db1, dt, 1.0, 1.0, 1.0);
   const real *vel = AX == 0 ? S.vx : (AX == 1 ? S.vy : S.vz);
-  return ct_pair<AX>(S.bpx, S.bpy, S.bpz, vel, dt);
+  double *const Bin[3] = {S.bx, S.by, S.bz};
+  double *const Bp[3] = {S.bpx, S.bpy, S.bpz};
+  return ct_pair<AX>(Bin, Bp, vel, dt, first ? 1 : (last ? -1 : 0));
 }
 
 template <int AX>
-int sweep_full(double dt, double pass[3]) {
+int sweep_full(double dt, double pass[3], bool first) {
   const int N[3] = {nx(), ny(), nz()};
   const real *vel = AX == 0 ? S.vx1 : (AX == 1 ? S.vy1 : S.vz1);
-  LAUNCH("geom", k_geom<AX>, zchunk_grid(cell_box()), box_block(), S.g, cell_box(), S.vx1, S.vy1, S.vz1, S.cv1, S.cv2, S.db1, dt, pass[0], pass[1],
-             pass[2], 1, 1);
-  int rc = ct_pair<AX>(S.bx, S.by, S.bz, vel, dt);
+  LAUNCH("geom", (k_geom<AX, 2>), zchunk_grid(cell_box()), box_block(), S.g, cell_box(), S.vx1, S.vy1, S.vz1, S.cv1,
+         S.cv2, S.db1, dt, pass[0], pass[1], pass[2]);
+  double *const B[3] = {S.bx, S.by, S.bz};
+  int rc = ct_pair<AX>(B, B, vel, dt, first ? 1 : 0);
   if (rc) return rc;
   int hi[3] = {N[0] + 1, N[1] + 1, N[2] + 1};
   hi[AX] = N[AX];
@@ -454,22 +483,25 @@ const int kStrang[6][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}, {0, 2, 1}, {2, 1, 0}
 int eulerian_remap(int step, bool predictor) {
   const double dt = S.dt;
   const int c = ((step % 6) + 6) % 6;
+  // The B -> face flux scaling (remap.f90:44-67) happens inside the first
+  // sweep's CT kernels; the scaling back (remap.f90:102-125) inside the last
+  // predictor sweep's update, and as its own pass after the full remap,
+  // whose sweeps refresh the B halos (on face fluxes) in between.
   if (predictor) {
-    LAUNCH_BOX("bscale", k_bscale, full_box(), S.bx, S.by, S.bz, S.bpx, S.bpy, S.bpz, 1);
     for (int q = 0; q < 3; ++q) {
       const int ax = kStrang[c][q];
-      const bool last = q == 2;
-      int rc = ax == 0 ? sweep_predictor<0>(dt, last) : (ax == 1 ? sweep_predictor<1>(dt, last) : sweep_predictor<2>(dt, last));
+      const bool first = q == 0, last = q == 2;
+      int rc = ax == 0 ? sweep_predictor<0>(dt, first, last)
+                       : (ax == 1 ? sweep_predictor<1>(dt, first, last) : sweep_predictor<2>(dt, first, last));
       if (rc) return rc;
     }
-    LAUNCH_BOX("bscale", k_bscale, full_box(), S.bpx, S.bpy, S.bpz, S.bpx, S.bpy, S.bpz, -1);
     return bfield_bcs(S.bpx, S.bpy, S.bpz);
   }
   double pass[3] = {1.0, 1.0, 1.0};
-  LAUNCH_BOX("bscale", k_bscale, full_box(), S.bx, S.by, S.bz, S.bx, S.by, S.bz, 1);
   for (int q = 0; q < 3; ++q) {
     const int ax = kStrang[c][q];
-    int rc = ax == 0 ? sweep_full<0>(dt, pass) : (ax == 1 ? sweep_full<1>(dt, pass) : sweep_full<2>(dt, pass));
+    const bool first = q == 0;
+    int rc = ax == 0 ? sweep_full<0>(dt, pass, first) : (ax == 1 ? sweep_full<1>(dt, pass, first) : sweep_full<2>(dt, pass, first));
     if (rc) return rc;
   }
   LAUNCH_BOX("bscale", k_bscale, full_box(), S.bx, S.by, S.bz, S.bx, S.by, S.bz, -1);

@@ -1,4 +1,5 @@
-"""Host-side grid of setup.F90:92-247 (uniform, unstretched).
+"""Host-side grid of setup.F90:92-247 (uniform, or with the tanh stretch of
+setup.F90:256-285 on any axis).
 
 The grid stays on the host as in the reference; it produces the 1-D metric
 arrays (xb, xc, dxb, dxc per axis) the device library receives at init.
@@ -8,11 +9,28 @@ import numpy as np
 from .abi import BC_PERIODIC
 
 
-def axis_metrics(n, lo, hi, periodic):
+def tanh_stretch(xb, n, length, new_length):
+    """Stretch node coordinates xb(-2:n+2) (indexed [i+2]) of a uniform axis
+    in place, as stretch_x does (setup.F90:256-285): the cell widths grow by
+    f (1 + tanh((|x| - L) / width)) dx with L = length / 1.5, width =
+    length / 10 and f chosen so that the axis spans new_length."""
+    L = length / 1.5
+    width = length / 10.0
+    f = (new_length - length) / (length - L) / 2.0
+    dx = length / float(n)
+    dxnew = dx + f * (1.0 + np.tanh((np.abs(xb) - L) / width)) * dx
+    for i in range(1, n + 3):
+        xb[i + 2] = xb[i + 1] + dxnew[i + 2]
+
+
+def axis_metrics(n, lo, hi, periodic, stretch=None):
     """Return (xb(-2:n+2), xc(-1:n+2), dxb(-1:n+2), dxc(-1:n+1)) for one axis,
-    following setup.F90:106-149 (one MPI rank, so local == global)."""
+    following setup.F90:106-149 (one MPI rank, so local == global).
+    stretch = the stretched length of the axis, or None for a uniform one."""
     d = (hi - lo) / float(n)
     xb = np.array([lo + float(i) * d for i in range(-2, n + 3)])  # index i+2
+    if stretch is not None:
+        tanh_stretch(xb, n, hi - lo, stretch)
     B = lambda i: xb[i + 2]
     if periodic:
         xb[n + 1 + 2] = B(n) + (B(1) - B(0))
@@ -38,15 +56,16 @@ def axis_metrics(n, lo, hi, periodic):
 class Grid:
     """Sizes, boundary codes, extents and metrics of one run."""
 
-    def __init__(self, nx, ny, nz, extent, bcs):
+    def __init__(self, nx, ny, nz, extent, bcs, stretch=(None, None, None)):
         self.nx, self.ny, self.nz = nx, ny, nz
         self.n = (nx, ny, nz)
         self.extent = extent            # ((x_min,x_max),(y_min,y_max),(z_min,z_max))
         self.bcs = tuple(bcs)           # (xbc_min, xbc_max, ybc_min, ybc_max, zbc_min, zbc_max)
+        self.stretch = tuple(stretch)   # stretched length per axis, or None
         self.xb, self.xc, self.dxb, self.dxc = [], [], [], []
         for a in range(3):
             per = self.bcs[2 * a] == BC_PERIODIC and self.bcs[2 * a + 1] == BC_PERIODIC
-            xb, xc, db, dc = axis_metrics(self.n[a], extent[a][0], extent[a][1], per)
+            xb, xc, db, dc = axis_metrics(self.n[a], extent[a][0], extent[a][1], per, self.stretch[a])
             self.xb.append(xb); self.xc.append(xc); self.dxb.append(db); self.dxc.append(dc)
         self.grav = np.zeros(nz + 4)
 

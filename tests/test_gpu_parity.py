@@ -65,6 +65,9 @@ def test_conduct_heat_arcade():
 CASES = {
     "orszag_tang": lambda: configs.orszag_tang(16),
     "harris": lambda: configs.harris(16),
+    # non-uniform metrics on every axis (tanh stretch), so that a metric read
+    # at a wrong index shows
+    "harris_stretched": lambda: configs.harris(16, stretch=(48.0, 16.0, 48.0)),
     "arcade": lambda: configs.arcade_conduction(16, 16, 24, s_target=7),
 }
 
