@@ -108,16 +108,13 @@ __device__ __forceinline__ double dc1(const Geo &g, const real *__restrict__ db1
   return 0.5 * (db1[c] + db1[s + 1 <= n + 2 ? c + S : c]);
 }
 
-// Van Leer CT flux from its stencil values along the sweep: field fr* at
-// s-1 (clamped), s, s+1, s+2 (clamped), the widths w* averaged along GS at
-// the same points, db1 at s and s+1, and dxc1 at s-1, s, s+1.
-__device__ __forceinline__ double ct_flux_core(double v_advect, double frm, double fri, double frp, double frp2,
-                                               double dbm, double db, double dbp, double dbp2, double db1c,
-                                               double db1p, double dcm, double dci, double dcp, double dt) {
-  const double fm = frm / dbm;
-  const double fi = fri / db;
-  const double fp = frp / dbp;
-  const double fp2 = frp2 / dbp2;
+// Van Leer CT flux (xremap.f90:340-470) from the quotients f* = field /
+// width at s-1 (clamped), s, s+1, s+2 (clamped), the widths db, dbp at s and
+// s+1 (the field's widths averaged along its geometry axis), db1 at s and
+// s+1, and dxc1 at s-1, s, s+1.
+__device__ __forceinline__ double ct_flux_q(double v_advect, double fm, double fi, double fp, double fp2, double db,
+                                            double dbp, double db1c, double db1p, double dcm, double dci, double dcp,
+                                            double dt) {
   const double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
   const double sign_v = copysign(1.0, v_advect);
   const double vad_p = (sign_v + 1.0) * 0.5;
@@ -130,6 +127,12 @@ __device__ __forceinline__ double ct_flux_core(double v_advect, double frm, doub
   const double phi = fabs(v_advect) * dt / du;
   const double Di = vl_di(dfi, dfu, sign_v, dci, dcu, dbu, phi);
   return (fu + Di * (1.0 - phi)) * v_advect * dt;
+}
+// The same from the field values fr* and widths db* at the four points.
+__device__ __forceinline__ double ct_flux_core(double v_advect, double frm, double fri, double frp, double frp2,
+                                               double dbm, double db, double dbp, double dbp2, double db1c,
+                                               double db1p, double dcm, double dci, double dcp, double dt) {
+  return ct_flux_q(v_advect, frm / dbm, fri / db, frp / dbp, frp2 / dbp2, db, dbp, db1c, db1p, dcm, dci, dcp, dt);
 }
 
 // Face B component Q at (i, j, k) as the face flux that the remap moves,
@@ -174,7 +177,8 @@ __device__ __forceinline__ double ct_load(const Geo &g, const double *__restrict
 // first field's velocity is averaged along z (VS1 = z stride) and the
 // second's widths along z (GS2 = z stride), so the kernel marches up
 // z-columns keeping v of plane k-1 and db1 of plane k+1 at the four stencil
-// points for the next plane.  GS1 and VS2 are in-plane strides.
+// points for the next plane.  GS1 and VS2 are in-plane strides.  The x
+// sweep uses it (AX = 0); the y and z sweeps use k_ct_flux_m below.
 template <int AX, int SC>
 __global__ __launch_bounds__(BX * BY) void k_ct_flux(Geo g, Box b, const double *__restrict__ fld1,
                                                      const double *__restrict__ fld2, const real *__restrict__ vel,
@@ -227,64 +231,66 @@ __global__ __launch_bounds__(BX * BY) void k_ct_flux(Geo g, Box b, const double 
   }
 }
 
-// The same fluxes for the z sweep, each thread marching up its column for
-// ZCHUNK planes with the four-plane stencils of both fields and of db1 in
-// registers, so every plane is read from memory once per column chunk.
-template <int SC>
-__global__ __launch_bounds__(BX * BY) void k_ct_flux_z(Geo g, Box b, const double *__restrict__ fld1,
+// The same fluxes for the y and z sweeps (M = 1, 2).  Each thread keeps its
+// (x, other transverse) column and marches along the sweep axis for ZCHUNK
+// points, holding the four-point stencils of db1, of both fields' widths
+// and of their quotients field / width in registers.  A quotient is formed
+// once per point and serves as the s+2, s+1, s and s-1 value of four
+// successive fluxes; the expressions are those of ct_flux_core.
+template <int M, int SC>
+__global__ __launch_bounds__(BX * BY) void k_ct_flux_m(Geo g, Box b, const double *__restrict__ fld1,
                                                        const double *__restrict__ fld2,
                                                        const real *__restrict__ vel, const real *__restrict__ db1,
                                                        double *__restrict__ F1, double *__restrict__ F2, int VS1,
                                                        int GS1, int VS2, int GS2, double dt) {
-  ZCOLUMN(b, i, j, k0, k1);
-  const int n = g.nz;
-  // window over planes k-1, k, k+1, k+2, each clamped to [-1, n+2]
-  double f1[4], f2[4];
-  real d[4], w1[4], w2[4];
+  constexpr int T1 = CtComp<M>::T1, T2 = CtComp<M>::T2;
+  // the launch box with the march axis last: (x, P, M), P the other axis
+  const Box pb = M == 2 ? b : Box{b.x0, b.x1, b.z0, b.z1, b.y0, b.y1};
+  PLANE_POS(pb, i, p)
+  ZRANGE(pb, m0, m1)
+  if (p > pb.y1) return;
+  const int n = g.n(M);
+  // index, and quotients of both fields, at march coordinate m
+  auto at = [&](int m) { return M == 2 ? g.idx(i, p, m) : g.idx(i, m, p); };
+  double q1[4], q2[4], w1[4], w2[4];
+  real d[4];
+  auto load = [&](int slot, int m) {
+    const int c = at(m);
+    const int jj = M == 1 ? m : p, kk = M == 2 ? m : p;
+    d[slot] = db1[c];
+    w1[slot] = (d[slot] + db1[c + GS1]) * 0.5;
+    w2[slot] = (d[slot] + db1[c + GS2]) * 0.5;
+    q1[slot] = ct_load<M, T1, SC>(g, fld1, c, m, i, jj, kk) / w1[slot];
+    q2[slot] = ct_load<M, T2, SC>(g, fld2, c, m, i, jj, kk) / w2[slot];
+  };
+  // window over m-1, m, m+1, m+2, each clamped to [-1, n+2]
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    int kk = k0 - 1 + q;
-    kk = kk < -1 ? -1 : (kk > n + 2 ? n + 2 : kk);
-    const int cq = g.idx(i, j, kk);
-    f1[q] = ct_load<2, 0, SC>(g, fld1, cq, kk, i, j, kk);
-    f2[q] = ct_load<2, 1, SC>(g, fld2, cq, kk, i, j, kk);
-    d[q] = db1[cq];
-    w1[q] = db1[cq + GS1];
-    w2[q] = db1[cq + GS2];
+    int mm = m0 - 1 + q;
+    mm = mm < -1 ? -1 : (mm > n + 2 ? n + 2 : mm);
+    load(q, mm);
   }
-  for (int k = k0; k <= k1; ++k) {
-    const int c = g.idx(i, j, k);
+  for (int m = m0; m <= m1; ++m) {
+    const int c = at(m);
     // dxc1 at the clamped s-1 (its upper point is s, or s+1 when s = -1),
     // at s, and at s+1 (upper point clamped to n+2, as d[3] already is)
-    const double dcm = 0.5 * (d[0] + (k > -1 ? d[1] : d[2]));
+    const double dcm = 0.5 * (d[0] + (m > -1 ? d[1] : d[2]));
     const double dci = 0.5 * (d[1] + d[2]);
     const double dcp = 0.5 * (d[2] + d[3]);
-    {
-      const double v_advect = (vel[c] + vel[c - VS1]) * 0.5;
-      F1[c] = ct_flux_core(v_advect, f1[0], f1[1], f1[2], f1[3], (d[0] + w1[0]) * 0.5, (d[1] + w1[1]) * 0.5,
-                           (d[2] + w1[2]) * 0.5, (d[3] + w1[3]) * 0.5, d[1], d[2], dcm, dci, dcp, dt);
-    }
-    {
-      const double v_advect = (vel[c] + vel[c - VS2]) * 0.5;
-      F2[c] = ct_flux_core(v_advect, f2[0], f2[1], f2[2], f2[3], (d[0] + w2[0]) * 0.5, (d[1] + w2[1]) * 0.5,
-                           (d[2] + w2[2]) * 0.5, (d[3] + w2[3]) * 0.5, d[1], d[2], dcm, dci, dcp, dt);
-    }
-    int kn = k + 3;
-    kn = kn > n + 2 ? n + 2 : kn;
-    const int cn = g.idx(i, j, kn);
+    const real vc = vel[c];
+    F1[c] = ct_flux_q((vc + vel[c - VS1]) * 0.5, q1[0], q1[1], q1[2], q1[3], w1[1], w1[2], d[1], d[2], dcm, dci, dcp,
+                      dt);
+    F2[c] = ct_flux_q((vc + vel[c - VS2]) * 0.5, q2[0], q2[1], q2[2], q2[3], w2[1], w2[2], d[1], d[2], dcm, dci, dcp,
+                      dt);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      f1[q] = f1[q + 1];
-      f2[q] = f2[q + 1];
-      d[q] = d[q + 1];
+      q1[q] = q1[q + 1];
+      q2[q] = q2[q + 1];
       w1[q] = w1[q + 1];
       w2[q] = w2[q + 1];
+      d[q] = d[q + 1];
     }
-    f1[3] = ct_load<2, 0, SC>(g, fld1, cn, kn, i, j, kn);
-    f2[3] = ct_load<2, 1, SC>(g, fld2, cn, kn, i, j, kn);
-    d[3] = db1[cn];
-    w1[3] = db1[cn + GS1];
-    w2[3] = db1[cn + GS2];
+    if (m < m1) load(3, m + 3 > n + 2 ? n + 2 : m + 3);
   }
 }
 

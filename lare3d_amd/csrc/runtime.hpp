@@ -380,21 +380,23 @@ int ct_pair(double *const Bin[3], double *const B[3], const real *vel, double dt
     t1 = 0; va1 = 1; ga1 = 0;  // vz_bx_flux
     t2 = 1; va2 = 0; ga2 = 1;  // vz_by_flux
   }
-  if constexpr (AX == 2) {
-    if (sc > 0)
-      LAUNCH("ct_flux", k_ct_flux_z<1>, zchunk_grid(fb), box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1, S.F,
-             S.F2, strides(va1), strides(ga1), strides(va2), strides(ga2), dt);
-    else
-      LAUNCH("ct_flux", k_ct_flux_z<0>, zchunk_grid(fb), box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1, S.F,
-             S.F2, strides(va1), strides(ga1), strides(va2), strides(ga2), dt);
-  } else {
-    // x and y sweeps: va1 = z and ga2 = z (see the flux table above)
+  if constexpr (AX == 0) {
+    // x sweep: va1 = z and ga2 = z (see the flux table above)
     if (sc > 0)
       LAUNCH("ct_flux", (k_ct_flux<AX, 1>), zchunk_grid(fb), box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1, S.F,
              S.F2, strides(ga1), strides(va2), dt);
     else
       LAUNCH("ct_flux", (k_ct_flux<AX, 0>), zchunk_grid(fb), box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1, S.F,
              S.F2, strides(ga1), strides(va2), dt);
+  } else {
+    // y and z sweeps march along the sweep axis; the grid covers the other two
+    const Box gb = AX == 2 ? fb : mkbox(fb.x0, fb.x1, fb.z0, fb.z1, fb.y0, fb.y1);
+    if (sc > 0)
+      LAUNCH("ct_flux", (k_ct_flux_m<AX, 1>), zchunk_grid(gb), box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1,
+             S.F, S.F2, strides(va1), strides(ga1), strides(va2), strides(ga2), dt);
+    else
+      LAUNCH("ct_flux", (k_ct_flux_m<AX, 0>), zchunk_grid(gb), box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1,
+             S.F, S.F2, strides(va1), strides(ga1), strides(va2), strides(ga2), dt);
   }
   if (sc == 0)
     LAUNCH_BOX("ct_update", (k_ct_update<AX, 0>), ub, S.F, S.F2, B[t1], B[t2], B[AX], B[t1], B[t2], B[AX],
