@@ -118,6 +118,18 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], const int *o
   ZRANGE(b, k0, k1)              \
   if (j > (b).y1) return;
 
+// Columns of a launch that marches along axis M (1 = y, 2 = z) for ZCHUNK
+// points: the flattened tile covers x and the other transverse axis, whose
+// index is p; MJ/MK give j and k at march coordinate m.  The grid is the
+// zchunk grid of the box with its y and z ranges swapped when M = 1.
+#define MCOLUMN(M, b, i, p, m0, m1)                                                         \
+  const Box pb_ = (M) == 2 ? (b) : Box{(b).x0, (b).x1, (b).z0, (b).z1, (b).y0, (b).y1};   \
+  PLANE_POS(pb_, i, p)                                                                     \
+  ZRANGE(pb_, m0, m1)                                                                      \
+  if (p > pb_.y1) return;
+#define MJ(M, m, p) ((M) == 1 ? (m) : (p))
+#define MK(M, m, p) ((M) == 2 ? (m) : (p))
+
 // A field at the four in-plane points (i, j), (i-1, j), (i, j-1), (i-1, j-1).
 struct Quad {
   real p0, px, py, pxy;

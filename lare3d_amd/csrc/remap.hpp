@@ -402,36 +402,39 @@ __global__ __launch_bounds__(BX * BY) void k_mass_flux(Geo g, Box b, const real 
   dm[c] = (fu + Di * (1.0 - phi)) * v_advect * dt * area;
 }
 
-// The z-sweep mass flux, marching in z with rho and db1 of planes k-1..k+2
-// in registers (box planes 0..n, so no clamping is needed).
-__global__ __launch_bounds__(BX * BY) void k_mass_flux_z(Geo g, Box b, const real *__restrict__ rho,
+// The y- and z-sweep mass flux (M = 1, 2), marching along the sweep axis
+// with rho and db1 of points m-1..m+2 in registers (sweep range 0..n, so no
+// clamping is needed).
+template <int M>
+__global__ __launch_bounds__(BX * BY) void k_mass_flux_m(Geo g, Box b, const real *__restrict__ rho,
                                                          const real *__restrict__ vel, const real *__restrict__ db1,
                                                          real *__restrict__ dm, double dt) {
-  ZCOLUMN(b, i, j, k0, k1);
-  const int Z = g.sz;
+  MCOLUMN(M, b, i, p, m0, m1)
+  const int MS = M == 2 ? g.sz : g.sy;
   real r[4], d[4];
-  const int cb = g.idx(i, j, k0 - 1);
+  const int cb = g.idx(i, MJ(M, m0 - 1, p), MK(M, m0 - 1, p));
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    r[q] = rho[cb + q * Z];
-    d[q] = db1[cb + q * Z];
+    r[q] = rho[cb + q * MS];
+    d[q] = db1[cb + q * MS];
   }
-  for (int k = k0; k <= k1; ++k) {
+  for (int m = m0; m <= m1; ++m) {
+    const int j = MJ(M, m, p), k = MK(M, m, p);
     const int c = g.idx(i, j, k);
-    const double area = face_area<2>(g, i, j, k);
-    const double v_advect = face_velocity<2>(g, vel, c);
+    const double area = face_area<M>(g, i, j, k);
+    const double v_advect = face_velocity<M>(g, vel, c);
     double vad_p, vad_m, dbu, fu, Di, phi;
     vl_cell(v_advect, r[0], r[1], r[2], r[3], 0.5 * (d[0] + d[1]), 0.5 * (d[1] + d[2]), 0.5 * (d[2] + d[3]), d[1],
             d[2], dt, vad_p, vad_m, dbu, fu, Di, phi);
     dm[c] = (fu + Di * (1.0 - phi)) * v_advect * dt * area;
-    if (k < k1) {
+    if (m < m1) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         r[q] = r[q + 1];
         d[q] = d[q + 1];
       }
-      r[3] = rho[c + 3 * Z];
-      d[3] = db1[c + 3 * Z];
+      r[3] = rho[c + 3 * MS];
+      d[3] = db1[c + 3 * MS];
     }
   }
 }
@@ -457,29 +460,31 @@ __global__ __launch_bounds__(BX * BY) void k_energy_flux(Geo g, Box b, const rea
   F[c] = (fu + Di * (1.0 - dmu)) * d;
 }
 
-// The z-sweep energy flux, marching in z (energy and db1 of planes k-1..k+2,
-// rho1 of k and k+1 in registers).
-__global__ __launch_bounds__(BX * BY) void k_energy_flux_z(Geo g, Box b, const real *__restrict__ energy,
+// The y- and z-sweep energy flux (M = 1, 2), marching along the sweep axis
+// (energy and db1 of points m-1..m+2, rho1 of m and m+1 in registers).
+template <int M>
+__global__ __launch_bounds__(BX * BY) void k_energy_flux_m(Geo g, Box b, const real *__restrict__ energy,
                                                            const real *__restrict__ rho1,
                                                            const real *__restrict__ vel,
                                                            const real *__restrict__ db1,
                                                            const real *__restrict__ dm, real *__restrict__ F,
                                                            double dt) {
-  ZCOLUMN(b, i, j, k0, k1);
-  const int Z = g.sz;
+  MCOLUMN(M, b, i, p, m0, m1)
+  const int MS = M == 2 ? g.sz : g.sy;
   real e[4], d[4], r0, r1;
-  const int cb = g.idx(i, j, k0 - 1);
+  const int cb = g.idx(i, MJ(M, m0 - 1, p), MK(M, m0 - 1, p));
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    e[q] = energy[cb + q * Z];
-    d[q] = db1[cb + q * Z];
+    e[q] = energy[cb + q * MS];
+    d[q] = db1[cb + q * MS];
   }
-  r0 = rho1[cb + Z];
-  r1 = rho1[cb + 2 * Z];
-  for (int k = k0; k <= k1; ++k) {
+  r0 = rho1[cb + MS];
+  r1 = rho1[cb + 2 * MS];
+  for (int m = m0; m <= m1; ++m) {
+    const int j = MJ(M, m, p), k = MK(M, m, p);
     const int c = g.idx(i, j, k);
-    const double area = face_area<2>(g, i, j, k);
-    const double v_advect = face_velocity<2>(g, vel, c);
+    const double area = face_area<M>(g, i, j, k);
+    const double v_advect = face_velocity<M>(g, vel, c);
     double vad_p, vad_m, dbu, fu, Di, phi;
     vl_cell(v_advect, e[0], e[1], e[2], e[3], 0.5 * (d[0] + d[1]), 0.5 * (d[1] + d[2]), 0.5 * (d[2] + d[3]), d[1],
             d[2], dt, vad_p, vad_m, dbu, fu, Di, phi);
@@ -487,16 +492,16 @@ __global__ __launch_bounds__(BX * BY) void k_energy_flux_z(Geo g, Box b, const r
     const double dd = dm[c];
     const double dmu = fabs(dd) / area / dbu / rhou;
     F[c] = (fu + Di * (1.0 - dmu)) * dd;
-    if (k < k1) {
+    if (m < m1) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         e[q] = e[q + 1];
         d[q] = d[q + 1];
       }
-      e[3] = energy[c + 3 * Z];
-      d[3] = db1[c + 3 * Z];
+      e[3] = energy[c + 3 * MS];
+      d[3] = db1[c + 3 * MS];
       r0 = r1;
-      r1 = rho1[c + 2 * Z];
+      r1 = rho1[c + 2 * MS];
     }
   }
 }
@@ -629,35 +634,37 @@ __global__ __launch_bounds__(BX * BY) void k_mom_flux(Geo g, Box b, const real *
   }
 }
 
-// The z-sweep momentum fluxes, marching in z: the three velocity components
-// of planes k-1..k+2, db1 of k..k+2, the sweep velocity and rho_v of k and
-// k+1 in registers.
-__global__ __launch_bounds__(BX * BY) void k_mom_flux_z(Geo g, Box b, const real *__restrict__ vx,
+// The y- and z-sweep momentum fluxes (M = 1, 2), marching along the sweep
+// axis: the three velocity components of points m-1..m+2, db1 of m..m+2,
+// the sweep velocity and rho_v of m and m+1 in registers.
+template <int M>
+__global__ __launch_bounds__(BX * BY) void k_mom_flux_m(Geo g, Box b, const real *__restrict__ vx,
                                                         const real *__restrict__ vy, const real *__restrict__ vz,
                                                         const real *__restrict__ vel1,
                                                         const real *__restrict__ db1,
                                                         const real *__restrict__ rho_v,
                                                         const real *__restrict__ dmv, real *__restrict__ Fx,
                                                         real *__restrict__ Fy, real *__restrict__ Fz, double dt) {
-  ZCOLUMN(b, i, j, k0, k1);
-  const int Z = g.sz;
+  MCOLUMN(M, b, i, p, m0, m1)
+  const int MS = M == 2 ? g.sz : g.sy;
   real u[3][4], d[3], v0, v1, rv0, rv1;
   const real *V[3] = {vx, vy, vz};
   real *FF[3] = {Fx, Fy, Fz};
-  const int cb = g.idx(i, j, k0);
+  const int cb = g.idx(i, MJ(M, m0, p), MK(M, m0, p));
 #pragma unroll
   for (int q = 0; q < 3; ++q)
 #pragma unroll
-    for (int p = 0; p < 4; ++p) u[q][p] = V[q][cb + (p - 1) * Z];
+    for (int t = 0; t < 4; ++t) u[q][t] = V[q][cb + (t - 1) * MS];
 #pragma unroll
-  for (int p = 0; p < 3; ++p) d[p] = db1[cb + p * Z];
+  for (int t = 0; t < 3; ++t) d[t] = db1[cb + t * MS];
   v0 = vel1[cb];
-  v1 = vel1[cb + Z];
+  v1 = vel1[cb + MS];
   rv0 = rho_v[cb];
-  rv1 = rho_v[cb + Z];
-  for (int k = k0; k <= k1; ++k) {
+  rv1 = rho_v[cb + MS];
+  for (int m = m0; m <= m1; ++m) {
+    const int j = MJ(M, m, p), k = MK(M, m, p);
     const int c = g.idx(i, j, k);
-    const double area = mom_area<2>(g, i, j, k);
+    const double area = mom_area<M>(g, i, j, k);
     const double v_advect = (v0 + v1) * 0.5;
     const double sign_v = copysign(1.0, v_advect);
     const double vad_p = (sign_v + 1.0) * 0.5;
@@ -679,21 +686,21 @@ __global__ __launch_bounds__(BX * BY) void k_mom_flux_z(Geo g, Box b, const real
       const double f = fu + Di * (1.0 - dmu);
       FF[q][c] = f * dd;
     }
-    if (k < k1) {
+    if (m < m1) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         u[q][0] = u[q][1];
         u[q][1] = u[q][2];
         u[q][2] = u[q][3];
-        u[q][3] = V[q][c + 3 * Z];
+        u[q][3] = V[q][c + 3 * MS];
       }
       d[0] = d[1];
       d[1] = d[2];
-      d[2] = db1[c + 3 * Z];
+      d[2] = db1[c + 3 * MS];
       v0 = v1;
-      v1 = vel1[c + 2 * Z];
+      v1 = vel1[c + 2 * MS];
       rv0 = rv1;
-      rv1 = rho_v[c + 2 * Z];
+      rv1 = rho_v[c + 2 * MS];
     }
   }
 }

@@ -146,6 +146,10 @@ dim3 zchunk_grid(const Box &b) {
   return dim3(plane_blocks(b), 1, (unsigned)((b.z1 - b.z0 + 1 + ZCHUNK - 1) / ZCHUNK));
 }
 
+// the box whose zchunk grid a launch marching along axis M (1, 2) uses:
+// y and z ranges swapped for M = 1 (MCOLUMN)
+Box march_box(const Box &b, int M) { return M == 2 ? b : mkbox(b.x0, b.x1, b.z0, b.z1, b.y0, b.y1); }
+
 // reduction over partials of a grid launched from index 0 on every axis
 dim3 grid_from0(int x1, int y1, int z1) { return box_grid(mkbox(0, x1, 0, y1, 0, z1)); }
 
@@ -390,7 +394,7 @@ int ct_pair(double *const Bin[3], double *const B[3], const real *vel, double dt
              S.F2, strides(ga1), strides(va2), dt);
   } else {
     // y and z sweeps march along the sweep axis; the grid covers the other two
-    const Box gb = AX == 2 ? fb : mkbox(fb.x0, fb.x1, fb.z0, fb.z1, fb.y0, fb.y1);
+    const Box gb = march_box(fb, AX);
     if (sc > 0)
       LAUNCH("ct_flux", (k_ct_flux_m<AX, 1>), zchunk_grid(gb), box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1,
              S.F, S.F2, strides(va1), strides(ga1), strides(va2), strides(ga2), dt);
@@ -438,8 +442,10 @@ int sweep_full(double dt, double pass[3], bool first) {
   int hi[3] = {N[0] + 1, N[1] + 1, N[2] + 1};
   hi[AX] = N[AX];
   Box mb = mkbox(0, hi[0], 0, hi[1], 0, hi[2]);
-  if (AX == 2) LAUNCH("mass_flux", k_mass_flux_z, zchunk_grid(mb), box_block(), S.g, mb, S.rho, vel, S.db1, S.dm, dt);
-  else LAUNCH_BOX("mass_flux", k_mass_flux<AX>, mb, S.rho, vel, S.db1, S.dm, dt);
+  // the y and z sweeps' flux kernels march along the sweep axis
+  if constexpr (AX == 0) LAUNCH_BOX("mass_flux", k_mass_flux<AX>, mb, S.rho, vel, S.db1, S.dm, dt);
+  else LAUNCH("mass_flux", k_mass_flux_m<AX>, zchunk_grid(march_box(mb, AX)), box_block(), S.g, mb, S.rho, vel,
+              S.db1, S.dm, dt);
   {
     BcSet s = {};
     s.nf = 1;
@@ -449,10 +455,9 @@ int sweep_full(double dt, double pass[3], bool first) {
     rc = launch_bc_axis(s, AX, "bc_dm");
     if (rc) return rc;
   }
-  if (AX == 2)
-    LAUNCH("energy_flux", k_energy_flux_z, zchunk_grid(nodes0n()), box_block(), S.g, nodes0n(), S.energy, S.rho, vel,
-           S.db1, S.dm, S.Fe, dt);
-  else LAUNCH_BOX("energy_flux", k_energy_flux<AX>, nodes0n(), S.energy, S.rho, vel, S.db1, S.dm, S.Fe, dt);
+  if constexpr (AX == 0) LAUNCH_BOX("energy_flux", k_energy_flux<AX>, nodes0n(), S.energy, S.rho, vel, S.db1, S.dm, S.Fe, dt);
+  else LAUNCH("energy_flux", k_energy_flux_m<AX>, zchunk_grid(march_box(nodes0n(), AX)), box_block(), S.g, nodes0n(),
+              S.energy, S.rho, vel, S.db1, S.dm, S.Fe, dt);
   LAUNCH_BOX("rho_energy_update", k_rho_energy_update<AX>, interior(), S.rho, S.rho_alt, S.energy, S.cv1, S.cv2,
              S.dm, S.Fe);
   {
@@ -464,10 +469,10 @@ int sweep_full(double dt, double pass[3], bool first) {
            S.cv2v, S.dmv);
     h2[AX] = N[AX];
     Box fb = mkbox(lo[0], h2[0], lo[1], h2[1], lo[2], h2[2]);
-    if (AX == 2)
-      LAUNCH("mom_flux", k_mom_flux_z, zchunk_grid(fb), box_block(), S.g, fb, S.vx, S.vy, S.vz, vel, S.db1, S.rho_vr,
-             S.dmv, S.Fx, S.Fy, S.Fz, dt);
-    else LAUNCH_BOX("mom_flux", k_mom_flux<AX>, fb, S.vx, S.vy, S.vz, vel, S.db1, S.rho_vr, S.dmv, S.Fx, S.Fy, S.Fz, dt);
+    if constexpr (AX == 0)
+      LAUNCH_BOX("mom_flux", k_mom_flux<AX>, fb, S.vx, S.vy, S.vz, vel, S.db1, S.rho_vr, S.dmv, S.Fx, S.Fy, S.Fz, dt);
+    else LAUNCH("mom_flux", k_mom_flux_m<AX>, zchunk_grid(march_box(fb, AX)), box_block(), S.g, fb, S.vx, S.vy, S.vz,
+                vel, S.db1, S.rho_vr, S.dmv, S.Fx, S.Fy, S.Fz, dt);
   }
   LAUNCH_BOX("mom_update", k_mom_update<AX>, nodes0n(), S.vx, S.vy, S.vz, S.rho_vr, S.cv1v, S.cv2v, S.dmv, S.Fx,
              S.Fy, S.Fz);
