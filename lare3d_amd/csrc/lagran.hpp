@@ -411,18 +411,21 @@ __global__ __launch_bounds__(BX * BY) void k_fill(Geo g, Box b, real *__restrict
   a[g.idx(i, j, k)] = v;
 }
 
-// rkstep (lagran.F90:1472-1503) over nodes 0..n, with the
-// total_ohmic_heating increments of :1309-1328 reduced in the same pass;
-// marches in z (box nodes 0..n).
+// rkstep (lagran.F90:1472-1503) over nodes 0..n, with the eta_calc of the
+// same substep (lagran.F90:1176-1230) fused in: the vertex current at each
+// node of the box (-1..n+1) sets eta there exactly as k_eta does, and at
+// nodes 0..n also gives the EMFs and curlb; the total_ohmic_heating
+// increments of :1309-1328 are reduced in the same pass.  Marches in z.
 __global__ __launch_bounds__(BX * BY) void k_rkstep(Geo g, Box b, const double *__restrict__ bx, const double *__restrict__ by,
-                         const double *__restrict__ bz, const real *__restrict__ eta, double *__restrict__ fxr,
+                         const double *__restrict__ bz, real *__restrict__ eta, double *__restrict__ fxr,
                          double *__restrict__ fyr, double *__restrict__ fzr, real *__restrict__ curlb,
-                         double *__restrict__ partial, double dt) {
+                         double *__restrict__ partial, double dt, double eta_bg, double eta0, double j_max) {
   PLANE_POS(b, i, j)
   ZRANGE(b, k0, k1)
   double v[1] = {0.0};
   if (j <= b.y1) {
     const int Y = g.sy, Z = g.sz;
+    const bool inxy = i >= 0 && i <= g.nx && j >= 0 && j <= g.ny;
     BPlane lo = bplane(bx, by, g.idx(i, j, k0), Y);
     for (int k = k0; k <= k1; ++k) {
       const int c = g.idx(i, j, k);
@@ -431,17 +434,21 @@ __global__ __launch_bounds__(BX * BY) void k_rkstep(Geo g, Box b, const double *
       vertex_current_vals(g.dxc[i], g.dyc[j], g.dzc[k], lo, hi, bz[c], bz[c + 1], bz[c + Y], bz[c + 1 + Y], jx, jy,
                           jz);
       lo = hi;
-      const double e = eta[c];
-      fxr[c] = -jx * e * g.dxc[i] * 0.5;
-      fyr[c] = -jy * e * g.dyc[j] * 0.5;
-      fzr[c] = -jz * e * g.dzc[k] * 0.5;
-      const double cb = e * (jx * jx + jy * jy + jz * jz);
-      curlb[c] = cb;
-      double w1 = dt * g.dxc[i] * g.dyc[j] * g.dzc[k] * cb;
-      if (i == 0 || i == g.nx) w1 = w1 * 0.5;
-      if (j == 0 || j == g.ny) w1 = w1 * 0.5;
-      if (k == 0 || k == g.nz) w1 = w1 * 0.5;
-      v[0] += w1;
+      const real es = sqrt(jx * jx + jy * jy + jz * jz) > j_max ? eta_bg + eta0 : eta_bg;
+      eta[c] = es;
+      if (inxy && k >= 0 && k <= g.nz) {
+        const double e = es;
+        fxr[c] = -jx * e * g.dxc[i] * 0.5;
+        fyr[c] = -jy * e * g.dyc[j] * 0.5;
+        fzr[c] = -jz * e * g.dzc[k] * 0.5;
+        const double cb = e * (jx * jx + jy * jy + jz * jz);
+        curlb[c] = cb;
+        double w1 = dt * g.dxc[i] * g.dyc[j] * g.dzc[k] * cb;
+        if (i == 0 || i == g.nx) w1 = w1 * 0.5;
+        if (j == 0 || j == g.ny) w1 = w1 * 0.5;
+        if (k == 0 || k == g.nz) w1 = w1 * 0.5;
+        v[0] += w1;
+      }
     }
   }
   const int ops[1] = {1};

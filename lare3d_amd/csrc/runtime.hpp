@@ -609,21 +609,22 @@ int lagrangian_step() {
     S.last_substeps = substeps;
     const double actual_dt = S.dt;
     S.dt = S.dt / (double)substeps;
+    // eta_calc of each substep runs inside k_rkstep (over the eta box
+    // -1..n+1); the B and energy halos of a substep are refreshed together
+    // (bstep does not read energy, ohmic does not read B)
+    const Box eb = mkbox(-1, nx() + 1, -1, ny() + 1, -1, nz() + 1);
     for (int sc = 1; sc <= substeps; ++sc) {
-      rc = eta_calc();
-      if (rc) return rc;
-      dim3 g2 = zchunk_grid(nodes0n());
-      LAUNCH("rkstep", k_rkstep, g2, box_block(), S.g, nodes0n(), S.bx, S.by, S.bz, S.eta, S.fxr, S.fyr, S.fzr,
-             S.curlb, S.partial, S.dt);
+      dim3 g2 = zchunk_grid(eb);
+      LAUNCH("rkstep", k_rkstep, g2, box_block(), S.g, eb, S.bx, S.by, S.bz, S.eta, S.fxr, S.fyr, S.fzr, S.curlb,
+             S.partial, S.dt, S.p.eta_background, S.p.eta0, S.p.j_max);
       rc = final_reduce(g2, 1, 0, 9, 1);
       if (rc) return rc;
       LAUNCH("bstep", k_bstep, zchunk_grid(nodes0n()), box_block(), S.g, nodes0n(), S.bx, S.by, S.bz, S.fxr, S.fyr,
              S.fzr, S.dt);
-      rc = bfield_bcs(S.bx, S.by, S.bz);
-      if (rc) return rc;
       LAUNCH("ohmic", k_ohmic, zchunk_grid(cell_box()), box_block(), S.g, cell_box(), S.energy, S.ohmic_dep, S.rho,
              S.curlb, S.dt);
-      rc = energy_bcs();
+      BcField f[4] = {{S.bx, FK_BX}, {S.by, FK_BY}, {S.bz, FK_BZ}, {S.energy, FK_CELL}};
+      rc = apply_bcs(f, 4, "bc_resistive");
       if (rc) return rc;
     }
     S.dt = actual_dt;
