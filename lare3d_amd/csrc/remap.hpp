@@ -128,12 +128,6 @@ __device__ __forceinline__ double ct_flux_q(double v_advect, double fm, double f
   const double Di = vl_di(dfi, dfu, sign_v, dci, dcu, dbu, phi);
   return (fu + Di * (1.0 - phi)) * v_advect * dt;
 }
-// The same from the field values fr* and widths db* at the four points.
-__device__ __forceinline__ double ct_flux_core(double v_advect, double frm, double fri, double frp, double frp2,
-                                               double dbm, double db, double dbp, double dbp2, double db1c,
-                                               double db1p, double dcm, double dci, double dcp, double dt) {
-  return ct_flux_q(v_advect, frm / dbm, fri / db, frp / dbp, frp2 / dbp2, db, dbp, db1c, db1p, dcm, dci, dcp, dt);
-}
 
 // Face B component Q at (i, j, k) as the face flux that the remap moves,
 // and back (remap.f90:46-48 and 104-106, same operation order), with the
@@ -172,62 +166,74 @@ __device__ __forceinline__ double ct_load(const Geo &g, const double *__restrict
   return to_flux<Q>(g, v, AX == 0 ? s : i, AX == 1 ? s : j, AX == 2 ? s : k);
 }
 
-// CT fluxes of the two transverse face fields of an x or y sweep (AX = 0, 1:
-// vx_by_flux and vx_bz_flux, vy_bx_flux and vy_bz_flux) in one pass.  The
-// first field's velocity is averaged along z (VS1 = z stride) and the
-// second's widths along z (GS2 = z stride), so the kernel marches up
-// z-columns keeping v of plane k-1 and db1 of plane k+1 at the four stencil
-// points for the next plane.  GS1 and VS2 are in-plane strides.  The x
-// sweep uses it (AX = 0); the y and z sweeps use k_ct_flux_m below.
-template <int AX, int SC>
-__global__ __launch_bounds__(BX * BY) void k_ct_flux(Geo g, Box b, const double *__restrict__ fld1,
-                                                     const double *__restrict__ fld2, const real *__restrict__ vel,
-                                                     const real *__restrict__ db1, double *__restrict__ F1,
-                                                     double *__restrict__ F2, int GS1, int VS2, double dt) {
-  ZCOLUMN(b, i, j, k0, k1);
-  constexpr int T1 = CtComp<AX>::T1, T2 = CtComp<AX>::T2;
-  const int S = sstride<AX>(g);
+// CT fluxes of the two transverse face fields of the x sweep (vx_by_flux and
+// vx_bz_flux, xremap.f90:340-470) in one pass, the stencil running along the
+// lanes.
+// Each wave covers 64 consecutive points of the (x, y) plane flattened over
+// x = x0..n+2 (one column beyond the flux box), successive waves overlapping
+// by three points.  Every lane forms the quotients field / width of both
+// fields at its own point, and lanes 1..CTX_OUT take the values at s-1, s+1
+// and s+2 from their neighbours by wavefront shuffles, so each quotient is
+// divided once.  The kernel marches in z with v of plane k-1 and db1 of
+// plane k+1 in registers.  GS1 and VS2 are the y stride (by's widths, bz's
+// velocity average); by's velocity and bz's widths are averaged along z.
+#define CTX_OUT 61
+template <int SC>
+__global__ __launch_bounds__(BX * BY) void k_ct_flux_x(Geo g, Box b, const double *__restrict__ fld1,
+                                                       const double *__restrict__ fld2,
+                                                       const real *__restrict__ vel, const real *__restrict__ db1,
+                                                       double *__restrict__ F1, double *__restrict__ F2, int GS1,
+                                                       int VS2, double dt) {
+  constexpr int T1 = CtComp<0>::T1, T2 = CtComp<0>::T2;
+  const int lane = (int)threadIdx.x;  // BX = 64: threadIdx.y is the wave
+  const int wq = g.nx + 2 - b.x0 + 1;  // row of quotient points x0..n+2
+  const int npts = wq * (b.y1 - b.y0 + 1);
+  const int f = (int)(blockIdx.x * BY + threadIdx.y) * CTX_OUT + lane - 1;
+  const bool valid = f >= 0 && f < npts;
+  const int ff = valid ? f : 0;  // lanes past the plane read point 0, unused
+  const int jr = ff / wq;
+  const int i = b.x0 + (ff - jr * wq);
+  const int j = b.y0 + jr;
+  const bool out = valid && lane >= 1 && lane <= CTX_OUT && i <= b.x1;
+  ZRANGE(b, k0, k1)
   const int Z = g.sz;
-  const int n = g.n(AX);
-  const int s = coord<AX>(i, j, 0);
-  const int sm = s - 1 > -1 ? s - 1 : -1;
-  const int sp2 = s + 2 < n + 2 ? s + 2 : n + 2;
-  const int om = (sm - s) * S, op2 = (sp2 - s) * S;  // offsets of the clamped s-1 and s+2 points
-  // stencil points along AX: m (s-1 clamped), i (s), p (s+1), q (s+2 clamped)
-  const int cb = g.idx(i, j, k0);
-  real vm = vel[cb - Z];                          // v(c - Z) for the first field
-  real dz[4] = {db1[cb + om + Z], db1[cb + Z], db1[cb + S + Z], db1[cb + op2 + Z]};  // db1(. + Z) for the second
-  for (int k = k0; k <= k1; ++k) {
-    const int c = g.idx(i, j, k);
-    const int cm = c + om, cp = c + S, cp2 = c + op2;
-    const real d0 = db1[cm], d1 = db1[c], d2 = db1[cp], d3 = db1[cp2];
-    const double dcm = 0.5 * (d0 + db1[sm + 1 <= n + 2 ? cm + S : cm]);
-    const double dci = 0.5 * (d1 + db1[s + 1 <= n + 2 ? c + S : c]);
-    const double dcp = 0.5 * (d2 + db1[s + 2 <= n + 2 ? cp + S : cp]);
-    const real vc = vel[c];
-    {
-      const double v_advect = (vc + vm) * 0.5;
-      F1[c] = ct_flux_core(v_advect, ct_load<AX, T1, SC>(g, fld1, cm, sm, i, j, k),
-                           ct_load<AX, T1, SC>(g, fld1, c, s, i, j, k), ct_load<AX, T1, SC>(g, fld1, cp, s + 1, i, j, k),
-                           ct_load<AX, T1, SC>(g, fld1, cp2, sp2, i, j, k), (d0 + db1[cm + GS1]) * 0.5,
-                           (d1 + db1[c + GS1]) * 0.5, (d2 + db1[cp + GS1]) * 0.5, (d3 + db1[cp2 + GS1]) * 0.5, d1,
-                           d2, dcm, dci, dcp, dt);
+  const int nx = g.nx;
+  int c = g.idx(i, j, k0);
+  real vm = vel[c - Z];
+  real d = db1[c];
+  for (int k = k0; k <= k1; ++k, c += Z) {
+    const real dzo = db1[c + Z];
+    const double w1 = (d + db1[c + GS1]) * 0.5;
+    const double w2 = (d + dzo) * 0.5;
+    const double q1 = ct_load<0, T1, SC>(g, fld1, c, i, i, j, k) / w1;
+    const double q2 = ct_load<0, T2, SC>(g, fld2, c, i, i, j, k) / w2;
+    const double dd = d;
+    // neighbours along x; s-1 is clamped to -1 and s+2 to n+2
+    double q1m = __shfl_up(q1, 1, 64), q2m = __shfl_up(q2, 1, 64), dm = __shfl_up(dd, 1, 64);
+    const double q1p = __shfl_down(q1, 1, 64), q2p = __shfl_down(q2, 1, 64), dp = __shfl_down(dd, 1, 64);
+    double q1q = __shfl_down(q1, 2, 64), q2q = __shfl_down(q2, 2, 64), dq = __shfl_down(dd, 2, 64);
+    const double w1p = __shfl_down(w1, 1, 64), w2p = __shfl_down(w2, 1, 64);
+    if (i - 1 < -1) {
+      q1m = q1;
+      q2m = q2;
+      dm = dd;
     }
-    {
-      const double v_advect = (vc + vel[c - VS2]) * 0.5;
-      F2[c] = ct_flux_core(v_advect, ct_load<AX, T2, SC>(g, fld2, cm, sm, i, j, k),
-                           ct_load<AX, T2, SC>(g, fld2, c, s, i, j, k), ct_load<AX, T2, SC>(g, fld2, cp, s + 1, i, j, k),
-                           ct_load<AX, T2, SC>(g, fld2, cp2, sp2, i, j, k), (d0 + dz[0]) * 0.5, (d1 + dz[1]) * 0.5,
-                           (d2 + dz[2]) * 0.5, (d3 + dz[3]) * 0.5, d1, d2, dcm, dci, dcp, dt);
+    if (i + 2 > nx + 2) {
+      q1q = q1p;
+      q2q = q2p;
+      dq = dp;
+    }
+    const real vc = vel[c];
+    if (out) {
+      // dxc1 at s-1 (upper point s, or s+1 when s = -1), s and s+1
+      const double dcm = 0.5 * (dm + (i > -1 ? dd : dp));
+      const double dci = 0.5 * (dd + dp);
+      const double dcp = 0.5 * (dp + dq);
+      F1[c] = ct_flux_q((vc + vm) * 0.5, q1m, q1, q1p, q1q, w1, w1p, dd, dp, dcm, dci, dcp, dt);
+      F2[c] = ct_flux_q((vc + vel[c - VS2]) * 0.5, q2m, q2, q2p, q2q, w2, w2p, dd, dp, dcm, dci, dcp, dt);
     }
     vm = vc;
-    if (k < k1) {
-      const int cz = c + 2 * Z;
-      dz[0] = db1[cz + om];
-      dz[1] = db1[cz];
-      dz[2] = db1[cz + S];
-      dz[3] = db1[cz + op2];
-    }
+    d = dzo;
   }
 }
 
@@ -236,7 +242,7 @@ __global__ __launch_bounds__(BX * BY) void k_ct_flux(Geo g, Box b, const double 
 // points, holding the four-point stencils of db1, of both fields' widths
 // and of their quotients field / width in registers.  A quotient is formed
 // once per point and serves as the s+2, s+1, s and s-1 value of four
-// successive fluxes; the expressions are those of ct_flux_core.
+// successive fluxes, with the same operations as forming it at each use.
 template <int M, int SC>
 __global__ __launch_bounds__(BX * BY) void k_ct_flux_m(Geo g, Box b, const double *__restrict__ fld1,
                                                        const double *__restrict__ fld2,

@@ -385,13 +385,17 @@ int ct_pair(double *const Bin[3], double *const B[3], const real *vel, double dt
     t2 = 1; va2 = 0; ga2 = 1;  // vz_by_flux
   }
   if constexpr (AX == 0) {
-    // x sweep: va1 = z and ga2 = z (see the flux table above)
+    // x sweep: va1 = z and ga2 = z (see the flux table above); waves of 64
+    // quotient points over x0..n+2, CTX_OUT outputs each
+    const long npts = (long)(N0 + 2 - fb.x0 + 1) * (fb.y1 - fb.y0 + 1);
+    const long waves = (npts + CTX_OUT - 1) / CTX_OUT;
+    const dim3 gx((unsigned)((waves + BY - 1) / BY), 1, (unsigned)((fb.z1 - fb.z0 + 1 + ZCHUNK - 1) / ZCHUNK));
     if (sc > 0)
-      LAUNCH("ct_flux", (k_ct_flux<AX, 1>), zchunk_grid(fb), box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1, S.F,
-             S.F2, strides(ga1), strides(va2), dt);
+      LAUNCH("ct_flux", k_ct_flux_x<1>, gx, box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1, S.F, S.F2,
+             strides(ga1), strides(va2), dt);
     else
-      LAUNCH("ct_flux", (k_ct_flux<AX, 0>), zchunk_grid(fb), box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1, S.F,
-             S.F2, strides(ga1), strides(va2), dt);
+      LAUNCH("ct_flux", k_ct_flux_x<0>, gx, box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1, S.F, S.F2,
+             strides(ga1), strides(va2), dt);
   } else {
     // y and z sweeps march along the sweep axis; the grid covers the other two
     const Box gb = march_box(fb, AX);
