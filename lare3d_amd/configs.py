@@ -92,12 +92,15 @@ def orszag_tang(n=64):
     return grid, params, f
 
 
-def harris(n=64, seed=12345, ny=None, nz=None, noise=0.01, stretch=(None, None, None)):
+def harris(n=64, seed=12345, ny=None, nz=None, noise=0.01, stretch=(None, None, None), damping=None):
     """Resistive Harris sheet (config 2): Bx = tanh(y/0.5),
     rho = 1 + sech^2(y/0.5), p = 0.5 rho (total pressure balance,
     isothermal), eta = 1e-3 plus 1e-2 above |J| = 1.0, seeded velocity noise,
     shock viscosity on.  Periodic in x and z, BC_USER walls in y.
-    stretch: stretched length per axis (setup.F90:256-285), or None."""
+    stretch: stretched length per axis (setup.F90:256-285), or None.
+    damping: if set, viscous damping (lagran.F90:483-524) is switched on with
+    visc3 = damping * (1 + 0.5 tanh(y) + 0.25 sin(2 pi x / Lx) cos(2 pi z / Lz))
+    on the nodes."""
     ny = ny or n
     nz = nz or n
     L = 4.0
@@ -116,6 +119,12 @@ def harris(n=64, seed=12345, ny=None, nz=None, noise=0.01, stretch=(None, None, 
     for name in ("vx", "vy", "vz"):
         a = getattr(f, name)
         a[...] = noise * (2.0 * rng.random(a.shape) - 1.0)
+    if damping is not None:
+        params.use_viscous_damping = 1
+        lx, lz = 4.0 * L, 4.0 * L
+        _fill(f.visc3, None, lambda x, y, z: damping * (1.0 + 0.5 * np.tanh(y) + 0.25 * np.sin(2.0 * np.pi * x / lx)
+                                                        * np.cos(2.0 * np.pi * z / lz)),
+              _coords(grid, "bbb", (-2, -2, -2)))
     return grid, params, f
 
 

@@ -68,6 +68,8 @@ CASES = {
     # non-uniform metrics on every axis (tanh stretch), so that a metric read
     # at a wrong index shows
     "harris_stretched": lambda: configs.harris(16, stretch=(48.0, 16.0, 48.0)),
+    # viscous damping with a non-uniform visc3
+    "harris_damped": lambda: configs.harris(16, damping=0.05),
     "arcade": lambda: configs.arcade_conduction(16, 16, 24, s_target=7),
 }
 

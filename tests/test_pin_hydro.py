@@ -215,3 +215,48 @@ def test_uniform_state_and_cfl(backend):
     assert abs(sc.dtr - 0.9 * 0.2 * dxl / 1e-3) < 1e-14 * sc.dtr
     assert sc.last_substeps == int(dt / sc.dtr) + 1
     close(sim)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_viscous_damping_shear_wave(backend):
+    """Viscous damping (lagran.F90:483-524) of a shear wave vz = A sin(2 pi x)
+    in a uniform periodic state.  Each step multiplies A by 1 - dt nu lambda
+    with lambda = (4 / dx^2) sin^2(k dx / 2), the discrete Laplacian's
+    eigenvalue; the damping runs before set_dt (lagran.F90:128-130), so step
+    s uses the dt of step s-1 (0 at step 1, which must leave A unchanged).
+    The damping updates nodes 0..n and the periodic ghost nodes keep their
+    undamped values until the step's halo refresh, as in the reference,
+    which moves the rest of the step off the exact mode by a small amount:
+    the decrement of A is checked to 1%, which a wrong coefficient, sign,
+    dt or stencil would miss by far more."""
+    from lare3d_amd.configs import default_params
+    from lare3d_amd.fields import Fields
+    from lare3d_amd.grid import Grid
+    n, nu, amp = 64, 2.0e-3, 0.02
+    grid = Grid(n, 4, 4, ((0.0, 1.0),) * 3, (BC_PERIODIC,) * 6)
+    params = default_params(use_viscous_damping=1)
+    f = Fields(n, 4, 4)
+    f.rho[...] = 1.0
+    f.energy[...] = 1.0
+    xb = grid.node_coords(0)                       # xb(-2:n+2) indexed [i+2]
+    prof = np.sin(2.0 * math.pi * xb)
+    f.vz[...] = amp * prof[:, None, None]
+    f.visc3[...] = nu
+    sim = make(backend, grid, params, f)
+    start(sim)
+    dx = 1.0 / n
+    lam = 4.0 / dx ** 2 * math.sin(math.pi * dx) ** 2
+    s = prof[3:n + 3]                              # nodes 1..n
+    a_prev, dt_prev = amp, 0.0
+    for step in range(1, 6):
+        dt, _ = sim.step(step)
+        vz = sim.fields().interior("vz")
+        got = float(np.sum(vz * s[:, None, None]) / (np.sum(s * s) * vz.shape[1] * vz.shape[2]))
+        want = a_prev * dt_prev * nu * lam        # expected decrement of this step
+        print(f"step {step}: decrement {a_prev - got:.6e} expected {want:.6e}")
+        if step == 1:
+            assert got == pytest.approx(amp, rel=1e-13)
+        else:
+            assert abs((a_prev - got) - want) <= 1e-2 * want
+        a_prev, dt_prev = got, dt
+    close(sim)
