@@ -46,7 +46,32 @@ __device__ __forceinline__ double ghost_read(const Geo &g, const real *__restric
   return (stale ? E0 : Y)[g.idx(c[0], c[1], c[2])];
 }
 
-__device__ __forceinline__ double pow25(double t) { return pow(t, 2.5); }
+// temperature**2.5 and **1.5 (conduct.f90:27, :83) correctly rounded: t^2
+// and t*sqrt(t) carried in double-double (FMA-exact products, sqrt(t) =
+// r + (t - r^2) / (2r)), rounded once.  The oracle computes the same
+// operations (a libm pow is not correctly rounded for every argument).
+__device__ __forceinline__ double pow25(double t) {
+  if (!(t > 0.0) || t == INFINITY) return t == 0.0 ? 0.0 : (t > 0.0 ? t : __builtin_nan(""));
+  const double r = sqrt(t);
+  const double e = fma(-r, r, t);
+  const double rl = e / (2.0 * r);
+  const double h2 = t * t;
+  const double l2 = fma(t, t, -h2);
+  const double p = h2 * r;
+  if (!(p < INFINITY)) return p;
+  const double pl = fma(h2, r, -p);
+  return p + (pl + (h2 * rl + l2 * r));
+}
+__device__ __forceinline__ double pow15(double t) {
+  if (!(t > 0.0) || t == INFINITY) return t == 0.0 ? 0.0 : (t > 0.0 ? t : __builtin_nan(""));
+  const double r = sqrt(t);
+  const double e = fma(-r, r, t);
+  const double rl = e / (2.0 * r);
+  const double p = t * r;
+  if (!(p < INFINITY)) return p;
+  const double pl = fma(t, r, -p);
+  return p + (pl + t * rl);
+}
 
 struct CondArgs {
   double kappa_0, tfac, dt;
@@ -68,8 +93,8 @@ __global__ __launch_bounds__(BX * BY) void k_larsen(Geo g, const real *__restric
     const double T = tfac * energy[c];
     double L = 1.0;
     if (limiter_on) {
-      const double temp = pow(T, 2.5);
-      const double q_fs = flux_limiter * 42.85 * rho[c] * pow(T, 1.5);
+      const double temp = pow25(T);
+      const double q_fs = flux_limiter * 42.85 * rho[c] * pow15(T);
       const double q_fs2 = q_fs * q_fs;
       const double q_spx = -kappa_0 * temp * (tfac * energy[c + 1] - tfac * energy[c - 1]) * 0.5 / g.dxb[i];
       const double q_spy = -kappa_0 * temp * (tfac * energy[c + g.sy] - tfac * energy[c - g.sy]) * 0.5 / g.dyb[j];
@@ -80,7 +105,7 @@ __global__ __launch_bounds__(BX * BY) void k_larsen(Geo g, const real *__restric
     larsen[c] = L;
     if (i >= 1 && j >= 1 && k >= 1) {
       const double kappa1 = kappa_0 * L;
-      const double tp = pow(T, 2.5);
+      const double tp = pow25(T);
       const double r = rho[c];
       const double dx = g.dxb[i], dy = g.dyb[j], dz = g.dzb[k];
       const double dt1 = r * (dx * dx) / (2.0 * kappa1 * tp);
@@ -165,72 +190,113 @@ __device__ __forceinline__ double flux_z_face(const Geo &g, const Tn &T, int lz,
   return -L * kappa_0 * pow25(tb) * (bzc * (tg * bzc + tg_a1 * bxf + tg_a2 * byf) + tg * MIN_B) / modb;
 }
 
-// Net heat flux into cell (i,j,k) (the value heat_flux leaves in flux(i,j,k)).
-__device__ __forceinline__ double cell_heat_flux(const Geo &g, const real *__restrict__ Y,
-                                                 const real *__restrict__ E0, int mode, double tfac,
-                                                 const double *__restrict__ bx, const double *__restrict__ by,
-                                                 const double *__restrict__ bz, const real *__restrict__ larsen,
-                                                 double kappa_0, int i, int j, int k) {
+// Ghost mapping of one coordinate under a read mode (the per-axis part of
+// ghost_read): the coordinate to read and whether to read E0 instead.
+template <int MODE>
+__device__ __forceinline__ void map_coord(int c, int n, int user, int &m, bool &stale) {
+  m = c;
+  stale = false;
+  if (MODE == TMODE_DIRECT || (c >= 1 && c <= n)) return;
+  if (user) m = mirror_idx(c, n);
+  else if (MODE == TMODE_STALE) stale = true;
+  else m = c < 1 ? c + n : c - n;
+}
+
+// RKL2 stages marching in z (conduct.f90:345-396; FIRST = stage 1).  T of
+// planes k-1, k, k+1 (3x3 in-plane each) stays in registers, so each plane
+// is read once per column; the ghost mapping of a lane's x and y neighbours
+// is worked out once.  The z face shared with the cell below is the
+// previous plane's out face, and the x face shared with the left neighbour
+// is that lane's out face: waves cover runs of 64 points of the plane
+// flattened over x = x0-1..x1, overlapping by one, and lanes at x0-1 only
+// provide their out face.  Each shared flux is the same expression on the
+// same values as the neighbour's own, and the six are summed in the order
+// of cell_heat_flux.
+#define RKL_OUT 63
+template <int MODE, int FIRST>
+__global__ __launch_bounds__(BX * BY) void k_rkl2_z(Geo g, Box b, const real *__restrict__ Y2, const real *__restrict__ Y1,
+                                                    const real *__restrict__ E0, const real *__restrict__ rho,
+                                                    const double *__restrict__ bx, const double *__restrict__ by,
+                                                    const double *__restrict__ bz, const real *__restrict__ larsen,
+                                                    real *__restrict__ Lc, real *__restrict__ Yout, CondArgs a,
+                                                    double mu, double nu, double one_m_mu_nu, double mu_t,
+                                                    double gam_t) {
+  const int lane = (int)threadIdx.x;  // BX = 64: threadIdx.y is the wave
+  const int wq = b.x1 - b.x0 + 2;     // x0-1 .. x1
+  const int npts = wq * (b.y1 - b.y0 + 1);
+  const int f = (int)(blockIdx.x * BY + threadIdx.y) * RKL_OUT + lane - 1;
+  const bool valid = f >= 0 && f < npts;
+  const int ff = valid ? f : 0;  // lanes past the plane read point 0, unused
+  const int jr = ff / wq;
+  const int i = b.x0 - 1 + (ff - jr * wq);
+  const int j = b.y0 + jr;
+  const bool out = valid && lane >= 1 && i >= b.x0;
+  ZRANGE(b, k0, k1)
+  int xm[3], ym[3];
+  bool xs[3], ys[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    map_coord<MODE>(i + q - 1, g.nx, g.user[0], xm[q], xs[q]);
+    map_coord<MODE>(j + q - 1, g.ny, g.user[1], ym[q], ys[q]);
+  }
+  const double tfac = a.tfac;
+  // T.t[p][q][r]: x offset p-1, y offset q-1, plane k-1+r
   Tn T;
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const int far = (a != 1) + (b != 1) + (c != 1);
-        T.t[a][b][c] = far == 3 ? 0.0 : tfac * ghost_read(g, Y, E0, i + a - 1, j + b - 1, k + c - 1, mode);
+#define RKL_LOAD_PLANE(kk, r)                                                    \
+  {                                                                              \
+    int zm;                                                                      \
+    bool zs;                                                                     \
+    map_coord<MODE>((kk), g.nz, g.user[2], zm, zs);                              \
+    _Pragma("unroll") for (int p = 0; p < 3; ++p)                                \
+      _Pragma("unroll") for (int q = 0; q < 3; ++q) {                            \
+        const int cq = g.idx(xm[p], ym[q], zm);                                  \
+        T.t[p][q][r] = tfac * (double)(((xs[p] || ys[q] || zs) ? E0 : Y2)[cq]);  \
+      }                                                                          \
+  }
+  RKL_LOAD_PLANE(k0 - 1, 0)
+  RKL_LOAD_PLANE(k0, 1)
+  double fz_in = 0.0;
+  for (int k = k0; k <= k1; ++k) {
+    RKL_LOAD_PLANE(k + 1, 2)
+    const int c0 = g.idx(i, j, k);
+    const double L0 = larsen[c0];
+    const double fx_out = flux_x_face(g, T, 1, i, j, k, bx, by, bz, L0, a.kappa_0);
+    const double fx_in = __shfl_up(fx_out, 1, 64);
+    if (k == k0) fz_in = flux_z_face(g, T, 0, i, j, k - 1, bx, by, bz, larsen[c0 - g.sz], a.kappa_0);
+    const double fy_in = flux_y_face(g, T, 0, i, j - 1, k, bx, by, bz, larsen[c0 - g.sy], a.kappa_0);
+    const double fy_out = flux_y_face(g, T, 1, i, j, k, bx, by, bz, L0, a.kappa_0);
+    const double fz_out = flux_z_face(g, T, 1, i, j, k, bx, by, bz, L0, a.kappa_0);
+    if (out) {
+      double fl = 0.0 + fz_in / g.dzb[k - 1];
+      fl = fl + fy_in / g.dyb[j - 1];
+      fl = fl + fx_in / g.dxb[i - 1];
+      fl = fl - fx_out / g.dxb[i];
+      fl = fl - fy_out / g.dyb[j];
+      fl = fl - fz_out / g.dzb[k];
+      if (FIRST) {
+        const double lc = (1.0 / rho[c0]) * fl;
+        Lc[c0] = lc;
+        const double cc = mu_t * a.dt * lc;
+        Yout[c0] = E0[c0] + cc;
+      } else {
+        const double cc0 = gam_t * a.dt * Lc[c0];
+        const double lj = (1.0 / rho[c0]) * fl;
+        const double cc1 = mu_t * a.dt * lj;
+        double y3 = mu * Y2[c0] + nu * Y1[c0] + one_m_mu_nu * E0[c0];
+        y3 = y3 + cc1 + cc0;
+        Yout[c0] = y3;
       }
-  const int c0 = g.idx(i, j, k);
-  // faces in: from (i,j,k-1) z, (i,j-1,k) y, (i-1,j,k) x; faces out: own x, y, z
-  // For the z face of the lower neighbour we need T around (i,j,k-1); the
-  // face lies between neighbourhood z indices 0 and 1 (lz = 0).
-  const double fz_in = flux_z_face(g, T, 0, i, j, k - 1, bx, by, bz, larsen[c0 - g.sz], kappa_0);
-  const double fy_in = flux_y_face(g, T, 0, i, j - 1, k, bx, by, bz, larsen[c0 - g.sy], kappa_0);
-  const double fx_in = flux_x_face(g, T, 0, i - 1, j, k, bx, by, bz, larsen[c0 - 1], kappa_0);
-  const double L0 = larsen[c0];
-  const double fx_out = flux_x_face(g, T, 1, i, j, k, bx, by, bz, L0, kappa_0);
-  const double fy_out = flux_y_face(g, T, 1, i, j, k, bx, by, bz, L0, kappa_0);
-  const double fz_out = flux_z_face(g, T, 1, i, j, k, bx, by, bz, L0, kappa_0);
-  double f = 0.0 + fz_in / g.dzb[k - 1];
-  f = f + fy_in / g.dyb[j - 1];
-  f = f + fx_in / g.dxb[i - 1];
-  f = f - fx_out / g.dxb[i];
-  f = f - fy_out / g.dyb[j];
-  f = f - fz_out / g.dzb[k];
-  return f;
-}
-
-// Stage 1 (conduct.f90:345-367): Lc = flux/rho, Y1 = Y0 + mu~1 dt Lc.
-__global__ __launch_bounds__(BX * BY) void k_rkl2_stage1(Geo g, Box b, const real *__restrict__ E0, const real *__restrict__ rho,
-                              const double *__restrict__ bx, const double *__restrict__ by,
-                              const double *__restrict__ bz, const real *__restrict__ larsen,
-                              real *__restrict__ Lc, real *__restrict__ Yout, CondArgs a, double mu_t1) {
-  BOX_INDEX(b, i, j, k);
-  const double f = cell_heat_flux(g, E0, E0, TMODE_DIRECT, a.tfac, bx, by, bz, larsen, a.kappa_0, i, j, k);
-  const int c = g.idx(i, j, k);
-  const double lc = (1.0 / rho[c]) * f;
-  Lc[c] = lc;
-  const double c0 = mu_t1 * a.dt * lc;
-  Yout[c] = E0[c] + c0;
-}
-
-// Stage j >= 2 (conduct.f90:369-396).
-__global__ __launch_bounds__(BX * BY) void k_rkl2_stage(Geo g, Box b, const real *__restrict__ Y2, const real *__restrict__ Y1,
-                             const real *__restrict__ E0, const real *__restrict__ rho,
-                             const double *__restrict__ bx, const double *__restrict__ by,
-                             const double *__restrict__ bz, const real *__restrict__ larsen,
-                             const real *__restrict__ Lc, real *__restrict__ Yout, CondArgs a, int mode,
-                             double mu, double nu, double one_m_mu_nu, double mu_t, double gam_t) {
-  BOX_INDEX(b, i, j, k);
-  const double f = cell_heat_flux(g, Y2, E0, mode, a.tfac, bx, by, bz, larsen, a.kappa_0, i, j, k);
-  const int c = g.idx(i, j, k);
-  const double c0 = gam_t * a.dt * Lc[c];
-  const double lj = (1.0 / rho[c]) * f;
-  const double c1 = mu_t * a.dt * lj;
-  double y3 = mu * Y2[c] + nu * Y1[c] + one_m_mu_nu * E0[c];
-  y3 = y3 + c1 + c0;
-  Yout[c] = y3;
+    }
+    fz_in = fz_out;
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        T.t[p][q][0] = T.t[p][q][1];
+        T.t[p][q][1] = T.t[p][q][2];
+      }
+  }
+#undef RKL_LOAD_PLANE
 }
 
 // Final write of conduct_heat (conduct.f90:409-410): energy interior from

@@ -307,10 +307,36 @@ int calc_s_stages() {
   return 0;
 }
 
+// grid of the z-marching stage kernels over the interior box: waves of
+// RKL_OUT cells over rows of x0-1..x1, ZCHUNK planes per block
+dim3 rkl2_grid(const Box &in) {
+  const long npts = (long)(in.x1 - in.x0 + 2) * (in.y1 - in.y0 + 1);
+  const long waves = (npts + RKL_OUT - 1) / RKL_OUT;
+  return dim3((unsigned)((waves + BY - 1) / BY), 1, (unsigned)((in.z1 - in.z0 + 1 + ZCHUNK - 1) / ZCHUNK));
+}
+// stage 1 from E0 (ghosts as stored): Lc and Y1 = E0 + mu_t1 dt Lc
+int rkl2_stage1(const real *E0, real *Yout, CondArgs ca, double mu_t1) {
+  const Box in = interior();
+  LAUNCH("rkl2_stage1", (k_rkl2_z<TMODE_DIRECT, 1>), rkl2_grid(in), box_block(), S.g, in, E0, E0, E0, S.rho, S.bx,
+         S.by, S.bz, S.larsen, S.Lc, Yout, ca, 0.0, 0.0, 0.0, mu_t1, 0.0);
+  return 0;
+}
+// stage j >= 2 from Y2 (and Y1, E0, Lc) under a ghost-read mode
+int rkl2_stage(int mode, const real *Y2, const real *Y1, const real *E0, real *Yout, CondArgs ca, double mu, double nu,
+               double one_m_mu_nu, double mu_t, double gam_t) {
+  const Box in = interior();
+  if (mode == TMODE_STALE)
+    LAUNCH("rkl2_stage", (k_rkl2_z<TMODE_STALE, 0>), rkl2_grid(in), box_block(), S.g, in, Y2, Y1, E0, S.rho, S.bx,
+           S.by, S.bz, S.larsen, S.Lc, Yout, ca, mu, nu, one_m_mu_nu, mu_t, gam_t);
+  else
+    LAUNCH("rkl2_stage", (k_rkl2_z<TMODE_WRAP, 0>), rkl2_grid(in), box_block(), S.g, in, Y2, Y1, E0, S.rho, S.bx,
+           S.by, S.bz, S.larsen, S.Lc, Yout, ca, mu, nu, one_m_mu_nu, mu_t, gam_t);
+  return 0;
+}
+
 // RKL2 super-step; delta_mode = 1 inside lagrangian_step
 int heat_conduct_sts2(int delta_mode) {
   const int s = S.n_s_stages;
-  Box in = interior();
   if (s <= 1) {  // the reference's Y(3) is never written: energy unchanged
     if (delta_mode) LAUNCH_BOX("energy_max", k_energy_max, cell_box(), S.energy);
     return 0;
@@ -335,15 +361,16 @@ int heat_conduct_sts2(int delta_mode) {
   CondArgs ca = {S.p.kappa_0, S.tfac, S.dt};
   real *E0 = S.energy;
   real *outp = S.Yb[0];
-  LAUNCH_BOX("rkl2_stage1", k_rkl2_stage1, in, E0, S.rho, S.bx, S.by, S.bz, S.larsen, S.Lc, outp, ca, mu_t[1]);
+  int rc = rkl2_stage1(E0, S.Yb[0], ca, mu_t[1]);
+  if (rc) return rc;
   real *y1 = E0, *y2 = outp;
   int next = 1;
   for (int j = 2; j <= s; ++j) {
     real *yo = S.Yb[next];
     next = (next + 1) % 3;
-    const int mode = j == 2 ? TMODE_STALE : TMODE_WRAP;
-    LAUNCH_BOX("rkl2_stage", k_rkl2_stage, in, y2, y1, E0, S.rho, S.bx, S.by, S.bz, S.larsen, S.Lc, yo, ca, mode,
-               mu[j], nu[j], (1.0 - mu[j]) - nu[j], mu_t[j], gt[j]);
+    rc = rkl2_stage(j == 2 ? TMODE_STALE : TMODE_WRAP, y2, y1, E0, yo, ca, mu[j], nu[j], (1.0 - mu[j]) - nu[j],
+                    mu_t[j], gt[j]);
+    if (rc) return rc;
     y1 = y2;
     y2 = yo;
   }
@@ -1220,10 +1247,10 @@ int sur_features(int normalise) {
 // history weights zero evaluate the trapezoid into SU.trap.
 int sur_residual(const real *E1, double *res) {
   CondArgs ca = {S.p.kappa_0, S.tfac, S.dt};
-  Box in = interior();
-  LAUNCH_BOX("rkl2_stage1", k_rkl2_stage1, in, S.energy, S.rho, S.bx, S.by, S.bz, S.larsen, S.Lc, S.Yb[0], ca, 0.0);
-  LAUNCH_BOX("rkl2_stage", k_rkl2_stage, in, E1, E1, S.energy, S.rho, S.bx, S.by, S.bz, S.larsen, S.Lc, SU.trap, ca,
-             TMODE_WRAP, 0.0, 0.0, 0.0, 0.5, 0.5);
+  int rc0 = rkl2_stage1(S.energy, S.Yb[0], ca, 0.0);
+  if (rc0) return rc0;
+  rc0 = rkl2_stage(TMODE_WRAP, E1, E1, S.energy, SU.trap, ca, 0.0, 0.0, 0.0, 0.5, 0.5);
+  if (rc0) return rc0;
   dim3 gr = box_grid(interior());
   LAUNCH("sur_residual", k_sur_residual, gr, box_block(), S.g, S.energy, E1, SU.trap, S.partial);
   int rc = final_reduce(gr, 2, 0, 0, 0);

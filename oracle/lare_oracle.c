@@ -28,6 +28,36 @@
 #define MIN_B 1.0e-10       /* conduct.f90:28 */
 #define POW 2.5             /* conduct.f90:27 */
 
+/* temperature**pow (conduct.f90:27, pow = 2.5) and temperature**1.5
+ * (conduct.f90:83) evaluated correctly rounded: t^2 and t*sqrt(t) are
+ * carried in double-double (FMA-exact products, sqrt(t) = r + (t - r^2) /
+ * (2r)) and rounded once, so the result is the double nearest the exact
+ * value (the relative error before the final rounding is near 2^-105).
+ * A libm pow is not always that: glibc's is one ULP off in about 0.08% of
+ * arguments.  The device computes the same operations. */
+double orc_pow25(double t) {
+  if (!(t > 0.0) || t == INFINITY) return t == 0.0 ? 0.0 : (t > 0.0 ? t : NAN);
+  const double r = sqrt(t);
+  const double e = fma(-r, r, t);
+  const double rl = e / (2.0 * r);
+  const double h2 = t * t;
+  const double l2 = fma(t, t, -h2);
+  const double p = h2 * r;
+  if (!(p < INFINITY)) return p;
+  const double pl = fma(h2, r, -p);
+  return p + (pl + (h2 * rl + l2 * r));
+}
+double orc_pow15(double t) {
+  if (!(t > 0.0) || t == INFINITY) return t == 0.0 ? 0.0 : (t > 0.0 ? t : NAN);
+  const double r = sqrt(t);
+  const double e = fma(-r, r, t);
+  const double rl = e / (2.0 * r);
+  const double p = t * r;
+  if (!(p < INFINITY)) return p;
+  const double pl = fma(t, r, -p);
+  return p + (pl + t * rl);
+}
+
 struct orc_state {
   int nx, ny, nz;
   long ex, ey, ez, ntot;
@@ -675,8 +705,8 @@ static void calc_s_stages(orc_state *o) {
     for (int iy = 0; iy <= ny; ++iy)
       for (int ix = 0; ix <= nx; ++ix) {
         double *T = o->temperature;
-        double temp = pow(A(T, ix, iy, iz), POW);
-        double q_fs = o->p.flux_limiter * 42.85 * A(o->rho, ix, iy, iz) * pow(A(T, ix, iy, iz), 1.5);
+        double temp = orc_pow25(A(T, ix, iy, iz));
+        double q_fs = o->p.flux_limiter * 42.85 * A(o->rho, ix, iy, iz) * orc_pow15(A(T, ix, iy, iz));
         double q_fs2 = q_fs * q_fs;
         double q_spx = -k0 * temp * (A(T, ix + 1, iy, iz) - A(T, ix - 1, iy, iz)) * 0.5 / DXB(ix);
         double q_spy = -k0 * temp * (A(T, ix, iy + 1, iz) - A(T, ix, iy - 1, iz)) * 0.5 / DYB(iy);
@@ -695,7 +725,7 @@ static void calc_s_stages(orc_state *o) {
         double kappa1 = k0 * A(o->larsen, ix, iy, iz);
         double temp = temp_of(o, A(o->energy, ix, iy, iz));
         double r = A(o->rho, ix, iy, iz);
-        double tp = pow(temp, POW);
+        double tp = orc_pow25(temp);
         double dt1 = r * SQ(DXB(ix)) / (2.0 * kappa1 * tp);
         double dt2 = r * SQ(DYB(iy)) / (2.0 * kappa1 * tp);
         double dt3 = r * SQ(DZB(iz)) / (2.0 * kappa1 * tp);
@@ -733,7 +763,7 @@ static void heat_flux(orc_state *o, const double *T, double *flux) {
         double tg_a2 = (tb_p - tb_m) / (DZC(iz) + DZC(izm));
         double tg = (A(T, ixp, iy, iz) - A(T, ix, iy, iz)) / DXC(ix);
         tb = dmax(tb, 0.0);
-        double fc_sp = -L * k0 * pow(tb, POW) * (bxc * (tg * bxc + tg_a1 * byf + tg_a2 * bzf) + tg * MIN_B) / modb;
+        double fc_sp = -L * k0 * orc_pow25(tb) * (bxc * (tg * bxc + tg_a1 * byf + tg_a2 * bzf) + tg * MIN_B) / modb;
         A(flux, ix, iy, iz) = A(flux, ix, iy, iz) - fc_sp / DXB(ix);
         A(flux, ixp, iy, iz) = A(flux, ixp, iy, iz) + fc_sp / DXB(ix);
         /* Y flux */
@@ -750,7 +780,7 @@ static void heat_flux(orc_state *o, const double *T, double *flux) {
         tg_a2 = (tb_p - tb_m) / (DZC(iz) + DZC(izm));
         tg = (A(T, ix, iyp, iz) - A(T, ix, iy, iz)) / DYC(iy);
         tb = dmax(tb, 0.0);
-        fc_sp = -L * k0 * pow(tb, POW) * (byc * (tg * byc + tg_a1 * bxf + tg_a2 * bzf) + tg * MIN_B) / modb;
+        fc_sp = -L * k0 * orc_pow25(tb) * (byc * (tg * byc + tg_a1 * bxf + tg_a2 * bzf) + tg * MIN_B) / modb;
         A(flux, ix, iy, iz) = A(flux, ix, iy, iz) - fc_sp / DYB(iy);
         A(flux, ix, iyp, iz) = A(flux, ix, iyp, iz) + fc_sp / DYB(iy);
         /* Z flux */
@@ -767,7 +797,7 @@ static void heat_flux(orc_state *o, const double *T, double *flux) {
         tg_a2 = (tb_p - tb_m) / (DYC(iy) + DYC(iym));
         tg = (A(T, ix, iy, izp) - A(T, ix, iy, iz)) / DZC(iz);
         tb = dmax(tb, 0.0);
-        fc_sp = -L * k0 * pow(tb, POW) * (bzc * (tg * bzc + tg_a1 * bxf + tg_a2 * byf) + tg * MIN_B) / modb;
+        fc_sp = -L * k0 * orc_pow25(tb) * (bzc * (tg * bzc + tg_a1 * bxf + tg_a2 * byf) + tg * MIN_B) / modb;
         A(flux, ix, iy, iz) = A(flux, ix, iy, iz) - fc_sp / DZB(iz);
         A(flux, ix, iy, izp) = A(flux, ix, iy, izp) + fc_sp / DZB(iz);
       }

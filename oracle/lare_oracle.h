@@ -41,6 +41,9 @@ void orc_boundary_conditions(orc_state *o);
 void orc_energy_account(orc_state *o, double en[3]);
 /* max over interior cells of |div B| * min(dxb,dyb,dzb) (device diagnostic) */
 double orc_divb(orc_state *o);
+/* correctly rounded t^2.5 and t^1.5 (the conduction temperature powers) */
+double orc_pow25(double t);
+double orc_pow15(double t);
 void orc_step(orc_state *o, int step, double *dt, double *time);
 
 #ifdef __cplusplus

@@ -38,6 +38,8 @@ def _load():
         "orc_conduct_heat": ([vp], None),
         "orc_boundary_conditions": ([vp], None),
         "orc_energy_account": ([vp, dp], None),
+        "orc_pow25": ([C.c_double], C.c_double),
+        "orc_pow15": ([C.c_double], C.c_double),
         "orc_divb": ([vp], C.c_double),
         "orc_step": ([vp, C.c_int, dp, dp], None),
     }

@@ -237,3 +237,25 @@ def test_boundary_slab_rules(bcs):
         ref = _slab_reference(getattr(f, name), lo, nn, per, kind, comp)
         assert np.array_equal(getattr(out, name), ref), name
     close(sim)
+
+
+def test_temperature_powers_correctly_rounded():
+    """T**2.5 and T**1.5 of the conduction flux and limiter (conduct.f90:27,
+    :83) are evaluated correctly rounded by the oracle (and by the device,
+    with the same operations): compare with the exact power rounded once,
+    from 50-digit decimal arithmetic, over nine decades of temperature and
+    at exact cases (perfect squares, 0)."""
+    from decimal import Decimal, localcontext
+    from oracle.oracle import lib
+    rng = np.random.default_rng(7)
+    ts = np.exp(rng.uniform(math.log(1e-6), math.log(1e3), 20000))
+    ts = np.concatenate([ts, [0.25, 1.0, 4.0, 9.0, 2.0 ** -20, 1e-300, 1e100, 0.7108961326867776]])
+    with localcontext() as ctx:
+        ctx.prec = 50
+        for t in ts:
+            d = Decimal(float(t))
+            r = d.sqrt()
+            assert lib().orc_pow25(float(t)) == float(d * d * r), t
+            assert lib().orc_pow15(float(t)) == float(d * r), t
+    assert lib().orc_pow25(0.0) == 0.0 and lib().orc_pow15(0.0) == 0.0
+    assert math.isnan(lib().orc_pow25(-1.0)) and math.isinf(lib().orc_pow25(1e200))
