@@ -711,6 +711,113 @@ __global__ __launch_bounds__(BX * BY) void k_mom_flux_m(Geo g, Box b, const real
   }
 }
 
+// Momentum remap of the y and z sweeps (M = 1, 2): the fluxes of
+// k_mom_flux_m and the update of k_mom_update in one pass.  Marching along
+// the sweep axis, the flux at s-1 that the update at s needs is the one the
+// previous iteration formed (rounded to the storage type, as a stored flux
+// is), so no flux array is written or read.  The new velocities go to
+// vxo..vzo, because neighbouring chunks still read the old ones; every point
+// of the launch box (the whole padded box) is written: nodes 0..n updated,
+// the rest copied.
+template <int M>
+__global__ __launch_bounds__(BX * BY) void k_mom_remap_m(Geo g, Box b, const real *__restrict__ vx,
+                                                         const real *__restrict__ vy, const real *__restrict__ vz,
+                                                         const real *__restrict__ vel1,
+                                                         const real *__restrict__ db1,
+                                                         const real *__restrict__ rho_v,
+                                                         const real *__restrict__ cv1v,
+                                                         const real *__restrict__ cv2v,
+                                                         const real *__restrict__ dmv, real *__restrict__ vxo,
+                                                         real *__restrict__ vyo, real *__restrict__ vzo, double dt) {
+  MCOLUMN(M, b, i, p, m0, m1)
+  const int MS = M == 2 ? g.sz : g.sy;
+  const int nM = g.n(M), nP = g.n(M == 2 ? 1 : 2);
+  const real *V[3] = {vx, vy, vz};
+  real *VO[3] = {vxo, vyo, vzo};
+  // update range of this column's chunk: nodes 0..n on every axis
+  const bool inT = i >= 0 && i <= g.nx && p >= 0 && p <= nP;
+  const int ms = inT ? (m0 > 0 ? m0 : 0) : m1 + 1;
+  const int me = inT ? (m1 < nM ? m1 : nM) : m0 - 1;
+  for (int m = m0; m <= m1; ++m) {
+    if (m >= ms && m <= me) continue;
+    const int c = g.idx(i, MJ(M, m, p), MK(M, m, p));
+#pragma unroll
+    for (int q = 0; q < 3; ++q) VO[q][c] = V[q][c];
+  }
+  if (ms > me) return;
+  // flux window at point ms-1 (>= -1): v at ms-2..ms+1, db1 at ms-1..ms+1,
+  // the sweep velocity and rho_v at ms-1, ms
+  real u[3][4], d[3], v0, v1, rv0, rv1;
+  int c = g.idx(i, MJ(M, ms - 1, p), MK(M, ms - 1, p));
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) u[q][t] = V[q][c + (t - 1) * MS];
+#pragma unroll
+  for (int t = 0; t < 3; ++t) d[t] = db1[c + t * MS];
+  v0 = vel1[c];
+  v1 = vel1[c + MS];
+  rv0 = rho_v[c];
+  rv1 = rho_v[c + MS];
+  real Fp[3], dp;
+  for (int m = ms - 1; m <= me; ++m) {
+    if (m > ms - 1) {
+      // advance the window from m-1 (index c) to m
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        u[q][0] = u[q][1];
+        u[q][1] = u[q][2];
+        u[q][2] = u[q][3];
+        u[q][3] = V[q][c + 3 * MS];
+      }
+      d[0] = d[1];
+      d[1] = d[2];
+      d[2] = db1[c + 3 * MS];
+      v0 = v1;
+      v1 = vel1[c + 2 * MS];
+      rv0 = rv1;
+      rv1 = rho_v[c + 2 * MS];
+      c += MS;
+    }
+    const int j = MJ(M, m, p), k = MK(M, m, p);
+    // fluxes at m: the arithmetic of k_mom_flux_m
+    const double area = mom_area<M>(g, i, j, k);
+    const double v_advect = (v0 + v1) * 0.5;
+    const double sign_v = copysign(1.0, v_advect);
+    const double vad_p = (sign_v + 1.0) * 0.5;
+    const double vad_m = 1.0 - vad_p;
+    const double dci = d[1];
+    const double dcu = d[0] * vad_p + d[2] * vad_m;
+    const double dbu = 0.5 * (d[0] + d[1]) * vad_p + 0.5 * (d[1] + d[2]) * vad_m;
+    const double phi = fabs(v_advect) * dt / dbu;
+    const double rhou = rv0 * vad_p + rv1 * vad_m;
+    const real dm_ = dmv[c];
+    const double dd = dm_;
+    const double dmu = fabs(dd) / area / dbu / rhou;
+    real F[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double fm = u[q][0], fi = u[q][1], fp = u[q][2], fp2 = u[q][3];
+      const double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
+      const double fu = fi * vad_p + fp * vad_m;
+      const double dfu = dfm * vad_p + dfp * vad_m;
+      const double Di = vl_di(dfi, dfu, sign_v, dci, dcu, dbu, phi);
+      const double f = fu + Di * (1.0 - dmu);
+      F[q] = f * dd;
+    }
+    if (m >= ms) {
+      // the update of k_mom_update at m
+      const double rv = rv0, c1 = cv1v[c], c2 = cv2v[c];
+      const double rvn = (rv * c1 + dp - dm_) / c2;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) VO[q][c] = (rv * u[q][1] * c1 + Fp[q] - F[q]) / (c2 * rvn);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) Fp[q] = F[q];
+    dp = dm_;
+  }
+}
+
 // momentum updates (xremap.f90:282-329) on nodes 0..n
 template <int AX>
 __global__ __launch_bounds__(BX * BY) void k_mom_update(Geo g, Box b, real *__restrict__ vx, real *__restrict__ vy, real *__restrict__ vz,

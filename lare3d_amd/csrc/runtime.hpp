@@ -500,13 +500,22 @@ int sweep_full(double dt, double pass[3], bool first) {
            S.cv2v, S.dmv);
     h2[AX] = N[AX];
     Box fb = mkbox(lo[0], h2[0], lo[1], h2[1], lo[2], h2[2]);
-    if constexpr (AX == 0)
+    if constexpr (AX == 0) {
       LAUNCH_BOX("mom_flux", k_mom_flux<AX>, fb, S.vx, S.vy, S.vz, vel, S.db1, S.rho_vr, S.dmv, S.Fx, S.Fy, S.Fz, dt);
-    else LAUNCH("mom_flux", k_mom_flux_m<AX>, zchunk_grid(march_box(fb, AX)), box_block(), S.g, fb, S.vx, S.vy, S.vz,
-                vel, S.db1, S.rho_vr, S.dmv, S.Fx, S.Fy, S.Fz, dt);
+      LAUNCH_BOX("mom_update", k_mom_update<AX>, nodes0n(), S.vx, S.vy, S.vz, S.rho_vr, S.cv1v, S.cv2v, S.dmv, S.Fx,
+                 S.Fy, S.Fz);
+    } else {
+      // fluxes and update in one marching pass; the new velocities land in
+      // the flux arrays, which then swap roles with the velocity arrays
+      const Box xb = full_box();
+      LAUNCH("mom_remap", k_mom_remap_m<AX>, zchunk_grid(march_box(xb, AX)), box_block(), S.g, xb, S.vx, S.vy, S.vz,
+             vel, S.db1, S.rho_vr, S.cv1v, S.cv2v, S.dmv, S.Fx, S.Fy, S.Fz, dt);
+      real *tv;
+      tv = S.vx; S.vx = S.Fx; S.Fx = tv;
+      tv = S.vy; S.vy = S.Fy; S.Fy = tv;
+      tv = S.vz; S.vz = S.Fz; S.Fz = tv;
+    }
   }
-  LAUNCH_BOX("mom_update", k_mom_update<AX>, nodes0n(), S.vx, S.vy, S.vz, S.rho_vr, S.cv1v, S.cv2v, S.dmv, S.Fx,
-             S.Fy, S.Fz);
   real *t = S.rho;
   S.rho = S.rho_alt;
   S.rho_alt = t;
