@@ -601,118 +601,89 @@ __device__ __forceinline__ double mom_area(const Geo &g, int i, int j, int k) {
   return g.dxc[i] * g.dyc[j];
 }
 
-// x_momx/y/z_flux (xremap.f90:605-895, rke off): sweep -1..n, transverse
-// 0..n; the three velocity components share the upwind switch and geometry.
-template <int AX>
-__global__ __launch_bounds__(BX * BY) void k_mom_flux(Geo g, Box b, const real *__restrict__ vx, const real *__restrict__ vy,
-                           const real *__restrict__ vz, const real *__restrict__ vel1,
-                           const real *__restrict__ db1, const real *__restrict__ rho_v,
-                           const real *__restrict__ dmv, real *__restrict__ Fx, real *__restrict__ Fy,
-                           real *__restrict__ Fz, double dt) {
-  BOX_INDEX(b, i, j, k);
+// Momentum remap of the x sweep: the fluxes x_momx/y/z_flux (xremap.f90:605-
+// 895, rke off; sweep -1..n, transverse 0..n, the three components sharing
+// the upwind switch and geometry) and the updates of xremap.f90:282-329
+// (nodes 0..n) in one pass, the flux and dm_v at s-1 taken from the
+// neighbouring lane.  Waves cover runs of 64 points of a plane of the launch
+// box (the whole padded box) flattened along x, successive waves
+// overlapping by one; lane 0 only provides its flux.  Every point is
+// written to vxo..vzo: nodes 0..n updated, the rest copied.
+#define MOMX_OUT 63
+__global__ __launch_bounds__(BX * BY) void k_mom_remap_x(Geo g, Box b, const real *__restrict__ vx,
+                                                         const real *__restrict__ vy, const real *__restrict__ vz,
+                                                         const real *__restrict__ vel1,
+                                                         const real *__restrict__ db1,
+                                                         const real *__restrict__ rho_v,
+                                                         const real *__restrict__ cv1v,
+                                                         const real *__restrict__ cv2v,
+                                                         const real *__restrict__ dmv, real *__restrict__ vxo,
+                                                         real *__restrict__ vyo, real *__restrict__ vzo, double dt) {
+  const int lane = (int)threadIdx.x;  // BX = 64: threadIdx.y is the wave
+  const int wq = b.x1 - b.x0 + 1;
+  const int npts = wq * (b.y1 - b.y0 + 1);
+  const int f = (int)(blockIdx.x * BY + threadIdx.y) * MOMX_OUT + lane - 1;
+  const bool valid = f >= 0 && f < npts;
+  const int ff = valid ? f : 0;  // lanes past the plane read point 0, unused
+  const int jr = ff / wq;
+  const int i = b.x0 + (ff - jr * wq);
+  const int j = b.y0 + jr;
+  const int k = b.z0 + (int)blockIdx.z;
   const int c = g.idx(i, j, k);
-  const int S = sstride<AX>(g);
-  const int s = coord<AX>(i, j, k);
-  const double area = mom_area<AX>(g, i, j, k);
-  const double v_advect = (vel1[c] + vel1[c + S]) * 0.5;  // averaged vx1 (xremap.f90:261)
-  const double sign_v = copysign(1.0, v_advect);
-  const double vad_p = (sign_v + 1.0) * 0.5;
-  const double vad_m = 1.0 - vad_p;
-  const double dci = db1[c + S];
-  const double dcu = db1[c] * vad_p + db1[c + 2 * S] * vad_m;
-  const double dbu = dc1<AX>(g, db1, c, s) * vad_p + dc1<AX>(g, db1, c + S, s + 1) * vad_m;
-  const double phi = fabs(v_advect) * dt / dbu;
-  const double rhou = rho_v[c] * vad_p + rho_v[c + S] * vad_m;
-  const double d = dmv[c];
-  const double dmu = fabs(d) / area / dbu / rhou;
   const real *V[3] = {vx, vy, vz};
-  real *FF[3] = {Fx, Fy, Fz};
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const real *u = V[q];
-    const double fm = u[c - S], fi = u[c], fp = u[c + S], fp2 = u[c + 2 * S];
-    const double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
-    const double fu = fi * vad_p + fp * vad_m;
-    const double dfu = dfm * vad_p + dfp * vad_m;
-    const double Di = vl_di(dfi, dfu, sign_v, dci, dcu, dbu, phi);
-    const double f = fu + Di * (1.0 - dmu);
-    FF[q][c] = f * d;
-  }
-}
-
-// The y- and z-sweep momentum fluxes (M = 1, 2), marching along the sweep
-// axis: the three velocity components of points m-1..m+2, db1 of m..m+2,
-// the sweep velocity and rho_v of m and m+1 in registers.
-template <int M>
-__global__ __launch_bounds__(BX * BY) void k_mom_flux_m(Geo g, Box b, const real *__restrict__ vx,
-                                                        const real *__restrict__ vy, const real *__restrict__ vz,
-                                                        const real *__restrict__ vel1,
-                                                        const real *__restrict__ db1,
-                                                        const real *__restrict__ rho_v,
-                                                        const real *__restrict__ dmv, real *__restrict__ Fx,
-                                                        real *__restrict__ Fy, real *__restrict__ Fz, double dt) {
-  MCOLUMN(M, b, i, p, m0, m1)
-  const int MS = M == 2 ? g.sz : g.sy;
-  real u[3][4], d[3], v0, v1, rv0, rv1;
-  const real *V[3] = {vx, vy, vz};
-  real *FF[3] = {Fx, Fy, Fz};
-  const int cb = g.idx(i, MJ(M, m0, p), MK(M, m0, p));
-#pragma unroll
-  for (int q = 0; q < 3; ++q)
-#pragma unroll
-    for (int t = 0; t < 4; ++t) u[q][t] = V[q][cb + (t - 1) * MS];
-#pragma unroll
-  for (int t = 0; t < 3; ++t) d[t] = db1[cb + t * MS];
-  v0 = vel1[cb];
-  v1 = vel1[cb + MS];
-  rv0 = rho_v[cb];
-  rv1 = rho_v[cb + MS];
-  for (int m = m0; m <= m1; ++m) {
-    const int j = MJ(M, m, p), k = MK(M, m, p);
-    const int c = g.idx(i, j, k);
-    const double area = mom_area<M>(g, i, j, k);
-    const double v_advect = (v0 + v1) * 0.5;
+  real *VO[3] = {vxo, vyo, vzo};
+  const int n = g.nx;
+  const bool iny = j >= 0 && j <= g.ny && k >= 0 && k <= g.nz;
+  // fluxes on x = -1..n (transverse 0..n)
+  real F[3] = {0, 0, 0}, dm_ = 0;
+  if (iny && i >= -1 && i <= n) {
+    const double area = mom_area<0>(g, i, j, k);
+    const double v_advect = (vel1[c] + vel1[c + 1]) * 0.5;
     const double sign_v = copysign(1.0, v_advect);
     const double vad_p = (sign_v + 1.0) * 0.5;
     const double vad_m = 1.0 - vad_p;
-    const double dci = d[1];
-    const double dcu = d[0] * vad_p + d[2] * vad_m;
-    const double dbu = 0.5 * (d[0] + d[1]) * vad_p + 0.5 * (d[1] + d[2]) * vad_m;
+    const double dci = db1[c + 1];
+    const double dcu = db1[c] * vad_p + db1[c + 2] * vad_m;
+    const double dbu = dc1<0>(g, db1, c, i) * vad_p + dc1<0>(g, db1, c + 1, i + 1) * vad_m;
     const double phi = fabs(v_advect) * dt / dbu;
-    const double rhou = rv0 * vad_p + rv1 * vad_m;
-    const double dd = dmv[c];
-    const double dmu = fabs(dd) / area / dbu / rhou;
+    const double rhou = rho_v[c] * vad_p + rho_v[c + 1] * vad_m;
+    dm_ = dmv[c];
+    const double d = dm_;
+    const double dmu = fabs(d) / area / dbu / rhou;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const double fm = u[q][0], fi = u[q][1], fp = u[q][2], fp2 = u[q][3];
+      const real *u = V[q];
+      const double fm = u[c - 1], fi = u[c], fp = u[c + 1], fp2 = u[c + 2];
       const double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
       const double fu = fi * vad_p + fp * vad_m;
       const double dfu = dfm * vad_p + dfp * vad_m;
       const double Di = vl_di(dfi, dfu, sign_v, dci, dcu, dbu, phi);
-      const double f = fu + Di * (1.0 - dmu);
-      FF[q][c] = f * dd;
+      const double fl = fu + Di * (1.0 - dmu);
+      F[q] = fl * d;
     }
-    if (m < m1) {
+  }
+  // lane l-1 holds point i-1 of the same row whenever i > x0
+  real Fm[3];
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        u[q][0] = u[q][1];
-        u[q][1] = u[q][2];
-        u[q][2] = u[q][3];
-        u[q][3] = V[q][c + 3 * MS];
-      }
-      d[0] = d[1];
-      d[1] = d[2];
-      d[2] = db1[c + 3 * MS];
-      v0 = v1;
-      v1 = vel1[c + 2 * MS];
-      rv0 = rv1;
-      rv1 = rho_v[c + 2 * MS];
-    }
+  for (int q = 0; q < 3; ++q) Fm[q] = __shfl_up(F[q], 1, 64);
+  const real dmm = __shfl_up(dm_, 1, 64);
+  if (!valid || lane < 1) return;
+  if (iny && i >= 0 && i <= n) {
+    // the update (xremap.f90:282-329)
+    const double rv = rho_v[c], c1 = cv1v[c], c2 = cv2v[c];
+    const double rvn = (rv * c1 + dmm - dm_) / c2;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) VO[q][c] = (rv * V[q][c] * c1 + Fm[q] - F[q]) / (c2 * rvn);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) VO[q][c] = V[q][c];
   }
 }
 
-// Momentum remap of the y and z sweeps (M = 1, 2): the fluxes of
-// k_mom_flux_m and the update of k_mom_update in one pass.  Marching along
+// Momentum remap of the y and z sweeps (M = 1, 2): the fluxes and the
+// update of the x-sweep kernel above, in one pass marching along the sweep
+// axis with the velocity stencil, db1, the sweep velocity and rho_v in
+// registers.  Marching along
 // the sweep axis, the flux at s-1 that the update at s needs is the one the
 // previous iteration formed (rounded to the storage type, as a stored flux
 // is), so no flux array is written or read.  The new velocities go to
@@ -780,7 +751,7 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_m(Geo g, Box b, const rea
       c += MS;
     }
     const int j = MJ(M, m, p), k = MK(M, m, p);
-    // fluxes at m: the arithmetic of k_mom_flux_m
+    // fluxes at m
     const double area = mom_area<M>(g, i, j, k);
     const double v_advect = (v0 + v1) * 0.5;
     const double sign_v = copysign(1.0, v_advect);
@@ -806,7 +777,7 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_m(Geo g, Box b, const rea
       F[q] = f * dd;
     }
     if (m >= ms) {
-      // the update of k_mom_update at m
+      // the update at m
       const double rv = rv0, c1 = cv1v[c], c2 = cv2v[c];
       const double rvn = (rv * c1 + dp - dm_) / c2;
 #pragma unroll
@@ -818,22 +789,6 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_m(Geo g, Box b, const rea
   }
 }
 
-// momentum updates (xremap.f90:282-329) on nodes 0..n
-template <int AX>
-__global__ __launch_bounds__(BX * BY) void k_mom_update(Geo g, Box b, real *__restrict__ vx, real *__restrict__ vy, real *__restrict__ vz,
-                             const real *__restrict__ rho_v, const real *__restrict__ cv1v,
-                             const real *__restrict__ cv2v, const real *__restrict__ dmv,
-                             const real *__restrict__ Fx, const real *__restrict__ Fy,
-                             const real *__restrict__ Fz) {
-  BOX_INDEX(b, i, j, k);
-  const int c = g.idx(i, j, k);
-  const int S = sstride<AX>(g);
-  const double rv = rho_v[c], c1 = cv1v[c], c2 = cv2v[c];
-  const double rv1 = (rv * c1 + dmv[c - S] - dmv[c]) / c2;
-  vx[c] = (rv * vx[c] * c1 + Fx[c - S] - Fx[c]) / (c2 * rv1);
-  vy[c] = (rv * vy[c] * c1 + Fy[c - S] - Fy[c]) / (c2 * rv1);
-  vz[c] = (rv * vz[c] * c1 + Fz[c - S] - Fz[c]) / (c2 * rv1);
-}
 
 // B <-> face flux scaling of remap.f90:43-69 / :101-127 over the whole
 // face arrays.  dir > 0 multiplies (out may differ from in: the predictor

@@ -498,23 +498,24 @@ int sweep_full(double dt, double pass[3], bool first) {
     Box vb = mkbox(lo[0], h2[0], lo[1], h2[1], lo[2], h2[2]);
     LAUNCH("vertex", k_vertex<AX>, zchunk_grid(vb), box_block(), S.g, vb, S.rho, S.cv1, S.cv2, S.dm, S.rho_vr, S.cv1v,
            S.cv2v, S.dmv);
-    h2[AX] = N[AX];
-    Box fb = mkbox(lo[0], h2[0], lo[1], h2[1], lo[2], h2[2]);
+    // momentum fluxes and update in one pass over the padded box; the new
+    // velocities land in the flux arrays, which then swap roles with the
+    // velocity arrays
+    const Box xb = full_box();
     if constexpr (AX == 0) {
-      LAUNCH_BOX("mom_flux", k_mom_flux<AX>, fb, S.vx, S.vy, S.vz, vel, S.db1, S.rho_vr, S.dmv, S.Fx, S.Fy, S.Fz, dt);
-      LAUNCH_BOX("mom_update", k_mom_update<AX>, nodes0n(), S.vx, S.vy, S.vz, S.rho_vr, S.cv1v, S.cv2v, S.dmv, S.Fx,
-                 S.Fy, S.Fz);
+      const long npts = (long)(xb.x1 - xb.x0 + 1) * (xb.y1 - xb.y0 + 1);
+      const long waves = (npts + MOMX_OUT - 1) / MOMX_OUT;
+      const dim3 gx((unsigned)((waves + BY - 1) / BY), 1, (unsigned)(xb.z1 - xb.z0 + 1));
+      LAUNCH("mom_remap", k_mom_remap_x, gx, box_block(), S.g, xb, S.vx, S.vy, S.vz, vel, S.db1, S.rho_vr, S.cv1v,
+             S.cv2v, S.dmv, S.Fx, S.Fy, S.Fz, dt);
     } else {
-      // fluxes and update in one marching pass; the new velocities land in
-      // the flux arrays, which then swap roles with the velocity arrays
-      const Box xb = full_box();
       LAUNCH("mom_remap", k_mom_remap_m<AX>, zchunk_grid(march_box(xb, AX)), box_block(), S.g, xb, S.vx, S.vy, S.vz,
              vel, S.db1, S.rho_vr, S.cv1v, S.cv2v, S.dmv, S.Fx, S.Fy, S.Fz, dt);
-      real *tv;
-      tv = S.vx; S.vx = S.Fx; S.Fx = tv;
-      tv = S.vy; S.vy = S.Fy; S.Fy = tv;
-      tv = S.vz; S.vz = S.Fz; S.Fz = tv;
     }
+    real *tv;
+    tv = S.vx; S.vx = S.Fx; S.Fx = tv;
+    tv = S.vy; S.vy = S.Fy; S.Fy = tv;
+    tv = S.vz; S.vz = S.Fz; S.Fz = tv;
   }
   real *t = S.rho;
   S.rho = S.rho_alt;
