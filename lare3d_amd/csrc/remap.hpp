@@ -445,60 +445,39 @@ __global__ __launch_bounds__(BX * BY) void k_mass_flux_m(Geo g, Box b, const rea
   }
 }
 
-// x_energy_flux (xremap.f90:539-601): box 0..n
-template <int AX>
-__global__ __launch_bounds__(BX * BY) void k_energy_flux(Geo g, Box b, const real *__restrict__ energy, const real *__restrict__ rho1,
-                              const real *__restrict__ vel, const real *__restrict__ db1,
-                              const real *__restrict__ dm, real *__restrict__ F, double dt) {
-  BOX_INDEX(b, i, j, k);
-  const int c = g.idx(i, j, k);
-  const int S = sstride<AX>(g);
-  const int s = coord<AX>(i, j, k);
-  const double area = face_area<AX>(g, i, j, k);
-  const double v_advect = face_velocity<AX>(g, vel, c);
-  double vad_p, vad_m, dbu, fu, Di, phi;
-  vl_cell(v_advect, energy[c - S], energy[c], energy[c + S], energy[c + 2 * S], dc1<AX>(g, db1, c - S, s - 1),
-          dc1<AX>(g, db1, c, s), dc1<AX>(g, db1, c + S, s + 1), db1[c], db1[c + S], dt, vad_p, vad_m, dbu, fu, Di,
-          phi);
-  const double rhou = rho1[c] * vad_p + rho1[c + S] * vad_m;
-  const double d = dm[c];
-  const double dmu = fabs(d) / area / dbu / rhou;
-  F[c] = (fu + Di * (1.0 - dmu)) * d;
-}
-
-// The y- and z-sweep energy flux (M = 1, 2), marching along the sweep axis
-// (energy and db1 of points m-1..m+2, rho1 of m and m+1 in registers).
+// Energy flux (x_energy_flux, xremap.f90:539-601) and the density and energy
+// updates (xremap.f90:174-196, cells 1..n) of the y and z sweeps (M = 1, 2)
+// in one pass, marching along the sweep axis with energy and db1 of points
+// m-1..m+2 and rho1 of m, m+1 in registers.  The flux at s-1 that the update
+// at s needs is the previous iteration's (rounded to the storage type, as a
+// stored flux is).  The new density goes to rho_new and the new energy to
+// e_new (neighbouring chunks still read the old energy); only cells 1..n are
+// written, as the separate update wrote only those (the halo refresh after
+// the sweep fills the rest).
 template <int M>
-__global__ __launch_bounds__(BX * BY) void k_energy_flux_m(Geo g, Box b, const real *__restrict__ energy,
-                                                           const real *__restrict__ rho1,
-                                                           const real *__restrict__ vel,
-                                                           const real *__restrict__ db1,
-                                                           const real *__restrict__ dm, real *__restrict__ F,
-                                                           double dt) {
+__global__ __launch_bounds__(BX * BY) void k_rho_energy_m(Geo g, Box b, const real *__restrict__ energy,
+                                                          const real *__restrict__ rho1,
+                                                          const real *__restrict__ vel,
+                                                          const real *__restrict__ db1,
+                                                          const real *__restrict__ dm,
+                                                          const real *__restrict__ cv1,
+                                                          const real *__restrict__ cv2, real *__restrict__ rho_new,
+                                                          real *__restrict__ e_new, double dt) {
   MCOLUMN(M, b, i, p, m0, m1)
   const int MS = M == 2 ? g.sz : g.sy;
+  // window at point m0-1 (>= 0): energy and db1 at m0-2..m0+1, rho1 at m0-1, m0
   real e[4], d[4], r0, r1;
-  const int cb = g.idx(i, MJ(M, m0 - 1, p), MK(M, m0 - 1, p));
+  int c = g.idx(i, MJ(M, m0 - 1, p), MK(M, m0 - 1, p));
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    e[q] = energy[cb + q * MS];
-    d[q] = db1[cb + q * MS];
+    e[q] = energy[c + (q - 1) * MS];
+    d[q] = db1[c + (q - 1) * MS];
   }
-  r0 = rho1[cb + MS];
-  r1 = rho1[cb + 2 * MS];
-  for (int m = m0; m <= m1; ++m) {
-    const int j = MJ(M, m, p), k = MK(M, m, p);
-    const int c = g.idx(i, j, k);
-    const double area = face_area<M>(g, i, j, k);
-    const double v_advect = face_velocity<M>(g, vel, c);
-    double vad_p, vad_m, dbu, fu, Di, phi;
-    vl_cell(v_advect, e[0], e[1], e[2], e[3], 0.5 * (d[0] + d[1]), 0.5 * (d[1] + d[2]), 0.5 * (d[2] + d[3]), d[1],
-            d[2], dt, vad_p, vad_m, dbu, fu, Di, phi);
-    const double rhou = r0 * vad_p + r1 * vad_m;
-    const double dd = dm[c];
-    const double dmu = fabs(dd) / area / dbu / rhou;
-    F[c] = (fu + Di * (1.0 - dmu)) * dd;
-    if (m < m1) {
+  r0 = rho1[c];
+  r1 = rho1[c + MS];
+  real Fp, dp;
+  for (int m = m0 - 1; m <= m1; ++m) {
+    if (m > m0 - 1) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         e[q] = e[q + 1];
@@ -508,24 +487,76 @@ __global__ __launch_bounds__(BX * BY) void k_energy_flux_m(Geo g, Box b, const r
       d[3] = db1[c + 3 * MS];
       r0 = r1;
       r1 = rho1[c + 2 * MS];
+      c += MS;
     }
+    const int j = MJ(M, m, p), k = MK(M, m, p);
+    const double area = face_area<M>(g, i, j, k);
+    const double v_advect = face_velocity<M>(g, vel, c);
+    double vad_p, vad_m, dbu, fu, Di, phi;
+    vl_cell(v_advect, e[0], e[1], e[2], e[3], 0.5 * (d[0] + d[1]), 0.5 * (d[1] + d[2]), 0.5 * (d[2] + d[3]), d[1],
+            d[2], dt, vad_p, vad_m, dbu, fu, Di, phi);
+    const double rhou = r0 * vad_p + r1 * vad_m;
+    const real dmc = dm[c];
+    const double dd = dmc;
+    const double dmu = fabs(dd) / area / dbu / rhou;
+    const real F = (fu + Di * (1.0 - dmu)) * dd;
+    if (m >= m0) {
+      const double rr = r0;
+      const double rn = (rr * cv1[c] + dp - dmc) / cv2[c];
+      rho_new[c] = rn;
+      e_new[c] = (e[1] * cv1[c] * rr + Fp - F) / (cv2[c] * rn);
+    }
+    Fp = F;
+    dp = dmc;
   }
 }
 
-// density and energy updates of xremap.f90:174-196 on cells 1..n
-template <int AX>
-__global__ __launch_bounds__(BX * BY) void k_rho_energy_update(Geo g, Box b, const real *__restrict__ rho1, real *__restrict__ rho_new,
-                                    real *__restrict__ energy, const real *__restrict__ cv1,
-                                    const real *__restrict__ cv2, const real *__restrict__ dm,
-                                    const real *__restrict__ F) {
-  BOX_INDEX(b, i, j, k);
+// The same for the x sweep, the flux and dm at s-1 taken from the
+// neighbouring lane: waves cover runs of 64 points of a plane flattened over
+// x = x0-1..x1 (fluxes on 0..n), overlapping by one; lanes at x0-1 only
+// provide their flux.
+#define REX_OUT 63
+__global__ __launch_bounds__(BX * BY) void k_rho_energy_x(Geo g, Box b, const real *__restrict__ energy,
+                                                          const real *__restrict__ rho1,
+                                                          const real *__restrict__ vel,
+                                                          const real *__restrict__ db1,
+                                                          const real *__restrict__ dm,
+                                                          const real *__restrict__ cv1,
+                                                          const real *__restrict__ cv2, real *__restrict__ rho_new,
+                                                          real *__restrict__ e_new, double dt) {
+  const int lane = (int)threadIdx.x;  // BX = 64: threadIdx.y is the wave
+  const int wq = b.x1 - b.x0 + 2;     // x0-1 .. x1
+  const int npts = wq * (b.y1 - b.y0 + 1);
+  const int f = (int)(blockIdx.x * BY + threadIdx.y) * REX_OUT + lane - 1;
+  const bool valid = f >= 0 && f < npts;
+  const int ff = valid ? f : 0;  // lanes past the plane read point 0, unused
+  const int jr = ff / wq;
+  const int i = b.x0 - 1 + (ff - jr * wq);
+  const int j = b.y0 + jr;
+  const int k = b.z0 + (int)blockIdx.z;
   const int c = g.idx(i, j, k);
-  const int S = sstride<AX>(g);
+  // the energy flux at this point
+  const double area = face_area<0>(g, i, j, k);
+  const double v_advect = face_velocity<0>(g, vel, c);
+  double vad_p, vad_m, dbu, fu, Di, phi;
+  vl_cell(v_advect, energy[c - 1], energy[c], energy[c + 1], energy[c + 2], dc1<0>(g, db1, c - 1, i - 1),
+          dc1<0>(g, db1, c, i), dc1<0>(g, db1, c + 1, i + 1), db1[c], db1[c + 1], dt, vad_p, vad_m, dbu, fu, Di,
+          phi);
+  const double rhou = rho1[c] * vad_p + rho1[c + 1] * vad_m;
+  const real dmc = dm[c];
+  const double d = dmc;
+  const double dmu = fabs(d) / area / dbu / rhou;
+  const real F = (fu + Di * (1.0 - dmu)) * d;
+  // lane l-1 holds point i-1 of the same row whenever i > x0-1
+  const real Fm = __shfl_up(F, 1, 64);
+  const real dmm = __shfl_up(dmc, 1, 64);
+  if (!valid || lane < 1 || i < b.x0) return;
   const double r1 = rho1[c];
-  const double rn = (r1 * cv1[c] + dm[c - S] - dm[c]) / cv2[c];
+  const double rn = (r1 * cv1[c] + dmm - dmc) / cv2[c];
   rho_new[c] = rn;
-  energy[c] = (energy[c] * cv1[c] * r1 + F[c - S] - F[c]) / (cv2[c] * rn);
+  e_new[c] = (energy[c] * cv1[c] * r1 + Fm - F) / (cv2[c] * rn);
 }
+
 
 // vertex quantities of xremap.f90:201-280: rho_v over sweep -1..n+1 and
 // transverse 0..n; cv1v, cv2v over 0..n; dm_v over sweep -1..n.  The cell

@@ -486,11 +486,25 @@ int sweep_full(double dt, double pass[3], bool first) {
     rc = launch_bc_axis(s, AX, "bc_dm");
     if (rc) return rc;
   }
-  if constexpr (AX == 0) LAUNCH_BOX("energy_flux", k_energy_flux<AX>, nodes0n(), S.energy, S.rho, vel, S.db1, S.dm, S.Fe, dt);
-  else LAUNCH("energy_flux", k_energy_flux_m<AX>, zchunk_grid(march_box(nodes0n(), AX)), box_block(), S.g, nodes0n(),
-              S.energy, S.rho, vel, S.db1, S.dm, S.Fe, dt);
-  LAUNCH_BOX("rho_energy_update", k_rho_energy_update<AX>, interior(), S.rho, S.rho_alt, S.energy, S.cv1, S.cv2,
-             S.dm, S.Fe);
+  // energy flux and the density and energy updates in one pass (cells
+  // 1..n); the new energy lands in the energy flux array, which then swaps
+  // roles with the energy array (the new density goes to rho_alt as before)
+  {
+    const Box in = interior();
+    if constexpr (AX == 0) {
+      const long npts = (long)(in.x1 - in.x0 + 2) * (in.y1 - in.y0 + 1);
+      const long waves = (npts + REX_OUT - 1) / REX_OUT;
+      const dim3 gx((unsigned)((waves + BY - 1) / BY), 1, (unsigned)(in.z1 - in.z0 + 1));
+      LAUNCH("rho_energy", k_rho_energy_x, gx, box_block(), S.g, in, S.energy, S.rho, vel, S.db1, S.dm, S.cv1,
+             S.cv2, S.rho_alt, S.Fe, dt);
+    } else {
+      LAUNCH("rho_energy", k_rho_energy_m<AX>, zchunk_grid(march_box(in, AX)), box_block(), S.g, in, S.energy,
+             S.rho, vel, S.db1, S.dm, S.cv1, S.cv2, S.rho_alt, S.Fe, dt);
+    }
+    real *te = S.energy;
+    S.energy = S.Fe;
+    S.Fe = te;
+  }
   {
     int lo[3] = {0, 0, 0}, h2[3] = {N[0], N[1], N[2]};
     lo[AX] = -1;
