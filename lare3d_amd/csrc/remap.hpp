@@ -558,72 +558,6 @@ __global__ __launch_bounds__(BX * BY) void k_rho_energy_x(Geo g, Box b, const re
 }
 
 
-// vertex quantities of xremap.f90:201-280: rho_v over sweep -1..n+1 and
-// transverse 0..n; cv1v, cv2v over 0..n; dm_v over sweep -1..n.  The cell
-// values of planes k and k+1 are kept in registers while marching in z;
-// every sum runs over the cells in the order (i,j,k), (i+1,j,k), (i,j+1,k),
-// (i+1,j+1,k), then the same four at k+1.
-template <int AX>
-__global__ __launch_bounds__(BX * BY) void k_vertex(Geo g, Box b, const real *__restrict__ rho1, const real *__restrict__ cv1,
-                         const real *__restrict__ cv2, const real *__restrict__ dm, real *__restrict__ rho_v,
-                         real *__restrict__ cv1v, real *__restrict__ cv2v, real *__restrict__ dmv) {
-  ZCOLUMN(b, i, j, k0, k1);
-  const int Y = g.sy;
-  const int n = g.n(AX);
-  const int cl = g.idx(i, j, k0);
-  Quad lr = quad_fwd(rho1, cl, Y), l1 = quad_fwd(cv1, cl, Y), l2 = quad_fwd(cv2, cl, Y), ld = quad_fwd(dm, cl, Y);
-  for (int k = k0; k <= k1; ++k) {
-    const int c = g.idx(i, j, k);
-    const int ch = c + g.sz;
-    const Quad hr = quad_fwd(rho1, ch, Y), h1 = quad_fwd(cv1, ch, Y), h2 = quad_fwd(cv2, ch, Y),
-               hd = quad_fwd(dm, ch, Y);
-    double s = (double)lr.p0 * l1.p0;
-    s = s + (double)lr.px * l1.px;
-    s = s + (double)lr.py * l1.py;
-    s = s + (double)lr.pxy * l1.pxy;
-    s = s + (double)hr.p0 * h1.p0;
-    s = s + (double)hr.px * h1.px;
-    s = s + (double)hr.py * h1.py;
-    s = s + (double)hr.pxy * h1.pxy;
-    double s1 = l1.p0;
-    s1 = s1 + l1.px;
-    s1 = s1 + l1.py;
-    s1 = s1 + l1.pxy;
-    s1 = s1 + h1.p0;
-    s1 = s1 + h1.px;
-    s1 = s1 + h1.py;
-    s1 = s1 + h1.pxy;
-    rho_v[c] = s / s1;
-    const int sc = coord<AX>(i, j, k);
-    if (sc >= 0 && sc <= n) {
-      cv1v[c] = s1 * 0.125;
-      double s2 = l2.p0;
-      s2 = s2 + l2.px;
-      s2 = s2 + l2.py;
-      s2 = s2 + l2.pxy;
-      s2 = s2 + h2.p0;
-      s2 = s2 + h2.px;
-      s2 = s2 + h2.py;
-      s2 = s2 + h2.pxy;
-      cv2v[c] = s2 * 0.125;
-    }
-    if (sc <= n) {
-      double sd = ld.p0;
-      sd = sd + ld.px;
-      sd = sd + ld.py;
-      sd = sd + ld.pxy;
-      sd = sd + hd.p0;
-      sd = sd + hd.px;
-      sd = sd + hd.py;
-      sd = sd + hd.pxy;
-      dmv[c] = sd * 0.125;
-    }
-    lr = hr;
-    l1 = h1;
-    l2 = h2;
-    ld = hd;
-  }
-}
 
 template <int AX>
 __device__ __forceinline__ double mom_area(const Geo &g, int i, int j, int k) {
@@ -632,23 +566,79 @@ __device__ __forceinline__ double mom_area(const Geo &g, int i, int j, int k) {
   return g.dxc[i] * g.dyc[j];
 }
 
-// Momentum remap of the x sweep: the fluxes x_momx/y/z_flux (xremap.f90:605-
-// 895, rke off; sweep -1..n, transverse 0..n, the three components sharing
-// the upwind switch and geometry) and the updates of xremap.f90:282-329
-// (nodes 0..n) in one pass, the flux and dm_v at s-1 taken from the
-// neighbouring lane.  Waves cover runs of 64 points of a plane of the launch
-// box (the whole padded box) flattened along x, successive waves
-// overlapping by one; lane 0 only provides its flux.  Every point is
-// written to vxo..vzo: nodes 0..n updated, the rest copied.
-#define MOMX_OUT 63
+// Vertex quantities of xremap.f90:201-280 at the node of index c: rho_v,
+// cv1_v, cv2_v and dm_v over the eight cells (i..i+1, j..j+1, k..k+1),
+// summed in the order (i,j,k), (i+1,j,k), (i,j+1,k), (i+1,j+1,k), then the
+// same four at k+1, and rounded to the storage type as stored values are.
+struct VertexV {
+  real rv, c1, c2, dm;
+};
+__device__ __forceinline__ VertexV vertex_at(const Geo &g, const real *__restrict__ rho1, const real *__restrict__ cv1,
+                                             const real *__restrict__ cv2, const real *__restrict__ dm, int c) {
+  const int Y = g.sy;
+  const Quad lr = quad_fwd(rho1, c, Y), l1 = quad_fwd(cv1, c, Y), l2 = quad_fwd(cv2, c, Y), ld = quad_fwd(dm, c, Y);
+  const int ch = c + g.sz;
+  const Quad hr = quad_fwd(rho1, ch, Y), h1 = quad_fwd(cv1, ch, Y), h2 = quad_fwd(cv2, ch, Y),
+             hd = quad_fwd(dm, ch, Y);
+  double s = (double)lr.p0 * l1.p0;
+  s = s + (double)lr.px * l1.px;
+  s = s + (double)lr.py * l1.py;
+  s = s + (double)lr.pxy * l1.pxy;
+  s = s + (double)hr.p0 * h1.p0;
+  s = s + (double)hr.px * h1.px;
+  s = s + (double)hr.py * h1.py;
+  s = s + (double)hr.pxy * h1.pxy;
+  double s1 = l1.p0;
+  s1 = s1 + l1.px;
+  s1 = s1 + l1.py;
+  s1 = s1 + l1.pxy;
+  s1 = s1 + h1.p0;
+  s1 = s1 + h1.px;
+  s1 = s1 + h1.py;
+  s1 = s1 + h1.pxy;
+  double s2 = l2.p0;
+  s2 = s2 + l2.px;
+  s2 = s2 + l2.py;
+  s2 = s2 + l2.pxy;
+  s2 = s2 + h2.p0;
+  s2 = s2 + h2.px;
+  s2 = s2 + h2.py;
+  s2 = s2 + h2.pxy;
+  double sd = ld.p0;
+  sd = sd + ld.px;
+  sd = sd + ld.py;
+  sd = sd + ld.pxy;
+  sd = sd + hd.p0;
+  sd = sd + hd.px;
+  sd = sd + hd.py;
+  sd = sd + hd.pxy;
+  VertexV v;
+  v.rv = s / s1;
+  v.c1 = s1 * 0.125;
+  v.c2 = s2 * 0.125;
+  v.dm = sd * 0.125;
+  return v;
+}
+
+// Momentum remap of the x sweep: the vertex quantities (xremap.f90:201-280),
+// the fluxes x_momx/y/z_flux (xremap.f90:605-895, rke off; sweep -1..n,
+// transverse 0..n, the three components sharing the upwind switch and
+// geometry) and the updates of xremap.f90:282-329 (nodes 0..n) in one pass.
+// Waves cover runs of 64 points of a plane of the launch box (the whole
+// padded box) flattened along x, successive waves overlapping by two: each
+// lane forms the vertex quantities of its node, takes rho_v of node i+1 from
+// the lane to its right and the flux and dm_v of node i-1 from the lane to
+// its left; lanes 1..MOMX_OUT write.  Every point is written to vxo..vzo:
+// nodes 0..n updated, the rest copied.
+#define MOMX_OUT 62
 __global__ __launch_bounds__(BX * BY) void k_mom_remap_x(Geo g, Box b, const real *__restrict__ vx,
                                                          const real *__restrict__ vy, const real *__restrict__ vz,
                                                          const real *__restrict__ vel1,
                                                          const real *__restrict__ db1,
-                                                         const real *__restrict__ rho_v,
-                                                         const real *__restrict__ cv1v,
-                                                         const real *__restrict__ cv2v,
-                                                         const real *__restrict__ dmv, real *__restrict__ vxo,
+                                                         const real *__restrict__ rho1,
+                                                         const real *__restrict__ cv1,
+                                                         const real *__restrict__ cv2,
+                                                         const real *__restrict__ dm, real *__restrict__ vxo,
                                                          real *__restrict__ vyo, real *__restrict__ vzo, double dt) {
   const int lane = (int)threadIdx.x;  // BX = 64: threadIdx.y is the wave
   const int wq = b.x1 - b.x0 + 1;
@@ -665,8 +655,12 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_x(Geo g, Box b, const rea
   real *VO[3] = {vxo, vyo, vzo};
   const int n = g.nx;
   const bool iny = j >= 0 && j <= g.ny && k >= 0 && k <= g.nz;
+  // vertex quantities of this node (vertex box: x -1..n+1, transverse 0..n)
+  VertexV vq = {0, 0, 0, 0};
+  if (iny && i >= -1 && i <= n + 1) vq = vertex_at(g, rho1, cv1, cv2, dm, c);
+  const real rvp = __shfl_down(vq.rv, 1, 64);  // rho_v of node i+1 (lanes 0..62)
   // fluxes on x = -1..n (transverse 0..n)
-  real F[3] = {0, 0, 0}, dm_ = 0;
+  real F[3] = {0, 0, 0};
   if (iny && i >= -1 && i <= n) {
     const double area = mom_area<0>(g, i, j, k);
     const double v_advect = (vel1[c] + vel1[c + 1]) * 0.5;
@@ -677,9 +671,8 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_x(Geo g, Box b, const rea
     const double dcu = db1[c] * vad_p + db1[c + 2] * vad_m;
     const double dbu = dc1<0>(g, db1, c, i) * vad_p + dc1<0>(g, db1, c + 1, i + 1) * vad_m;
     const double phi = fabs(v_advect) * dt / dbu;
-    const double rhou = rho_v[c] * vad_p + rho_v[c + 1] * vad_m;
-    dm_ = dmv[c];
-    const double d = dm_;
+    const double rhou = (double)vq.rv * vad_p + (double)rvp * vad_m;
+    const double d = vq.dm;
     const double dmu = fabs(d) / area / dbu / rhou;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -693,16 +686,16 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_x(Geo g, Box b, const rea
       F[q] = fl * d;
     }
   }
-  // lane l-1 holds point i-1 of the same row whenever i > x0
+  // lane l-1 holds node i-1 of the same row whenever i > x0
   real Fm[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) Fm[q] = __shfl_up(F[q], 1, 64);
-  const real dmm = __shfl_up(dm_, 1, 64);
-  if (!valid || lane < 1) return;
+  const real dmm = __shfl_up(vq.dm, 1, 64);
+  if (!valid || lane < 1 || lane > MOMX_OUT) return;
   if (iny && i >= 0 && i <= n) {
     // the update (xremap.f90:282-329)
-    const double rv = rho_v[c], c1 = cv1v[c], c2 = cv2v[c];
-    const double rvn = (rv * c1 + dmm - dm_) / c2;
+    const double rv = vq.rv, c1 = vq.c1, c2 = vq.c2;
+    const double rvn = (rv * c1 + dmm - vq.dm) / c2;
 #pragma unroll
     for (int q = 0; q < 3; ++q) VO[q][c] = (rv * V[q][c] * c1 + Fm[q] - F[q]) / (c2 * rvn);
   } else {
@@ -711,25 +704,24 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_x(Geo g, Box b, const rea
   }
 }
 
-// Momentum remap of the y and z sweeps (M = 1, 2): the fluxes and the
-// update of the x-sweep kernel above, in one pass marching along the sweep
-// axis with the velocity stencil, db1, the sweep velocity and rho_v in
-// registers.  Marching along
-// the sweep axis, the flux at s-1 that the update at s needs is the one the
-// previous iteration formed (rounded to the storage type, as a stored flux
-// is), so no flux array is written or read.  The new velocities go to
-// vxo..vzo, because neighbouring chunks still read the old ones; every point
-// of the launch box (the whole padded box) is written: nodes 0..n updated,
-// the rest copied.
+// Momentum remap of the y and z sweeps (M = 1, 2): the vertex quantities,
+// fluxes and update of the x-sweep kernel above, in one pass marching along
+// the sweep axis with the velocity stencil, db1 and the sweep velocity in
+// registers; the vertex quantities of node m+1 are formed once and reused as
+// node m's in the next iteration, and the flux at s-1 that the update at s
+// needs is the previous iteration's (rounded to the storage type, as a
+// stored flux is).  The new velocities go to vxo..vzo, because neighbouring
+// chunks still read the old ones; every point of the launch box (the whole
+// padded box) is written: nodes 0..n updated, the rest copied.
 template <int M>
 __global__ __launch_bounds__(BX * BY) void k_mom_remap_m(Geo g, Box b, const real *__restrict__ vx,
                                                          const real *__restrict__ vy, const real *__restrict__ vz,
                                                          const real *__restrict__ vel1,
                                                          const real *__restrict__ db1,
-                                                         const real *__restrict__ rho_v,
-                                                         const real *__restrict__ cv1v,
-                                                         const real *__restrict__ cv2v,
-                                                         const real *__restrict__ dmv, real *__restrict__ vxo,
+                                                         const real *__restrict__ rho1,
+                                                         const real *__restrict__ cv1,
+                                                         const real *__restrict__ cv2,
+                                                         const real *__restrict__ dm, real *__restrict__ vxo,
                                                          real *__restrict__ vyo, real *__restrict__ vzo, double dt) {
   MCOLUMN(M, b, i, p, m0, m1)
   const int MS = M == 2 ? g.sz : g.sy;
@@ -747,9 +739,9 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_m(Geo g, Box b, const rea
     for (int q = 0; q < 3; ++q) VO[q][c] = V[q][c];
   }
   if (ms > me) return;
-  // flux window at point ms-1 (>= -1): v at ms-2..ms+1, db1 at ms-1..ms+1,
-  // the sweep velocity and rho_v at ms-1, ms
-  real u[3][4], d[3], v0, v1, rv0, rv1;
+  // window at node ms-1 (>= -1): v at ms-2..ms+1, db1 at ms-1..ms+1, the
+  // sweep velocity at ms-1, ms; vertex quantities of nodes ms-1 and ms
+  real u[3][4], d[3], v0, v1;
   int c = g.idx(i, MJ(M, ms - 1, p), MK(M, ms - 1, p));
 #pragma unroll
   for (int q = 0; q < 3; ++q)
@@ -759,8 +751,7 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_m(Geo g, Box b, const rea
   for (int t = 0; t < 3; ++t) d[t] = db1[c + t * MS];
   v0 = vel1[c];
   v1 = vel1[c + MS];
-  rv0 = rho_v[c];
-  rv1 = rho_v[c + MS];
+  VertexV vc = vertex_at(g, rho1, cv1, cv2, dm, c), vn = vertex_at(g, rho1, cv1, cv2, dm, c + MS);
   real Fp[3], dp;
   for (int m = ms - 1; m <= me; ++m) {
     if (m > ms - 1) {
@@ -777,8 +768,8 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_m(Geo g, Box b, const rea
       d[2] = db1[c + 3 * MS];
       v0 = v1;
       v1 = vel1[c + 2 * MS];
-      rv0 = rv1;
-      rv1 = rho_v[c + 2 * MS];
+      vc = vn;
+      vn = vertex_at(g, rho1, cv1, cv2, dm, c + 2 * MS);
       c += MS;
     }
     const int j = MJ(M, m, p), k = MK(M, m, p);
@@ -792,8 +783,8 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_m(Geo g, Box b, const rea
     const double dcu = d[0] * vad_p + d[2] * vad_m;
     const double dbu = 0.5 * (d[0] + d[1]) * vad_p + 0.5 * (d[1] + d[2]) * vad_m;
     const double phi = fabs(v_advect) * dt / dbu;
-    const double rhou = rv0 * vad_p + rv1 * vad_m;
-    const real dm_ = dmv[c];
+    const double rhou = (double)vc.rv * vad_p + (double)vn.rv * vad_m;
+    const real dm_ = vc.dm;
     const double dd = dm_;
     const double dmu = fabs(dd) / area / dbu / rhou;
     real F[3];
@@ -809,7 +800,7 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_m(Geo g, Box b, const rea
     }
     if (m >= ms) {
       // the update at m
-      const double rv = rv0, c1 = cv1v[c], c2 = cv2v[c];
+      const double rv = vc.rv, c1 = vc.c1, c2 = vc.c2;
       const double rvn = (rv * c1 + dp - dm_) / c2;
 #pragma unroll
       for (int q = 0; q < 3; ++q) VO[q][c] = (rv * u[q][1] * c1 + Fp[q] - F[q]) / (c2 * rvn);

@@ -50,7 +50,7 @@ struct Dev {
   double *fxr, *fyr, *fzr;       // resistive EMFs (CT update of B)
   real *curlb, *qx, *qy, *qz;
   // remap scratch: F is the CT face flux (FP64), Fe the energy flux
-  real *cv1, *cv2, *db1, *Fe, *dm, *Fx, *Fy, *Fz, *rho_vr, *cv1v, *cv2v, *dmv;
+  real *cv1, *cv2, *db1, *Fe, *dm, *Fx, *Fy, *Fz;
   double *F, *F2;
   // conduction scratch
   real *larsen, *Lc, *Yb[3];
@@ -506,25 +506,20 @@ int sweep_full(double dt, double pass[3], bool first) {
     S.Fe = te;
   }
   {
-    int lo[3] = {0, 0, 0}, h2[3] = {N[0], N[1], N[2]};
-    lo[AX] = -1;
-    h2[AX] = N[AX] + 1;
-    Box vb = mkbox(lo[0], h2[0], lo[1], h2[1], lo[2], h2[2]);
-    LAUNCH("vertex", k_vertex<AX>, zchunk_grid(vb), box_block(), S.g, vb, S.rho, S.cv1, S.cv2, S.dm, S.rho_vr, S.cv1v,
-           S.cv2v, S.dmv);
-    // momentum fluxes and update in one pass over the padded box; the new
-    // velocities land in the flux arrays, which then swap roles with the
-    // velocity arrays
+    // vertex quantities, momentum fluxes and update in one pass over the
+    // padded box (rho1 = rho before this sweep's update, which went to
+    // rho_alt); the new velocities land in the flux arrays, which then swap
+    // roles with the velocity arrays
     const Box xb = full_box();
     if constexpr (AX == 0) {
       const long npts = (long)(xb.x1 - xb.x0 + 1) * (xb.y1 - xb.y0 + 1);
       const long waves = (npts + MOMX_OUT - 1) / MOMX_OUT;
       const dim3 gx((unsigned)((waves + BY - 1) / BY), 1, (unsigned)(xb.z1 - xb.z0 + 1));
-      LAUNCH("mom_remap", k_mom_remap_x, gx, box_block(), S.g, xb, S.vx, S.vy, S.vz, vel, S.db1, S.rho_vr, S.cv1v,
-             S.cv2v, S.dmv, S.Fx, S.Fy, S.Fz, dt);
+      LAUNCH("mom_remap", k_mom_remap_x, gx, box_block(), S.g, xb, S.vx, S.vy, S.vz, vel, S.db1, S.rho, S.cv1,
+             S.cv2, S.dm, S.Fx, S.Fy, S.Fz, dt);
     } else {
       LAUNCH("mom_remap", k_mom_remap_m<AX>, zchunk_grid(march_box(xb, AX)), box_block(), S.g, xb, S.vx, S.vy, S.vz,
-             vel, S.db1, S.rho_vr, S.cv1v, S.cv2v, S.dmv, S.Fx, S.Fy, S.Fz, dt);
+             vel, S.db1, S.rho, S.cv1, S.cv2, S.dm, S.Fx, S.Fy, S.Fz, dt);
     }
     real *tv;
     tv = S.vx; S.vx = S.Fx; S.Fx = tv;
@@ -861,7 +856,7 @@ int api_init(const lare_grid *grid, const lare_params *params) {
   real **bulk[] = {&S.rho, &S.rho_alt, &S.energy, &S.vx, &S.vy, &S.vz, &S.vx1, &S.vy1, &S.vz1, &S.eta,
                     &S.visc3, &S.visc_dep, &S.ohmic_dep, &S.rho_v, &S.cs_v, &S.al1, &S.al2, &S.al3,
                     &S.visc_heat, &S.p0, &S.pp, &S.curlb, &S.cv1, &S.cv2, &S.db1, &S.Fe, &S.dm, &S.Fx, &S.Fy,
-                    &S.Fz, &S.rho_vr, &S.cv1v, &S.cv2v, &S.dmv, &S.larsen, &S.Lc, &S.Yb[0], &S.Yb[1], &S.Yb[2]};
+                    &S.Fz, &S.larsen, &S.Lc, &S.Yb[0], &S.Yb[1], &S.Yb[2]};
   for (real **pp : bulk) {
     int rc = dalloc(pp);
     if (rc) return rc;
