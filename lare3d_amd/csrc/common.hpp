@@ -43,9 +43,19 @@ struct Box {
 // block, so a box a few points wider than a multiple of 64 (n+1 .. n+4 for
 // n = 256) does not launch a column of blocks whose waves are almost empty.
 // i always lies inside [x0, x1]; j may pass y1 in the last block.
+//
+// Blocks are dispatched to the eight XCDs round-robin by linear block id,
+// and each XCD has its own L2.  Tile t of a plane goes to block
+// xcd_tile(t): the tiles that share an XCD (block ids congruent mod 8) get
+// consecutive plane positions, so the rows a tile's stencil reads from its
+// neighbour tiles are mostly in the same L2.
+__device__ __forceinline__ unsigned xcd_tile(unsigned bx, unsigned gx) {
+  const unsigned q = gx >> 3, r = gx & 7u, t = bx & 7u;
+  return t * q + (t < r ? t : r) + (bx >> 3);
+}
 #define PLANE_POS(b, i, j)                                                                  \
   const unsigned pw_ = (unsigned)((b).x1 - (b).x0 + 1);                                     \
-  const unsigned pp_ = blockIdx.x * (unsigned)(BX * BY) + threadIdx.y * (unsigned)BX + threadIdx.x; \
+  const unsigned pp_ = xcd_tile(blockIdx.x, gridDim.x) * (unsigned)(BX * BY) + threadIdx.y * (unsigned)BX + threadIdx.x; \
   const unsigned pj_ = pp_ / pw_;                                                           \
   const int i = (b).x0 + (int)(pp_ - pj_ * pw_);                                            \
   const int j = (b).y0 + (int)pj_;

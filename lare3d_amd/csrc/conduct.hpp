@@ -224,7 +224,7 @@ __global__ __launch_bounds__(BX * BY) void k_rkl2_z(Geo g, Box b, const real *__
   const int lane = (int)threadIdx.x;  // BX = 64: threadIdx.y is the wave
   const int wq = b.x1 - b.x0 + 2;     // x0-1 .. x1
   const int npts = wq * (b.y1 - b.y0 + 1);
-  const int f = (int)(blockIdx.x * BY + threadIdx.y) * RKL_OUT + lane - 1;
+  const int f = (int)(xcd_tile(blockIdx.x, gridDim.x) * BY + threadIdx.y) * RKL_OUT + lane - 1;
   const bool valid = f >= 0 && f < npts;
   const int ff = valid ? f : 0;  // lanes past the plane read point 0, unused
   const int jr = ff / wq;

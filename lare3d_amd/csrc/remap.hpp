@@ -188,7 +188,7 @@ __global__ __launch_bounds__(BX * BY) void k_ct_flux_x(Geo g, Box b, const doubl
   const int lane = (int)threadIdx.x;  // BX = 64: threadIdx.y is the wave
   const int wq = g.nx + 2 - b.x0 + 1;  // row of quotient points x0..n+2
   const int npts = wq * (b.y1 - b.y0 + 1);
-  const int f = (int)(blockIdx.x * BY + threadIdx.y) * CTX_OUT + lane - 1;
+  const int f = (int)(xcd_tile(blockIdx.x, gridDim.x) * BY + threadIdx.y) * CTX_OUT + lane - 1;
   const bool valid = f >= 0 && f < npts;
   const int ff = valid ? f : 0;  // lanes past the plane read point 0, unused
   const int jr = ff / wq;
@@ -527,7 +527,7 @@ __global__ __launch_bounds__(BX * BY) void k_rho_energy_x(Geo g, Box b, const re
   const int lane = (int)threadIdx.x;  // BX = 64: threadIdx.y is the wave
   const int wq = b.x1 - b.x0 + 2;     // x0-1 .. x1
   const int npts = wq * (b.y1 - b.y0 + 1);
-  const int f = (int)(blockIdx.x * BY + threadIdx.y) * REX_OUT + lane - 1;
+  const int f = (int)(xcd_tile(blockIdx.x, gridDim.x) * BY + threadIdx.y) * REX_OUT + lane - 1;
   const bool valid = f >= 0 && f < npts;
   const int ff = valid ? f : 0;  // lanes past the plane read point 0, unused
   const int jr = ff / wq;
@@ -643,7 +643,7 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_x(Geo g, Box b, const rea
   const int lane = (int)threadIdx.x;  // BX = 64: threadIdx.y is the wave
   const int wq = b.x1 - b.x0 + 1;
   const int npts = wq * (b.y1 - b.y0 + 1);
-  const int f = (int)(blockIdx.x * BY + threadIdx.y) * MOMX_OUT + lane - 1;
+  const int f = (int)(xcd_tile(blockIdx.x, gridDim.x) * BY + threadIdx.y) * MOMX_OUT + lane - 1;
   const bool valid = f >= 0 && f < npts;
   const int ff = valid ? f : 0;  // lanes past the plane read point 0, unused
   const int jr = ff / wq;
