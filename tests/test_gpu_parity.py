@@ -70,7 +70,10 @@ CASES = {
     "harris_stretched": lambda: configs.harris(16, stretch=(48.0, 16.0, 48.0)),
     # viscous damping with a non-uniform visc3
     "harris_damped": lambda: configs.harris(16, damping=0.05),
+    # odd, unequal sizes: partial tiles, waves and z-chunks on every axis
+    "harris_odd": lambda: configs.harris(19, ny=13, nz=11),
     "arcade": lambda: configs.arcade_conduction(16, 16, 24, s_target=7),
+    "arcade_odd": lambda: configs.arcade_conduction(13, 11, 19, s_target=5),
 }
 
 
