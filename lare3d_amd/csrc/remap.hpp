@@ -812,21 +812,3 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_m(Geo g, Box b, const rea
 }
 
 
-// B <-> face flux scaling of remap.f90:43-69 / :101-127 over the whole
-// face arrays.  dir > 0 multiplies (out may differ from in: the predictor
-// remap works on a copy); dir < 0 divides in the reference's order.
-__global__ __launch_bounds__(BX * BY) void k_bscale(Geo g, Box b, const double *__restrict__ bxi, const double *__restrict__ byi,
-                         const double *__restrict__ bzi, double *__restrict__ bxo, double *__restrict__ byo,
-                         double *__restrict__ bzo, int dir) {
-  BOX_INDEX(b, i, j, k);
-  const int c = g.idx(i, j, k);
-  if (dir > 0) {
-    if (bface_ok<0>(i, j, k)) bxo[c] = to_flux<0>(g, bxi[c], i, j, k);
-    if (bface_ok<1>(i, j, k)) byo[c] = to_flux<1>(g, byi[c], i, j, k);
-    if (bface_ok<2>(i, j, k)) bzo[c] = to_flux<2>(g, bzi[c], i, j, k);
-  } else {
-    if (bface_ok<0>(i, j, k)) bxo[c] = from_flux<0>(g, bxi[c], i, j, k);
-    if (bface_ok<1>(i, j, k)) byo[c] = from_flux<1>(g, byi[c], i, j, k);
-    if (bface_ok<2>(i, j, k)) bzo[c] = from_flux<2>(g, bzi[c], i, j, k);
-  }
-}

@@ -462,13 +462,13 @@ int sweep_predictor(double dt, bool first, bool last) {
 }
 
 template <int AX>
-int sweep_full(double dt, double pass[3], bool first) {
+int sweep_full(double dt, double pass[3], bool first, bool last) {
   const int N[3] = {nx(), ny(), nz()};
   const real *vel = AX == 0 ? S.vx1 : (AX == 1 ? S.vy1 : S.vz1);
   LAUNCH("geom", (k_geom<AX, 2>), zchunk_grid(cell_box()), box_block(), S.g, cell_box(), S.vx1, S.vy1, S.vz1, S.cv1,
          S.cv2, S.db1, dt, pass[0], pass[1], pass[2]);
   double *const B[3] = {S.bx, S.by, S.bz};
-  int rc = ct_pair<AX>(B, B, vel, dt, first ? 1 : 0);
+  int rc = ct_pair<AX>(B, B, vel, dt, first ? 1 : (last ? -1 : 0));
   if (rc) return rc;
   int hi[3] = {N[0] + 1, N[1] + 1, N[2] + 1};
   hi[AX] = N[AX];
@@ -541,9 +541,11 @@ int eulerian_remap(int step, bool predictor) {
   const double dt = S.dt;
   const int c = ((step % 6) + 6) % 6;
   // The B -> face flux scaling (remap.f90:44-67) happens inside the first
-  // sweep's CT kernels; the scaling back (remap.f90:102-125) inside the last
-  // predictor sweep's update, and as its own pass after the full remap,
-  // whose sweeps refresh the B halos (on face fluxes) in between.
+  // sweep's CT kernels and the scaling back (remap.f90:102-125) inside the
+  // last sweep's update.  In the full remap the last sweep's halo refresh
+  // then runs on B instead of on face fluxes; the B halos it writes are
+  // written again from the same interior by the bfield_bcs that follows the
+  // scaling back in the reference, and nothing reads them in between.
   if (predictor) {
     for (int q = 0; q < 3; ++q) {
       const int ax = kStrang[c][q];
@@ -557,11 +559,11 @@ int eulerian_remap(int step, bool predictor) {
   double pass[3] = {1.0, 1.0, 1.0};
   for (int q = 0; q < 3; ++q) {
     const int ax = kStrang[c][q];
-    const bool first = q == 0;
-    int rc = ax == 0 ? sweep_full<0>(dt, pass, first) : (ax == 1 ? sweep_full<1>(dt, pass, first) : sweep_full<2>(dt, pass, first));
+    const bool first = q == 0, last = q == 2;
+    int rc = ax == 0 ? sweep_full<0>(dt, pass, first, last)
+                     : (ax == 1 ? sweep_full<1>(dt, pass, first, last) : sweep_full<2>(dt, pass, first, last));
     if (rc) return rc;
   }
-  LAUNCH_BOX("bscale", k_bscale, full_box(), S.bx, S.by, S.bz, S.bx, S.by, S.bz, -1);
   return bfield_bcs(S.bx, S.by, S.bz);
 }
 
