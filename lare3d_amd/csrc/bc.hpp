@@ -111,8 +111,12 @@ __global__ __launch_bounds__(64) void k_bc_axis(Geo g, BcSet s, int axis) {
 // copies of one operation touch disjoint destination and source layers
 // (n >= 4), so doing them in parallel gives the sequential result, and the
 // eight lanes of a line share its two end cache lines.
-template <class T>
-__device__ __forceinline__ void bc_slot(T *a, int op, int base, int n, int slot, bool dm_ok) {
+// Destination and source offsets (along the line) of one slot of a fill.
+struct BcMove {
+  int dst, src;
+  bool zero, act;
+};
+__device__ __forceinline__ BcMove bc_slot(int op, int n, int slot, bool dm_ok) {
   int dst = 0, src = 0;
   bool zero = false, act = false;
   switch (op) {
@@ -154,7 +158,8 @@ __device__ __forceinline__ void bc_slot(T *a, int op, int base, int n, int slot,
     default:
       break;
   }
-  if (act) a[base + dst] = zero ? (T)0 : a[base + src];
+  BcMove m = {dst, src, zero, act};
+  return m;
 }
 
 __global__ __launch_bounds__(256) void k_bc_x(Geo g, BcSet s) {
@@ -165,8 +170,21 @@ __global__ __launch_bounds__(256) void k_bc_x(Geo g, BcSet s) {
   const int j = -2 + line % ly, k = -2 + line / ly;
   const int base = g.idx(0, j, k);
   const bool dm_ok = j >= 0 && j <= g.ny + 1 && k >= 0 && k <= g.nz + 1;
-  for (int q = 0; q < s.nf; ++q) {
-    if (s.wide[q]) bc_slot((double *)s.f[q], s.op[q], base, g.nx, slot, dm_ok);
-    else bc_slot((float *)s.f[q], s.op[q], base, g.nx, slot, dm_ok);
+  // The fields are distinct arrays, so all loads are issued before any
+  // store (values pass through double, which is exact for FP32 too).
+  BcMove mv[BC_MAXF];
+  double v[BC_MAXF];
+#pragma unroll
+  for (int q = 0; q < BC_MAXF; ++q) {
+    mv[q] = q < s.nf ? bc_slot(s.op[q], g.nx, slot, dm_ok) : BcMove{0, 0, false, false};
+    v[q] = 0.0;
+    if (mv[q].act && !mv[q].zero)
+      v[q] = s.wide[q] ? ((const double *)s.f[q])[base + mv[q].src] : (double)((const float *)s.f[q])[base + mv[q].src];
+  }
+#pragma unroll
+  for (int q = 0; q < BC_MAXF; ++q) {
+    if (!mv[q].act) continue;
+    if (s.wide[q]) ((double *)s.f[q])[base + mv[q].dst] = v[q];
+    else ((float *)s.f[q])[base + mv[q].dst] = (float)v[q];
   }
 }
