@@ -1291,11 +1291,10 @@ int sur_conduct_step(int delta_mode, double *res_out, int *acc_out) {
   if (rc) return rc;
   rc = sur_features(1);
   if (rc) return rc;
-  const int gpr = (nx() + 15) / 16;
-  const int ngroups = gpr * ny() * nz();
-  const unsigned nblk = (unsigned)((ngroups + SUR_INFER_WAVES - 1) / SUR_INFER_WAVES);
-  LAUNCH("sur_infer", k_sur_infer, dim3(nblk < 2048u ? nblk : 2048u), dim3(64 * SUR_INFER_WAVES), S.g, SU.feat,
-         SU.w1t, SU.w2t, SU.w, S.energy, SU.E1, S.tfac, SU.nm.y_scale, ngroups);
+  const dim3 gi((unsigned)((nx() + SIT_WX - 1) / SIT_WX), (unsigned)((ny() + SIT_NW - 1) / SIT_NW),
+                (unsigned)((nz() + SIT_ZC - 1) / SIT_ZC));
+  LAUNCH("sur_infer", k_sur_infer_t, gi, dim3(64 * SIT_NW), S.g, SU.feat, SU.w1t, SU.w2t, SU.w, S.energy, SU.E1,
+         S.tfac, SU.nm.y_scale);
   double r;
   rc = sur_residual(SU.E1, &r);
   if (rc) return rc;

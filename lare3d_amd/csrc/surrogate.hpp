@@ -317,15 +317,28 @@ __global__ void k_sur_pack(const float *__restrict__ w, unsigned short *__restri
   }
 }
 
-#define SUR_INFER_WAVES 8
-// E1 = E0 + dT / tfac over the interior, 16 cells (consecutive in x) per
-// wave-group.  Features must be valid on cells 0..n+1.
-__global__ __launch_bounds__(64 * SUR_INFER_WAVES, 2) void k_sur_infer(
+// Inference over the interior with the features staged in LDS.  A block
+// covers a tile of SIT_NW rows of SIT_WX cells (one row per wave) and
+// marches in z over SIT_ZC planes, keeping the BF16 features of planes
+// k-1, k, k+1 of the tile and its one-cell halo in a three-plane ring; each
+// plane is read from HBM once per tile.  A wave computes its 64 cells as
+// four 16-cell groups that share every W1^T fragment, so one LDS read of
+// the weights feeds four MFMAs.  Per cell the layer-1 products accumulate
+// over the 21 K steps in order, with the same BF16 operands as a per-cell
+// gather.
+#define SIT_WX 64
+#define SIT_NW 8
+#define SIT_ZC 8
+#define SIT_RX (SIT_WX + 2)
+#define SIT_RY (SIT_NW + 2)
+#define SIT_PLANE (SIT_RX * SIT_RY * SUR_CH)  // bf16 elements of one staged plane
+__global__ __launch_bounds__(64 * SIT_NW) void k_sur_infer_t(
     Geo g, const float *__restrict__ feat, const unsigned short *__restrict__ w1t,
     const unsigned short *__restrict__ w2t, const float *__restrict__ w, const real *__restrict__ E0,
-    real *__restrict__ E1, double tfac, float y_scale, int ngroups) {
+    real *__restrict__ E1, double tfac, float y_scale) {
   __shared__ __attribute__((aligned(16))) unsigned short sW1[SUR_H * SUR_KP];
   __shared__ __attribute__((aligned(16))) unsigned short sW2[SUR_H * SUR_HP];
+  __shared__ __attribute__((aligned(16))) unsigned short sF[3 * SIT_PLANE];
   __shared__ float sB1[SUR_H], sB2[SUR_H], sW3[SUR_H];
   for (int t = threadIdx.x; t < SUR_H * SUR_KP; t += blockDim.x) sW1[t] = w1t[t];
   for (int t = threadIdx.x; t < SUR_H * SUR_HP; t += blockDim.x) sW2[t] = w2t[t];
@@ -334,80 +347,99 @@ __global__ __launch_bounds__(64 * SUR_INFER_WAVES, 2) void k_sur_infer(
     sB2[t] = w[SUR_OFF_B2 + t];
     sW3[t] = w[SUR_OFF_W3 + t];
   }
-  __syncthreads();
   const float b3 = w[SUR_OFF_B3];
   const int l = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int col = l & 15, kq = 4 * (l >> 4);
-  const int gpr = (g.nx + 15) / 16;  // groups per x-row
-  for (int gi = blockIdx.x * SUR_INFER_WAVES + wave; gi < ngroups; gi += gridDim.x * SUR_INFER_WAVES) {
-    const int gx = gi % gpr, row = gi / gpr;
-    const int j = 1 + row % g.ny, k = 1 + row / g.ny;
-    const int i = 1 + gx * 16 + col;
-    const int ic = i <= g.nx ? i : g.nx;  // clamp loads of the masked lanes
-    const int c = g.idx(ic, j, k);
-    // layer 1: Z1^T (64 x 16 cells) = W1^T (64 x 336) . X^T (336 x 16), in
-    // three chunks of 7 MFMA steps; the 7 feature gathers of a chunk are
-    // issued together before its MFMAs, so their latencies overlap.
-    f32x4 acc[4];
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int ck = 0; ck < 3; ++ck) {
-      bf16x4 bops[7];
-#pragma unroll
-      for (int u = 0; u < 7; ++u) {
-        const int kb = (ck * 7 + u) * 16 + kq;
-        const int q = kb / SUR_CH, ch = kb % SUR_CH;
-        float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (kb < SUR_KREAL) x = *(const float4 *)(feat + (size_t)(c + nb_offset(g, q)) * SUR_CH + ch);
-        bops[u] = pack4(x.x, x.y, x.z, x.w);
-      }
-#pragma unroll
-      for (int u = 0; u < 7; ++u) {
-        const int kb = (ck * 7 + u) * 16 + kq;
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt) {
-          const bf16x4 aop = *(const bf16x4 *)(sW1 + (mt * 16 + col) * SUR_KP + kb);
-          acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aop, bops[u], acc[mt], 0, 0, 0);
-        }
-      }
-      __builtin_amdgcn_sched_barrier(0);
+  const int i0 = 1 + (int)blockIdx.x * SIT_WX, j0 = 1 + (int)blockIdx.y * SIT_NW, k0 = 1 + (int)blockIdx.z * SIT_ZC;
+  const int k1 = k0 + SIT_ZC - 1 < g.nz ? k0 + SIT_ZC - 1 : g.nz;
+  const int j = j0 + wave;
+  // stage plane kk (0..n+1) of the tile and its halo into ring slot kk % 3;
+  // halo points beyond n+1 (partial tiles) read cell n+1 and stay unused
+  auto stage = [&](int kk) {
+    unsigned short *P = sF + (kk % 3) * SIT_PLANE;
+    for (int t = threadIdx.x; t < SIT_RX * SIT_RY * 3; t += blockDim.x) {
+      const int cq = t % 3, r = t / 3;
+      const int xx = r % SIT_RX, yy = r / SIT_RX;
+      int ii = i0 - 1 + xx, jj = j0 - 1 + yy;
+      ii = ii <= g.nx + 1 ? ii : g.nx + 1;
+      jj = jj <= g.ny + 1 ? jj : g.ny + 1;
+      const float4 x = *(const float4 *)(feat + (size_t)g.idx(ii, jj, kk) * SUR_CH + 4 * cq);
+      *(bf16x4 *)(P + (yy * SIT_RX + xx) * SUR_CH + 4 * cq) = pack4(x.x, x.y, x.z, x.w);
     }
-    bf16x4 h1[4];
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-      float hv[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) hv[r] = gelu(acc[mt][r] + sB1[mt * 16 + kq + r]);
-      h1[mt] = pack4(hv[0], hv[1], hv[2], hv[3]);
-    }
-    // layer 2: Z2^T = W2^T (64 x 64) . H1^T (64 x 16); H1^T fragments are h1[]
-    f32x4 acc2[4];
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-      acc2[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-        const bf16x4 aop = *(const bf16x4 *)(sW2 + (mt * 16 + col) * SUR_HP + kk * 16 + kq);
-        acc2[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aop, h1[kk], acc2[mt], 0, 0, 0);
-      }
-    }
-    // layer 3 (FP32): y = W3 . GELU(Z2 + b2) + b3, summed over the 4 lane groups
-    float part = 0.0f;
+  };
+  stage(k0 - 1);
+  stage(k0);
+  for (int k = k0; k <= k1; ++k) {
+    stage(k + 1);
+    __syncthreads();
+    // layer 1: Z1^T (64 x 16 cells) per group = W1^T (64 x 336) . X^T
+    f32x4 acc[4][4];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int f = mt * 16 + kq + r;
-        part += sW3[f] * gelu(acc2[mt][r] + sB2[f]);
+      for (int cg = 0; cg < 4; ++cg) acc[mt][cg] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int st = 0; st < SUR_K / 16; ++st) {
+      const int kb = st * 16 + kq;
+      bf16x4 bop[4];
+      if (kb < SUR_KREAL) {
+        const int q = kb / SUR_CH, ch = kb % SUR_CH;
+        const int dz = q / 9 - 1, dy = (q / 3) % 3 - 1, dx = q % 3 - 1;
+        const unsigned short *P = sF + ((k + dz) % 3) * SIT_PLANE + ((wave + 1 + dy) * SIT_RX + 1 + dx) * SUR_CH + ch;
+#pragma unroll
+        for (int cg = 0; cg < 4; ++cg) bop[cg] = *(const bf16x4 *)(P + (16 * cg + col) * SUR_CH);
+      } else {
+#pragma unroll
+        for (int cg = 0; cg < 4; ++cg) bop[cg] = bf16x4{0, 0, 0, 0};
       }
-    part += __shfl_xor(part, 16, 64);
-    part += __shfl_xor(part, 32, 64);
-    if (l < 16 && i <= g.nx) {
-      const double dT = (double)((part + b3) * y_scale);
-      E1[c] = E0[c] + dT / tfac;
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        const bf16x4 aop = *(const bf16x4 *)(sW1 + (mt * 16 + col) * SUR_KP + kb);
+#pragma unroll
+        for (int cg = 0; cg < 4; ++cg)
+          acc[mt][cg] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aop, bop[cg], acc[mt][cg], 0, 0, 0);
+      }
     }
+#pragma unroll
+    for (int cg = 0; cg < 4; ++cg) {
+      bf16x4 h1[4];
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        float hv[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) hv[r] = gelu(acc[mt][cg][r] + sB1[mt * 16 + kq + r]);
+        h1[mt] = pack4(hv[0], hv[1], hv[2], hv[3]);
+      }
+      // layer 2: Z2^T = W2^T (64 x 64) . H1^T (64 x 16)
+      f32x4 acc2[4];
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        acc2[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+          const bf16x4 aop = *(const bf16x4 *)(sW2 + (mt * 16 + col) * SUR_HP + kk * 16 + kq);
+          acc2[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aop, h1[kk], acc2[mt], 0, 0, 0);
+        }
+      }
+      // layer 3 (FP32): y = W3 . GELU(Z2 + b2) + b3, summed over the 4 lane groups
+      float part = 0.0f;
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int f = mt * 16 + kq + r;
+          part += sW3[f] * gelu(acc2[mt][r] + sB2[f]);
+        }
+      part += __shfl_xor(part, 16, 64);
+      part += __shfl_xor(part, 32, 64);
+      const int i = i0 + 16 * cg + l;
+      if (l < 16 && i <= g.nx && j <= g.ny) {
+        const int c = g.idx(i, j, k);
+        const double dT = (double)((part + b3) * y_scale);
+        E1[c] = E0[c] + dT / tfac;
+      }
+    }
+    __syncthreads();
   }
 }
 

@@ -173,6 +173,52 @@ def test_device_forward_and_gradient_match_restatement():
 
 
 @pytest.mark.gpu
+def test_device_inference_matches_restatement():
+    """One surrogate super-step accepted unconditionally (tolerance 1e30) leaves
+    E1 = E0 + dT / tfac on every interior cell, dT being the network's output
+    times the target scale: compare with the restatement's forward pass (BF16
+    operands) on the standardised features of the same state.  The 16^3 grid
+    leaves the inference tiles partial in x (64-cell rows)."""
+    from lare3d_amd.surrogate import Surrogate
+    grid, params, f, dt = configs.harris_conduction(16, seed=3, s_target=5)
+    dev = _device(grid, params, f)
+    try:
+        dev.boundary_conditions()
+        dev.set_scalars(dt=dt)
+        f0 = dev.fields()
+        sur = Surrogate(dev)
+        w = ref.init_params(7)
+        w[ref.OFF_B3] = 0.02
+        sur.set_params(w)
+        print("\n[case] surrogate inference 16^3", flush=True)
+        sur.collect(2000, seed=5)
+        sur.fit_norm()
+        mu, sd, y_scale = sur.norm()
+        sur.enable(False, 1.0e30)
+        res, acc = sur.conduct()
+        assert acc
+        f1 = dev.fields()
+    finally:
+        dev.close()
+    n = grid.nx
+    cells = [(i, j, k) for k in range(1, n + 1) for j in range(1, n + 1) for i in range(1, n + 1)]
+    X = ref.features(grid, f0, dt, params.gamma, params.xi_n, cells)
+    for q in range(27):
+        for ch in range(ref.NCH):
+            col = q * ref.CH + ch
+            X[:, col] = (X[:, col] - mu[ch]) / sd[ch]
+    y = ref.forward(w, X, emulate_bf16=True)
+    dT_orc = (y * np.float32(y_scale)).astype(np.float64)
+    tfac = (params.gamma - 1.0) / (2.0 - params.xi_n)
+    e0 = np.array([f0.energy[i + 1, j + 1, k + 1] for i, j, k in cells])
+    e1 = np.array([f1.energy[i + 1, j + 1, k + 1] for i, j, k in cells])
+    dT_dev = tfac * (e1 - e0)
+    err = float(np.linalg.norm(dT_dev - dT_orc) / np.linalg.norm(dT_orc))
+    print(f"inference dT rel L2 err {err:.2e} over {len(cells)} cells")
+    assert err <= 2e-3
+
+
+@pytest.mark.gpu
 def test_device_training_rows_match_restatement():
     from oracle.oracle import Oracle
     from lare3d_amd.surrogate import Surrogate
