@@ -566,58 +566,119 @@ __device__ __forceinline__ double mom_area(const Geo &g, int i, int j, int k) {
   return g.dxc[i] * g.dyc[j];
 }
 
-// Vertex quantities of xremap.f90:201-280 at the node of index c: rho_v,
-// cv1_v, cv2_v and dm_v over the eight cells (i..i+1, j..j+1, k..k+1),
-// summed in the order (i,j,k), (i+1,j,k), (i,j+1,k), (i+1,j+1,k), then the
-// same four at k+1, and rounded to the storage type as stored values are.
+// Vertex quantities of xremap.f90:201-280 at a node: rho_v, cv1_v, cv2_v
+// and dm_v over the eight cells (i..i+1, j..j+1, k..k+1), summed in the
+// order (i,j,k), (i+1,j,k), (i,j+1,k), (i+1,j+1,k), then the same four at
+// k+1, and rounded to the storage type as stored values are.  A kernel that
+// marches along z or y forms them node after node: VSum holds the four
+// running sums, and VMarch<M> what the next node along M can take over
+// from this one, so that only the new plane (z) or row (y) of cells is
+// loaded there.  The sums are formed in the same order either way.
 struct VertexV {
   real rv, c1, c2, dm;
 };
-__device__ __forceinline__ VertexV vertex_at(const Geo &g, const real *__restrict__ rho1, const real *__restrict__ cv1,
-                                             const real *__restrict__ cv2, const real *__restrict__ dm, int c) {
-  const int Y = g.sy;
-  const Quad lr = quad_fwd(rho1, c, Y), l1 = quad_fwd(cv1, c, Y), l2 = quad_fwd(cv2, c, Y), ld = quad_fwd(dm, c, Y);
-  const int ch = c + g.sz;
-  const Quad hr = quad_fwd(rho1, ch, Y), h1 = quad_fwd(cv1, ch, Y), h2 = quad_fwd(cv2, ch, Y),
-             hd = quad_fwd(dm, ch, Y);
-  double s = (double)lr.p0 * l1.p0;
-  s = s + (double)lr.px * l1.px;
-  s = s + (double)lr.py * l1.py;
-  s = s + (double)lr.pxy * l1.pxy;
-  s = s + (double)hr.p0 * h1.p0;
-  s = s + (double)hr.px * h1.px;
-  s = s + (double)hr.py * h1.py;
-  s = s + (double)hr.pxy * h1.pxy;
-  double s1 = l1.p0;
-  s1 = s1 + l1.px;
-  s1 = s1 + l1.py;
-  s1 = s1 + l1.pxy;
-  s1 = s1 + h1.p0;
-  s1 = s1 + h1.px;
-  s1 = s1 + h1.py;
-  s1 = s1 + h1.pxy;
-  double s2 = l2.p0;
-  s2 = s2 + l2.px;
-  s2 = s2 + l2.py;
-  s2 = s2 + l2.pxy;
-  s2 = s2 + h2.p0;
-  s2 = s2 + h2.px;
-  s2 = s2 + h2.py;
-  s2 = s2 + h2.pxy;
-  double sd = ld.p0;
-  sd = sd + ld.px;
-  sd = sd + ld.py;
-  sd = sd + ld.pxy;
-  sd = sd + hd.p0;
-  sd = sd + hd.px;
-  sd = sd + hd.py;
-  sd = sd + hd.pxy;
-  VertexV v;
-  v.rv = s / s1;
-  v.c1 = s1 * 0.125;
-  v.c2 = s2 * 0.125;
-  v.dm = sd * 0.125;
+struct VSum {
+  double s, s1, s2, sd;  // sum of rho1*cv1, of cv1, of cv2, of dm
+};
+// cells at c, c+1 (and c+D, c+1+D for N = 4): one x pair, or a quad
+template <int N>
+struct VCellN {
+  real r[N], a[N], b[N], d[N];
+};
+template <int N>
+__device__ __forceinline__ VCellN<N> vcells_at(const real *__restrict__ rho1, const real *__restrict__ cv1,
+                                               const real *__restrict__ cv2, const real *__restrict__ dm, int c,
+                                               int D) {
+  VCellN<N> v;
+#pragma unroll
+  for (int t = 0; t < N; ++t) {
+    const int o = c + (t & 1) + (t >> 1) * D;
+    v.r[t] = rho1[o];
+    v.a[t] = cv1[o];
+    v.b[t] = cv2[o];
+    v.d[t] = dm[o];
+  }
   return v;
+}
+// sums over the cells of v, starting with v's first cell
+template <int N>
+__device__ __forceinline__ VSum vsum_first(const VCellN<N> &v) {
+  VSum p;
+  p.s = (double)v.r[0] * v.a[0];
+  p.s1 = v.a[0];
+  p.s2 = v.b[0];
+  p.sd = v.d[0];
+#pragma unroll
+  for (int t = 1; t < N; ++t) {
+    p.s = p.s + (double)v.r[t] * v.a[t];
+    p.s1 = p.s1 + v.a[t];
+    p.s2 = p.s2 + v.b[t];
+    p.sd = p.sd + v.d[t];
+  }
+  return p;
+}
+// p continued over the cells of v
+template <int N>
+__device__ __forceinline__ VSum vsum_add(VSum p, const VCellN<N> &v) {
+#pragma unroll
+  for (int t = 0; t < N; ++t) {
+    p.s = p.s + (double)v.r[t] * v.a[t];
+    p.s1 = p.s1 + v.a[t];
+    p.s2 = p.s2 + v.b[t];
+    p.sd = p.sd + v.d[t];
+  }
+  return p;
+}
+__device__ __forceinline__ VertexV vertex_of(const VSum &p) {
+  VertexV v;
+  v.rv = p.s / p.s1;
+  v.c1 = p.s1 * 0.125;
+  v.c2 = p.s2 * 0.125;
+  v.dm = p.sd * 0.125;
+  return v;
+}
+// What a node passes to the next one along M.  Along z (M = 2) the next
+// node's first four cells are this node's last four, so their sums carry
+// over whole.  Along y (M = 1) the next node's cells 0, 1 are this node's
+// 2, 3 (their sums carry over) and its cells 4, 5 are this node's 6, 7, which
+// come between new cells in the order and so carry over as values.
+template <int M>
+struct VMarch {
+  VSum head;
+  VCellN<M == 1 ? 2 : 1> tail;  // unused for M = 2
+};
+// Vertex quantities of the node at c.  With have set, m holds what the node
+// one step below c along M passed on; on return it holds what this node
+// passes on.
+template <int M>
+__device__ __forceinline__ VertexV vertex_march(const Geo &g, const real *__restrict__ rho1,
+                                                const real *__restrict__ cv1, const real *__restrict__ cv2,
+                                                const real *__restrict__ dm, int c, VMarch<M> &m, bool have) {
+  const int Y = g.sy, Z = g.sz;
+  if constexpr (M == 2) {
+    const VSum lo = have ? m.head : vsum_first(vcells_at<4>(rho1, cv1, cv2, dm, c, Y));
+    const VCellN<4> hi = vcells_at<4>(rho1, cv1, cv2, dm, c + Z, Y);
+    m.head = vsum_first(hi);
+    return vertex_of(vsum_add(lo, hi));
+  } else {
+    VSum p;
+    VCellN<2> c45;
+    if (have) {
+      p = m.head;
+      c45 = m.tail;
+    } else {
+      p = vsum_first(vcells_at<2>(rho1, cv1, cv2, dm, c, Y));
+      c45 = vcells_at<2>(rho1, cv1, cv2, dm, c + Z, Y);
+    }
+    const VCellN<2> c23 = vcells_at<2>(rho1, cv1, cv2, dm, c + Y, Y);
+    const VCellN<2> c67 = vcells_at<2>(rho1, cv1, cv2, dm, c + Y + Z, Y);
+    p = vsum_add(p, c23);
+    p = vsum_add(p, c45);
+    p = vsum_add(p, c67);
+    m.head = vsum_first(c23);
+    m.tail = c67;
+    return vertex_of(p);
+  }
 }
 
 // Momentum remap of the x sweep: the vertex quantities (xremap.f90:201-280),
@@ -625,11 +686,12 @@ __device__ __forceinline__ VertexV vertex_at(const Geo &g, const real *__restric
 // transverse 0..n, the three components sharing the upwind switch and
 // geometry) and the updates of xremap.f90:282-329 (nodes 0..n) in one pass.
 // Waves cover runs of 64 points of a plane of the launch box (the whole
-// padded box) flattened along x, successive waves overlapping by two: each
-// lane forms the vertex quantities of its node, takes rho_v of node i+1 from
-// the lane to its right and the flux and dm_v of node i-1 from the lane to
-// its left; lanes 1..MOMX_OUT write.  Every point is written to vxo..vzo:
-// nodes 0..n updated, the rest copied.
+// padded box) flattened along x, successive waves overlapping by two, and
+// march in z over ZCHUNK planes, keeping the upper plane of vertex cells for
+// the next node: each lane forms the vertex quantities of its node, takes
+// rho_v of node i+1 from the lane to its right and the flux and dm_v of node
+// i-1 from the lane to its left; lanes 1..MOMX_OUT write.  Every point is
+// written to vxo..vzo: nodes 0..n updated, the rest copied.
 #define MOMX_OUT 62
 __global__ __launch_bounds__(BX * BY) void k_mom_remap_x(Geo g, Box b, const real *__restrict__ vx,
                                                          const real *__restrict__ vy, const real *__restrict__ vz,
@@ -649,66 +711,76 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_x(Geo g, Box b, const rea
   const int jr = ff / wq;
   const int i = b.x0 + (ff - jr * wq);
   const int j = b.y0 + jr;
-  const int k = b.z0 + (int)blockIdx.z;
-  const int c = g.idx(i, j, k);
+  ZRANGE(b, k0, k1)
   const real *V[3] = {vx, vy, vz};
   real *VO[3] = {vxo, vyo, vzo};
   const int n = g.nx;
-  const bool iny = j >= 0 && j <= g.ny && k >= 0 && k <= g.nz;
-  // vertex quantities of this node (vertex box: x -1..n+1, transverse 0..n)
-  VertexV vq = {0, 0, 0, 0};
-  if (iny && i >= -1 && i <= n + 1) vq = vertex_at(g, rho1, cv1, cv2, dm, c);
-  const real rvp = __shfl_down(vq.rv, 1, 64);  // rho_v of node i+1 (lanes 0..62)
-  // fluxes on x = -1..n (transverse 0..n)
-  real F[3] = {0, 0, 0};
-  if (iny && i >= -1 && i <= n) {
-    const double area = mom_area<0>(g, i, j, k);
-    const double v_advect = (vel1[c] + vel1[c + 1]) * 0.5;
-    const double sign_v = copysign(1.0, v_advect);
-    const double vad_p = (sign_v + 1.0) * 0.5;
-    const double vad_m = 1.0 - vad_p;
-    const double dci = db1[c + 1];
-    const double dcu = db1[c] * vad_p + db1[c + 2] * vad_m;
-    const double dbu = dc1<0>(g, db1, c, i) * vad_p + dc1<0>(g, db1, c + 1, i + 1) * vad_m;
-    const double phi = fabs(v_advect) * dt / dbu;
-    const double rhou = (double)vq.rv * vad_p + (double)rvp * vad_m;
-    const double d = vq.dm;
-    const double dmu = fabs(d) / area / dbu / rhou;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const real *u = V[q];
-      const double fm = u[c - 1], fi = u[c], fp = u[c + 1], fp2 = u[c + 2];
-      const double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
-      const double fu = fi * vad_p + fp * vad_m;
-      const double dfu = dfm * vad_p + dfp * vad_m;
-      const double Di = vl_di(dfi, dfu, sign_v, dci, dcu, dbu, phi);
-      const double fl = fu + Di * (1.0 - dmu);
-      F[q] = fl * d;
+  const bool writer = valid && lane >= 1 && lane <= MOMX_OUT;
+  // vertex box: x -1..n+1, transverse 0..n
+  const bool vin = j >= 0 && j <= g.ny && i >= -1 && i <= n + 1;
+  VMarch<2> w;
+  bool have = false;  // w holds what node k-1 passed on
+  for (int k = k0; k <= k1; ++k) {
+    const int c = g.idx(i, j, k);
+    const bool iny = j >= 0 && j <= g.ny && k >= 0 && k <= g.nz;
+    VertexV vq = {0, 0, 0, 0};
+    if (vin && k >= 0 && k <= g.nz) {
+      vq = vertex_march<2>(g, rho1, cv1, cv2, dm, c, w, have);
+      have = true;
     }
-  }
-  // lane l-1 holds node i-1 of the same row whenever i > x0
-  real Fm[3];
+    const real rvp = __shfl_down(vq.rv, 1, 64);  // rho_v of node i+1 (lanes 0..62)
+    // fluxes on x = -1..n (transverse 0..n)
+    real F[3] = {0, 0, 0};
+    if (iny && i >= -1 && i <= n) {
+      const double area = mom_area<0>(g, i, j, k);
+      const double v_advect = (vel1[c] + vel1[c + 1]) * 0.5;
+      const double sign_v = copysign(1.0, v_advect);
+      const double vad_p = (sign_v + 1.0) * 0.5;
+      const double vad_m = 1.0 - vad_p;
+      const double dci = db1[c + 1];
+      const double dcu = db1[c] * vad_p + db1[c + 2] * vad_m;
+      const double dbu = dc1<0>(g, db1, c, i) * vad_p + dc1<0>(g, db1, c + 1, i + 1) * vad_m;
+      const double phi = fabs(v_advect) * dt / dbu;
+      const double rhou = (double)vq.rv * vad_p + (double)rvp * vad_m;
+      const double d = vq.dm;
+      const double dmu = fabs(d) / area / dbu / rhou;
 #pragma unroll
-  for (int q = 0; q < 3; ++q) Fm[q] = __shfl_up(F[q], 1, 64);
-  const real dmm = __shfl_up(vq.dm, 1, 64);
-  if (!valid || lane < 1 || lane > MOMX_OUT) return;
-  if (iny && i >= 0 && i <= n) {
-    // the update (xremap.f90:282-329)
-    const double rv = vq.rv, c1 = vq.c1, c2 = vq.c2;
-    const double rvn = (rv * c1 + dmm - vq.dm) / c2;
+      for (int q = 0; q < 3; ++q) {
+        const real *u = V[q];
+        const double fm = u[c - 1], fi = u[c], fp = u[c + 1], fp2 = u[c + 2];
+        const double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
+        const double fu = fi * vad_p + fp * vad_m;
+        const double dfu = dfm * vad_p + dfp * vad_m;
+        const double Di = vl_di(dfi, dfu, sign_v, dci, dcu, dbu, phi);
+        const double fl = fu + Di * (1.0 - dmu);
+        F[q] = fl * d;
+      }
+    }
+    // lane l-1 holds node i-1 of the same row whenever i > x0
+    real Fm[3];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) VO[q][c] = (rv * V[q][c] * c1 + Fm[q] - F[q]) / (c2 * rvn);
-  } else {
+    for (int q = 0; q < 3; ++q) Fm[q] = __shfl_up(F[q], 1, 64);
+    const real dmm = __shfl_up(vq.dm, 1, 64);
+    if (!writer) continue;
+    if (iny && i >= 0 && i <= n) {
+      // the update (xremap.f90:282-329)
+      const double rv = vq.rv, c1 = vq.c1, c2 = vq.c2;
+      const double rvn = (rv * c1 + dmm - vq.dm) / c2;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) VO[q][c] = V[q][c];
+      for (int q = 0; q < 3; ++q) VO[q][c] = (rv * V[q][c] * c1 + Fm[q] - F[q]) / (c2 * rvn);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) VO[q][c] = V[q][c];
+    }
   }
 }
 
 // Momentum remap of the y and z sweeps (M = 1, 2): the vertex quantities,
 // fluxes and update of the x-sweep kernel above, in one pass marching along
 // the sweep axis with the velocity stencil, db1 and the sweep velocity in
-// registers; the vertex quantities of node m+1 are formed once and reused as
-// node m's in the next iteration, and the flux at s-1 that the update at s
+// registers; the vertex quantities of node m+1 are formed once, from the
+// cells it shares with node m and the next row or plane of cells, and reused
+// as node m's in the next iteration, and the flux at s-1 that the update at s
 // needs is the previous iteration's (rounded to the storage type, as a
 // stored flux is).  The new velocities go to vxo..vzo, because neighbouring
 // chunks still read the old ones; every point of the launch box (the whole
@@ -751,7 +823,9 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_m(Geo g, Box b, const rea
   for (int t = 0; t < 3; ++t) d[t] = db1[c + t * MS];
   v0 = vel1[c];
   v1 = vel1[c + MS];
-  VertexV vc = vertex_at(g, rho1, cv1, cv2, dm, c), vn = vertex_at(g, rho1, cv1, cv2, dm, c + MS);
+  VMarch<M> w;
+  VertexV vc = vertex_march<M>(g, rho1, cv1, cv2, dm, c, w, false);
+  VertexV vn = vertex_march<M>(g, rho1, cv1, cv2, dm, c + MS, w, true);
   real Fp[3], dp;
   for (int m = ms - 1; m <= me; ++m) {
     if (m > ms - 1) {
@@ -769,7 +843,7 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_m(Geo g, Box b, const rea
       v0 = v1;
       v1 = vel1[c + 2 * MS];
       vc = vn;
-      vn = vertex_at(g, rho1, cv1, cv2, dm, c + 2 * MS);
+      vn = vertex_march<M>(g, rho1, cv1, cv2, dm, c + 2 * MS, w, true);
       c += MS;
     }
     const int j = MJ(M, m, p), k = MK(M, m, p);

@@ -514,7 +514,7 @@ int sweep_full(double dt, double pass[3], bool first, bool last) {
     if constexpr (AX == 0) {
       const long npts = (long)(xb.x1 - xb.x0 + 1) * (xb.y1 - xb.y0 + 1);
       const long waves = (npts + MOMX_OUT - 1) / MOMX_OUT;
-      const dim3 gx((unsigned)((waves + BY - 1) / BY), 1, (unsigned)(xb.z1 - xb.z0 + 1));
+      const dim3 gx((unsigned)((waves + BY - 1) / BY), 1, (unsigned)((xb.z1 - xb.z0 + ZCHUNK) / ZCHUNK));
       LAUNCH("mom_remap", k_mom_remap_x, gx, box_block(), S.g, xb, S.vx, S.vy, S.vz, vel, S.db1, S.rho, S.cv1,
              S.cv2, S.dm, S.Fx, S.Fy, S.Fz, dt);
     } else {
