@@ -692,7 +692,11 @@ int lagrangian_step() {
       rc = heat_conduct_sts2(1);
       if (rc) return rc;
     }
-  } else {
+  } else if (!S.p.resistive_mhd) {
+    // energy = MAX(energy0 + 0, 0) (lagran.F90:180-181).  After a
+    // resistive step it changes nothing: the last substep's ohmic heating
+    // already clamped the whole cell box (:1305) and energy_bcs only copies
+    // clamped values, so the launch is left out then.
     LAUNCH_BOX("energy_max", k_energy_max, cell_box(), S.energy);
   }
 
