@@ -8,7 +8,9 @@
 //   k_visc_dt         p_visc + visc_heat over cells 0..n+1 (:680-737), the
 //                     start-of-step pressure, and the CFL minimum of set_dt
 //                     (:1094-1143) fused in the same pass
-//   k_eta / k_rkstep / k_bstep / k_ohmic   resistive sub-cycle (:1176-1635)
+//   k_resist / k_ohmic  resistive sub-cycle (:1176-1635): eta_calc, rkstep
+//                     and bstep in one pass, then the ohmic heating
+//   k_eta             eta_calc after the remap (lare3d.f90:87)
 //   k_pred_pressure   predictor energy/pressure over 0..n+1 (:234-249)
 //   k_force           f_visc (:745-789) + grad p + J x B (half-step B)
 //                     + gravity -> v1 at nodes 0..n (:251-387)
@@ -411,83 +413,116 @@ __global__ __launch_bounds__(BX * BY) void k_fill(Geo g, Box b, real *__restrict
   a[g.idx(i, j, k)] = v;
 }
 
-// rkstep (lagran.F90:1472-1503) over nodes 0..n, with the eta_calc of the
-// same substep (lagran.F90:1176-1230) fused in: the vertex current at each
-// node of the box (-1..n+1) sets eta there exactly as k_eta does, and at
-// nodes 0..n also gives the EMFs and curlb; the total_ohmic_heating
-// increments of :1309-1328 are reduced in the same pass.  Marches in z.
-__global__ __launch_bounds__(BX * BY) void k_rkstep(Geo g, Box b, const double *__restrict__ bx, const double *__restrict__ by,
-                         const double *__restrict__ bz, real *__restrict__ eta, double *__restrict__ fxr,
-                         double *__restrict__ fyr, double *__restrict__ fzr, real *__restrict__ curlb,
-                         double *__restrict__ partial, double dt, double eta_bg, double eta0, double j_max) {
-  PLANE_POS(b, i, j)
+// EMFs around a node in one plane, named as in bstep (lagran.F90:1582-1631):
+// kx at (i,j), (i-1,j); ky at (i,j), (i,j-1); kz at (i,j), (i,j-1), (i-1,j)
+struct EPlane {
+  double kx0, kxm, ky0, kym, kz0, kzm, kzx;
+};
+
+// One resistive substep's update of B in a single pass: at every node of
+// the box -1..n+1 the vertex current and eta of the substep's eta_calc
+// (lagran.F90:1176-1230), at nodes 0..n the rkstep EMFs and curlb
+// (:1472-1503) with the total_ohmic_heating increments (:1309-1328), and
+// bstep (:1582-1631), which takes the EMFs of the node and of its (i-1),
+// (j-1) neighbours in planes k and k-1.  A block is a tile of RS_ROWS rows of
+// 64 nodes marching in z over ZCHUNK planes (plus plane k0-1 for the EMFs of
+// plane k-1); lane 0 and row 0 only supply neighbour EMFs, which pass through
+// LDS, so a tile writes RS_OUT_X x RS_OUT_Y nodes.  Neighbouring tiles still
+// read the old B, so the new B goes to bxo..bzo at every node of the launch
+// box (the padded box): updated where bstep updates it, copied elsewhere.
+#define RS_ROWS 8
+#define RS_OUT_X 63
+#define RS_OUT_Y (RS_ROWS - 1)
+__global__ __launch_bounds__(64 * RS_ROWS) void k_resist(Geo g, Box b, int tiles_x, const double *__restrict__ bx,
+                                                         const double *__restrict__ by, const double *__restrict__ bz,
+                                                         double *__restrict__ bxo, double *__restrict__ byo,
+                                                         double *__restrict__ bzo, real *__restrict__ eta,
+                                                         real *__restrict__ curlb, double *__restrict__ partial,
+                                                         double dt, double eta_bg, double eta0, double j_max) {
+  __shared__ double sk[2][3][RS_ROWS][64];  // EMFs kx, ky, kz of the tile's nodes, by plane parity
+  __shared__ double red[RS_ROWS];
+  const int l = (int)threadIdx.x, w = (int)threadIdx.y;
+  const int t = (int)xcd_tile(blockIdx.x, gridDim.x);
+  const int ty = t / tiles_x, tx = t - ty * tiles_x;
+  const int i = b.x0 + tx * RS_OUT_X + l - 1;
+  const int j = b.y0 + ty * RS_OUT_Y + w - 1;
   ZRANGE(b, k0, k1)
-  double v[1] = {0.0};
-  if (j <= b.y1) {
-    const int Y = g.sy, Z = g.sz;
-    const bool inxy = i >= 0 && i <= g.nx && j >= 0 && j <= g.ny;
-    BPlane lo = bplane(bx, by, g.idx(i, j, k0), Y);
-    for (int k = k0; k <= k1; ++k) {
-      const int c = g.idx(i, j, k);
+  const bool out = l >= 1 && w >= 1 && i <= b.x1 && j <= b.y1;
+  const int Y = g.sy, Z = g.sz;
+  const bool eij = i >= -1 && i <= g.nx + 1 && j >= -1 && j <= g.ny + 1;  // eta box, in the plane
+  const bool kij = i >= 0 && i <= g.nx && j >= 0 && j <= g.ny;            // EMF box, in the plane
+  double v = 0.0;
+  EPlane m = {0, 0, 0, 0, 0, 0, 0};  // plane k-1
+  BPlane lo = {0, 0, 0, 0};
+  bool have_lo = false;
+  for (int k = k0 - 1; k <= k1; ++k) {
+    const int c = g.idx(i, j, k);
+    double kx = 0.0, ky = 0.0, kz = 0.0;
+    if (eij && k >= -1 && k <= g.nz + 1) {
+      if (!have_lo) lo = bplane(bx, by, c, Y);
       const BPlane hi = bplane(bx, by, c + Z, Y);
       double jx, jy, jz;
       vertex_current_vals(g.dxc[i], g.dyc[j], g.dzc[k], lo, hi, bz[c], bz[c + 1], bz[c + Y], bz[c + 1 + Y], jx, jy,
                           jz);
       lo = hi;
+      have_lo = true;
       const real es = sqrt(jx * jx + jy * jy + jz * jz) > j_max ? eta_bg + eta0 : eta_bg;
-      eta[c] = es;
-      if (inxy && k >= 0 && k <= g.nz) {
+      if (out && k >= k0) eta[c] = es;
+      if (kij && k >= 0 && k <= g.nz) {
         const double e = es;
-        fxr[c] = -jx * e * g.dxc[i] * 0.5;
-        fyr[c] = -jy * e * g.dyc[j] * 0.5;
-        fzr[c] = -jz * e * g.dzc[k] * 0.5;
-        const double cb = e * (jx * jx + jy * jy + jz * jz);
-        curlb[c] = cb;
-        double w1 = dt * g.dxc[i] * g.dyc[j] * g.dzc[k] * cb;
-        if (i == 0 || i == g.nx) w1 = w1 * 0.5;
-        if (j == 0 || j == g.ny) w1 = w1 * 0.5;
-        if (k == 0 || k == g.nz) w1 = w1 * 0.5;
-        v[0] += w1;
+        kx = -jx * e * g.dxc[i] * 0.5;
+        ky = -jy * e * g.dyc[j] * 0.5;
+        kz = -jz * e * g.dzc[k] * 0.5;
+        if (out && k >= k0) {
+          const double cb = e * (jx * jx + jy * jy + jz * jz);
+          curlb[c] = cb;
+          double w1 = dt * g.dxc[i] * g.dyc[j] * g.dzc[k] * cb;
+          if (i == 0 || i == g.nx) w1 = w1 * 0.5;
+          if (j == 0 || j == g.ny) w1 = w1 * 0.5;
+          if (k == 0 || k == g.nz) w1 = w1 * 0.5;
+          v += w1;
+        }
       }
     }
-  }
-  const int ops[1] = {1};
-  block_reduce_store<1>(v, ops, partial);
-}
-
-// bstep (lagran.F90:1582-1631); B updated in place (bx1 = bx at the same
-// point, :1259-1261).  Marches in z with the EMFs of plane k-1 in registers.
-struct EPlane {
-  double kx0, kxm, ky0, kym, kz0, kzm, kzx;  // kx at (i,j),(i-1,j); ky at (i,j),(i,j-1); kz at (i,j),(i,j-1),(i-1,j)
-};
-__device__ __forceinline__ EPlane eplane(const double *__restrict__ kx, const double *__restrict__ ky,
-                                         const double *__restrict__ kz, int c, int Y) {
-  EPlane p = {kx[c], kx[c - 1], ky[c], ky[c - Y], kz[c], kz[c - Y], kz[c - 1]};
-  return p;
-}
-__global__ __launch_bounds__(BX * BY) void k_bstep(Geo g, Box b, double *__restrict__ bx, double *__restrict__ by, double *__restrict__ bz,
-                        const double *__restrict__ kx, const double *__restrict__ ky, const double *__restrict__ kz,
-                        double dt) {
-  ZCOLUMN(b, i, j, k0, k1);
-  const int Y = g.sy, Z = g.sz;
-  EPlane m = eplane(kx, ky, kz, g.idx(i, j, k0) - Z, Y);  // plane k-1
-  for (int k = k0; k <= k1; ++k) {
-    const int c = g.idx(i, j, k);
-    const EPlane p = eplane(kx, ky, kz, c, Y);
-    if (j >= 1 && k >= 1) {
-      const double area = g.dyb[j] * g.dzb[k];
-      bx[c] = bx[c] + (p.kz0 - p.kzm + m.kz0 - m.kzm - p.ky0 + m.ky0 - p.kym + m.kym) * dt / area;
-    }
-    if (i >= 1 && k >= 1) {
-      const double area = g.dxb[i] * g.dzb[k];
-      by[c] = by[c] + (p.kx0 - m.kx0 + p.kxm - m.kxm - p.kz0 + p.kzx - m.kz0 + m.kzx) * dt / area;
-    }
-    if (i >= 1 && j >= 1) {
-      const double area = g.dxb[i] * g.dyb[j];
-      bz[c] = bz[c] + (p.ky0 - ky[c - 1] + p.kym - ky[c - 1 - Y] - p.kx0 + kx[c - Y] - p.kxm + kx[c - 1 - Y]) *
+    double(&s)[3][RS_ROWS][64] = sk[k & 1];
+    s[0][w][l] = kx;
+    s[1][w][l] = ky;
+    s[2][w][l] = kz;
+    __syncthreads();
+    if (out) {
+      // EMFs at (i,j), (i-1,j), (i,j-1) of this plane, as EPlane names them
+      const EPlane p = {kx, s[0][w][l - 1], ky, s[1][w - 1][l], kz, s[2][w - 1][l], s[2][w][l - 1]};
+      if (k >= k0) {
+        const bool upd = kij && k >= 0 && k <= g.nz;
+        double nbx = bx[c], nby = by[c], nbz = bz[c];
+        if (upd && j >= 1 && k >= 1) {
+          const double area = g.dyb[j] * g.dzb[k];
+          nbx = nbx + (p.kz0 - p.kzm + m.kz0 - m.kzm - p.ky0 + m.ky0 - p.kym + m.kym) * dt / area;
+        }
+        if (upd && i >= 1 && k >= 1) {
+          const double area = g.dxb[i] * g.dzb[k];
+          nby = nby + (p.kx0 - m.kx0 + p.kxm - m.kxm - p.kz0 + p.kzx - m.kz0 + m.kzx) * dt / area;
+        }
+        if (upd && i >= 1 && j >= 1) {
+          const double area = g.dxb[i] * g.dyb[j];
+          nbz = nbz + (p.ky0 - s[1][w][l - 1] + p.kym - s[1][w - 1][l - 1] - p.kx0 + s[0][w - 1][l] - p.kxm +
+                       s[0][w - 1][l - 1]) *
                           dt / area;
+        }
+        bxo[c] = nbx;
+        byo[c] = nby;
+        bzo[c] = nbz;
+      }
+      m = p;
     }
-    m = p;
+  }
+  v = wave_sum(v);
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  if (l == 0 && w == 0) {
+    double r = red[0];
+    for (int q = 1; q < RS_ROWS; ++q) r = r + red[q];
+    partial[(size_t)blockIdx.z * gridDim.x + blockIdx.x] = r;
   }
 }
 

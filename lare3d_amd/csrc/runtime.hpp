@@ -141,6 +141,15 @@ Box cell_box() { return mkbox(-1, nx() + 2, -1, ny() + 2, -1, nz() + 2); }
 Box interior() { return mkbox(1, nx(), 1, ny(), 1, nz()); }
 Box nodes0n() { return mkbox(0, nx(), 0, ny(), 0, nz()); }
 
+// grid of k_resist over the padded box: tiles of RS_OUT_X x RS_OUT_Y
+// written nodes, ZCHUNK planes per block
+dim3 resist_grid(int *tiles_x) {
+  const Box fb = full_box();
+  *tiles_x = (fb.x1 - fb.x0 + RS_OUT_X) / RS_OUT_X;
+  const int tiles_y = (fb.y1 - fb.y0 + RS_OUT_Y) / RS_OUT_Y;
+  return dim3((unsigned)(*tiles_x * tiles_y), 1, (unsigned)((fb.z1 - fb.z0 + ZCHUNK) / ZCHUNK));
+}
+
 // grid of a z-marching launch: BX*BY columns, ZCHUNK planes per block
 dim3 zchunk_grid(const Box &b) {
   return dim3(plane_blocks(b), 1, (unsigned)((b.z1 - b.z0 + 1 + ZCHUNK - 1) / ZCHUNK));
@@ -661,18 +670,24 @@ int lagrangian_step() {
     S.last_substeps = substeps;
     const double actual_dt = S.dt;
     S.dt = S.dt / (double)substeps;
-    // eta_calc of each substep runs inside k_rkstep (over the eta box
-    // -1..n+1); the B and energy halos of a substep are refreshed together
-    // (bstep does not read energy, ohmic does not read B)
-    const Box eb = mkbox(-1, nx() + 1, -1, ny() + 1, -1, nz() + 1);
+    // eta_calc, rkstep and bstep of each substep run in k_resist, which
+    // writes the new B to the fxr..fzr arrays; they then swap roles with
+    // bx..bz.  The B and energy halos of a substep are refreshed together
+    // (ohmic does not read B).
+    const Box fb = full_box();
+    int tiles_x;
+    const dim3 g2 = resist_grid(&tiles_x);
+    if ((size_t)g2.x * g2.y * g2.z > S.partial_cap)
+      return fail(LARE_E_ARG, "resistive grid of %u blocks exceeds the partial buffer", g2.x * g2.z);
     for (int sc = 1; sc <= substeps; ++sc) {
-      dim3 g2 = zchunk_grid(eb);
-      LAUNCH("rkstep", k_rkstep, g2, box_block(), S.g, eb, S.bx, S.by, S.bz, S.eta, S.fxr, S.fyr, S.fzr, S.curlb,
-             S.partial, S.dt, S.p.eta_background, S.p.eta0, S.p.j_max);
+      LAUNCH("resist", k_resist, g2, dim3(64, RS_ROWS, 1), S.g, fb, tiles_x, S.bx, S.by, S.bz, S.fxr, S.fyr, S.fzr,
+             S.eta, S.curlb, S.partial, S.dt, S.p.eta_background, S.p.eta0, S.p.j_max);
       rc = final_reduce(g2, 1, 0, 9, 1);
       if (rc) return rc;
-      LAUNCH("bstep", k_bstep, zchunk_grid(nodes0n()), box_block(), S.g, nodes0n(), S.bx, S.by, S.bz, S.fxr, S.fyr,
-             S.fzr, S.dt);
+      double *tb;
+      tb = S.bx; S.bx = S.fxr; S.fxr = tb;
+      tb = S.by; S.by = S.fyr; S.fyr = tb;
+      tb = S.bz; S.bz = S.fzr; S.fzr = tb;
       LAUNCH("ohmic", k_ohmic, zchunk_grid(cell_box()), box_block(), S.g, cell_box(), S.energy, S.ohmic_dep, S.rho,
              S.curlb, S.dt);
       BcField f[4] = {{S.bx, FK_BX}, {S.by, FK_BY}, {S.bz, FK_BZ}, {S.energy, FK_CELL}};
@@ -883,6 +898,9 @@ int api_init(const lare_grid *grid, const lare_params *params) {
   // partial sums: largest reduction grid (0..n+1 per axis) x 4 values
   dim3 gr = grid_from0(n[0] + 2, n[1] + 2, n[2] + 2);
   S.partial_cap = (size_t)gr.x * gr.y * gr.z * 4;
+  int tiles_x;
+  const dim3 grs = resist_grid(&tiles_x);
+  if ((size_t)grs.x * grs.y * grs.z > S.partial_cap) S.partial_cap = (size_t)grs.x * grs.y * grs.z;
   // + RED_ROWS x RED_MAXV level-1 results after the per-block partials
   HIPCHK(hipMalloc((void **)&S.partial, sizeof(double) * (S.partial_cap + RED_ROWS * RED_MAXV)));
   HIPCHK(hipMalloc((void **)&S.dscal, sizeof(double) * 16));
