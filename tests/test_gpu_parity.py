@@ -74,6 +74,10 @@ CASES = {
     "harris_damped": lambda: configs.harris(16, damping=0.05),
     # odd, unequal sizes: partial tiles, waves and z-chunks on every axis
     "harris_odd": lambda: configs.harris(19, ny=13, nz=11),
+    # a padded x extent of 69: two x tiles of the resistive kernel and two
+    # waves per row of the x-sweep kernels, so neighbours cross tile and wave
+    # edges inside the row
+    "harris_wide": lambda: configs.harris(64, ny=9, nz=6),
     "arcade": lambda: configs.arcade_conduction(16, 16, 24, s_target=7),
     "arcade_odd": lambda: configs.arcade_conduction(13, 11, 19, s_target=5),
 }
