@@ -15,8 +15,8 @@
 //   k_force           f_visc (:745-789) + grad p + J x B (half-step B)
 //                     + gravity -> v1 at nodes 0..n (:251-387)
 //   k_shock_corr      shock_heating (:861-937) fused with the corrector
-//                     (:415-462) over cells 1..n
-//   k_vupdate         v = 2 v1 - v (:399-407)
+//                     (:415-462) over cells 1..n, and v = 2 v1 - v
+//                     (:399-407) at nodes 0..n into a second buffer
 
 #define NONE_ZERO 2.2250738585072014e-308
 #define LARGEST 1.7976931348623157e+308
@@ -667,9 +667,12 @@ __global__ __launch_bounds__(BX * BY) void k_shock_corr(Geo g, Box b, const real
                              const real *__restrict__ al1, const real *__restrict__ al2,
                              const real *__restrict__ al3, const real *__restrict__ pp,
                              real *__restrict__ energy, real *__restrict__ rho, real *__restrict__ visc_dep,
+                             real *__restrict__ vxo, real *__restrict__ vyo, real *__restrict__ vzo,
                              double *__restrict__ partial, double dt) {
   // cells 1..n, marching in z: the node values (v, v1) and al1/al2 of plane
-  // k are reused as plane k-1 / k values of the next cell
+  // k are reused as plane k-1 / k values of the next cell.  The new
+  // velocity v = 2 v1 - v of node (i, j, k), the cell's upper corner, goes
+  // to vxo..vzo, since neighbouring cells still read the old v.
   PLANE_POS(b, i, j)
   ZRANGE(b, k0, k1)
   double v[1] = {0.0};
@@ -708,6 +711,11 @@ __global__ __launch_bounds__(BX * BY) void k_shock_corr(Geo g, Box b, const real
       visc_dep[c] = visc_dep[c] + dt * vh;
       rho[c] = r / (1.0 + dv);
       v[0] += dt * vh * g.cv(i, j, k);
+      // v = 2 v1 - v (:399-407) at node (i, j, k); k_vupdate_low does the
+      // nodes with an index 0
+      vxo[c] = (real)(2.0 * hx1.c - hx.c);
+      vyo[c] = (real)(2.0 * hy1.c - hy.c);
+      vzo[c] = (real)(2.0 * hz1.c - hz.c);
       lx = hx;
       ly = hy;
       lz = hz;
@@ -721,18 +729,47 @@ __global__ __launch_bounds__(BX * BY) void k_shock_corr(Geo g, Box b, const real
   block_reduce_store<1>(v, ops, partial);
 }
 
-__global__ __launch_bounds__(BX * BY) void k_vupdate(Geo g, Box b, real *__restrict__ vx, real *__restrict__ vy, real *__restrict__ vz,
-                          const real *__restrict__ vx1, const real *__restrict__ vy1,
-                          const real *__restrict__ vz1) {
-  BOX_INDEX(b, i, j, k);
-  const int c = g.idx(i, j, k);
-  vx[c] = 2.0 * vx1[c] - vx[c];
-  vy[c] = 2.0 * vy1[c] - vy[c];
-  vz[c] = 2.0 * vz1[c] - vz[c];
-}
 
 // energy_account (diagnostics.F90:517-609) + max |div B| * min(dx):
 // partial[0..3] = (en_b, en_ke, en_int, -max divb) per block over box 0..n
+// v = 2 v1 - v (:399-407) into vxo..vzo at the nodes 0..n that have an
+// index 0 (k_shock_corr does the rest): the plane k = 0 and, above it, the
+// faces i = 0 and j = 0.  One thread per node of the (n+1)^2 plane, then
+// one per node of the two faces, flattened.
+__global__ __launch_bounds__(256) void k_vupdate_low(Geo g, const real *__restrict__ vx, const real *__restrict__ vy,
+                                                     const real *__restrict__ vz, const real *__restrict__ vx1,
+                                                     const real *__restrict__ vy1, const real *__restrict__ vz1,
+                                                     real *__restrict__ vxo, real *__restrict__ vyo,
+                                                     real *__restrict__ vzo) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long px = g.nx + 1, py = g.ny + 1;
+  const long plane = px * py;                 // k = 0
+  const long face = (long)g.nz * (px + py - 1);  // k = 1..n: i = 0 or j = 0
+  if (t >= plane + face) return;
+  int i, j, k;
+  if (t < plane) {
+    k = 0;
+    j = (int)(t / px);
+    i = (int)(t - (long)j * px);
+  } else {
+    const long u = t - plane;
+    const long row = px + py - 1;  // (0, j) for j = 0..ny, then (i, 0) for i = 1..nx
+    k = 1 + (int)(u / row);
+    const int r = (int)(u - (long)(k - 1) * row);
+    if (r < py) {
+      i = 0;
+      j = r;
+    } else {
+      i = r - (int)py + 1;
+      j = 0;
+    }
+  }
+  const int c = g.idx(i, j, k);
+  vxo[c] = (real)(2.0 * vx1[c] - vx[c]);
+  vyo[c] = (real)(2.0 * vy1[c] - vy[c]);
+  vzo[c] = (real)(2.0 * vz1[c] - vz[c]);
+}
+
 __global__ __launch_bounds__(BX * BY) void k_energy(Geo g, const real *__restrict__ rho, const real *__restrict__ energy,
                          const real *__restrict__ vx, const real *__restrict__ vy, const real *__restrict__ vz,
                          const double *__restrict__ bx, const double *__restrict__ by,

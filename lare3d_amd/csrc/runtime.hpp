@@ -732,10 +732,22 @@ int lagrangian_step() {
   }
   dim3 g3 = zchunk_grid(interior());
   LAUNCH("shock_corr", k_shock_corr, g3, box_block(), S.g, interior(), S.vx, S.vy, S.vz, S.vx1, S.vy1, S.vz1, S.al1,
-         S.al2, S.al3, S.pp, S.energy, S.rho, S.visc_dep, S.partial, S.dt);
+         S.al2, S.al3, S.pp, S.energy, S.rho, S.visc_dep, S.Fx, S.Fy, S.Fz, S.partial, S.dt);
   rc = final_reduce(g3, 1, 0, 8, 1);
   if (rc) return rc;
-  LAUNCH_BOX("vupdate", k_vupdate, nodes0n(), S.vx, S.vy, S.vz, S.vx1, S.vy1, S.vz1);
+  {
+    const long nlow = (long)(nx() + 1) * (ny() + 1) + (long)nz() * (nx() + ny() + 1);
+    LAUNCH("vupdate_low", k_vupdate_low, dim3((unsigned)((nlow + 255) / 256)), dim3(256), S.g, S.vx, S.vy, S.vz,
+           S.vx1, S.vy1, S.vz1, S.Fx, S.Fy, S.Fz);
+  }
+  // the new velocities went to the momentum-flux arrays, which swap roles
+  // with the velocity arrays; the boundary fill below sets their ghosts
+  {
+    real *tv;
+    tv = S.vx; S.vx = S.Fx; S.Fx = tv;
+    tv = S.vy; S.vy = S.Fy; S.Fy = tv;
+    tv = S.vz; S.vz = S.Fz; S.Fz = tv;
+  }
   {
     BcField f[5] = {{S.energy, FK_CELL}, {S.rho, FK_CELL}, {S.vx, FK_NODE}, {S.vy, FK_NODE}, {S.vz, FK_NODE}};
     rc = apply_bcs(f, 5, "bc_lagrangian");
