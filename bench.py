@@ -41,7 +41,7 @@ KERNEL_BYTES_PER_POINT = {
     "visc_dt": 8 * (3 + 2 + 3 + 1 + 3 + 2),         # v x3, rho, e, B x3, eta, alpha x3 -> visc_heat, p0
     "vertex_prep": 8 * (5 + 2),                     # rho, e, B x3 -> rho_v, cs_v
     "force": 8 * (3 + 1 + 3 + 1 + 3 + 3),           # v x3, pp, Bpred x3, rho_v, alpha x3 -> v1 x3
-    "shock_corr": 8 * (6 + 3 + 1 + 3 + 3),          # v, v1, alpha, pp, e, rho, visc_dep (rw)
+    "shock_corr": 8 * (6 + 3 + 1 + 3 + 3 + 3),      # v, v1, alpha, pp, e, rho, visc_dep (rw) -> v_new x3
     "mass_flux": 8 * (1 + 1 + 1 + 1),               # rho, v_sweep, db1 -> dm
     "rho_energy": 8 * (7 + 2),                      # e, rho1, v_sweep, db1, dm, cv1, cv2 -> rho_new, e_new
     "ct_update": 8 * (2 + 2 + 2 + 2),               # F1, F2, t1 (rw), t2 (rw), a (rw)
