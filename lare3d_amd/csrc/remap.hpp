@@ -408,100 +408,94 @@ __global__ __launch_bounds__(BX * BY) void k_mass_flux(Geo g, Box b, const real 
   dm[c] = (fu + Di * (1.0 - phi)) * v_advect * dt * area;
 }
 
-// The y- and z-sweep mass flux (M = 1, 2), marching along the sweep axis
-// with rho and db1 of points m-1..m+2 in registers (sweep range 0..n, so no
-// clamping is needed).
+// Mass flux (x_mass_flux, xremap.f90:474-533), energy flux (x_energy_flux,
+// :539-601) and the density and energy updates (:174-196) of the y and z
+// sweeps (M = 1, 2) in one pass, marching along the sweep axis.  The launch
+// box is the mass flux's: sweep 0..n, transverse 0..n+1.  Every column
+// forms dm on its points from rho1 and db1 of m-1..m+2 in registers and
+// stores it.  The columns with transverse 1..n also form the energy flux
+// from the same window (energy of m-1..m+2) and update rho and energy at
+// 1..n, the flux and dm at m-1 carried from the previous point (rounded to
+// the storage type, as stored values are).  A chunk that starts at m0 > 0
+// first forms the fluxes of m0-1, whose dm the chunk below stores.  On a
+// periodic sweep axis the dm halo fill that follows overwrites dm(0) with
+// dm(n), and the energy flux at 0 and the update at 1 take dm(0) after that
+// fill, so there the first chunk forms dm(n) from its own window at n and
+// uses it at 0.  The new density goes to rho_new and the new energy to
+// e_new, since neighbouring chunks still read the old energy; only cells
+// 1..n are written (the halo refresh after the sweep fills the rest).
 template <int M>
-__global__ __launch_bounds__(BX * BY) void k_mass_flux_m(Geo g, Box b, const real *__restrict__ rho,
-                                                         const real *__restrict__ vel, const real *__restrict__ db1,
-                                                         real *__restrict__ dm, double dt) {
+__global__ __launch_bounds__(BX * BY) void k_mass_rho_energy_m(Geo g, Box b, const real *__restrict__ energy,
+                                                               const real *__restrict__ rho1,
+                                                               const real *__restrict__ vel,
+                                                               const real *__restrict__ db1, real *__restrict__ dm,
+                                                               const real *__restrict__ cv1,
+                                                               const real *__restrict__ cv2,
+                                                               real *__restrict__ rho_new, real *__restrict__ e_new,
+                                                               double dt, int periodic) {
   MCOLUMN(M, b, i, p, m0, m1)
   const int MS = M == 2 ? g.sz : g.sy;
-  real r[4], d[4];
-  const int cb = g.idx(i, MJ(M, m0 - 1, p), MK(M, m0 - 1, p));
+  const int nP = g.n(M == 2 ? 1 : 2);
+  const bool inner = i >= 1 && i <= g.nx && p >= 1 && p <= nP;
+  const int ms = m0 > 0 ? m0 - 1 : m0;  // first point whose fluxes are formed
+  // dm(n) for the wrapped dm(0) of a periodic axis (first chunk only)
+  real dm_wrap = 0;
+  if (periodic && ms == 0 && inner) {
+    const int nM = g.n(M);
+    const int cn = g.idx(i, MJ(M, nM, p), MK(M, nM, p));
+    real rw[4], dw[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      rw[q] = rho1[cn + (q - 1) * MS];
+      dw[q] = db1[cn + (q - 1) * MS];
+    }
+    const double area = face_area<M>(g, i, MJ(M, nM, p), MK(M, nM, p));
+    const double v_advect = face_velocity<M>(g, vel, cn);
+    double vad_p, vad_m, dbu, fu, Di, phi;
+    vl_cell(v_advect, rw[0], rw[1], rw[2], rw[3], 0.5 * (dw[0] + dw[1]), 0.5 * (dw[1] + dw[2]),
+            0.5 * (dw[2] + dw[3]), dw[1], dw[2], dt, vad_p, vad_m, dbu, fu, Di, phi);
+    dm_wrap = (fu + Di * (1.0 - phi)) * v_advect * dt * area;
+  }
+  // window at ms: rho1, db1 (and energy in inner columns) at ms-1..ms+2
+  real r[4], d[4], e[4] = {0, 0, 0, 0};
+  int c = g.idx(i, MJ(M, ms, p), MK(M, ms, p));
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    r[q] = rho[cb + q * MS];
-    d[q] = db1[cb + q * MS];
+    r[q] = rho1[c + (q - 1) * MS];
+    d[q] = db1[c + (q - 1) * MS];
+    if (inner) e[q] = energy[c + (q - 1) * MS];
   }
-  for (int m = m0; m <= m1; ++m) {
-    const int j = MJ(M, m, p), k = MK(M, m, p);
-    const int c = g.idx(i, j, k);
-    const double area = face_area<M>(g, i, j, k);
-    const double v_advect = face_velocity<M>(g, vel, c);
-    double vad_p, vad_m, dbu, fu, Di, phi;
-    vl_cell(v_advect, r[0], r[1], r[2], r[3], 0.5 * (d[0] + d[1]), 0.5 * (d[1] + d[2]), 0.5 * (d[2] + d[3]), d[1],
-            d[2], dt, vad_p, vad_m, dbu, fu, Di, phi);
-    dm[c] = (fu + Di * (1.0 - phi)) * v_advect * dt * area;
-    if (m < m1) {
+  real Fp = 0, dp = 0;
+  for (int m = ms; m <= m1; ++m) {
+    if (m > ms) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         r[q] = r[q + 1];
         d[q] = d[q + 1];
-      }
-      r[3] = rho[c + 3 * MS];
-      d[3] = db1[c + 3 * MS];
-    }
-  }
-}
-
-// Energy flux (x_energy_flux, xremap.f90:539-601) and the density and energy
-// updates (xremap.f90:174-196, cells 1..n) of the y and z sweeps (M = 1, 2)
-// in one pass, marching along the sweep axis with energy and db1 of points
-// m-1..m+2 and rho1 of m, m+1 in registers.  The flux at s-1 that the update
-// at s needs is the previous iteration's (rounded to the storage type, as a
-// stored flux is).  The new density goes to rho_new and the new energy to
-// e_new (neighbouring chunks still read the old energy); only cells 1..n are
-// written, as the separate update wrote only those (the halo refresh after
-// the sweep fills the rest).
-template <int M>
-__global__ __launch_bounds__(BX * BY) void k_rho_energy_m(Geo g, Box b, const real *__restrict__ energy,
-                                                          const real *__restrict__ rho1,
-                                                          const real *__restrict__ vel,
-                                                          const real *__restrict__ db1,
-                                                          const real *__restrict__ dm,
-                                                          const real *__restrict__ cv1,
-                                                          const real *__restrict__ cv2, real *__restrict__ rho_new,
-                                                          real *__restrict__ e_new, double dt) {
-  MCOLUMN(M, b, i, p, m0, m1)
-  const int MS = M == 2 ? g.sz : g.sy;
-  // window at point m0-1 (>= 0): energy and db1 at m0-2..m0+1, rho1 at m0-1, m0
-  real e[4], d[4], r0, r1;
-  int c = g.idx(i, MJ(M, m0 - 1, p), MK(M, m0 - 1, p));
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    e[q] = energy[c + (q - 1) * MS];
-    d[q] = db1[c + (q - 1) * MS];
-  }
-  r0 = rho1[c];
-  r1 = rho1[c + MS];
-  real Fp, dp;
-  for (int m = m0 - 1; m <= m1; ++m) {
-    if (m > m0 - 1) {
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
         e[q] = e[q + 1];
-        d[q] = d[q + 1];
       }
-      e[3] = energy[c + 3 * MS];
+      r[3] = rho1[c + 3 * MS];
       d[3] = db1[c + 3 * MS];
-      r0 = r1;
-      r1 = rho1[c + 2 * MS];
+      if (inner) e[3] = energy[c + 3 * MS];
       c += MS;
     }
     const int j = MJ(M, m, p), k = MK(M, m, p);
     const double area = face_area<M>(g, i, j, k);
     const double v_advect = face_velocity<M>(g, vel, c);
+    const double dcm = 0.5 * (d[0] + d[1]), dci = 0.5 * (d[1] + d[2]), dcp = 0.5 * (d[2] + d[3]);
     double vad_p, vad_m, dbu, fu, Di, phi;
-    vl_cell(v_advect, e[0], e[1], e[2], e[3], 0.5 * (d[0] + d[1]), 0.5 * (d[1] + d[2]), 0.5 * (d[2] + d[3]), d[1],
-            d[2], dt, vad_p, vad_m, dbu, fu, Di, phi);
-    const double rhou = r0 * vad_p + r1 * vad_m;
-    const real dmc = dm[c];
+    vl_cell(v_advect, r[0], r[1], r[2], r[3], dcm, dci, dcp, d[1], d[2], dt, vad_p, vad_m, dbu, fu, Di, phi);
+    const real dm_own = (fu + Di * (1.0 - phi)) * v_advect * dt * area;
+    if (m >= m0) dm[c] = dm_own;
+    if (!inner) continue;
+    const real dmc = (periodic && m == 0) ? dm_wrap : dm_own;
+    vl_cell(v_advect, e[0], e[1], e[2], e[3], dcm, dci, dcp, d[1], d[2], dt, vad_p, vad_m, dbu, fu, Di, phi);
+    const double rhou = (double)r[1] * vad_p + (double)r[2] * vad_m;
     const double dd = dmc;
     const double dmu = fabs(dd) / area / dbu / rhou;
     const real F = (fu + Di * (1.0 - dmu)) * dd;
-    if (m >= m0) {
-      const double rr = r0;
+    if (m >= m0 && m >= 1) {
+      const double rr = r[1];
       const double rn = (rr * cv1[c] + dp - dmc) / cv2[c];
       rho_new[c] = rn;
       e_new[c] = (e[1] * cv1[c] * rr + Fp - F) / (cv2[c] * rn);

@@ -482,10 +482,14 @@ int sweep_full(double dt, double pass[3], bool first, bool last) {
   int hi[3] = {N[0] + 1, N[1] + 1, N[2] + 1};
   hi[AX] = N[AX];
   Box mb = mkbox(0, hi[0], 0, hi[1], 0, hi[2]);
-  // the y and z sweeps' flux kernels march along the sweep axis
-  if constexpr (AX == 0) LAUNCH_BOX("mass_flux", k_mass_flux<AX>, mb, S.rho, vel, S.db1, S.dm, dt);
-  else LAUNCH("mass_flux", k_mass_flux_m<AX>, zchunk_grid(march_box(mb, AX)), box_block(), S.g, mb, S.rho, vel,
-              S.db1, S.dm, dt);
+  // x sweep: the mass flux on its own; the y and z sweeps form it inside
+  // the density/energy kernel below, which marches along the sweep axis
+  if constexpr (AX == 0) {
+    LAUNCH_BOX("mass_flux", k_mass_flux<AX>, mb, S.rho, vel, S.db1, S.dm, dt);
+  } else {
+    LAUNCH("rho_energy", k_mass_rho_energy_m<AX>, zchunk_grid(march_box(mb, AX)), box_block(), S.g, mb, S.energy,
+           S.rho, vel, S.db1, S.dm, S.cv1, S.cv2, S.rho_alt, S.Fe, dt, S.per[AX] ? 1 : 0);
+  }
   {
     BcSet s = {};
     s.nf = 1;
@@ -495,9 +499,10 @@ int sweep_full(double dt, double pass[3], bool first, bool last) {
     rc = launch_bc_axis(s, AX, "bc_dm");
     if (rc) return rc;
   }
-  // energy flux and the density and energy updates in one pass (cells
-  // 1..n); the new energy lands in the energy flux array, which then swaps
-  // roles with the energy array (the new density goes to rho_alt as before)
+  // x sweep: the energy flux and the density and energy updates in one pass
+  // (cells 1..n).  In every sweep the new energy lands in the energy flux
+  // array, which then swaps roles with the energy array (the new density
+  // goes to rho_alt as before).
   {
     const Box in = interior();
     if constexpr (AX == 0) {
@@ -506,9 +511,6 @@ int sweep_full(double dt, double pass[3], bool first, bool last) {
       const dim3 gx((unsigned)((waves + BY - 1) / BY), 1, (unsigned)(in.z1 - in.z0 + 1));
       LAUNCH("rho_energy", k_rho_energy_x, gx, box_block(), S.g, in, S.energy, S.rho, vel, S.db1, S.dm, S.cv1,
              S.cv2, S.rho_alt, S.Fe, dt);
-    } else {
-      LAUNCH("rho_energy", k_rho_energy_m<AX>, zchunk_grid(march_box(in, AX)), box_block(), S.g, in, S.energy,
-             S.rho, vel, S.db1, S.dm, S.cv1, S.cv2, S.rho_alt, S.Fe, dt);
     }
     real *te = S.energy;
     S.energy = S.Fe;
