@@ -391,23 +391,6 @@ __device__ __forceinline__ void vl_cell(double v_advect, double fm, double fi, d
   Di = vl_di(dfi, dfu, sign_v, dci, dcu, dbu, phi);
 }
 
-// x_mass_flux (xremap.f90:474-533): sweep 0..n, transverse 0..n+1
-template <int AX>
-__global__ __launch_bounds__(BX * BY) void k_mass_flux(Geo g, Box b, const real *__restrict__ rho, const real *__restrict__ vel,
-                            const real *__restrict__ db1, real *__restrict__ dm, double dt) {
-  BOX_INDEX(b, i, j, k);
-  const int c = g.idx(i, j, k);
-  const int S = sstride<AX>(g);
-  const int s = coord<AX>(i, j, k);
-  const double area = face_area<AX>(g, i, j, k);
-  const double v_advect = face_velocity<AX>(g, vel, c);
-  double vad_p, vad_m, dbu, fu, Di, phi;
-  vl_cell(v_advect, rho[c - S], rho[c], rho[c + S], rho[c + 2 * S], dc1<AX>(g, db1, c - S, s - 1),
-          dc1<AX>(g, db1, c, s), dc1<AX>(g, db1, c + S, s + 1), db1[c], db1[c + S], dt, vad_p, vad_m, dbu, fu, Di,
-          phi);
-  dm[c] = (fu + Di * (1.0 - phi)) * v_advect * dt * area;
-}
-
 // Mass flux (x_mass_flux, xremap.f90:474-533), energy flux (x_energy_flux,
 // :539-601) and the density and energy updates (:174-196) of the y and z
 // sweeps (M = 1, 2) in one pass, marching along the sweep axis.  The launch
@@ -505,46 +488,70 @@ __global__ __launch_bounds__(BX * BY) void k_mass_rho_energy_m(Geo g, Box b, con
   }
 }
 
-// The same for the x sweep, the flux and dm at s-1 taken from the
-// neighbouring lane: waves cover runs of 64 points of a plane flattened over
-// x = x0-1..x1 (fluxes on 0..n), overlapping by one; lanes at x0-1 only
-// provide their flux.
+// The same for the x sweep (mass flux, energy flux and the density and
+// energy updates in one pass), the flux and dm at s-1 taken from the
+// neighbouring lane: waves cover runs of 64 points of a plane of the mass
+// flux's box (x 0..n, transverse 0..n+1) flattened over x, overlapping by
+// one; lanes 1..REX_OUT own their points.  Every point forms dm and its
+// owner stores it; points with transverse 1..n also form the energy flux,
+// and those at x >= 1 update rho and energy.  On a periodic x axis the
+// lanes at x = 0 form dm(n) as well and use it, as the dm halo fill would
+// have made them.
 #define REX_OUT 63
-__global__ __launch_bounds__(BX * BY) void k_rho_energy_x(Geo g, Box b, const real *__restrict__ energy,
-                                                          const real *__restrict__ rho1,
-                                                          const real *__restrict__ vel,
-                                                          const real *__restrict__ db1,
-                                                          const real *__restrict__ dm,
-                                                          const real *__restrict__ cv1,
-                                                          const real *__restrict__ cv2, real *__restrict__ rho_new,
-                                                          real *__restrict__ e_new, double dt) {
+__device__ __forceinline__ double mass_flux_x(const Geo &g, const real *__restrict__ rho,
+                                             const real *__restrict__ vel, const real *__restrict__ db1, int c,
+                                             int i, int j, int k, double dt) {
+  const double area = face_area<0>(g, i, j, k);
+  const double v_advect = face_velocity<0>(g, vel, c);
+  double vad_p, vad_m, dbu, fu, Di, phi;
+  vl_cell(v_advect, rho[c - 1], rho[c], rho[c + 1], rho[c + 2], dc1<0>(g, db1, c - 1, i - 1), dc1<0>(g, db1, c, i),
+          dc1<0>(g, db1, c + 1, i + 1), db1[c], db1[c + 1], dt, vad_p, vad_m, dbu, fu, Di, phi);
+  return (fu + Di * (1.0 - phi)) * v_advect * dt * area;
+}
+__global__ __launch_bounds__(BX * BY) void k_mass_rho_energy_x(Geo g, Box b, const real *__restrict__ energy,
+                                                               const real *__restrict__ rho1,
+                                                               const real *__restrict__ vel,
+                                                               const real *__restrict__ db1, real *__restrict__ dm,
+                                                               const real *__restrict__ cv1,
+                                                               const real *__restrict__ cv2,
+                                                               real *__restrict__ rho_new, real *__restrict__ e_new,
+                                                               double dt, int periodic) {
   const int lane = (int)threadIdx.x;  // BX = 64: threadIdx.y is the wave
-  const int wq = b.x1 - b.x0 + 2;     // x0-1 .. x1
+  const int wq = b.x1 - b.x0 + 1;
   const int npts = wq * (b.y1 - b.y0 + 1);
   const int f = (int)(xcd_tile(blockIdx.x, gridDim.x) * BY + threadIdx.y) * REX_OUT + lane - 1;
   const bool valid = f >= 0 && f < npts;
   const int ff = valid ? f : 0;  // lanes past the plane read point 0, unused
   const int jr = ff / wq;
-  const int i = b.x0 - 1 + (ff - jr * wq);
+  const int i = b.x0 + (ff - jr * wq);
   const int j = b.y0 + jr;
   const int k = b.z0 + (int)blockIdx.z;
   const int c = g.idx(i, j, k);
-  // the energy flux at this point
-  const double area = face_area<0>(g, i, j, k);
-  const double v_advect = face_velocity<0>(g, vel, c);
-  double vad_p, vad_m, dbu, fu, Di, phi;
-  vl_cell(v_advect, energy[c - 1], energy[c], energy[c + 1], energy[c + 2], dc1<0>(g, db1, c - 1, i - 1),
-          dc1<0>(g, db1, c, i), dc1<0>(g, db1, c + 1, i + 1), db1[c], db1[c + 1], dt, vad_p, vad_m, dbu, fu, Di,
-          phi);
-  const double rhou = rho1[c] * vad_p + rho1[c + 1] * vad_m;
-  const real dmc = dm[c];
-  const double d = dmc;
-  const double dmu = fabs(d) / area / dbu / rhou;
-  const real F = (fu + Di * (1.0 - dmu)) * d;
-  // lane l-1 holds point i-1 of the same row whenever i > x0-1
+  const real dm_own = mass_flux_x(g, rho1, vel, db1, c, i, j, k, dt);
+  if (valid && lane >= 1) dm[c] = dm_own;
+  const bool inner = j >= 1 && j <= g.ny && k >= 1 && k <= g.nz;
+  real dmc = dm_own, F = 0;
+  if (inner) {
+    if (periodic && i == 0) {
+      const int cn = g.idx(g.nx, j, k);
+      dmc = mass_flux_x(g, rho1, vel, db1, cn, g.nx, j, k, dt);
+    }
+    // the energy flux at this point
+    const double area = face_area<0>(g, i, j, k);
+    const double v_advect = face_velocity<0>(g, vel, c);
+    double vad_p, vad_m, dbu, fu, Di, phi;
+    vl_cell(v_advect, energy[c - 1], energy[c], energy[c + 1], energy[c + 2], dc1<0>(g, db1, c - 1, i - 1),
+            dc1<0>(g, db1, c, i), dc1<0>(g, db1, c + 1, i + 1), db1[c], db1[c + 1], dt, vad_p, vad_m, dbu, fu, Di,
+            phi);
+    const double rhou = rho1[c] * vad_p + rho1[c + 1] * vad_m;
+    const double d = dmc;
+    const double dmu = fabs(d) / area / dbu / rhou;
+    F = (fu + Di * (1.0 - dmu)) * d;
+  }
+  // lane l-1 holds point i-1 of the same row whenever i > x0
   const real Fm = __shfl_up(F, 1, 64);
   const real dmm = __shfl_up(dmc, 1, 64);
-  if (!valid || lane < 1 || i < b.x0) return;
+  if (!valid || lane < 1 || !inner || i < 1) return;
   const double r1 = rho1[c];
   const double rn = (r1 * cv1[c] + dmm - dmc) / cv2[c];
   rho_new[c] = rn;

@@ -482,10 +482,16 @@ int sweep_full(double dt, double pass[3], bool first, bool last) {
   int hi[3] = {N[0] + 1, N[1] + 1, N[2] + 1};
   hi[AX] = N[AX];
   Box mb = mkbox(0, hi[0], 0, hi[1], 0, hi[2]);
-  // x sweep: the mass flux on its own; the y and z sweeps form it inside
-  // the density/energy kernel below, which marches along the sweep axis
+  // mass flux, energy flux and the density and energy updates in one pass
+  // over the mass flux's box (updates on cells 1..n); the new energy lands
+  // in the energy flux array, which then swaps roles with the energy array
+  // (the new density goes to rho_alt as before)
   if constexpr (AX == 0) {
-    LAUNCH_BOX("mass_flux", k_mass_flux<AX>, mb, S.rho, vel, S.db1, S.dm, dt);
+    const long npts = (long)(mb.x1 - mb.x0 + 1) * (mb.y1 - mb.y0 + 1);
+    const long waves = (npts + REX_OUT - 1) / REX_OUT;
+    const dim3 gx((unsigned)((waves + BY - 1) / BY), 1, (unsigned)(mb.z1 - mb.z0 + 1));
+    LAUNCH("rho_energy", k_mass_rho_energy_x, gx, box_block(), S.g, mb, S.energy, S.rho, vel, S.db1, S.dm, S.cv1,
+           S.cv2, S.rho_alt, S.Fe, dt, S.per[AX] ? 1 : 0);
   } else {
     LAUNCH("rho_energy", k_mass_rho_energy_m<AX>, zchunk_grid(march_box(mb, AX)), box_block(), S.g, mb, S.energy,
            S.rho, vel, S.db1, S.dm, S.cv1, S.cv2, S.rho_alt, S.Fe, dt, S.per[AX] ? 1 : 0);
@@ -499,19 +505,7 @@ int sweep_full(double dt, double pass[3], bool first, bool last) {
     rc = launch_bc_axis(s, AX, "bc_dm");
     if (rc) return rc;
   }
-  // x sweep: the energy flux and the density and energy updates in one pass
-  // (cells 1..n).  In every sweep the new energy lands in the energy flux
-  // array, which then swaps roles with the energy array (the new density
-  // goes to rho_alt as before).
   {
-    const Box in = interior();
-    if constexpr (AX == 0) {
-      const long npts = (long)(in.x1 - in.x0 + 2) * (in.y1 - in.y0 + 1);
-      const long waves = (npts + REX_OUT - 1) / REX_OUT;
-      const dim3 gx((unsigned)((waves + BY - 1) / BY), 1, (unsigned)(in.z1 - in.z0 + 1));
-      LAUNCH("rho_energy", k_rho_energy_x, gx, box_block(), S.g, in, S.energy, S.rho, vel, S.db1, S.dm, S.cv1,
-             S.cv2, S.rho_alt, S.Fe, dt);
-    }
     real *te = S.energy;
     S.energy = S.Fe;
     S.Fe = te;
