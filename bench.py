@@ -42,8 +42,7 @@ KERNEL_BYTES_PER_POINT = {
     "vertex_prep": 8 * (5 + 2),                     # rho, e, B x3 -> rho_v, cs_v
     "force": 8 * (3 + 1 + 3 + 1 + 3 + 3),           # v x3, pp, Bpred x3, rho_v, alpha x3 -> v1 x3
     "shock_corr": 8 * (6 + 3 + 1 + 3 + 3 + 3),      # v, v1, alpha, pp, e, rho, visc_dep (rw) -> v_new x3
-    "mass_flux": 8 * (1 + 1 + 1 + 1),               # rho, v_sweep, db1 -> dm
-    "rho_energy": 8 * (7 + 2),                      # e, rho1, v_sweep, db1, dm, cv1, cv2 -> rho_new, e_new
+    "rho_energy": 8 * (6 + 3),                      # e, rho1, v_sweep, db1, cv1, cv2 -> dm, rho_new, e_new
     "ct_update": 8 * (2 + 2 + 2 + 2),               # F1, F2, t1 (rw), t2 (rw), a (rw)
     "rkl2_stage": 8 * (9 + 1),                      # Y2,Y1,E0,rho,B x3,larsen,Lc -> Y3
 }
@@ -57,8 +56,7 @@ def box_points(name, n):
         "mom_remap": (n + 1) ** 3, "ct_flux": (n + 3) ** 3, "geom": (n + 4) ** 3,
         "vertex": (n + 3) * (n + 1) ** 2, "alpha": (n + 3) * (n + 4) * (n + 3),
         "visc_dt": (n + 2) ** 3, "vertex_prep": (n + 3) ** 3, "force": (n + 1) ** 3,
-        "shock_corr": n ** 3, "mass_flux": (n + 1) * (n + 2) ** 2,
-        "rho_energy": n ** 3, "ct_update": (n + 2) ** 3,
+        "shock_corr": n ** 3, "rho_energy": (n + 1) * (n + 2) ** 2, "ct_update": (n + 2) ** 3,
         "rkl2_stage": n ** 3,
     }
     return sizes.get(name)
