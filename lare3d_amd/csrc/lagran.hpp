@@ -8,8 +8,8 @@
 //   k_visc_dt         p_visc + visc_heat over cells 0..n+1 (:680-737), the
 //                     start-of-step pressure, and the CFL minimum of set_dt
 //                     (:1094-1143) fused in the same pass
-//   k_resist / k_ohmic  resistive sub-cycle (:1176-1635): eta_calc, rkstep
-//                     and bstep in one pass, then the ohmic heating
+//   k_resist          resistive sub-cycle (:1176-1635): eta_calc, rkstep,
+//                     bstep and the ohmic heating in one pass
 //   k_eta             eta_calc after the remap (lare3d.f90:87)
 //   k_pred_pressure   predictor energy/pressure over 0..n+1 (:234-249)
 //   k_force           f_visc (:745-789) + grad p + J x B (half-step B)
@@ -419,12 +419,15 @@ struct EPlane {
   double kx0, kxm, ky0, kym, kz0, kzm, kzx;
 };
 
-// One resistive substep's update of B in a single pass: at every node of
-// the box -1..n+1 the vertex current and eta of the substep's eta_calc
+// One resistive substep in a single pass: at every node of the box
+// -1..n+1 the vertex current and eta of the substep's eta_calc
 // (lagran.F90:1176-1230), at nodes 0..n the rkstep EMFs and curlb
 // (:1472-1503) with the total_ohmic_heating increments (:1309-1328), and
 // bstep (:1582-1631), which takes the EMFs of the node and of its (i-1),
-// (j-1) neighbours in planes k and k-1.  A block is a tile of RS_ROWS rows of
+// (j-1) neighbours in planes k and k-1.  The ohmic heating of cell (i,j,k)
+// (:1270-1288, cells 1..n) takes curlb of the same eight nodes, rounded to
+// the storage type as stored values are, and energy = MAX(energy, 0)
+// (:1305) follows over the cell box -1..n+2.  A block is a tile of RS_ROWS rows of
 // 64 nodes marching in z over ZCHUNK planes (plus plane k0-1 for the EMFs of
 // plane k-1); lane 0 and row 0 only supply neighbour EMFs, which pass through
 // LDS, so a tile writes RS_OUT_X x RS_OUT_Y nodes.  Neighbouring tiles still
@@ -437,9 +440,11 @@ __global__ __launch_bounds__(64 * RS_ROWS) void k_resist(Geo g, Box b, int tiles
                                                          const double *__restrict__ by, const double *__restrict__ bz,
                                                          double *__restrict__ bxo, double *__restrict__ byo,
                                                          double *__restrict__ bzo, real *__restrict__ eta,
-                                                         real *__restrict__ curlb, double *__restrict__ partial,
+                                                         real *__restrict__ energy, real *__restrict__ ohmic_dep,
+                                                         const real *__restrict__ rho, double *__restrict__ partial,
                                                          double dt, double eta_bg, double eta0, double j_max) {
-  __shared__ double sk[2][3][RS_ROWS][64];  // EMFs kx, ky, kz of the tile's nodes, by plane parity
+  // EMFs kx, ky, kz and curlb of the tile's nodes, by plane parity
+  __shared__ double sk[2][4][RS_ROWS][64];
   __shared__ double red[RS_ROWS];
   const int l = (int)threadIdx.x, w = (int)threadIdx.y;
   const int t = (int)xcd_tile(blockIdx.x, gridDim.x);
@@ -451,13 +456,17 @@ __global__ __launch_bounds__(64 * RS_ROWS) void k_resist(Geo g, Box b, int tiles
   const int Y = g.sy, Z = g.sz;
   const bool eij = i >= -1 && i <= g.nx + 1 && j >= -1 && j <= g.ny + 1;  // eta box, in the plane
   const bool kij = i >= 0 && i <= g.nx && j >= 0 && j <= g.ny;            // EMF box, in the plane
+  // cell box of the energy clamp, in the plane
+  const bool cij = i >= -1 && i <= g.nx + 2 && j >= -1 && j <= g.ny + 2;
+  const bool hij = i >= 1 && i <= g.nx && j >= 1 && j <= g.ny;  // heated cells, in the plane
   double v = 0.0;
   EPlane m = {0, 0, 0, 0, 0, 0, 0};  // plane k-1
+  double mq[4] = {0, 0, 0, 0};       // curlb of plane k-1 at (i,j), (i-1,j), (i,j-1), (i-1,j-1)
   BPlane lo = {0, 0, 0, 0};
   bool have_lo = false;
   for (int k = k0 - 1; k <= k1; ++k) {
     const int c = g.idx(i, j, k);
-    double kx = 0.0, ky = 0.0, kz = 0.0;
+    double kx = 0.0, ky = 0.0, kz = 0.0, cbr = 0.0;
     if (eij && k >= -1 && k <= g.nz + 1) {
       if (!have_lo) lo = bplane(bx, by, c, Y);
       const BPlane hi = bplane(bx, by, c + Z, Y);
@@ -473,9 +482,9 @@ __global__ __launch_bounds__(64 * RS_ROWS) void k_resist(Geo g, Box b, int tiles
         kx = -jx * e * g.dxc[i] * 0.5;
         ky = -jy * e * g.dyc[j] * 0.5;
         kz = -jz * e * g.dzc[k] * 0.5;
+        const double cb = e * (jx * jx + jy * jy + jz * jz);
+        cbr = (real)cb;
         if (out && k >= k0) {
-          const double cb = e * (jx * jx + jy * jy + jz * jz);
-          curlb[c] = cb;
           double w1 = dt * g.dxc[i] * g.dyc[j] * g.dzc[k] * cb;
           if (i == 0 || i == g.nx) w1 = w1 * 0.5;
           if (j == 0 || j == g.ny) w1 = w1 * 0.5;
@@ -484,10 +493,11 @@ __global__ __launch_bounds__(64 * RS_ROWS) void k_resist(Geo g, Box b, int tiles
         }
       }
     }
-    double(&s)[3][RS_ROWS][64] = sk[k & 1];
+    double(&s)[4][RS_ROWS][64] = sk[k & 1];
     s[0][w][l] = kx;
     s[1][w][l] = ky;
     s[2][w][l] = kz;
+    s[3][w][l] = cbr;
     __syncthreads();
     if (out) {
       // EMFs at (i,j), (i-1,j), (i,j-1) of this plane, as EPlane names them
@@ -513,6 +523,20 @@ __global__ __launch_bounds__(64 * RS_ROWS) void k_resist(Geo g, Box b, int tiles
         byo[c] = nby;
         bzo[c] = nbz;
       }
+      const double pq[4] = {cbr, s[3][w][l - 1], s[3][w - 1][l], s[3][w - 1][l - 1]};
+      if (k >= k0 && cij && k >= -1 && k <= g.nz + 2) {
+        double e = energy[c];
+        if (hij && k >= 1 && k <= g.nz) {
+          // curlb(c) + (c-X) + (c-Y) + (c-X-Y) + (c-Z) + (c-X-Z) + (c-Y-Z) + (c-X-Y-Z)
+          const double sq = pq[0] + pq[1] + pq[2] + pq[3] + mq[0] + mq[1] + mq[2] + mq[3];
+          const double lh = sq * dt / (8.0 * rho[c]);
+          ohmic_dep[c] = ohmic_dep[c] + lh;
+          e = e + lh;
+        }
+        energy[c] = max0(e);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) mq[q] = pq[q];
       m = p;
     }
   }
@@ -523,31 +547,6 @@ __global__ __launch_bounds__(64 * RS_ROWS) void k_resist(Geo g, Box b, int tiles
     double r = red[0];
     for (int q = 1; q < RS_ROWS; ++q) r = r + red[q];
     partial[(size_t)blockIdx.z * gridDim.x + blockIdx.x] = r;
-  }
-}
-
-// Ohmic heating (lagran.F90:1270-1288) on cells 1..n, then
-// energy = MAX(energy, 0) over the whole cell box (:1305); marches in z with
-// the vertex heating of plane k-1 in registers.
-__global__ __launch_bounds__(BX * BY) void k_ohmic(Geo g, Box b, real *__restrict__ energy, real *__restrict__ ohmic_dep,
-                        const real *__restrict__ rho, const real *__restrict__ curlb, double dt) {
-  ZCOLUMN(b, i, j, k0, k1);
-  const int Y = g.sy;
-  const bool inxy = i >= 1 && i <= g.nx && j >= 1 && j <= g.ny;
-  Quad m = quad_back(curlb, g.idx(i, j, k0 - 1), Y);
-  for (int k = k0; k <= k1; ++k) {
-    const int c = g.idx(i, j, k);
-    const Quad p = quad_back(curlb, c, Y);
-    double e = energy[c];
-    if (inxy && k >= 1 && k <= g.nz) {
-      // curlb(c) + (c-X) + (c-Y) + (c-X-Y) + (c-Z) + (c-X-Z) + (c-Y-Z) + (c-X-Y-Z)
-      const double s = (double)p.p0 + p.px + p.py + p.pxy + m.p0 + m.px + m.py + m.pxy;
-      const double lh = s * dt / (8.0 * rho[c]);
-      ohmic_dep[c] = ohmic_dep[c] + lh;
-      e = e + lh;
-    }
-    energy[c] = max0(e);
-    m = p;
   }
 }
 

@@ -48,7 +48,7 @@ struct Dev {
   real *rho_v, *cs_v, *al1, *al2, *al3, *visc_heat, *p0, *pp;
   double *bpx, *bpy, *bpz;       // predictor B
   double *fxr, *fyr, *fzr;       // resistive EMFs (CT update of B)
-  real *curlb, *qx, *qy, *qz;
+  real *qx, *qy, *qz;
   // remap scratch: F is the CT face flux (FP64), Fe the energy flux
   real *cv1, *cv2, *db1, *Fe, *dm, *Fx, *Fy, *Fz;
   double *F, *F2;
@@ -666,10 +666,10 @@ int lagrangian_step() {
     S.last_substeps = substeps;
     const double actual_dt = S.dt;
     S.dt = S.dt / (double)substeps;
-    // eta_calc, rkstep and bstep of each substep run in k_resist, which
-    // writes the new B to the fxr..fzr arrays; they then swap roles with
-    // bx..bz.  The B and energy halos of a substep are refreshed together
-    // (ohmic does not read B).
+    // eta_calc, rkstep, bstep and the ohmic heating of each substep run in
+    // k_resist, which writes the new B to the fxr..fzr arrays; they then
+    // swap roles with bx..bz.  The B and energy halos of a substep are
+    // refreshed together.
     const Box fb = full_box();
     int tiles_x;
     const dim3 g2 = resist_grid(&tiles_x);
@@ -677,15 +677,13 @@ int lagrangian_step() {
       return fail(LARE_E_ARG, "resistive grid of %u blocks exceeds the partial buffer", g2.x * g2.z);
     for (int sc = 1; sc <= substeps; ++sc) {
       LAUNCH("resist", k_resist, g2, dim3(64, RS_ROWS, 1), S.g, fb, tiles_x, S.bx, S.by, S.bz, S.fxr, S.fyr, S.fzr,
-             S.eta, S.curlb, S.partial, S.dt, S.p.eta_background, S.p.eta0, S.p.j_max);
+             S.eta, S.energy, S.ohmic_dep, S.rho, S.partial, S.dt, S.p.eta_background, S.p.eta0, S.p.j_max);
       rc = final_reduce(g2, 1, 0, 9, 1);
       if (rc) return rc;
       double *tb;
       tb = S.bx; S.bx = S.fxr; S.fxr = tb;
       tb = S.by; S.by = S.fyr; S.fyr = tb;
       tb = S.bz; S.bz = S.fzr; S.fzr = tb;
-      LAUNCH("ohmic", k_ohmic, zchunk_grid(cell_box()), box_block(), S.g, cell_box(), S.energy, S.ohmic_dep, S.rho,
-             S.curlb, S.dt);
       BcField f[4] = {{S.bx, FK_BX}, {S.by, FK_BY}, {S.bz, FK_BZ}, {S.energy, FK_CELL}};
       rc = apply_bcs(f, 4, "bc_resistive");
       if (rc) return rc;
@@ -884,7 +882,7 @@ int api_init(const lare_grid *grid, const lare_params *params) {
   }
   real **bulk[] = {&S.rho, &S.rho_alt, &S.energy, &S.vx, &S.vy, &S.vz, &S.vx1, &S.vy1, &S.vz1, &S.eta,
                     &S.visc3, &S.visc_dep, &S.ohmic_dep, &S.rho_v, &S.cs_v, &S.al1, &S.al2, &S.al3,
-                    &S.visc_heat, &S.p0, &S.pp, &S.curlb, &S.cv1, &S.cv2, &S.db1, &S.Fe, &S.dm, &S.Fx, &S.Fy,
+                    &S.visc_heat, &S.p0, &S.pp, &S.cv1, &S.cv2, &S.db1, &S.Fe, &S.dm, &S.Fx, &S.Fy,
                     &S.Fz, &S.larsen, &S.Lc, &S.Yb[0], &S.Yb[1], &S.Yb[2]};
   for (real **pp : bulk) {
     int rc = dalloc(pp);
