@@ -11,7 +11,8 @@
 //   k_resist          resistive sub-cycle (:1176-1635): eta_calc, rkstep,
 //                     bstep and the ohmic heating in one pass
 //   k_eta             eta_calc after the remap (lare3d.f90:87)
-//   k_pred_pressure   predictor energy/pressure over 0..n+1 (:234-249)
+//   predictor energy/pressure (:234-249): in the predictor remap's last
+//                     geometry pass (k_geom<AX, 1>, remap.hpp)
 //   k_force           f_visc (:745-789) + grad p + J x B (half-step B)
 //                     + gravity -> v1 at nodes 0..n (:251-387)
 //   k_shock_corr      shock_heating (:861-937) fused with the corrector
@@ -554,21 +555,6 @@ __global__ __launch_bounds__(BX * BY) void k_energy_max(Geo g, Box b, real *__re
   BOX_INDEX(b, i, j, k);
   const int c = g.idx(i, j, k);
   energy[c] = max0(energy[c]);
-}
-
-// predictor energy and pressure (lagran.F90:234-249) over cells 0..n+1;
-// p0 is the start-of-step pressure (:93), cv1 the predictor-remap volume
-__global__ __launch_bounds__(BX * BY) void k_pred_pressure(Geo g, Box b, const real *__restrict__ energy, const real *__restrict__ rho,
-                                const real *__restrict__ cv1, const real *__restrict__ visc_heat,
-                                const real *__restrict__ p0, real *__restrict__ pp, double gamma, double dt2) {
-  BOX_INDEX(b, i, j, k);
-  const int c = g.idx(i, j, k);
-  const double cv = g.cv(i, j, k);
-  const double r = rho[c];
-  const double dv = cv1[c] / cv - 1.0;
-  double e1 = energy[c] - p0[c] * dv / r;
-  e1 = e1 + visc_heat[c] * dt2 / r;
-  pp[c] = (e1 - 0.0) * (gamma - 1.0) * r * cv / cv1[c];
 }
 
 // predictor momentum (lagran.F90:251-387) with fx/fy/fz_visc (:745-789)

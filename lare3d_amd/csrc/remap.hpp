@@ -37,15 +37,25 @@ __device__ __forceinline__ double vl_di(double dfi, double dfu, double sign_v, d
   return sign_v * ss * dmin(dmin(fabs(Da) * dbu, fabs(dfi)), fabs(dfu));
 }
 
+// Inputs of the predictor energy and pressure (lagran.F90:234-249), which
+// the predictor remap's last geometry pass (CV = 1) forms on cells 0..n+1
+// from the cv1 it has just formed (p0 is the start-of-step pressure).
+struct PredP {
+  const real *energy, *rho, *visc_heat, *p0;
+  real *pp;
+  double gamma, dt2;
+};
+
 // Lagrangian geometry of a sweep (xremap.f90:57-105 and analogues).
 // Velocities are the time-centred ones (v in predictor mode).  The node
 // velocities of planes k and k-1 are kept in registers while marching in z.
 // CV selects the outputs: 0 = d?b1 only (which needs only the sweep
-// component of the velocity), 1 = cv1 and d?b1, 2 = cv1, cv2 and d?b1.
+// component of the velocity), 1 = cv1 and d?b1 (and the predictor pressure
+// from pr), 2 = cv1, cv2 and d?b1.
 template <int AX, int CV>
 __global__ __launch_bounds__(BX * BY) void k_geom(Geo g, Box b, const real *__restrict__ ux, const real *__restrict__ uy,
                        const real *__restrict__ uz, real *__restrict__ cv1, real *__restrict__ cv2,
-                       real *__restrict__ db1, double dt, double xpass, double ypass, double zpass) {
+                       real *__restrict__ db1, double dt, double xpass, double ypass, double zpass, PredP pr) {
   ZCOLUMN(b, i, j, k0, k1);
   constexpr bool NX = CV > 0 || AX == 0, NY = CV > 0 || AX == 1, NZ = CV > 0 || AX == 2;
   const int Y = g.sy;
@@ -90,8 +100,17 @@ __global__ __launch_bounds__(BX * BY) void k_geom(Geo g, Box b, const real *__re
         dv1 = (tA + tB + (vzb - vzbm) / g.dzb[k]) * dt;
         dv2 = (tA + tB) * dt;
       }
-      cv1[c] = cv * (1.0 + dv1);
+      const real c1 = cv * (1.0 + dv1);
+      cv1[c] = c1;
       if (CV > 1) cv2[c] = cv * (1.0 + dv2);
+      if (CV == 1 && i >= 0 && i <= g.nx + 1 && j >= 0 && j <= g.ny + 1 && k >= 0 && k <= g.nz + 1) {
+        // predictor energy and pressure (lagran.F90:234-249)
+        const double r = pr.rho[c];
+        const double dv = c1 / cv - 1.0;
+        double e1 = pr.energy[c] - pr.p0[c] * dv / r;
+        e1 = e1 + pr.visc_heat[c] * pr.dt2 / r;
+        pr.pp[c] = (e1 - 0.0) * (pr.gamma - 1.0) * r * cv / c1;
+      }
     }
     db1[c] = d1;
     ax = bx_;

@@ -458,12 +458,14 @@ int ct_pair(double *const Bin[3], double *const B[3], const real *vel, double dt
 // copy bp of B; the first sweep reads B itself, the last converts back
 template <int AX>
 int sweep_predictor(double dt, bool first, bool last) {
+  // the last pass also forms the predictor energy and pressure
+  const PredP pr = {S.energy, S.rho, S.visc_heat, S.p0, S.pp, S.p.gamma, S.dt2};
   if (last)
     LAUNCH("geom", (k_geom<AX, 1>), zchunk_grid(cell_box()), box_block(), S.g, cell_box(), S.vx, S.vy, S.vz, S.cv1,
-           S.cv2, S.db1, dt, 1.0, 1.0, 1.0);
+           S.cv2, S.db1, dt, 1.0, 1.0, 1.0, pr);
   else
     LAUNCH("geom", (k_geom<AX, 0>), zchunk_grid(cell_box()), box_block(), S.g, cell_box(), S.vx, S.vy, S.vz, S.cv1,
-           S.cv2, S.db1, dt, 1.0, 1.0, 1.0);
+           S.cv2, S.db1, dt, 1.0, 1.0, 1.0, pr);
   const real *vel = AX == 0 ? S.vx : (AX == 1 ? S.vy : S.vz);
   double *const Bin[3] = {S.bx, S.by, S.bz};
   double *const Bp[3] = {S.bpx, S.bpy, S.bpz};
@@ -475,7 +477,7 @@ int sweep_full(double dt, double pass[3], bool first, bool last) {
   const int N[3] = {nx(), ny(), nz()};
   const real *vel = AX == 0 ? S.vx1 : (AX == 1 ? S.vy1 : S.vz1);
   LAUNCH("geom", (k_geom<AX, 2>), zchunk_grid(cell_box()), box_block(), S.g, cell_box(), S.vx1, S.vy1, S.vz1, S.cv1,
-         S.cv2, S.db1, dt, pass[0], pass[1], pass[2]);
+         S.cv2, S.db1, dt, pass[0], pass[1], pass[2], PredP{});
   double *const B[3] = {S.bx, S.by, S.bz};
   int rc = ct_pair<AX>(B, B, vel, dt, first ? 1 : (last ? -1 : 0));
   if (rc) return rc;
@@ -715,8 +717,8 @@ int lagrangian_step() {
   rc = eulerian_remap(S.step, true);
   S.dt = dt_full;
   if (rc) return rc;
-  LAUNCH_BOX("pred_pressure", k_pred_pressure, mkbox(0, N0 + 1, 0, N1 + 1, 0, N2 + 1), S.energy, S.rho, S.cv1,
-             S.visc_heat, S.p0, S.pp, S.p.gamma, S.dt2);
+  // the predictor energy and pressure were formed in the predictor
+  // remap's last geometry pass
   LAUNCH("force", k_force, zchunk_grid(nodes0n()), box_block(), S.g, nodes0n(), S.vx, S.vy, S.vz, S.pp, S.bpx, S.bpy,
          S.bpz, S.rho_v, S.al1, S.al2, S.al3, S.vx1, S.vy1, S.vz1, S.dt2);
   {
