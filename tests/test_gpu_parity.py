@@ -30,10 +30,12 @@ def _init(dev, orc):
         d.eta_calc()
 
 
+# n = 32: rows wider than two 128-byte lines, several waves per y-z plane
+@pytest.mark.parametrize("n", [6, 32])
 @pytest.mark.parametrize("bcs", [(BC_PERIODIC,) * 6, (BC_USER,) * 6,
                                  (BC_PERIODIC, BC_PERIODIC, BC_USER, BC_USER, BC_PERIODIC, BC_PERIODIC)])
-def test_boundary_conditions_bit_exact(bcs):
-    grid, params, f = configs.uniform(6, bcs=bcs)
+def test_boundary_conditions_bit_exact(bcs, n):
+    grid, params, f = configs.uniform(n, bcs=bcs)
     rng = np.random.default_rng(7)
     for name in STATE:
         a = getattr(f, name)
