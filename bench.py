@@ -291,7 +291,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--cpu-n", type=int, default=96)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage run")
     ap.add_argument("--no-conduction", action="store_true",
