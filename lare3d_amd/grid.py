@@ -23,14 +23,23 @@ def tanh_stretch(xb, n, length, new_length):
         xb[i + 2] = xb[i + 1] + dxnew[i + 2]
 
 
-def axis_metrics(n, lo, hi, periodic, stretch=None):
+def axis_metrics(n, lo, hi, periodic, stretch=None, widths=None):
     """Return (xb(-2:n+2), xc(-1:n+2), dxb(-1:n+2), dxc(-1:n+1)) for one axis,
     following setup.F90:106-149 (one MPI rank, so local == global).
-    stretch = the stretched length of the axis, or None for a uniform one."""
+    stretch = the stretched length of the axis, or None for a uniform one.
+    widths = the n cell widths of cells 1..n (nodes from lo), or None; the
+    C-ABI takes any 1-D metrics, so a grid need not come from the
+    reference's uniform or tanh-stretched forms."""
     d = (hi - lo) / float(n)
     xb = np.array([lo + float(i) * d for i in range(-2, n + 3)])  # index i+2
     if stretch is not None:
         tanh_stretch(xb, n, hi - lo, stretch)
+    if widths is not None:
+        w = np.asarray(widths, dtype=float)
+        assert w.shape == (n,)
+        xb[2] = lo
+        for i in range(1, n + 1):
+            xb[i + 2] = xb[i + 1] + w[i - 1]
     B = lambda i: xb[i + 2]
     if periodic:
         xb[n + 1 + 2] = B(n) + (B(1) - B(0))
@@ -56,16 +65,18 @@ def axis_metrics(n, lo, hi, periodic, stretch=None):
 class Grid:
     """Sizes, boundary codes, extents and metrics of one run."""
 
-    def __init__(self, nx, ny, nz, extent, bcs, stretch=(None, None, None)):
+    def __init__(self, nx, ny, nz, extent, bcs, stretch=(None, None, None), widths=(None, None, None)):
         self.nx, self.ny, self.nz = nx, ny, nz
         self.n = (nx, ny, nz)
         self.extent = extent            # ((x_min,x_max),(y_min,y_max),(z_min,z_max))
         self.bcs = tuple(bcs)           # (xbc_min, xbc_max, ybc_min, ybc_max, zbc_min, zbc_max)
         self.stretch = tuple(stretch)   # stretched length per axis, or None
+        self.widths = tuple(widths)     # explicit cell widths per axis, or None
         self.xb, self.xc, self.dxb, self.dxc = [], [], [], []
         for a in range(3):
             per = self.bcs[2 * a] == BC_PERIODIC and self.bcs[2 * a + 1] == BC_PERIODIC
-            xb, xc, db, dc = axis_metrics(self.n[a], extent[a][0], extent[a][1], per, self.stretch[a])
+            xb, xc, db, dc = axis_metrics(self.n[a], extent[a][0], extent[a][1], per, self.stretch[a],
+                                          self.widths[a])
             self.xb.append(xb); self.xc.append(xc); self.dxb.append(db); self.dxc.append(dc)
         self.grav = np.zeros(nz + 4)
 

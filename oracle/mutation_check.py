@@ -1,0 +1,125 @@
+"""Mutation check of the oracle's pins (test infrastructure).
+
+Plants one plausible mistake at a time in a copy of lare_oracle.c, builds
+the copy, and runs the CPU pin tests against it (LARE_ORACLE_LIB points the
+test wrapper at the copy).  A mutant is "caught" when at least one pin
+fails.  The oracle is only trusted where every mutant of this list is
+caught; the output table is the record of that.
+
+    python oracle/mutation_check.py            # all mutants
+    python oracle/mutation_check.py M2 M6      # a subset
+
+Each mutant names the reference lines whose restatement it breaks.
+"""
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "lare_oracle.c")
+CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-shared"]
+PIN_TESTS = ["tests/test_pin_hydro.py", "tests/test_pin_transport.py", "tests/test_pin_oracle3d.py"]
+
+# (name, what it breaks, [(old, new), ...]); each `old` occurs exactly once
+MUTANTS = [
+    ("M1", "vertex average of vy1 at the y face (xremap.f90:74-77): wrong y index",
+     [("double vyb = (A(o->vy1, ix, iy, iz) + A(o->vy1, ixm, iy, iz) + A(o->vy1, ix, iy, izm) + A(o->vy1, ixm, iy, izm)) * 0.25;",
+       "double vyb = (A(o->vy1, ix, iy, iz) + A(o->vy1, ixm, iy, iz) + A(o->vy1, ix, iy, izm) + A(o->vy1, ixm, iym, izm)) * 0.25;")]),
+    ("M2", "12-edge heating: alpha1(ix,iyp,iz) for alpha1(ix,iy,izp) (lagran.F90:726, :920)",
+     [("A(o->alpha1, ix, iy, izp) * a[4]", "A(o->alpha1, ix, iyp, iz) * a[4]")]),
+    ("M3", "edge_viscosity: visc1 and visc2 swapped (lagran.F90:850-851, :547)",
+     [("o->visc2_norm = 0.25 * (o->p.gamma + 1.0) * o->p.visc2;", "o->visc2_norm = 0.25 * (o->p.gamma + 1.0) * o->p.visc1;"),
+      ("SQ(o->p.visc1 * cs_edge)", "SQ(o->p.visc2 * cs_edge)")]),
+    ("M4", "temperature_bcs exchanges T instead of rho (boundary.f90:214, quirk Q3 'fixed')",
+     [("for (int ax = 0; ax < 3; ++ax) exchange(o, o->rho, ax, 0);\n  mirror_cell(o, o->temperature);",
+       "for (int ax = 0; ax < 3; ++ax) exchange(o, o->temperature, ax, 0);\n  mirror_cell(o, o->temperature);")]),
+    ("M5", "x momentum-flux area dyb*dzc for dyc*dzc (xremap.f90:615)",
+     [("double area = ax == 0 ? DYC(iy) * DZC(iz)", "double area = ax == 0 ? DYB(iy) * DZC(iz)")]),
+    ("M6", "Larsen constant 4.285 for 42.85 (conduct.f90:90)",
+     [("o->p.flux_limiter * 42.85", "o->p.flux_limiter * 4.285")]),
+    ("M7", "y-face temperature gradient over two cells (conduct.f90:232)",
+     [("tg = (A(T, ix, iyp, iz) - A(T, ix, iy, iz)) / DYC(iy);", "tg = (A(T, ix, iyp, iz) - A(T, ix, iym, iz)) / DYC(iy);")]),
+    ("M8", "edge density: arithmetic for harmonic mean (lagran.F90:820-821)",
+     [("double rho_edge = 2.0 * r1 * r2 / (r1 + r2);", "double rho_edge = 0.5 * (r1 + r2);")]),
+    ("M9", "edge sound speed: max for min (lagran.F90:822)",
+     [("double cs_edge = dmin(A(cs_v, i1, j1, k1), A(cs_v, i2, j2, k2));",
+       "double cs_edge = dmax(A(cs_v, i1, j1, k1), A(cs_v, i2, j2, k2));")]),
+    ("M10", "viscous force weight a4: alpha2(ix,iy,izp) for alpha2(ix,iyp,izp) (lagran.F90:763)",
+     [("double a4 = A(o->alpha2, ix, iyp, izp) * DXC(ix) * DZC(iz);", "double a4 = A(o->alpha2, ix, iy, izp) * DXC(ix) * DZC(iz);")]),
+    ("M11", "Larsen q_sp: gradient without the 1/2 of the centred difference (conduct.f90:93-95)",
+     [("double q_spx = -k0 * temp * (A(T, ix + 1, iy, iz) - A(T, ix - 1, iy, iz)) * 0.5 / DXB(ix);",
+       "double q_spx = -k0 * temp * (A(T, ix + 1, iy, iz) - A(T, ix - 1, iy, iz)) * 1.0 / DXB(ix);")]),
+    ("M12", "RKL2 nu_j with b(j-1) for b(j-2) (conduct.f90:334)",
+     [("nu[j] = -1.0 * ((dj - 1.0) / dj) * (b[j] / b[j - 2]);", "nu[j] = -1.0 * ((dj - 1.0) / dj) * (b[j] / b[j - 1]);")]),
+    ("M13", "z-face transverse y gradient taken over one cell but divided by two (conduct.f90:274-277)",
+     [("tb_m = (A(T, ix, iym, izp) + A(T, ix, iym, iz)) / 2.0;\n        tg_a2 = (tb_p - tb_m) / (DYC(iy) + DYC(iym));",
+       "tb_m = (A(T, ix, iy, izp) + A(T, ix, iy, iz)) / 2.0;\n        tg_a2 = (tb_p - tb_m) / (DYC(iy) + DYC(iym));")]),
+    ("M14", "z-face averaged bx with the x index unshifted (conduct.f90:258-259)",
+     [("bxf = 0.25 * (A(o->bx, ix, iy, iz) + A(o->bx, ix, iy, izp) + A(o->bx, ixm, iy, iz) + A(o->bx, ixm, iy, izp));",
+       "bxf = 0.25 * (A(o->bx, ix, iy, iz) + A(o->bx, ix, iy, izp) + A(o->bx, ixp, iy, iz) + A(o->bx, ixp, iy, izp));")]),
+    ("M15", "energy flux donor density not upwinded (xremap.f90:594)",
+     [("double rhou = AT(o->rho1, c) * r.vad_p + AT(o->rho1, p) * r.vad_m;", "double rhou = AT(o->rho1, c);")]),
+    ("M16", "shock-viscosity threshold on the current cell width of the other axis (lagran.F90:825)",
+     [("A(o->alpha1, ix, iy, iz) = edge_viscosity(o, cs_v, dvdots, DXB(ix), i1, j1, k1, i2, j2, k2);",
+       "A(o->alpha1, ix, iy, iz) = edge_viscosity(o, cs_v, dvdots, DYB(iy), i1, j1, k1, i2, j2, k2);")]),
+]
+
+
+def build(src_text, out_so):
+    with tempfile.NamedTemporaryFile("w", suffix=".c", dir=HERE, delete=False) as fh:
+        fh.write(src_text)
+        cpath = fh.name
+    try:
+        subprocess.run(["gcc", *CFLAGS, "-I", HERE, "-o", out_so, cpath, "-lm"], check=True)
+    finally:
+        os.remove(cpath)
+
+
+def run_pins(lib):
+    env = dict(os.environ, LARE_ORACLE_LIB=lib)
+    r = subprocess.run([sys.executable, "-m", "pytest", *PIN_TESTS, "-q", "-m", "not gpu", "-p", "no:cacheprovider",
+                        "--timeout", "300"], cwd=ROOT, env=env, capture_output=True, text=True)
+    failed = [ln.split(" ")[1] for ln in r.stdout.splitlines() if ln.startswith("FAILED")]
+    summary = r.stdout.strip().splitlines()[-1] if r.stdout.strip() else r.stderr.strip()[-200:]
+    return r.returncode, failed, summary
+
+
+def main(names):
+    base = open(SRC).read()
+    todo = [m for m in MUTANTS if not names or m[0] in names]
+    rows = []
+    with tempfile.TemporaryDirectory() as tmp:
+        # the unmutated oracle must pass every pin
+        so = os.path.join(tmp, "base.so")
+        build(base, so)
+        rc, failed, summary = run_pins(so)
+        print(f"unmutated: {summary}", flush=True)
+        if rc != 0:
+            print("the unmutated oracle fails its pins:", failed)
+            return 2
+        for name, what, edits in todo:
+            text = base
+            for old, new in edits:
+                if text.count(old) != 1:
+                    print(f"{name}: pattern occurs {text.count(old)} times: {old[:60]}")
+                    return 2
+                text = text.replace(old, new)
+            so = os.path.join(tmp, f"{name}.so")
+            build(text, so)
+            t0 = time.time()
+            rc, failed, summary = run_pins(so)
+            caught = rc != 0
+            rows.append((name, what, caught, failed))
+            print(f"{name:4s} {'caught' if caught else 'MISSED':6s} {time.time() - t0:5.1f}s  {what}", flush=True)
+            for f in failed[:6]:
+                print(f"       fails {f}", flush=True)
+    missed = [r[0] for r in rows if not r[2]]
+    print(f"{len(rows) - len(missed)}/{len(rows)} mutants caught" + (f"; missed: {' '.join(missed)}" if missed else ""))
+    return 1 if missed else 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(set(sys.argv[1:])))
