@@ -25,7 +25,14 @@ NAME_LEN = 48
 
 
 class Device:
+    # The library holds one device state per process (as the reference's
+    # module variables do); a second open Device would silently share it.
+    _open = None
+
     def __init__(self, grid, params, fields):
+        if Device._open is not None and Device._open.lib is not None:
+            raise DeviceError("the device library holds one state per process: "
+                              "close the open Device before creating another")
         self.lib = load_library()
         self.grid = grid
         self.n = grid.n
@@ -35,6 +42,7 @@ class Device:
         self._f = fields.copy()
         st = self._f.to_struct(with_visc3=bool(params.use_viscous_damping))
         check(self.lib, self.lib.lare_dev_upload(C.byref(st)), "lare_dev_upload")
+        Device._open = self
 
     def upload(self, fields):
         """Replace the device state with new host fields (same grid and
@@ -47,6 +55,8 @@ class Device:
         if self.lib is not None:
             self.lib.lare_dev_finalize()
             self.lib = None
+        if Device._open is self:
+            Device._open = None
 
     def fields(self):
         f = Fields(*self.n)

@@ -17,3 +17,13 @@ def oracle_lib():
     from oracle import oracle
     oracle.build()
     return oracle
+
+
+@pytest.fixture(autouse=True)
+def _close_open_device():
+    """The device library holds one state per process: a test that fails
+    before closing its Device must not leave it open for the next test."""
+    yield
+    from lare3d_amd.device import Device
+    if Device._open is not None:
+        Device._open.close()
