@@ -137,6 +137,22 @@ int lare_dev_boundary_conditions(void);
  * reference counterpart). */
 int lare_dev_energy_account(double en[3], double *divb);
 
+/* energy_account on the caller's own device arrays, allocating nothing.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same results as lare_dev_energy_account
+ * for the same field values, and needs no lare_dev_init.  dev_fields holds
+ * DEVICE pointers to FP64 arrays in the Fortran layout above (rho, energy,
+ * vx, vy, vz, bx, by, bz; the others are ignored); grid supplies the sizes
+ * and the host metrics dxb, dyb, dzb.  Partial sums live in a fixed device
+ * array of about 8.1 MiB that each GPU holds until the process ends: at most
+ * 4096 cells per axis and 262144 reduction blocks ((nx+1)(ny+1)/256 rounded
+ * up, times nz+1 -- about 400^3 cells); larger grids return LARE_E_ARG before
+ * any device call.  Calls that overlap on one GPU share that array and race.
+ * Runs on the null stream and returns after the sums reach the host. */
+int lare_dev_energy_account_noalloc(const lare_grid *grid, const lare_fields *dev_fields, double en[3],
+                                    double *divb);
+
 /* One full iteration of the lare3d.f90:82-88 loop body:
  * lagrangian_step, eulerian_remap(step), eta_calc.  Returns dt and time. */
 int lare_dev_step(int32_t step, double *dt, double *time);

@@ -69,6 +69,7 @@ DEVICE_SYMBOLS = {
     "lare_dev_boundary_conditions": [],
     "lare_dev_energy_account": [_dp, _dp],
     "lare_dev_step": [C.c_int32, _dp, _dp],
+    "lare_dev_energy_account_noalloc": [C.POINTER(LareGrid), C.POINTER(LareFields), _dp, _dp],
     "lare_dev_profile": [C.c_int32],
     "lare_dev_kernel_stats": [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), _dp, C.c_int32],
     "lare_dev_sur_set_params": [_fp, C.c_int32],

@@ -755,43 +755,83 @@ __global__ __launch_bounds__(256) void k_vupdate_low(Geo g, const real *__restri
   vzo[c] = (real)(2.0 * vz1[c] - vz[c]);
 }
 
-__global__ __launch_bounds__(BX * BY) void k_energy(Geo g, const real *__restrict__ rho, const real *__restrict__ energy,
-                         const real *__restrict__ vx, const real *__restrict__ vy, const real *__restrict__ vz,
-                         const double *__restrict__ bx, const double *__restrict__ by,
-                         const double *__restrict__ bz, double *__restrict__ partial) {
+// Element offsets of the fields energy_account reads, per staggering.
+// PadLayout: the library's padded box (one index expression for all kinds).
+struct PadLayout {
+  Geo g;
+  __device__ __forceinline__ int cell(int i, int j, int k) const { return g.idx(i, j, k); }
+  __device__ __forceinline__ int node(int i, int j, int k) const { return g.idx(i, j, k); }
+  __device__ __forceinline__ int fx(int i, int j, int k) const { return g.idx(i, j, k); }
+  __device__ __forceinline__ int fy(int i, int j, int k) const { return g.idx(i, j, k); }
+  __device__ __forceinline__ int fz(int i, int j, int k) const { return g.idx(i, j, k); }
+};
+// FtnLayout: the caller's Fortran arrays with the bounds of
+// mpi_routines.F90:295-334 -- cell (-1:n+2)^3, node (-2:n+2)^3,
+// bx (-2:nx+2, -1:ny+2, -1:nz+2) and likewise by, bz.
+struct FtnLayout {
+  int nx, ny, nz;
+  __device__ __forceinline__ int cell(int i, int j, int k) const {
+    return (i + 1) + (nx + 4) * ((j + 1) + (ny + 4) * (k + 1));
+  }
+  __device__ __forceinline__ int node(int i, int j, int k) const {
+    return (i + 2) + (nx + 5) * ((j + 2) + (ny + 5) * (k + 2));
+  }
+  __device__ __forceinline__ int fx(int i, int j, int k) const {
+    return (i + 2) + (nx + 5) * ((j + 1) + (ny + 4) * (k + 1));
+  }
+  __device__ __forceinline__ int fy(int i, int j, int k) const {
+    return (i + 1) + (nx + 4) * ((j + 2) + (ny + 5) * (k + 1));
+  }
+  __device__ __forceinline__ int fz(int i, int j, int k) const {
+    return (i + 1) + (nx + 4) * ((j + 1) + (ny + 4) * (k + 2));
+  }
+};
+
+// energy_account (diagnostics.F90:517-609) plus max |div B| dx: per-block
+// partials of (en_b, en_ke, en_int, -|div B| dx) over nodes 0..n; the cell
+// terms on cells 1..n.  The layout L says where each staggering keeps
+// element (i, j, k); the arithmetic does not depend on it.
+template <class L, class R>
+__global__ __launch_bounds__(BX * BY) void k_energy(Geo g, L lay, const R *__restrict__ rho,
+                                                    const R *__restrict__ energy, const R *__restrict__ vx,
+                                                    const R *__restrict__ vy, const R *__restrict__ vz,
+                                                    const double *__restrict__ bx, const double *__restrict__ by,
+                                                    const double *__restrict__ bz, double *__restrict__ partial) {
   const Box b = {0, g.nx, 0, g.ny, 0, g.nz};
   PLANE_POS(b, i, j)
   const int k = (int)blockIdx.z;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (j <= g.ny && k <= g.nz) {
-    const int c = g.idx(i, j, k);
-    const int X = 1, Y = g.sy, Z = g.sz;
     if (i >= 1 && j >= 1 && k >= 1) {
-      double w1 = (bx[c] * bx[c] + bx[c - X] * bx[c - X]) * 0.5;
-      const double w2 = (by[c] * by[c] + by[c - Y] * by[c - Y]) * 0.5;
-      const double w3 = (bz[c] * bz[c] + bz[c - Z] * bz[c - Z]) * 0.5;
+      const double bx0 = bx[lay.fx(i, j, k)], bxm = bx[lay.fx(i - 1, j, k)];
+      const double by0 = by[lay.fy(i, j, k)], bym = by[lay.fy(i, j - 1, k)];
+      const double bz0 = bz[lay.fz(i, j, k)], bzm = bz[lay.fz(i, j, k - 1)];
+      double w1 = (bx0 * bx0 + bxm * bxm) * 0.5;
+      const double w2 = (by0 * by0 + bym * bym) * 0.5;
+      const double w3 = (bz0 * bz0 + bzm * bzm) * 0.5;
       w1 = (w1 + w2 + w3) * 0.5;
       const double cv = g.cv(i, j, k);
+      const int c = lay.cell(i, j, k);
       v[0] = w1 * cv;
       v[2] = energy[c] * rho[c] * cv;
       const double dx = g.dxb[i], dy = g.dyb[j], dz = g.dzb[k];
-      const double d = (bx[c] - bx[c - X]) / dx + (by[c] - by[c - Y]) / dy + (bz[c] - bz[c - Z]) / dz;
+      const double d = (bx0 - bxm) / dx + (by0 - bym) / dy + (bz0 - bzm) / dz;
       v[3] = -(fabs(d) * dmin(dmin(dx, dy), dz));
     }
     double cvs[8];
     const int di[8] = {0, 1, 0, 1, 0, 1, 0, 1}, dj[8] = {0, 0, 1, 1, 0, 0, 1, 1}, dk[8] = {0, 0, 0, 0, 1, 1, 1, 1};
-    const int cc[8] = {c, c + X, c + Y, c + X + Y, c + Z, c + X + Z, c + Y + Z, c + X + Y + Z};
 #pragma unroll
     for (int q = 0; q < 8; ++q) cvs[q] = g.cv(i + di[q], j + dj[q], k + dk[q]);
-    double rv = rho[cc[0]] * cvs[0];
+    double rv = rho[lay.cell(i, j, k)] * cvs[0];
 #pragma unroll
-    for (int q = 1; q < 8; ++q) rv = rv + rho[cc[q]] * cvs[q];
+    for (int q = 1; q < 8; ++q) rv = rv + rho[lay.cell(i + di[q], j + dj[q], k + dk[q])] * cvs[q];
     double cvv = cvs[0];
 #pragma unroll
     for (int q = 1; q < 8; ++q) cvv = cvv + cvs[q];
     rv = rv / cvv;
     cvv = cvv * 0.125;
-    double w1 = rv * cvv * (vx[c] * vx[c] + vy[c] * vy[c] + vz[c] * vz[c]);
+    const int n = lay.node(i, j, k);
+    double w1 = rv * cvv * (vx[n] * vx[n] + vy[n] * vy[n] + vz[n] * vz[n]);
     if (i == 0 || i == g.nx) w1 = w1 * 0.5;
     if (j == 0 || j == g.ny) w1 = w1 * 0.5;
     if (k == 0 || k == g.nz) w1 = w1 * 0.5;

@@ -56,6 +56,8 @@ namespace fp32 {
 }  // namespace fp32
 #undef LARE_REAL
 
+#include "noalloc.hpp"
+
 namespace {
 int g_prec = LARE_PREC_FP64;  // storage precision of the initialised state
 }  // namespace
@@ -90,6 +92,9 @@ int lare_dev_conduct_heat(void) { return FWD(api_conduct_heat()); }
 int lare_dev_boundary_conditions(void) { return FWD(api_boundary_conditions()); }
 int lare_dev_energy_account(double en[3], double *divb) { return FWD(api_energy_account(en, divb)); }
 int lare_dev_step(int32_t step, double *dt, double *time) { return FWD(api_step(step, dt, time)); }
+int lare_dev_energy_account_noalloc(const lare_grid *grid, const lare_fields *dev_fields, double en[3], double *divb) {
+  return api_energy_account_noalloc(grid, dev_fields, en, divb);
+}
 int lare_dev_profile(int32_t enable) { return FWD(api_profile(enable)); }
 int lare_dev_kernel_stats(char *names, int32_t name_len, int32_t *counts, double *total_ms, int32_t cap) {
   return FWD(api_kernel_stats(names, name_len, counts, total_ms, cap));

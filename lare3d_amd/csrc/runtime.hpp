@@ -1035,7 +1035,8 @@ int api_energy_account(double en[3], double *divb) {
   REQUIRE_LOADED();
   if (!en) return fail(LARE_E_ARG, "energy_account: NULL output");
   dim3 gr = grid_from0(nx(), ny(), nz());
-  LAUNCH("energy", k_energy, gr, box_block(), S.g, S.rho, S.energy, S.vx, S.vy, S.vz, S.bx, S.by, S.bz, S.partial);
+  LAUNCH("energy", (k_energy<PadLayout, real>), gr, box_block(), S.g, PadLayout{S.g}, S.rho, S.energy, S.vx, S.vy,
+         S.vz, S.bx, S.by, S.bz, S.partial);
   int rc = final_reduce(gr, 4, 8, 0, 0);
   if (rc) return rc;
   double v[4];

@@ -46,3 +46,25 @@ def test_call_order_and_argument_errors():
     g = grid_struct(grid)
     p = default_params(precision=7)
     assert lib.lare_dev_init(C.byref(g), C.byref(p)) == -1002
+
+
+def test_noalloc_rejects_sizes_before_any_device_call():
+    """Sizes beyond the fixed scratch array are refused with LARE_E_ARG
+    before the first HIP call (so this runs without a GPU)."""
+    lib = abi.load_library()
+    dummy = (C.c_double * 8)()
+    fields = abi.LareFields()
+    for name in ("rho", "energy", "vx", "vy", "vz", "bx", "by", "bz"):
+        setattr(fields, name, C.cast(dummy, C.POINTER(C.c_double)))
+    en = (C.c_double * 3)()
+    for n, msg in (((4097, 8, 8), b"outside 4..4096"), ((512, 512, 512), b"exceed the scratch maximum"),
+                   ((3, 8, 8), b"outside 4..4096")):
+        grid = Grid(*n, ((0, 1),) * 3, (1,) * 6)
+        g = grid_struct(grid)
+        assert lib.lare_dev_energy_account_noalloc(C.byref(g), C.byref(fields), en, None) == -1002
+        assert msg in lib.lare_dev_last_error(), lib.lare_dev_last_error()
+    grid = Grid(8, 8, 8, ((0, 1),) * 3, (1,) * 6)
+    g = grid_struct(grid)
+    fields.bz = None
+    assert lib.lare_dev_energy_account_noalloc(C.byref(g), C.byref(fields), en, None) == -1002
+    assert b"missing field" in lib.lare_dev_last_error()
