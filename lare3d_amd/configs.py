@@ -213,6 +213,54 @@ def harris_conduction(n=32, seed=0, s_target=9, kappa_0=1.0e-2, blobs=12):
     return grid, params, f, dt
 
 
+def kink_tube(nx=64, ny=None, nz=None, beta=0.05, eta=1.0e-3, kappa_0=0.1, amp=1.0e-3, half_width=4.0):
+    """Kink-unstable twisted flux tube (config 5): the uniform-twist field
+    B_theta = 2r/(1 + r^2) about the z axis with B_z = 1, from the edge
+    vector potential A_z = -ln(1 + r^2) (so the face field is discretely
+    divergence-free), rho = 1 and the gas pressure that balances it,
+    p = beta/2 + 2/(1 + r^2)^2 (beta = 0.05 in the field B = 1 far from the
+    axis; the stated B_z = 1 with this B_theta is not force-free, so the
+    pressure rises to 2 + beta/2 on the axis).  Resistive (uniform eta) with
+    Braginskii conduction.  The tube is seeded with a helical m = 1 velocity
+    v = amp exp(-r^2/2) (cos kz, sin kz, 0), k = 2 pi / Lz.
+    x, y in [-half_width, half_width] with walls (BC_USER); z periodic over
+    [-Lz/2, Lz/2], Lz = 2 half_width nz / nx (cubic cells).
+    Profiles are built in the x-y plane and broadcast along z, so the
+    512 x 512 x 1024 case needs no 3-D temporaries."""
+    ny = ny or nx
+    nz = nz or 2 * nx
+    L = half_width
+    Lz = 2.0 * L * nz / nx
+    grid = Grid(nx, ny, nz, ((-L, L), (-L, L), (-0.5 * Lz, 0.5 * Lz)),
+                (BC_USER, BC_USER, BC_USER, BC_USER, BC_PERIODIC, BC_PERIODIC))
+    params = default_params(resistive_mhd=1, eta_background=eta, eta0=0.0, j_max=1.0e10,
+                            conduction=1, kappa_0=kappa_0, flux_limiter=0.1, heat_flux_limiter=1)
+    f = Fields(nx, ny, nz)
+    gam = params.gamma
+    xb, yb, zb = grid.xb
+    xc, yc, zc = grid.xc
+    dxb, dyb, dzb = grid.dxb
+    # A_z on z-edges (xb_i, yb_j), i, j = -2..n+2
+    az = -np.log1p(xb[:, None] ** 2 + yb[None, :] ** 2)
+    # bx(i, j) = (Az(i, j) - Az(i, j-1)) / dyb(j): i = -2..nx+2, j = -1..ny+2
+    bx2 = (az[:, 1:] - az[:, :-1]) / dyb[None, :]
+    # by(i, j) = -(Az(i, j) - Az(i-1, j)) / dxb(i): i = -1..nx+2, j = -2..ny+2
+    by2 = -(az[1:, :] - az[:-1, :]) / dxb[:, None]
+    f.bx[...] = bx2[:, :, None]
+    f.by[...] = by2[:, :, None]
+    f.bz[...] = 1.0
+    r2 = xc[:, None] ** 2 + yc[None, :] ** 2
+    p2 = 0.5 * beta + 2.0 / (1.0 + r2) ** 2
+    f.rho[...] = 1.0
+    f.energy[...] = (p2 / (gam - 1.0))[:, :, None]
+    k = 2.0 * math.pi / Lz
+    env = amp * np.exp(-0.5 * (xb[:, None] ** 2 + yb[None, :] ** 2))
+    f.vx[...] = env[:, :, None] * np.cos(k * zb)[None, None, :]
+    f.vy[...] = env[:, :, None] * np.sin(k * zb)[None, None, :]
+    f.vz[...] = 0.0
+    return grid, params, f
+
+
 def uniform(n=8, bcs=(BC_PERIODIC,) * 6, **kw):
     """A uniform state, for invariance tests."""
     grid = Grid(n, n, n, ((0.0, 1.0),) * 3, bcs)

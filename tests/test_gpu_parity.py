@@ -82,6 +82,9 @@ CASES = {
     "harris_wide": lambda: configs.harris(64, ny=9, nz=6),
     "arcade": lambda: configs.arcade_conduction(16, 16, 24, s_target=7),
     "arcade_odd": lambda: configs.arcade_conduction(13, 11, 19, s_target=5),
+    # the twisted flux tube (walls in x and y, periodic z; resistive with
+    # Braginskii conduction)
+    "kink": lambda: configs.kink_tube(16),
 }
 
 
@@ -111,3 +114,25 @@ def test_full_steps(case):
     assert abs(sd.total_ohmic_heating - so.total_ohmic_heating) <= 1e-10 * abs(so.total_ohmic_heating) + 1e-300
     assert sd.last_substeps == so.last_substeps and sd.n_s_stages == so.n_s_stages
     dev.close()
+
+
+def test_harris_128_two_steps():
+    """The bench workload's structure at 128^3 (many tiles per plane, many
+    z-chunks and reduction blocks): two full steps, whole arrays equal to
+    the oracle's element for element."""
+    dev, orc = _pair(configs.harris(128))
+    _init(dev, orc)
+    for step in (1, 2):
+        dtd, _ = dev.step(step)
+        dto, _ = orc.step(step)
+        fd, fo = dev.fields(), orc.fields()
+        ndiff = {n: int(np.count_nonzero(getattr(fd, n) != getattr(fo, n))) for n in STATE}
+        print(f"harris 128^3 step {step}: dt dev {dtd:.16e} orc {dto:.16e} differing elements {sum(ndiff.values())}",
+              flush=True)
+        assert dtd == dto
+        assert sum(ndiff.values()) == 0, ndiff
+    en_d, en_o = dev.energy_account(), orc.energy_account()
+    for a, b in zip(en_d, en_o):
+        assert abs(a - b) <= 1e-12 * abs(b)
+    dev.close()
+    orc.close()
