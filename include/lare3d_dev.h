@@ -45,7 +45,8 @@ extern "C" {
 #define LARE_E_ARG     -1002 /* invalid argument / unsupported option       */
 #define LARE_E_NOMEM   -1003 /* device allocation failed                    */
 /* Physics faults (positive). */
-#define LARE_F_NONFINITE 1   /* non-finite dt or energy after a step         */
+#define LARE_F_NONFINITE 1   /* non-finite dt, or a state with a non-finite
+                                value or rho <= 0 entering a step            */
 
 /* Grid: sizes, boundary codes and the 1-D metrics of setup.F90:92-247.
  * Replaces the module state nx/ny/nz, x/y/zbc_*, dxb/dxc/dyb/dyc/dzb/dzc
@@ -99,6 +100,19 @@ typedef struct lare_scalars {
   int32_t first_set_dt;    /* 1 until set_dt has run once (restart logic) */
   int32_t step;            /* step counter of lare3d.f90:82 (Strang order) */
 } lare_scalars;
+
+/* Scalars of one step, as lare_dev_step_info returns them. */
+typedef struct lare_step_info {
+  double dt, dtr, time;    /* shared_data.F90:126-139                    */
+  double total_visc_heating, total_ohmic_heating;
+  double en_b, en_ke, en_int; /* energy_account of the new state         */
+  double max_divb;         /* max |div B| dx of the new state             */
+  int32_t n_s_stages;      /* RKL2 stages of the step (conduction)        */
+  int32_t substeps;        /* resistive substeps of the step              */
+  int32_t peak_substeps;
+  int32_t sur_accepted, sur_rejected; /* surrogate super-steps so far     */
+  int32_t with_energy;     /* 1 if the energy fields are set              */
+} lare_step_info;
 
 /* Initialise the device state for one grid (allocates all device memory).
  * Replaces the allocation part of mpi_initialise (mpi_routines.F90:295-334)
@@ -156,6 +170,15 @@ int lare_dev_energy_account_noalloc(const lare_grid *grid, const lare_fields *de
 /* One full iteration of the lare3d.f90:82-88 loop body:
  * lagrangian_step, eulerian_remap(step), eta_calc.  Returns dt and time. */
 int lare_dev_step(int32_t step, double *dt, double *time);
+
+/* lare_dev_step, then the scalars the driver reads after a step (dt, dtr,
+ * time, heating totals, stage and substep counts, surrogate counts) and,
+ * when with_energy != 0, energy_account (diagnostics.F90:517) and max
+ * |div B| of the new state, read back together in one device-to-host copy.
+ * LARE_F_NONFINITE reports a state that holds a non-finite value or
+ * rho <= 0 (the field and cell are in lare_dev_last_error) or a non-finite
+ * dt. */
+int lare_dev_step_info(int32_t step, int32_t with_energy, lare_step_info *info);
 
 /* Per-kernel timing with HIP events on the library stream (for roofline
  * measurement).  enable != 0 starts recording; lare_dev_kernel_stats fills

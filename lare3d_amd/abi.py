@@ -34,6 +34,14 @@ class LareParams(C.Structure):
                 ("restart", C.c_int32), ("precision", C.c_int32)]
 
 
+class LareStepInfo(C.Structure):
+    _fields_ = [("dt", C.c_double), ("dtr", C.c_double), ("time", C.c_double),
+                ("total_visc_heating", C.c_double), ("total_ohmic_heating", C.c_double),
+                ("en_b", C.c_double), ("en_ke", C.c_double), ("en_int", C.c_double), ("max_divb", C.c_double),
+                ("n_s_stages", C.c_int32), ("substeps", C.c_int32), ("peak_substeps", C.c_int32),
+                ("sur_accepted", C.c_int32), ("sur_rejected", C.c_int32), ("with_energy", C.c_int32)]
+
+
 class LareFields(C.Structure):
     _fields_ = [(n, _dp) for n in ("rho", "energy", "vx", "vy", "vz", "bx", "by", "bz",
                                    "visc3", "eta", "visc_dep", "ohmic_dep")]
@@ -69,6 +77,7 @@ DEVICE_SYMBOLS = {
     "lare_dev_boundary_conditions": [],
     "lare_dev_energy_account": [_dp, _dp],
     "lare_dev_step": [C.c_int32, _dp, _dp],
+    "lare_dev_step_info": [C.c_int32, C.c_int32, C.POINTER(LareStepInfo)],
     "lare_dev_energy_account_noalloc": [C.POINTER(LareGrid), C.POINTER(LareFields), _dp, _dp],
     "lare_dev_profile": [C.c_int32],
     "lare_dev_kernel_stats": [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), _dp, C.c_int32],

@@ -270,7 +270,10 @@ __global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, Box bx0, const real 
   // become those of plane k-1 / k for the next cell
   PLANE_POS(bx0, i, j)
   ZRANGE(bx0, k0, k1)
-  double v[2] = {LARGEST, LARGEST};
+  // v[2]: state check of the cells and nodes 1..n (no reference
+  // counterpart): -((field + 1) 2^32 + cell) for a non-finite field value
+  // or rho <= 0, fields numbered rho, energy, vx, vy, vz, bx, by, bz
+  double v[3] = {LARGEST, LARGEST, LARGEST};
   if (j <= bx0.y1) {
     const int Y = g.sy, Z = g.sz;
     const int cb = g.idx(i, j, k0);
@@ -309,6 +312,15 @@ __global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, Box bx0, const real 
         const double dt2 = dmin(dmin(dxb / dmax(fabs((double)hx.c), NONE_ZERO), dyb / dmax(fabs((double)hy.c), NONE_ZERO)),
                                 dzb / dmax(fabs((double)hz.c), NONE_ZERO));
         v[0] = dmin(v[0], dmin(dt1, dt2));
+        if (i >= 1 && j >= 1 && k >= 1) {
+          const double vals[8] = {r, (double)energy[c], (double)hx.c, (double)hy.c, (double)hz.c,
+                                  bx[c], by[c], bz[c]};
+          const double cell = (double)((i - 1) + g.nx * ((j - 1) + g.ny * (k - 1)));
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+            if (!__builtin_isfinite(vals[q]) || (q == 0 && !(r > 0.0)))
+              v[2] = dmin(v[2], -((double)(q + 1) * 4294967296.0 + cell));
+        }
         if (a.resistive) {
           const double dxlocal = 1.0 / (1.0 / (dxb * dxb) + 1.0 / (dyb * dyb) + 1.0 / (dzb * dzb));
           v[1] = dmin(v[1], 0.2 * dxlocal / dmax((double)eta[c], NONE_ZERO));
@@ -320,8 +332,8 @@ __global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, Box bx0, const real 
       alo = ahi;
     }
   }
-  const int ops[2] = {0, 0};
-  block_reduce_store<2>(v, ops, partial);
+  const int ops[3] = {0, 0, 0};
+  block_reduce_store<3>(v, ops, partial);
 }
 
 // viscous_damping (lagran.F90:483-524), two passes: q then v += dt q

@@ -92,6 +92,9 @@ int lare_dev_conduct_heat(void) { return FWD(api_conduct_heat()); }
 int lare_dev_boundary_conditions(void) { return FWD(api_boundary_conditions()); }
 int lare_dev_energy_account(double en[3], double *divb) { return FWD(api_energy_account(en, divb)); }
 int lare_dev_step(int32_t step, double *dt, double *time) { return FWD(api_step(step, dt, time)); }
+int lare_dev_step_info(int32_t step, int32_t with_energy, lare_step_info *info) {
+  return FWD(api_step_info(step, with_energy, info));
+}
 int lare_dev_energy_account_noalloc(const lare_grid *grid, const lare_fields *dev_fields, double en[3], double *divb) {
   return api_energy_account_noalloc(grid, dev_fields, en, divb);
 }

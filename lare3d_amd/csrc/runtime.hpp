@@ -651,11 +651,21 @@ int lagrangian_step() {
   dim3 gr = zchunk_grid(vdb);
   LAUNCH("visc_dt", k_visc_dt, gr, box_block(), S.g, vdb, S.vx, S.vy, S.vz, S.rho, S.energy, S.bx, S.by, S.bz, S.eta,
          S.al1, S.al2, S.al3, S.visc_heat, S.p0, S.partial, a);
-  int rc = final_reduce(gr, 2, 3, 0, 0);
+  int rc = final_reduce(gr, 3, 7, 0, 0);
   if (rc) return rc;
-  double mins[2];
-  rc = read_scalars(0, 2, mins);
+  double mins[3];
+  rc = read_scalars(0, 3, mins);
   if (rc) return rc;
+  if (mins[2] < 0.0) {
+    // the state entering the step holds a non-finite value or rho <= 0
+    static const char *names[8] = {"rho", "energy", "vx", "vy", "vz", "bx", "by", "bz"};
+    const double code = -mins[2];
+    const int fid = (int)(code / 4294967296.0) - 1;
+    const long cell = (long)(code - (double)(fid + 1) * 4294967296.0);
+    const int ci = (int)(cell % N0) + 1, cj = (int)((cell / N0) % N1) + 1, ck = (int)(cell / ((long)N0 * N1)) + 1;
+    return fail(LARE_F_NONFINITE, "step %d: %s is not finite%s at (%d, %d, %d) entering lagrangian_step", S.step,
+                fid >= 0 && fid < 8 ? names[fid] : "?", fid == 0 ? " or not positive" : "", ci, cj, ck);
+  }
   rc = set_dt_and_stages(mins[0], mins[1]);
   if (rc) return rc;
   S.dt2 = 0.5 * S.dt;
@@ -1058,7 +1068,7 @@ int api_step(int32_t step, double *dt, double *time) {
   if (rc) return rc;
   rc = eta_calc();
   if (rc) return rc;
-  if (!(S.dt == S.dt) || (S.dt <= 0.0 && !S.p.restart))
+  if (!isfinite(S.dt) || (S.dt <= 0.0 && !S.p.restart))
     return fail(LARE_F_NONFINITE, "step %d: invalid dt %g", step, S.dt);
   if (dt) *dt = S.dt;
   if (time) *time = S.time;
@@ -1601,3 +1611,38 @@ int api_sur_stats(int32_t *accepted, int32_t *rejected, double *last_residual) {
   return 0;
 }
 
+// lare_dev_step followed by the scalars a driver reads after it, and
+// optionally energy_account of the new state, read back in one copy.
+int api_step_info(int32_t step, int32_t with_energy, lare_step_info *o) {
+  REQUIRE_LOADED();
+  if (!o) return fail(LARE_E_ARG, "step_info: NULL output");
+  double dt = 0.0, t = 0.0;
+  int rc = api_step(step, &dt, &t);
+  if (rc) return rc;
+  if (with_energy) {
+    dim3 gr = grid_from0(nx(), ny(), nz());
+    LAUNCH("energy", (k_energy<PadLayout, real>), gr, box_block(), S.g, PadLayout{S.g}, S.rho, S.energy, S.vx, S.vy,
+           S.vz, S.bx, S.by, S.bz, S.partial);
+    rc = final_reduce(gr, 4, 8, 0, 0);
+    if (rc) return rc;
+  }
+  double v[10];
+  rc = read_scalars(0, 10, v);
+  if (rc) return rc;
+  o->dt = S.dt;
+  o->dtr = S.dtr;
+  o->time = S.time;
+  o->total_visc_heating = v[8];
+  o->total_ohmic_heating = v[9];
+  o->en_b = with_energy ? v[0] : 0.0;
+  o->en_ke = with_energy ? v[1] : 0.0;
+  o->en_int = with_energy ? v[2] : 0.0;
+  o->max_divb = with_energy ? -v[3] : 0.0;
+  o->n_s_stages = S.n_s_stages;
+  o->substeps = S.last_substeps;
+  o->peak_substeps = S.peak_substeps;
+  o->sur_accepted = SU.accepted;
+  o->sur_rejected = SU.rejected;
+  o->with_energy = with_energy ? 1 : 0;
+  return 0;
+}

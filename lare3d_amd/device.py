@@ -16,7 +16,7 @@ import ctypes as C
 
 import numpy as np
 
-from .abi import DeviceError, LareScalars, check, load_library
+from .abi import DeviceError, LareScalars, LareStepInfo, check, load_library
 from .fields import Fields
 from .grid import grid_struct
 
@@ -103,6 +103,12 @@ class Device:
         dt, t = C.c_double(), C.c_double()
         check(self.lib, self.lib.lare_dev_step(step, C.byref(dt), C.byref(t)), f"step {step}")
         return dt.value, t.value
+
+    def step_info(self, step, with_energy=True):
+        """step() plus the per-step scalars (and energies) in one read-back."""
+        info = LareStepInfo()
+        check(self.lib, self.lib.lare_dev_step_info(step, 1 if with_energy else 0, C.byref(info)), f"step {step}")
+        return info
 
     def synchronize(self):
         check(self.lib, self.lib.lare_dev_synchronize(), "synchronize")
