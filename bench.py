@@ -2,7 +2,8 @@
 """Throughput of the Lare3d hot path on one MI355X.
 
 Workload (one "step" = one iteration of the lare3d.f90:82-88 loop body:
-lagrangian_step + eulerian_remap + eta_calc): the resistive Harris-sheet
+lagrangian_step + eulerian_remap + eta_calc, with the energy_account that
+output_routines makes every step): the resistive Harris-sheet
 configuration at 256^3 in FP64 -- Bx = tanh(y/0.5), rho = 1 + sech^2(y/0.5),
 pressure balance, eta = 1e-3 plus 1e-2 above |J| = 1, seeded 1% velocity
 noise, shock viscosity on, periodic x/z and BC_USER walls in y.
@@ -15,6 +16,10 @@ after the same steps), and the CPU restatement timed on a bounded sample of
 the same workload as a reported baseline.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
+
+Also the kink-unstable twisted flux tube at 512 x 512 x 1024 (resistive
+with conduction) in FP32 storage against FP64 ("kink"), and the conduction
+super-step with the learned surrogate ("conduction").
 """
 import argparse
 import json
@@ -131,19 +136,22 @@ def copy_probe(nbytes=1 << 30, reps=3):
 
 
 def timed_run(dev, warmup, steps):
-    """warmup untimed steps, then `steps` timed ones; (CU/s, ms/step, last step)."""
+    """warmup untimed steps, then `steps` timed ones; (CU/s, ms/step, last step).
+    A step is the lare3d.f90:84-88 loop body with the energy_account of
+    output_routines (diagnostics.F90:46,76): lare_dev_step_info with the
+    energies, which also returns the step's scalars."""
     n = dev.n
     dev.boundary_conditions()
     dev.eta_calc()
     step = 0
     for _ in range(warmup):
         step += 1
-        dev.step(step)
+        dev.step_info(step, with_energy=True)
     dev.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         step += 1
-        dev.step(step)
+        dev.step_info(step, with_energy=True)
     dev.synchronize()
     elapsed = time.perf_counter() - t0
     return n[0] * n[1] * n[2] * steps / elapsed, 1e3 * elapsed / steps, step
@@ -172,6 +180,68 @@ def fp32_run(n, warmup, steps, f64_end):
     return {"value": cups, "unit": "cell-updates/s", "ms_per_step": ms,
             "storage": "FP32 bulk fields; FP64 face B, CT fluxes, metrics, reductions and arithmetic",
             "field_rel_l2_vs_fp64": errs, "steps_compared": warmup + steps, "max_divb_dx": divb}
+
+
+def _vector_errors(fa, fb):
+    """Relative L2 of fa against fb over the interior: rho and energy per
+    field, v and B as vectors."""
+    errs = {}
+    for name, comps in (("rho", ("rho",)), ("energy", ("energy",)), ("v", ("vx", "vy", "vz")),
+                        ("B", ("bx", "by", "bz"))):
+        num = sum(float(np.linalg.norm(fa.interior(c) - fb.interior(c))) ** 2 for c in comps)
+        den = sum(float(np.linalg.norm(fb.interior(c))) ** 2 for c in comps)
+        errs[name] = (num / den) ** 0.5
+    return errs
+
+
+def kink_leg(n, warmup, steps):
+    """The kink-unstable twisted flux tube (n x n x 2n; resistive with
+    Braginskii conduction) in FP32 storage and in FP64: throughput, the
+    energy budget and div B of each run, and the field errors of the FP32
+    run against the FP64 run after the same steps."""
+    from lare3d_amd import configs
+    from lare3d_amd.abi import PREC_FP32
+    from lare3d_amd.device import Device
+    out = {"workload": f"kink-unstable twisted flux tube {n}x{n}x{2 * n}, B_theta = 2r/(1+r^2), B_z = 1, rho = 1, "
+                       "beta = 0.05 outside the tube, eta = 1e-3, Braginskii conduction (kappa_0 = 0.1)",
+           "steps_timed": steps, "steps_total": warmup + steps}
+    ends = {}
+    for prec in ("fp64", "fp32"):
+        grid, params, f = configs.kink_tube(n)
+        params.precision = PREC_FP32 if prec == "fp32" else 0
+        t0 = time.perf_counter()
+        dev = Device(grid, params, f)
+        del f
+        t_up = time.perf_counter() - t0
+        dev.boundary_conditions()
+        dev.eta_calc()
+        en0, divb0 = dev.energy_account(with_divb=True)
+        heat0 = 0.0
+        step = 0
+        for _ in range(warmup):
+            step += 1
+            dev.step_info(step, with_energy=True)
+        dev.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step += 1
+            info = dev.step_info(step, with_energy=True)
+        dev.synchronize()
+        el = time.perf_counter() - t0
+        ncell = grid.n[0] * grid.n[1] * grid.n[2]
+        e1 = (info.en_b, info.en_ke, info.en_int)
+        out[prec] = {"value": ncell * steps / el, "unit": "cell-updates/s", "ms_per_step": 1e3 * el / steps,
+                     "rkl2_stages": info.n_s_stages, "resistive_substeps": info.substeps, "upload_s": round(t_up, 1),
+                     "energy_start": {"magnetic": en0[0], "kinetic": en0[1], "internal": en0[2]},
+                     "energy_end": {"magnetic": e1[0], "kinetic": e1[1], "internal": e1[2]},
+                     "total_energy_drift": (sum(e1) - sum(en0)) / sum(en0),
+                     "visc_heating": info.total_visc_heating - heat0, "ohmic_heating": info.total_ohmic_heating,
+                     "max_divb_dx": info.max_divb, "max_divb_dx_start": divb0}
+        ends[prec] = dev.fields()
+        dev.close()
+    out["fp32"]["field_rel_l2_vs_fp64"] = _vector_errors(ends["fp32"], ends["fp64"])
+    out["fp32"]["speedup_vs_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
+    return out
 
 
 def conduction_leg(n, seed=1, kappa_0=0.5, s_target=7, rows_per_state=17000, train_steps=6000):
@@ -296,6 +366,9 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage run")
     ap.add_argument("--no-conduction", action="store_true",
                     help="skip the conduction leg (RKL2 vs the learned surrogate)")
+    ap.add_argument("--no-kink", action="store_true", help="skip the twisted flux tube leg")
+    ap.add_argument("--kink-n", type=int, default=512, help="flux tube size n (grid n x n x 2n)")
+    ap.add_argument("--kink-steps", type=int, default=8)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy and exit (counter calibration for HBM traffic; "
                          "run it as its own process under the same counters)")
@@ -348,12 +421,14 @@ def main():
         fp32["speedup_vs_fp64"] = fp32["value"] / cups
 
     cond = None if args.no_conduction else conduction_leg(n)
+    kink = None if args.no_kink else kink_leg(args.kink_n, 2, args.kink_steps)
 
     cpu = None
     if not args.no_cpu:
         rate, secs = cpu_baseline(args.cpu_n, args.cpu_steps)
         cpu = {"value": rate, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-               "sample": f"CPU restatement (oracle/), Harris sheet {args.cpu_n}^3 FP64, "
+               "sample": f"this repository's C restatement of the reference (oracle/, one thread; the reference's "
+                         f"own MPI build needs the absent SDF library), Harris sheet {args.cpu_n}^3 FP64, "
                          f"{args.cpu_steps} steps after one untimed step, {secs:.1f} s"}
 
     out = {
@@ -370,7 +445,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded Harris-sheet initial condition)",
         "config": {"workload": f"resistive Harris-sheet reconnection {n}^3 FP64, shock viscosity on, "
-                               "eta = 1e-3 + 1e-2 above |J| = 1 (lagrangian_step + eulerian_remap + eta_calc)",
+                               "eta = 1e-3 + 1e-2 above |J| = 1 (lagrangian_step + eulerian_remap + eta_calc "
+                               "+ energy_account per step)",
                    "n": [n, n, n], "resistive_substeps": sc.last_substeps, "rkl2_stages": sc.n_s_stages,
                    "parallelism": "single GPU"},
         "roofline": roofline,
@@ -378,6 +454,7 @@ def main():
         "cpu_baseline": cpu,
         "fp32": fp32,
         "conduction": cond,
+        "kink": kink,
         "kernels": top,
         "diagnostics": {"energy": en, "max_divb_dx": divb, "dt": sc.dt},
     }

@@ -39,16 +39,16 @@ class Device:
         self._g = grid_struct(grid)
         self._p = params
         check(self.lib, self.lib.lare_dev_init(C.byref(self._g), C.byref(self._p)), "lare_dev_init")
-        self._f = fields.copy()
-        st = self._f.to_struct(with_visc3=bool(params.use_viscous_damping))
+        # the library copies the host arrays during the call; no host copy
+        # is kept (at 512 x 512 x 1024 one field set is 26 GB)
+        st = fields.to_struct(with_visc3=bool(params.use_viscous_damping))
         check(self.lib, self.lib.lare_dev_upload(C.byref(st)), "lare_dev_upload")
         Device._open = self
 
     def upload(self, fields):
         """Replace the device state with new host fields (same grid and
         parameters; the surrogate's parameters and rows are kept)."""
-        self._f = fields.copy()
-        st = self._f.to_struct(with_visc3=bool(self._p.use_viscous_damping))
+        st = fields.to_struct(with_visc3=bool(self._p.use_viscous_damping))
         check(self.lib, self.lib.lare_dev_upload(C.byref(st)), "lare_dev_upload")
 
     def close(self):
