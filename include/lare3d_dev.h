@@ -114,6 +114,22 @@ typedef struct lare_step_info {
   int32_t with_energy;     /* 1 if the energy fields are set              */
 } lare_step_info;
 
+/* Inputs of potential_field (initial_conditions.f90:114-296): red-black
+ * SOR for the scalar potential phi of a current-free field, B = -grad phi,
+ * scaled so that the largest bz on the lower z boundary is 1. */
+typedef struct lare_potential {
+  double omega;            /* over-relaxation w (:131, 1.6)                */
+  double fractional_error; /* stop when max |residual| < this (:132, 5e-5) */
+  double dx1, dy1, dz1;    /* (n_global / length)^2 per axis (:126-128)    */
+  const double *flux0;     /* nx*ny host values, x fastest: the lower
+                              boundary sets phi(ix,iy,0) = phi(ix,iy,1) +
+                              flux0[(ix-1) + nx*(iy-1)] (:285-293, where
+                              the reference's example is amp*dzc(1)*
+                              (1 - tanh((r - radius)/0.2)))             */
+  int32_t max_iterations;  /* iteration limit (:135, 1000000)              */
+  int32_t pad_;
+} lare_potential;
+
 /* Initialise the device state for one grid (allocates all device memory).
  * Replaces the allocation part of mpi_initialise (mpi_routines.F90:295-334)
  * and grid() for the device copy.  Re-initialising frees the old state. */
@@ -170,6 +186,14 @@ int lare_dev_energy_account_noalloc(const lare_grid *grid, const lare_fields *de
 /* One full iteration of the lare3d.f90:82-88 loop body:
  * lagrangian_step, eulerian_remap(step), eta_calc.  Returns dt and time. */
 int lare_dev_step(int32_t step, double *dt, double *time);
+
+/* initial_conditions.f90:114 potential_field on the device: replaces the
+ * face B of the loaded state by the scaled potential field.  Needs a
+ * BC_USER z axis (the flux patch sits on the lower z boundary) and
+ * nz <= ny + 2 (the reference's search for the largest bz runs over
+ * bz(1:nx, 1:nz, 0), :221).  *iterations receives the SOR iterations done,
+ * *residual the max |residual| of the last one, *bz_max the scale. */
+int lare_dev_potential_field(const lare_potential *pf, int32_t *iterations, double *residual, double *bz_max);
 
 /* lare_dev_step, then the scalars the driver reads after a step (dt, dtr,
  * time, heating totals, stage and substep counts, surrogate counts) and,

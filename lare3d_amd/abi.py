@@ -42,6 +42,12 @@ class LareStepInfo(C.Structure):
                 ("sur_accepted", C.c_int32), ("sur_rejected", C.c_int32), ("with_energy", C.c_int32)]
 
 
+class LarePotential(C.Structure):
+    _fields_ = [("omega", C.c_double), ("fractional_error", C.c_double), ("dx1", C.c_double),
+                ("dy1", C.c_double), ("dz1", C.c_double), ("flux0", C.POINTER(C.c_double)),
+                ("max_iterations", C.c_int32), ("pad_", C.c_int32)]
+
+
 class LareFields(C.Structure):
     _fields_ = [(n, _dp) for n in ("rho", "energy", "vx", "vy", "vz", "bx", "by", "bz",
                                    "visc3", "eta", "visc_dep", "ohmic_dep")]
@@ -78,6 +84,7 @@ DEVICE_SYMBOLS = {
     "lare_dev_energy_account": [_dp, _dp],
     "lare_dev_step": [C.c_int32, _dp, _dp],
     "lare_dev_step_info": [C.c_int32, C.c_int32, C.POINTER(LareStepInfo)],
+    "lare_dev_potential_field": [C.POINTER(LarePotential), C.POINTER(C.c_int32), _dp, _dp],
     "lare_dev_energy_account_noalloc": [C.POINTER(LareGrid), C.POINTER(LareFields), _dp, _dp],
     "lare_dev_profile": [C.c_int32],
     "lare_dev_kernel_stats": [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), _dp, C.c_int32],

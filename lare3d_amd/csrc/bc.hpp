@@ -17,7 +17,8 @@ enum BcOp {
   BC_MIR_CELL,   // 0 <- 1, -1 <- 2, n+1 <- n, n+2 <- n-1     boundary.f90:134-160 etc.
   BC_MIR_FACE,   // -1 <- 1, -2 <- 2, n+1 <- n-1, n+2 <- n-2  boundary.f90:68-69,77-78
   BC_ZERO_NODE,  // -2..0 = 0, n..n+2 = 0                      boundary.f90:260-292
-  BC_DM_CLAMP    // dm(n+1) <- dm(n), dm(-1) <- dm(0) over transverse 0..n+1 (xremap.f90:906-915)
+  BC_DM_CLAMP,   // dm(n+1) <- dm(n), dm(-1) <- dm(0) over transverse 0..n+1 (xremap.f90:906-915)
+  BC_PHI_NEU     // 0 <- 1, -1 <- 1, n+1 <- n, n+2 <- n     phi_mpi walls (initial_conditions.f90:276-291)
 };
 
 #define BC_MAXF 8
@@ -72,6 +73,12 @@ __device__ __forceinline__ void bc_line(T *a, int op, int base, int st, int n, b
       a[base + n * st] = (T)0;
       a[base + (n + 1) * st] = (T)0;
       a[base + (n + 2) * st] = (T)0;
+      break;
+    case BC_PHI_NEU:
+      a[base + 0 * st] = a[base + 1 * st];
+      a[base - 1 * st] = a[base + 1 * st];
+      a[base + (n + 1) * st] = a[base + n * st];
+      a[base + (n + 2) * st] = a[base + n * st];
       break;
     case BC_DM_CLAMP:
       if (dm_ok) {
@@ -148,6 +155,11 @@ __device__ __forceinline__ BcMove bc_slot(int op, int n, int slot, bool dm_ok) {
     case BC_ZERO_NODE: {
       const int d[6] = {-2, -1, 0, n, n + 1, n + 2};
       if (slot < 6) { dst = d[slot]; zero = true; act = true; }
+      break;
+    }
+    case BC_PHI_NEU: {
+      const int d[4] = {0, -1, n + 1, n + 2}, r[4] = {1, 1, n, n};
+      if (slot < 4) { dst = d[slot]; src = r[slot]; act = true; }
       break;
     }
     case BC_DM_CLAMP: {

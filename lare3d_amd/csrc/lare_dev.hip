@@ -95,6 +95,9 @@ int lare_dev_step(int32_t step, double *dt, double *time) { return FWD(api_step(
 int lare_dev_step_info(int32_t step, int32_t with_energy, lare_step_info *info) {
   return FWD(api_step_info(step, with_energy, info));
 }
+int lare_dev_potential_field(const lare_potential *pf, int32_t *iterations, double *residual, double *bz_max) {
+  return FWD(api_potential_field(pf, iterations, residual, bz_max));
+}
 int lare_dev_energy_account_noalloc(const lare_grid *grid, const lare_fields *dev_fields, double en[3], double *divb) {
   return api_energy_account_noalloc(grid, dev_fields, en, divb);
 }

@@ -14,6 +14,7 @@ typedef LARE_REAL real;
 #include "lagran.hpp"
 #include "remap.hpp"
 #include "surrogate.hpp"
+#include "initial.hpp"
 
 // ----------------------------------------------------------------------------
 // kernel timing with HIP events on the library stream
@@ -1644,5 +1645,89 @@ int api_step_info(int32_t step, int32_t with_energy, lare_step_info *o) {
   o->sur_accepted = SU.accepted;
   o->sur_rejected = SU.rejected;
   o->with_energy = with_energy ? 1 : 0;
+  return 0;
+}
+
+// initial_conditions.f90:114-296 potential_field.  phi lives in the FP64
+// scratch array F, the flux patch in F2.
+int api_potential_field(const lare_potential *pf, int32_t *iterations, double *residual, double *bz_max) {
+  REQUIRE_LOADED();
+  if (!pf || !pf->flux0) return fail(LARE_E_ARG, "potential_field: NULL argument");
+  const int N0 = nx(), N1 = ny(), N2 = nz();
+  if (S.per[2]) return fail(LARE_E_ARG, "potential_field: the z axis must be BC_USER (flux patch below)");
+  if (N2 > N1 + 2)
+    return fail(LARE_E_ARG, "potential_field: nz = %d > ny + 2 = %d (the reference reads bz(1:nx, 1:nz, 0))", N2,
+                N1 + 2);
+  if (pf->max_iterations < 1) return fail(LARE_E_ARG, "potential_field: max_iterations < 1");
+  double *phi = S.F, *flux0 = S.F2;
+  HIPCHK(hipStreamSynchronize(S.stream));
+  HIPCHK(hipMemcpy(flux0, pf->flux0, sizeof(double) * (size_t)N0 * N1, hipMemcpyHostToDevice));
+  LAUNCH_BOX("fill", k_fill64, full_box(), phi, 0.0);
+  LAUNCH("sor_reset", k_sor_reset, dim3(1), dim3(64), S.dscal);
+  const int ncol = (N0 + 1) / 2;
+  const dim3 gs((unsigned)(((long)ncol * N1 + BX * BY - 1) / (BX * BY)));
+  const Box zb = mkbox(-1, N0 + 2, -1, N1 + 2, 0, 0);
+  const double w = pf->omega, dx1 = pf->dx1, dy1 = pf->dy1, dz1 = pf->dz1;
+  auto phi_bcs = [&]() -> int {
+    // phi_mpi: exchanges on periodic axes, the z boundaries, then the
+    // zero-gradient x and y walls
+    for (int ax = 0; ax < 3; ++ax) {
+      if (!S.per[ax]) continue;
+      BcSet b = {};
+      b.nf = 1;
+      b.f[0] = phi;
+      b.op[0] = BC_PER_CELL;
+      b.wide[0] = 1;
+      int rc = launch_bc_axis(b, ax, "bc_phi");
+      if (rc) return rc;
+    }
+    LAUNCH("phi_zbc", k_phi_zbc, box_grid(zb), box_block(), S.g, zb, phi, flux0);
+    for (int ax = 0; ax < 2; ++ax) {
+      if (S.per[ax]) continue;
+      BcSet b = {};
+      b.nf = 1;
+      b.f[0] = phi;
+      b.op[0] = BC_PHI_NEU;
+      b.wide[0] = 1;
+      int rc = launch_bc_axis(b, ax, "bc_phi");
+      if (rc) return rc;
+    }
+    return 0;
+  };
+  const int batch = 32;
+  double h[5] = {0, 0, 0, 0, 0};
+  for (;;) {
+    for (int q = 0; q < batch; ++q) {
+      for (int z1 = 1; z1 <= 2; ++z1) {
+        LAUNCH("sor", k_sor_pass, gs, box_block(), S.g, phi, z1, w, dx1, dy1, dz1, S.dscal + 12, S.partial);
+        int rc = final_reduce(gs, 1, 1, 9 + z1, 0);
+        if (rc) return rc;
+        rc = phi_bcs();
+        if (rc) return rc;
+      }
+      LAUNCH("sor_check", k_sor_check, dim3(1), dim3(64), S.dscal, pf->fractional_error, pf->max_iterations);
+    }
+    int rc = read_scalars(10, 5, h);
+    if (rc) return rc;
+    if (h[2] != 0.0) break;
+  }
+  // B = -grad phi, bfield_bcs, scale by the largest bz below, bfield_bcs
+  LAUNCH_BOX("phi_to_b", k_phi_to_b, full_box(), phi, S.bx, S.by, S.bz);
+  int rc = bfield_bcs(S.bx, S.by, S.bz);
+  if (rc) return rc;
+  const Box lb = mkbox(1, N0, 1, N2, 0, 0);
+  const dim3 gb = box_grid(lb);
+  LAUNCH("bz_lower", k_bz_lower, gb, box_block(), S.g, lb, S.bz, S.partial);
+  rc = final_reduce(gb, 1, 1, 15, 0);
+  if (rc) return rc;
+  LAUNCH_BOX("scale_b", k_scale_b, full_box(), S.bx, S.by, S.bz, S.dscal);
+  rc = bfield_bcs(S.bx, S.by, S.bz);
+  if (rc) return rc;
+  double bm;
+  rc = read_scalars(15, 1, &bm);
+  if (rc) return rc;
+  if (iterations) *iterations = (int32_t)h[3];
+  if (residual) *residual = h[4];
+  if (bz_max) *bz_max = -bm;
   return 0;
 }

@@ -110,6 +110,15 @@ class Device:
         check(self.lib, self.lib.lare_dev_step_info(step, 1 if with_energy else 0, C.byref(info)), f"step {step}")
         return info
 
+    def potential_field(self, pf):
+        """initial_conditions.f90:114 potential_field on the device (pf from
+        lare3d_amd.initial.potential_inputs); returns (iterations, residual,
+        bz_max)."""
+        it, res, bm = C.c_int32(), C.c_double(), C.c_double()
+        check(self.lib, self.lib.lare_dev_potential_field(C.byref(pf), C.byref(it), C.byref(res), C.byref(bm)),
+              "potential_field")
+        return it.value, res.value, bm.value
+
     def synchronize(self):
         check(self.lib, self.lib.lare_dev_synchronize(), "synchronize")
 

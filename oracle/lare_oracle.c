@@ -1538,6 +1538,101 @@ void orc_step(orc_state *o, int step, double *dt, double *time) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* potential_field (initial_conditions.f90:114-296)                          */
+/* ------------------------------------------------------------------------ */
+static void xfer(orc_state *o, double *a, double *f, int lx, int ly, int lz, int to_box);
+/* phi_mpi (:238-294): exchanges on periodic axes, then the flux patch on the
+ * lower z boundary, phi = 0 above the upper one, and zero-gradient x and y
+ * walls, in that order. */
+static void phi_bcs(orc_state *o, double *phi, const double *flux0) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  for (int ax = 0; ax < 3; ++ax) exchange(o, phi, ax, 0);
+  if (!o->per[2]) {
+    for (int iy = 1; iy <= ny; ++iy)
+      for (int ix = 1; ix <= nx; ++ix) {
+        A(phi, ix, iy, 0) = A(phi, ix, iy, 1) + flux0[(ix - 1) + (long)nx * (iy - 1)];
+        A(phi, ix, iy, -1) = A(phi, ix, iy, 0);
+      }
+    for (int iy = -1; iy <= ny + 2; ++iy)
+      for (int ix = -1; ix <= nx + 2; ++ix) {
+        A(phi, ix, iy, nz + 1) = 0.0;
+        A(phi, ix, iy, nz + 2) = 0.0;
+      }
+  }
+  if (!o->per[0]) {
+    copy_plane(o, phi, 0, 0, 1); copy_plane(o, phi, 0, -1, 1);
+    copy_plane(o, phi, 0, nx + 1, nx); copy_plane(o, phi, 0, nx + 2, nx);
+  }
+  if (!o->per[1]) {
+    copy_plane(o, phi, 1, 0, 1); copy_plane(o, phi, 1, -1, 1);
+    copy_plane(o, phi, 1, ny + 1, ny); copy_plane(o, phi, 1, ny + 2, ny);
+  }
+}
+
+int orc_potential_field(orc_state *o, const lare_potential *pf, int *iterations, double *residual, double *bz_max,
+                        double *phi_out) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  /* bz(1:nx, 1:nz, 0) below is inside the array only for nz <= ny + 2 */
+  if (nz > ny + 2 || o->per[2]) return -1;
+  double *phi = cellbox(o);
+  const double dx1 = pf->dx1, dy1 = pf->dy1, dz1 = pf->dz1, w = pf->omega;
+  double error = 0.0;
+  int loop, done = 0;
+  for (loop = 1; loop <= pf->max_iterations; ++loop) {
+    double errmax = 0.0;
+    int z1 = 1;
+    for (int redblack = 1; redblack <= 2; ++redblack) {
+      for (int iz = 1; iz <= nz; ++iz) {
+        int x1 = z1;
+        for (int iy = 1; iy <= ny; ++iy) {
+          for (int ix = x1; ix <= nx; ix += 2) {
+            double r = (((A(phi, ix + 1, iy, iz) - 2.0 * A(phi, ix, iy, iz)) + A(phi, ix - 1, iy, iz)) * dx1 +
+                        ((A(phi, ix, iy + 1, iz) - 2.0 * A(phi, ix, iy, iz)) + A(phi, ix, iy - 1, iz)) * dy1 +
+                        ((A(phi, ix, iy, iz + 1) - 2.0 * A(phi, ix, iy, iz)) + A(phi, ix, iy, iz - 1)) * dz1) /
+                       2.0 / (dx1 + dy1 + dz1);
+            A(phi, ix, iy, iz) = A(phi, ix, iy, iz) + w * r;
+            errmax = dmax(errmax, fabs(r));
+          }
+          x1 = 3 - x1;
+        }
+      }
+      phi_bcs(o, phi, pf->flux0);
+      z1 = 3 - z1;
+    }
+    error = errmax;
+    if (error < pf->fractional_error) { done = 1; break; }
+  }
+  *iterations = done ? loop : pf->max_iterations;
+  *residual = error;
+  for (int iz = 1; iz <= nz; ++iz)
+    for (int iy = 1; iy <= ny; ++iy)
+      for (int ix = 0; ix <= nx; ++ix) A(o->bx, ix, iy, iz) = -(A(phi, ix + 1, iy, iz) - A(phi, ix, iy, iz)) / DXC(ix);
+  for (int iz = 1; iz <= nz; ++iz)
+    for (int iy = 0; iy <= ny; ++iy)
+      for (int ix = 1; ix <= nx; ++ix) A(o->by, ix, iy, iz) = -(A(phi, ix, iy + 1, iz) - A(phi, ix, iy, iz)) / DYC(iy);
+  for (int iz = 0; iz <= nz; ++iz)
+    for (int iy = 1; iy <= ny; ++iy)
+      for (int ix = 1; ix <= nx; ++ix) A(o->bz, ix, iy, iz) = -(A(phi, ix, iy, iz + 1) - A(phi, ix, iy, iz)) / DZC(iz);
+  bfield_bcs(o);
+  /* MAXVAL(bz(1:nx,1:nz,0)): the y range is 1:nz in the reference (:221) */
+  double bm = -LARGEST;
+  for (int iy = 1; iy <= nz; ++iy)
+    for (int ix = 1; ix <= nx; ++ix) bm = dmax(bm, A(o->bz, ix, iy, 0));
+  *bz_max = bm;
+  double *b[3] = {o->bx, o->by, o->bz};
+  int lo[3][3] = {{-2, -1, -1}, {-1, -2, -1}, {-1, -1, -2}};
+  for (int c = 0; c < 3; ++c)
+    for (int iz = lo[c][2]; iz <= nz + 2; ++iz)
+      for (int iy = lo[c][1]; iy <= ny + 2; ++iy)
+        for (int ix = lo[c][0]; ix <= nx + 2; ++ix) A(b[c], ix, iy, iz) = A(b[c], ix, iy, iz) / bm;
+  bfield_bcs(o);
+  /* phi(-1:nx+2, -1:ny+2, -1:nz+2), Fortran order, for the tests */
+  if (phi_out) xfer(o, phi, phi_out, -1, -1, -1, 0);
+  drop(&phi);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* state management                                                          */
 /* ------------------------------------------------------------------------ */
 orc_state *orc_create(const lare_grid *g, const lare_params *p) {

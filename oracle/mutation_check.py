@@ -21,7 +21,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "lare_oracle.c")
 CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-shared"]
-PIN_TESTS = ["tests/test_pin_hydro.py", "tests/test_pin_transport.py", "tests/test_pin_oracle3d.py"]
+PIN_TESTS = ["tests/test_pin_hydro.py", "tests/test_pin_transport.py", "tests/test_pin_oracle3d.py",
+             "tests/test_pin_potential.py"]
 
 # (name, what it breaks, [(old, new), ...]); each `old` occurs exactly once
 MUTANTS = [
@@ -65,6 +66,15 @@ MUTANTS = [
     ("M16", "shock-viscosity threshold on the current cell width of the other axis (lagran.F90:825)",
      [("A(o->alpha1, ix, iy, iz) = edge_viscosity(o, cs_v, dvdots, DXB(ix), i1, j1, k1, i2, j2, k2);",
        "A(o->alpha1, ix, iy, iz) = edge_viscosity(o, cs_v, dvdots, DYB(iy), i1, j1, k1, i2, j2, k2);")]),
+    ("M17", "potential_field: colouring in three dimensions (x1 restarted from z1 and iz, :141-143)",
+     [("      for (int iz = 1; iz <= nz; ++iz) {\n        int x1 = z1;",
+       "      for (int iz = 1; iz <= nz; ++iz) {\n        int x1 = (iz % 2) ? z1 : 3 - z1;")]),
+    ("M18", "potential_field: flux patch subtracted (:285-288)",
+     [("A(phi, ix, iy, 0) = A(phi, ix, iy, 1) + flux0[", "A(phi, ix, iy, 0) = A(phi, ix, iy, 1) - flux0[")]),
+    ("M19", "potential_field: residual without the 1/2 (:152-155)",
+     [("                       2.0 / (dx1 + dy1 + dz1);", "                       1.0 / (dx1 + dy1 + dz1);")]),
+    ("M20", "potential_field: upper boundary zero-gradient instead of phi = 0 (:270-273)",
+     [("        A(phi, ix, iy, nz + 1) = 0.0;", "        A(phi, ix, iy, nz + 1) = A(phi, ix, iy, nz);")]),
 ]
 
 

@@ -7,7 +7,7 @@ import ctypes as C
 import os
 import subprocess
 
-from lare3d_amd.abi import LareFields, LareGrid, LareParams, LareScalars
+from lare3d_amd.abi import LareFields, LareGrid, LareParams, LarePotential, LareScalars
 from lare3d_amd.fields import Fields
 from lare3d_amd.grid import grid_struct
 
@@ -45,6 +45,7 @@ def _load():
         "orc_pow15": ([C.c_double], C.c_double),
         "orc_divb": ([vp], C.c_double),
         "orc_step": ([vp, C.c_int, dp, dp], None),
+        "orc_potential_field": ([vp, C.POINTER(LarePotential), C.POINTER(C.c_int), dp, dp, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -125,3 +126,13 @@ class Oracle:
         dt, t = C.c_double(), C.c_double()
         lib().orc_step(self.h, step, C.byref(dt), C.byref(t))
         return dt.value, t.value
+
+    def potential_field(self, pf, want_phi=False):
+        """initial_conditions.f90:114 potential_field; returns (iterations,
+        residual, bz_max), plus phi(-1:n+2)^3 when want_phi."""
+        import numpy as np
+        it, res, bm = C.c_int(), C.c_double(), C.c_double()
+        phi = np.zeros(tuple(n + 4 for n in self.n), order="F") if want_phi else None
+        out = phi.ctypes.data_as(C.POINTER(C.c_double)) if want_phi else None
+        lib().orc_potential_field(self.h, C.byref(pf), C.byref(it), C.byref(res), C.byref(bm), out)
+        return (it.value, res.value, bm.value) + ((phi,) if want_phi else ())
