@@ -77,6 +77,13 @@ typedef struct lare_params {
   int32_t use_viscous_damping;
   int32_t restart;
   int32_t precision;       /* LARE_PREC_FP64 or LARE_PREC_FP32            */
+  double alpha_av;         /* cooling_term averaging weight (control.f90:206) */
+  double va_max;           /* Boris Alfven speed limit (control.f90:184)  */
+  int32_t cooling_term;    /* lagran.F90:464-476, :1290-1302              */
+  int32_t boris;           /* lagran.F90:1092-1108, :375-377, :556        */
+  int32_t shock_limiter;   /* the SHOCKLIMITER variant of edge_viscosity
+                              (lagran.F90:801-803, :831-845)              */
+  int32_t shock_compression; /* the SHOCKCOMPRESSION variant (:807-808)   */
 } lare_params;
 
 /* Field pointers (Fortran arrays, see the array convention above).

@@ -31,7 +31,10 @@ class LareParams(C.Structure):
                 ("dt_from_restart", C.c_double),
                 ("resistive_mhd", C.c_int32), ("conduction", C.c_int32),
                 ("heat_flux_limiter", C.c_int32), ("use_viscous_damping", C.c_int32),
-                ("restart", C.c_int32), ("precision", C.c_int32)]
+                ("restart", C.c_int32), ("precision", C.c_int32),
+                ("alpha_av", C.c_double), ("va_max", C.c_double),
+                ("cooling_term", C.c_int32), ("boris", C.c_int32),
+                ("shock_limiter", C.c_int32), ("shock_compression", C.c_int32)]
 
 
 class LareStepInfo(C.Structure):

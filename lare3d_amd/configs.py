@@ -35,6 +35,12 @@ def default_params(**kw):
     p.use_viscous_damping = 0
     p.restart = 0
     p.precision = PREC_FP64
+    p.alpha_av = 0.05
+    p.va_max = 10.0
+    p.cooling_term = 0
+    p.boris = 0
+    p.shock_limiter = 0
+    p.shock_compression = 0
     for k, v in kw.items():
         setattr(p, k, v)
     return p

@@ -25,7 +25,13 @@
 struct LagArgs {
   double gamma, visc1, visc2_norm, dt, dt2;
   double eta_bg, eta0, j_max;
+  double va_max2;   // boris: va_max^2 (setup.F90:80)
+  double alpha_av;  // cooling_term weight
   int resistive;
+  int boris;        // compute gamma_boris in set_dt (lagran.F90:1104-1108)
+  int write_gb;     // 0 when set_dt returns early (first step of a restart)
+  int limiter;      // the SHOCKLIMITER variant of edge_viscosity
+  int compression;  // the SHOCKCOMPRESSION variant
 };
 
 // 8-cell sum around node (i,j,k) in the reference order (lagran.F90:105-117)
@@ -41,11 +47,12 @@ __device__ __forceinline__ double cv_sum8(const Geo &g, int i, int j, int k) {
          g.cv(i + 1, j, k + 1) + g.cv(i, j + 1, k + 1) + g.cv(i + 1, j + 1, k + 1);
 }
 
-// fast magnetosonic speed of shock_viscosity (:554-556) in cell c
+// fast magnetosonic speed of shock_viscosity (:554-556) in cell c; gb is
+// gamma_boris of the previous set_dt, or NULL for 1 (boris off)
 __device__ __forceinline__ double cell_cs(const Geo &g, const real *__restrict__ rho,
                                           const real *__restrict__ energy, const double *__restrict__ bx,
-                                          const double *__restrict__ by, const double *__restrict__ bz, int c,
-                                          double gamma) {
+                                          const double *__restrict__ by, const double *__restrict__ bz,
+                                          const real *__restrict__ gb, int c, double gamma) {
   const double r = rho[c];
   const double rmin = dmax(r, NONE_ZERO);
   const double b1x = (bx[c] + bx[c - 1]) * 0.5;
@@ -53,7 +60,8 @@ __device__ __forceinline__ double cell_cs(const Geo &g, const real *__restrict__
   const double b1z = (bz[c] + bz[c - g.sz]) * 0.5;
   const double b2 = b1x * b1x + b1y * b1y + b1z * b1z;
   const double p = (gamma - 1.0) * r * (energy[c] - 0.0);
-  return sqrt((gamma * p + 1.0 * b2) / rmin);
+  const double g_b = gb ? (double)gb[c] : 1.0;
+  return sqrt((gamma * p + g_b * b2) / rmin);
 }
 
 // rho and the fast speed of the four cells (i,j), (i+1,j), (i,j+1), (i+1,j+1)
@@ -63,14 +71,14 @@ struct CellQuad {
 };
 __device__ __forceinline__ CellQuad cell_quad(const Geo &g, const real *__restrict__ rho,
                                               const real *__restrict__ energy, const double *__restrict__ bx,
-                                              const double *__restrict__ by, const double *__restrict__ bz, int c,
-                                              double gamma) {
+                                              const double *__restrict__ by, const double *__restrict__ bz,
+                                              const real *__restrict__ gb, int c, double gamma) {
   CellQuad q;
   const int off[4] = {0, 1, g.sy, 1 + g.sy};
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
     q.r[p] = rho[c + off[p]];
-    q.cs[p] = cell_cs(g, rho, energy, bx, by, bz, c + off[p], gamma);
+    q.cs[p] = cell_cs(g, rho, energy, bx, by, bz, gb, c + off[p], gamma);
   }
   return q;
 }
@@ -82,14 +90,14 @@ __device__ __forceinline__ CellQuad cell_quad(const Geo &g, const real *__restri
 // (i,j+1,k), (i+1,j+1,k), then the same four at k+1.
 __global__ __launch_bounds__(BX * BY) void k_vertex_prep(Geo g, Box b, const real *__restrict__ rho, const real *__restrict__ energy,
                               const double *__restrict__ bx, const double *__restrict__ by,
-                              const double *__restrict__ bz, real *__restrict__ rho_v, real *__restrict__ cs_v,
-                              double gamma) {
+                              const double *__restrict__ bz, const real *__restrict__ gb, real *__restrict__ rho_v,
+                              real *__restrict__ cs_v, double gamma) {
   ZCOLUMN(b, i, j, k0, k1);
   const int Z = g.sz;
-  CellQuad lo = cell_quad(g, rho, energy, bx, by, bz, g.idx(i, j, k0), gamma);
+  CellQuad lo = cell_quad(g, rho, energy, bx, by, bz, gb, g.idx(i, j, k0), gamma);
   for (int k = k0; k <= k1; ++k) {
     const int c = g.idx(i, j, k);
-    const CellQuad hi = cell_quad(g, rho, energy, bx, by, bz, c + Z, gamma);
+    const CellQuad hi = cell_quad(g, rho, energy, bx, by, bz, gb, c + Z, gamma);
     double cvs[8];
     const int di[8] = {0, 1, 0, 1, 0, 1, 0, 1}, dj[8] = {0, 0, 1, 1, 0, 0, 1, 1}, dk[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 #pragma unroll
@@ -121,10 +129,26 @@ __device__ __forceinline__ NodeV node_v(const real *__restrict__ vx, const real 
   return n;
 }
 
-// edge_viscosity (lagran.F90:795-855) between nodes 1 and 2
+// Velocity of a node (the outer neighbours of an edge for the limiter).
+struct Vel3 {
+  double x, y, z;
+};
+__device__ __forceinline__ Vel3 vel3(const real *__restrict__ vx, const real *__restrict__ vy,
+                                     const real *__restrict__ vz, int c) {
+  Vel3 v = {vx[c], vy[c], vz[c]};
+  return v;
+}
+
+// edge_viscosity (lagran.F90:795-855) between nodes 1 and 2.  With the
+// SHOCKLIMITER variant (a.limiter) the velocities of node 0 (before node 1
+// along the edge) and node 3 (after node 2) and the widths dxm, dxp of the
+// neighbouring cells give the limiter psi (:831-845); with SHOCKCOMPRESSION
+// (a.compression) an expanding edge gets no viscosity (:807-808).
 __device__ __forceinline__ double edge_visc(const NodeV &n1, const NodeV &n2, double dvdots, double dx,
-                                            const LagArgs &a) {
-  dvdots = -fabs(dvdots);
+                                            const LagArgs &a, const Vel3 &v0 = Vel3{0, 0, 0},
+                                            const Vel3 &v3 = Vel3{0, 0, 0}, double dxm = 1.0, double dxp = 1.0) {
+  if (a.compression) dvdots = dmin(0.0, dvdots);
+  else dvdots = -fabs(dvdots);
   const double r1 = n1.r, r2 = n2.r;
   const double rho_edge = 2.0 * r1 * r2 / (r1 + r2);
   const double cs_edge = dmin(n1.cs, n2.cs);
@@ -133,9 +157,24 @@ __device__ __forceinline__ double edge_visc(const NodeV &n1, const NodeV &n2, do
   const double dvz = n1.vz - n2.vz;
   const double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
   const double dv = sqrt(dv2);
-  const double psi = 0.0;
+  double psi = 0.0;
   if (dv * a.dt / dx < 1.0e-14) dvdots = 0.0;
   else dvdots = dvdots / dv;
+  if (a.limiter) {
+    const double dvxm = v0.x - (double)n1.vx, dvxp = (double)n2.vx - v3.x;
+    const double dvym = v0.y - (double)n1.vy, dvyp = (double)n2.vy - v3.y;
+    const double dvzm = v0.z - (double)n1.vz, dvzp = (double)n2.vz - v3.z;
+    double rl, rr;
+    if (dv * a.dt / dx < 1.0e-14) {
+      rl = 1.0;
+      rr = 1.0;
+    } else {
+      rl = (dvxp * dvx + dvyp * dvy + dvzp * dvz) * dx / (dxp * dv2);
+      rr = (dvxm * dvx + dvym * dvy + dvzm * dvz) * dx / (dxm * dv2);
+    }
+    psi = dmin(dmin(dmin(0.5 * (rr + rl), 2.0 * rl), 2.0 * rr), 1.0);
+    psi = dmax(0.0, psi);
+  }
   const double vn = a.visc2_norm;
   const double vc = a.visc1 * cs_edge;
   const double q_k_bar = rho_edge * (vn * dv + sqrt(vn * vn * dv2 + vc * vc));
@@ -159,12 +198,28 @@ __global__ __launch_bounds__(BX * BY) void k_alpha(Geo g, Box b, const real *__r
   for (int k = k0; k <= k1; ++k) {
     const int c = g.idx(i, j, k);
     const NodeV p3 = node_v(vx, vy, vz, rho_v, cs_v, c);
-    if (i >= 0 && i <= nx + 1 && j >= 0 && j <= ny + 2 && k >= 0 && k <= nz + 2)
-      al1[c] = edge_visc(m1, m2, -(m1.vx - m2.vx), g.dxb[i], a);
-    if (i >= -1 && i <= nx + 1 && j >= 0 && j <= ny + 1 && k >= 0 && k <= nz + 2)
-      al2[c] = edge_visc(m2, m3, -(m2.vy - m3.vy), g.dyb[j], a);
-    if (i >= -1 && i <= nx + 1 && j >= -1 && j <= ny + 1 && k >= 0 && k <= nz + 1)
-      al3[c] = edge_visc(m3, p3, -(m3.vz - p3.vz), g.dzb[k], a);
+    const int Z = g.sz;
+    if (i >= 0 && i <= nx + 1 && j >= 0 && j <= ny + 2 && k >= 0 && k <= nz + 2) {
+      if (a.limiter)
+        al1[c] = edge_visc(m1, m2, -(m1.vx - m2.vx), g.dxb[i], a, vel3(vx, vy, vz, c - Z - Y - 2),
+                           vel3(vx, vy, vz, c - Z - Y + 1), g.dxb[i - 1], g.dxb[i + 1]);
+      else
+        al1[c] = edge_visc(m1, m2, -(m1.vx - m2.vx), g.dxb[i], a);
+    }
+    if (i >= -1 && i <= nx + 1 && j >= 0 && j <= ny + 1 && k >= 0 && k <= nz + 2) {
+      if (a.limiter)
+        al2[c] = edge_visc(m2, m3, -(m2.vy - m3.vy), g.dyb[j], a, vel3(vx, vy, vz, c - Z - 2 * Y),
+                           vel3(vx, vy, vz, c - Z + Y), g.dyb[j - 1], g.dyb[j + 1]);
+      else
+        al2[c] = edge_visc(m2, m3, -(m2.vy - m3.vy), g.dyb[j], a);
+    }
+    if (i >= -1 && i <= nx + 1 && j >= -1 && j <= ny + 1 && k >= 0 && k <= nz + 1) {
+      if (a.limiter)
+        al3[c] = edge_visc(m3, p3, -(m3.vz - p3.vz), g.dzb[k], a, vel3(vx, vy, vz, c - 2 * Z), vel3(vx, vy, vz, c + Z),
+                           g.dzb[k - 1], g.dzb[k + 1]);
+      else
+        al3[c] = edge_visc(m3, p3, -(m3.vz - p3.vz), g.dzb[k], a);
+    }
     m1 = node_v(vx, vy, vz, rho_v, cs_v, c - 1 - Y);
     m2 = node_v(vx, vy, vz, rho_v, cs_v, c - Y);
     m3 = p3;
@@ -264,8 +319,8 @@ __global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, Box bx0, const real 
                           const double *__restrict__ by, const double *__restrict__ bz,
                           const real *__restrict__ eta, const real *__restrict__ al1,
                           const real *__restrict__ al2, const real *__restrict__ al3,
-                          real *__restrict__ visc_heat, real *__restrict__ p0, double *__restrict__ partial,
-                          LagArgs a) {
+                          real *__restrict__ visc_heat, real *__restrict__ p0, real *__restrict__ gbo,
+                          double *__restrict__ partial, LagArgs a) {
   // cells 0..n+1, marching in z: node velocities and al1/al2 of plane k
   // become those of plane k-1 / k for the next cell
   PLANE_POS(bx0, i, j)
@@ -304,7 +359,14 @@ __global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, Box bx0, const real 
       if (i <= g.nx && j <= g.ny && k <= g.nz) {
         const double w1 = bx[c] * bx[c] + by[c] * by[c] + bz[c] * bz[c];
         const double rho0 = dmax(r, NONE_ZERO);
-        const double cs2 = (a.gamma * p + 1.0 * w1) / rho0;
+        double g_b = 1.0;
+        if (a.boris) {
+          const double w2 = w1 / rho0;
+          if (w2 >= a.va_max2) g_b = 1.0 / (1.0 + w2 / a.va_max2);
+          if (a.write_gb) gbo[c] = (real)g_b;
+          g_b = (real)g_b;
+        }
+        const double cs2 = (a.gamma * p + g_b * w1) / rho0;
         const double c_visc2 = pv / rho0;
         const double dxb = g.dxb[i], dyb = g.dyb[j], dzb = g.dzb[k];
         const double length = dmin(dmin(dxb, dyb), dzb);
@@ -454,8 +516,9 @@ __global__ __launch_bounds__(64 * RS_ROWS) void k_resist(Geo g, Box b, int tiles
                                                          double *__restrict__ bxo, double *__restrict__ byo,
                                                          double *__restrict__ bzo, real *__restrict__ eta,
                                                          real *__restrict__ energy, real *__restrict__ ohmic_dep,
-                                                         const real *__restrict__ rho, double *__restrict__ partial,
-                                                         double dt, double eta_bg, double eta0, double j_max) {
+                                                         const real *__restrict__ rho, real *__restrict__ curlb_out,
+                                                         double *__restrict__ partial, double dt, double eta_bg,
+                                                         double eta0, double j_max) {
   // EMFs kx, ky, kz and curlb of the tile's nodes, by plane parity
   __shared__ double sk[2][4][RS_ROWS][64];
   __shared__ double red[RS_ROWS];
@@ -498,6 +561,7 @@ __global__ __launch_bounds__(64 * RS_ROWS) void k_resist(Geo g, Box b, int tiles
         const double cb = e * (jx * jx + jy * jy + jz * jz);
         cbr = (real)cb;
         if (out && k >= k0) {
+          if (curlb_out) curlb_out[c] = cbr;
           double w1 = dt * g.dxc[i] * g.dyc[j] * g.dzc[k] * cb;
           if (i == 0 || i == g.nx) w1 = w1 * 0.5;
           if (j == 0 || j == g.ny) w1 = w1 * 0.5;
@@ -546,7 +610,9 @@ __global__ __launch_bounds__(64 * RS_ROWS) void k_resist(Geo g, Box b, int tiles
           ohmic_dep[c] = ohmic_dep[c] + lh;
           e = e + lh;
         }
-        energy[c] = max0(e);
+        // with the cooling_term, k_cool_b takes the cooling off first and
+        // then clamps
+        energy[c] = curlb_out ? e : max0(e);
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) mq[q] = pq[q];
@@ -569,6 +635,24 @@ __global__ __launch_bounds__(BX * BY) void k_energy_max(Geo g, Box b, real *__re
   energy[c] = max0(energy[c]);
 }
 
+// The ohmic part of the cooling_term after a resistive substep
+// (lagran.F90:1290-1302) on cells 1..n, then energy = MAX(energy, 0).  The
+// reference's loop sets no ixm or iym, which keep the last values of the
+// heating loop before it, nx - 1 and ny - 1: the average takes curlb at
+// (i, j, k), (nx-1, j, k), (i, ny-1, k) and (nx-1, ny-1, k).
+__global__ __launch_bounds__(BX * BY) void k_cool_b(Geo g, Box b, const real *__restrict__ curlb,
+                                                    const real *__restrict__ rho, real *__restrict__ cool_b,
+                                                    real *__restrict__ energy, double alpha_av, double dt) {
+  BOX_INDEX(b, i, j, k);
+  const int c = g.idx(i, j, k);
+  const double lh = ((double)curlb[c] + curlb[g.idx(g.nx - 1, j, k)] + curlb[g.idx(i, g.ny - 1, k)] +
+                     curlb[g.idx(g.nx - 1, g.ny - 1, k)]) *
+                    dt / (4.0 * rho[c]);
+  const real ct = alpha_av * lh + (1.0 - alpha_av) * cool_b[c];
+  cool_b[c] = ct;
+  energy[c] = max0(energy[c] - ct);
+}
+
 // predictor momentum (lagran.F90:251-387) with fx/fy/fz_visc (:745-789)
 // recomputed here from alpha and v; B is the half-step (predictor-remapped)
 // field.
@@ -576,8 +660,9 @@ __global__ __launch_bounds__(BX * BY) void k_force(Geo g, Box b, const real *__r
                         const real *__restrict__ vz, const real *__restrict__ pp,
                         const double *__restrict__ bx, const double *__restrict__ by, const double *__restrict__ bz,
                         const real *__restrict__ rho_v, const real *__restrict__ al1,
-                        const real *__restrict__ al2, const real *__restrict__ al3, real *__restrict__ vx1,
-                        real *__restrict__ vy1, real *__restrict__ vz1, double dt2) {
+                        const real *__restrict__ al2, const real *__restrict__ al3, const real *__restrict__ gb,
+                        real *__restrict__ vx1, real *__restrict__ vy1, real *__restrict__ vz1, double dt2) {
+  // gb: gamma_boris of this step's set_dt (lagran.F90:375-377), NULL for 1
   ZCOLUMN(b, i, j, k0, k1);
   const int X = 1, Y = g.sy, Z = g.sz;
   const real *V[3] = {vx, vy, vz};
@@ -637,9 +722,10 @@ __global__ __launch_bounds__(BX * BY) void k_force(Geo g, Box b, const real *__r
     const double bzv = 0.25 * (bz0 + bzy + bzx + bzxy);
     double jx, jy, jz;
     vertex_current_vals(dxc, dyc, dzc, blo, bhi, bz0, bzx, bzy, bzxy, jx, jy, jz);
-    fx = fx + 1.0 * (jy * bzv - jz * byv);
-    fy = fy + 1.0 * (jz * bxv - jx * bzv);
-    fz = fz + 1.0 * (jx * byv - jy * bxv);
+    const double g_b = gb ? (double)gb[c] : 1.0;
+    fx = fx + g_b * (jy * bzv - jz * byv);
+    fy = fy + g_b * (jz * bxv - jx * bzv);
+    fz = fz + g_b * (jx * byv - jy * bxv);
     const double rv = rho_v[c];
     fz = fz - rv * g.grav[k];
     vx1[c] = u0[0] + dt2 * (fv[0] + fx) / rv;
@@ -665,7 +751,9 @@ __global__ __launch_bounds__(BX * BY) void k_shock_corr(Geo g, Box b, const real
                              const real *__restrict__ al3, const real *__restrict__ pp,
                              real *__restrict__ energy, real *__restrict__ rho, real *__restrict__ visc_dep,
                              real *__restrict__ vxo, real *__restrict__ vyo, real *__restrict__ vzo,
-                             double *__restrict__ partial, double dt) {
+                             real *__restrict__ cool_v, double alpha_av, double *__restrict__ partial, double dt) {
+  // cool_v: cool_term_v of the cooling_term (lagran.F90:464-476), NULL when
+  // it is off
   // cells 1..n, marching in z: the node values (v, v1) and al1/al2 of plane
   // k are reused as plane k-1 / k values of the next cell.  The new
   // velocity v = 2 v1 - v of node (i, j, k), the cell's upper corner, goes
@@ -704,9 +792,19 @@ __global__ __launch_bounds__(BX * BY) void k_shock_corr(Geo g, Box b, const real
       const double vzbm = (lz1.c + lz1.d + lz1.b + lz1.a) * 0.25;
       const double dv = ((vxb - vxbm) / g.dxb[i] + (vyb - vybm) / g.dyb[j] + (vzb - vzbm) / g.dzb[k]) * dt;
       const double r = rho[c];
-      energy[c] = energy[c] + (dt * vh - dv * pp[c]) / r;
+      const real e1 = energy[c] + (dt * vh - dv * pp[c]) / r;
       visc_dep[c] = visc_dep[c] + dt * vh;
-      rho[c] = r / (1.0 + dv);
+      const real rn = r / (1.0 + dv);
+      rho[c] = rn;
+      if (cool_v) {
+        // a running average of the viscous heating is removed again, and
+        // energy = MAX(energy, 0)
+        const real ct = alpha_av * dt * vh / rn + (1.0 - alpha_av) * cool_v[c];
+        cool_v[c] = ct;
+        energy[c] = max0(e1 - ct);
+      } else {
+        energy[c] = e1;
+      }
       v[0] += dt * vh * g.cv(i, j, k);
       // v = 2 v1 - v (:399-407) at node (i, j, k); k_vupdate_low does the
       // nodes with an index 0
@@ -727,8 +825,6 @@ __global__ __launch_bounds__(BX * BY) void k_shock_corr(Geo g, Box b, const real
 }
 
 
-// energy_account (diagnostics.F90:517-609) + max |div B| * min(dx):
-// partial[0..3] = (en_b, en_ke, en_int, -max divb) per block over box 0..n
 // v = 2 v1 - v (:399-407) into vxo..vzo at the nodes 0..n that have an
 // index 0 (k_shock_corr does the rest): the plane k = 0 and, above it, the
 // faces i = 0 and j = 0.  One thread per node of the (n+1)^2 plane, then

@@ -55,6 +55,9 @@ struct Dev {
   double *F, *F2;
   // conduction scratch
   real *larsen, *Lc, *Yb[3];
+  // options: gamma_boris (boris), cool_term_v / cool_term_b and the stored
+  // curlb of the ohmic cooling (cooling_term); NULL when off
+  real *gamma_boris = nullptr, *cool_v = nullptr, *cool_b = nullptr, *curlb = nullptr;
   // reductions
   double *partial = nullptr;
   size_t partial_cap = 0;
@@ -599,7 +602,13 @@ LagArgs lag_args() {
   a.eta_bg = S.p.eta_background;
   a.eta0 = S.p.eta0;
   a.j_max = S.p.j_max;
+  a.va_max2 = S.p.va_max * S.p.va_max;
+  a.alpha_av = S.p.alpha_av;
   a.resistive = S.p.resistive_mhd;
+  a.boris = S.p.boris;
+  a.write_gb = !(S.first_set_dt && S.p.restart);
+  a.limiter = S.p.shock_limiter;
+  a.compression = S.p.shock_compression;
   return a;
 }
 
@@ -637,7 +646,7 @@ int lagrangian_step() {
   {
     const Box vb = mkbox(-1, N0 + 1, -1, N1 + 1, -1, N2 + 1);
     LAUNCH("vertex_prep", k_vertex_prep, zchunk_grid(vb), box_block(), S.g, vb, S.rho, S.energy, S.bx, S.by, S.bz,
-           S.rho_v, S.cs_v, S.p.gamma);
+           S.gamma_boris, S.rho_v, S.cs_v, S.p.gamma);
   }
   if (S.p.use_viscous_damping) {
     LAUNCH_BOX("damp_q", k_damp_q, nodes0n(), S.vx, S.vy, S.vz, S.visc3, S.qx, S.qy, S.qz);
@@ -651,7 +660,7 @@ int lagrangian_step() {
   const Box vdb = mkbox(0, N0 + 1, 0, N1 + 1, 0, N2 + 1);
   dim3 gr = zchunk_grid(vdb);
   LAUNCH("visc_dt", k_visc_dt, gr, box_block(), S.g, vdb, S.vx, S.vy, S.vz, S.rho, S.energy, S.bx, S.by, S.bz, S.eta,
-         S.al1, S.al2, S.al3, S.visc_heat, S.p0, S.partial, a);
+         S.al1, S.al2, S.al3, S.visc_heat, S.p0, S.gamma_boris, S.partial, a);
   int rc = final_reduce(gr, 3, 7, 0, 0);
   if (rc) return rc;
   double mins[3];
@@ -690,9 +699,12 @@ int lagrangian_step() {
       return fail(LARE_E_ARG, "resistive grid of %u blocks exceeds the partial buffer", g2.x * g2.z);
     for (int sc = 1; sc <= substeps; ++sc) {
       LAUNCH("resist", k_resist, g2, dim3(64, RS_ROWS, 1), S.g, fb, tiles_x, S.bx, S.by, S.bz, S.fxr, S.fyr, S.fzr,
-             S.eta, S.energy, S.ohmic_dep, S.rho, S.partial, S.dt, S.p.eta_background, S.p.eta0, S.p.j_max);
+             S.eta, S.energy, S.ohmic_dep, S.rho, S.curlb, S.partial, S.dt, S.p.eta_background, S.p.eta0,
+             S.p.j_max);
       rc = final_reduce(g2, 1, 0, 9, 1);
       if (rc) return rc;
+      if (S.curlb)
+        LAUNCH_BOX("cool_b", k_cool_b, interior(), S.curlb, S.rho, S.cool_b, S.energy, S.p.alpha_av, S.dt);
       double *tb;
       tb = S.bx; S.bx = S.fxr; S.fxr = tb;
       tb = S.by; S.by = S.fyr; S.fyr = tb;
@@ -731,7 +743,7 @@ int lagrangian_step() {
   // the predictor energy and pressure were formed in the predictor
   // remap's last geometry pass
   LAUNCH("force", k_force, zchunk_grid(nodes0n()), box_block(), S.g, nodes0n(), S.vx, S.vy, S.vz, S.pp, S.bpx, S.bpy,
-         S.bpz, S.rho_v, S.al1, S.al2, S.al3, S.vx1, S.vy1, S.vz1, S.dt2);
+         S.bpz, S.rho_v, S.al1, S.al2, S.al3, S.gamma_boris, S.vx1, S.vy1, S.vz1, S.dt2);
   {
     BcField f[3] = {{S.vx1, FK_NODE}, {S.vy1, FK_NODE}, {S.vz1, FK_NODE}};
     rc = apply_bcs(f, 3, "bc_v1");
@@ -739,7 +751,8 @@ int lagrangian_step() {
   }
   dim3 g3 = zchunk_grid(interior());
   LAUNCH("shock_corr", k_shock_corr, g3, box_block(), S.g, interior(), S.vx, S.vy, S.vz, S.vx1, S.vy1, S.vz1, S.al1,
-         S.al2, S.al3, S.pp, S.energy, S.rho, S.visc_dep, S.Fx, S.Fy, S.Fz, S.partial, S.dt);
+         S.al2, S.al3, S.pp, S.energy, S.rho, S.visc_dep, S.Fx, S.Fy, S.Fz, S.cool_v, S.p.alpha_av, S.partial,
+         S.dt);
   rc = final_reduce(g3, 1, 0, 8, 1);
   if (rc) return rc;
   {
@@ -904,6 +917,19 @@ int api_init(const lare_grid *grid, const lare_params *params) {
   double **wide[] = {&S.bx, &S.by, &S.bz, &S.bpx, &S.bpy, &S.bpz, &S.fxr, &S.fyr, &S.fzr, &S.F, &S.F2};
   for (double **pp : wide) {
     int rc = dalloc(pp);
+    if (rc) return rc;
+  }
+  S.gamma_boris = S.cool_v = S.cool_b = S.curlb = nullptr;
+  if (params->boris) {
+    int rc = dalloc(&S.gamma_boris);
+    if (rc) return rc;
+    // 1 until the first set_dt (gamma_boris(:,:,:) = 1, lagran.F90:1092)
+    LAUNCH_BOX("fill", k_fill, full_box(), S.gamma_boris, 1.0);
+  }
+  if (params->cooling_term) {
+    int rc = dalloc(&S.cool_v);
+    if (!rc) rc = dalloc(&S.cool_b);
+    if (!rc && params->resistive_mhd) rc = dalloc(&S.curlb);
     if (rc) return rc;
   }
   if (params->use_viscous_damping) {

@@ -24,6 +24,8 @@ MODULE lare_gpu_bridge
     REAL(C_DOUBLE) :: kappa_0, flux_limiter, xi_n, dt_from_restart
     INTEGER(C_INT32_T) :: resistive_mhd, conduction, heat_flux_limiter
     INTEGER(C_INT32_T) :: use_viscous_damping, restart, precision
+    REAL(C_DOUBLE) :: alpha_av, va_max
+    INTEGER(C_INT32_T) :: cooling_term, boris, shock_limiter, shock_compression
   END TYPE lare_params
 
   TYPE, BIND(C) :: lare_fields

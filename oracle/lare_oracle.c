@@ -69,6 +69,8 @@ struct orc_state {
   double *rho, *energy, *vx, *vy, *vz, *vx1, *vy1, *vz1, *bx, *by, *bz;
   double *eta, *cv, *cv1, *visc3, *visc_dep, *ohmic_dep, *p_visc, *temperature;
   double *larsen;          /* larsen_factor(0:nx,0:ny,0:nz), conduct.f90:41 */
+  double *gamma_boris;     /* gamma_boris(-1:n+2)^3, mpi_routines.F90:311 */
+  double *cool_term_v, *cool_term_b; /* (1:n)^3, mpi_routines.F90:315-319 */
   /* lagran module arrays live only within one lagrangian_step */
   double *bx1, *by1, *bz1, *pressure, *rho_vl, *cv_vl, *visc_heat;
   double *alpha1, *alpha2, *alpha3, *fx_visc, *fy_visc, *fz_visc;
@@ -1012,6 +1014,20 @@ static void resistive_effects(orc_state *o, double *fxr, double *fyr, double *fz
       }
     }
   }
+  if (o->p.cooling_term) {
+    /* :1290-1302.  The loop sets no ixm or iym: they keep the last values of
+     * the heating loop above, nx - 1 and ny - 1 */
+    const int ixm = nx - 1, iym = ny - 1;
+    const double aav = o->p.alpha_av;
+    for (int iz = 1; iz <= nz; ++iz)
+      for (int iy = 1; iy <= ny; ++iy)
+        for (int ix = 1; ix <= nx; ++ix) {
+          double lh = (A(curlb, ix, iy, iz) + A(curlb, ixm, iy, iz) + A(curlb, ix, iym, iz) + A(curlb, ixm, iym, iz)) *
+                      dt / (4.0 * A(o->rho, ix, iy, iz));
+          A(o->cool_term_b, ix, iy, iz) = aav * lh + (1.0 - aav) * A(o->cool_term_b, ix, iy, iz);
+          A(o->energy, ix, iy, iz) = A(o->energy, ix, iy, iz) - A(o->cool_term_b, ix, iy, iz);
+        }
+  }
   for (int iz = -1; iz <= nz + 2; ++iz)
     for (int iy = -1; iy <= ny + 2; ++iy)
       for (int ix = -1; ix <= nx + 2; ++ix) A(o->energy, ix, iy, iz) = dmax(A(o->energy, ix, iy, iz), 0.0);
@@ -1056,9 +1072,11 @@ static void viscous_damping(orc_state *o) {
 }
 
 /* edge_viscosity (lagran.F90:795-855), SHOCKLIMITER and SHOCKCOMPRESSION off */
-static double edge_viscosity(orc_state *o, const double *cs_v, double dvdots, double dx,
-                             int i1, int j1, int k1, int i2, int j2, int k2) {
-  dvdots = -fabs(dvdots);
+static double edge_viscosity(orc_state *o, const double *cs_v, double dvdots, double dx, double dxm, double dxp,
+                             int i1, int j1, int k1, int i2, int j2, int k2, int i0, int j0, int k0, int i3,
+                             int j3, int k3) {
+  if (o->p.shock_compression) dvdots = dmin(0.0, dvdots); /* SHOCKCOMPRESSION (:807-808) */
+  else dvdots = -fabs(dvdots);
   double r1 = A(o->rho_vl, i1, j1, k1), r2 = A(o->rho_vl, i2, j2, k2);
   double rho_edge = 2.0 * r1 * r2 / (r1 + r2);
   double cs_edge = dmin(A(cs_v, i1, j1, k1), A(cs_v, i2, j2, k2));
@@ -1070,6 +1088,25 @@ static double edge_viscosity(orc_state *o, const double *cs_v, double dvdots, do
   double psi = 0.0;
   if (dv * o->dt / dx < 1.0e-14) dvdots = 0.0;
   else dvdots = dvdots / dv;
+  if (o->p.shock_limiter) {
+    /* SHOCKLIMITER (:831-845) */
+    double dvxm = A(o->vx, i0, j0, k0) - A(o->vx, i1, j1, k1);
+    double dvxp = A(o->vx, i2, j2, k2) - A(o->vx, i3, j3, k3);
+    double dvym = A(o->vy, i0, j0, k0) - A(o->vy, i1, j1, k1);
+    double dvyp = A(o->vy, i2, j2, k2) - A(o->vy, i3, j3, k3);
+    double dvzm = A(o->vz, i0, j0, k0) - A(o->vz, i1, j1, k1);
+    double dvzp = A(o->vz, i2, j2, k2) - A(o->vz, i3, j3, k3);
+    double rl, rr;
+    if (dv * o->dt / dx < 1.0e-14) {
+      rl = 1.0;
+      rr = 1.0;
+    } else {
+      rl = (dvxp * dvx + dvyp * dvy + dvzp * dvz) * dx / (dxp * dv2);
+      rr = (dvxm * dvx + dvym * dvy + dvzm * dvz) * dx / (dxm * dv2);
+    }
+    psi = dmin(dmin(dmin(0.5 * (rr + rl), 2.0 * rl), 2.0 * rr), 1.0);
+    psi = dmax(0.0, psi);
+  }
   double vn = o->visc2_norm;
   double q_k_bar = rho_edge * (vn * dv + sqrt(vn * vn * dv2 + SQ(o->p.visc1 * cs_edge)));
   return q_k_bar * (1.0 - psi) * dvdots;
@@ -1133,8 +1170,8 @@ static void shock_viscosity(orc_state *o) {
       for (int iy = -1; iy <= ny + 2; ++iy) {
         double rmin = dmax(A(o->rho, ix, iy, iz), NONE_ZERO);
         double b2 = SQ(A(o->bx1, ix, iy, iz)) + SQ(A(o->by1, ix, iy, iz)) + SQ(A(o->bz1, ix, iy, iz));
-        /* gamma_boris = 1 (boris off) */
-        A(cs, ix, iy, iz) = sqrt((o->p.gamma * A(o->pressure, ix, iy, iz) + 1.0 * b2) / rmin);
+        /* gamma_boris of the previous set_dt (1 before the first) */
+        A(cs, ix, iy, iz) = sqrt((o->p.gamma * A(o->pressure, ix, iy, iz) + A(o->gamma_boris, ix, iy, iz) * b2) / rmin);
       }
   for (int iz = -1; iz <= nz + 1; ++iz)
     for (int iy = -1; iy <= ny + 1; ++iy)
@@ -1151,21 +1188,24 @@ static void shock_viscosity(orc_state *o) {
       for (int ix = 0; ix <= nx + 1; ++ix) {
         int i1 = ix - 1, j1 = iy - 1, k1 = iz - 1, i2 = ix, j2 = iy - 1, k2 = iz - 1;
         double dvdots = -(A(o->vx, i1, j1, k1) - A(o->vx, i2, j2, k2));
-        A(o->alpha1, ix, iy, iz) = edge_viscosity(o, cs_v, dvdots, DXB(ix), i1, j1, k1, i2, j2, k2);
+        A(o->alpha1, ix, iy, iz) = edge_viscosity(o, cs_v, dvdots, DXB(ix), DXB(ix - 1), DXB(ix + 1), i1, j1, k1, i2,
+                                                  j2, k2, i1 - 1, j1, k1, i2 + 1, j2, k2);
       }
   for (int iz = 0; iz <= nz + 2; ++iz)
     for (int iy = 0; iy <= ny + 1; ++iy)
       for (int ix = -1; ix <= nx + 1; ++ix) {
         int i1 = ix, j1 = iy - 1, k1 = iz - 1, i2 = ix, j2 = iy, k2 = iz - 1;
         double dvdots = -(A(o->vy, i1, j1, k1) - A(o->vy, i2, j2, k2));
-        A(o->alpha2, ix, iy, iz) = edge_viscosity(o, cs_v, dvdots, DYB(iy), i1, j1, k1, i2, j2, k2);
+        A(o->alpha2, ix, iy, iz) = edge_viscosity(o, cs_v, dvdots, DYB(iy), DYB(iy - 1), DYB(iy + 1), i1, j1, k1, i2,
+                                                  j2, k2, i1, j1 - 1, k1, i2, j2 + 1, k2);
       }
   for (int iz = 0; iz <= nz + 1; ++iz)
     for (int iy = -1; iy <= ny + 1; ++iy)
       for (int ix = -1; ix <= nx + 1; ++ix) {
         int i1 = ix, j1 = iy, k1 = iz - 1, i2 = ix, j2 = iy, k2 = iz;
         double dvdots = -(A(o->vz, i1, j1, k1) - A(o->vz, i2, j2, k2));
-        A(o->alpha3, ix, iy, iz) = edge_viscosity(o, cs_v, dvdots, DZB(iz), i1, j1, k1, i2, j2, k2);
+        A(o->alpha3, ix, iy, iz) = edge_viscosity(o, cs_v, dvdots, DZB(iz), DZB(iz - 1), DZB(iz + 1), i1, j1, k1, i2,
+                                                  j2, k2, i1, j1, k1 - 1, i2, j2, k2 + 1);
       }
   for (int iz = 0; iz <= nz + 1; ++iz)
     for (int iy = 0; iy <= ny + 1; ++iy)
@@ -1222,7 +1262,7 @@ static void shock_heating(orc_state *o) {
       for (int ix = 0; ix <= nx + 1; ++ix) A(o->visc_heat, ix, iy, iz) = dmax(A(o->visc_heat, ix, iy, iz), 0.0);
 }
 
-/* lagran.F90:1070-1166 set_dt (boris off, no cowling resistivity) */
+/* lagran.F90:1070-1166 set_dt (no cowling resistivity) */
 static void set_dt(orc_state *o) {
   int nx = o->nx, ny = o->ny, nz = o->nz;
   if (o->first_set_dt) {
@@ -1233,12 +1273,18 @@ static void set_dt(orc_state *o) {
     }
   }
   double dt_local = LARGEST, dtr_local = LARGEST, dt3 = LARGEST;
+  for (int iz = -1; iz <= nz + 2; ++iz)
+    for (int iy = -1; iy <= ny + 2; ++iy)
+      for (int ix = -1; ix <= nx + 2; ++ix) A(o->gamma_boris, ix, iy, iz) = 1.0;
+  const double va_max2 = o->p.va_max * o->p.va_max; /* setup.F90:80 */
   for (int iz = 0; iz <= nz; ++iz)
     for (int iy = 0; iy <= ny; ++iy)
       for (int ix = 0; ix <= nx; ++ix) {
         double w1 = SQ(A(o->bx, ix, iy, iz)) + SQ(A(o->by, ix, iy, iz)) + SQ(A(o->bz, ix, iy, iz));
         double rho0 = dmax(A(o->rho, ix, iy, iz), NONE_ZERO);
-        double cs2 = (o->p.gamma * A(o->pressure, ix, iy, iz) + 1.0 * w1) / rho0;
+        double w2 = w1 / rho0;
+        if (o->p.boris && w2 >= va_max2) A(o->gamma_boris, ix, iy, iz) = 1.0 / (1.0 + w2 / va_max2);
+        double cs2 = (o->p.gamma * A(o->pressure, ix, iy, iz) + A(o->gamma_boris, ix, iy, iz) * w1) / rho0;
         double c_visc2 = A(o->p_visc, ix, iy, iz) / rho0;
         double length = min3(DXB(ix), DYB(iy), DZB(iz));
         double dt1 = length / (sqrt(c_visc2) + sqrt(cs2 + c_visc2));
@@ -1264,7 +1310,7 @@ static void set_dt(orc_state *o) {
   o->time = o->time + o->dt;
 }
 
-/* lagran.F90:208-478 predictor_corrector_step (non-CAUCHY, cooling off) */
+/* lagran.F90:208-478 predictor_corrector_step (non-CAUCHY) */
 static void predictor_corrector_step(orc_state *o) {
   int nx = o->nx, ny = o->ny, nz = o->nz;
   double gm1 = o->p.gamma - 1.0, xi = o->p.xi_n;
@@ -1336,9 +1382,10 @@ static void predictor_corrector_step(orc_state *o) {
         double bzv = 0.25 * (A(o->bz, ix, iy, iz) + A(o->bz, ix, iyp, iz) + A(o->bz, ixp, iy, iz) + A(o->bz, ixp, iyp, iz));
         double jx, jy, jz;
         vertex_j(o, ix, iy, iz, &jx, &jy, &jz);
-        fx = fx + 1.0 * (jy * bzv - jz * byv);
-        fy = fy + 1.0 * (jz * bxv - jx * bzv);
-        fz = fz + 1.0 * (jx * byv - jy * bxv);
+        const double gb = A(o->gamma_boris, ix, iy, iz);
+        fx = fx + gb * (jy * bzv - jz * byv);
+        fy = fy + gb * (jz * bxv - jx * bzv);
+        fz = fz + gb * (jx * byv - jy * bxv);
         double rv = A(o->rho_vl, ix, iy, iz);
         fz = fz - rv * GRAV(iz);
         A(o->vx1, ix, iy, iz) = A(o->vx, ix, iy, iz) + o->dt2 * (A(o->fx_visc, ix, iy, iz) + fx) / rv;
@@ -1386,6 +1433,21 @@ static void predictor_corrector_step(orc_state *o) {
         o->tvh = o->tvh + o->dt * vh * A(o->cv, ix, iy, iz);
       }
     }
+  }
+  if (o->p.cooling_term) {
+    /* lagran.F90:464-476: a running average of the viscous heating is
+     * removed again; energy = MAX(energy, 0) over the whole array */
+    const double aav = o->p.alpha_av;
+    for (int iz = 1; iz <= nz; ++iz)
+      for (int iy = 1; iy <= ny; ++iy)
+        for (int ix = 1; ix <= nx; ++ix) {
+          A(o->cool_term_v, ix, iy, iz) = aav * o->dt * A(o->visc_heat, ix, iy, iz) / A(o->rho, ix, iy, iz) +
+                                          (1.0 - aav) * A(o->cool_term_v, ix, iy, iz);
+          A(o->energy, ix, iy, iz) = A(o->energy, ix, iy, iz) - A(o->cool_term_v, ix, iy, iz);
+        }
+    for (int iz = -1; iz <= nz + 2; ++iz)
+      for (int iy = -1; iy <= ny + 2; ++iy)
+        for (int ix = -1; ix <= nx + 2; ++ix) A(o->energy, ix, iy, iz) = dmax(A(o->energy, ix, iy, iz), 0.0);
   }
 }
 
@@ -1664,6 +1726,11 @@ orc_state *orc_create(const lare_grid *g, const lare_params *p) {
   o->visc3 = nodebox(o); o->visc_dep = cellbox(o); o->ohmic_dep = cellbox(o);
   o->p_visc = cellbox(o); o->temperature = cellbox(o);
   o->larsen = box(o, 0, o->nx, 0, o->ny, 0, o->nz);
+  o->gamma_boris = cellbox(o);
+  for (long q = 0; q < o->ntot; ++q)
+    if (o->gamma_boris[q] == 0.0) o->gamma_boris[q] = 1.0;
+  o->cool_term_v = box(o, 1, o->nx, 1, o->ny, 1, o->nz);
+  o->cool_term_b = box(o, 1, o->nx, 1, o->ny, 1, o->nz);
   for (int iz = -1; iz <= o->nz + 2; ++iz)
     for (int iy = -1; iy <= o->ny + 2; ++iy)
       for (int ix = -1; ix <= o->nx + 2; ++ix) A(o->cv, ix, iy, iz) = DXB(ix) * DYB(iy) * DZB(iz);
@@ -1675,7 +1742,8 @@ orc_state *orc_create(const lare_grid *g, const lare_params *p) {
 void orc_destroy(orc_state *o) {
   if (!o) return;
   double **all[] = {&o->rho, &o->energy, &o->vx, &o->vy, &o->vz, &o->vx1, &o->vy1, &o->vz1, &o->bx, &o->by, &o->bz, &o->eta,
-                    &o->cv, &o->cv1, &o->visc3, &o->visc_dep, &o->ohmic_dep, &o->p_visc, &o->temperature, &o->larsen};
+                    &o->cv, &o->cv1, &o->visc3, &o->visc_dep, &o->ohmic_dep, &o->p_visc, &o->temperature, &o->larsen,
+                    &o->gamma_boris, &o->cool_term_v, &o->cool_term_b};
   for (unsigned q = 0; q < sizeof(all) / sizeof(all[0]); ++q) drop(all[q]);
   for (int q = 0; q < 7; ++q) free(o->m[q]);
   free(o);

@@ -22,7 +22,7 @@ ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "lare_oracle.c")
 CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-shared"]
 PIN_TESTS = ["tests/test_pin_hydro.py", "tests/test_pin_transport.py", "tests/test_pin_oracle3d.py",
-             "tests/test_pin_potential.py"]
+             "tests/test_pin_potential.py", "tests/test_pin_variants.py"]
 
 # (name, what it breaks, [(old, new), ...]); each `old` occurs exactly once
 MUTANTS = [
@@ -75,6 +75,17 @@ MUTANTS = [
      [("                       2.0 / (dx1 + dy1 + dz1);", "                       1.0 / (dx1 + dy1 + dz1);")]),
     ("M20", "potential_field: upper boundary zero-gradient instead of phi = 0 (:270-273)",
      [("        A(phi, ix, iy, nz + 1) = 0.0;", "        A(phi, ix, iy, nz + 1) = A(phi, ix, iy, nz);")]),
+    ("M21", "SHOCKCOMPRESSION: MAX for MIN (lagran.F90:808)",
+     [("if (o->p.shock_compression) dvdots = dmin(0.0, dvdots);", "if (o->p.shock_compression) dvdots = dmax(0.0, dvdots);")]),
+    ("M22", "SHOCKLIMITER: rl from the node before the edge (lagran.F90:839)",
+     [("rl = (dvxp * dvx + dvyp * dvy + dvzp * dvz) * dx / (dxp * dv2);", "rl = (dvxm * dvx + dvyp * dvy + dvzp * dvz) * dx / (dxp * dv2);")]),
+    ("M23", "boris: gamma_b from va_max^2 / w2 (lagran.F90:1107)",
+     [("A(o->gamma_boris, ix, iy, iz) = 1.0 / (1.0 + w2 / va_max2);", "A(o->gamma_boris, ix, iy, iz) = 1.0 / (1.0 + va_max2 / w2);")]),
+    ("M24", "boris: J x B unscaled (lagran.F90:375-377)",
+     [("fx = fx + gb * (jy * bzv - jz * byv);", "fx = fx + 1.0 * (jy * bzv - jz * byv);")]),
+    ("M25", "cooling_term: heating averaged with the old density (lagran.F90:468)",
+     [("A(o->cool_term_v, ix, iy, iz) = aav * o->dt * A(o->visc_heat, ix, iy, iz) / A(o->rho, ix, iy, iz) +",
+       "A(o->cool_term_v, ix, iy, iz) = aav * o->dt * A(o->visc_heat, ix, iy, iz) / A(o->rho, ix, iy, iz) / A(o->cv1, ix, iy, iz) * A(o->cv, ix, iy, iz) +")]),
 ]
 
 

@@ -66,6 +66,12 @@ def test_conduct_heat_arcade():
     dev.close()
 
 
+def _with(cfg, **kw):
+    for k, v in kw.items():
+        setattr(cfg[1], k, v)
+    return cfg
+
+
 CASES = {
     "orszag_tang": lambda: configs.orszag_tang(16),
     "harris": lambda: configs.harris(16),
@@ -85,6 +91,13 @@ CASES = {
     # the twisted flux tube (walls in x and y, periodic z; resistive with
     # Braginskii conduction)
     "kink": lambda: configs.kink_tube(16),
+    # the optional physics: the cooling_term (viscous and ohmic, with the
+    # reference's stale-index average), the Boris correction (va_max below
+    # the Alfven speed of the sheet) and the shock limiter; and on the
+    # vortex the SHOCKCOMPRESSION variant with the cooling
+    "harris_options": lambda: _with(configs.harris(16, ny=13, nz=11), cooling_term=1, alpha_av=0.05, boris=1,
+                                    va_max=0.6, shock_limiter=1),
+    "ot_compression": lambda: _with(configs.orszag_tang(16), shock_compression=1, cooling_term=1, alpha_av=0.3),
 }
 
 

@@ -96,11 +96,19 @@ def test_advection_second_order():
     assert order > 1.5
 
 
-def _fast_wave(backend, axis, n=64, frac=0.5, eps=1e-4):
+def _fast_wave(backend, axis, n=64, frac=0.5, eps=1e-4, va_max=None):
+    """Density phase of a fast magnetosonic wave after `frac` / c_f.  With
+    va_max the Boris correction is on: gamma_b = 1 / (1 + v_A^2 / va_max^2)
+    scales the J x B force (lagran.F90:375-377), so the wave runs at
+    sqrt(c_s^2 + gamma_b v_A^2)."""
     gam = 5.0 / 3.0
     rho0, p0, b0 = 1.0, 0.6, 1.0  # c_s = 1, v_A = 1, c_f = sqrt(2)
-    cf = math.sqrt(gam * p0 / rho0 + b0 * b0 / rho0)
+    gb = 1.0 if va_max is None else 1.0 / (1.0 + (b0 * b0 / rho0) / va_max ** 2)
+    cf = math.sqrt(gam * p0 / rho0 + gb * b0 * b0 / rho0)
     grid, params, f = line_problem(n, axis, bc_axis=BC_PERIODIC, gamma=gam, visc1=0.0, visc2=0.0)
+    if va_max is not None:
+        params.boris = 1
+        params.va_max = va_max
     k = 2.0 * math.pi
     xc, xb = grid.xc[axis], grid.xb[axis]
     wc = np.cos(k * xc)
@@ -135,6 +143,17 @@ def test_fast_magnetosonic_speed(backend):
     print(f"fast wave: amplitude {amp:.4f} phase {phase:.4f} expected {expect:.4f} err {err:.4f}")
     assert amp > 0.9
     assert abs(err) < 0.05  # a sound wave (J x B missing) would lag by ~0.9 rad
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_fast_wave_with_boris(backend):
+    """va_max = 0.5 with v_A = 1: gamma_b = 0.2 and c_f = sqrt(1.2), where
+    an unscaled J x B would keep sqrt(2) and lead by ~0.9 rad."""
+    amp, phase, expect = _fast_wave(backend, 0, va_max=0.5)
+    err = (phase - expect + math.pi) % (2 * math.pi) - math.pi
+    print(f"fast wave with boris: amplitude {amp:.4f} phase {phase:.4f} expected {expect:.4f} err {err:.4f}")
+    assert amp > 0.9
+    assert abs(err) < 0.05
 
 
 def test_fast_wave_orientations():
