@@ -79,9 +79,12 @@ static inline Box mkbox(int x0, int x1, int y0, int y1, int z0, int z1) {
 
 __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ float dmin(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float dmax(float a, float b) { return a > b ? a : b; }
 
 // Fortran MAX(x, 0.0_num) as used on energy and visc_heat.
 __device__ __forceinline__ double max0(double x) { return x > 0.0 ? x : 0.0; }
+__device__ __forceinline__ float max0(float x) { return x > 0.0f ? x : 0.0f; }
 
 // Wavefront (64-lane) reductions.
 __device__ __forceinline__ double wave_min(double v) {

@@ -21,6 +21,8 @@
 
 #define NONE_ZERO 2.2250738585072014e-308
 #define LARGEST 1.7976931348623157e+308
+// TINY of the arithmetic type (the smallest normal float in the FP32 mode)
+#define ACC_NONE_ZERO ((acc)(sizeof(acc) == sizeof(double) ? NONE_ZERO : 1.1754943508222875e-38))
 
 struct LagArgs {
   double gamma, visc1, visc2_norm, dt, dt2;
@@ -49,25 +51,26 @@ __device__ __forceinline__ double cv_sum8(const Geo &g, int i, int j, int k) {
 
 // fast magnetosonic speed of shock_viscosity (:554-556) in cell c; gb is
 // gamma_boris of the previous set_dt, or NULL for 1 (boris off)
-__device__ __forceinline__ double cell_cs(const Geo &g, const real *__restrict__ rho,
-                                          const real *__restrict__ energy, const double *__restrict__ bx,
-                                          const double *__restrict__ by, const double *__restrict__ bz,
-                                          const real *__restrict__ gb, int c, double gamma) {
-  const double r = rho[c];
-  const double rmin = dmax(r, NONE_ZERO);
-  const double b1x = (bx[c] + bx[c - 1]) * 0.5;
-  const double b1y = (by[c] + by[c - g.sy]) * 0.5;
-  const double b1z = (bz[c] + bz[c - g.sz]) * 0.5;
-  const double b2 = b1x * b1x + b1y * b1y + b1z * b1z;
-  const double p = (gamma - 1.0) * r * (energy[c] - 0.0);
-  const double g_b = gb ? (double)gb[c] : 1.0;
-  return sqrt((gamma * p + g_b * b2) / rmin);
+__device__ __forceinline__ acc cell_cs(const Geo &g, const real *__restrict__ rho, const real *__restrict__ energy,
+                                       const double *__restrict__ bx, const double *__restrict__ by,
+                                       const double *__restrict__ bz, const real *__restrict__ gb, int c,
+                                       double gamma) {
+  const acc r = rho[c];
+  const acc rmin = dmax(r, ACC_NONE_ZERO);
+  const acc b1x = (bx[c] + bx[c - 1]) * 0.5;
+  const acc b1y = (by[c] + by[c - g.sy]) * 0.5;
+  const acc b1z = (bz[c] + bz[c - g.sz]) * 0.5;
+  const acc b2 = b1x * b1x + b1y * b1y + b1z * b1z;
+  const acc ga = (acc)gamma;
+  const acc p = (ga - (acc)1.0) * r * (energy[c] - (acc)0.0);
+  const acc g_b = gb ? (acc)gb[c] : (acc)1.0;
+  return sqrt((ga * p + g_b * b2) / rmin);
 }
 
 // rho and the fast speed of the four cells (i,j), (i+1,j), (i,j+1), (i+1,j+1)
 // of one plane, starting at cell index c
 struct CellQuad {
-  double r[4], cs[4];
+  acc r[4], cs[4];
 };
 __device__ __forceinline__ CellQuad cell_quad(const Geo &g, const real *__restrict__ rho,
                                               const real *__restrict__ energy, const double *__restrict__ bx,
@@ -98,22 +101,22 @@ __global__ __launch_bounds__(BX * BY) void k_vertex_prep(Geo g, Box b, const rea
   for (int k = k0; k <= k1; ++k) {
     const int c = g.idx(i, j, k);
     const CellQuad hi = cell_quad(g, rho, energy, bx, by, bz, gb, c + Z, gamma);
-    double cvs[8];
+    acc cvs[8];
     const int di[8] = {0, 1, 0, 1, 0, 1, 0, 1}, dj[8] = {0, 0, 1, 1, 0, 0, 1, 1}, dk[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 #pragma unroll
-    for (int q = 0; q < 8; ++q) cvs[q] = g.cv(i + di[q], j + dj[q], k + dk[q]);
-    double s = lo.r[0] * cvs[0];
+    for (int q = 0; q < 8; ++q) cvs[q] = (acc)g.cv(i + di[q], j + dj[q], k + dk[q]);
+    acc s = lo.r[0] * cvs[0];
 #pragma unroll
     for (int q = 1; q < 8; ++q) s = s + (q < 4 ? lo.r[q] : hi.r[q - 4]) * cvs[q];
-    double sc = cvs[0];
+    acc sc = cvs[0];
 #pragma unroll
     for (int q = 1; q < 8; ++q) sc = sc + cvs[q];
     rho_v[c] = s / sc;
-    const double cvv = 0.125 * sc;
-    double scs = lo.cs[0] * cvs[0];
+    const acc cvv = (acc)0.125 * sc;
+    acc scs = lo.cs[0] * cvs[0];
 #pragma unroll
     for (int q = 1; q < 8; ++q) scs = scs + (q < 4 ? lo.cs[q] : hi.cs[q - 4]) * cvs[q];
-    cs_v[c] = 0.125 * scs / cvv;
+    cs_v[c] = (acc)0.125 * scs / cvv;
     lo = hi;
   }
 }
@@ -131,7 +134,7 @@ __device__ __forceinline__ NodeV node_v(const real *__restrict__ vx, const real 
 
 // Velocity of a node (the outer neighbours of an edge for the limiter).
 struct Vel3 {
-  double x, y, z;
+  acc x, y, z;
 };
 __device__ __forceinline__ Vel3 vel3(const real *__restrict__ vx, const real *__restrict__ vy,
                                      const real *__restrict__ vz, int c) {
@@ -144,41 +147,42 @@ __device__ __forceinline__ Vel3 vel3(const real *__restrict__ vx, const real *__
 // along the edge) and node 3 (after node 2) and the widths dxm, dxp of the
 // neighbouring cells give the limiter psi (:831-845); with SHOCKCOMPRESSION
 // (a.compression) an expanding edge gets no viscosity (:807-808).
-__device__ __forceinline__ double edge_visc(const NodeV &n1, const NodeV &n2, double dvdots, double dx,
-                                            const LagArgs &a, const Vel3 &v0 = Vel3{0, 0, 0},
-                                            const Vel3 &v3 = Vel3{0, 0, 0}, double dxm = 1.0, double dxp = 1.0) {
-  if (a.compression) dvdots = dmin(0.0, dvdots);
+__device__ __forceinline__ acc edge_visc(const NodeV &n1, const NodeV &n2, acc dvdots, double dx_,
+                                         const LagArgs &a, const Vel3 &v0 = Vel3{0, 0, 0},
+                                         const Vel3 &v3 = Vel3{0, 0, 0}, double dxm_ = 1.0, double dxp_ = 1.0) {
+  const acc dx = (acc)dx_, dxm = (acc)dxm_, dxp = (acc)dxp_, dt = (acc)a.dt;
+  if (a.compression) dvdots = dmin((acc)0.0, dvdots);
   else dvdots = -fabs(dvdots);
-  const double r1 = n1.r, r2 = n2.r;
-  const double rho_edge = 2.0 * r1 * r2 / (r1 + r2);
-  const double cs_edge = dmin(n1.cs, n2.cs);
-  const double dvx = n1.vx - n2.vx;
-  const double dvy = n1.vy - n2.vy;
-  const double dvz = n1.vz - n2.vz;
-  const double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
-  const double dv = sqrt(dv2);
-  double psi = 0.0;
-  if (dv * a.dt / dx < 1.0e-14) dvdots = 0.0;
+  const acc r1 = n1.r, r2 = n2.r;
+  const acc rho_edge = (acc)2.0 * r1 * r2 / (r1 + r2);
+  const acc cs_edge = dmin((acc)n1.cs, (acc)n2.cs);
+  const acc dvx = n1.vx - n2.vx;
+  const acc dvy = n1.vy - n2.vy;
+  const acc dvz = n1.vz - n2.vz;
+  const acc dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
+  const acc dv = sqrt(dv2);
+  acc psi = 0.0;
+  if (dv * dt / dx < (acc)1.0e-14) dvdots = 0.0;
   else dvdots = dvdots / dv;
   if (a.limiter) {
-    const double dvxm = v0.x - (double)n1.vx, dvxp = (double)n2.vx - v3.x;
-    const double dvym = v0.y - (double)n1.vy, dvyp = (double)n2.vy - v3.y;
-    const double dvzm = v0.z - (double)n1.vz, dvzp = (double)n2.vz - v3.z;
-    double rl, rr;
-    if (dv * a.dt / dx < 1.0e-14) {
+    const acc dvxm = v0.x - (acc)n1.vx, dvxp = (acc)n2.vx - v3.x;
+    const acc dvym = v0.y - (acc)n1.vy, dvyp = (acc)n2.vy - v3.y;
+    const acc dvzm = v0.z - (acc)n1.vz, dvzp = (acc)n2.vz - v3.z;
+    acc rl, rr;
+    if (dv * dt / dx < (acc)1.0e-14) {
       rl = 1.0;
       rr = 1.0;
     } else {
       rl = (dvxp * dvx + dvyp * dvy + dvzp * dvz) * dx / (dxp * dv2);
       rr = (dvxm * dvx + dvym * dvy + dvzm * dvz) * dx / (dxm * dv2);
     }
-    psi = dmin(dmin(dmin(0.5 * (rr + rl), 2.0 * rl), 2.0 * rr), 1.0);
-    psi = dmax(0.0, psi);
+    psi = dmin(dmin(dmin((acc)0.5 * (rr + rl), (acc)2.0 * rl), (acc)2.0 * rr), (acc)1.0);
+    psi = dmax((acc)0.0, psi);
   }
-  const double vn = a.visc2_norm;
-  const double vc = a.visc1 * cs_edge;
-  const double q_k_bar = rho_edge * (vn * dv + sqrt(vn * vn * dv2 + vc * vc));
-  return q_k_bar * (1.0 - psi) * dvdots;
+  const acc vn = (acc)a.visc2_norm;
+  const acc vc = (acc)a.visc1 * cs_edge;
+  const acc q_k_bar = rho_edge * (vn * dv + sqrt(vn * vn * dv2 + vc * vc));
+  return q_k_bar * ((acc)1.0 - psi) * dvdots;
 }
 
 // Caramana edge viscosities alpha1..3 (lagran.F90:583-678) on their edge
@@ -293,21 +297,20 @@ __device__ __forceinline__ Al12 al12(const real *__restrict__ al1, const real *_
 
 // edge_heat from values: al1/al2 of planes k (lo) and k+1 (hi), al3 at
 // (i,j), (i-1,j), (i-1,j-1), (i,j-1) of plane k; same terms and order
-__device__ __forceinline__ double edge_heat_v(double dxb, double dyb, double dzb, double cv, const Al12 &lo,
-                                              const Al12 &hi, double a30, double a3x, double a3xy, double a3y,
-                                              const double aa[12]) {
-  double h = -0.25 * dyb * dzb * lo.a1 * aa[0];
-  h = h - 0.25 * dxb * dzb * lo.a2 * aa[1];
-  h = h - 0.25 * dyb * dzb * lo.a1y * aa[2];
-  h = h - 0.25 * dxb * dzb * lo.a2x * aa[3];
-  h = h - 0.25 * dyb * dzb * hi.a1 * aa[4];
-  h = h - 0.25 * dxb * dzb * hi.a2 * aa[5];
-  h = h - 0.25 * dyb * dzb * hi.a1y * aa[6];
-  h = h - 0.25 * dxb * dzb * hi.a2x * aa[7];
-  h = h - 0.25 * dxb * dyb * a30 * aa[8];
-  h = h - 0.25 * dxb * dyb * a3x * aa[9];
-  h = h - 0.25 * dxb * dyb * a3xy * aa[10];
-  h = h - 0.25 * dxb * dyb * a3y * aa[11];
+__device__ __forceinline__ acc edge_heat_v(acc dxb, acc dyb, acc dzb, acc cv, const Al12 &lo, const Al12 &hi, acc a30,
+                                           acc a3x, acc a3xy, acc a3y, const acc aa[12]) {
+  acc h = -(acc)0.25 * dyb * dzb * lo.a1 * aa[0];
+  h = h - (acc)0.25 * dxb * dzb * lo.a2 * aa[1];
+  h = h - (acc)0.25 * dyb * dzb * lo.a1y * aa[2];
+  h = h - (acc)0.25 * dxb * dzb * lo.a2x * aa[3];
+  h = h - (acc)0.25 * dyb * dzb * hi.a1 * aa[4];
+  h = h - (acc)0.25 * dxb * dzb * hi.a2 * aa[5];
+  h = h - (acc)0.25 * dyb * dzb * hi.a1y * aa[6];
+  h = h - (acc)0.25 * dxb * dzb * hi.a2x * aa[7];
+  h = h - (acc)0.25 * dxb * dyb * a30 * aa[8];
+  h = h - (acc)0.25 * dxb * dyb * a3x * aa[9];
+  h = h - (acc)0.25 * dxb * dyb * a3xy * aa[10];
+  h = h - (acc)0.25 * dxb * dyb * a3y * aa[11];
   return h / cv;
 }
 
@@ -338,27 +341,28 @@ __global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, Box bx0, const real 
       const int c = g.idx(i, j, k);
       const Node4 hx = node4(vx, c, Y), hy = node4(vy, c, Y), hz = node4(vz, c, Y);
       const Al12 ahi = al12(al1, al2, c + Z, Y);
-      double aa[12];
+      acc aa[12];
       // |v(first) - v(second)|^2 per edge
 #define AAV(q, P1, N1, P2, N2)                                                                    \
   {                                                                                               \
-    const double dx = P1##x.N1 - P2##x.N2, dy = P1##y.N1 - P2##y.N2, dz = P1##z.N1 - P2##z.N2; \
+    const acc dx = P1##x.N1 - P2##x.N2, dy = P1##y.N1 - P2##y.N2, dz = P1##z.N1 - P2##z.N2;    \
     aa[q] = dx * dx + dy * dy + dz * dz;                                                          \
   }
       EDGES(AAV)
 #undef AAV
-      const double a30 = al3[c], a3x = al3[c - 1], a3xy = al3[c - 1 - Y], a3y = al3[c - Y];
-      double pv = 0.0;
-      pv = dmax(pv, -(double)alo.a1 * sqrt(aa[0]));
-      pv = dmax(pv, -(double)alo.a2 * sqrt(aa[1]));
+      const acc a30 = al3[c], a3x = al3[c - 1], a3xy = al3[c - 1 - Y], a3y = al3[c - Y];
+      acc pv = 0.0;
+      pv = dmax(pv, -(acc)alo.a1 * sqrt(aa[0]));
+      pv = dmax(pv, -(acc)alo.a2 * sqrt(aa[1]));
       pv = dmax(pv, -a30 * sqrt(aa[8]));
-      visc_heat[c] = edge_heat_v(g.dxb[i], g.dyb[j], g.dzb[k], g.cv(i, j, k), alo, ahi, a30, a3x, a3xy, a3y, aa);
-      const double r = rho[c];
-      const double p = (a.gamma - 1.0) * r * (energy[c] - 0.0);
+      visc_heat[c] = edge_heat_v((acc)g.dxb[i], (acc)g.dyb[j], (acc)g.dzb[k], (acc)g.cv(i, j, k), alo, ahi, a30, a3x,
+                                 a3xy, a3y, aa);
+      const acc r = rho[c];
+      const acc p = ((acc)a.gamma - (acc)1.0) * r * (energy[c] - (acc)0.0);
       p0[c] = p;
       if (i <= g.nx && j <= g.ny && k <= g.nz) {
         const double w1 = bx[c] * bx[c] + by[c] * by[c] + bz[c] * bz[c];
-        const double rho0 = dmax(r, NONE_ZERO);
+        const double rho0 = dmax((double)r, NONE_ZERO);
         double g_b = 1.0;
         if (a.boris) {
           const double w2 = w1 / rho0;
@@ -366,7 +370,7 @@ __global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, Box bx0, const real 
           if (a.write_gb) gbo[c] = (real)g_b;
           g_b = (real)g_b;
         }
-        const double cs2 = (a.gamma * p + g_b * w1) / rho0;
+        const double cs2 = (a.gamma * (double)p + g_b * w1) / rho0;
         const double c_visc2 = pv / rho0;
         const double dxb = g.dxb[i], dyb = g.dyb[j], dzb = g.dzb[k];
         const double length = dmin(dmin(dxb, dyb), dzb);
@@ -455,6 +459,25 @@ __device__ __forceinline__ void vertex_current_vals(double dxc, double dyc, doub
   jz = (jz1 + jz2) * 0.5;
 }
 
+// The same vertex current in the arithmetic type A: the differences of
+// the FP64 face field are taken in FP64 and divided by the widths in A (for
+// A = double, the operations of vertex_current_vals).
+template <class A>
+__device__ __forceinline__ void vertex_current_acc(double dxc, double dyc, double dzc, const BPlane &lo,
+                                                   const BPlane &hi, double bz0, double bzx, double bzy, double bzxy,
+                                                   A &jx, A &jy, A &jz) {
+  const A ax = (A)dxc, ay = (A)dyc, az = (A)dzc;
+  const A jx1 = (A)(bzy - bz0) / ay - (A)(hi.by0 - lo.by0) / az;
+  const A jx2 = (A)(bzxy - bzx) / ay - (A)(hi.byx - lo.byx) / az;
+  const A jy1 = (A)(hi.bx0 - lo.bx0) / az - (A)(bzx - bz0) / ax;
+  const A jy2 = (A)(hi.bxy - lo.bxy) / az - (A)(bzxy - bzy) / ax;
+  const A jz1 = (A)(lo.byx - lo.by0) / ax - (A)(lo.bxy - lo.bx0) / ay;
+  const A jz2 = (A)(hi.byx - hi.by0) / ax - (A)(hi.bxy - hi.bx0) / ay;
+  jx = (jx1 + jx2) * (A)0.5;
+  jy = (jy1 + jy2) * (A)0.5;
+  jz = (jz1 + jz2) * (A)0.5;
+}
+
 // the same vertex current, read from memory at node (i, j, k)
 __device__ __forceinline__ void vertex_current(const Geo &g, const double *__restrict__ bx,
                                                const double *__restrict__ by, const double *__restrict__ bz, int i,
@@ -476,9 +499,10 @@ __global__ __launch_bounds__(BX * BY) void k_eta(Geo g, Box b, const double *__r
   for (int k = k0; k <= k1; ++k) {
     const int c = g.idx(i, j, k);
     const BPlane hi = bplane(bx, by, c + Z, Y);
-    double jx, jy, jz;
-    vertex_current_vals(g.dxc[i], g.dyc[j], g.dzc[k], lo, hi, bz[c], bz[c + 1], bz[c + Y], bz[c + 1 + Y], jx, jy, jz);
-    eta[c] = sqrt(jx * jx + jy * jy + jz * jz) > j_max ? eta_bg + eta0 : eta_bg;
+    acc jx, jy, jz;
+    vertex_current_acc<acc>(g.dxc[i], g.dyc[j], g.dzc[k], lo, hi, bz[c], bz[c + 1], bz[c + Y], bz[c + 1 + Y], jx, jy,
+                            jz);
+    eta[c] = sqrt(jx * jx + jy * jy + jz * jz) > (acc)j_max ? eta_bg + eta0 : eta_bg;
     lo = hi;
   }
 }
@@ -491,7 +515,7 @@ __global__ __launch_bounds__(BX * BY) void k_fill(Geo g, Box b, real *__restrict
 // EMFs around a node in one plane, named as in bstep (lagran.F90:1582-1631):
 // kx at (i,j), (i-1,j); ky at (i,j), (i,j-1); kz at (i,j), (i,j-1), (i-1,j)
 struct EPlane {
-  double kx0, kxm, ky0, kym, kz0, kzm, kzx;
+  acc kx0, kxm, ky0, kym, kz0, kzm, kzx;
 };
 
 // One resistive substep in a single pass: at every node of the box
@@ -520,7 +544,7 @@ __global__ __launch_bounds__(64 * RS_ROWS) void k_resist(Geo g, Box b, int tiles
                                                          double *__restrict__ partial, double dt, double eta_bg,
                                                          double eta0, double j_max) {
   // EMFs kx, ky, kz and curlb of the tile's nodes, by plane parity
-  __shared__ double sk[2][4][RS_ROWS][64];
+  __shared__ acc sk[2][4][RS_ROWS][64];
   __shared__ double red[RS_ROWS];
   const int l = (int)threadIdx.x, w = (int)threadIdx.y;
   const int t = (int)xcd_tile(blockIdx.x, gridDim.x);
@@ -537,28 +561,28 @@ __global__ __launch_bounds__(64 * RS_ROWS) void k_resist(Geo g, Box b, int tiles
   const bool hij = i >= 1 && i <= g.nx && j >= 1 && j <= g.ny;  // heated cells, in the plane
   double v = 0.0;
   EPlane m = {0, 0, 0, 0, 0, 0, 0};  // plane k-1
-  double mq[4] = {0, 0, 0, 0};       // curlb of plane k-1 at (i,j), (i-1,j), (i,j-1), (i-1,j-1)
+  acc mq[4] = {0, 0, 0, 0};          // curlb of plane k-1 at (i,j), (i-1,j), (i,j-1), (i-1,j-1)
   BPlane lo = {0, 0, 0, 0};
   bool have_lo = false;
   for (int k = k0 - 1; k <= k1; ++k) {
     const int c = g.idx(i, j, k);
-    double kx = 0.0, ky = 0.0, kz = 0.0, cbr = 0.0;
+    acc kx = 0.0, ky = 0.0, kz = 0.0, cbr = 0.0;
     if (eij && k >= -1 && k <= g.nz + 1) {
       if (!have_lo) lo = bplane(bx, by, c, Y);
       const BPlane hi = bplane(bx, by, c + Z, Y);
-      double jx, jy, jz;
-      vertex_current_vals(g.dxc[i], g.dyc[j], g.dzc[k], lo, hi, bz[c], bz[c + 1], bz[c + Y], bz[c + 1 + Y], jx, jy,
-                          jz);
+      acc jx, jy, jz;
+      vertex_current_acc<acc>(g.dxc[i], g.dyc[j], g.dzc[k], lo, hi, bz[c], bz[c + 1], bz[c + Y], bz[c + 1 + Y], jx,
+                              jy, jz);
       lo = hi;
       have_lo = true;
-      const real es = sqrt(jx * jx + jy * jy + jz * jz) > j_max ? eta_bg + eta0 : eta_bg;
+      const real es = sqrt(jx * jx + jy * jy + jz * jz) > (acc)j_max ? eta_bg + eta0 : eta_bg;
       if (out && k >= k0) eta[c] = es;
       if (kij && k >= 0 && k <= g.nz) {
-        const double e = es;
-        kx = -jx * e * g.dxc[i] * 0.5;
-        ky = -jy * e * g.dyc[j] * 0.5;
-        kz = -jz * e * g.dzc[k] * 0.5;
-        const double cb = e * (jx * jx + jy * jy + jz * jz);
+        const acc e = es;
+        kx = -jx * e * (acc)g.dxc[i] * (acc)0.5;
+        ky = -jy * e * (acc)g.dyc[j] * (acc)0.5;
+        kz = -jz * e * (acc)g.dzc[k] * (acc)0.5;
+        const acc cb = e * (jx * jx + jy * jy + jz * jz);
         cbr = (real)cb;
         if (out && k >= k0) {
           if (curlb_out) curlb_out[c] = cbr;
@@ -570,7 +594,7 @@ __global__ __launch_bounds__(64 * RS_ROWS) void k_resist(Geo g, Box b, int tiles
         }
       }
     }
-    double(&s)[4][RS_ROWS][64] = sk[k & 1];
+    acc(&s)[4][RS_ROWS][64] = sk[k & 1];
     s[0][w][l] = kx;
     s[1][w][l] = ky;
     s[2][w][l] = kz;
@@ -584,15 +608,15 @@ __global__ __launch_bounds__(64 * RS_ROWS) void k_resist(Geo g, Box b, int tiles
         double nbx = bx[c], nby = by[c], nbz = bz[c];
         if (upd && j >= 1 && k >= 1) {
           const double area = g.dyb[j] * g.dzb[k];
-          nbx = nbx + (p.kz0 - p.kzm + m.kz0 - m.kzm - p.ky0 + m.ky0 - p.kym + m.kym) * dt / area;
+          nbx = nbx + ((double)p.kz0 - p.kzm + m.kz0 - m.kzm - p.ky0 + m.ky0 - p.kym + m.kym) * dt / area;
         }
         if (upd && i >= 1 && k >= 1) {
           const double area = g.dxb[i] * g.dzb[k];
-          nby = nby + (p.kx0 - m.kx0 + p.kxm - m.kxm - p.kz0 + p.kzx - m.kz0 + m.kzx) * dt / area;
+          nby = nby + ((double)p.kx0 - m.kx0 + p.kxm - m.kxm - p.kz0 + p.kzx - m.kz0 + m.kzx) * dt / area;
         }
         if (upd && i >= 1 && j >= 1) {
           const double area = g.dxb[i] * g.dyb[j];
-          nbz = nbz + (p.ky0 - s[1][w][l - 1] + p.kym - s[1][w - 1][l - 1] - p.kx0 + s[0][w - 1][l] - p.kxm +
+          nbz = nbz + ((double)p.ky0 - s[1][w][l - 1] + p.kym - s[1][w - 1][l - 1] - p.kx0 + s[0][w - 1][l] - p.kxm +
                        s[0][w - 1][l - 1]) *
                           dt / area;
         }
@@ -600,13 +624,13 @@ __global__ __launch_bounds__(64 * RS_ROWS) void k_resist(Geo g, Box b, int tiles
         byo[c] = nby;
         bzo[c] = nbz;
       }
-      const double pq[4] = {cbr, s[3][w][l - 1], s[3][w - 1][l], s[3][w - 1][l - 1]};
+      const acc pq[4] = {cbr, s[3][w][l - 1], s[3][w - 1][l], s[3][w - 1][l - 1]};
       if (k >= k0 && cij && k >= -1 && k <= g.nz + 2) {
-        double e = energy[c];
+        acc e = energy[c];
         if (hij && k >= 1 && k <= g.nz) {
           // curlb(c) + (c-X) + (c-Y) + (c-X-Y) + (c-Z) + (c-X-Z) + (c-Y-Z) + (c-X-Y-Z)
-          const double sq = pq[0] + pq[1] + pq[2] + pq[3] + mq[0] + mq[1] + mq[2] + mq[3];
-          const double lh = sq * dt / (8.0 * rho[c]);
+          const acc sq = pq[0] + pq[1] + pq[2] + pq[3] + mq[0] + mq[1] + mq[2] + mq[3];
+          const acc lh = sq * (acc)dt / ((acc)8.0 * rho[c]);
           ohmic_dep[c] = ohmic_dep[c] + lh;
           e = e + lh;
         }
@@ -680,7 +704,7 @@ __global__ __launch_bounds__(BX * BY) void k_force(Geo g, Box b, const real *__r
   real a3lo = al3[cb];
   for (int k = k0; k <= k1; ++k) {
     const int c = g.idx(i, j, k);
-    const double dxc = g.dxc[i], dyc = g.dyc[j], dzc = g.dzc[k];
+    const acc dxc = (acc)g.dxc[i], dyc = (acc)g.dyc[j], dzc = (acc)g.dzc[k];
     real up[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) up[q] = V[q][c + Z];
@@ -689,48 +713,51 @@ __global__ __launch_bounds__(BX * BY) void k_force(Geo g, Box b, const real *__r
     const double bz0 = bz[c], bzx = bz[c + X], bzy = bz[c + Y], bzxy = bz[c + X + Y];
     const real a3hi = al3[c + Z];
     // viscous forces
-    const double a1 = al1[c + Y + Z] * dyc * dzc;
-    const double a2 = al1[c + X + Y + Z] * dyc * dzc;
-    const double a3 = al2[c + Z] * dxc * dzc;
-    const double a4 = al2[c + Y + Z] * dxc * dzc;
-    const double a5 = a3lo * dxc * dyc;
-    const double a6 = a3hi * dxc * dyc;
-    const double cvv = 0.125 * cv_sum8(g, i, j, k);
-    double fv[3];
+    const acc a1 = al1[c + Y + Z] * dyc * dzc;
+    const acc a2 = al1[c + X + Y + Z] * dyc * dzc;
+    const acc a3 = al2[c + Z] * dxc * dzc;
+    const acc a4 = al2[c + Y + Z] * dxc * dzc;
+    const acc a5 = a3lo * dxc * dyc;
+    const acc a6 = a3hi * dxc * dyc;
+    const acc cvv = (acc)0.125 * (acc)cv_sum8(g, i, j, k);
+    acc fv[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       const real *u = V[q];
-      const double uc = u0[q];
+      const acc uc = u0[q];
       fv[q] = (a1 * (uc - u[c - X]) + a2 * (uc - u[c + X]) + a3 * (uc - u[c - Y]) + a4 * (uc - u[c + Y]) +
                a5 * (uc - um[q]) + a6 * (uc - up[q])) /
               cvv;
     }
-    const double p = plo.p0, ppx = plo.px, ppy = plo.py, ppz = phi.p0;
-    const double ppxy = plo.pxy, ppxz = phi.px, ppyz = phi.py, ppxyz = phi.pxy;
-    double w1 = (p + ppy + ppz + ppyz) * 0.25;
-    double w2 = (ppx + ppxy + ppxz + ppxyz) * 0.25;
-    double fx = -(w2 - w1) / dxc;
-    w1 = (p + ppx + ppz + ppxz) * 0.25;
-    w2 = (ppy + ppxy + ppyz + ppxyz) * 0.25;
-    double fy = -(w2 - w1) / dyc;
-    w1 = (p + ppx + ppy + ppxy) * 0.25;
-    w2 = (ppz + ppxz + ppyz + ppxyz) * 0.25;
-    double fz = -(w2 - w1) / dzc;
+    const acc q4 = 0.25;
+    const acc p = plo.p0, ppx = plo.px, ppy = plo.py, ppz = phi.p0;
+    const acc ppxy = plo.pxy, ppxz = phi.px, ppyz = phi.py, ppxyz = phi.pxy;
+    acc w1 = (p + ppy + ppz + ppyz) * q4;
+    acc w2 = (ppx + ppxy + ppxz + ppxyz) * q4;
+    acc fx = -(w2 - w1) / dxc;
+    w1 = (p + ppx + ppz + ppxz) * q4;
+    w2 = (ppy + ppxy + ppyz + ppxyz) * q4;
+    acc fy = -(w2 - w1) / dyc;
+    w1 = (p + ppx + ppy + ppxy) * q4;
+    w2 = (ppz + ppxz + ppyz + ppxyz) * q4;
+    acc fz = -(w2 - w1) / dzc;
     // bx(c) + bx(c+Y) + bx(c+Z) + bx(c+Y+Z), and likewise for by and bz
-    const double bxv = 0.25 * (blo.bx0 + blo.bxy + bhi.bx0 + bhi.bxy);
-    const double byv = 0.25 * (blo.by0 + blo.byx + bhi.by0 + bhi.byx);
-    const double bzv = 0.25 * (bz0 + bzy + bzx + bzxy);
-    double jx, jy, jz;
-    vertex_current_vals(dxc, dyc, dzc, blo, bhi, bz0, bzx, bzy, bzxy, jx, jy, jz);
-    const double g_b = gb ? (double)gb[c] : 1.0;
+    // (the FP64 face field averaged in FP64)
+    const acc bxv = 0.25 * (blo.bx0 + blo.bxy + bhi.bx0 + bhi.bxy);
+    const acc byv = 0.25 * (blo.by0 + blo.byx + bhi.by0 + bhi.byx);
+    const acc bzv = 0.25 * (bz0 + bzy + bzx + bzxy);
+    acc jx, jy, jz;
+    vertex_current_acc<acc>(g.dxc[i], g.dyc[j], g.dzc[k], blo, bhi, bz0, bzx, bzy, bzxy, jx, jy, jz);
+    const acc g_b = gb ? (acc)gb[c] : (acc)1.0;
     fx = fx + g_b * (jy * bzv - jz * byv);
     fy = fy + g_b * (jz * bxv - jx * bzv);
     fz = fz + g_b * (jx * byv - jy * bxv);
-    const double rv = rho_v[c];
-    fz = fz - rv * g.grav[k];
-    vx1[c] = u0[0] + dt2 * (fv[0] + fx) / rv;
-    vy1[c] = u0[1] + dt2 * (fv[1] + fy) / rv;
-    vz1[c] = u0[2] + dt2 * (fv[2] + fz) / rv;
+    const acc rv = rho_v[c];
+    fz = fz - rv * (acc)g.grav[k];
+    const acc h = (acc)dt2;
+    vx1[c] = u0[0] + h * (fv[0] + fx) / rv;
+    vy1[c] = u0[1] + h * (fv[1] + fy) / rv;
+    vz1[c] = u0[2] + h * (fv[2] + fz) / rv;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       um[q] = u0[q];
@@ -772,45 +799,49 @@ __global__ __launch_bounds__(BX * BY) void k_shock_corr(Geo g, Box b, const real
       const Node4 hx = node4(vx, c, Y), hy = node4(vy, c, Y), hz = node4(vz, c, Y);
       const Node4 hx1 = node4(vx1, c, Y), hy1 = node4(vy1, c, Y), hz1 = node4(vz1, c, Y);
       const Al12 ahi = al12(al1, al2, c + Z, Y);
-      double aa[12];
+      acc aa[12];
       // (v(first) - v(second)) . (v1(first) - v1(second)) per edge
 #define AAS(q, P1, N1, P2, N2)                                                            \
   aa[q] = (P1##x.N1 - P2##x.N2) * (P1##x1.N1 - P2##x1.N2) + (P1##y.N1 - P2##y.N2) * (P1##y1.N1 - P2##y1.N2) + \
           (P1##z.N1 - P2##z.N2) * (P1##z1.N1 - P2##z1.N2);
       EDGES(AAS)
 #undef AAS
-      const double a30 = al3[c], a3x = al3[c - 1], a3xy = al3[c - 1 - Y], a3y = al3[c - Y];
-      const double vh =
-          max0(edge_heat_v(g.dxb[i], g.dyb[j], g.dzb[k], g.cv(i, j, k), alo, ahi, a30, a3x, a3xy, a3y, aa));
+      const acc a30 = al3[c], a3x = al3[c - 1], a3xy = al3[c - 1 - Y], a3y = al3[c - Y];
+      const acc vh = max0(edge_heat_v((acc)g.dxb[i], (acc)g.dyb[j], (acc)g.dzb[k], (acc)g.cv(i, j, k), alo, ahi, a30,
+                                      a3x, a3xy, a3y, aa));
       // face-averaged v1 (quad sums in the order c, c-Y, c-Z, c-Y-Z etc.;
       // node C = c, B = c-Y, D = c-X, A = c-X-Y)
-      const double vxb = (hx1.c + hx1.b + lx1.c + lx1.b) * 0.25;
-      const double vxbm = (hx1.d + hx1.a + lx1.d + lx1.a) * 0.25;
-      const double vyb = (hy1.c + hy1.d + ly1.c + ly1.d) * 0.25;
-      const double vybm = (hy1.b + hy1.a + ly1.b + ly1.a) * 0.25;
-      const double vzb = (hz1.c + hz1.d + hz1.b + hz1.a) * 0.25;
-      const double vzbm = (lz1.c + lz1.d + lz1.b + lz1.a) * 0.25;
-      const double dv = ((vxb - vxbm) / g.dxb[i] + (vyb - vybm) / g.dyb[j] + (vzb - vzbm) / g.dzb[k]) * dt;
-      const double r = rho[c];
-      const real e1 = energy[c] + (dt * vh - dv * pp[c]) / r;
-      visc_dep[c] = visc_dep[c] + dt * vh;
-      const real rn = r / (1.0 + dv);
+      const acc q4 = 0.25;
+      const acc vxb = (hx1.c + hx1.b + lx1.c + lx1.b) * q4;
+      const acc vxbm = (hx1.d + hx1.a + lx1.d + lx1.a) * q4;
+      const acc vyb = (hy1.c + hy1.d + ly1.c + ly1.d) * q4;
+      const acc vybm = (hy1.b + hy1.a + ly1.b + ly1.a) * q4;
+      const acc vzb = (hz1.c + hz1.d + hz1.b + hz1.a) * q4;
+      const acc vzbm = (lz1.c + lz1.d + lz1.b + lz1.a) * q4;
+      const acc dta = (acc)dt;
+      const acc dv =
+          ((vxb - vxbm) / (acc)g.dxb[i] + (vyb - vybm) / (acc)g.dyb[j] + (vzb - vzbm) / (acc)g.dzb[k]) * dta;
+      const acc r = rho[c];
+      const real e1 = energy[c] + (dta * vh - dv * pp[c]) / r;
+      visc_dep[c] = visc_dep[c] + dta * vh;
+      const real rn = r / ((acc)1.0 + dv);
       rho[c] = rn;
       if (cool_v) {
         // a running average of the viscous heating is removed again, and
         // energy = MAX(energy, 0)
-        const real ct = alpha_av * dt * vh / rn + (1.0 - alpha_av) * cool_v[c];
+        const acc aav = (acc)alpha_av;
+        const real ct = aav * dta * vh / rn + ((acc)1.0 - aav) * cool_v[c];
         cool_v[c] = ct;
-        energy[c] = max0(e1 - ct);
+        energy[c] = max0((acc)(e1 - ct));
       } else {
         energy[c] = e1;
       }
-      v[0] += dt * vh * g.cv(i, j, k);
+      v[0] += dt * (double)vh * g.cv(i, j, k);
       // v = 2 v1 - v (:399-407) at node (i, j, k); k_vupdate_low does the
       // nodes with an index 0
-      vxo[c] = (real)(2.0 * hx1.c - hx.c);
-      vyo[c] = (real)(2.0 * hy1.c - hy.c);
-      vzo[c] = (real)(2.0 * hz1.c - hz.c);
+      vxo[c] = (real)((acc)2.0 * hx1.c - hx.c);
+      vyo[c] = (real)((acc)2.0 * hy1.c - hy.c);
+      vzo[c] = (real)((acc)2.0 * hz1.c - hz.c);
       lx = hx;
       ly = hy;
       lz = hz;

@@ -45,16 +45,20 @@ int fail(int code, const char *fmt, ...) {
 }  // namespace
 
 #define LARE_REAL double
+#define LARE_ACC double
 namespace fp64 {
 #include "runtime.hpp"
 }  // namespace fp64
 #undef LARE_REAL
+#undef LARE_ACC
 
 #define LARE_REAL float
+#define LARE_ACC float
 namespace fp32 {
 #include "runtime.hpp"
 }  // namespace fp32
 #undef LARE_REAL
+#undef LARE_ACC
 
 #include "noalloc.hpp"
 

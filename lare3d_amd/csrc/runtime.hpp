@@ -7,6 +7,11 @@
 // metrics and every reduction stay FP64, and kernels do their arithmetic in
 // FP64 registers.  No include guard: one inclusion per namespace.
 typedef LARE_REAL real;
+// Arithmetic type of the bulk kernels that have been given one: double in
+// the FP64 mode (the reference's arithmetic), float in the FP32 mode.  The
+// face field's CT updates, the limiter comparisons and every reduction
+// stay FP64 in both.
+typedef LARE_ACC acc;
 
 #include "common.hpp"
 #include "bc.hpp"
