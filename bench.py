@@ -367,6 +367,8 @@ def main():
     ap.add_argument("--no-conduction", action="store_true",
                     help="skip the conduction leg (RKL2 vs the learned surrogate)")
     ap.add_argument("--no-kink", action="store_true", help="skip the twisted flux tube leg")
+    ap.add_argument("--cond-stages", type=int, default=7,
+                    help="RKL2 stage count the conduction leg's super-step is sized for")
     ap.add_argument("--kink-n", type=int, default=512, help="flux tube size n (grid n x n x 2n)")
     ap.add_argument("--kink-steps", type=int, default=8)
     ap.add_argument("--copy-probe", action="store_true",
@@ -420,7 +422,7 @@ def main():
         fp32 = fp32_run(n, args.warmup, args.steps, f64_end)
         fp32["speedup_vs_fp64"] = fp32["value"] / cups
 
-    cond = None if args.no_conduction else conduction_leg(n)
+    cond = None if args.no_conduction else conduction_leg(n, s_target=args.cond_stages)
     kink = None if args.no_kink else kink_leg(args.kink_n, 2, args.kink_steps)
 
     cpu = None
