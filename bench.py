@@ -10,15 +10,16 @@ noise, shock viscosity on, periodic x/z and BC_USER walls in y.
 
 Prints one JSON line: cell-updates/s of the FP64 path over the timed steps
 (inputs resident in HBM), the roofline of the dominant kernel from HIP events
-on the library stream, the same workload with FP32 bulk storage ("fp32":
-throughput and the relative L2 error of every field against the FP64 run
-after the same steps), and the CPU restatement timed on a bounded sample of
-the same workload as a reported baseline.
+on the library stream, the same workload in the FP32 mode (FP32 bulk storage
+and arithmetic; "fp32": throughput, per-kernel times and the relative L2
+error of every field against the FP64 run after the same steps), and the CPU
+restatement timed on a bounded sample of the same workload as a reported
+baseline.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
 
 Also the kink-unstable twisted flux tube at 512 x 512 x 1024 (resistive
-with conduction) in FP32 storage against FP64 ("kink"), and the conduction
+with conduction) in the FP32 mode against FP64 ("kink"), and the conduction
 super-step with the learned surrogate ("conduction").
 """
 import argparse
@@ -166,10 +167,19 @@ def fp32_run(n, warmup, steps, f64_end):
     grid, params, fields = configs.harris(n)
     params.precision = PREC_FP32
     dev = Device(grid, params, fields)
-    cups, ms, _ = timed_run(dev, warmup, steps)
+    cups, ms, step = timed_run(dev, warmup, steps)
     f32_end = dev.fields()
     _, divb = dev.energy_account(with_divb=True)
+    # per-kernel durations of three more steps (HIP events on the library stream)
+    dev.profile(True)
+    for _ in range(3):
+        step += 1
+        dev.step(step)
+    stats = dev.kernel_stats()
+    dev.profile(False)
     dev.close()
+    ranked = sorted(stats.items(), key=lambda kv: -kv[1][1])
+    top = {k: {"launches": v[0], "ms": round(v[1], 3)} for k, v in ranked[:12]}
     errs = {}
     # scalar fields, and the velocity and magnetic field as vectors
     for name, comps in (("rho", ("rho",)), ("energy", ("energy",)), ("v", ("vx", "vy", "vz")),
@@ -178,8 +188,10 @@ def fp32_run(n, warmup, steps, f64_end):
         b = np.concatenate([f64_end.interior(c).ravel() for c in comps])
         errs[name] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
     return {"value": cups, "unit": "cell-updates/s", "ms_per_step": ms,
-            "storage": "FP32 bulk fields; FP64 face B, CT fluxes, metrics, reductions and arithmetic",
-            "field_rel_l2_vs_fp64": errs, "steps_compared": warmup + steps, "max_divb_dx": divb}
+            "storage": "FP32 bulk fields and FP32 bulk arithmetic; FP64 face B and its CT path (quotients, "
+                       "fluxes, update), Van Leer limiter comparisons, metrics and reductions",
+            "field_rel_l2_vs_fp64": errs, "steps_compared": warmup + steps, "max_divb_dx": divb,
+            "kernels": top}
 
 
 def _vector_errors(fa, fb):
@@ -196,7 +208,7 @@ def _vector_errors(fa, fb):
 
 def kink_leg(n, warmup, steps):
     """The kink-unstable twisted flux tube (n x n x 2n; resistive with
-    Braginskii conduction) in FP32 storage and in FP64: throughput, the
+    Braginskii conduction) in the FP32 mode and in FP64: throughput, the
     energy budget and div B of each run, and the field errors of the FP32
     run against the FP64 run after the same steps."""
     from lare3d_amd import configs
@@ -247,7 +259,7 @@ def kink_leg(n, warmup, steps):
 def conduction_leg(n, seed=1, kappa_0=0.5, s_target=7, rows_per_state=17000, train_steps=6000):
     """Braginskii conduction super-steps on the Harris sheet with seeded hot
     spots (n^3): the plain RKL2 solver against the learned surrogate, in FP64
-    and FP32 storage.  The surrogate is trained here, on the device, from
+    and the FP32 mode.  The surrogate is trained here, on the device, from
     three seeded states (rows and RKL2 targets collected by the library),
     and evaluated on a fourth.  Reports super-step times, the temperature
     error of the surrogate against RKL2, and its residual-check outcome."""
@@ -363,7 +375,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=96)
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage run")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-mode run")
     ap.add_argument("--no-conduction", action="store_true",
                     help="skip the conduction leg (RKL2 vs the learned surrogate)")
     ap.add_argument("--no-kink", action="store_true", help="skip the twisted flux tube leg")

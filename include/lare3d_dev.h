@@ -84,6 +84,8 @@ typedef struct lare_params {
   int32_t shock_limiter;   /* the SHOCKLIMITER variant of edge_viscosity
                               (lagran.F90:801-803, :831-845)              */
   int32_t shock_compression; /* the SHOCKCOMPRESSION variant (:807-808)   */
+  int32_t rke;             /* remap kinetic-energy correction (control.f90:148;
+                              xremap.f90:664-695, diagnostics.F90:613-629) */
 } lare_params;
 
 /* Field pointers (Fortran arrays, see the array convention above).
@@ -162,6 +164,11 @@ int lare_dev_eulerian_remap(int32_t step);
 /* lagran.F90:1176 eta_calc. */
 int lare_dev_eta_calc(void);
 
+/* diagnostics.F90:613-629 energy_correction (called by lare3d.f90:86 after
+ * eulerian_remap when rke): the kinetic energy the last remap lost added to
+ * the internal energy, then energy_bcs.  LARE_E_STATE when rke is off. */
+int lare_dev_energy_correction(void);
+
 /* conduct.f90:39 conduct_heat -- one RKL2 super-step of length dt. */
 int lare_dev_conduct_heat(void);
 
@@ -191,7 +198,8 @@ int lare_dev_energy_account_noalloc(const lare_grid *grid, const lare_fields *de
                                     double *divb);
 
 /* One full iteration of the lare3d.f90:82-88 loop body:
- * lagrangian_step, eulerian_remap(step), eta_calc.  Returns dt and time. */
+ * lagrangian_step, eulerian_remap(step), energy_correction when rke is on,
+ * eta_calc.  Returns dt and time. */
 int lare_dev_step(int32_t step, double *dt, double *time);
 
 /* initial_conditions.f90:114 potential_field on the device: replaces the

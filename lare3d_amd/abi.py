@@ -34,7 +34,7 @@ class LareParams(C.Structure):
                 ("restart", C.c_int32), ("precision", C.c_int32),
                 ("alpha_av", C.c_double), ("va_max", C.c_double),
                 ("cooling_term", C.c_int32), ("boris", C.c_int32),
-                ("shock_limiter", C.c_int32), ("shock_compression", C.c_int32)]
+                ("shock_limiter", C.c_int32), ("shock_compression", C.c_int32), ("rke", C.c_int32)]
 
 
 class LareStepInfo(C.Structure):
@@ -82,6 +82,7 @@ DEVICE_SYMBOLS = {
     "lare_dev_lagrangian_step": [],
     "lare_dev_eulerian_remap": [C.c_int32],
     "lare_dev_eta_calc": [],
+    "lare_dev_energy_correction": [],
     "lare_dev_conduct_heat": [],
     "lare_dev_boundary_conditions": [],
     "lare_dev_energy_account": [_dp, _dp],

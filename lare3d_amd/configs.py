@@ -41,6 +41,7 @@ def default_params(**kw):
     p.boris = 0
     p.shock_limiter = 0
     p.shock_compression = 0
+    p.rke = 0
     for k, v in kw.items():
         setattr(p, k, v)
     return p

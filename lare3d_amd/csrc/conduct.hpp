@@ -72,6 +72,10 @@ __device__ __forceinline__ double pow15(double t) {
   const double pl = fma(t, r, -p);
   return p + (pl + t * rl);
 }
+// The same powers in FP32 (FP32 arithmetic mode): a few ulp, with t^1.5
+// formed first so that t^2.5 overflows only where the result does.
+__device__ __forceinline__ float pow25(float t) { return t * sqrtf(t) * t; }
+__device__ __forceinline__ float pow15(float t) { return t * sqrtf(t); }
 
 struct CondArgs {
   double kappa_0, tfac, dt;
@@ -125,69 +129,69 @@ __global__ __launch_bounds__(BX * BY) void k_larsen(Geo g, const real *__restric
 // operation order.
 
 struct Tn {  // 3x3x3 temperature neighbourhood of a cell (corners unused)
-  double t[3][3][3];
+  acc t[3][3][3];
 };
 
 // x face between (ox,oy,oz)+(0,0,0) and +(1,0,0) relative to centre index
 // (1,1,1) of the neighbourhood, lower cell at neighbourhood x index lx.
-__device__ __forceinline__ double flux_x_face(const Geo &g, const Tn &T, int lx, int ix, int iy, int iz,
+__device__ __forceinline__ acc flux_x_face(const Geo &g, const Tn &T, int lx, int ix, int iy, int iz,
                                               const double *__restrict__ bx, const double *__restrict__ by,
-                                              const double *__restrict__ bz, double L, double kappa_0) {
+                                              const double *__restrict__ bz, acc L, acc kappa_0) {
   const int c = g.idx(ix, iy, iz);
-  const double byf = 0.25 * (by[c] + by[c + 1] + by[c - g.sy] + by[c + 1 - g.sy]);
-  const double bzf = 0.25 * (bz[c] + bz[c + 1] + bz[c - g.sz] + bz[c + 1 - g.sz]);
-  const double bxc = bx[c];
-  const double modb = bxc * bxc + byf * byf + bzf * bzf + MIN_B;
-  double tb = (T.t[lx + 1][1][1] + T.t[lx][1][1]) / 2.0;
-  double tb_p = (T.t[lx + 1][2][1] + T.t[lx][2][1]) / 2.0;
-  double tb_m = (T.t[lx + 1][0][1] + T.t[lx][0][1]) / 2.0;
-  const double tg_a1 = (tb_p - tb_m) / (g.dyc[iy] + g.dyc[iy - 1]);
-  tb_p = (T.t[lx + 1][1][2] + T.t[lx][1][2]) / 2.0;
-  tb_m = (T.t[lx + 1][1][0] + T.t[lx][1][0]) / 2.0;
-  const double tg_a2 = (tb_p - tb_m) / (g.dzc[iz] + g.dzc[iz - 1]);
-  const double tg = (T.t[lx + 1][1][1] - T.t[lx][1][1]) / g.dxc[ix];
-  tb = dmax(tb, 0.0);
-  return -L * kappa_0 * pow25(tb) * (bxc * (tg * bxc + tg_a1 * byf + tg_a2 * bzf) + tg * MIN_B) / modb;
+  const acc byf = (acc)(0.25 * (by[c] + by[c + 1] + by[c - g.sy] + by[c + 1 - g.sy]));
+  const acc bzf = (acc)(0.25 * (bz[c] + bz[c + 1] + bz[c - g.sz] + bz[c + 1 - g.sz]));
+  const acc bxc = (acc)bx[c];
+  const acc modb = bxc * bxc + byf * byf + bzf * bzf + (acc)MIN_B;
+  acc tb = (T.t[lx + 1][1][1] + T.t[lx][1][1]) / (acc)2.0;
+  acc tb_p = (T.t[lx + 1][2][1] + T.t[lx][2][1]) / (acc)2.0;
+  acc tb_m = (T.t[lx + 1][0][1] + T.t[lx][0][1]) / (acc)2.0;
+  const acc tg_a1 = (tb_p - tb_m) / (acc)(g.dyc[iy] + g.dyc[iy - 1]);
+  tb_p = (T.t[lx + 1][1][2] + T.t[lx][1][2]) / (acc)2.0;
+  tb_m = (T.t[lx + 1][1][0] + T.t[lx][1][0]) / (acc)2.0;
+  const acc tg_a2 = (tb_p - tb_m) / (acc)(g.dzc[iz] + g.dzc[iz - 1]);
+  const acc tg = (T.t[lx + 1][1][1] - T.t[lx][1][1]) / (acc)g.dxc[ix];
+  tb = dmax(tb, (acc)0.0);
+  return -L * kappa_0 * pow25(tb) * (bxc * (tg * bxc + tg_a1 * byf + tg_a2 * bzf) + tg * (acc)MIN_B) / modb;
 }
 
-__device__ __forceinline__ double flux_y_face(const Geo &g, const Tn &T, int ly, int ix, int iy, int iz,
+__device__ __forceinline__ acc flux_y_face(const Geo &g, const Tn &T, int ly, int ix, int iy, int iz,
                                               const double *__restrict__ bx, const double *__restrict__ by,
-                                              const double *__restrict__ bz, double L, double kappa_0) {
+                                              const double *__restrict__ bz, acc L, acc kappa_0) {
   const int c = g.idx(ix, iy, iz);
-  const double bxf = 0.25 * (bx[c] + bx[c + g.sy] + bx[c - 1] + bx[c - 1 + g.sy]);
-  const double bzf = 0.25 * (bz[c] + bz[c + g.sy] + bz[c - g.sz] + bz[c + g.sy - g.sz]);
-  const double byc = by[c];
-  const double modb = bxf * bxf + byc * byc + bzf * bzf + MIN_B;
-  double tb = (T.t[1][ly + 1][1] + T.t[1][ly][1]) / 2.0;
-  double tb_p = (T.t[2][ly][1] + T.t[2][ly + 1][1]) / 2.0;
-  double tb_m = (T.t[0][ly + 1][1] + T.t[0][ly][1]) / 2.0;
-  const double tg_a1 = (tb_p - tb_m) / (g.dxc[ix] + g.dxc[ix - 1]);
-  tb_p = (T.t[1][ly + 1][2] + T.t[1][ly][2]) / 2.0;
-  tb_m = (T.t[1][ly + 1][0] + T.t[1][ly][0]) / 2.0;
-  const double tg_a2 = (tb_p - tb_m) / (g.dzc[iz] + g.dzc[iz - 1]);
-  const double tg = (T.t[1][ly + 1][1] - T.t[1][ly][1]) / g.dyc[iy];
-  tb = dmax(tb, 0.0);
-  return -L * kappa_0 * pow25(tb) * (byc * (tg * byc + tg_a1 * bxf + tg_a2 * bzf) + tg * MIN_B) / modb;
+  const acc bxf = (acc)(0.25 * (bx[c] + bx[c + g.sy] + bx[c - 1] + bx[c - 1 + g.sy]));
+  const acc bzf = (acc)(0.25 * (bz[c] + bz[c + g.sy] + bz[c - g.sz] + bz[c + g.sy - g.sz]));
+  const acc byc = (acc)by[c];
+  const acc modb = bxf * bxf + byc * byc + bzf * bzf + (acc)MIN_B;
+  acc tb = (T.t[1][ly + 1][1] + T.t[1][ly][1]) / (acc)2.0;
+  acc tb_p = (T.t[2][ly][1] + T.t[2][ly + 1][1]) / (acc)2.0;
+  acc tb_m = (T.t[0][ly + 1][1] + T.t[0][ly][1]) / (acc)2.0;
+  const acc tg_a1 = (tb_p - tb_m) / (acc)(g.dxc[ix] + g.dxc[ix - 1]);
+  tb_p = (T.t[1][ly + 1][2] + T.t[1][ly][2]) / (acc)2.0;
+  tb_m = (T.t[1][ly + 1][0] + T.t[1][ly][0]) / (acc)2.0;
+  const acc tg_a2 = (tb_p - tb_m) / (acc)(g.dzc[iz] + g.dzc[iz - 1]);
+  const acc tg = (T.t[1][ly + 1][1] - T.t[1][ly][1]) / (acc)g.dyc[iy];
+  tb = dmax(tb, (acc)0.0);
+  return -L * kappa_0 * pow25(tb) * (byc * (tg * byc + tg_a1 * bxf + tg_a2 * bzf) + tg * (acc)MIN_B) / modb;
 }
 
-__device__ __forceinline__ double flux_z_face(const Geo &g, const Tn &T, int lz, int ix, int iy, int iz,
+__device__ __forceinline__ acc flux_z_face(const Geo &g, const Tn &T, int lz, int ix, int iy, int iz,
                                               const double *__restrict__ bx, const double *__restrict__ by,
-                                              const double *__restrict__ bz, double L, double kappa_0) {
+                                              const double *__restrict__ bz, acc L, acc kappa_0) {
   const int c = g.idx(ix, iy, iz);
-  const double bxf = 0.25 * (bx[c] + bx[c + g.sz] + bx[c - 1] + bx[c - 1 + g.sz]);
-  const double byf = 0.25 * (by[c] + by[c + g.sz] + by[c - g.sy] + by[c - g.sy + g.sz]);
-  const double bzc = bz[c];
-  const double modb = bxf * bxf + byf * byf + bzc * bzc + MIN_B;
-  double tb = (T.t[1][1][lz + 1] + T.t[1][1][lz]) / 2.0;
-  double tb_p = (T.t[2][1][lz + 1] + T.t[2][1][lz]) / 2.0;
-  double tb_m = (T.t[0][1][lz + 1] + T.t[0][1][lz]) / 2.0;
-  const double tg_a1 = (tb_p - tb_m) / (g.dxc[ix] + g.dxc[ix - 1]);
-  tb_p = (T.t[1][2][lz + 1] + T.t[1][2][lz]) / 2.0;
-  tb_m = (T.t[1][0][lz + 1] + T.t[1][0][lz]) / 2.0;
-  const double tg_a2 = (tb_p - tb_m) / (g.dyc[iy] + g.dyc[iy - 1]);
-  const double tg = (T.t[1][1][lz + 1] - T.t[1][1][lz]) / g.dzc[iz];
-  tb = dmax(tb, 0.0);
-  return -L * kappa_0 * pow25(tb) * (bzc * (tg * bzc + tg_a1 * bxf + tg_a2 * byf) + tg * MIN_B) / modb;
+  const acc bxf = (acc)(0.25 * (bx[c] + bx[c + g.sz] + bx[c - 1] + bx[c - 1 + g.sz]));
+  const acc byf = (acc)(0.25 * (by[c] + by[c + g.sz] + by[c - g.sy] + by[c - g.sy + g.sz]));
+  const acc bzc = (acc)bz[c];
+  const acc modb = bxf * bxf + byf * byf + bzc * bzc + (acc)MIN_B;
+  acc tb = (T.t[1][1][lz + 1] + T.t[1][1][lz]) / (acc)2.0;
+  acc tb_p = (T.t[2][1][lz + 1] + T.t[2][1][lz]) / (acc)2.0;
+  acc tb_m = (T.t[0][1][lz + 1] + T.t[0][1][lz]) / (acc)2.0;
+  const acc tg_a1 = (tb_p - tb_m) / (acc)(g.dxc[ix] + g.dxc[ix - 1]);
+  tb_p = (T.t[1][2][lz + 1] + T.t[1][2][lz]) / (acc)2.0;
+  tb_m = (T.t[1][0][lz + 1] + T.t[1][0][lz]) / (acc)2.0;
+  const acc tg_a2 = (tb_p - tb_m) / (acc)(g.dyc[iy] + g.dyc[iy - 1]);
+  const acc tg = (T.t[1][1][lz + 1] - T.t[1][1][lz]) / (acc)g.dzc[iz];
+  tb = dmax(tb, (acc)0.0);
+  return -L * kappa_0 * pow25(tb) * (bzc * (tg * bzc + tg_a1 * bxf + tg_a2 * byf) + tg * (acc)MIN_B) / modb;
 }
 
 // Ghost mapping of one coordinate under a read mode (the per-axis part of
@@ -239,7 +243,7 @@ __global__ __launch_bounds__(BX * BY) void k_rkl2_z(Geo g, Box b, const real *__
     map_coord<MODE>(i + q - 1, g.nx, g.user[0], xm[q], xs[q]);
     map_coord<MODE>(j + q - 1, g.ny, g.user[1], ym[q], ys[q]);
   }
-  const double tfac = a.tfac;
+  const acc tfac = (acc)a.tfac, kap = (acc)a.kappa_0, tdt = (acc)a.dt;
   // T.t[p][q][r]: x offset p-1, y offset q-1, plane k-1+r
   Tn T;
 #define RKL_LOAD_PLANE(kk, r)                                                    \
@@ -250,40 +254,52 @@ __global__ __launch_bounds__(BX * BY) void k_rkl2_z(Geo g, Box b, const real *__
     _Pragma("unroll") for (int p = 0; p < 3; ++p)                                \
       _Pragma("unroll") for (int q = 0; q < 3; ++q) {                            \
         const int cq = g.idx(xm[p], ym[q], zm);                                  \
-        T.t[p][q][r] = tfac * (double)(((xs[p] || ys[q] || zs) ? E0 : Y2)[cq]);  \
+        T.t[p][q][r] = tfac * (acc)(((xs[p] || ys[q] || zs) ? E0 : Y2)[cq]);     \
       }                                                                          \
   }
   RKL_LOAD_PLANE(k0 - 1, 0)
   RKL_LOAD_PLANE(k0, 1)
-  double fz_in = 0.0;
+  acc fz_in = 0.0;
   for (int k = k0; k <= k1; ++k) {
     RKL_LOAD_PLANE(k + 1, 2)
     const int c0 = g.idx(i, j, k);
-    const double L0 = larsen[c0];
-    const double fx_out = flux_x_face(g, T, 1, i, j, k, bx, by, bz, L0, a.kappa_0);
-    const double fx_in = __shfl_up(fx_out, 1, 64);
-    if (k == k0) fz_in = flux_z_face(g, T, 0, i, j, k - 1, bx, by, bz, larsen[c0 - g.sz], a.kappa_0);
-    const double fy_in = flux_y_face(g, T, 0, i, j - 1, k, bx, by, bz, larsen[c0 - g.sy], a.kappa_0);
-    const double fy_out = flux_y_face(g, T, 1, i, j, k, bx, by, bz, L0, a.kappa_0);
-    const double fz_out = flux_z_face(g, T, 1, i, j, k, bx, by, bz, L0, a.kappa_0);
+    const acc L0 = larsen[c0];
+    const acc fx_out = flux_x_face(g, T, 1, i, j, k, bx, by, bz, L0, kap);
+    const acc fx_in = __shfl_up(fx_out, 1, 64);
+    if (k == k0) fz_in = flux_z_face(g, T, 0, i, j, k - 1, bx, by, bz, (acc)larsen[c0 - g.sz], kap);
+    const acc fy_in = flux_y_face(g, T, 0, i, j - 1, k, bx, by, bz, (acc)larsen[c0 - g.sy], kap);
+    const acc fy_out = flux_y_face(g, T, 1, i, j, k, bx, by, bz, L0, kap);
+    const acc fz_out = flux_z_face(g, T, 1, i, j, k, bx, by, bz, L0, kap);
     if (out) {
-      double fl = 0.0 + fz_in / g.dzb[k - 1];
-      fl = fl + fy_in / g.dyb[j - 1];
-      fl = fl + fx_in / g.dxb[i - 1];
-      fl = fl - fx_out / g.dxb[i];
-      fl = fl - fy_out / g.dyb[j];
-      fl = fl - fz_out / g.dzb[k];
+      acc fl = (acc)0.0 + fz_in / (acc)g.dzb[k - 1];
+      fl = fl + fy_in / (acc)g.dyb[j - 1];
+      fl = fl + fx_in / (acc)g.dxb[i - 1];
+      fl = fl - fx_out / (acc)g.dxb[i];
+      fl = fl - fy_out / (acc)g.dyb[j];
+      fl = fl - fz_out / (acc)g.dzb[k];
       if (FIRST) {
-        const double lc = (1.0 / rho[c0]) * fl;
+        const acc lc = ((acc)1.0 / rho[c0]) * fl;
         Lc[c0] = lc;
-        const double cc = mu_t * a.dt * lc;
+        const acc cc = (acc)mu_t * tdt * lc;
         Yout[c0] = E0[c0] + cc;
       } else {
-        const double cc0 = gam_t * a.dt * Lc[c0];
-        const double lj = (1.0 / rho[c0]) * fl;
-        const double cc1 = mu_t * a.dt * lj;
-        double y3 = mu * Y2[c0] + nu * Y1[c0] + one_m_mu_nu * E0[c0];
-        y3 = y3 + cc1 + cc0;
+        const acc cc0 = (acc)gam_t * tdt * Lc[c0];
+        const acc lj = ((acc)1.0 / rho[c0]) * fl;
+        const acc cc1 = (acc)mu_t * tdt * lj;
+        acc y3;
+        if constexpr (sizeof(acc) == sizeof(double)) {
+          y3 = (acc)mu * Y2[c0] + (acc)nu * Y1[c0] + (acc)one_m_mu_nu * E0[c0];
+          y3 = y3 + cc1 + cc0;
+        } else {
+          // FP32 arithmetic: the same combination in increment form,
+          // w E0 + mu (Y2 - E0) + nu (Y1 - E0) with w = mu + nu + (1 - mu -
+          // nu), so that rounding scales with the stage increments, not
+          // with E0.  w rounds to exactly 1 for an RKL2 stage; the
+          // trapezoid evaluation (all three weights zero) has w = 0.
+          const acc e0 = E0[c0];
+          const acc w = (acc)(mu + nu + one_m_mu_nu);
+          y3 = w * e0 + (((acc)mu * (Y2[c0] - e0) + (acc)nu * (Y1[c0] - e0)) + (cc1 + cc0));
+        }
         Yout[c0] = y3;
       }
     }

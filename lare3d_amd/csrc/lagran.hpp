@@ -49,8 +49,9 @@ __device__ __forceinline__ double cv_sum8(const Geo &g, int i, int j, int k) {
          g.cv(i + 1, j, k + 1) + g.cv(i, j + 1, k + 1) + g.cv(i + 1, j + 1, k + 1);
 }
 
-// fast magnetosonic speed of shock_viscosity (:554-556) in cell c; gb is
-// gamma_boris of the previous set_dt, or NULL for 1 (boris off)
+// fast magnetosonic speed of shock_viscosity (:554-556) in cell c; with GB,
+// gb is gamma_boris of the previous set_dt, else 1 (boris off) and gb unused
+template <bool GB>
 __device__ __forceinline__ acc cell_cs(const Geo &g, const real *__restrict__ rho, const real *__restrict__ energy,
                                        const double *__restrict__ bx, const double *__restrict__ by,
                                        const double *__restrict__ bz, const real *__restrict__ gb, int c,
@@ -63,7 +64,7 @@ __device__ __forceinline__ acc cell_cs(const Geo &g, const real *__restrict__ rh
   const acc b2 = b1x * b1x + b1y * b1y + b1z * b1z;
   const acc ga = (acc)gamma;
   const acc p = (ga - (acc)1.0) * r * (energy[c] - (acc)0.0);
-  const acc g_b = gb ? (acc)gb[c] : (acc)1.0;
+  const acc g_b = GB ? (acc)gb[c] : (acc)1.0;
   return sqrt((ga * p + g_b * b2) / rmin);
 }
 
@@ -72,6 +73,7 @@ __device__ __forceinline__ acc cell_cs(const Geo &g, const real *__restrict__ rh
 struct CellQuad {
   acc r[4], cs[4];
 };
+template <bool GB>
 __device__ __forceinline__ CellQuad cell_quad(const Geo &g, const real *__restrict__ rho,
                                               const real *__restrict__ energy, const double *__restrict__ bx,
                                               const double *__restrict__ by, const double *__restrict__ bz,
@@ -81,7 +83,7 @@ __device__ __forceinline__ CellQuad cell_quad(const Geo &g, const real *__restri
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
     q.r[p] = rho[c + off[p]];
-    q.cs[p] = cell_cs(g, rho, energy, bx, by, bz, gb, c + off[p], gamma);
+    q.cs[p] = cell_cs<GB>(g, rho, energy, bx, by, bz, gb, c + off[p], gamma);
   }
   return q;
 }
@@ -90,17 +92,18 @@ __device__ __forceinline__ CellQuad cell_quad(const Geo &g, const real *__restri
 // over nodes -1..n+1; marches in z, keeping the cell values of plane k+1 for
 // the next node plane, so each cell's fast speed is computed once per
 // column chunk.  Sums run over the 8 cells in the order (i,j,k), (i+1,j,k),
-// (i,j+1,k), (i+1,j+1,k), then the same four at k+1.
+// (i,j+1,k), (i+1,j+1,k), then the same four at k+1.  GB: boris on.
+template <bool GB>
 __global__ __launch_bounds__(BX * BY) void k_vertex_prep(Geo g, Box b, const real *__restrict__ rho, const real *__restrict__ energy,
                               const double *__restrict__ bx, const double *__restrict__ by,
                               const double *__restrict__ bz, const real *__restrict__ gb, real *__restrict__ rho_v,
                               real *__restrict__ cs_v, double gamma) {
   ZCOLUMN(b, i, j, k0, k1);
   const int Z = g.sz;
-  CellQuad lo = cell_quad(g, rho, energy, bx, by, bz, gb, g.idx(i, j, k0), gamma);
+  CellQuad lo = cell_quad<GB>(g, rho, energy, bx, by, bz, gb, g.idx(i, j, k0), gamma);
   for (int k = k0; k <= k1; ++k) {
     const int c = g.idx(i, j, k);
-    const CellQuad hi = cell_quad(g, rho, energy, bx, by, bz, gb, c + Z, gamma);
+    const CellQuad hi = cell_quad<GB>(g, rho, energy, bx, by, bz, gb, c + Z, gamma);
     acc cvs[8];
     const int di[8] = {0, 1, 0, 1, 0, 1, 0, 1}, dj[8] = {0, 0, 1, 1, 0, 0, 1, 1}, dk[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 #pragma unroll
@@ -770,7 +773,8 @@ __global__ __launch_bounds__(BX * BY) void k_force(Geo g, Box b, const real *__r
 }
 
 // shock_heating (lagran.F90:861-937) + corrector (:415-462) over cells
-// 1..n; total_visc_heating partial sums.
+// 1..n; total_visc_heating partial sums.  COOL: cooling_term on.
+template <bool COOL>
 __global__ __launch_bounds__(BX * BY) void k_shock_corr(Geo g, Box b, const real *__restrict__ vx, const real *__restrict__ vy,
                              const real *__restrict__ vz, const real *__restrict__ vx1,
                              const real *__restrict__ vy1, const real *__restrict__ vz1,
@@ -779,8 +783,8 @@ __global__ __launch_bounds__(BX * BY) void k_shock_corr(Geo g, Box b, const real
                              real *__restrict__ energy, real *__restrict__ rho, real *__restrict__ visc_dep,
                              real *__restrict__ vxo, real *__restrict__ vyo, real *__restrict__ vzo,
                              real *__restrict__ cool_v, double alpha_av, double *__restrict__ partial, double dt) {
-  // cool_v: cool_term_v of the cooling_term (lagran.F90:464-476), NULL when
-  // it is off
+  // cool_v: cool_term_v of the cooling_term (lagran.F90:464-476), unused
+  // without COOL
   // cells 1..n, marching in z: the node values (v, v1) and al1/al2 of plane
   // k are reused as plane k-1 / k values of the next cell.  The new
   // velocity v = 2 v1 - v of node (i, j, k), the cell's upper corner, goes
@@ -826,7 +830,7 @@ __global__ __launch_bounds__(BX * BY) void k_shock_corr(Geo g, Box b, const real
       visc_dep[c] = visc_dep[c] + dta * vh;
       const real rn = r / ((acc)1.0 + dv);
       rho[c] = rn;
-      if (cool_v) {
+      if (COOL) {
         // a running average of the viscous heating is removed again, and
         // energy = MAX(energy, 0)
         const acc aav = (acc)alpha_av;

@@ -92,6 +92,7 @@ int lare_dev_set_scalars(const lare_scalars *in) { return FWD(api_set_scalars(in
 int lare_dev_lagrangian_step(void) { return FWD(api_lagrangian_step()); }
 int lare_dev_eulerian_remap(int32_t step) { return FWD(api_eulerian_remap(step)); }
 int lare_dev_eta_calc(void) { return FWD(api_eta_calc()); }
+int lare_dev_energy_correction(void) { return FWD(api_energy_correction()); }
 int lare_dev_conduct_heat(void) { return FWD(api_conduct_heat()); }
 int lare_dev_boundary_conditions(void) { return FWD(api_boundary_conditions()); }
 int lare_dev_energy_account(double en[3], double *divb) { return FWD(api_energy_account(en, divb)); }

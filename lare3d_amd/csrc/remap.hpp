@@ -28,13 +28,16 @@ __device__ __forceinline__ int coord(int i, int j, int k) {
   return AX == 0 ? i : (AX == 1 ? j : k);
 }
 
-// Van Leer limited slope Di (xremap.f90:390-396).
-__device__ __forceinline__ double vl_di(double dfi, double dfu, double sign_v, double dci, double dcu, double dbu,
-                                        double phi) {
-  double Da = (2.0 - phi) * fabs(dfi) / dci + (1.0 + phi) * fabs(dfu) / dcu;
-  Da = Da * SIXTH;
+// Van Leer limited slope Di (xremap.f90:390-396) in the arithmetic type A.
+// The gradients dfi and dfu come in FP64, and the limiter's sign tests and
+// three-way minimum are made in FP64, so that near-zero gradients compare
+// exactly; the slope estimate Da (two divisions) is formed in A.
+template <class A>
+__device__ __forceinline__ A vl_di(double dfi, double dfu, double sign_v, A dci, A dcu, A dbu, A phi) {
+  A Da = ((A)2.0 - phi) * (A)fabs(dfi) / dci + ((A)1.0 + phi) * (A)fabs(dfu) / dcu;
+  Da = Da * (A)SIXTH;
   const double ss = 0.5 * (copysign(1.0, dfi) + copysign(1.0, dfu));
-  return sign_v * ss * dmin(dmin(fabs(Da) * dbu, fabs(dfi)), fabs(dfu));
+  return (A)(sign_v * ss * dmin(dmin((double)fabs(Da) * (double)dbu, fabs(dfi)), fabs(dfu)));
 }
 
 // Inputs of the predictor energy and pressure (lagran.F90:234-249), which
@@ -71,45 +74,47 @@ __global__ __launch_bounds__(BX * BY) void k_geom(Geo g, Box b, const real *__re
     if (NY) by_ = quad_back(uy, c, Y);
     if (NZ) bz_ = quad_back(uz, c, Y);
     // sums in the order of u(c) + u(c-Y) + u(c-Z) + u(c-Y-Z) etc.
-    const double vxb = (bx_.p0 + bx_.py + ax.p0 + ax.py) * 0.25;
-    const double vxbm = (bx_.px + bx_.pxy + ax.px + ax.pxy) * 0.25;
-    const double vyb = (by_.p0 + by_.px + ay.p0 + ay.px) * 0.25;
-    const double vybm = (by_.py + by_.pxy + ay.py + ay.pxy) * 0.25;
-    const double vzb = (bz_.p0 + bz_.px + bz_.py + bz_.pxy) * 0.25;
-    const double vzbm = (az.p0 + az.px + az.py + az.pxy) * 0.25;
-    double d1;
-    if (AX == 0) d1 = g.dxb[i] + (vxb - vxbm) * dt;
-    else if (AX == 1) d1 = g.dyb[j] + (vyb - vybm) * dt;
-    else d1 = g.dzb[k] + (vzb - vzbm) * dt;
+    const acc q4 = 0.25, tdt = (acc)dt;
+    const acc vxb = (bx_.p0 + bx_.py + ax.p0 + ax.py) * q4;
+    const acc vxbm = (bx_.px + bx_.pxy + ax.px + ax.pxy) * q4;
+    const acc vyb = (by_.p0 + by_.px + ay.p0 + ay.px) * q4;
+    const acc vybm = (by_.py + by_.pxy + ay.py + ay.pxy) * q4;
+    const acc vzb = (bz_.p0 + bz_.px + bz_.py + bz_.pxy) * q4;
+    const acc vzbm = (az.p0 + az.px + az.py + az.pxy) * q4;
+    const acc dxb = (acc)g.dxb[i], dyb = (acc)g.dyb[j], dzb = (acc)g.dzb[k];
+    acc d1;
+    if (AX == 0) d1 = dxb + (vxb - vxbm) * tdt;
+    else if (AX == 1) d1 = dyb + (vyb - vybm) * tdt;
+    else d1 = dzb + (vzb - vzbm) * tdt;
     if (CV > 0) {
-      const double cv = g.cv(i, j, k);
-      double dv1, dv2;
+      const acc cv = (acc)g.cv(i, j, k);
+      acc dv1, dv2;
       if (AX == 0) {
-        const double tA = ypass * (vyb - vybm) / g.dyb[j];
-        const double tB = zpass * (vzb - vzbm) / g.dzb[k];
-        dv1 = (tA + tB + (vxb - vxbm) / g.dxb[i]) * dt;
-        dv2 = (tA + tB) * dt;
+        const acc tA = (acc)ypass * (vyb - vybm) / dyb;
+        const acc tB = (acc)zpass * (vzb - vzbm) / dzb;
+        dv1 = (tA + tB + (vxb - vxbm) / dxb) * tdt;
+        dv2 = (tA + tB) * tdt;
       } else if (AX == 1) {
-        const double tA = xpass * (vxb - vxbm) / g.dxb[i];
-        const double tB = zpass * (vzb - vzbm) / g.dzb[k];
-        dv1 = (tA + tB + (vyb - vybm) / g.dyb[j]) * dt;
-        dv2 = (tA + tB) * dt;
+        const acc tA = (acc)xpass * (vxb - vxbm) / dxb;
+        const acc tB = (acc)zpass * (vzb - vzbm) / dzb;
+        dv1 = (tA + tB + (vyb - vybm) / dyb) * tdt;
+        dv2 = (tA + tB) * tdt;
       } else {
-        const double tA = xpass * (vxb - vxbm) / g.dxb[i];
-        const double tB = ypass * (vyb - vybm) / g.dyb[j];
-        dv1 = (tA + tB + (vzb - vzbm) / g.dzb[k]) * dt;
-        dv2 = (tA + tB) * dt;
+        const acc tA = (acc)xpass * (vxb - vxbm) / dxb;
+        const acc tB = (acc)ypass * (vyb - vybm) / dyb;
+        dv1 = (tA + tB + (vzb - vzbm) / dzb) * tdt;
+        dv2 = (tA + tB) * tdt;
       }
-      const real c1 = cv * (1.0 + dv1);
+      const real c1 = cv * ((acc)1.0 + dv1);
       cv1[c] = c1;
-      if (CV > 1) cv2[c] = cv * (1.0 + dv2);
+      if (CV > 1) cv2[c] = cv * ((acc)1.0 + dv2);
       if (CV == 1 && i >= 0 && i <= g.nx + 1 && j >= 0 && j <= g.ny + 1 && k >= 0 && k <= g.nz + 1) {
         // predictor energy and pressure (lagran.F90:234-249)
-        const double r = pr.rho[c];
-        const double dv = c1 / cv - 1.0;
-        double e1 = pr.energy[c] - pr.p0[c] * dv / r;
-        e1 = e1 + pr.visc_heat[c] * pr.dt2 / r;
-        pr.pp[c] = (e1 - 0.0) * (pr.gamma - 1.0) * r * cv / c1;
+        const acc r = pr.rho[c];
+        const acc dv = c1 / cv - (acc)1.0;
+        acc e1 = pr.energy[c] - pr.p0[c] * dv / r;
+        e1 = e1 + pr.visc_heat[c] * (acc)pr.dt2 / r;
+        pr.pp[c] = (e1 - (acc)0.0) * ((acc)pr.gamma - (acc)1.0) * r * cv / c1;
       }
     }
     db1[c] = d1;
@@ -121,31 +126,33 @@ __global__ __launch_bounds__(BX * BY) void k_geom(Geo g, Box b, const real *__re
 
 // dxc1 at sweep coordinate s (index c), clamped as xremap.f90:110
 template <int AX>
-__device__ __forceinline__ double dc1(const Geo &g, const real *__restrict__ db1, int c, int s) {
+__device__ __forceinline__ acc dc1(const Geo &g, const real *__restrict__ db1, int c, int s) {
   const int S = sstride<AX>(g);
   const int n = g.n(AX);
-  return 0.5 * (db1[c] + db1[s + 1 <= n + 2 ? c + S : c]);
+  return (acc)0.5 * (db1[c] + db1[s + 1 <= n + 2 ? c + S : c]);
 }
 
 // Van Leer CT flux (xremap.f90:340-470) from the quotients f* = field /
 // width at s-1 (clamped), s, s+1, s+2 (clamped), the widths db, dbp at s and
 // s+1 (the field's widths averaged along its geometry axis), db1 at s and
-// s+1, and dxc1 at s-1, s, s+1.
-__device__ __forceinline__ double ct_flux_q(double v_advect, double fm, double fi, double fp, double fp2, double db,
-                                            double dbp, double db1c, double db1p, double dcm, double dci, double dcp,
-                                            double dt) {
+// s+1, and dxc1 at s-1, s, s+1.  The quotients, the upwind value and the
+// flux itself are FP64; the Courant number phi and the limited slope are
+// formed in the arithmetic type.
+__device__ __forceinline__ double ct_flux_q(double v_advect, double fm, double fi, double fp, double fp2, acc db,
+                                            acc dbp, acc db1c, acc db1p, acc dcm, acc dci, acc dcp, double dt) {
   const double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
   const double sign_v = copysign(1.0, v_advect);
   const double vad_p = (sign_v + 1.0) * 0.5;
   const double vad_m = 1.0 - vad_p;
   const double fu = fi * vad_p + fp * vad_m;
   const double dfu = dfm * vad_p + dfp * vad_m;
-  const double dcu = dcm * vad_p + dcp * vad_m;
-  const double dbu = db1c * vad_p + db1p * vad_m;
-  const double du = db * vad_p + dbp * vad_m;
-  const double phi = fabs(v_advect) * dt / du;
-  const double Di = vl_di(dfi, dfu, sign_v, dci, dcu, dbu, phi);
-  return (fu + Di * (1.0 - phi)) * v_advect * dt;
+  const acc ap = (acc)vad_p, am = (acc)vad_m;
+  const acc dcu = dcm * ap + dcp * am;
+  const acc dbu = db1c * ap + db1p * am;
+  const acc du = db * ap + dbp * am;
+  const acc phi = (acc)fabs(v_advect) * (acc)dt / du;
+  const acc Di = vl_di<acc>(dfi, dfu, sign_v, dci, dcu, dbu, phi);
+  return (fu + (double)(Di * ((acc)1.0 - phi))) * v_advect * dt;
 }
 
 // Face B component Q at (i, j, k) as the face flux that the remap moves,
@@ -201,7 +208,7 @@ template <int SC>
 __global__ __launch_bounds__(BX * BY) void k_ct_flux_x(Geo g, Box b, const double *__restrict__ fld1,
                                                        const double *__restrict__ fld2,
                                                        const real *__restrict__ vel, const real *__restrict__ db1,
-                                                       double *__restrict__ F1, double *__restrict__ F2, int GS1,
+                                                       real *__restrict__ F1, real *__restrict__ F2, int GS1,
                                                        int VS2, double dt) {
   constexpr int T1 = CtComp<0>::T1, T2 = CtComp<0>::T2;
   const int lane = (int)threadIdx.x;  // BX = 64: threadIdx.y is the wave
@@ -222,16 +229,19 @@ __global__ __launch_bounds__(BX * BY) void k_ct_flux_x(Geo g, Box b, const doubl
   real d = db1[c];
   for (int k = k0; k <= k1; ++k, c += Z) {
     const real dzo = db1[c + Z];
-    const double w1 = (d + db1[c + GS1]) * 0.5;
-    const double w2 = (d + dzo) * 0.5;
-    const double q1 = ct_load<0, T1, SC>(g, fld1, c, i, i, j, k) / w1;
-    const double q2 = ct_load<0, T2, SC>(g, fld2, c, i, i, j, k) / w2;
-    const double dd = d;
+    const acc w1 = (d + db1[c + GS1]) * (acc)0.5;
+    const acc w2 = (d + dzo) * (acc)0.5;
+    const double q1 = ct_load<0, T1, SC>(g, fld1, c, i, i, j, k) / (double)w1;
+    const double q2 = ct_load<0, T2, SC>(g, fld2, c, i, i, j, k) / (double)w2;
+    const acc dd = d;
     // neighbours along x; s-1 is clamped to -1 and s+2 to n+2
-    double q1m = __shfl_up(q1, 1, 64), q2m = __shfl_up(q2, 1, 64), dm = __shfl_up(dd, 1, 64);
-    const double q1p = __shfl_down(q1, 1, 64), q2p = __shfl_down(q2, 1, 64), dp = __shfl_down(dd, 1, 64);
-    double q1q = __shfl_down(q1, 2, 64), q2q = __shfl_down(q2, 2, 64), dq = __shfl_down(dd, 2, 64);
-    const double w1p = __shfl_down(w1, 1, 64), w2p = __shfl_down(w2, 1, 64);
+    double q1m = __shfl_up(q1, 1, 64), q2m = __shfl_up(q2, 1, 64);
+    acc dm = __shfl_up(dd, 1, 64);
+    const double q1p = __shfl_down(q1, 1, 64), q2p = __shfl_down(q2, 1, 64);
+    const acc dp = __shfl_down(dd, 1, 64);
+    double q1q = __shfl_down(q1, 2, 64), q2q = __shfl_down(q2, 2, 64);
+    acc dq = __shfl_down(dd, 2, 64);
+    const acc w1p = __shfl_down(w1, 1, 64), w2p = __shfl_down(w2, 1, 64);
     if (i - 1 < -1) {
       q1m = q1;
       q2m = q2;
@@ -245,9 +255,9 @@ __global__ __launch_bounds__(BX * BY) void k_ct_flux_x(Geo g, Box b, const doubl
     const real vc = vel[c];
     if (out) {
       // dxc1 at s-1 (upper point s, or s+1 when s = -1), s and s+1
-      const double dcm = 0.5 * (dm + (i > -1 ? dd : dp));
-      const double dci = 0.5 * (dd + dp);
-      const double dcp = 0.5 * (dp + dq);
+      const acc dcm = (acc)0.5 * (dm + (i > -1 ? dd : dp));
+      const acc dci = (acc)0.5 * (dd + dp);
+      const acc dcp = (acc)0.5 * (dp + dq);
       F1[c] = ct_flux_q((vc + vm) * 0.5, q1m, q1, q1p, q1q, w1, w1p, dd, dp, dcm, dci, dcp, dt);
       F2[c] = ct_flux_q((vc + vel[c - VS2]) * 0.5, q2m, q2, q2p, q2q, w2, w2p, dd, dp, dcm, dci, dcp, dt);
     }
@@ -266,7 +276,7 @@ template <int M, int SC>
 __global__ __launch_bounds__(BX * BY) void k_ct_flux_m(Geo g, Box b, const double *__restrict__ fld1,
                                                        const double *__restrict__ fld2,
                                                        const real *__restrict__ vel, const real *__restrict__ db1,
-                                                       double *__restrict__ F1, double *__restrict__ F2, int VS1,
+                                                       real *__restrict__ F1, real *__restrict__ F2, int VS1,
                                                        int GS1, int VS2, int GS2, double dt) {
   constexpr int T1 = CtComp<M>::T1, T2 = CtComp<M>::T2;
   // the launch box with the march axis last: (x, P, M), P the other axis
@@ -277,16 +287,17 @@ __global__ __launch_bounds__(BX * BY) void k_ct_flux_m(Geo g, Box b, const doubl
   const int n = g.n(M);
   // index, and quotients of both fields, at march coordinate m
   auto at = [&](int m) { return M == 2 ? g.idx(i, p, m) : g.idx(i, m, p); };
-  double q1[4], q2[4], w1[4], w2[4];
+  double q1[4], q2[4];
+  acc w1[4], w2[4];
   real d[4];
   auto load = [&](int slot, int m) {
     const int c = at(m);
     const int jj = M == 1 ? m : p, kk = M == 2 ? m : p;
     d[slot] = db1[c];
-    w1[slot] = (d[slot] + db1[c + GS1]) * 0.5;
-    w2[slot] = (d[slot] + db1[c + GS2]) * 0.5;
-    q1[slot] = ct_load<M, T1, SC>(g, fld1, c, m, i, jj, kk) / w1[slot];
-    q2[slot] = ct_load<M, T2, SC>(g, fld2, c, m, i, jj, kk) / w2[slot];
+    w1[slot] = (d[slot] + db1[c + GS1]) * (acc)0.5;
+    w2[slot] = (d[slot] + db1[c + GS2]) * (acc)0.5;
+    q1[slot] = ct_load<M, T1, SC>(g, fld1, c, m, i, jj, kk) / (double)w1[slot];
+    q2[slot] = ct_load<M, T2, SC>(g, fld2, c, m, i, jj, kk) / (double)w2[slot];
   };
   // window over m-1, m, m+1, m+2, each clamped to [-1, n+2]
 #pragma unroll
@@ -299,9 +310,9 @@ __global__ __launch_bounds__(BX * BY) void k_ct_flux_m(Geo g, Box b, const doubl
     const int c = at(m);
     // dxc1 at the clamped s-1 (its upper point is s, or s+1 when s = -1),
     // at s, and at s+1 (upper point clamped to n+2, as d[3] already is)
-    const double dcm = 0.5 * (d[0] + (m > -1 ? d[1] : d[2]));
-    const double dci = 0.5 * (d[1] + d[2]);
-    const double dcp = 0.5 * (d[2] + d[3]);
+    const acc dcm = (acc)0.5 * (d[0] + (m > -1 ? d[1] : d[2]));
+    const acc dci = (acc)0.5 * (d[1] + d[2]);
+    const acc dcp = (acc)0.5 * (d[2] + d[3]);
     const real vc = vel[c];
     F1[c] = ct_flux_q((vc + vel[c - VS1]) * 0.5, q1[0], q1[1], q1[2], q1[3], w1[1], w1[2], d[1], d[2], dcm, dci, dcp,
                       dt);
@@ -329,9 +340,12 @@ __global__ __launch_bounds__(BX * BY) void k_ct_flux_m(Geo g, Box b, const doubl
 // updated where its box says so, and stored to the outputs.
 // SC = -1 (last sweep of the predictor remap): the updated face flux of
 // every such point is converted back to B (remap.f90:102-125).
+// The fluxes F1, F2 are stored in the storage type (FP32 in the FP32 mode);
+// each is added to one face and subtracted from its neighbour as the same
+// stored value, and B accumulates in FP64, so div B stays at FP64 round-off.
 template <int AX, int SC>
-__global__ __launch_bounds__(BX * BY) void k_ct_update(Geo g, Box b, const double *__restrict__ F1,
-                                                       const double *__restrict__ F2, const double *t1i,
+__global__ __launch_bounds__(BX * BY) void k_ct_update(Geo g, Box b, const real *__restrict__ F1,
+                                                       const real *__restrict__ F2, const double *t1i,
                                                        const double *t2i, const double *ai, double *t1, double *t2,
                                                        double *a, int TS, int AS1, int AS2, Box bt1, Box ba1,
                                                        Box bt2, Box ba2) {
@@ -379,35 +393,39 @@ __global__ __launch_bounds__(BX * BY) void k_ct_update(Geo g, Box b, const doubl
 
 // Face-averaged sweep velocity of the mass/energy fluxes (xremap.f90:492-493)
 template <int AX>
-__device__ __forceinline__ double face_velocity(const Geo &g, const real *__restrict__ vel, int c) {
+__device__ __forceinline__ acc face_velocity(const Geo &g, const real *__restrict__ vel, int c) {
   const int X = 1, Y = g.sy, Z = g.sz;
-  if (AX == 0) return (vel[c] + vel[c - Y] + vel[c - Z] + vel[c - Y - Z]) * 0.25;
-  if (AX == 1) return (vel[c] + vel[c - X] + vel[c - Z] + vel[c - X - Z]) * 0.25;
-  return (vel[c] + vel[c - X] + vel[c - Y] + vel[c - X - Y]) * 0.25;
+  if (AX == 0) return (vel[c] + vel[c - Y] + vel[c - Z] + vel[c - Y - Z]) * (acc)0.25;
+  if (AX == 1) return (vel[c] + vel[c - X] + vel[c - Z] + vel[c - X - Z]) * (acc)0.25;
+  return (vel[c] + vel[c - X] + vel[c - Y] + vel[c - X - Y]) * (acc)0.25;
 }
+// face area, formed in FP64 from the metrics and rounded once
 template <int AX>
-__device__ __forceinline__ double face_area(const Geo &g, int i, int j, int k) {
-  if (AX == 0) return g.dyb[j] * g.dzb[k];
-  if (AX == 1) return g.dxb[i] * g.dzb[k];
-  return g.dxb[i] * g.dyb[j];
+__device__ __forceinline__ acc face_area(const Geo &g, int i, int j, int k) {
+  if (AX == 0) return (acc)(g.dyb[j] * g.dzb[k]);
+  if (AX == 1) return (acc)(g.dxb[i] * g.dzb[k]);
+  return (acc)(g.dxb[i] * g.dyb[j]);
 }
 
 // Van Leer upwind value fu and limited slope Di of a cell-centred quantity
 // along the sweep (xremap.f90:474-533), from its values at s-1..s+2, dxc1 at
 // s-1, s, s+1 and db1 at s, s+1; also the upwind weights, dbu and phi.
-__device__ __forceinline__ void vl_cell(double v_advect, double fm, double fi, double fp, double fp2, double dcm,
-                                        double dci, double dcp, double db1c, double db1p, double dt, double &vad_p,
-                                        double &vad_m, double &dbu, double &fu, double &Di, double &phi) {
+// The field values come in FP64 (exact for FP32 storage), so the gradients
+// that the limiter compares are FP64; the rest is formed in the arithmetic
+// type.
+__device__ __forceinline__ void vl_cell(acc v_advect, double fm, double fi, double fp, double fp2, acc dcm, acc dci,
+                                        acc dcp, acc db1c, acc db1p, acc dt, acc &vad_p, acc &vad_m, acc &dbu,
+                                        acc &fu, acc &Di, acc &phi) {
   const double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
-  const double sign_v = copysign(1.0, v_advect);
-  vad_p = (sign_v + 1.0) * 0.5;
-  vad_m = 1.0 - vad_p;
-  fu = fi * vad_p + fp * vad_m;
-  const double dfu = dfm * vad_p + dfp * vad_m;
-  const double dcu = dcm * vad_p + dcp * vad_m;
+  const double sign_v = copysign(1.0, (double)v_advect);
+  vad_p = (acc)((sign_v + 1.0) * 0.5);
+  vad_m = (acc)1.0 - vad_p;
+  fu = (acc)fi * vad_p + (acc)fp * vad_m;
+  const double dfu = dfm * (double)vad_p + dfp * (double)vad_m;
+  const acc dcu = dcm * vad_p + dcp * vad_m;
   dbu = db1c * vad_p + db1p * vad_m;
   phi = fabs(v_advect) * dt / dbu;
-  Di = vl_di(dfi, dfu, sign_v, dci, dcu, dbu, phi);
+  Di = vl_di<acc>(dfi, dfu, sign_v, dci, dcu, dbu, phi);
 }
 
 // Mass flux (x_mass_flux, xremap.f90:474-533), energy flux (x_energy_flux,
@@ -440,6 +458,7 @@ __global__ __launch_bounds__(BX * BY) void k_mass_rho_energy_m(Geo g, Box b, con
   const int nP = g.n(M == 2 ? 1 : 2);
   const bool inner = i >= 1 && i <= g.nx && p >= 1 && p <= nP;
   const int ms = m0 > 0 ? m0 - 1 : m0;  // first point whose fluxes are formed
+  const acc tdt = (acc)dt;
   // dm(n) for the wrapped dm(0) of a periodic axis (first chunk only)
   real dm_wrap = 0;
   if (periodic && ms == 0 && inner) {
@@ -451,12 +470,12 @@ __global__ __launch_bounds__(BX * BY) void k_mass_rho_energy_m(Geo g, Box b, con
       rw[q] = rho1[cn + (q - 1) * MS];
       dw[q] = db1[cn + (q - 1) * MS];
     }
-    const double area = face_area<M>(g, i, MJ(M, nM, p), MK(M, nM, p));
-    const double v_advect = face_velocity<M>(g, vel, cn);
-    double vad_p, vad_m, dbu, fu, Di, phi;
-    vl_cell(v_advect, rw[0], rw[1], rw[2], rw[3], 0.5 * (dw[0] + dw[1]), 0.5 * (dw[1] + dw[2]),
-            0.5 * (dw[2] + dw[3]), dw[1], dw[2], dt, vad_p, vad_m, dbu, fu, Di, phi);
-    dm_wrap = (fu + Di * (1.0 - phi)) * v_advect * dt * area;
+    const acc area = face_area<M>(g, i, MJ(M, nM, p), MK(M, nM, p));
+    const acc v_advect = face_velocity<M>(g, vel, cn);
+    acc vad_p, vad_m, dbu, fu, Di, phi;
+    vl_cell(v_advect, rw[0], rw[1], rw[2], rw[3], (acc)0.5 * (dw[0] + dw[1]), (acc)0.5 * (dw[1] + dw[2]),
+            (acc)0.5 * (dw[2] + dw[3]), dw[1], dw[2], tdt, vad_p, vad_m, dbu, fu, Di, phi);
+    dm_wrap = (fu + Di * ((acc)1.0 - phi)) * v_advect * tdt * area;
   }
   // window at ms: rho1, db1 (and energy in inner columns) at ms-1..ms+2
   real r[4], d[4], e[4] = {0, 0, 0, 0};
@@ -482,23 +501,23 @@ __global__ __launch_bounds__(BX * BY) void k_mass_rho_energy_m(Geo g, Box b, con
       c += MS;
     }
     const int j = MJ(M, m, p), k = MK(M, m, p);
-    const double area = face_area<M>(g, i, j, k);
-    const double v_advect = face_velocity<M>(g, vel, c);
-    const double dcm = 0.5 * (d[0] + d[1]), dci = 0.5 * (d[1] + d[2]), dcp = 0.5 * (d[2] + d[3]);
-    double vad_p, vad_m, dbu, fu, Di, phi;
-    vl_cell(v_advect, r[0], r[1], r[2], r[3], dcm, dci, dcp, d[1], d[2], dt, vad_p, vad_m, dbu, fu, Di, phi);
-    const real dm_own = (fu + Di * (1.0 - phi)) * v_advect * dt * area;
+    const acc area = face_area<M>(g, i, j, k);
+    const acc v_advect = face_velocity<M>(g, vel, c);
+    const acc dcm = (acc)0.5 * (d[0] + d[1]), dci = (acc)0.5 * (d[1] + d[2]), dcp = (acc)0.5 * (d[2] + d[3]);
+    acc vad_p, vad_m, dbu, fu, Di, phi;
+    vl_cell(v_advect, r[0], r[1], r[2], r[3], dcm, dci, dcp, d[1], d[2], tdt, vad_p, vad_m, dbu, fu, Di, phi);
+    const real dm_own = (fu + Di * ((acc)1.0 - phi)) * v_advect * tdt * area;
     if (m >= m0) dm[c] = dm_own;
     if (!inner) continue;
     const real dmc = (periodic && m == 0) ? dm_wrap : dm_own;
-    vl_cell(v_advect, e[0], e[1], e[2], e[3], dcm, dci, dcp, d[1], d[2], dt, vad_p, vad_m, dbu, fu, Di, phi);
-    const double rhou = (double)r[1] * vad_p + (double)r[2] * vad_m;
-    const double dd = dmc;
-    const double dmu = fabs(dd) / area / dbu / rhou;
-    const real F = (fu + Di * (1.0 - dmu)) * dd;
+    vl_cell(v_advect, e[0], e[1], e[2], e[3], dcm, dci, dcp, d[1], d[2], tdt, vad_p, vad_m, dbu, fu, Di, phi);
+    const acc rhou = (acc)r[1] * vad_p + (acc)r[2] * vad_m;
+    const acc dd = dmc;
+    const acc dmu = fabs(dd) / area / dbu / rhou;
+    const real F = (fu + Di * ((acc)1.0 - dmu)) * dd;
     if (m >= m0 && m >= 1) {
-      const double rr = r[1];
-      const double rn = (rr * cv1[c] + dp - dmc) / cv2[c];
+      const acc rr = r[1];
+      const acc rn = (rr * cv1[c] + dp - dmc) / cv2[c];
       rho_new[c] = rn;
       e_new[c] = (e[1] * cv1[c] * rr + Fp - F) / (cv2[c] * rn);
     }
@@ -517,15 +536,14 @@ __global__ __launch_bounds__(BX * BY) void k_mass_rho_energy_m(Geo g, Box b, con
 // lanes at x = 0 form dm(n) as well and use it, as the dm halo fill would
 // have made them.
 #define REX_OUT 63
-__device__ __forceinline__ double mass_flux_x(const Geo &g, const real *__restrict__ rho,
-                                             const real *__restrict__ vel, const real *__restrict__ db1, int c,
-                                             int i, int j, int k, double dt) {
-  const double area = face_area<0>(g, i, j, k);
-  const double v_advect = face_velocity<0>(g, vel, c);
-  double vad_p, vad_m, dbu, fu, Di, phi;
+__device__ __forceinline__ acc mass_flux_x(const Geo &g, const real *__restrict__ rho, const real *__restrict__ vel,
+                                          const real *__restrict__ db1, int c, int i, int j, int k, acc dt) {
+  const acc area = face_area<0>(g, i, j, k);
+  const acc v_advect = face_velocity<0>(g, vel, c);
+  acc vad_p, vad_m, dbu, fu, Di, phi;
   vl_cell(v_advect, rho[c - 1], rho[c], rho[c + 1], rho[c + 2], dc1<0>(g, db1, c - 1, i - 1), dc1<0>(g, db1, c, i),
           dc1<0>(g, db1, c + 1, i + 1), db1[c], db1[c + 1], dt, vad_p, vad_m, dbu, fu, Di, phi);
-  return (fu + Di * (1.0 - phi)) * v_advect * dt * area;
+  return (fu + Di * ((acc)1.0 - phi)) * v_advect * dt * area;
 }
 __global__ __launch_bounds__(BX * BY) void k_mass_rho_energy_x(Geo g, Box b, const real *__restrict__ energy,
                                                                const real *__restrict__ rho1,
@@ -546,33 +564,34 @@ __global__ __launch_bounds__(BX * BY) void k_mass_rho_energy_x(Geo g, Box b, con
   const int j = b.y0 + jr;
   const int k = b.z0 + (int)blockIdx.z;
   const int c = g.idx(i, j, k);
-  const real dm_own = mass_flux_x(g, rho1, vel, db1, c, i, j, k, dt);
+  const acc tdt = (acc)dt;
+  const real dm_own = mass_flux_x(g, rho1, vel, db1, c, i, j, k, tdt);
   if (valid && lane >= 1) dm[c] = dm_own;
   const bool inner = j >= 1 && j <= g.ny && k >= 1 && k <= g.nz;
   real dmc = dm_own, F = 0;
   if (inner) {
     if (periodic && i == 0) {
       const int cn = g.idx(g.nx, j, k);
-      dmc = mass_flux_x(g, rho1, vel, db1, cn, g.nx, j, k, dt);
+      dmc = mass_flux_x(g, rho1, vel, db1, cn, g.nx, j, k, tdt);
     }
     // the energy flux at this point
-    const double area = face_area<0>(g, i, j, k);
-    const double v_advect = face_velocity<0>(g, vel, c);
-    double vad_p, vad_m, dbu, fu, Di, phi;
+    const acc area = face_area<0>(g, i, j, k);
+    const acc v_advect = face_velocity<0>(g, vel, c);
+    acc vad_p, vad_m, dbu, fu, Di, phi;
     vl_cell(v_advect, energy[c - 1], energy[c], energy[c + 1], energy[c + 2], dc1<0>(g, db1, c - 1, i - 1),
-            dc1<0>(g, db1, c, i), dc1<0>(g, db1, c + 1, i + 1), db1[c], db1[c + 1], dt, vad_p, vad_m, dbu, fu, Di,
+            dc1<0>(g, db1, c, i), dc1<0>(g, db1, c + 1, i + 1), db1[c], db1[c + 1], tdt, vad_p, vad_m, dbu, fu, Di,
             phi);
-    const double rhou = rho1[c] * vad_p + rho1[c + 1] * vad_m;
-    const double d = dmc;
-    const double dmu = fabs(d) / area / dbu / rhou;
-    F = (fu + Di * (1.0 - dmu)) * d;
+    const acc rhou = rho1[c] * vad_p + rho1[c + 1] * vad_m;
+    const acc d = dmc;
+    const acc dmu = fabs(d) / area / dbu / rhou;
+    F = (fu + Di * ((acc)1.0 - dmu)) * d;
   }
   // lane l-1 holds point i-1 of the same row whenever i > x0
   const real Fm = __shfl_up(F, 1, 64);
   const real dmm = __shfl_up(dmc, 1, 64);
   if (!valid || lane < 1 || !inner || i < 1) return;
-  const double r1 = rho1[c];
-  const double rn = (r1 * cv1[c] + dmm - dmc) / cv2[c];
+  const acc r1 = rho1[c];
+  const acc rn = (r1 * cv1[c] + dmm - dmc) / cv2[c];
   rho_new[c] = rn;
   e_new[c] = (energy[c] * cv1[c] * r1 + Fm - F) / (cv2[c] * rn);
 }
@@ -580,10 +599,10 @@ __global__ __launch_bounds__(BX * BY) void k_mass_rho_energy_x(Geo g, Box b, con
 
 
 template <int AX>
-__device__ __forceinline__ double mom_area(const Geo &g, int i, int j, int k) {
-  if (AX == 0) return g.dyc[j] * g.dzc[k];
-  if (AX == 1) return g.dxc[i] * g.dzc[k];
-  return g.dxc[i] * g.dyc[j];
+__device__ __forceinline__ acc mom_area(const Geo &g, int i, int j, int k) {
+  if (AX == 0) return (acc)(g.dyc[j] * g.dzc[k]);
+  if (AX == 1) return (acc)(g.dxc[i] * g.dzc[k]);
+  return (acc)(g.dxc[i] * g.dyc[j]);
 }
 
 // Vertex quantities of xremap.f90:201-280 at a node: rho_v, cv1_v, cv2_v
@@ -598,7 +617,7 @@ struct VertexV {
   real rv, c1, c2, dm;
 };
 struct VSum {
-  double s, s1, s2, sd;  // sum of rho1*cv1, of cv1, of cv2, of dm
+  acc s, s1, s2, sd;  // sum of rho1*cv1, of cv1, of cv2, of dm
 };
 // cells at c, c+1 (and c+D, c+1+D for N = 4): one x pair, or a quad
 template <int N>
@@ -624,13 +643,13 @@ __device__ __forceinline__ VCellN<N> vcells_at(const real *__restrict__ rho1, co
 template <int N>
 __device__ __forceinline__ VSum vsum_first(const VCellN<N> &v) {
   VSum p;
-  p.s = (double)v.r[0] * v.a[0];
+  p.s = (acc)v.r[0] * v.a[0];
   p.s1 = v.a[0];
   p.s2 = v.b[0];
   p.sd = v.d[0];
 #pragma unroll
   for (int t = 1; t < N; ++t) {
-    p.s = p.s + (double)v.r[t] * v.a[t];
+    p.s = p.s + (acc)v.r[t] * v.a[t];
     p.s1 = p.s1 + v.a[t];
     p.s2 = p.s2 + v.b[t];
     p.sd = p.sd + v.d[t];
@@ -642,7 +661,7 @@ template <int N>
 __device__ __forceinline__ VSum vsum_add(VSum p, const VCellN<N> &v) {
 #pragma unroll
   for (int t = 0; t < N; ++t) {
-    p.s = p.s + (double)v.r[t] * v.a[t];
+    p.s = p.s + (acc)v.r[t] * v.a[t];
     p.s1 = p.s1 + v.a[t];
     p.s2 = p.s2 + v.b[t];
     p.sd = p.sd + v.d[t];
@@ -652,9 +671,9 @@ __device__ __forceinline__ VSum vsum_add(VSum p, const VCellN<N> &v) {
 __device__ __forceinline__ VertexV vertex_of(const VSum &p) {
   VertexV v;
   v.rv = p.s / p.s1;
-  v.c1 = p.s1 * 0.125;
-  v.c2 = p.s2 * 0.125;
-  v.dm = p.sd * 0.125;
+  v.c1 = p.s1 * (acc)0.125;
+  v.c2 = p.s2 * (acc)0.125;
+  v.dm = p.sd * (acc)0.125;
   return v;
 }
 // What a node passes to the next one along M.  Along z (M = 2) the next
@@ -712,7 +731,31 @@ __device__ __forceinline__ VertexV vertex_march(const Geo &g, const real *__rest
 // rho_v of node i+1 from the lane to its right and the flux and dm_v of node
 // i-1 from the lane to its left; lanes 1..MOMX_OUT write.  Every point is
 // written to vxo..vzo: nodes 0..n updated, the rest copied.
+//
+// RKE: also the kinetic energy the sweep loses at each edge between nodes
+// s and s+1 (xremap.f90:664-695, s = 0..n-1, transverse 0..n), one array
+// per velocity component, indexed by the edge's lower node; rke_gather
+// adds them to delta_ke in the reference's order.  The edge term of node i
+// needs the flux velocity of node i+1, whose upwind density is node i+2's,
+// so with RKE the waves overlap by three and lanes 1..61 write.
+struct RkeOut {
+  real *dk[3];
+};
+// The edge term dk of xremap.f90:673-686 from the old velocities vs, vp of
+// the edge's nodes, the flux velocities (before the dm factor) at the edges
+// s-1, s, s+1, the vertex dm there, and the node masses ms, mp after the
+// sweep (rho_v1 cv2).
+__device__ __forceinline__ acc rke_edge(acc vs, acc vp, acc flm, acc fl, acc flp, acc dmm, acc dms, acc dmp, acc ms,
+                                        acc mp) {
+  const acc ai = (vs - flm) * dmm / ms - (vs - fl) * dms / ms;
+  const acc aip = (vp - fl) * dms / mp - (vp - flp) * dmp / mp;
+  acc dk = (vp - vs) * (fl - (acc)0.5 * (vp + vs)) - (acc)0.5 * ai * (vs - fl) + (acc)0.5 * aip * (vp - fl);
+  dk = dk * dms * (acc)0.5;
+  return dk;
+}
 #define MOMX_OUT 62
+#define MOMX_OUT_RKE 61
+template <bool RKE>
 __global__ __launch_bounds__(BX * BY) void k_mom_remap_x(Geo g, Box b, const real *__restrict__ vx,
                                                          const real *__restrict__ vy, const real *__restrict__ vz,
                                                          const real *__restrict__ vel1,
@@ -721,11 +764,13 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_x(Geo g, Box b, const rea
                                                          const real *__restrict__ cv1,
                                                          const real *__restrict__ cv2,
                                                          const real *__restrict__ dm, real *__restrict__ vxo,
-                                                         real *__restrict__ vyo, real *__restrict__ vzo, double dt) {
+                                                         real *__restrict__ vyo, real *__restrict__ vzo, double dt,
+                                                         RkeOut rk) {
   const int lane = (int)threadIdx.x;  // BX = 64: threadIdx.y is the wave
   const int wq = b.x1 - b.x0 + 1;
   const int npts = wq * (b.y1 - b.y0 + 1);
-  const int f = (int)(xcd_tile(blockIdx.x, gridDim.x) * BY + threadIdx.y) * MOMX_OUT + lane - 1;
+  constexpr int OUT = RKE ? MOMX_OUT_RKE : MOMX_OUT;
+  const int f = (int)(xcd_tile(blockIdx.x, gridDim.x) * BY + threadIdx.y) * OUT + lane - 1;
   const bool valid = f >= 0 && f < npts;
   const int ff = valid ? f : 0;  // lanes past the plane read point 0, unused
   const int jr = ff / wq;
@@ -735,7 +780,7 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_x(Geo g, Box b, const rea
   const real *V[3] = {vx, vy, vz};
   real *VO[3] = {vxo, vyo, vzo};
   const int n = g.nx;
-  const bool writer = valid && lane >= 1 && lane <= MOMX_OUT;
+  const bool writer = valid && lane >= 1 && lane <= OUT;
   // vertex box: x -1..n+1, transverse 0..n
   const bool vin = j >= 0 && j <= g.ny && i >= -1 && i <= n + 1;
   VMarch<2> w;
@@ -749,30 +794,32 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_x(Geo g, Box b, const rea
       have = true;
     }
     const real rvp = __shfl_down(vq.rv, 1, 64);  // rho_v of node i+1 (lanes 0..62)
-    // fluxes on x = -1..n (transverse 0..n)
-    real F[3] = {0, 0, 0};
+    // fluxes on x = -1..n (transverse 0..n); FL: the flux velocity before
+    // the dm factor (stored as the reference's flux array is)
+    real F[3] = {0, 0, 0}, FL[3] = {0, 0, 0};
     if (iny && i >= -1 && i <= n) {
-      const double area = mom_area<0>(g, i, j, k);
-      const double v_advect = (vel1[c] + vel1[c + 1]) * 0.5;
-      const double sign_v = copysign(1.0, v_advect);
-      const double vad_p = (sign_v + 1.0) * 0.5;
-      const double vad_m = 1.0 - vad_p;
-      const double dci = db1[c + 1];
-      const double dcu = db1[c] * vad_p + db1[c + 2] * vad_m;
-      const double dbu = dc1<0>(g, db1, c, i) * vad_p + dc1<0>(g, db1, c + 1, i + 1) * vad_m;
-      const double phi = fabs(v_advect) * dt / dbu;
-      const double rhou = (double)vq.rv * vad_p + (double)rvp * vad_m;
-      const double d = vq.dm;
-      const double dmu = fabs(d) / area / dbu / rhou;
+      const acc area = mom_area<0>(g, i, j, k);
+      const acc v_advect = (vel1[c] + vel1[c + 1]) * (acc)0.5;
+      const double sign_v = copysign(1.0, (double)v_advect);
+      const acc vad_p = (acc)((sign_v + 1.0) * 0.5);
+      const acc vad_m = (acc)1.0 - vad_p;
+      const acc dci = db1[c + 1];
+      const acc dcu = db1[c] * vad_p + db1[c + 2] * vad_m;
+      const acc dbu = dc1<0>(g, db1, c, i) * vad_p + dc1<0>(g, db1, c + 1, i + 1) * vad_m;
+      const acc phi = fabs(v_advect) * (acc)dt / dbu;
+      const acc rhou = (acc)vq.rv * vad_p + (acc)rvp * vad_m;
+      const acc d = vq.dm;
+      const acc dmu = fabs(d) / area / dbu / rhou;
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         const real *u = V[q];
         const double fm = u[c - 1], fi = u[c], fp = u[c + 1], fp2 = u[c + 2];
         const double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
-        const double fu = fi * vad_p + fp * vad_m;
-        const double dfu = dfm * vad_p + dfp * vad_m;
-        const double Di = vl_di(dfi, dfu, sign_v, dci, dcu, dbu, phi);
-        const double fl = fu + Di * (1.0 - dmu);
+        const acc fu = (acc)fi * vad_p + (acc)fp * vad_m;
+        const double dfu = dfm * (double)vad_p + dfp * (double)vad_m;
+        const acc Di = vl_di<acc>(dfi, dfu, sign_v, dci, dcu, dbu, phi);
+        const acc fl = fu + Di * ((acc)1.0 - dmu);
+        FL[q] = fl;
         F[q] = fl * d;
       }
     }
@@ -781,11 +828,31 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_x(Geo g, Box b, const rea
 #pragma unroll
     for (int q = 0; q < 3; ++q) Fm[q] = __shfl_up(F[q], 1, 64);
     const real dmm = __shfl_up(vq.dm, 1, 64);
+    // rho_v1 of node i (xremap.f90:282-290)
+    const bool upd = iny && i >= 0 && i <= n;
+    const acc rv = vq.rv, c1 = vq.c1, c2 = vq.c2;
+    acc rvn = 0;
+    if (upd) rvn = (rv * c1 + dmm - vq.dm) / c2;
+    if constexpr (RKE) {
+      // lane l+1 holds node i+1 of the same row for the writers
+      const acc mass = (acc)(real)rvn * c2;
+      const acc mp = __shfl_down(mass, 1, 64);
+      const real dmp = __shfl_down(vq.dm, 1, 64);
+      real FLm[3], FLp[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        FLm[q] = __shfl_up(FL[q], 1, 64);
+        FLp[q] = __shfl_down(FL[q], 1, 64);
+      }
+      if (writer && iny && i >= 0 && i < n) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          rk.dk[q][c] = rke_edge(V[q][c], V[q][c + 1], FLm[q], FL[q], FLp[q], dmm, vq.dm, dmp, mass, mp);
+      }
+    }
     if (!writer) continue;
-    if (iny && i >= 0 && i <= n) {
+    if (upd) {
       // the update (xremap.f90:282-329)
-      const double rv = vq.rv, c1 = vq.c1, c2 = vq.c2;
-      const double rvn = (rv * c1 + dmm - vq.dm) / c2;
 #pragma unroll
       for (int q = 0; q < 3; ++q) VO[q][c] = (rv * V[q][c] * c1 + Fm[q] - F[q]) / (c2 * rvn);
     } else {
@@ -805,7 +872,12 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_x(Geo g, Box b, const rea
 // stored flux is).  The new velocities go to vxo..vzo, because neighbouring
 // chunks still read the old ones; every point of the launch box (the whole
 // padded box) is written: nodes 0..n updated, the rest copied.
-template <int M>
+//
+// RKE: the edge terms of yremap.f90:665-696 / zremap.f90:664-695 (edges s =
+// 0..n-1 along the sweep, transverse 0..n), formed one point behind the
+// march: the term of the edge at m-1 needs the flux velocity at m, so a
+// chunk runs one point past its last node when that node's edge exists.
+template <int M, bool RKE>
 __global__ __launch_bounds__(BX * BY) void k_mom_remap_m(Geo g, Box b, const real *__restrict__ vx,
                                                          const real *__restrict__ vy, const real *__restrict__ vz,
                                                          const real *__restrict__ vel1,
@@ -814,7 +886,8 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_m(Geo g, Box b, const rea
                                                          const real *__restrict__ cv1,
                                                          const real *__restrict__ cv2,
                                                          const real *__restrict__ dm, real *__restrict__ vxo,
-                                                         real *__restrict__ vyo, real *__restrict__ vzo, double dt) {
+                                                         real *__restrict__ vyo, real *__restrict__ vzo, double dt,
+                                                         RkeOut rk) {
   MCOLUMN(M, b, i, p, m0, m1)
   const int MS = M == 2 ? g.sz : g.sy;
   const int nM = g.n(M), nP = g.n(M == 2 ? 1 : 2);
@@ -846,8 +919,12 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_m(Geo g, Box b, const rea
   VMarch<M> w;
   VertexV vc = vertex_march<M>(g, rho1, cv1, cv2, dm, c, w, false);
   VertexV vn = vertex_march<M>(g, rho1, cv1, cv2, dm, c + MS, w, true);
-  real Fp[3], dp;
-  for (int m = ms - 1; m <= me; ++m) {
+  real Fp[3], dp = 0;
+  // rke: flux velocities, dm and node mass carried from m-1 and m-2
+  real FLp[3] = {0, 0, 0}, FLpp[3] = {0, 0, 0}, dpp = 0;
+  acc massp = 0;
+  const int mend = (RKE && me < nM) ? me + 1 : me;
+  for (int m = ms - 1; m <= mend; ++m) {
     if (m > ms - 1) {
       // advance the window from m-1 (index c) to m
 #pragma unroll
@@ -868,41 +945,100 @@ __global__ __launch_bounds__(BX * BY) void k_mom_remap_m(Geo g, Box b, const rea
     }
     const int j = MJ(M, m, p), k = MK(M, m, p);
     // fluxes at m
-    const double area = mom_area<M>(g, i, j, k);
-    const double v_advect = (v0 + v1) * 0.5;
-    const double sign_v = copysign(1.0, v_advect);
-    const double vad_p = (sign_v + 1.0) * 0.5;
-    const double vad_m = 1.0 - vad_p;
-    const double dci = d[1];
-    const double dcu = d[0] * vad_p + d[2] * vad_m;
-    const double dbu = 0.5 * (d[0] + d[1]) * vad_p + 0.5 * (d[1] + d[2]) * vad_m;
-    const double phi = fabs(v_advect) * dt / dbu;
-    const double rhou = (double)vc.rv * vad_p + (double)vn.rv * vad_m;
+    const acc area = mom_area<M>(g, i, j, k);
+    const acc v_advect = (v0 + v1) * (acc)0.5;
+    const double sign_v = copysign(1.0, (double)v_advect);
+    const acc vad_p = (acc)((sign_v + 1.0) * 0.5);
+    const acc vad_m = (acc)1.0 - vad_p;
+    const acc dci = d[1];
+    const acc dcu = d[0] * vad_p + d[2] * vad_m;
+    const acc dbu = (acc)0.5 * (d[0] + d[1]) * vad_p + (acc)0.5 * (d[1] + d[2]) * vad_m;
+    const acc phi = fabs(v_advect) * (acc)dt / dbu;
+    const acc rhou = (acc)vc.rv * vad_p + (acc)vn.rv * vad_m;
     const real dm_ = vc.dm;
-    const double dd = dm_;
-    const double dmu = fabs(dd) / area / dbu / rhou;
-    real F[3];
+    const acc dd = dm_;
+    const acc dmu = fabs(dd) / area / dbu / rhou;
+    real F[3], FL[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       const double fm = u[q][0], fi = u[q][1], fp = u[q][2], fp2 = u[q][3];
       const double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
-      const double fu = fi * vad_p + fp * vad_m;
-      const double dfu = dfm * vad_p + dfp * vad_m;
-      const double Di = vl_di(dfi, dfu, sign_v, dci, dcu, dbu, phi);
-      const double f = fu + Di * (1.0 - dmu);
+      const acc fu = (acc)fi * vad_p + (acc)fp * vad_m;
+      const double dfu = dfm * (double)vad_p + dfp * (double)vad_m;
+      const acc Di = vl_di<acc>(dfi, dfu, sign_v, dci, dcu, dbu, phi);
+      const acc f = fu + Di * ((acc)1.0 - dmu);
+      FL[q] = f;
       F[q] = f * dd;
     }
     if (m >= ms) {
-      // the update at m
-      const double rv = vc.rv, c1 = vc.c1, c2 = vc.c2;
-      const double rvn = (rv * c1 + dp - dm_) / c2;
+      // rho_v1 at m, and the update at m (nodes ms..me)
+      const acc rv = vc.rv, c1 = vc.c1, c2 = vc.c2;
+      const acc rvn = (rv * c1 + dp - dm_) / c2;
+      if (m <= me) {
 #pragma unroll
-      for (int q = 0; q < 3; ++q) VO[q][c] = (rv * u[q][1] * c1 + Fp[q] - F[q]) / (c2 * rvn);
+        for (int q = 0; q < 3; ++q) VO[q][c] = (rv * u[q][1] * c1 + Fp[q] - F[q]) / (c2 * rvn);
+      }
+      if constexpr (RKE) {
+        const acc mass = (acc)(real)rvn * c2;
+        // the edge between m-1 and m (its lower node m-1 in ms..me, below n)
+        if (m > ms && m <= nM) {
+#pragma unroll
+          for (int q = 0; q < 3; ++q)
+            rk.dk[q][c - MS] = rke_edge(u[q][0], u[q][1], FLpp[q], FLp[q], FL[q], dpp, dp, dm_, massp, mass);
+        }
+        massp = mass;
+      }
     }
 #pragma unroll
     for (int q = 0; q < 3; ++q) Fp[q] = F[q];
+    if constexpr (RKE) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        FLpp[q] = FLp[q];
+        FLp[q] = FL[q];
+      }
+      dpp = dp;
+    }
     dp = dm_;
   }
 }
 
+// delta_ke of the cells 1..n gathers the edge terms of one sweep along AX
+// (xremap.f90:687-690, yremap.f90:688-691, zremap.f90:687-690): cell
+// (.., s, ..) takes the edges whose lower node has sweep coordinate s-1 and
+// transverse coordinates t-1 or t.  For each velocity component in turn
+// (x, y, z) the four terms are added in the order the reference's loops
+// reach them: the outer transverse axis T2 lower first, then the inner T1.
+template <int AX>
+__global__ __launch_bounds__(BX * BY) void k_rke_gather(Geo g, Box b, RkeOut rk, real *__restrict__ dke) {
+  BOX_INDEX(b, i, j, k);
+  constexpr int T1 = CtComp<AX>::T1, T2 = CtComp<AX>::T2;
+  const int c = g.idx(i, j, k);
+  const int S1 = g.stride(T1), S2 = g.stride(T2);
+  const int e = c - g.stride(AX);
+  acc v = dke[c];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const real *d = rk.dk[q];
+    v = v + d[e - S1 - S2];
+    v = v + d[e - S2];
+    v = v + d[e - S1];
+    v = v + d[e];
+  }
+  dke[c] = v;
+}
 
+// energy_correction (diagnostics.F90:613-629) on cells 1..n: the kinetic
+// energy lost by the remap (delta_ke < 0) added to the specific internal
+// energy.  Only the interior of delta_ke is used, so only it is formed.
+__global__ __launch_bounds__(BX * BY) void k_energy_correction(Geo g, Box b, real *__restrict__ dke,
+                                                               const real *__restrict__ rho,
+                                                               real *__restrict__ energy) {
+  BOX_INDEX(b, i, j, k);
+  const int c = g.idx(i, j, k);
+  acc d = -(acc)dke[c];
+  if (d < (acc)0.0) d = 0.0;
+  d = d / ((acc)rho[c] * (acc)g.cv(i, j, k));
+  dke[c] = d;
+  energy[c] = energy[c] + d;
+}

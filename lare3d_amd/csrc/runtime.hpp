@@ -55,7 +55,9 @@ struct Dev {
   double *bpx, *bpy, *bpz;       // predictor B
   double *fxr, *fyr, *fzr;       // resistive EMFs (CT update of B)
   real *qx, *qy, *qz;
-  // remap scratch: F is the CT face flux (FP64), Fe the energy flux
+  // remap scratch: F, F2 the CT face fluxes (FP64-sized arrays, holding
+  // the fluxes in the storage type; the potential field's FP64 scratch),
+  // Fe the energy flux
   real *cv1, *cv2, *db1, *Fe, *dm, *Fx, *Fy, *Fz;
   double *F, *F2;
   // conduction scratch
@@ -63,6 +65,9 @@ struct Dev {
   // options: gamma_boris (boris), cool_term_v / cool_term_b and the stored
   // curlb of the ohmic cooling (cooling_term); NULL when off
   real *gamma_boris = nullptr, *cool_v = nullptr, *cool_b = nullptr, *curlb = nullptr;
+  // rke: delta_ke (cells) and one sweep's edge terms per velocity component
+  real *delta_ke = nullptr;
+  RkeOut rk = {};
   // reductions
   double *partial = nullptr;
   size_t partial_cap = 0;
@@ -436,29 +441,29 @@ int ct_pair(double *const Bin[3], double *const B[3], const real *vel, double dt
     const long waves = (npts + CTX_OUT - 1) / CTX_OUT;
     const dim3 gx((unsigned)((waves + BY - 1) / BY), 1, (unsigned)((fb.z1 - fb.z0 + 1 + ZCHUNK - 1) / ZCHUNK));
     if (sc > 0)
-      LAUNCH("ct_flux", k_ct_flux_x<1>, gx, box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1, S.F, S.F2,
+      LAUNCH("ct_flux", k_ct_flux_x<1>, gx, box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1, (real *)S.F, (real *)S.F2,
              strides(ga1), strides(va2), dt);
     else
-      LAUNCH("ct_flux", k_ct_flux_x<0>, gx, box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1, S.F, S.F2,
+      LAUNCH("ct_flux", k_ct_flux_x<0>, gx, box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1, (real *)S.F, (real *)S.F2,
              strides(ga1), strides(va2), dt);
   } else {
     // y and z sweeps march along the sweep axis; the grid covers the other two
     const Box gb = march_box(fb, AX);
     if (sc > 0)
       LAUNCH("ct_flux", (k_ct_flux_m<AX, 1>), zchunk_grid(gb), box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1,
-             S.F, S.F2, strides(va1), strides(ga1), strides(va2), strides(ga2), dt);
+             (real *)S.F, (real *)S.F2, strides(va1), strides(ga1), strides(va2), strides(ga2), dt);
     else
       LAUNCH("ct_flux", (k_ct_flux_m<AX, 0>), zchunk_grid(gb), box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1,
-             S.F, S.F2, strides(va1), strides(ga1), strides(va2), strides(ga2), dt);
+             (real *)S.F, (real *)S.F2, strides(va1), strides(ga1), strides(va2), strides(ga2), dt);
   }
   if (sc == 0)
-    LAUNCH_BOX("ct_update", (k_ct_update<AX, 0>), ub, S.F, S.F2, B[t1], B[t2], B[AX], B[t1], B[t2], B[AX],
+    LAUNCH_BOX("ct_update", (k_ct_update<AX, 0>), ub, (real *)S.F, (real *)S.F2, B[t1], B[t2], B[AX], B[t1], B[t2], B[AX],
                strides(AX), strides(t1), strides(t2), bt1, ba1, bt2, ba2);
   else if (sc > 0)
-    LAUNCH_BOX("ct_update", (k_ct_update<AX, 1>), full_box(), S.F, S.F2, Bin[t1], Bin[t2], Bin[AX], B[t1], B[t2],
+    LAUNCH_BOX("ct_update", (k_ct_update<AX, 1>), full_box(), (real *)S.F, (real *)S.F2, Bin[t1], Bin[t2], Bin[AX], B[t1], B[t2],
                B[AX], strides(AX), strides(t1), strides(t2), bt1, ba1, bt2, ba2);
   else
-    LAUNCH_BOX("ct_update", (k_ct_update<AX, -1>), full_box(), S.F, S.F2, B[t1], B[t2], B[AX], B[t1], B[t2], B[AX],
+    LAUNCH_BOX("ct_update", (k_ct_update<AX, -1>), full_box(), (real *)S.F, (real *)S.F2, B[t1], B[t2], B[AX], B[t1], B[t2], B[AX],
                strides(AX), strides(t1), strides(t2), bt1, ba1, bt2, ba2);
   return 0;
 }
@@ -529,14 +534,25 @@ int sweep_full(double dt, double pass[3], bool first, bool last) {
     const Box xb = full_box();
     if constexpr (AX == 0) {
       const long npts = (long)(xb.x1 - xb.x0 + 1) * (xb.y1 - xb.y0 + 1);
-      const long waves = (npts + MOMX_OUT - 1) / MOMX_OUT;
+      const long out = S.delta_ke ? MOMX_OUT_RKE : MOMX_OUT;
+      const long waves = (npts + out - 1) / out;
       const dim3 gx((unsigned)((waves + BY - 1) / BY), 1, (unsigned)((xb.z1 - xb.z0 + ZCHUNK) / ZCHUNK));
-      LAUNCH("mom_remap", k_mom_remap_x, gx, box_block(), S.g, xb, S.vx, S.vy, S.vz, vel, S.db1, S.rho, S.cv1,
-             S.cv2, S.dm, S.Fx, S.Fy, S.Fz, dt);
+      if (S.delta_ke)
+        LAUNCH("mom_remap", k_mom_remap_x<true>, gx, box_block(), S.g, xb, S.vx, S.vy, S.vz, vel, S.db1, S.rho,
+               S.cv1, S.cv2, S.dm, S.Fx, S.Fy, S.Fz, dt, S.rk);
+      else
+        LAUNCH("mom_remap", k_mom_remap_x<false>, gx, box_block(), S.g, xb, S.vx, S.vy, S.vz, vel, S.db1, S.rho,
+               S.cv1, S.cv2, S.dm, S.Fx, S.Fy, S.Fz, dt, S.rk);
     } else {
-      LAUNCH("mom_remap", k_mom_remap_m<AX>, zchunk_grid(march_box(xb, AX)), box_block(), S.g, xb, S.vx, S.vy, S.vz,
-             vel, S.db1, S.rho, S.cv1, S.cv2, S.dm, S.Fx, S.Fy, S.Fz, dt);
+      if (S.delta_ke)
+        LAUNCH("mom_remap", (k_mom_remap_m<AX, true>), zchunk_grid(march_box(xb, AX)), box_block(), S.g, xb, S.vx,
+               S.vy, S.vz, vel, S.db1, S.rho, S.cv1, S.cv2, S.dm, S.Fx, S.Fy, S.Fz, dt, S.rk);
+      else
+        LAUNCH("mom_remap", (k_mom_remap_m<AX, false>), zchunk_grid(march_box(xb, AX)), box_block(), S.g, xb, S.vx,
+               S.vy, S.vz, vel, S.db1, S.rho, S.cv1, S.cv2, S.dm, S.Fx, S.Fy, S.Fz, dt, S.rk);
     }
+    // rke: this sweep's edge terms into delta_ke
+    if (S.delta_ke) LAUNCH_BOX("rke_gather", k_rke_gather<AX>, interior(), S.rk, S.delta_ke);
     real *tv;
     tv = S.vx; S.vx = S.Fx; S.Fx = tv;
     tv = S.vy; S.vy = S.Fy; S.Fy = tv;
@@ -572,6 +588,8 @@ int eulerian_remap(int step, bool predictor) {
     }
     return bfield_bcs(S.bpx, S.bpy, S.bpz);
   }
+  // rke: delta_ke = 0 (remap.f90:38); only the interior is ever read
+  if (S.delta_ke) LAUNCH_BOX("fill", k_fill, interior(), S.delta_ke, 0.0);
   double pass[3] = {1.0, 1.0, 1.0};
   for (int q = 0; q < 3; ++q) {
     const int ax = kStrang[c][q];
@@ -650,8 +668,12 @@ int lagrangian_step() {
   LagArgs a = lag_args();  // dt = previous step's dt (Q1)
   {
     const Box vb = mkbox(-1, N0 + 1, -1, N1 + 1, -1, N2 + 1);
-    LAUNCH("vertex_prep", k_vertex_prep, zchunk_grid(vb), box_block(), S.g, vb, S.rho, S.energy, S.bx, S.by, S.bz,
-           S.gamma_boris, S.rho_v, S.cs_v, S.p.gamma);
+    if (S.gamma_boris)
+      LAUNCH("vertex_prep", k_vertex_prep<true>, zchunk_grid(vb), box_block(), S.g, vb, S.rho, S.energy, S.bx, S.by,
+             S.bz, S.gamma_boris, S.rho_v, S.cs_v, S.p.gamma);
+    else
+      LAUNCH("vertex_prep", k_vertex_prep<false>, zchunk_grid(vb), box_block(), S.g, vb, S.rho, S.energy, S.bx, S.by,
+             S.bz, S.gamma_boris, S.rho_v, S.cs_v, S.p.gamma);
   }
   if (S.p.use_viscous_damping) {
     LAUNCH_BOX("damp_q", k_damp_q, nodes0n(), S.vx, S.vy, S.vz, S.visc3, S.qx, S.qy, S.qz);
@@ -755,9 +777,14 @@ int lagrangian_step() {
     if (rc) return rc;
   }
   dim3 g3 = zchunk_grid(interior());
-  LAUNCH("shock_corr", k_shock_corr, g3, box_block(), S.g, interior(), S.vx, S.vy, S.vz, S.vx1, S.vy1, S.vz1, S.al1,
-         S.al2, S.al3, S.pp, S.energy, S.rho, S.visc_dep, S.Fx, S.Fy, S.Fz, S.cool_v, S.p.alpha_av, S.partial,
-         S.dt);
+  if (S.cool_v)
+    LAUNCH("shock_corr", k_shock_corr<true>, g3, box_block(), S.g, interior(), S.vx, S.vy, S.vz, S.vx1, S.vy1, S.vz1,
+           S.al1, S.al2, S.al3, S.pp, S.energy, S.rho, S.visc_dep, S.Fx, S.Fy, S.Fz, S.cool_v, S.p.alpha_av,
+           S.partial, S.dt);
+  else
+    LAUNCH("shock_corr", k_shock_corr<false>, g3, box_block(), S.g, interior(), S.vx, S.vy, S.vz, S.vx1, S.vy1, S.vz1,
+           S.al1, S.al2, S.al3, S.pp, S.energy, S.rho, S.visc_dep, S.Fx, S.Fy, S.Fz, S.cool_v, S.p.alpha_av,
+           S.partial, S.dt);
   rc = final_reduce(g3, 1, 0, 8, 1);
   if (rc) return rc;
   {
@@ -925,6 +952,13 @@ int api_init(const lare_grid *grid, const lare_params *params) {
     if (rc) return rc;
   }
   S.gamma_boris = S.cool_v = S.cool_b = S.curlb = nullptr;
+  S.delta_ke = nullptr;
+  S.rk = RkeOut{};
+  if (params->rke) {
+    int rc = dalloc(&S.delta_ke);
+    for (int q = 0; q < 3 && !rc; ++q) rc = dalloc(&S.rk.dk[q]);
+    if (rc) return rc;
+  }
   if (params->boris) {
     int rc = dalloc(&S.gamma_boris);
     if (rc) return rc;
@@ -1091,6 +1125,20 @@ int api_energy_account(double en[3], double *divb) {
   return 0;
 }
 
+// diagnostics.F90:613-629 energy_correction, then energy_bcs
+int energy_correction() {
+  LAUNCH_BOX("energy_correction", k_energy_correction, interior(), S.delta_ke, S.rho, S.energy);
+  return energy_bcs();
+}
+
+int api_energy_correction() {
+  REQUIRE_LOADED();
+  if (!S.delta_ke) return fail(LARE_E_STATE, "energy_correction needs rke on");
+  return energy_correction();
+}
+
+// lare3d.f90:82-87 loop body (lagrangian_step, eulerian_remap,
+// energy_correction when rke, eta_calc)
 int api_step(int32_t step, double *dt, double *time) {
   REQUIRE_LOADED();
   S.step = step;
@@ -1098,6 +1146,10 @@ int api_step(int32_t step, double *dt, double *time) {
   if (rc) return rc;
   rc = eulerian_remap(step, false);
   if (rc) return rc;
+  if (S.delta_ke) {
+    rc = energy_correction();
+    if (rc) return rc;
+  }
   rc = eta_calc();
   if (rc) return rc;
   if (!isfinite(S.dt) || (S.dt <= 0.0 && !S.p.restart))

@@ -84,6 +84,11 @@ class Device:
     def eta_calc(self):
         check(self.lib, self.lib.lare_dev_eta_calc(), "eta_calc")
 
+    def energy_correction(self):
+        """diagnostics.F90:613-629 (rke on): the remap's kinetic-energy loss
+        added to the internal energy."""
+        check(self.lib, self.lib.lare_dev_energy_correction(), "energy_correction")
+
     def conduct_heat(self):
         check(self.lib, self.lib.lare_dev_conduct_heat(), "conduct_heat")
 

@@ -25,7 +25,7 @@ MODULE lare_gpu_bridge
     INTEGER(C_INT32_T) :: resistive_mhd, conduction, heat_flux_limiter
     INTEGER(C_INT32_T) :: use_viscous_damping, restart, precision
     REAL(C_DOUBLE) :: alpha_av, va_max
-    INTEGER(C_INT32_T) :: cooling_term, boris, shock_limiter, shock_compression
+    INTEGER(C_INT32_T) :: cooling_term, boris, shock_limiter, shock_compression, rke
   END TYPE lare_params
 
   TYPE, BIND(C) :: lare_fields
@@ -69,6 +69,9 @@ MODULE lare_gpu_bridge
       INTEGER(C_INT32_T), VALUE :: step
     END FUNCTION
     INTEGER(C_INT) FUNCTION lare_dev_eta_calc() BIND(C, NAME='lare_dev_eta_calc')
+      IMPORT :: C_INT
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_energy_correction() BIND(C, NAME='lare_dev_energy_correction')
       IMPORT :: C_INT
     END FUNCTION
     INTEGER(C_INT) FUNCTION lare_dev_conduct_heat() BIND(C, NAME='lare_dev_conduct_heat')

@@ -71,6 +71,7 @@ struct orc_state {
   double *larsen;          /* larsen_factor(0:nx,0:ny,0:nz), conduct.f90:41 */
   double *gamma_boris;     /* gamma_boris(-1:n+2)^3, mpi_routines.F90:311 */
   double *cool_term_v, *cool_term_b; /* (1:n)^3, mpi_routines.F90:315-319 */
+  double *delta_ke;        /* delta_ke(-1:n+2)^3 when rke, mpi_routines.F90:313 */
   /* lagran module arrays live only within one lagrangian_step */
   double *bx1, *by1, *bz1, *pressure, *rho_vl, *cv_vl, *visc_heat;
   double *alpha1, *alpha2, *alpha3, *fx_visc, *fy_visc, *fz_visc;
@@ -460,8 +461,10 @@ static void energy_flux(orc_state *o, int ax) {
 }
 
 /* momentum flux of one velocity component (x_momx_flux xremap.f90:605-699
- * and the five analogues, rke off): sweep axis -1..n, transverse 0..n;
- * then flux *= dm over the same box. */
+ * and the eight analogues): sweep axis -1..n, transverse 0..n; with rke the
+ * kinetic energy the remap loses at each edge (xremap.f90:664-695,
+ * yremap.f90:665-696, zremap.f90:664-695); then flux *= dm over the same
+ * box. */
 static void mom_flux(orc_state *o, int ax, const double *v) {
   const double *vel = ax == 0 ? o->vx1 : (ax == 1 ? o->vy1 : o->vz1);
   int lo[3] = {0, 0, 0};
@@ -485,6 +488,35 @@ static void mom_flux(orc_state *o, int ax, const double *v) {
         double dmu = fabs(AT(o->dm, c)) / area / dbu / rhou;
         AT(o->flux, c) = fu + Di * (1.0 - dmu);
       }
+  if (o->p.rke) {
+    /* edges between nodes s and s+1 for s = 0..n-1 along the sweep,
+     * transverse 0..n; each edge adds dk to the four cells around it (sweep
+     * coordinate s+1), the lower transverse axis varying fastest */
+    int hi[3] = {o->nx, o->ny, o->nz};
+    hi[ax] -= 1;
+    const int t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+    for (int iz = 0; iz <= hi[2]; ++iz)
+      for (int iy = 0; iy <= hi[1]; ++iy)
+        for (int ix = 0; ix <= hi[0]; ++ix) {
+          int c[3] = {ix, iy, iz}, m[3] = {ix, iy, iz}, p[3] = {ix, iy, iz};
+          m[ax] -= 1;
+          p[ax] += 1;
+          double mc = AT(o->rho_v1, c) * AT(o->cv2, c);
+          double mp = AT(o->rho_v1, p) * AT(o->cv2, p);
+          double ai = (AT(v, c) - AT(o->flux, m)) * AT(o->dm, m) / mc - (AT(v, c) - AT(o->flux, c)) * AT(o->dm, c) / mc;
+          double aip = (AT(v, p) - AT(o->flux, c)) * AT(o->dm, c) / mp - (AT(v, p) - AT(o->flux, p)) * AT(o->dm, p) / mp;
+          double dk = (AT(v, p) - AT(v, c)) * (AT(o->flux, c) - 0.5 * (AT(v, p) + AT(v, c))) -
+                      0.5 * ai * (AT(v, c) - AT(o->flux, c)) + 0.5 * aip * (AT(v, p) - AT(o->flux, c));
+          dk = dk * AT(o->dm, c) * 0.5;
+          for (int q = 0; q < 4; ++q) {
+            int d[3] = {ix, iy, iz};
+            d[ax] += 1;
+            d[t1] += q & 1;
+            d[t2] += q >> 1;
+            AT(o->delta_ke, d) = AT(o->delta_ke, d) + dk;
+          }
+        }
+  }
   for (int iz = lo[2]; iz <= o->nz; ++iz)
     for (int iy = lo[1]; iy <= o->ny; ++iy)
       for (int ix = lo[0]; ix <= o->nx; ++ix) A(o->flux, ix, iy, iz) = A(o->flux, ix, iy, iz) * A(o->dm, ix, iy, iz);
@@ -653,6 +685,10 @@ static void remap_dir(orc_state *o, int ax) {
 /* remap.f90:33-131 eulerian_remap */
 void orc_eulerian_remap(orc_state *o, int step) {
   int nx = o->nx, ny = o->ny, nz = o->nz;
+  if (o->p.rke)
+    for (int iz = -1; iz <= nz + 2; ++iz)
+      for (int iy = -1; iy <= ny + 2; ++iy)
+        for (int ix = -1; ix <= nx + 2; ++ix) A(o->delta_ke, ix, iy, iz) = 0.0;
   o->xpass = o->ypass = o->zpass = 1;
   for (int iz = -1; iz <= nz + 2; ++iz)
     for (int iy = -1; iy <= ny + 2; ++iy)
@@ -1589,11 +1625,37 @@ double orc_divb(orc_state *o) {
   return m;
 }
 
-/* lare3d.f90:82-87 loop body (no rke, EOS_IDEAL) */
+/* diagnostics.F90:613-629 energy_correction: the kinetic energy lost by
+ * the last remap (where delta_ke < 0) added to the internal energy */
+void orc_energy_correction(orc_state *o) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  for (int iz = -1; iz <= nz + 2; ++iz)
+    for (int iy = -1; iy <= ny + 2; ++iy)
+      for (int ix = -1; ix <= nx + 2; ++ix) {
+        double d = -A(o->delta_ke, ix, iy, iz);
+        if (d < 0.0) d = 0.0;
+        A(o->delta_ke, ix, iy, iz) = d / (A(o->rho, ix, iy, iz) * A(o->cv, ix, iy, iz));
+      }
+  for (int iz = 1; iz <= nz; ++iz)
+    for (int iy = 1; iy <= ny; ++iy)
+      for (int ix = 1; ix <= nx; ++ix) A(o->energy, ix, iy, iz) = A(o->energy, ix, iy, iz) + A(o->delta_ke, ix, iy, iz);
+  cell_bcs(o, o->energy);
+}
+
+static void xfer(orc_state *o, double *a, double *f, int lx, int ly, int lz, int to_box);
+/* delta_ke(-1:n+2)^3 as it stands (rke on), Fortran order */
+int orc_get_delta_ke(orc_state *o, double *out) {
+  if (!o->delta_ke) return -1;
+  xfer(o, o->delta_ke, out, -1, -1, -1, 0);
+  return 0;
+}
+
+/* lare3d.f90:82-87 loop body (EOS_IDEAL) */
 void orc_step(orc_state *o, int step, double *dt, double *time) {
   o->step = step;
   orc_lagrangian_step(o);
   orc_eulerian_remap(o, step);
+  if (o->p.rke) orc_energy_correction(o);
   orc_eta_calc(o);
   if (dt) *dt = o->dt;
   if (time) *time = o->time;
@@ -1731,6 +1793,7 @@ orc_state *orc_create(const lare_grid *g, const lare_params *p) {
     if (o->gamma_boris[q] == 0.0) o->gamma_boris[q] = 1.0;
   o->cool_term_v = box(o, 1, o->nx, 1, o->ny, 1, o->nz);
   o->cool_term_b = box(o, 1, o->nx, 1, o->ny, 1, o->nz);
+  if (o->p.rke) o->delta_ke = cellbox(o);
   for (int iz = -1; iz <= o->nz + 2; ++iz)
     for (int iy = -1; iy <= o->ny + 2; ++iy)
       for (int ix = -1; ix <= o->nx + 2; ++ix) A(o->cv, ix, iy, iz) = DXB(ix) * DYB(iy) * DZB(iz);
@@ -1743,7 +1806,7 @@ void orc_destroy(orc_state *o) {
   if (!o) return;
   double **all[] = {&o->rho, &o->energy, &o->vx, &o->vy, &o->vz, &o->vx1, &o->vy1, &o->vz1, &o->bx, &o->by, &o->bz, &o->eta,
                     &o->cv, &o->cv1, &o->visc3, &o->visc_dep, &o->ohmic_dep, &o->p_visc, &o->temperature, &o->larsen,
-                    &o->gamma_boris, &o->cool_term_v, &o->cool_term_b};
+                    &o->gamma_boris, &o->cool_term_v, &o->cool_term_b, &o->delta_ke};
   for (unsigned q = 0; q < sizeof(all) / sizeof(all[0]); ++q) drop(all[q]);
   for (int q = 0; q < 7; ++q) free(o->m[q]);
   free(o);

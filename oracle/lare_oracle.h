@@ -45,6 +45,10 @@ double orc_divb(orc_state *o);
 double orc_pow25(double t);
 double orc_pow15(double t);
 void orc_step(orc_state *o, int step, double *dt, double *time);
+/* diagnostics.F90:613-629 energy_correction (rke) */
+void orc_energy_correction(orc_state *o);
+/* delta_ke(-1:n+2)^3 (Fortran order) as it stands; -1 when rke is off */
+int orc_get_delta_ke(orc_state *o, double *out);
 /* initial_conditions.f90:114-296 potential_field (replaces bx, by, bz);
  * phi_out (may be NULL) receives phi(-1:nx+2, -1:ny+2, -1:nz+2) */
 int orc_potential_field(orc_state *o, const lare_potential *pf, int *iterations, double *residual, double *bz_max,

@@ -22,7 +22,7 @@ ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "lare_oracle.c")
 CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-shared"]
 PIN_TESTS = ["tests/test_pin_hydro.py", "tests/test_pin_transport.py", "tests/test_pin_oracle3d.py",
-             "tests/test_pin_potential.py", "tests/test_pin_variants.py"]
+             "tests/test_pin_potential.py", "tests/test_pin_variants.py", "tests/test_pin_rke.py"]
 
 # (name, what it breaks, [(old, new), ...]); each `old` occurs exactly once
 MUTANTS = [
@@ -86,6 +86,18 @@ MUTANTS = [
     ("M25", "cooling_term: heating averaged with the old density (lagran.F90:468)",
      [("A(o->cool_term_v, ix, iy, iz) = aav * o->dt * A(o->visc_heat, ix, iy, iz) / A(o->rho, ix, iy, iz) +",
        "A(o->cool_term_v, ix, iy, iz) = aav * o->dt * A(o->visc_heat, ix, iy, iz) / A(o->rho, ix, iy, iz) / A(o->cv1, ix, iy, iz) * A(o->cv, ix, iy, iz) +")]),
+    ("M26", "rke: ai with dm at the edge s for dm at s-1 (xremap.f90:676)",
+     [("double ai = (AT(v, c) - AT(o->flux, m)) * AT(o->dm, m) / mc", "double ai = (AT(v, c) - AT(o->flux, m)) * AT(o->dm, c) / mc")]),
+    ("M27", "rke: edge term added to the cells at sweep coordinate s instead of s+1 (xremap.f90:687-690)",
+     [("            d[ax] += 1;\n", "            d[ax] += 0;\n")]),
+    ("M28", "rke: transverse cells on the lower side of the edge (iym for iyp, xremap.f90:688)",
+     [("d[t1] += q & 1;", "d[t1] -= q & 1;")]),
+    ("M29", "rke: edges over the sweep range 0..n-2 instead of 0..n-1 (xremap.f90:665-670)",
+     [("    hi[ax] -= 1;\n    const int t1", "    hi[ax] -= 2;\n    const int t1")]),
+    ("M30", "energy_correction: the gains added as well as the losses (diagnostics.F90:616)",
+     [("        if (d < 0.0) d = 0.0;\n", "")]),
+    ("M31", "rke: node masses before the sweep for after it (xremap.f90:673-674)",
+     [("double mc = AT(o->rho_v1, c) * AT(o->cv2, c);", "double mc = AT(o->rho_v, c) * AT(o->cv1, c);")]),
 ]
 
 

@@ -46,6 +46,8 @@ def _load():
         "orc_divb": ([vp], C.c_double),
         "orc_step": ([vp, C.c_int, dp, dp], None),
         "orc_potential_field": ([vp, C.POINTER(LarePotential), C.POINTER(C.c_int), dp, dp, dp], C.c_int),
+        "orc_energy_correction": ([vp], None),
+        "orc_get_delta_ke": ([vp, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -126,6 +128,17 @@ class Oracle:
         dt, t = C.c_double(), C.c_double()
         lib().orc_step(self.h, step, C.byref(dt), C.byref(t))
         return dt.value, t.value
+
+    def energy_correction(self):
+        lib().orc_energy_correction(self.h)
+
+    def delta_ke(self):
+        """delta_ke(-1:n+2)^3 as it stands (rke on)."""
+        import numpy as np
+        out = np.zeros(tuple(n + 4 for n in self.n), order="F")
+        if lib().orc_get_delta_ke(self.h, out.ctypes.data_as(C.POINTER(C.c_double))) != 0:
+            raise RuntimeError("delta_ke: rke is off")
+        return out
 
     def potential_field(self, pf, want_phi=False):
         """initial_conditions.f90:114 potential_field; returns (iterations,

@@ -1,16 +1,17 @@
-"""The FP32 storage mode on the conduction and flux-tube configurations,
-against the FP64 mode of the same device code.
+"""The FP32 mode (FP32 bulk storage and arithmetic) on the conduction and
+flux-tube configurations, against the FP64 mode of the same device code.
 
 * Braginskii conduction on the sheared arcade (RKL2 super-steps with a
   fixed dt, FP32 bulk with FP64 reductions): per super-step relative L2 of
-  the temperature, 20 super-steps.
+  the temperature, 20 super-steps; and the surrogate's trapezoid residual of
+  the plain RKL2 super-step in both precisions.
 * The kink-unstable twisted flux tube (resistive MHD with Braginskii
   conduction, walls in x and y, periodic z): per-field relative L2 after
   each step, the energy budget (magnetic, kinetic, internal, and the total
   with the viscous and ohmic heating) and div B, which the FP64 CT update
   keeps at round-off in both modes.
 
-Bars are stated per quantity; FP32 storage is not bit-exact by design.
+Bars are stated per quantity; the FP32 mode is not bit-exact by design.
 """
 import numpy as np
 import pytest
@@ -104,3 +105,29 @@ def test_kink_tube_fp32_vs_fp64():
             assert abs(x - y) <= 1e-5 * abs(y) + 1e-7 * tot64
         assert abs(tot64 - e_start) <= 1e-3 * e_start
         assert dvb64 <= 1e-12 and dvb32 <= 1e-12
+
+
+def _plain_residual(precision):
+    from lare3d_amd.device import Device
+    from lare3d_amd.surrogate import Surrogate
+    grid, params, f, dt = configs.harris_conduction(32, seed=999, s_target=5)
+    params.precision = precision
+    dev = Device(grid, params, f)
+    try:
+        dev.boundary_conditions()
+        dev.set_scalars(dt=dt)
+        return Surrogate(dev).plain_residual()
+    finally:
+        dev.close()
+
+
+def test_trapezoid_residual_fp32_vs_fp64():
+    """The residual check of the conduction surrogate evaluates the
+    trapezoid with the stage kernel (all history weights zero).  Applied to
+    the plain RKL2 super-step it must read the same in both precisions."""
+    print("\n[case] trapezoid residual of plain RKL2, Harris + conduction 32^3, FP32 vs FP64", flush=True)
+    r64 = _plain_residual(PREC_FP64)
+    r32 = _plain_residual(PREC_FP32)
+    print(f"residual FP64 {r64:.4e}, FP32 {r32:.4e}")
+    assert 0.0 < r64 < 1e-4
+    assert abs(r32 - r64) <= 0.05 * r64 + 1e-6

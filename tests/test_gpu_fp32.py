@@ -1,8 +1,11 @@
-"""FP32 storage mode (LARE_PREC_FP32) against the FP64 mode of the same code.
+"""FP32 mode (LARE_PREC_FP32) against the FP64 mode of the same code.
 
-FP32 mode keeps the bulk fields in FP32 and the face magnetic field, its CT
-fluxes, the metrics and all reductions in FP64, with FP64 register
-arithmetic.  Bar (stated, not bit-exact): per-field relative L2 error against
+FP32 mode stores the bulk fields in FP32 and computes the bulk arithmetic in
+FP32 (Lagrangian step, resistive substep, the remap's geometry, mass, energy
+and momentum fluxes, RKL2 stages).  FP64 stays for the face magnetic field
+and its constrained-transport path (quotients, upwind values, fluxes and the
+update), the Van Leer limiter's gradient comparisons, the metrics and all
+reductions.  Bar (stated, not bit-exact): per-field relative L2 error against
 the FP64 run after each step, and div B at FP64 round-off because the CT
 update of B stays FP64.
 """

@@ -98,6 +98,12 @@ CASES = {
     "harris_options": lambda: _with(configs.harris(16, ny=13, nz=11), cooling_term=1, alpha_av=0.05, boris=1,
                                     va_max=0.6, shock_limiter=1),
     "ot_compression": lambda: _with(configs.orszag_tang(16), shock_compression=1, cooling_term=1, alpha_av=0.3),
+    # the remap kinetic-energy correction: odd sizes (chunks of the marching
+    # momentum remap that end inside the edge range) and a wide x extent
+    # (edge terms whose neighbours sit in the next wave)
+    "harris_rke": lambda: _with(configs.harris(19, ny=13, nz=11), rke=1),
+    "wide_rke": lambda: _with(configs.harris(64, ny=9, nz=6), rke=1),
+    "ot_rke": lambda: _with(configs.orszag_tang(16), rke=1),
 }
 
 
