@@ -86,6 +86,9 @@ typedef struct lare_params {
   int32_t shock_compression; /* the SHOCKCOMPRESSION variant (:807-808)   */
   int32_t rke;             /* remap kinetic-energy correction (control.f90:148;
                               xremap.f90:664-695, diagnostics.F90:613-629) */
+  int32_t cauchy;          /* the CAUCHY predictor: cell-centred B advanced by
+                              the Cauchy solution instead of the predictor
+                              remap (lagran.F90:285-348, :945-1048) */
 } lare_params;
 
 /* Field pointers (Fortran arrays, see the array convention above).

@@ -42,6 +42,7 @@ def default_params(**kw):
     p.shock_limiter = 0
     p.shock_compression = 0
     p.rke = 0
+    p.cauchy = 0
     for k, v in kw.items():
         setattr(p, k, v)
     return p

@@ -1347,17 +1347,127 @@ static void set_dt(orc_state *o) {
 }
 
 /* lagran.F90:208-478 predictor_corrector_step (non-CAUCHY) */
+/* b_field_and_cv1_update with CAUCHY (lagran.F90:945-1048): cv1, and the
+ * cell-centred B advanced by the Cauchy solution
+ * B1 = (B1 + (B1 . grad) v dt/2) / (1 + div v dt/2); no predictor remap */
+static void cauchy_b_update(orc_state *o) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  double *vx = o->vx, *vy = o->vy, *vz = o->vz;
+  for (int iz = -1; iz <= nz + 2; ++iz) {
+    int izm = iz - 1;
+    for (int iy = -1; iy <= ny + 2; ++iy) {
+      int iym = iy - 1;
+      for (int ix = -1; ix <= nx + 2; ++ix) {
+        int ixm = ix - 1;
+        /* the normal derivatives at the faces of B, as without CAUCHY */
+        double vxb = (A(vx, ix, iy, iz) + A(vx, ix, iym, iz) + A(vx, ix, iy, izm) + A(vx, ix, iym, izm)) * 0.25;
+        double vxbm = (A(vx, ixm, iy, iz) + A(vx, ixm, iym, iz) + A(vx, ixm, iy, izm) + A(vx, ixm, iym, izm)) * 0.25;
+        double vyb = (A(vy, ix, iy, iz) + A(vy, ixm, iy, iz) + A(vy, ix, iy, izm) + A(vy, ixm, iy, izm)) * 0.25;
+        double vybm = (A(vy, ix, iym, iz) + A(vy, ixm, iym, iz) + A(vy, ix, iym, izm) + A(vy, ixm, iym, izm)) * 0.25;
+        double vzb = (A(vz, ix, iy, iz) + A(vz, ixm, iy, iz) + A(vz, ix, iym, iz) + A(vz, ixm, iym, iz)) * 0.25;
+        double vzbm = (A(vz, ix, iy, izm) + A(vz, ixm, iy, izm) + A(vz, ix, iym, izm) + A(vz, ixm, iym, izm)) * 0.25;
+        double dvxdx = (vxb - vxbm) / DXB(ix);
+        double dvydy = (vyb - vybm) / DYB(iy);
+        double dvzdz = (vzb - vzbm) / DZB(iz);
+        double dv = (dvxdx + dvydy + dvzdz) * o->dt2;
+        A(o->cv1, ix, iy, iz) = A(o->cv, ix, iy, iz) * (1.0 + dv);
+        /* the transverse derivatives (:989-1033) */
+        vxb = (A(vx, ix, iy, iz) + A(vx, ixm, iy, iz) + A(vx, ix, iy, izm) + A(vx, ixm, iy, izm)) * 0.25;
+        vxbm = (A(vx, ix, iym, iz) + A(vx, ixm, iym, iz) + A(vx, ix, iym, izm) + A(vx, ixm, iym, izm)) * 0.25;
+        vyb = (A(vy, ix, iy, iz) + A(vy, ix, iym, iz) + A(vy, ix, iy, izm) + A(vy, ix, iym, izm)) * 0.25;
+        vybm = (A(vy, ixm, iy, iz) + A(vy, ixm, iym, iz) + A(vy, ixm, iy, izm) + A(vy, ixm, iym, izm)) * 0.25;
+        double dvxdy = (vxb - vxbm) / DYB(iy);
+        double dvydx = (vyb - vybm) / DXB(ix);
+        vxb = (A(vx, ix, iy, iz) + A(vx, ixm, iy, iz) + A(vx, ix, iym, iz) + A(vx, ixm, iym, iz)) * 0.25;
+        vxbm = (A(vx, ix, iy, izm) + A(vx, ixm, iy, izm) + A(vx, ix, iym, izm) + A(vx, ixm, iym, izm)) * 0.25;
+        vzb = (A(vz, ix, iy, iz) + A(vz, ix, iym, iz) + A(vz, ix, iy, izm) + A(vz, ix, iym, izm)) * 0.25;
+        vzbm = (A(vz, ixm, iy, iz) + A(vz, ixm, iym, iz) + A(vz, ixm, iy, izm) + A(vz, ixm, iym, izm)) * 0.25;
+        double dvxdz = (vxb - vxbm) / DZB(iz);
+        double dvzdx = (vzb - vzbm) / DXB(ix);
+        vyb = (A(vy, ix, iy, iz) + A(vy, ixm, iy, iz) + A(vy, ix, iym, iz) + A(vy, ixm, iym, iz)) * 0.25;
+        vybm = (A(vy, ix, iy, izm) + A(vy, ixm, iy, izm) + A(vy, ix, iym, izm) + A(vy, ixm, iym, izm)) * 0.25;
+        vzb = (A(vz, ix, iy, iz) + A(vz, ixm, iy, iz) + A(vz, ix, iy, izm) + A(vz, ixm, iy, izm)) * 0.25;
+        vzbm = (A(vz, ix, iym, iz) + A(vz, ixm, iym, iz) + A(vz, ix, iym, izm) + A(vz, ixm, iym, izm)) * 0.25;
+        double dvydz = (vyb - vybm) / DZB(iz);
+        double dvzdy = (vzb - vzbm) / DYB(iy);
+        double b1x = A(o->bx1, ix, iy, iz), b1y = A(o->by1, ix, iy, iz), b1z = A(o->bz1, ix, iy, iz);
+        double w3 = b1x * dvxdx + b1y * dvxdy + b1z * dvxdz;
+        double w4 = b1x * dvydx + b1y * dvydy + b1z * dvydz;
+        double w5 = b1x * dvzdx + b1y * dvzdy + b1z * dvzdz;
+        A(o->bx1, ix, iy, iz) = (b1x + w3 * o->dt2) / (1.0 + dv);
+        A(o->by1, ix, iy, iz) = (b1y + w4 * o->dt2) / (1.0 + dv);
+        A(o->bz1, ix, iy, iz) = (b1z + w5 * o->dt2) / (1.0 + dv);
+      }
+    }
+  }
+}
+
+/* J x B of the CAUCHY predictor at node (ix, iy, iz) (lagran.F90:285-348):
+ * currents and B averaged from the cell-centred B1 cv1 of the eight cells
+ * around the node */
+static void cauchy_jb(orc_state *o, int ix, int iy, int iz, double *jx, double *jy, double *jz, double *bxv,
+                      double *byv, double *bzv) {
+  int ixp = ix + 1, iyp = iy + 1, izp = iz + 1;
+  double *c1 = o->cv1, *X = o->bx1, *Y = o->by1, *Z = o->bz1;
+  double cvx = A(c1, ix, iy, iz) + A(c1, ix, iyp, iz) + A(c1, ix, iy, izp) + A(c1, ix, iyp, izp);
+  double cvxp = A(c1, ixp, iy, iz) + A(c1, ixp, iyp, iz) + A(c1, ixp, iy, izp) + A(c1, ixp, iyp, izp);
+  double cvy = A(c1, ix, iy, iz) + A(c1, ixp, iy, iz) + A(c1, ix, iy, izp) + A(c1, ixp, iy, izp);
+  double cvyp = A(c1, ix, iyp, iz) + A(c1, ixp, iyp, iz) + A(c1, ix, iyp, izp) + A(c1, ixp, iyp, izp);
+  double cvz = A(c1, ix, iy, iz) + A(c1, ixp, iy, iz) + A(c1, ix, iyp, iz) + A(c1, ixp, iyp, iz);
+  double cvzp = A(c1, ix, iy, izp) + A(c1, ixp, iy, izp) + A(c1, ix, iyp, izp) + A(c1, ixp, iyp, izp);
+  double w1 = (A(Z, ix, iy, iz) + A(Z, ixp, iy, iz) + A(Z, ix, iy, izp) + A(Z, ixp, iy, izp)) / cvy;
+  double w2 = (A(Z, ix, iyp, iz) + A(Z, ixp, iyp, iz) + A(Z, ix, iyp, izp) + A(Z, ixp, iyp, izp)) / cvyp;
+  double x = (w2 - w1) / DYC(iy);
+  w1 = (A(Y, ix, iy, iz) + A(Y, ixp, iy, iz) + A(Y, ix, iyp, iz) + A(Y, ixp, iyp, iz)) / cvz;
+  w2 = (A(Y, ix, iy, izp) + A(Y, ixp, iy, izp) + A(Y, ix, iyp, izp) + A(Y, ixp, iyp, izp)) / cvzp;
+  *jx = x - (w2 - w1) / DZC(iz);
+  w1 = (A(Z, ix, iy, iz) + A(Z, ix, iyp, iz) + A(Z, ix, iy, izp) + A(Z, ix, iyp, izp)) / cvx;
+  w2 = (A(Z, ixp, iy, iz) + A(Z, ixp, iyp, iz) + A(Z, ixp, iy, izp) + A(Z, ixp, iyp, izp)) / cvxp;
+  double y = -(w2 - w1) / DXC(ix);
+  w1 = (A(X, ix, iy, iz) + A(X, ixp, iy, iz) + A(X, ix, iyp, iz) + A(X, ixp, iyp, iz)) / cvz;
+  w2 = (A(X, ix, iy, izp) + A(X, ixp, iy, izp) + A(X, ix, iyp, izp) + A(X, ixp, iyp, izp)) / cvzp;
+  *jy = y + (w2 - w1) / DZC(iz);
+  w1 = (A(Y, ix, iy, iz) + A(Y, ix, iyp, iz) + A(Y, ix, iy, izp) + A(Y, ix, iyp, izp)) / cvx;
+  w2 = (A(Y, ixp, iy, iz) + A(Y, ixp, iyp, iz) + A(Y, ixp, iy, izp) + A(Y, ixp, iyp, izp)) / cvxp;
+  double z = (w2 - w1) / DXC(ix);
+  w1 = (A(X, ix, iy, iz) + A(X, ixp, iy, iz) + A(X, ix, iy, izp) + A(X, ixp, iy, izp)) / cvy;
+  w2 = (A(X, ix, iyp, iz) + A(X, ixp, iyp, iz) + A(X, ix, iyp, izp) + A(X, ixp, iyp, izp)) / cvyp;
+  *jz = z - (w2 - w1) / DYC(iy);
+  *bxv = (A(X, ix, iy, iz) + A(X, ixp, iy, iz) + A(X, ix, iyp, iz) + A(X, ixp, iyp, iz) + A(X, ix, iy, izp) +
+          A(X, ixp, iy, izp) + A(X, ix, iyp, izp) + A(X, ixp, iyp, izp)) / (cvx + cvxp);
+  *byv = (A(Y, ix, iy, iz) + A(Y, ixp, iy, iz) + A(Y, ix, iyp, iz) + A(Y, ixp, iyp, iz) + A(Y, ix, iy, izp) +
+          A(Y, ixp, iy, izp) + A(Y, ix, iyp, izp) + A(Y, ixp, iyp, izp)) / (cvx + cvxp);
+  *bzv = (A(Z, ix, iy, iz) + A(Z, ixp, iy, iz) + A(Z, ix, iyp, iz) + A(Z, ixp, iyp, iz) + A(Z, ix, iy, izp) +
+          A(Z, ixp, iy, izp) + A(Z, ix, iyp, izp) + A(Z, ixp, iyp, izp)) / (cvx + cvxp);
+}
+
 static void predictor_corrector_step(orc_state *o) {
   int nx = o->nx, ny = o->ny, nz = o->nz;
   double gm1 = o->p.gamma - 1.0, xi = o->p.xi_n;
-  double *bx0 = (double *)malloc(sizeof(double) * o->ntot), *by0 = (double *)malloc(sizeof(double) * o->ntot),
-         *bz0 = (double *)malloc(sizeof(double) * o->ntot);
-  copy_all(o, bx0, o->bx);
-  copy_all(o, by0, o->by);
-  copy_all(o, bz0, o->bz);
+  const int cauchy = o->p.cauchy;
+  double *bx0 = NULL, *by0 = NULL, *bz0 = NULL;
+  if (!cauchy) {
+    bx0 = (double *)malloc(sizeof(double) * o->ntot);
+    by0 = (double *)malloc(sizeof(double) * o->ntot);
+    bz0 = (double *)malloc(sizeof(double) * o->ntot);
+    copy_all(o, bx0, o->bx);
+    copy_all(o, by0, o->by);
+    copy_all(o, bz0, o->bz);
+  }
 
   /* b_field_and_cv1_update (:945-1062): cv1 over the cell box, then the
    * B-only predictor remap with dt/2 (its cv1 supersedes this one) */
+  if (cauchy) {
+    cauchy_b_update(o);
+    /* bx1 = bx1 * cv1 (:230-232) */
+    for (int iz = -1; iz <= nz + 2; ++iz)
+      for (int iy = -1; iy <= ny + 2; ++iy)
+        for (int ix = -1; ix <= nx + 2; ++ix) {
+          A(o->bx1, ix, iy, iz) = A(o->bx1, ix, iy, iz) * A(o->cv1, ix, iy, iz);
+          A(o->by1, ix, iy, iz) = A(o->by1, ix, iy, iz) * A(o->cv1, ix, iy, iz);
+          A(o->bz1, ix, iy, iz) = A(o->bz1, ix, iy, iz) * A(o->cv1, ix, iy, iz);
+        }
+  } else
   for (int iz = -1; iz <= nz + 2; ++iz) {
     int izm = iz - 1;
     for (int iy = -1; iy <= ny + 2; ++iy) {
@@ -1375,15 +1485,17 @@ static void predictor_corrector_step(orc_state *o) {
       }
     }
   }
-  copy_all(o, o->vx1, o->vx);
-  copy_all(o, o->vy1, o->vy);
-  copy_all(o, o->vz1, o->vz);
-  o->predictor_step = 1;
-  o->dt = 0.5 * o->dt;
-  orc_eulerian_remap(o, o->step);
-  o->dt = 2.0 * o->dt;
-  o->predictor_step = 0;
-  /* (bx1 = bx1 * cv1, :230-232, is not read again in this configuration) */
+  if (!cauchy) {
+    copy_all(o, o->vx1, o->vx);
+    copy_all(o, o->vy1, o->vy);
+    copy_all(o, o->vz1, o->vz);
+    o->predictor_step = 1;
+    o->dt = 0.5 * o->dt;
+    orc_eulerian_remap(o, o->step);
+    o->dt = 2.0 * o->dt;
+    o->predictor_step = 0;
+    /* (bx1 = bx1 * cv1, :230-232, is not read again without CAUCHY) */
+  }
 
   for (int iz = 0; iz <= nz + 1; ++iz)
     for (int iy = 0; iy <= ny + 1; ++iy)
@@ -1413,11 +1525,15 @@ static void predictor_corrector_step(orc_state *o) {
         w1 = (pp + ppx + ppy + ppxy) * 0.25;
         w2 = (ppz + ppxz + ppyz + ppxyz) * 0.25;
         double fz = -(w2 - w1) / DZC(iz);
-        double bxv = 0.25 * (A(o->bx, ix, iy, iz) + A(o->bx, ix, iyp, iz) + A(o->bx, ix, iy, izp) + A(o->bx, ix, iyp, izp));
-        double byv = 0.25 * (A(o->by, ix, iy, iz) + A(o->by, ixp, iy, iz) + A(o->by, ix, iy, izp) + A(o->by, ixp, iy, izp));
-        double bzv = 0.25 * (A(o->bz, ix, iy, iz) + A(o->bz, ix, iyp, iz) + A(o->bz, ixp, iy, iz) + A(o->bz, ixp, iyp, iz));
-        double jx, jy, jz;
-        vertex_j(o, ix, iy, iz, &jx, &jy, &jz);
+        double bxv, byv, bzv, jx, jy, jz;
+        if (cauchy) {
+          cauchy_jb(o, ix, iy, iz, &jx, &jy, &jz, &bxv, &byv, &bzv);
+        } else {
+          bxv = 0.25 * (A(o->bx, ix, iy, iz) + A(o->bx, ix, iyp, iz) + A(o->bx, ix, iy, izp) + A(o->bx, ix, iyp, izp));
+          byv = 0.25 * (A(o->by, ix, iy, iz) + A(o->by, ixp, iy, iz) + A(o->by, ix, iy, izp) + A(o->by, ixp, iy, izp));
+          bzv = 0.25 * (A(o->bz, ix, iy, iz) + A(o->bz, ix, iyp, iz) + A(o->bz, ixp, iy, iz) + A(o->bz, ixp, iyp, iz));
+          vertex_j(o, ix, iy, iz, &jx, &jy, &jz);
+        }
         const double gb = A(o->gamma_boris, ix, iy, iz);
         fx = fx + gb * (jy * bzv - jz * byv);
         fy = fy + gb * (jz * bxv - jx * bzv);
@@ -1430,10 +1546,12 @@ static void predictor_corrector_step(orc_state *o) {
       }
     }
   }
-  copy_all(o, o->bx, bx0);
-  copy_all(o, o->by, by0);
-  copy_all(o, o->bz, bz0);
-  free(bx0); free(by0); free(bz0);
+  if (!cauchy) {
+    copy_all(o, o->bx, bx0);
+    copy_all(o, o->by, by0);
+    copy_all(o, o->bz, bz0);
+    free(bx0); free(by0); free(bz0);
+  }
 
   remap_v_bcs(o);
   shock_heating(o);
