@@ -89,6 +89,8 @@ typedef struct lare_params {
   int32_t cauchy;          /* the CAUCHY predictor: cell-centred B advanced by
                               the Cauchy solution instead of the predictor
                               remap (lagran.F90:285-348, :945-1048) */
+  int32_t fourth_order;    /* FOURTHORDER: resistive substeps by classical
+                              fourth-order Runge-Kutta (lagran.F90:1329-1415) */
 } lare_params;
 
 /* Field pointers (Fortran arrays, see the array convention above).

@@ -35,7 +35,7 @@ class LareParams(C.Structure):
                 ("alpha_av", C.c_double), ("va_max", C.c_double),
                 ("cooling_term", C.c_int32), ("boris", C.c_int32),
                 ("shock_limiter", C.c_int32), ("shock_compression", C.c_int32), ("rke", C.c_int32),
-                ("cauchy", C.c_int32)]
+                ("cauchy", C.c_int32), ("fourth_order", C.c_int32)]
 
 
 class LareStepInfo(C.Structure):

@@ -43,6 +43,7 @@ def default_params(**kw):
     p.shock_compression = 0
     p.rke = 0
     p.cauchy = 0
+    p.fourth_order = 0
     for k, v in kw.items():
         setattr(p, k, v)
     return p

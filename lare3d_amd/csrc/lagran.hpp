@@ -662,6 +662,94 @@ __global__ __launch_bounds__(BX * BY) void k_energy_max(Geo g, Box b, real *__re
   energy[c] = max0(energy[c]);
 }
 
+// FOURTHORDER resistive substep (lagran.F90:1329-1415): four rksteps, each
+// followed by a bstep from the substep's starting field B1.  k_rk4_emf is
+// rkstep (:1472-1503) at nodes 0..n from the current B and this substep's
+// eta: the EMFs and curlb of the stage go to k, and the weighted sum
+// k1 + 2 k2 + 2 k3 + k4 builds up in ks in the reference's order (w = 0
+// starts it with k1, then w = 2, 2, 1).
+struct RkK {
+  real *k[4];  // kx, ky, kz, curlb
+};
+__global__ __launch_bounds__(BX * BY) void k_rk4_emf(Geo g, Box b, const double *__restrict__ bx,
+                                                     const double *__restrict__ by, const double *__restrict__ bz,
+                                                     const real *__restrict__ eta, RkK k, RkK ks, int w) {
+  BOX_INDEX(b, i, j, kk);
+  const int Y = g.sy, Z = g.sz;
+  const int c = g.idx(i, j, kk);
+  const BPlane lo = bplane(bx, by, c, Y), hi = bplane(bx, by, c + Z, Y);
+  acc jx, jy, jz;
+  vertex_current_acc<acc>(g.dxc[i], g.dyc[j], g.dzc[kk], lo, hi, bz[c], bz[c + 1], bz[c + Y], bz[c + 1 + Y], jx, jy,
+                          jz);
+  const acc e = eta[c];
+  const real v[4] = {-jx * e * (acc)g.dxc[i] * (acc)0.5, -jy * e * (acc)g.dyc[j] * (acc)0.5,
+                     -jz * e * (acc)g.dzc[kk] * (acc)0.5, e * (jx * jx + jy * jy + jz * jz)};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    k.k[q][c] = v[q];
+    ks.k[q][c] = w == 0 ? v[q] : (real)((acc)ks.k[q][c] + (acc)w * (acc)v[q]);
+  }
+}
+
+// bstep(kx, ky, kz, h) (:1582-1631) of the EMFs k from B1: bx on x = 0..n,
+// y, z = 1..n, and likewise by and bz; the other faces are left to the
+// bfield_bcs that follows.  B accumulates in FP64.
+__global__ __launch_bounds__(BX * BY) void k_rk4_bstep(Geo g, Box b, const double *__restrict__ b1x,
+                                                       const double *__restrict__ b1y,
+                                                       const double *__restrict__ b1z, RkK k, double *__restrict__ bx,
+                                                       double *__restrict__ by, double *__restrict__ bz, double h) {
+  BOX_INDEX(b, i, j, kk);
+  const int X = 1, Y = g.sy, Z = g.sz;
+  const int c = g.idx(i, j, kk);
+  const real *kx = k.k[0], *ky = k.k[1], *kz = k.k[2];
+  if (j >= 1 && kk >= 1) {
+    const double area = g.dyb[j] * g.dzb[kk];
+    bx[c] = b1x[c] + ((double)kz[c] - kz[c - Y] + kz[c - Z] - kz[c - Y - Z] - ky[c] + ky[c - Z] - ky[c - Y] +
+                      ky[c - Y - Z]) * h / area;
+  }
+  if (i >= 1 && kk >= 1) {
+    const double area = g.dxb[i] * g.dzb[kk];
+    by[c] = b1y[c] + ((double)kx[c] - kx[c - Z] + kx[c - X] - kx[c - X - Z] - kz[c] + kz[c - X] - kz[c - Z] +
+                      kz[c - X - Z]) * h / area;
+  }
+  if (i >= 1 && j >= 1) {
+    const double area = g.dxb[i] * g.dyb[j];
+    bz[c] = b1z[c] + ((double)ky[c] - ky[c - X] + ky[c - Y] - ky[c - X - Y] - kx[c] + kx[c - Y] - kx[c - X] +
+                      kx[c - X - Y]) * h / area;
+  }
+}
+
+// the heating of the weighted curlb cs on cells 1..n (:1403-1413), with no
+// ohmic_dep, cooling or floor in this variant
+__global__ __launch_bounds__(BX * BY) void k_rk4_heat(Geo g, Box b, const real *__restrict__ cs,
+                                                      const real *__restrict__ rho, real *__restrict__ energy,
+                                                      double dt6) {
+  BOX_INDEX(b, i, j, k);
+  const int X = 1, Y = g.sy, Z = g.sz;
+  const int c = g.idx(i, j, k);
+  const acc sq = (acc)cs[c] + cs[c - X] + cs[c - Y] + cs[c - X - Y] + cs[c - Z] + cs[c - X - Z] + cs[c - Y - Z] +
+                 cs[c - X - Y - Z];
+  energy[c] = energy[c] + sq * (acc)dt6 / ((acc)8.0 * rho[c]);
+}
+
+// total_ohmic_heating increments dt dxc dyc dzc curlb over nodes 0..n,
+// halved on each boundary plane (:1309-1328); block partial sums
+__global__ __launch_bounds__(BX * BY) void k_ohmic_total(Geo g, Box b, const real *__restrict__ cb,
+                                                         double *__restrict__ partial, double dt) {
+  PLANE_POS(b, i, j)
+  const int k = b.z0 + (int)blockIdx.z;
+  double v[1] = {0.0};
+  if (j <= b.y1) {
+    double w1 = dt * g.dxc[i] * g.dyc[j] * g.dzc[k] * (double)cb[g.idx(i, j, k)];
+    if (i == 0 || i == g.nx) w1 = w1 * 0.5;
+    if (j == 0 || j == g.ny) w1 = w1 * 0.5;
+    if (k == 0 || k == g.nz) w1 = w1 * 0.5;
+    v[0] = w1;
+  }
+  const int ops[1] = {1};
+  block_reduce_store<1>(v, ops, partial);
+}
+
 // The ohmic part of the cooling_term after a resistive substep
 // (lagran.F90:1290-1302) on cells 1..n, then energy = MAX(energy, 0).  The
 // reference's loop sets no ixm or iym, which keep the last values of the
@@ -680,15 +768,146 @@ __global__ __launch_bounds__(BX * BY) void k_cool_b(Geo g, Box b, const real *__
   energy[c] = max0(energy[c] - ct);
 }
 
+// b_field_and_cv1_update with CAUCHY (lagran.F90:945-1048) over the cell box
+// -1..n+2: cv1, and the cell-centred B1 (the average of each component's two
+// faces, :93-95) advanced by the Cauchy solution
+// B1 <- (B1 + (B1 . grad) v dt/2) / (1 + div v dt/2), stored times cv1
+// (:230-232) in b1x..b1z; on cells 0..n+1 also the predictor energy and
+// pressure (:234-249), from the start-of-step pressure p0.
+__global__ __launch_bounds__(BX * BY) void k_cauchy_b(Geo g, Box b, const real *__restrict__ vx,
+                                                      const real *__restrict__ vy, const real *__restrict__ vz,
+                                                      const double *__restrict__ bx, const double *__restrict__ by,
+                                                      const double *__restrict__ bz, real *__restrict__ cv1,
+                                                      real *__restrict__ b1x, real *__restrict__ b1y,
+                                                      real *__restrict__ b1z, const real *__restrict__ energy,
+                                                      const real *__restrict__ rho,
+                                                      const real *__restrict__ visc_heat,
+                                                      const real *__restrict__ p0, real *__restrict__ pp,
+                                                      double gamma, double dt2) {
+  BOX_INDEX(b, i, j, k);
+  const int X = 1, Y = g.sy, Z = g.sz;
+  const int c = g.idx(i, j, k);
+  // node velocity at offset (di, dj, dk) in {-1, 0} from node (i, j, k)
+  auto at = [&](const real *__restrict__ v, int di, int dj, int dk) -> acc { return v[c + di * X + dj * Y + dk * Z]; };
+  const acc q4 = 0.25, h = (acc)dt2;
+  const acc dxb = (acc)g.dxb[i], dyb = (acc)g.dyb[j], dzb = (acc)g.dzb[k];
+  // normal derivatives at the faces of B (:958-984)
+  acc vxb = (at(vx, 0, 0, 0) + at(vx, 0, -1, 0) + at(vx, 0, 0, -1) + at(vx, 0, -1, -1)) * q4;
+  acc vxbm = (at(vx, -1, 0, 0) + at(vx, -1, -1, 0) + at(vx, -1, 0, -1) + at(vx, -1, -1, -1)) * q4;
+  acc vyb = (at(vy, 0, 0, 0) + at(vy, -1, 0, 0) + at(vy, 0, 0, -1) + at(vy, -1, 0, -1)) * q4;
+  acc vybm = (at(vy, 0, -1, 0) + at(vy, -1, -1, 0) + at(vy, 0, -1, -1) + at(vy, -1, -1, -1)) * q4;
+  acc vzb = (at(vz, 0, 0, 0) + at(vz, -1, 0, 0) + at(vz, 0, -1, 0) + at(vz, -1, -1, 0)) * q4;
+  acc vzbm = (at(vz, 0, 0, -1) + at(vz, -1, 0, -1) + at(vz, 0, -1, -1) + at(vz, -1, -1, -1)) * q4;
+  const acc dvxdx = (vxb - vxbm) / dxb;
+  const acc dvydy = (vyb - vybm) / dyb;
+  const acc dvzdz = (vzb - vzbm) / dzb;
+  const acc dv = (dvxdx + dvydy + dvzdz) * h;
+  const acc cv = (acc)g.cv(i, j, k);
+  const real c1 = cv * ((acc)1.0 + dv);
+  cv1[c] = c1;
+  // transverse derivatives (:989-1033)
+  vxb = (at(vx, 0, 0, 0) + at(vx, -1, 0, 0) + at(vx, 0, 0, -1) + at(vx, -1, 0, -1)) * q4;
+  vxbm = (at(vx, 0, -1, 0) + at(vx, -1, -1, 0) + at(vx, 0, -1, -1) + at(vx, -1, -1, -1)) * q4;
+  vyb = (at(vy, 0, 0, 0) + at(vy, 0, -1, 0) + at(vy, 0, 0, -1) + at(vy, 0, -1, -1)) * q4;
+  vybm = (at(vy, -1, 0, 0) + at(vy, -1, -1, 0) + at(vy, -1, 0, -1) + at(vy, -1, -1, -1)) * q4;
+  const acc dvxdy = (vxb - vxbm) / dyb;
+  const acc dvydx = (vyb - vybm) / dxb;
+  vxb = (at(vx, 0, 0, 0) + at(vx, -1, 0, 0) + at(vx, 0, -1, 0) + at(vx, -1, -1, 0)) * q4;
+  vxbm = (at(vx, 0, 0, -1) + at(vx, -1, 0, -1) + at(vx, 0, -1, -1) + at(vx, -1, -1, -1)) * q4;
+  acc vzb2 = (at(vz, 0, 0, 0) + at(vz, 0, -1, 0) + at(vz, 0, 0, -1) + at(vz, 0, -1, -1)) * q4;
+  acc vzbm2 = (at(vz, -1, 0, 0) + at(vz, -1, -1, 0) + at(vz, -1, 0, -1) + at(vz, -1, -1, -1)) * q4;
+  const acc dvxdz = (vxb - vxbm) / dzb;
+  const acc dvzdx = (vzb2 - vzbm2) / dxb;
+  vyb = (at(vy, 0, 0, 0) + at(vy, -1, 0, 0) + at(vy, 0, -1, 0) + at(vy, -1, -1, 0)) * q4;
+  vybm = (at(vy, 0, 0, -1) + at(vy, -1, 0, -1) + at(vy, 0, -1, -1) + at(vy, -1, -1, -1)) * q4;
+  vzb2 = (at(vz, 0, 0, 0) + at(vz, -1, 0, 0) + at(vz, 0, 0, -1) + at(vz, -1, 0, -1)) * q4;
+  vzbm2 = (at(vz, 0, -1, 0) + at(vz, -1, -1, 0) + at(vz, 0, -1, -1) + at(vz, -1, -1, -1)) * q4;
+  const acc dvydz = (vyb - vybm) / dzb;
+  const acc dvzdy = (vzb2 - vzbm2) / dyb;
+  const acc bx1 = (bx[c] + bx[c - X]) * 0.5, by1 = (by[c] + by[c - Y]) * 0.5, bz1 = (bz[c] + bz[c - Z]) * 0.5;
+  const acc w3 = bx1 * dvxdx + by1 * dvxdy + bz1 * dvxdz;
+  const acc w4 = bx1 * dvydx + by1 * dvydy + bz1 * dvydz;
+  const acc w5 = bx1 * dvzdx + by1 * dvzdy + bz1 * dvzdz;
+  const real nx = (bx1 + w3 * h) / ((acc)1.0 + dv);
+  const real ny = (by1 + w4 * h) / ((acc)1.0 + dv);
+  const real nz = (bz1 + w5 * h) / ((acc)1.0 + dv);
+  b1x[c] = (acc)nx * (acc)c1;
+  b1y[c] = (acc)ny * (acc)c1;
+  b1z[c] = (acc)nz * (acc)c1;
+  if (i >= 0 && i <= g.nx + 1 && j >= 0 && j <= g.ny + 1 && k >= 0 && k <= g.nz + 1) {
+    // predictor energy and pressure (lagran.F90:234-249)
+    const acc r = rho[c];
+    const acc dvv = c1 / cv - (acc)1.0;
+    acc e1 = energy[c] - p0[c] * dvv / r;
+    e1 = e1 + visc_heat[c] * h / r;
+    pp[c] = (e1 - (acc)0.0) * ((acc)gamma - (acc)1.0) * r * cv / c1;
+  }
+}
+
+// B1 cv1 and cv1 of the four cells (i, j), (i+1, j), (i, j+1), (i+1, j+1)
+// of one plane (CAUCHY predictor)
+struct CPlane {
+  acc v[4], x[4], y[4], z[4];
+};
+__device__ __forceinline__ CPlane cplane(const real *__restrict__ cv1, const real *__restrict__ b1x,
+                                         const real *__restrict__ b1y, const real *__restrict__ b1z, int c, int Y) {
+  CPlane p;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int o = c + (q & 1) + (q >> 1) * Y;
+    p.v[q] = cv1[o];
+    p.x[q] = b1x[o];
+    p.y[q] = b1y[o];
+    p.z[q] = b1z[o];
+  }
+  return p;
+}
+// J and B at node (i, j, k) of the CAUCHY predictor (lagran.F90:285-348)
+// from the cells of planes k (lo) and k+1 (hi), in the reference's order
+__device__ __forceinline__ void cauchy_jb(acc dxc, acc dyc, acc dzc, const CPlane &lo, const CPlane &hi, acc &jx,
+                                          acc &jy, acc &jz, acc &bxv, acc &byv, acc &bzv) {
+  const acc cvx = lo.v[0] + lo.v[2] + hi.v[0] + hi.v[2];
+  const acc cvxp = lo.v[1] + lo.v[3] + hi.v[1] + hi.v[3];
+  const acc cvy = lo.v[0] + lo.v[1] + hi.v[0] + hi.v[1];
+  const acc cvyp = lo.v[2] + lo.v[3] + hi.v[2] + hi.v[3];
+  const acc cvz = lo.v[0] + lo.v[1] + lo.v[2] + lo.v[3];
+  const acc cvzp = hi.v[0] + hi.v[1] + hi.v[2] + hi.v[3];
+  acc w1 = (lo.z[0] + lo.z[1] + hi.z[0] + hi.z[1]) / cvy;
+  acc w2 = (lo.z[2] + lo.z[3] + hi.z[2] + hi.z[3]) / cvyp;
+  jx = (w2 - w1) / dyc;
+  w1 = (lo.y[0] + lo.y[1] + lo.y[2] + lo.y[3]) / cvz;
+  w2 = (hi.y[0] + hi.y[1] + hi.y[2] + hi.y[3]) / cvzp;
+  jx = jx - (w2 - w1) / dzc;
+  w1 = (lo.z[0] + lo.z[2] + hi.z[0] + hi.z[2]) / cvx;
+  w2 = (lo.z[1] + lo.z[3] + hi.z[1] + hi.z[3]) / cvxp;
+  jy = -(w2 - w1) / dxc;
+  w1 = (lo.x[0] + lo.x[1] + lo.x[2] + lo.x[3]) / cvz;
+  w2 = (hi.x[0] + hi.x[1] + hi.x[2] + hi.x[3]) / cvzp;
+  jy = jy + (w2 - w1) / dzc;
+  w1 = (lo.y[0] + lo.y[2] + hi.y[0] + hi.y[2]) / cvx;
+  w2 = (lo.y[1] + lo.y[3] + hi.y[1] + hi.y[3]) / cvxp;
+  jz = (w2 - w1) / dxc;
+  w1 = (lo.x[0] + lo.x[1] + hi.x[0] + hi.x[1]) / cvy;
+  w2 = (lo.x[2] + lo.x[3] + hi.x[2] + hi.x[3]) / cvyp;
+  jz = jz - (w2 - w1) / dyc;
+  bxv = (lo.x[0] + lo.x[1] + lo.x[2] + lo.x[3] + hi.x[0] + hi.x[1] + hi.x[2] + hi.x[3]) / (cvx + cvxp);
+  byv = (lo.y[0] + lo.y[1] + lo.y[2] + lo.y[3] + hi.y[0] + hi.y[1] + hi.y[2] + hi.y[3]) / (cvx + cvxp);
+  bzv = (lo.z[0] + lo.z[1] + lo.z[2] + lo.z[3] + hi.z[0] + hi.z[1] + hi.z[2] + hi.z[3]) / (cvx + cvxp);
+}
+
 // predictor momentum (lagran.F90:251-387) with fx/fy/fz_visc (:745-789)
-// recomputed here from alpha and v; B is the half-step (predictor-remapped)
-// field.
+// recomputed here from alpha and v.  Without CAUCHY, B is the half-step
+// (predictor-remapped) face field; with CAUCHY, J and B come from the
+// cell-centred B1 cv1 (b1x..b1z) and cv1 of k_cauchy_b.
+template <bool CAUCHY>
 __global__ __launch_bounds__(BX * BY) void k_force(Geo g, Box b, const real *__restrict__ vx, const real *__restrict__ vy,
                         const real *__restrict__ vz, const real *__restrict__ pp,
                         const double *__restrict__ bx, const double *__restrict__ by, const double *__restrict__ bz,
                         const real *__restrict__ rho_v, const real *__restrict__ al1,
                         const real *__restrict__ al2, const real *__restrict__ al3, const real *__restrict__ gb,
-                        real *__restrict__ vx1, real *__restrict__ vy1, real *__restrict__ vz1, double dt2) {
+                        real *__restrict__ vx1, real *__restrict__ vy1, real *__restrict__ vz1, double dt2,
+                        const real *__restrict__ cv1, const real *__restrict__ b1x, const real *__restrict__ b1y,
+                        const real *__restrict__ b1z) {
   // gb: gamma_boris of this step's set_dt (lagran.F90:375-377), NULL for 1
   ZCOLUMN(b, i, j, k0, k1);
   const int X = 1, Y = g.sy, Z = g.sz;
@@ -703,7 +922,10 @@ __global__ __launch_bounds__(BX * BY) void k_force(Geo g, Box b, const real *__r
     u0[q] = V[q][cb];
   }
   Quad plo = quad_fwd(pp, cb, Y);
-  BPlane blo = bplane(bx, by, cb, Y);
+  BPlane blo = {};
+  CPlane clo = {};
+  if (CAUCHY) clo = cplane(cv1, b1x, b1y, b1z, cb, Y);
+  else blo = bplane(bx, by, cb, Y);
   real a3lo = al3[cb];
   for (int k = k0; k <= k1; ++k) {
     const int c = g.idx(i, j, k);
@@ -712,8 +934,10 @@ __global__ __launch_bounds__(BX * BY) void k_force(Geo g, Box b, const real *__r
 #pragma unroll
     for (int q = 0; q < 3; ++q) up[q] = V[q][c + Z];
     const Quad phi = quad_fwd(pp, c + Z, Y);
-    const BPlane bhi = bplane(bx, by, c + Z, Y);
-    const double bz0 = bz[c], bzx = bz[c + X], bzy = bz[c + Y], bzxy = bz[c + X + Y];
+    BPlane bhi = {};
+    CPlane chi = {};
+    if (CAUCHY) chi = cplane(cv1, b1x, b1y, b1z, c + Z, Y);
+    else bhi = bplane(bx, by, c + Z, Y);
     const real a3hi = al3[c + Z];
     // viscous forces
     const acc a1 = al1[c + Y + Z] * dyc * dzc;
@@ -744,13 +968,18 @@ __global__ __launch_bounds__(BX * BY) void k_force(Geo g, Box b, const real *__r
     w1 = (p + ppx + ppy + ppxy) * q4;
     w2 = (ppz + ppxz + ppyz + ppxyz) * q4;
     acc fz = -(w2 - w1) / dzc;
-    // bx(c) + bx(c+Y) + bx(c+Z) + bx(c+Y+Z), and likewise for by and bz
-    // (the FP64 face field averaged in FP64)
-    const acc bxv = 0.25 * (blo.bx0 + blo.bxy + bhi.bx0 + bhi.bxy);
-    const acc byv = 0.25 * (blo.by0 + blo.byx + bhi.by0 + bhi.byx);
-    const acc bzv = 0.25 * (bz0 + bzy + bzx + bzxy);
-    acc jx, jy, jz;
-    vertex_current_acc<acc>(g.dxc[i], g.dyc[j], g.dzc[k], blo, bhi, bz0, bzx, bzy, bzxy, jx, jy, jz);
+    acc bxv, byv, bzv, jx, jy, jz;
+    if (CAUCHY) {
+      cauchy_jb(dxc, dyc, dzc, clo, chi, jx, jy, jz, bxv, byv, bzv);
+    } else {
+      // bx(c) + bx(c+Y) + bx(c+Z) + bx(c+Y+Z), and likewise for by and bz
+      // (the FP64 face field averaged in FP64)
+      const double bz0 = bz[c], bzx = bz[c + X], bzy = bz[c + Y], bzxy = bz[c + X + Y];
+      bxv = 0.25 * (blo.bx0 + blo.bxy + bhi.bx0 + bhi.bxy);
+      byv = 0.25 * (blo.by0 + blo.byx + bhi.by0 + bhi.byx);
+      bzv = 0.25 * (bz0 + bzy + bzx + bzxy);
+      vertex_current_acc<acc>(g.dxc[i], g.dyc[j], g.dzc[k], blo, bhi, bz0, bzx, bzy, bzxy, jx, jy, jz);
+    }
     const acc g_b = gb ? (acc)gb[c] : (acc)1.0;
     fx = fx + g_b * (jy * bzv - jz * byv);
     fy = fy + g_b * (jz * bxv - jx * bzv);
@@ -768,6 +997,7 @@ __global__ __launch_bounds__(BX * BY) void k_force(Geo g, Box b, const real *__r
     }
     plo = phi;
     blo = bhi;
+    clo = chi;
     a3lo = a3hi;
   }
 }

@@ -68,6 +68,8 @@ struct Dev {
   // rke: delta_ke (cells) and one sweep's edge terms per velocity component
   real *delta_ke = nullptr;
   RkeOut rk = {};
+  // FOURTHORDER: one stage's EMFs and curlb, and their weighted sum
+  RkK rk4k = {}, rk4s = {};
   // reductions
   double *partial = nullptr;
   size_t partial_cap = 0;
@@ -663,6 +665,39 @@ int set_dt_and_stages(double dt_local, double dtr_local) {
 int sur_enabled();
 int sur_conduct_step(int delta_mode, double *res_out, int *acc_out);
 
+// One FOURTHORDER resistive substep (lagran.F90:146-149 with :1329-1415):
+// eta_calc; B1 = B (in the spare fxr..fzr arrays); rkstep and bstep four
+// times; the final bstep with the weighted EMFs; the heating; energy_bcs.
+int resist_rk4_substep() {
+  int rc = eta_calc();
+  if (rc) return rc;
+  const size_t nb = sizeof(double) * S.ntot;
+  HIPCHK(hipMemcpyAsync(S.fxr, S.bx, nb, hipMemcpyDeviceToDevice, S.stream));
+  HIPCHK(hipMemcpyAsync(S.fyr, S.by, nb, hipMemcpyDeviceToDevice, S.stream));
+  HIPCHK(hipMemcpyAsync(S.fzr, S.bz, nb, hipMemcpyDeviceToDevice, S.stream));
+  const Box n0 = nodes0n();
+  const double dt = S.dt, half_dt = dt * 0.5, dt6 = dt * (1.0 / 6.0);
+  const double h[3] = {half_dt, half_dt, dt};
+  const int w[4] = {0, 2, 2, 1};
+  for (int q = 0; q < 4; ++q) {
+    if (q > 0) {
+      LAUNCH_BOX("rk4_bstep", k_rk4_bstep, n0, S.fxr, S.fyr, S.fzr, S.rk4k, S.bx, S.by, S.bz, h[q - 1]);
+      rc = bfield_bcs(S.bx, S.by, S.bz);
+      if (rc) return rc;
+    }
+    LAUNCH_BOX("rk4_emf", k_rk4_emf, n0, S.bx, S.by, S.bz, S.eta, S.rk4k, S.rk4s, w[q]);
+  }
+  LAUNCH_BOX("rk4_bstep", k_rk4_bstep, n0, S.fxr, S.fyr, S.fzr, S.rk4s, S.bx, S.by, S.bz, dt6);
+  rc = bfield_bcs(S.bx, S.by, S.bz);
+  if (rc) return rc;
+  LAUNCH_BOX("rk4_heat", k_rk4_heat, interior(), S.rk4s.k[3], S.rho, S.energy, dt6);
+  rc = energy_bcs();
+  if (rc) return rc;
+  const dim3 gr = box_grid(n0);
+  LAUNCH("ohmic_total", k_ohmic_total, gr, box_block(), S.g, n0, S.rk4s.k[3], S.partial, dt6);
+  return final_reduce(gr, 1, 0, 9, 1);
+}
+
 int lagrangian_step() {
   const int N0 = nx(), N1 = ny(), N2 = nz();
   LagArgs a = lag_args();  // dt = previous step's dt (Q1)
@@ -725,6 +760,11 @@ int lagrangian_step() {
     if ((size_t)g2.x * g2.y * g2.z > S.partial_cap)
       return fail(LARE_E_ARG, "resistive grid of %u blocks exceeds the partial buffer", g2.x * g2.z);
     for (int sc = 1; sc <= substeps; ++sc) {
+      if (S.p.fourth_order) {
+        rc = resist_rk4_substep();
+        if (rc) return rc;
+        continue;
+      }
       LAUNCH("resist", k_resist, g2, dim3(64, RS_ROWS, 1), S.g, fb, tiles_x, S.bx, S.by, S.bz, S.fxr, S.fyr, S.fzr,
              S.eta, S.energy, S.ohmic_dep, S.rho, S.curlb, S.partial, S.dt, S.p.eta_background, S.p.eta0,
              S.p.j_max);
@@ -753,24 +793,36 @@ int lagrangian_step() {
       rc = heat_conduct_sts2(1);
       if (rc) return rc;
     }
-  } else if (!S.p.resistive_mhd) {
+  } else if (!S.p.resistive_mhd || S.p.fourth_order) {
     // energy = MAX(energy0 + 0, 0) (lagran.F90:180-181).  After a
-    // resistive step it changes nothing: the last substep's ohmic heating
-    // already clamped the whole cell box (:1305) and energy_bcs only copies
-    // clamped values, so the launch is left out then.
+    // first-order resistive step it changes nothing: the last substep's ohmic
+    // heating already clamped the whole cell box (:1305) and energy_bcs only
+    // copies clamped values, so the launch is left out then.
     LAUNCH_BOX("energy_max", k_energy_max, cell_box(), S.energy);
   }
 
   // predictor-corrector (lagran.F90:208-478)
-  const double dt_full = S.dt;
-  S.dt = 0.5 * dt_full;
-  rc = eulerian_remap(S.step, true);
-  S.dt = dt_full;
-  if (rc) return rc;
-  // the predictor energy and pressure were formed in the predictor
-  // remap's last geometry pass
-  LAUNCH("force", k_force, zchunk_grid(nodes0n()), box_block(), S.g, nodes0n(), S.vx, S.vy, S.vz, S.pp, S.bpx, S.bpy,
-         S.bpz, S.rho_v, S.al1, S.al2, S.al3, S.gamma_boris, S.vx1, S.vy1, S.vz1, S.dt2);
+  if (S.p.cauchy) {
+    // the Cauchy solution for the cell-centred B1 (B1 cv1 in the spare
+    // predictor-B arrays), cv1 and the predictor pressure; no predictor remap
+    real *b1[3] = {(real *)S.bpx, (real *)S.bpy, (real *)S.bpz};
+    LAUNCH_BOX("cauchy_b", k_cauchy_b, cell_box(), S.vx, S.vy, S.vz, S.bx, S.by, S.bz, S.cv1, b1[0], b1[1], b1[2],
+               S.energy, S.rho, S.visc_heat, S.p0, S.pp, S.p.gamma, S.dt2);
+    LAUNCH("force", k_force<true>, zchunk_grid(nodes0n()), box_block(), S.g, nodes0n(), S.vx, S.vy, S.vz, S.pp,
+           S.bx, S.by, S.bz, S.rho_v, S.al1, S.al2, S.al3, S.gamma_boris, S.vx1, S.vy1, S.vz1, S.dt2, S.cv1, b1[0],
+           b1[1], b1[2]);
+  } else {
+    const double dt_full = S.dt;
+    S.dt = 0.5 * dt_full;
+    rc = eulerian_remap(S.step, true);
+    S.dt = dt_full;
+    if (rc) return rc;
+    // the predictor energy and pressure were formed in the predictor
+    // remap's last geometry pass
+    LAUNCH("force", k_force<false>, zchunk_grid(nodes0n()), box_block(), S.g, nodes0n(), S.vx, S.vy, S.vz, S.pp,
+           S.bpx, S.bpy, S.bpz, S.rho_v, S.al1, S.al2, S.al3, S.gamma_boris, S.vx1, S.vy1, S.vz1, S.dt2, S.cv1,
+           nullptr, nullptr, nullptr);
+  }
   {
     BcField f[3] = {{S.vx1, FK_NODE}, {S.vy1, FK_NODE}, {S.vz1, FK_NODE}};
     rc = apply_bcs(f, 3, "bc_v1");
@@ -954,6 +1006,14 @@ int api_init(const lare_grid *grid, const lare_params *params) {
   S.gamma_boris = S.cool_v = S.cool_b = S.curlb = nullptr;
   S.delta_ke = nullptr;
   S.rk = RkeOut{};
+  S.rk4k = RkK{};
+  S.rk4s = RkK{};
+  if (params->fourth_order && params->resistive_mhd) {
+    int rc = 0;
+    for (int q = 0; q < 4 && !rc; ++q) rc = dalloc(&S.rk4k.k[q]);
+    for (int q = 0; q < 4 && !rc; ++q) rc = dalloc(&S.rk4s.k[q]);
+    if (rc) return rc;
+  }
   if (params->rke) {
     int rc = dalloc(&S.delta_ke);
     for (int q = 0; q < 3 && !rc; ++q) rc = dalloc(&S.rk.dk[q]);

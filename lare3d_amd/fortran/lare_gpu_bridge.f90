@@ -25,7 +25,7 @@ MODULE lare_gpu_bridge
     INTEGER(C_INT32_T) :: resistive_mhd, conduction, heat_flux_limiter
     INTEGER(C_INT32_T) :: use_viscous_damping, restart, precision
     REAL(C_DOUBLE) :: alpha_av, va_max
-    INTEGER(C_INT32_T) :: cooling_term, boris, shock_limiter, shock_compression, rke, cauchy
+    INTEGER(C_INT32_T) :: cooling_term, boris, shock_limiter, shock_compression, rke, cauchy, fourth_order
   END TYPE lare_params
 
   TYPE, BIND(C) :: lare_fields

@@ -92,7 +92,7 @@ PROGRAM ot_driver
   params%use_viscous_damping = 0; params%restart = 0; params%precision = LARE_PREC_FP64
   params%alpha_av = 0.05_C_DOUBLE; params%va_max = 10.0_C_DOUBLE
   params%cooling_term = 0; params%boris = 0; params%shock_limiter = 0; params%shock_compression = 0
-  params%rke = 0; params%cauchy = 0
+  params%rke = 0; params%cauchy = 0; params%fourth_order = 0
 
   fields%rho = C_LOC(rho); fields%energy = C_LOC(energy)
   fields%vx = C_LOC(vx); fields%vy = C_LOC(vy); fields%vz = C_LOC(vz)

@@ -979,17 +979,10 @@ static inline void vertex_j(orc_state *o, int ix, int iy, int iz, double *jx, do
 
 /* lagran.F90:1239-1452 resistive_effects (first order, no cowling) with
  * rkstep :1460-1503 and bstep :1582-1635. */
-static void resistive_effects(orc_state *o, double *fxr, double *fyr, double *fzr, double *curlb) {
+/* rkstep (:1472-1503, no Cowling resistivity): EMFs and curlb at nodes 0..n
+ * from the current B and eta */
+static void rkstep(orc_state *o, double *fxr, double *fyr, double *fzr, double *curlb) {
   int nx = o->nx, ny = o->ny, nz = o->nz;
-  double dt = o->dt;
-  for (int iz = -1; iz <= nz + 2; ++iz)
-    for (int iy = -1; iy <= ny + 2; ++iy)
-      for (int ix = -1; ix <= nx + 2; ++ix) {
-        A(o->bx1, ix, iy, iz) = A(o->bx, ix, iy, iz);
-        A(o->by1, ix, iy, iz) = A(o->by, ix, iy, iz);
-        A(o->bz1, ix, iy, iz) = A(o->bz, ix, iy, iz);
-      }
-  /* rkstep */
   for (int iz = 0; iz <= nz; ++iz)
     for (int iy = 0; iy <= ny; ++iy)
       for (int ix = 0; ix <= nx; ++ix) {
@@ -1001,7 +994,12 @@ static void resistive_effects(orc_state *o, double *fxr, double *fyr, double *fz
         A(fzr, ix, iy, iz) = -jz * e * DZC(iz) * 0.5;
         A(curlb, ix, iy, iz) = e * (jx * jx + jy * jy + jz * jz);
       }
-  /* bstep(flux_x, flux_y, flux_z, dt) */
+}
+
+/* bstep(kx, ky, kz, dt) (:1582-1635): B = B1 + curl of the EMFs, then
+ * bfield_bcs */
+static void bstep(orc_state *o, const double *fxr, const double *fyr, const double *fzr, double dt) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
   for (int iz = 1; iz <= nz; ++iz) {
     int izm = iz - 1;
     for (int iy = 1; iy <= ny; ++iy) {
@@ -1036,6 +1034,85 @@ static void resistive_effects(orc_state *o, double *fxr, double *fyr, double *fz
       }
     }
   bfield_bcs(o);
+}
+
+/* total_ohmic_heating increments of curlb c over nodes 0..n (:1309-1328) */
+static void ohmic_total(orc_state *o, const double *c, double dt) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  for (int iz = 0; iz <= nz; ++iz)
+    for (int iy = 0; iy <= ny; ++iy)
+      for (int ix = 0; ix <= nx; ++ix) {
+        double w1 = dt * DXC(ix) * DYC(iy) * DZC(iz) * A(c, ix, iy, iz);
+        if (ix == 0 || ix == nx) w1 = w1 * 0.5;
+        if (iy == 0 || iy == ny) w1 = w1 * 0.5;
+        if (iz == 0 || iz == nz) w1 = w1 * 0.5;
+        o->toh = o->toh + w1;
+      }
+}
+
+/* the fourth-order Runge-Kutta substep of FOURTHORDER (:1329-1415): four
+ * rksteps at B1, B1 + dt/2 k1, B1 + dt/2 k2 and B1 + dt k3; B = B1 + dt/6
+ * (k1 + 2 k2 + 2 k3 + k4), and the heating of the weighted curlb.  No
+ * ohmic_dep, cooling or energy floor in this variant. */
+static void resistive_rk4(orc_state *o, double *fxr, double *fyr, double *fzr, double *curlb) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  const double dt = o->dt, half_dt = dt * 0.5, dt6 = dt * (1.0 / 6.0);
+  double *k[4][4];
+  for (int q = 0; q < 4; ++q)
+    for (int c = 0; c < 4; ++c) k[q][c] = box(o, 0, nx, 0, ny, 0, nz);
+  const double h[3] = {half_dt, half_dt, dt};
+  for (int q = 0; q < 4; ++q) {
+    if (q > 0) {
+      bstep(o, k[q - 1][0], k[q - 1][1], k[q - 1][2], h[q - 1]);
+      rkstep(o, fxr, fyr, fzr, curlb);
+    }
+    copy_all(o, k[q][0], fxr);
+    copy_all(o, k[q][1], fyr);
+    copy_all(o, k[q][2], fzr);
+    copy_all(o, k[q][3], curlb);
+  }
+  for (int c = 0; c < 4; ++c)
+    for (int iz = 0; iz <= nz; ++iz)
+      for (int iy = 0; iy <= ny; ++iy)
+        for (int ix = 0; ix <= nx; ++ix)
+          A(k[0][c], ix, iy, iz) = A(k[0][c], ix, iy, iz) + 2.0 * A(k[1][c], ix, iy, iz) + 2.0 * A(k[2][c], ix, iy, iz) +
+                                   A(k[3][c], ix, iy, iz);
+  bstep(o, k[0][0], k[0][1], k[0][2], dt6);
+  const double *c1 = k[0][3];
+  for (int iz = 1; iz <= nz; ++iz) {
+    int izm = iz - 1;
+    for (int iy = 1; iy <= ny; ++iy) {
+      int iym = iy - 1;
+      for (int ix = 1; ix <= nx; ++ix) {
+        int ixm = ix - 1;
+        A(o->energy, ix, iy, iz) = A(o->energy, ix, iy, iz) +
+            (A(c1, ix, iy, iz) + A(c1, ixm, iy, iz) + A(c1, ix, iym, iz) + A(c1, ixm, iym, iz) + A(c1, ix, iy, izm) +
+             A(c1, ixm, iy, izm) + A(c1, ix, iym, izm) + A(c1, ixm, iym, izm)) * dt6 / (8.0 * A(o->rho, ix, iy, iz));
+      }
+    }
+  }
+  cell_bcs(o, o->energy);
+  ohmic_total(o, c1, dt6);
+  for (int q = 0; q < 4; ++q)
+    for (int c = 0; c < 4; ++c) free(k[q][c]);
+}
+
+static void resistive_effects(orc_state *o, double *fxr, double *fyr, double *fzr, double *curlb) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  double dt = o->dt;
+  for (int iz = -1; iz <= nz + 2; ++iz)
+    for (int iy = -1; iy <= ny + 2; ++iy)
+      for (int ix = -1; ix <= nx + 2; ++ix) {
+        A(o->bx1, ix, iy, iz) = A(o->bx, ix, iy, iz);
+        A(o->by1, ix, iy, iz) = A(o->by, ix, iy, iz);
+        A(o->bz1, ix, iy, iz) = A(o->bz, ix, iy, iz);
+      }
+  rkstep(o, fxr, fyr, fzr, curlb);
+  if (o->p.fourth_order) {
+    resistive_rk4(o, fxr, fyr, fzr, curlb);
+    return;
+  }
+  bstep(o, fxr, fyr, fzr, dt);
   /* ohmic heating :1270-1288 */
   for (int iz = 1; iz <= nz; ++iz) {
     int izm = iz - 1;
@@ -1068,15 +1145,7 @@ static void resistive_effects(orc_state *o, double *fxr, double *fyr, double *fz
     for (int iy = -1; iy <= ny + 2; ++iy)
       for (int ix = -1; ix <= nx + 2; ++ix) A(o->energy, ix, iy, iz) = dmax(A(o->energy, ix, iy, iz), 0.0);
   cell_bcs(o, o->energy);
-  for (int iz = 0; iz <= nz; ++iz)
-    for (int iy = 0; iy <= ny; ++iy)
-      for (int ix = 0; ix <= nx; ++ix) {
-        double w1 = dt * DXC(ix) * DYC(iy) * DZC(iz) * A(curlb, ix, iy, iz);
-        if (ix == 0 || ix == nx) w1 = w1 * 0.5;
-        if (iy == 0 || iy == ny) w1 = w1 * 0.5;
-        if (iz == 0 || iz == nz) w1 = w1 * 0.5;
-        o->toh = o->toh + w1;
-      }
+  ohmic_total(o, curlb, dt);
   /* jx_r/jy_r/jz_r and the final rkstep (:1417-1445) only feed output. */
 }
 
@@ -1661,7 +1730,15 @@ void orc_lagrangian_step(orc_state *o) {
       orc_eta_calc(o);
       resistive_effects(o, fxr, fyr, fzr, curlb);
     }
-    /* bx1.. recomputed at :152-163 are not read again in this configuration */
+    /* bx1.. recomputed from the new B (:152-163); only the CAUCHY predictor
+     * reads them again */
+    for (int iz = -1; iz <= nz + 2; ++iz)
+      for (int iy = -1; iy <= ny + 2; ++iy)
+        for (int ix = -1; ix <= nx + 2; ++ix) {
+          A(o->bx1, ix, iy, iz) = (A(o->bx, ix, iy, iz) + A(o->bx, ix - 1, iy, iz)) * 0.5;
+          A(o->by1, ix, iy, iz) = (A(o->by, ix, iy, iz) + A(o->by, ix, iy - 1, iz)) * 0.5;
+          A(o->bz1, ix, iy, iz) = (A(o->bz, ix, iy, iz) + A(o->bz, ix, iy, iz - 1)) * 0.5;
+        }
     o->dt = actual_dt;
   }
 

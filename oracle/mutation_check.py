@@ -22,7 +22,8 @@ ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "lare_oracle.c")
 CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-shared"]
 PIN_TESTS = ["tests/test_pin_hydro.py", "tests/test_pin_transport.py", "tests/test_pin_oracle3d.py",
-             "tests/test_pin_potential.py", "tests/test_pin_variants.py", "tests/test_pin_rke.py"]
+             "tests/test_pin_potential.py", "tests/test_pin_variants.py", "tests/test_pin_rke.py",
+             "tests/test_pin_cauchy.py"]
 
 # (name, what it breaks, [(old, new), ...]); each `old` occurs exactly once
 MUTANTS = [
@@ -98,6 +99,22 @@ MUTANTS = [
      [("        if (d < 0.0) d = 0.0;\n", "")]),
     ("M31", "rke: node masses before the sweep for after it (xremap.f90:673-674)",
      [("double mc = AT(o->rho_v1, c) * AT(o->cv2, c);", "double mc = AT(o->rho_v, c) * AT(o->cv1, c);")]),
+    ("M32", "CAUCHY: stretching term of By from dvx/dy instead of dvy/dx (lagran.F90:1037)",
+     [("double w4 = b1x * dvydx + b1y * dvydy + b1z * dvydz;", "double w4 = b1x * dvxdy + b1y * dvydy + b1z * dvydz;")]),
+    ("M33", "CAUCHY: By1 without the 1/(1 + div v dt/2) of compression (lagran.F90:1043)",
+     [("A(o->by1, ix, iy, iz) = (b1y + w4 * o->dt2) / (1.0 + dv);", "A(o->by1, ix, iy, iz) = (b1y + w4 * o->dt2);")]),
+    ("M34", "CAUCHY: jy with the sign of its first term flipped (lagran.F90:315)",
+     [("double y = -(w2 - w1) / DXC(ix);", "double y = (w2 - w1) / DXC(ix);")]),
+    ("M35", "CAUCHY: dvz/dx from a stencil one node off in x (lagran.F90:1013-1014)",
+     [("vzbm = (A(vz, ixm, iy, iz) + A(vz, ixm, iym, iz) + A(vz, ixm, iy, izm) + A(vz, ixm, iym, izm)) * 0.25;",
+       "vzbm = (A(vz, ix, iy, iz) + A(vz, ixm, iym, iz) + A(vz, ixm, iy, izm) + A(vz, ixm, iym, izm)) * 0.25;")]),
+    ("M36", "FOURTHORDER: k3 weighted 1 instead of 2 (lagran.F90:1370)",
+     [("2.0 * A(k[2][c], ix, iy, iz)", "1.0 * A(k[2][c], ix, iy, iz)")]),
+    ("M37", "FOURTHORDER: the third stage from B1 + dt k2 instead of dt/2 (lagran.F90:1350)",
+     [("const double h[3] = {half_dt, half_dt, dt};", "const double h[3] = {half_dt, dt, dt};")]),
+    ("M38", "FOURTHORDER: the heating with dt instead of dt/6 (lagran.F90:1388)",
+     [("A(c1, ixm, iy, izm) + A(c1, ix, iym, izm) + A(c1, ixm, iym, izm)) * dt6 /",
+       "A(c1, ixm, iy, izm) + A(c1, ix, iym, izm) + A(c1, ixm, iym, izm)) * dt /")]),
 ]
 
 

@@ -104,6 +104,12 @@ CASES = {
     "harris_rke": lambda: _with(configs.harris(19, ny=13, nz=11), rke=1),
     "wide_rke": lambda: _with(configs.harris(64, ny=9, nz=6), rke=1),
     "ot_rke": lambda: _with(configs.orszag_tang(16), rke=1),
+    # the CAUCHY predictor (cell-centred Cauchy solution for the half-step B)
+    "harris_cauchy": lambda: _with(configs.harris(19, ny=13, nz=11), cauchy=1),
+    "ot_cauchy": lambda: _with(configs.orszag_tang(16), cauchy=1, rke=1),
+    # FOURTHORDER resistive substeps, alone and with conduction
+    "harris_rk4": lambda: _with(configs.harris(19, ny=13, nz=11), fourth_order=1),
+    "kink_rk4": lambda: _with(configs.kink_tube(16), fourth_order=1),
 }
 
 

@@ -116,6 +116,57 @@ def test_resistive_diffusion(backend):
     assert abs(sc.total_ohmic_heating - heat) < 1e-3 * heat
 
 
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_resistive_diffusion_rk4(backend):
+    """FOURTHORDER (lagran.F90:1329-1415): a By mode of the discrete
+    Laplacian's eigenvalue lam is multiplied per substep by the classical
+    Runge-Kutta polynomial R4(z) = 1 + z + z^2/2 + z^3/6 + z^4/24,
+    z = -eta lam dt_sub; its four stages see the amplitudes 1, 1 + z/2,
+    1 + z/2 + z^2/4 and 1 + z + z^2/2 + z^3/4, and the heating is
+    eta lam V / 2 a^2 dt_sub / 6 (s1^2 + 2 s2^2 + 2 s3^2 + s4^2).  The
+    mode has 4 cells per wavelength, so that |z| ~ 0.1 and a first-order
+    step (1 + z) misses by z^2/2 ~ 5e-3."""
+    n, eps, eta, gam = 32, 1e-3, 2e-3, 5.0 / 3.0
+    grid, params, f = line_problem(n, 0, bc_axis=BC_PERIODIC, gamma=gam, resistive_mhd=1,
+                                   eta_background=eta, eta0=0.0, j_max=1e30, visc1=0.0, visc2=0.0)
+    params.fourth_order = 1
+    k = 2.0 * math.pi * 8
+    xc = grid.xc[0]
+    f.by[...] = along(f.by.shape, 0, eps * np.cos(k * xc))
+    f.rho[...] = 1.0
+    f.energy[...] = 1.0 / (gam - 1.0)
+    sim = make(backend, grid, params, f)
+    start(sim)
+    f0 = sim.fields()
+    dt, _ = sim.step(1)
+    sc = sim.scalars()
+    out = sim.fields()
+    close(sim)
+    by = out.interior("by")[:, 0, 0]
+    amp = 2.0 / n * float(np.sum(by * np.cos(k * xc[2:n + 2]))) / eps
+    dx = 1.0 / n
+    # internal energy gained: the heat (the work of the eps^2 forces is
+    # second order in it)
+    cv = dx ** 3
+    gain = float(np.sum(out.interior("rho") * out.interior("energy") - f0.interior("rho") * f0.interior("energy"))) * cv
+    lam = laplacian_eig(k, dx)
+    ns = sc.last_substeps
+    z = -eta * lam * dt / ns
+    r4 = 1.0 + z + z * z / 2.0 + z ** 3 / 6.0 + z ** 4 / 24.0
+    expect = r4 ** ns
+    stages = (1.0, 1.0 + z / 2.0, 1.0 + z / 2.0 + z * z / 4.0, 1.0 + z + z * z / 2.0 + z ** 3 / 4.0)
+    weight = (stages[0] ** 2 + 2.0 * stages[1] ** 2 + 2.0 * stages[2] ** 2 + stages[3] ** 2) / 6.0
+    V = 1.0 * (4 * dx) * (4 * dx)
+    heat = sum(eta * lam * V / 2.0 * (eps * r4 ** q) ** 2 * weight * dt / ns for q in range(ns))
+    print(f"rk4 resistive: substeps {ns}, z {z:.4f}: mode {amp:.10f} expected {expect:.10f} "
+          f"(first order {(1.0 + z) ** ns:.10f}); ohmic {sc.total_ohmic_heating:.6e} expected {heat:.6e}, "
+          f"internal energy gained {gain:.6e}")
+    assert abs(z) > 0.05
+    assert abs(amp - expect) < 1e-3 * abs(expect - (1.0 + z) ** ns)
+    assert abs(sc.total_ohmic_heating - heat) < 1e-3 * heat
+    assert abs(gain - heat) < 1e-4 * heat
+
+
 def test_eta_threshold():
     """eta = eta_bg + eta0 where |J| > j_max (lagran.F90:1218)."""
     n = 8
