@@ -35,6 +35,7 @@ extern "C" {
 /* Boundary codes, as constants.f90 of the reference (shared_data.F90:65-66). */
 #define LARE_BC_PERIODIC 1
 #define LARE_BC_USER 2
+#define LARE_BC_OPEN 3   /* far-field characteristic boundary (openboundary.f90) */
 
 /* Precision modes of the bulk fields. */
 #define LARE_PREC_FP64 0
@@ -47,6 +48,15 @@ extern "C" {
 /* Physics faults (positive). */
 #define LARE_F_NONFINITE 1   /* non-finite dt, or a state with a non-finite
                                 value or rho <= 0 entering a step            */
+#define LARE_F_TBOUNDS 2     /* neutral_fraction's temperature bracket is
+                                inverted in some cell (energy <= 0): the
+                                reference STOPs (neutral.f90:172-175)        */
+
+/* Equations of state (shared_data.F90:73, control.f90:187-199). */
+#define LARE_EOS_IDEAL 1     /* xi_n uniform (lare_params.xi_n); 0 also
+                                means EOS_IDEAL                              */
+#define LARE_EOS_ION 2       /* partially ionised, with ionisation energy    */
+#define LARE_EOS_PI 3        /* partially ionised, ionise_pot = 0            */
 
 /* Grid: sizes, boundary codes and the 1-D metrics of setup.F90:92-247.
  * Replaces the module state nx/ny/nz, x/y/zbc_*, dxb/dxc/dyb/dyc/dzb/dzc
@@ -61,8 +71,11 @@ typedef struct lare_grid {
 } lare_grid;
 
 /* Run parameters of control.f90 (control_variables) and normalise.f90.
- * Only EOS_IDEAL is supported: xi_n is uniform (lare3d.f90:72) and the
- * ionisation potential is 0 (normalise.f90:48). */
+ * With EOS_IDEAL xi_n is the uniform value of lare3d.f90:72 and the
+ * ionisation potential is 0 (normalise.f90:48); EOS_ION and EOS_PI keep a
+ * neutral fraction per cell (neutral.f90:158-195) from the normalised
+ * constants of setup_neutral and normalise_transport (neutral.f90:31-64,
+ * normalise.f90:36-52), which the caller computes once. */
 typedef struct lare_params {
   double gamma;            /* setup.F90:46 default 5/3                    */
   double visc1, visc2;     /* shock viscosity coefficients               */
@@ -91,6 +104,18 @@ typedef struct lare_params {
                               remap (lagran.F90:285-348, :945-1048) */
   int32_t fourth_order;    /* FOURTHORDER: resistive substeps by classical
                               fourth-order Runge-Kutta (lagran.F90:1329-1415) */
+  int32_t eos_number;      /* LARE_EOS_* (control.f90:191)                 */
+  int32_t cowling_resistivity; /* Cowling eta_perp (control.f90:162,
+                              neutral.f90:68-136); needs eos_number
+                              EOS_ION or EOS_PI                           */
+  int32_t pad2_;
+  double ionise_pot;       /* normalised ionisation potential (used with
+                              EOS_ION only, normalise.f90:47-48)           */
+  double f_bar;            /* (2 pi m_e k_B / h^2)^1.5, SI (neutral.f90:50-51) */
+  double t_bar;            /* ionisation temperature / temp_norm (:54, normalise.f90:40) */
+  double r_bar;            /* 4 / mbar, normalised (:57-58, normalise.f90:41) */
+  double eta_bar;          /* Cowling coefficient, normalised (:61, normalise.f90:42-43) */
+  double t_rad;            /* radiation temperature tr / temp_norm (:47, normalise.f90:49) */
 } lare_params;
 
 /* Field pointers (Fortran arrays, see the array convention above).
@@ -179,6 +204,30 @@ int lare_dev_conduct_heat(void);
 
 /* boundary.f90:48 boundary_conditions (B, energy, density, velocity). */
 int lare_dev_boundary_conditions(void);
+
+/* neutral.f90:158-195 neutral_fraction: the neutral fraction xi_n of every
+ * cell (-1..n+2) from energy and rho, by the reference's bisection for the
+ * temperature.  lare_dev_step calls it before each lagrangian_step
+ * (lare3d.f90:83), lagrangian_step before each resistive substep
+ * (lagran.F90:147) and conduct_heat between RKL2 stages (conduct.f90:404);
+ * the caller calls it once after the initial boundary_conditions
+ * (lare3d.f90:70).  LARE_E_STATE with EOS_IDEAL, LARE_F_TBOUNDS for an
+ * inverted bracket. */
+int lare_dev_neutral_fraction(void);
+
+/* neutral.f90:68-136 perpendicular_resistivity: the Cowling resistivity
+ * eta_perp at nodes 0..n from rho, energy, xi_n and B.  lagrangian_step
+ * calls it before each resistive substep (lagran.F90:148); the caller calls
+ * it once at start-up (lare3d.f90:74).  LARE_E_STATE without
+ * cowling_resistivity. */
+int lare_dev_perpendicular_resistivity(void);
+
+/* Copy xi_n (cell layout (-1:nx+2, -1:ny+2, -1:nz+2)) and eta_perp (the
+ * same layout; set at 0..n, zero elsewhere) to host arrays; either may be
+ * NULL.  The arrays exist with EOS_ION/EOS_PI and cowling_resistivity
+ * (the dump variables of diagnostics.F90:377-394); LARE_E_STATE when a
+ * requested one does not. */
+int lare_dev_get_neutral(double *xi_n, double *eta_perp);
 
 /* diagnostics.F90:517 energy_account: en[0] = magnetic, en[1] = kinetic,
  * en[2] = internal (FP64 sums).  divb (may be NULL) receives max |div B|

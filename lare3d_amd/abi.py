@@ -8,8 +8,14 @@ import os
 
 BC_PERIODIC = 1
 BC_USER = 2
+BC_OPEN = 3
 PREC_FP64 = 0
 PREC_FP32 = 1
+EOS_IDEAL = 1
+EOS_ION = 2
+EOS_PI = 3
+F_NONFINITE = 1
+F_TBOUNDS = 2
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
@@ -35,7 +41,10 @@ class LareParams(C.Structure):
                 ("alpha_av", C.c_double), ("va_max", C.c_double),
                 ("cooling_term", C.c_int32), ("boris", C.c_int32),
                 ("shock_limiter", C.c_int32), ("shock_compression", C.c_int32), ("rke", C.c_int32),
-                ("cauchy", C.c_int32), ("fourth_order", C.c_int32)]
+                ("cauchy", C.c_int32), ("fourth_order", C.c_int32),
+                ("eos_number", C.c_int32), ("cowling_resistivity", C.c_int32), ("pad2_", C.c_int32),
+                ("ionise_pot", C.c_double), ("f_bar", C.c_double), ("t_bar", C.c_double),
+                ("r_bar", C.c_double), ("eta_bar", C.c_double), ("t_rad", C.c_double)]
 
 
 class LareStepInfo(C.Structure):
@@ -86,6 +95,9 @@ DEVICE_SYMBOLS = {
     "lare_dev_energy_correction": [],
     "lare_dev_conduct_heat": [],
     "lare_dev_boundary_conditions": [],
+    "lare_dev_neutral_fraction": [],
+    "lare_dev_perpendicular_resistivity": [],
+    "lare_dev_get_neutral": [_dp, _dp],
     "lare_dev_energy_account": [_dp, _dp],
     "lare_dev_step": [C.c_int32, _dp, _dp],
     "lare_dev_step_info": [C.c_int32, C.c_int32, C.POINTER(LareStepInfo)],
@@ -117,7 +129,9 @@ LIB_PATH = os.path.join(PKG_DIR, "liblare3d_dev.so")
 
 
 class DeviceError(RuntimeError):
-    pass
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 _lib = None
@@ -145,4 +159,4 @@ def load_library(path=None):
 def check(lib, rc, what):
     if rc != 0:
         msg = lib.lare_dev_last_error().decode(errors="replace")
-        raise DeviceError(f"{what} failed with status {rc}: {msg}")
+        raise DeviceError(f"{what} failed with status {rc}: {msg}", rc)

@@ -44,6 +44,10 @@ def default_params(**kw):
     p.rke = 0
     p.cauchy = 0
     p.fourth_order = 0
+    p.eos_number = 1
+    p.cowling_resistivity = 0
+    for k in ("ionise_pot", "f_bar", "t_bar", "r_bar", "eta_bar", "t_rad"):
+        setattr(p, k, 0.0)
     for k, v in kw.items():
         setattr(p, k, v)
     return p
@@ -268,6 +272,21 @@ def kink_tube(nx=64, ny=None, nz=None, beta=0.05, eta=1.0e-3, kappa_0=0.1, amp=1
     f.vy[...] = env[:, :, None] * np.sin(k * zb)[None, None, :]
     f.vz[...] = 0.0
     return grid, params, f
+
+
+def partially_ionised(cfg, eos_number, cowling=0, temp_norm=4.0e4, rho_norm=1.0e-6, L_norm=10.0, mf=1.2):
+    """A configuration switched to a partially ionised equation of state
+    (EOS_ION or EOS_PI, control.f90:187-199), optionally with Cowling
+    resistivity.  The normalisation puts the temperatures of the benchmark
+    states (T ~ 0.15-0.5 in code units) across the ionisation transition:
+    B_norm follows from the requested normalising temperature
+    temp_norm = (mbar / k_B) B_norm^2 / (mu0 rho_norm) (normalise.f90:36-40);
+    the short L_norm makes eta_perp comparable with the resistive eta of
+    the sheet."""
+    from .neutral import KB_SI, MH_SI, MU0_SI, set_neutral
+    b_norm = math.sqrt(temp_norm * KB_SI / (mf * MH_SI) * MU0_SI * rho_norm)
+    set_neutral(cfg[1], b_norm, L_norm, rho_norm, mf, eos_number, cowling)
+    return cfg
 
 
 def uniform(n=8, bcs=(BC_PERIODIC,) * 6, **kw):

@@ -7,7 +7,10 @@
 // walls of boundary.f90 apply.  The reference runs all exchanges (x, y, z)
 // before all user fills (x, y, z); the host launches one kernel per axis in
 // that order.  One launch handles up to 8 fields so that a whole
-// boundary_conditions call is at most three launches.
+// boundary_conditions call is at most three launches.  A BC_OPEN side
+// (openboundary.f90) gets no fill at all: the MPI exchange is a no-op there
+// and the user fills of boundary.f90 apply to BC_USER sides only, so the
+// ghosts keep the far-field values until open_bcs sets the first layer.
 
 enum BcOp {
   BC_NONE = 0,
@@ -27,10 +30,11 @@ struct BcSet {
   int op[BC_MAXF];
   int wide[BC_MAXF];  // 1: FP64 array, 0: FP32 array
   int nf;
+  int skip_lo, skip_hi;  // leave the low / high side alone (a BC_OPEN side)
 };
 
 template <class T>
-__device__ __forceinline__ void bc_line(T *a, int op, int base, int st, int n, bool dm_ok) {
+__device__ __forceinline__ void bc_line(T *a, int op, int base, int st, int n, bool dm_ok, bool lo, bool hi) {
   switch (op) {
     case BC_PER_CELL:
       a[base + (n + 1) * st] = a[base + 1 * st];
@@ -55,24 +59,36 @@ __device__ __forceinline__ void bc_line(T *a, int op, int base, int st, int n, b
       break;
     }
     case BC_MIR_CELL:
-      a[base + 0 * st] = a[base + 1 * st];
-      a[base - 1 * st] = a[base + 2 * st];
-      a[base + (n + 1) * st] = a[base + n * st];
-      a[base + (n + 2) * st] = a[base + (n - 1) * st];
+      if (lo) {
+        a[base + 0 * st] = a[base + 1 * st];
+        a[base - 1 * st] = a[base + 2 * st];
+      }
+      if (hi) {
+        a[base + (n + 1) * st] = a[base + n * st];
+        a[base + (n + 2) * st] = a[base + (n - 1) * st];
+      }
       break;
     case BC_MIR_FACE:
-      a[base - 1 * st] = a[base + 1 * st];
-      a[base - 2 * st] = a[base + 2 * st];
-      a[base + (n + 1) * st] = a[base + (n - 1) * st];
-      a[base + (n + 2) * st] = a[base + (n - 2) * st];
+      if (lo) {
+        a[base - 1 * st] = a[base + 1 * st];
+        a[base - 2 * st] = a[base + 2 * st];
+      }
+      if (hi) {
+        a[base + (n + 1) * st] = a[base + (n - 1) * st];
+        a[base + (n + 2) * st] = a[base + (n - 2) * st];
+      }
       break;
     case BC_ZERO_NODE:
-      a[base - 2 * st] = (T)0;
-      a[base - 1 * st] = (T)0;
-      a[base + 0 * st] = (T)0;
-      a[base + n * st] = (T)0;
-      a[base + (n + 1) * st] = (T)0;
-      a[base + (n + 2) * st] = (T)0;
+      if (lo) {
+        a[base - 2 * st] = (T)0;
+        a[base - 1 * st] = (T)0;
+        a[base + 0 * st] = (T)0;
+      }
+      if (hi) {
+        a[base + n * st] = (T)0;
+        a[base + (n + 1) * st] = (T)0;
+        a[base + (n + 2) * st] = (T)0;
+      }
       break;
     case BC_PHI_NEU:
       a[base + 0 * st] = a[base + 1 * st];
@@ -106,9 +122,10 @@ __global__ __launch_bounds__(64) void k_bc_axis(Geo g, BcSet s, int axis) {
   c[w] = iw;
   const int base = g.idx(c[0], c[1], c[2]);  // index of layer 0 on this line
   const bool dm_ok = iu >= 0 && iu <= g.n(u) + 1 && iw >= 0 && iw <= g.n(w) + 1;
+  const bool lo = !s.skip_lo, hi = !s.skip_hi;
   for (int q = 0; q < s.nf; ++q) {
-    if (s.wide[q]) bc_line((double *)s.f[q], s.op[q], base, st, n, dm_ok);
-    else bc_line((float *)s.f[q], s.op[q], base, st, n, dm_ok);
+    if (s.wide[q]) bc_line((double *)s.f[q], s.op[q], base, st, n, dm_ok, lo, hi);
+    else bc_line((float *)s.f[q], s.op[q], base, st, n, dm_ok, lo, hi);
   }
 }
 
@@ -173,6 +190,12 @@ __device__ __forceinline__ BcMove bc_slot(int op, int n, int slot, bool dm_ok) {
   BcMove m = {dst, src, zero, act};
   return m;
 }
+// the slot's move, unless it writes a side that is left alone
+__device__ __forceinline__ BcMove bc_slot_side(int op, int n, int slot, bool dm_ok, const BcSet &s) {
+  BcMove m = bc_slot(op, n, slot, dm_ok);
+  if (m.act && (m.dst <= 0 ? s.skip_lo : s.skip_hi)) m.act = false;
+  return m;
+}
 
 __global__ __launch_bounds__(256) void k_bc_x(Geo g, BcSet s) {
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -188,7 +211,7 @@ __global__ __launch_bounds__(256) void k_bc_x(Geo g, BcSet s) {
   double v[BC_MAXF];
 #pragma unroll
   for (int q = 0; q < BC_MAXF; ++q) {
-    mv[q] = q < s.nf ? bc_slot(s.op[q], g.nx, slot, dm_ok) : BcMove{0, 0, false, false};
+    mv[q] = q < s.nf ? bc_slot_side(s.op[q], g.nx, slot, dm_ok, s) : BcMove{0, 0, false, false};
     v[q] = 0.0;
     if (mv[q].act && !mv[q].zero)
       v[q] = s.wide[q] ? ((const double *)s.f[q])[base + mv[q].src] : (double)((const float *)s.f[q])[base + mv[q].src];

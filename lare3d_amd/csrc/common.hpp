@@ -18,7 +18,8 @@ struct Geo {
   // 1-D metrics, indexable by Fortran index (pointer already offset)
   const double *dxb, *dxc, *dyb, *dyc, *dzb, *dzc, *grav;
   int per[3];     // axis periodic (both sides)
-  int user[3];    // axis BC_USER (both sides)
+  int user[3];    // axis not periodic (each side BC_USER or BC_OPEN)
+  int open_lo[3], open_hi[3];  // that side BC_OPEN: no fill, ghosts as stored
   __host__ __device__ __forceinline__ int idx(int i, int j, int k) const {
     return (k + 2) * sz + (j + 2) * sy + (i + OX);
   }

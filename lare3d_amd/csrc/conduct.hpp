@@ -28,7 +28,10 @@ __device__ __forceinline__ int mirror_idx(int c, int n) {
   return c < 1 ? 1 - c : 2 * n + 1 - c;  // 0->1, -1->2, n+1->n, n+2->n-1
 }
 
-// Energy-like value at (i,j,k) under a ghost-read mode.
+// Energy-like value at (i,j,k) under a ghost-read mode.  A BC_OPEN side
+// is never filled (energy_bcs and temperature_bcs leave it alone), so its
+// ghosts keep the values stored before the super-step: E0 at the same
+// index, in every mode.
 __device__ __forceinline__ double ghost_read(const Geo &g, const real *__restrict__ Y,
                                              const real *__restrict__ E0, int i, int j, int k, int mode) {
   if (mode == TMODE_DIRECT) return Y[g.idx(i, j, k)];
@@ -38,7 +41,8 @@ __device__ __forceinline__ double ghost_read(const Geo &g, const real *__restric
   for (int a = 0; a < 3; ++a) {
     const int n = g.n(a);
     if (c[a] < 1 || c[a] > n) {
-      if (g.user[a]) c[a] = mirror_idx(c[a], n);
+      if (c[a] < 1 ? g.open_lo[a] : g.open_hi[a]) stale = true;
+      else if (g.user[a]) c[a] = mirror_idx(c[a], n);
       else if (mode == TMODE_STALE) stale = true;
       else c[a] = c[a] < 1 ? c[a] + n : c[a] - n;
     }
@@ -79,14 +83,32 @@ __device__ __forceinline__ float pow15(float t) { return t * sqrtf(t); }
 
 struct CondArgs {
   double kappa_0, tfac, dt;
+  // partially ionised: T = gm1 / (2 - xi_n) (e - (1 - xi_n) ionise_pot) per
+  // cell (conduct.f90:72-73); otherwise T = tfac e with the uniform xi_n
+  const real *xi;
+  double gm1, ip;
 };
+
+// temperature of the energy-like value e of cell c
+template <bool PI>
+__device__ __forceinline__ acc cond_temp(const CondArgs &a, acc e, int c) {
+  if constexpr (PI) {
+    const acc x = (acc)a.xi[c];
+    return (acc)a.gm1 / ((acc)2.0 - x) * (e - ((acc)1.0 - x) * (acc)a.ip);
+  } else {
+    return (acc)a.tfac * e;
+  }
+}
 
 // Larsen flux-limiter factor over cells 0..n and the explicit parabolic
 // time-step minimum over 1..n (calc_s_stages, conduct.f90:69-129).
-// Temperatures from energy as stored (ghosts valid at call time).
+// Temperatures from energy as stored (ghosts valid at call time).  PI:
+// per-cell temperatures of the partially ionised gas.
+template <bool PI>
 __global__ __launch_bounds__(BX * BY) void k_larsen(Geo g, const real *__restrict__ energy, const real *__restrict__ rho,
                          real *__restrict__ larsen, double *__restrict__ partial, double flux_limiter,
-                         double kappa_0, double tfac, int limiter_on) {
+                         double kappa_0, CondArgs ca, int limiter_on) {
+  const double tfac = ca.tfac;
   const Box b = {0, g.nx, 0, g.ny, 0, g.nz};
   PLANE_POS(b, i, j)
   const int k = (int)blockIdx.z;
@@ -94,15 +116,19 @@ __global__ __launch_bounds__(BX * BY) void k_larsen(Geo g, const real *__restric
   const bool in = j <= g.ny && k <= g.nz;
   if (in) {
     const int c = g.idx(i, j, k);
-    const double T = tfac * energy[c];
+    auto T_at = [&](int q) -> double {
+      if constexpr (PI) return (double)cond_temp<true>(ca, (acc)energy[q], q);
+      else return tfac * energy[q];
+    };
+    const double T = T_at(c);
     double L = 1.0;
     if (limiter_on) {
       const double temp = pow25(T);
       const double q_fs = flux_limiter * 42.85 * rho[c] * pow15(T);
       const double q_fs2 = q_fs * q_fs;
-      const double q_spx = -kappa_0 * temp * (tfac * energy[c + 1] - tfac * energy[c - 1]) * 0.5 / g.dxb[i];
-      const double q_spy = -kappa_0 * temp * (tfac * energy[c + g.sy] - tfac * energy[c - g.sy]) * 0.5 / g.dyb[j];
-      const double q_spz = -kappa_0 * temp * (tfac * energy[c + g.sz] - tfac * energy[c - g.sz]) * 0.5 / g.dzb[k];
+      const double q_spx = -kappa_0 * temp * (T_at(c + 1) - T_at(c - 1)) * 0.5 / g.dxb[i];
+      const double q_spy = -kappa_0 * temp * (T_at(c + g.sy) - T_at(c - g.sy)) * 0.5 / g.dyb[j];
+      const double q_spz = -kappa_0 * temp * (T_at(c + g.sz) - T_at(c - g.sz)) * 0.5 / g.dzb[k];
       const double q_sp2 = q_spx * q_spx + q_spy * q_spy + q_spz * q_spz;
       L = q_fs / sqrt(q_fs2 + q_sp2);
     }
@@ -197,11 +223,12 @@ __device__ __forceinline__ acc flux_z_face(const Geo &g, const Tn &T, int lz, in
 // Ghost mapping of one coordinate under a read mode (the per-axis part of
 // ghost_read): the coordinate to read and whether to read E0 instead.
 template <int MODE>
-__device__ __forceinline__ void map_coord(int c, int n, int user, int &m, bool &stale) {
+__device__ __forceinline__ void map_coord(int c, int n, int user, int olo, int ohi, int &m, bool &stale) {
   m = c;
   stale = false;
   if (MODE == TMODE_DIRECT || (c >= 1 && c <= n)) return;
-  if (user) m = mirror_idx(c, n);
+  if (c < 1 ? olo : ohi) stale = true;
+  else if (user) m = mirror_idx(c, n);
   else if (MODE == TMODE_STALE) stale = true;
   else m = c < 1 ? c + n : c - n;
 }
@@ -217,7 +244,7 @@ __device__ __forceinline__ void map_coord(int c, int n, int user, int &m, bool &
 // same values as the neighbour's own, and the six are summed in the order
 // of cell_heat_flux.
 #define RKL_OUT 63
-template <int MODE, int FIRST>
+template <int MODE, int FIRST, bool PI>
 __global__ __launch_bounds__(BX * BY) void k_rkl2_z(Geo g, Box b, const real *__restrict__ Y2, const real *__restrict__ Y1,
                                                     const real *__restrict__ E0, const real *__restrict__ rho,
                                                     const double *__restrict__ bx, const double *__restrict__ by,
@@ -240,21 +267,21 @@ __global__ __launch_bounds__(BX * BY) void k_rkl2_z(Geo g, Box b, const real *__
   bool xs[3], ys[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    map_coord<MODE>(i + q - 1, g.nx, g.user[0], xm[q], xs[q]);
-    map_coord<MODE>(j + q - 1, g.ny, g.user[1], ym[q], ys[q]);
+    map_coord<MODE>(i + q - 1, g.nx, g.user[0], g.open_lo[0], g.open_hi[0], xm[q], xs[q]);
+    map_coord<MODE>(j + q - 1, g.ny, g.user[1], g.open_lo[1], g.open_hi[1], ym[q], ys[q]);
   }
-  const acc tfac = (acc)a.tfac, kap = (acc)a.kappa_0, tdt = (acc)a.dt;
+  const acc kap = (acc)a.kappa_0, tdt = (acc)a.dt;
   // T.t[p][q][r]: x offset p-1, y offset q-1, plane k-1+r
   Tn T;
 #define RKL_LOAD_PLANE(kk, r)                                                    \
   {                                                                              \
     int zm;                                                                      \
     bool zs;                                                                     \
-    map_coord<MODE>((kk), g.nz, g.user[2], zm, zs);                              \
+    map_coord<MODE>((kk), g.nz, g.user[2], g.open_lo[2], g.open_hi[2], zm, zs); \
     _Pragma("unroll") for (int p = 0; p < 3; ++p)                                \
       _Pragma("unroll") for (int q = 0; q < 3; ++q) {                            \
         const int cq = g.idx(xm[p], ym[q], zm);                                  \
-        T.t[p][q][r] = tfac * (acc)(((xs[p] || ys[q] || zs) ? E0 : Y2)[cq]);     \
+        T.t[p][q][r] = cond_temp<PI>(a, (acc)(((xs[p] || ys[q] || zs) ? E0 : Y2)[cq]), cq); \
       }                                                                          \
   }
   RKL_LOAD_PLANE(k0 - 1, 0)
@@ -322,7 +349,7 @@ __global__ __launch_bounds__(BX * BY) void k_rkl2_z(Geo g, Box b, const real *__
 __global__ __launch_bounds__(BX * BY) void k_rkl2_final(Geo g, Box b, const real *__restrict__ Ys, real *__restrict__ energy,
                              int delta_mode) {
   BOX_INDEX(b, i, j, k);
-  const double v = ghost_read(g, Ys, Ys, i, j, k, TMODE_WRAP);
+  const double v = ghost_read(g, Ys, energy, i, j, k, TMODE_WRAP);
   const int c = g.idx(i, j, k);
   if (delta_mode) {
     const double e0 = energy[c];

@@ -34,7 +34,24 @@ struct LagArgs {
   int write_gb;     // 0 when set_dt returns early (first step of a restart)
   int limiter;      // the SHOCKLIMITER variant of edge_viscosity
   int compression;  // the SHOCKCOMPRESSION variant
+  int cowling;      // Cowling resistivity: set_dt takes max(eta, eta_perp)
+  const real *eta_perp;
 };
+
+// Partially ionised equation of state: the neutral fraction per cell and
+// the ionisation potential (zero unless EOS_ION, normalise.f90:48).  The
+// pressure is (gamma - 1) rho (e - (1 - xi_n) ionise_pot)
+// (lagran.F90:93-94, :244).
+struct EosArgs {
+  const real *xi;
+  double ip;
+};
+// e - (1 - xi_n) ionise_pot of cell c; without PI, ionise_pot is 0
+template <bool PI>
+__device__ __forceinline__ acc eos_e(acc e, const EosArgs &q, int c) {
+  if constexpr (PI) return e - ((acc)1.0 - (acc)q.xi[c]) * (acc)q.ip;
+  else return e - (acc)0.0;
+}
 
 // 8-cell sum around node (i,j,k) in the reference order (lagran.F90:105-117)
 #define SUM8(F, c, sy, sz)                                                                            \
@@ -51,11 +68,11 @@ __device__ __forceinline__ double cv_sum8(const Geo &g, int i, int j, int k) {
 
 // fast magnetosonic speed of shock_viscosity (:554-556) in cell c; with GB,
 // gb is gamma_boris of the previous set_dt, else 1 (boris off) and gb unused
-template <bool GB>
+template <bool GB, bool PI>
 __device__ __forceinline__ acc cell_cs(const Geo &g, const real *__restrict__ rho, const real *__restrict__ energy,
                                        const double *__restrict__ bx, const double *__restrict__ by,
                                        const double *__restrict__ bz, const real *__restrict__ gb, int c,
-                                       double gamma) {
+                                       double gamma, const EosArgs &q) {
   const acc r = rho[c];
   const acc rmin = dmax(r, ACC_NONE_ZERO);
   const acc b1x = (bx[c] + bx[c - 1]) * 0.5;
@@ -63,7 +80,7 @@ __device__ __forceinline__ acc cell_cs(const Geo &g, const real *__restrict__ rh
   const acc b1z = (bz[c] + bz[c - g.sz]) * 0.5;
   const acc b2 = b1x * b1x + b1y * b1y + b1z * b1z;
   const acc ga = (acc)gamma;
-  const acc p = (ga - (acc)1.0) * r * (energy[c] - (acc)0.0);
+  const acc p = (ga - (acc)1.0) * r * eos_e<PI>(energy[c], q, c);
   const acc g_b = GB ? (acc)gb[c] : (acc)1.0;
   return sqrt((ga * p + g_b * b2) / rmin);
 }
@@ -73,17 +90,17 @@ __device__ __forceinline__ acc cell_cs(const Geo &g, const real *__restrict__ rh
 struct CellQuad {
   acc r[4], cs[4];
 };
-template <bool GB>
+template <bool GB, bool PI>
 __device__ __forceinline__ CellQuad cell_quad(const Geo &g, const real *__restrict__ rho,
                                               const real *__restrict__ energy, const double *__restrict__ bx,
                                               const double *__restrict__ by, const double *__restrict__ bz,
-                                              const real *__restrict__ gb, int c, double gamma) {
+                                              const real *__restrict__ gb, int c, double gamma, const EosArgs &eq) {
   CellQuad q;
   const int off[4] = {0, 1, g.sy, 1 + g.sy};
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
     q.r[p] = rho[c + off[p]];
-    q.cs[p] = cell_cs<GB>(g, rho, energy, bx, by, bz, gb, c + off[p], gamma);
+    q.cs[p] = cell_cs<GB, PI>(g, rho, energy, bx, by, bz, gb, c + off[p], gamma, eq);
   }
   return q;
 }
@@ -92,18 +109,19 @@ __device__ __forceinline__ CellQuad cell_quad(const Geo &g, const real *__restri
 // over nodes -1..n+1; marches in z, keeping the cell values of plane k+1 for
 // the next node plane, so each cell's fast speed is computed once per
 // column chunk.  Sums run over the 8 cells in the order (i,j,k), (i+1,j,k),
-// (i,j+1,k), (i+1,j+1,k), then the same four at k+1.  GB: boris on.
-template <bool GB>
+// (i,j+1,k), (i+1,j+1,k), then the same four at k+1.  GB: boris on.  PI:
+// partially ionised pressure.
+template <bool GB, bool PI>
 __global__ __launch_bounds__(BX * BY) void k_vertex_prep(Geo g, Box b, const real *__restrict__ rho, const real *__restrict__ energy,
                               const double *__restrict__ bx, const double *__restrict__ by,
                               const double *__restrict__ bz, const real *__restrict__ gb, real *__restrict__ rho_v,
-                              real *__restrict__ cs_v, double gamma) {
+                              real *__restrict__ cs_v, double gamma, EosArgs eq) {
   ZCOLUMN(b, i, j, k0, k1);
   const int Z = g.sz;
-  CellQuad lo = cell_quad<GB>(g, rho, energy, bx, by, bz, gb, g.idx(i, j, k0), gamma);
+  CellQuad lo = cell_quad<GB, PI>(g, rho, energy, bx, by, bz, gb, g.idx(i, j, k0), gamma, eq);
   for (int k = k0; k <= k1; ++k) {
     const int c = g.idx(i, j, k);
-    const CellQuad hi = cell_quad<GB>(g, rho, energy, bx, by, bz, gb, c + Z, gamma);
+    const CellQuad hi = cell_quad<GB, PI>(g, rho, energy, bx, by, bz, gb, c + Z, gamma, eq);
     acc cvs[8];
     const int di[8] = {0, 1, 0, 1, 0, 1, 0, 1}, dj[8] = {0, 0, 1, 1, 0, 0, 1, 1}, dk[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 #pragma unroll
@@ -318,7 +336,9 @@ __device__ __forceinline__ acc edge_heat_v(acc dxb, acc dyb, acc dzb, acc cv, co
 }
 
 // p_visc + visc_heat over cells 0..n+1; pressure p0 of lagran.F90:93 stored
-// for the predictor; set_dt partial minima over 0..n.
+// for the predictor; set_dt partial minima over 0..n.  PI: partially
+// ionised pressure.
+template <bool PI>
 __global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, Box bx0, const real *__restrict__ vx, const real *__restrict__ vy,
                           const real *__restrict__ vz, const real *__restrict__ rho,
                           const real *__restrict__ energy, const double *__restrict__ bx,
@@ -326,7 +346,7 @@ __global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, Box bx0, const real 
                           const real *__restrict__ eta, const real *__restrict__ al1,
                           const real *__restrict__ al2, const real *__restrict__ al3,
                           real *__restrict__ visc_heat, real *__restrict__ p0, real *__restrict__ gbo,
-                          double *__restrict__ partial, LagArgs a) {
+                          double *__restrict__ partial, LagArgs a, EosArgs eq) {
   // cells 0..n+1, marching in z: node velocities and al1/al2 of plane k
   // become those of plane k-1 / k for the next cell
   PLANE_POS(bx0, i, j)
@@ -361,7 +381,7 @@ __global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, Box bx0, const real 
       visc_heat[c] = edge_heat_v((acc)g.dxb[i], (acc)g.dyb[j], (acc)g.dzb[k], (acc)g.cv(i, j, k), alo, ahi, a30, a3x,
                                  a3xy, a3y, aa);
       const acc r = rho[c];
-      const acc p = ((acc)a.gamma - (acc)1.0) * r * (energy[c] - (acc)0.0);
+      const acc p = ((acc)a.gamma - (acc)1.0) * r * eos_e<PI>(energy[c], eq, c);
       p0[c] = p;
       if (i <= g.nx && j <= g.ny && k <= g.nz) {
         const double w1 = bx[c] * bx[c] + by[c] * by[c] + bz[c] * bz[c];
@@ -390,7 +410,10 @@ __global__ __launch_bounds__(BX * BY) void k_visc_dt(Geo g, Box bx0, const real 
             if (!__builtin_isfinite(vals[q]) || (q == 0 && !(r > 0.0)))
               v[2] = dmin(v[2], -((double)(q + 1) * 4294967296.0 + cell));
         }
-        if (a.resistive) {
+        if (a.cowling) {
+          const double dxlocal = 1.0 / (1.0 / (dxb * dxb) + 1.0 / (dyb * dyb) + 1.0 / (dzb * dzb));
+          v[1] = dmin(v[1], 0.2 * dxlocal / dmax(dmax((double)eta[c], (double)a.eta_perp[c]), NONE_ZERO));
+        } else if (a.resistive) {
           const double dxlocal = 1.0 / (1.0 / (dxb * dxb) + 1.0 / (dyb * dyb) + 1.0 / (dzb * dzb));
           v[1] = dmin(v[1], 0.2 * dxlocal / dmax((double)eta[c], NONE_ZERO));
         }
@@ -515,6 +538,34 @@ __global__ __launch_bounds__(BX * BY) void k_fill(Geo g, Box b, real *__restrict
   a[g.idx(i, j, k)] = v;
 }
 
+// rkstep's EMFs and curlb at one node with Cowling resistivity
+// (lagran.F90:1504-1577): the vertex current j splits into j_par along the
+// vertex field and j_perp = j - j_par; the EMF is -((j_par + j_perp) eta +
+// j_perp eta_perp) d/2 per component and curlb = eta |j_par|^2 + (eta_perp
+// + eta) |j_perp|^2.  lo, hi: face bx, by of planes k and k+1; bz0..bzxy: bz
+// at (i,j), (i+1,j), (i,j+1), (i+1,j+1) of plane k.
+__device__ __forceinline__ void cowling_emf(const BPlane &lo, const BPlane &hi, double bz0, double bzx, double bzy,
+                                            double bzxy, acc jx, acc jy, acc jz, acc e, acc ep, acc dxc, acc dyc,
+                                            acc dzc, acc &kx, acc &ky, acc &kz, acc &cb) {
+  const acc bxv = ((acc)lo.bx0 + (acc)lo.bxy + (acc)hi.bx0 + (acc)hi.bxy) * (acc)0.25;
+  const acc byv = ((acc)lo.by0 + (acc)lo.byx + (acc)hi.by0 + (acc)hi.byx) * (acc)0.25;
+  const acc bzv = ((acc)bz0 + (acc)bzx + (acc)bzy + (acc)bzxy) * (acc)0.25;
+  const acc magn_b = bxv * bxv + byv * byv + bzv * bzv;
+  acc jpx = 0.0, jpy = 0.0, jpz = 0.0;
+  if (magn_b > ACC_NONE_ZERO) {
+    jpx = (jx * bxv + jy * byv + jz * bzv) * bxv / magn_b;
+    jpy = (jx * bxv + jy * byv + jz * bzv) * byv / magn_b;
+    jpz = (jx * bxv + jy * byv + jz * bzv) * bzv / magn_b;
+  }
+  const acc jqx = jx - jpx, jqy = jy - jpy, jqz = jz - jpz;
+  const acc mpar = sqrt(jpx * jpx + jpy * jpy + jpz * jpz);
+  const acc mperp = sqrt(jqx * jqx + jqy * jqy + jqz * jqz);
+  cb = e * (mpar * mpar) + (ep + e) * (mperp * mperp);
+  kx = -((jpx + jqx) * e + jqx * ep) * dxc * (acc)0.5;
+  ky = -((jpy + jqy) * e + jqy * ep) * dyc * (acc)0.5;
+  kz = -((jpz + jqz) * e + jqz * ep) * dzc * (acc)0.5;
+}
+
 // EMFs around a node in one plane, named as in bstep (lagran.F90:1582-1631):
 // kx at (i,j), (i-1,j); ky at (i,j), (i,j-1); kz at (i,j), (i,j-1), (i-1,j)
 struct EPlane {
@@ -535,9 +586,11 @@ struct EPlane {
 // LDS, so a tile writes RS_OUT_X x RS_OUT_Y nodes.  Neighbouring tiles still
 // read the old B, so the new B goes to bxo..bzo at every node of the launch
 // box (the padded box): updated where bstep updates it, copied elsewhere.
+// COWL: rkstep's Cowling form with eta_perp at the nodes.
 #define RS_ROWS 8
 #define RS_OUT_X 63
 #define RS_OUT_Y (RS_ROWS - 1)
+template <bool COWL>
 __global__ __launch_bounds__(64 * RS_ROWS) void k_resist(Geo g, Box b, int tiles_x, const double *__restrict__ bx,
                                                          const double *__restrict__ by, const double *__restrict__ bz,
                                                          double *__restrict__ bxo, double *__restrict__ byo,
@@ -545,7 +598,7 @@ __global__ __launch_bounds__(64 * RS_ROWS) void k_resist(Geo g, Box b, int tiles
                                                          real *__restrict__ energy, real *__restrict__ ohmic_dep,
                                                          const real *__restrict__ rho, real *__restrict__ curlb_out,
                                                          double *__restrict__ partial, double dt, double eta_bg,
-                                                         double eta0, double j_max) {
+                                                         double eta0, double j_max, const real *__restrict__ eta_perp) {
   // EMFs kx, ky, kz and curlb of the tile's nodes, by plane parity
   __shared__ acc sk[2][4][RS_ROWS][64];
   __shared__ double red[RS_ROWS];
@@ -576,16 +629,23 @@ __global__ __launch_bounds__(64 * RS_ROWS) void k_resist(Geo g, Box b, int tiles
       acc jx, jy, jz;
       vertex_current_acc<acc>(g.dxc[i], g.dyc[j], g.dzc[k], lo, hi, bz[c], bz[c + 1], bz[c + Y], bz[c + 1 + Y], jx,
                               jy, jz);
+      const BPlane plo = lo;
       lo = hi;
       have_lo = true;
       const real es = sqrt(jx * jx + jy * jy + jz * jz) > (acc)j_max ? eta_bg + eta0 : eta_bg;
       if (out && k >= k0) eta[c] = es;
       if (kij && k >= 0 && k <= g.nz) {
         const acc e = es;
-        kx = -jx * e * (acc)g.dxc[i] * (acc)0.5;
-        ky = -jy * e * (acc)g.dyc[j] * (acc)0.5;
-        kz = -jz * e * (acc)g.dzc[k] * (acc)0.5;
-        const acc cb = e * (jx * jx + jy * jy + jz * jz);
+        acc cb;
+        if constexpr (COWL) {
+          cowling_emf(plo, hi, bz[c], bz[c + 1], bz[c + Y], bz[c + 1 + Y], jx, jy, jz, e, (acc)eta_perp[c],
+                      (acc)g.dxc[i], (acc)g.dyc[j], (acc)g.dzc[k], kx, ky, kz, cb);
+        } else {
+          kx = -jx * e * (acc)g.dxc[i] * (acc)0.5;
+          ky = -jy * e * (acc)g.dyc[j] * (acc)0.5;
+          kz = -jz * e * (acc)g.dzc[k] * (acc)0.5;
+          cb = e * (jx * jx + jy * jy + jz * jz);
+        }
         cbr = (real)cb;
         if (out && k >= k0) {
           if (curlb_out) curlb_out[c] = cbr;
@@ -671,9 +731,11 @@ __global__ __launch_bounds__(BX * BY) void k_energy_max(Geo g, Box b, real *__re
 struct RkK {
   real *k[4];  // kx, ky, kz, curlb
 };
+template <bool COWL>
 __global__ __launch_bounds__(BX * BY) void k_rk4_emf(Geo g, Box b, const double *__restrict__ bx,
                                                      const double *__restrict__ by, const double *__restrict__ bz,
-                                                     const real *__restrict__ eta, RkK k, RkK ks, int w) {
+                                                     const real *__restrict__ eta, RkK k, RkK ks, int w,
+                                                     const real *__restrict__ eta_perp) {
   BOX_INDEX(b, i, j, kk);
   const int Y = g.sy, Z = g.sz;
   const int c = g.idx(i, j, kk);
@@ -682,8 +744,21 @@ __global__ __launch_bounds__(BX * BY) void k_rk4_emf(Geo g, Box b, const double 
   vertex_current_acc<acc>(g.dxc[i], g.dyc[j], g.dzc[kk], lo, hi, bz[c], bz[c + 1], bz[c + Y], bz[c + 1 + Y], jx, jy,
                           jz);
   const acc e = eta[c];
-  const real v[4] = {-jx * e * (acc)g.dxc[i] * (acc)0.5, -jy * e * (acc)g.dyc[j] * (acc)0.5,
-                     -jz * e * (acc)g.dzc[kk] * (acc)0.5, e * (jx * jx + jy * jy + jz * jz)};
+  real v[4];
+  if constexpr (COWL) {
+    acc kx, ky, kz, cb;
+    cowling_emf(lo, hi, bz[c], bz[c + 1], bz[c + Y], bz[c + 1 + Y], jx, jy, jz, e, (acc)eta_perp[c],
+                (acc)g.dxc[i], (acc)g.dyc[j], (acc)g.dzc[kk], kx, ky, kz, cb);
+    v[0] = kx;
+    v[1] = ky;
+    v[2] = kz;
+    v[3] = cb;
+  } else {
+    v[0] = -jx * e * (acc)g.dxc[i] * (acc)0.5;
+    v[1] = -jy * e * (acc)g.dyc[j] * (acc)0.5;
+    v[2] = -jz * e * (acc)g.dzc[kk] * (acc)0.5;
+    v[3] = e * (jx * jx + jy * jy + jz * jz);
+  }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     k.k[q][c] = v[q];
@@ -783,7 +858,7 @@ __global__ __launch_bounds__(BX * BY) void k_cauchy_b(Geo g, Box b, const real *
                                                       const real *__restrict__ rho,
                                                       const real *__restrict__ visc_heat,
                                                       const real *__restrict__ p0, real *__restrict__ pp,
-                                                      double gamma, double dt2) {
+                                                      double gamma, double dt2, EosArgs eq) {
   BOX_INDEX(b, i, j, k);
   const int X = 1, Y = g.sy, Z = g.sz;
   const int c = g.idx(i, j, k);
@@ -840,7 +915,8 @@ __global__ __launch_bounds__(BX * BY) void k_cauchy_b(Geo g, Box b, const real *
     const acc dvv = c1 / cv - (acc)1.0;
     acc e1 = energy[c] - p0[c] * dvv / r;
     e1 = e1 + visc_heat[c] * h / r;
-    pp[c] = (e1 - (acc)0.0) * ((acc)gamma - (acc)1.0) * r * cv / c1;
+    const acc ei = eq.xi ? eos_e<true>(e1, eq, c) : eos_e<false>(e1, eq, c);
+    pp[c] = ei * ((acc)gamma - (acc)1.0) * r * cv / c1;
   }
 }
 

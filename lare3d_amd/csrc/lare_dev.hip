@@ -95,6 +95,9 @@ int lare_dev_eta_calc(void) { return FWD(api_eta_calc()); }
 int lare_dev_energy_correction(void) { return FWD(api_energy_correction()); }
 int lare_dev_conduct_heat(void) { return FWD(api_conduct_heat()); }
 int lare_dev_boundary_conditions(void) { return FWD(api_boundary_conditions()); }
+int lare_dev_neutral_fraction(void) { return FWD(api_neutral_fraction()); }
+int lare_dev_perpendicular_resistivity(void) { return FWD(api_perpendicular_resistivity()); }
+int lare_dev_get_neutral(double *xi_n, double *eta_perp) { return FWD(api_get_neutral(xi_n, eta_perp)); }
 int lare_dev_energy_account(double en[3], double *divb) { return FWD(api_energy_account(en, divb)); }
 int lare_dev_step(int32_t step, double *dt, double *time) { return FWD(api_step(step, dt, time)); }
 int lare_dev_step_info(int32_t step, int32_t with_energy, lare_step_info *info) {

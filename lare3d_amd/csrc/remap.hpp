@@ -47,6 +47,7 @@ struct PredP {
   const real *energy, *rho, *visc_heat, *p0;
   real *pp;
   double gamma, dt2;
+  EosArgs eq;  // partially ionised pressure when eq.xi is set
 };
 
 // Lagrangian geometry of a sweep (xremap.f90:57-105 and analogues).
@@ -114,7 +115,8 @@ __global__ __launch_bounds__(BX * BY) void k_geom(Geo g, Box b, const real *__re
         const acc dv = c1 / cv - (acc)1.0;
         acc e1 = pr.energy[c] - pr.p0[c] * dv / r;
         e1 = e1 + pr.visc_heat[c] * (acc)pr.dt2 / r;
-        pr.pp[c] = (e1 - (acc)0.0) * ((acc)pr.gamma - (acc)1.0) * r * cv / c1;
+        const acc ei = pr.eq.xi ? eos_e<true>(e1, pr.eq, c) : eos_e<false>(e1, pr.eq, c);
+        pr.pp[c] = ei * ((acc)pr.gamma - (acc)1.0) * r * cv / c1;
       }
     }
     db1[c] = d1;

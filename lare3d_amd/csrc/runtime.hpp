@@ -20,6 +20,8 @@ typedef LARE_ACC acc;
 #include "remap.hpp"
 #include "surrogate.hpp"
 #include "initial.hpp"
+#include "neutral.hpp"
+#include "openbc.hpp"
 
 // ----------------------------------------------------------------------------
 // kernel timing with HIP events on the library stream
@@ -41,6 +43,7 @@ struct Dev {
   size_t ntot;
   int bc[6];
   int per[3], user[3];
+  bool any_open = false;  // some side is BC_OPEN
   lare_params p;
   double tfac, visc2_norm;
   hipStream_t stream = nullptr;
@@ -70,6 +73,16 @@ struct Dev {
   RkeOut rk = {};
   // FOURTHORDER: one stage's EMFs and curlb, and their weighted sum
   RkK rk4k = {}, rk4s = {};
+  // partially ionised EOS: xi_n (cells), eta_perp (nodes, with Cowling), the
+  // bracket fault flag of neutral_fraction; ip = ionise_pot (EOS_ION only)
+  real *xi = nullptr, *eta_perp = nullptr;
+  int *nf_flag = nullptr;
+  // BC_OPEN: the pre-corrector energy and density of each open face's first
+  // interior layer (6 faces x 2 fields x (nmax+2)^2)
+  double *obc_save = nullptr;
+  size_t obc_face = 0;
+  bool pi_eos = false;
+  double ip = 0.0;
   // reductions
   double *partial = nullptr;
   size_t partial_cap = 0;
@@ -290,6 +303,9 @@ int apply_bcs(const BcField *f, int nf, const char *name) {
         s.op[q] = op_for(f[q].kind, ax, periodic);
         s.wide[q] = wide_kind(f[q].kind) ? 1 : 0;
       }
+      s.skip_lo = S.bc[2 * ax] == LARE_BC_OPEN;
+      s.skip_hi = S.bc[2 * ax + 1] == LARE_BC_OPEN;
+      if (s.skip_lo && s.skip_hi) continue;
       int rc = launch_bc_axis(s, ax, name);
       if (rc) return rc;
     }
@@ -313,10 +329,25 @@ int boundary_conditions() {
 // ----------------------------------------------------------------------------
 // conduction
 // ----------------------------------------------------------------------------
+CondArgs cond_args() {
+  CondArgs ca;
+  ca.kappa_0 = S.p.kappa_0;
+  ca.tfac = S.tfac;
+  ca.dt = S.dt;
+  ca.xi = S.pi_eos ? S.xi : nullptr;
+  ca.gm1 = S.p.gamma - 1.0;
+  ca.ip = S.ip;
+  return ca;
+}
+
 int calc_s_stages() {
   dim3 gr = grid_from0(nx(), ny(), nz());
-  LAUNCH("larsen", k_larsen, gr, box_block(), S.g, S.energy, S.rho, S.larsen, S.partial, S.p.flux_limiter,
-         S.p.kappa_0, S.tfac, S.p.heat_flux_limiter);
+  if (S.pi_eos)
+    LAUNCH("larsen", k_larsen<true>, gr, box_block(), S.g, S.energy, S.rho, S.larsen, S.partial, S.p.flux_limiter,
+           S.p.kappa_0, cond_args(), S.p.heat_flux_limiter);
+  else
+    LAUNCH("larsen", k_larsen<false>, gr, box_block(), S.g, S.energy, S.rho, S.larsen, S.partial, S.p.flux_limiter,
+           S.p.kappa_0, cond_args(), S.p.heat_flux_limiter);
   int rc = final_reduce(gr, 1, 1, 0, 0);
   if (rc) return rc;
   double dtp;
@@ -342,22 +373,34 @@ dim3 rkl2_grid(const Box &in) {
 // stage 1 from E0 (ghosts as stored): Lc and Y1 = E0 + mu_t1 dt Lc
 int rkl2_stage1(const real *E0, real *Yout, CondArgs ca, double mu_t1) {
   const Box in = interior();
-  LAUNCH("rkl2_stage1", (k_rkl2_z<TMODE_DIRECT, 1>), rkl2_grid(in), box_block(), S.g, in, E0, E0, E0, S.rho, S.bx,
-         S.by, S.bz, S.larsen, S.Lc, Yout, ca, 0.0, 0.0, 0.0, mu_t1, 0.0);
+  if (ca.xi)
+    LAUNCH("rkl2_stage1", (k_rkl2_z<TMODE_DIRECT, 1, true>), rkl2_grid(in), box_block(), S.g, in, E0, E0, E0, S.rho,
+           S.bx, S.by, S.bz, S.larsen, S.Lc, Yout, ca, 0.0, 0.0, 0.0, mu_t1, 0.0);
+  else
+    LAUNCH("rkl2_stage1", (k_rkl2_z<TMODE_DIRECT, 1, false>), rkl2_grid(in), box_block(), S.g, in, E0, E0, E0, S.rho,
+           S.bx, S.by, S.bz, S.larsen, S.Lc, Yout, ca, 0.0, 0.0, 0.0, mu_t1, 0.0);
   return 0;
 }
 // stage j >= 2 from Y2 (and Y1, E0, Lc) under a ghost-read mode
 int rkl2_stage(int mode, const real *Y2, const real *Y1, const real *E0, real *Yout, CondArgs ca, double mu, double nu,
                double one_m_mu_nu, double mu_t, double gam_t) {
   const Box in = interior();
-  if (mode == TMODE_STALE)
-    LAUNCH("rkl2_stage", (k_rkl2_z<TMODE_STALE, 0>), rkl2_grid(in), box_block(), S.g, in, Y2, Y1, E0, S.rho, S.bx,
-           S.by, S.bz, S.larsen, S.Lc, Yout, ca, mu, nu, one_m_mu_nu, mu_t, gam_t);
-  else
-    LAUNCH("rkl2_stage", (k_rkl2_z<TMODE_WRAP, 0>), rkl2_grid(in), box_block(), S.g, in, Y2, Y1, E0, S.rho, S.bx,
-           S.by, S.bz, S.larsen, S.Lc, Yout, ca, mu, nu, one_m_mu_nu, mu_t, gam_t);
+#define RKL2_STAGE(M, P)                                                                                         \
+  LAUNCH("rkl2_stage", (k_rkl2_z<M, 0, P>), rkl2_grid(in), box_block(), S.g, in, Y2, Y1, E0, S.rho, S.bx, S.by, \
+         S.bz, S.larsen, S.Lc, Yout, ca, mu, nu, one_m_mu_nu, mu_t, gam_t)
+  if (mode == TMODE_STALE) {
+    if (ca.xi) RKL2_STAGE(TMODE_STALE, true);
+    else RKL2_STAGE(TMODE_STALE, false);
+  } else {
+    if (ca.xi) RKL2_STAGE(TMODE_WRAP, true);
+    else RKL2_STAGE(TMODE_WRAP, false);
+  }
+#undef RKL2_STAGE
   return 0;
 }
+
+int neutral_fraction(const real *Y, int mode);
+EosArgs eos_args();
 
 // RKL2 super-step; delta_mode = 1 inside lagrangian_step
 int heat_conduct_sts2(int delta_mode) {
@@ -383,7 +426,7 @@ int heat_conduct_sts2(int delta_mode) {
     nu[j] = -1.0 * ((dj - 1.0) / dj) * (b[j] / b[j - 2]);
     gt[j] = -1.0 * a[j - 1] * mu_t[j];
   }
-  CondArgs ca = {S.p.kappa_0, S.tfac, S.dt};
+  CondArgs ca = cond_args();
   real *E0 = S.energy;
   real *outp = S.Yb[0];
   int rc = rkl2_stage1(E0, S.Yb[0], ca, mu_t[1]);
@@ -396,6 +439,12 @@ int heat_conduct_sts2(int delta_mode) {
     rc = rkl2_stage(j == 2 ? TMODE_STALE : TMODE_WRAP, y2, y1, E0, yo, ca, mu[j], nu[j], (1.0 - mu[j]) - nu[j],
                     mu_t[j], gt[j]);
     if (rc) return rc;
+    // the neutral fraction of the stage value with its boundary fill
+    // (conduct.f90:400-405), read by the next stage's temperatures
+    if (S.pi_eos && j < s) {
+      rc = neutral_fraction(yo, TMODE_WRAP);
+      if (rc) return rc;
+    }
     y1 = y2;
     y2 = yo;
   }
@@ -475,7 +524,7 @@ int ct_pair(double *const Bin[3], double *const B[3], const real *vel, double dt
 template <int AX>
 int sweep_predictor(double dt, bool first, bool last) {
   // the last pass also forms the predictor energy and pressure
-  const PredP pr = {S.energy, S.rho, S.visc_heat, S.p0, S.pp, S.p.gamma, S.dt2};
+  const PredP pr = {S.energy, S.rho, S.visc_heat, S.p0, S.pp, S.p.gamma, S.dt2, eos_args()};
   if (last)
     LAUNCH("geom", (k_geom<AX, 1>), zchunk_grid(cell_box()), box_block(), S.g, cell_box(), S.vx, S.vy, S.vz, S.cv1,
            S.cv2, S.db1, dt, 1.0, 1.0, 1.0, pr);
@@ -522,6 +571,12 @@ int sweep_full(double dt, double pass[3], bool first, bool last) {
     s.wide[0] = wide_kind(FK_CELL) ? 1 : 0;
     rc = launch_bc_axis(s, AX, "bc_dm");
     if (rc) return rc;
+  }
+  if (S.any_open) {
+    // the new energy and density cover cells 1..n; the ghosts of a BC_OPEN
+    // side keep their values (no fill there)
+    LAUNCH_BOX("copy_open", k_copy_open, full_box(), S.energy, S.Fe, 0);
+    LAUNCH_BOX("copy_open", k_copy_open, full_box(), S.rho, S.rho_alt, 0);
   }
   {
     real *te = S.energy;
@@ -634,7 +689,102 @@ LagArgs lag_args() {
   a.write_gb = !(S.first_set_dt && S.p.restart);
   a.limiter = S.p.shock_limiter;
   a.compression = S.p.shock_compression;
+  a.cowling = S.p.cowling_resistivity;
+  a.eta_perp = S.eta_perp;
   return a;
+}
+
+// ----------------------------------------------------------------------------
+// partially ionised plasma (neutral.f90)
+// ----------------------------------------------------------------------------
+EosArgs eos_args() { return EosArgs{S.pi_eos ? S.xi : nullptr, S.ip}; }
+
+NeutralArgs neutral_args() {
+  NeutralArgs a;
+  a.gm1 = S.p.gamma - 1.0;
+  a.ip = S.ip;
+  a.f_bar = S.p.f_bar;
+  a.t_bar = S.p.t_bar;
+  a.r_bar = S.p.r_bar;
+  a.eta_bar = S.p.eta_bar;
+  a.t_rad = S.p.t_rad;
+  return a;
+}
+
+// neutral_fraction over the cell box from Y (energy, or an RKL2 stage value
+// read through the boundary map when mode is TMODE_WRAP)
+int neutral_fraction(const real *Y, int mode) {
+  if (mode == TMODE_WRAP)
+    LAUNCH_BOX("neutral", k_neutral<TMODE_WRAP>, cell_box(), Y, S.energy, S.rho, S.xi, S.nf_flag, neutral_args());
+  else
+    LAUNCH_BOX("neutral", k_neutral<TMODE_DIRECT>, cell_box(), Y, S.energy, S.rho, S.xi, S.nf_flag, neutral_args());
+  return 0;
+}
+
+int perpendicular_resistivity() {
+  LAUNCH_BOX("eta_perp", k_eta_perp, nodes0n(), S.rho, S.energy, S.xi, S.bx, S.by, S.bz, S.eta_perp,
+             neutral_args());
+  return 0;
+}
+
+// ----------------------------------------------------------------------------
+// open boundaries (openboundary.f90)
+// ----------------------------------------------------------------------------
+OpenFace open_face(int a, int hi) {
+  OpenFace f;
+  const int n = S.n[a];
+  f.a = a;
+  f.hi = hi;
+  f.u = a == 0 ? 1 : 0;
+  f.w = a == 2 ? 1 : 2;
+  f.ty = a == 1 ? 0 : 1;
+  f.tz = a == 2 ? 0 : 2;
+  f.lb = hi ? n : 0;
+  f.lc = hi ? n : 1;
+  f.lfc = hi ? n + 2 : -1;
+  f.lfn = hi ? n + 2 : -2;
+  f.lset_n = hi ? n + 1 : -1;
+  f.lset_c = hi ? n + 1 : 0;
+  f.sg = hi ? 1.0 : -1.0;
+  return f;
+}
+
+unsigned face_points(const OpenFace &f) { return (unsigned)((S.n[f.u] + 2) * (S.n[f.w] + 2)); }
+
+// the pre-corrector energy and density of each open face's first interior layer
+int open_save() {
+  for (int a = 0; a < 3; ++a)
+    for (int hi = 0; hi <= 1; ++hi) {
+      if (S.bc[2 * a + hi] != LARE_BC_OPEN) continue;
+      const OpenFace f = open_face(a, hi);
+      LAUNCH("open_save", k_open_save, dim3((face_points(f) + 255) / 256), dim3(256), S.g, f, S.energy, S.rho,
+             S.obc_save + (size_t)(2 * a + hi) * S.obc_face);
+    }
+  return 0;
+}
+
+// open_bcs: the faces one launch each, in the reference's order
+int open_bcs() {
+  for (int a = 0; a < 3; ++a)
+    for (int hi = 0; hi <= 1; ++hi) {
+      if (S.bc[2 * a + hi] != LARE_BC_OPEN) continue;
+      const OpenFace f = open_face(a, hi);
+      LAUNCH("open_bcs", k_open_bcs, dim3((face_points(f) + 255) / 256), dim3(256), S.g, f, S.vx, S.vy, S.vz, S.bx,
+             S.by, S.bz, S.rho, S.energy, S.obc_save + (size_t)(2 * a + hi) * S.obc_face, S.p.gamma);
+    }
+  return 0;
+}
+
+// the reference STOPs on an inverted temperature bracket (neutral.f90:172-175):
+// report it once the work queued so far has run, and clear the flag
+int check_neutral_fault(const char *where) {
+  if (!S.nf_flag) return 0;
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, S.nf_flag, sizeof(int), hipMemcpyDeviceToHost, S.stream));
+  HIPCHK(hipStreamSynchronize(S.stream));
+  if (!h) return 0;
+  HIPCHK(hipMemsetAsync(S.nf_flag, 0, sizeof(int), S.stream));
+  return fail(LARE_F_TBOUNDS, "%s: neutral_fraction temperature bracket inverted (energy <= 0 in some cell)", where);
 }
 
 int set_dt_and_stages(double dt_local, double dtr_local) {
@@ -685,7 +835,10 @@ int resist_rk4_substep() {
       rc = bfield_bcs(S.bx, S.by, S.bz);
       if (rc) return rc;
     }
-    LAUNCH_BOX("rk4_emf", k_rk4_emf, n0, S.bx, S.by, S.bz, S.eta, S.rk4k, S.rk4s, w[q]);
+    if (S.eta_perp)
+      LAUNCH_BOX("rk4_emf", k_rk4_emf<true>, n0, S.bx, S.by, S.bz, S.eta, S.rk4k, S.rk4s, w[q], S.eta_perp);
+    else
+      LAUNCH_BOX("rk4_emf", k_rk4_emf<false>, n0, S.bx, S.by, S.bz, S.eta, S.rk4k, S.rk4s, w[q], S.eta_perp);
   }
   LAUNCH_BOX("rk4_bstep", k_rk4_bstep, n0, S.fxr, S.fyr, S.fzr, S.rk4s, S.bx, S.by, S.bz, dt6);
   rc = bfield_bcs(S.bx, S.by, S.bz);
@@ -703,12 +856,18 @@ int lagrangian_step() {
   LagArgs a = lag_args();  // dt = previous step's dt (Q1)
   {
     const Box vb = mkbox(-1, N0 + 1, -1, N1 + 1, -1, N2 + 1);
-    if (S.gamma_boris)
-      LAUNCH("vertex_prep", k_vertex_prep<true>, zchunk_grid(vb), box_block(), S.g, vb, S.rho, S.energy, S.bx, S.by,
-             S.bz, S.gamma_boris, S.rho_v, S.cs_v, S.p.gamma);
-    else
-      LAUNCH("vertex_prep", k_vertex_prep<false>, zchunk_grid(vb), box_block(), S.g, vb, S.rho, S.energy, S.bx, S.by,
-             S.bz, S.gamma_boris, S.rho_v, S.cs_v, S.p.gamma);
+    const EosArgs eq = eos_args();
+#define VERTEX_PREP(GB, PI)                                                                                        \
+  LAUNCH("vertex_prep", (k_vertex_prep<GB, PI>), zchunk_grid(vb), box_block(), S.g, vb, S.rho, S.energy, S.bx, S.by, \
+         S.bz, S.gamma_boris, S.rho_v, S.cs_v, S.p.gamma, eq)
+    if (S.gamma_boris) {
+      if (S.pi_eos) VERTEX_PREP(true, true);
+      else VERTEX_PREP(true, false);
+    } else {
+      if (S.pi_eos) VERTEX_PREP(false, true);
+      else VERTEX_PREP(false, false);
+    }
+#undef VERTEX_PREP
   }
   if (S.p.use_viscous_damping) {
     LAUNCH_BOX("damp_q", k_damp_q, nodes0n(), S.vx, S.vy, S.vz, S.visc3, S.qx, S.qy, S.qz);
@@ -721,8 +880,12 @@ int lagrangian_step() {
   }
   const Box vdb = mkbox(0, N0 + 1, 0, N1 + 1, 0, N2 + 1);
   dim3 gr = zchunk_grid(vdb);
-  LAUNCH("visc_dt", k_visc_dt, gr, box_block(), S.g, vdb, S.vx, S.vy, S.vz, S.rho, S.energy, S.bx, S.by, S.bz, S.eta,
-         S.al1, S.al2, S.al3, S.visc_heat, S.p0, S.gamma_boris, S.partial, a);
+  if (S.pi_eos)
+    LAUNCH("visc_dt", k_visc_dt<true>, gr, box_block(), S.g, vdb, S.vx, S.vy, S.vz, S.rho, S.energy, S.bx, S.by, S.bz,
+           S.eta, S.al1, S.al2, S.al3, S.visc_heat, S.p0, S.gamma_boris, S.partial, a, eos_args());
+  else
+    LAUNCH("visc_dt", k_visc_dt<false>, gr, box_block(), S.g, vdb, S.vx, S.vy, S.vz, S.rho, S.energy, S.bx, S.by, S.bz,
+           S.eta, S.al1, S.al2, S.al3, S.visc_heat, S.p0, S.gamma_boris, S.partial, a, eos_args());
   int rc = final_reduce(gr, 3, 7, 0, 0);
   if (rc) return rc;
   double mins[3];
@@ -760,14 +923,29 @@ int lagrangian_step() {
     if ((size_t)g2.x * g2.y * g2.z > S.partial_cap)
       return fail(LARE_E_ARG, "resistive grid of %u blocks exceeds the partial buffer", g2.x * g2.z);
     for (int sc = 1; sc <= substeps; ++sc) {
+      // eta_calc runs inside the substep kernels; neutral_fraction and
+      // perpendicular_resistivity read nothing it writes (lagran.F90:146-149)
+      if (S.pi_eos) {
+        rc = neutral_fraction(S.energy, TMODE_DIRECT);
+        if (rc) return rc;
+      }
+      if (S.eta_perp) {
+        rc = perpendicular_resistivity();
+        if (rc) return rc;
+      }
       if (S.p.fourth_order) {
         rc = resist_rk4_substep();
         if (rc) return rc;
         continue;
       }
-      LAUNCH("resist", k_resist, g2, dim3(64, RS_ROWS, 1), S.g, fb, tiles_x, S.bx, S.by, S.bz, S.fxr, S.fyr, S.fzr,
-             S.eta, S.energy, S.ohmic_dep, S.rho, S.curlb, S.partial, S.dt, S.p.eta_background, S.p.eta0,
-             S.p.j_max);
+      if (S.eta_perp)
+        LAUNCH("resist", k_resist<true>, g2, dim3(64, RS_ROWS, 1), S.g, fb, tiles_x, S.bx, S.by, S.bz, S.fxr, S.fyr,
+               S.fzr, S.eta, S.energy, S.ohmic_dep, S.rho, S.curlb, S.partial, S.dt, S.p.eta_background, S.p.eta0,
+               S.p.j_max, S.eta_perp);
+      else
+        LAUNCH("resist", k_resist<false>, g2, dim3(64, RS_ROWS, 1), S.g, fb, tiles_x, S.bx, S.by, S.bz, S.fxr, S.fyr,
+               S.fzr, S.eta, S.energy, S.ohmic_dep, S.rho, S.curlb, S.partial, S.dt, S.p.eta_background, S.p.eta0,
+               S.p.j_max, S.eta_perp);
       rc = final_reduce(g2, 1, 0, 9, 1);
       if (rc) return rc;
       if (S.curlb)
@@ -807,7 +985,7 @@ int lagrangian_step() {
     // predictor-B arrays), cv1 and the predictor pressure; no predictor remap
     real *b1[3] = {(real *)S.bpx, (real *)S.bpy, (real *)S.bpz};
     LAUNCH_BOX("cauchy_b", k_cauchy_b, cell_box(), S.vx, S.vy, S.vz, S.bx, S.by, S.bz, S.cv1, b1[0], b1[1], b1[2],
-               S.energy, S.rho, S.visc_heat, S.p0, S.pp, S.p.gamma, S.dt2);
+               S.energy, S.rho, S.visc_heat, S.p0, S.pp, S.p.gamma, S.dt2, eos_args());
     LAUNCH("force", k_force<true>, zchunk_grid(nodes0n()), box_block(), S.g, nodes0n(), S.vx, S.vy, S.vz, S.pp,
            S.bx, S.by, S.bz, S.rho_v, S.al1, S.al2, S.al3, S.gamma_boris, S.vx1, S.vy1, S.vz1, S.dt2, S.cv1, b1[0],
            b1[1], b1[2]);
@@ -828,6 +1006,10 @@ int lagrangian_step() {
     rc = apply_bcs(f, 3, "bc_v1");
     if (rc) return rc;
   }
+  if (S.any_open) {
+    rc = open_save();
+    if (rc) return rc;
+  }
   dim3 g3 = zchunk_grid(interior());
   if (S.cool_v)
     LAUNCH("shock_corr", k_shock_corr<true>, g3, box_block(), S.g, interior(), S.vx, S.vy, S.vz, S.vx1, S.vy1, S.vz1,
@@ -846,11 +1028,27 @@ int lagrangian_step() {
   }
   // the new velocities went to the momentum-flux arrays, which swap roles
   // with the velocity arrays; the boundary fill below sets their ghosts
+  if (S.any_open) {
+    LAUNCH_BOX("copy_open", k_copy_open, full_box(), S.vx, S.Fx, 1);
+    LAUNCH_BOX("copy_open", k_copy_open, full_box(), S.vy, S.Fy, 1);
+    LAUNCH_BOX("copy_open", k_copy_open, full_box(), S.vz, S.Fz, 1);
+  }
   {
     real *tv;
     tv = S.vx; S.vx = S.Fx; S.Fx = tv;
     tv = S.vy; S.vy = S.Fy; S.Fy = tv;
     tv = S.vz; S.vz = S.Fz; S.Fz = tv;
+  }
+  if (S.any_open) {
+    // velocity_bcs, open_bcs, then (with the end of lagrangian_step)
+    // energy_bcs and density_bcs (lagran.F90:408-412, :196-198)
+    BcField fv[3] = {{S.vx, FK_NODE}, {S.vy, FK_NODE}, {S.vz, FK_NODE}};
+    rc = apply_bcs(fv, 3, "bc_lagrangian");
+    if (rc) return rc;
+    rc = open_bcs();
+    if (rc) return rc;
+    BcField fc[2] = {{S.energy, FK_CELL}, {S.rho, FK_CELL}};
+    return apply_bcs(fc, 2, "bc_lagrangian");
   }
   {
     BcField f[5] = {{S.energy, FK_CELL}, {S.rho, FK_CELL}, {S.vx, FK_NODE}, {S.vy, FK_NODE}, {S.vz, FK_NODE}};
@@ -947,18 +1145,42 @@ int api_init(const lare_grid *grid, const lare_params *params) {
   for (int a = 0; a < 3; ++a) {
     if (n[a] < 4) return fail(LARE_E_ARG, "grid size %d on axis %d is below 4", n[a], a);
     const int lo = bc[2 * a], hi = bc[2 * a + 1];
-    if (lo != hi || (lo != LARE_BC_PERIODIC && lo != LARE_BC_USER))
-      return fail(LARE_E_ARG, "axis %d: boundary codes (%d, %d) unsupported (need both periodic or both BC_USER)", a, lo, hi);
+    const bool ok = (lo == LARE_BC_PERIODIC && hi == LARE_BC_PERIODIC) ||
+                    ((lo == LARE_BC_USER || lo == LARE_BC_OPEN) && (hi == LARE_BC_USER || hi == LARE_BC_OPEN));
+    if (!ok)
+      return fail(LARE_E_ARG,
+                  "axis %d: boundary codes (%d, %d) unsupported (both periodic, or each side BC_USER or BC_OPEN)", a,
+                  lo, hi);
   }
   if (!grid->dxb || !grid->dxc || !grid->dyb || !grid->dyc || !grid->dzb || !grid->dzc)
     return fail(LARE_E_ARG, "lare_dev_init: metric arrays missing");
   memcpy(S.n, n, sizeof(n));
   memcpy(S.bc, bc, sizeof(bc));
+  S.any_open = false;
   for (int a = 0; a < 3; ++a) {
     S.per[a] = bc[2 * a] == LARE_BC_PERIODIC;
-    S.user[a] = bc[2 * a] == LARE_BC_USER;
+    S.user[a] = !S.per[a];
+    S.any_open = S.any_open || bc[2 * a] == LARE_BC_OPEN || bc[2 * a + 1] == LARE_BC_OPEN;
   }
+  const int eos = params->eos_number == 0 ? LARE_EOS_IDEAL : params->eos_number;
+  if (eos != LARE_EOS_IDEAL && eos != LARE_EOS_ION && eos != LARE_EOS_PI)
+    return fail(LARE_E_ARG, "eos_number %d unknown", params->eos_number);
+  const bool pi = eos != LARE_EOS_IDEAL;
+  if (params->cowling_resistivity && !pi)
+    return fail(LARE_E_ARG, "cowling_resistivity needs a partially ionised eos_number (control.f90:158-161)");
+  if (pi && sizeof(real) != sizeof(double))
+    return fail(LARE_E_ARG, "the partially ionised equation of state runs in the FP64 mode only");
+  for (int q = 0; q < 6; ++q)
+    if (bc[q] == LARE_BC_OPEN && sizeof(real) != sizeof(double))
+      return fail(LARE_E_ARG, "BC_OPEN boundaries run in the FP64 mode only");
+  if (pi && !(params->f_bar > 0.0 && params->t_rad > 0.0 && params->r_bar > 0.0 && params->t_bar >= 0.0))
+    return fail(LARE_E_ARG, "partially ionised: f_bar, t_rad and r_bar must be > 0, t_bar >= 0");
+  if (pi && params->cowling_resistivity && !(params->eta_bar >= 0.0))
+    return fail(LARE_E_ARG, "cowling_resistivity: eta_bar must be >= 0");
   S.p = *params;
+  S.p.eos_number = eos;
+  S.pi_eos = pi;
+  S.ip = eos == LARE_EOS_ION ? params->ionise_pot : 0.0;
   S.tfac = (params->gamma - 1.0) / (2.0 - params->xi_n);
   S.visc2_norm = 0.25 * (params->gamma + 1.0) * params->visc2;
   S.sy = ((n[0] + 19 + 15) / 16) * 16;
@@ -989,6 +1211,8 @@ int api_init(const lare_grid *grid, const lare_params *params) {
   for (int a = 0; a < 3; ++a) {
     S.g.per[a] = S.per[a];
     S.g.user[a] = S.user[a];
+    S.g.open_lo[a] = bc[2 * a] == LARE_BC_OPEN;
+    S.g.open_hi[a] = bc[2 * a + 1] == LARE_BC_OPEN;
   }
   real **bulk[] = {&S.rho, &S.rho_alt, &S.energy, &S.vx, &S.vy, &S.vz, &S.vx1, &S.vy1, &S.vz1, &S.eta,
                     &S.visc3, &S.visc_dep, &S.ohmic_dep, &S.rho_v, &S.cs_v, &S.al1, &S.al2, &S.al3,
@@ -1018,6 +1242,25 @@ int api_init(const lare_grid *grid, const lare_params *params) {
     int rc = dalloc(&S.delta_ke);
     for (int q = 0; q < 3 && !rc; ++q) rc = dalloc(&S.rk.dk[q]);
     if (rc) return rc;
+  }
+  S.xi = S.eta_perp = nullptr;
+  S.nf_flag = nullptr;
+  S.obc_save = nullptr;
+  S.obc_face = 0;
+  if (S.any_open) {
+    const size_t e = (size_t)(nmax + 2) * (size_t)(nmax + 2);
+    S.obc_face = 2 * e;
+    HIPCHK(hipMalloc((void **)&S.obc_save, sizeof(double) * 6 * S.obc_face));
+    S.allocs.push_back(S.obc_save);
+  }
+  if (pi) {
+    // xi_n = 0 until the first neutral_fraction (neutral.f90:38)
+    int rc = dalloc(&S.xi);
+    if (!rc && params->cowling_resistivity) rc = dalloc(&S.eta_perp);
+    if (rc) return rc;
+    HIPCHK(hipMalloc((void **)&S.nf_flag, sizeof(int)));
+    S.allocs.push_back(S.nf_flag);
+    HIPCHK(hipMemsetAsync(S.nf_flag, 0, sizeof(int), S.stream));
   }
   if (params->boris) {
     int rc = dalloc(&S.gamma_boris);
@@ -1141,7 +1384,9 @@ int api_set_scalars(const lare_scalars *in) {
 
 int api_lagrangian_step(void) {
   REQUIRE_LOADED();
-  return lagrangian_step();
+  int rc = lagrangian_step();
+  if (rc) return rc;
+  return check_neutral_fault("lagrangian_step");
 }
 
 int api_eulerian_remap(int32_t step) {
@@ -1159,12 +1404,49 @@ int api_conduct_heat(void) {
   if (S.p.kappa_0 <= 0.0) return fail(LARE_E_ARG, "conduct_heat needs kappa_0 > 0");
   int rc = calc_s_stages();
   if (rc) return rc;
-  return heat_conduct_sts2(0);
+  rc = heat_conduct_sts2(0);
+  if (rc) return rc;
+  return check_neutral_fault("conduct_heat");
 }
 
 int api_boundary_conditions(void) {
   REQUIRE_LOADED();
   return boundary_conditions();
+}
+
+int api_neutral_fraction(void) {
+  REQUIRE_LOADED();
+  if (!S.pi_eos) return fail(LARE_E_STATE, "neutral_fraction needs eos_number EOS_ION or EOS_PI");
+  int rc = neutral_fraction(S.energy, TMODE_DIRECT);
+  if (rc) return rc;
+  return check_neutral_fault("neutral_fraction");
+}
+
+int api_perpendicular_resistivity(void) {
+  REQUIRE_LOADED();
+  if (!S.eta_perp) return fail(LARE_E_STATE, "perpendicular_resistivity needs cowling_resistivity");
+  return perpendicular_resistivity();
+}
+
+// xi_n and eta_perp (0..n, zero elsewhere) in the cell layout
+int api_get_neutral(double *xi_h, double *ep_h) {
+  REQUIRE_LOADED();
+  if ((xi_h && !S.xi) || (ep_h && !S.eta_perp))
+    return fail(LARE_E_STATE, "get_neutral: the partially ionised arrays are off");
+  HIPCHK(hipStreamSynchronize(S.stream));
+  int rc = 0;
+  if (xi_h) rc = xfer(S.xi, xi_h, FK_CELL, false);
+  if (!rc && ep_h) {
+    rc = xfer(S.eta_perp, ep_h, FK_CELL, false);
+    if (rc) return rc;
+    // the array exists at nodes 0..n only (neutral.f90:73-131)
+    const long ex = nx() + 4, ey = ny() + 4, ez = nz() + 4;
+    for (long k = 0; k < ez; ++k)
+      for (long j = 0; j < ey; ++j)
+        for (long i = 0; i < ex; ++i)
+          if (i < 1 || i > nx() + 1 || j < 1 || j > ny() + 1 || k < 1 || k > nz() + 1) ep_h[(k * ey + j) * ex + i] = 0.0;
+  }
+  return rc;
 }
 
 int api_energy_account(double en[3], double *divb) {
@@ -1202,7 +1484,12 @@ int api_energy_correction() {
 int api_step(int32_t step, double *dt, double *time) {
   REQUIRE_LOADED();
   S.step = step;
-  int rc = lagrangian_step();
+  int rc = 0;
+  if (S.pi_eos) {
+    rc = neutral_fraction(S.energy, TMODE_DIRECT);
+    if (rc) return rc;
+  }
+  rc = lagrangian_step();
   if (rc) return rc;
   rc = eulerian_remap(step, false);
   if (rc) return rc;
@@ -1211,6 +1498,8 @@ int api_step(int32_t step, double *dt, double *time) {
     if (rc) return rc;
   }
   rc = eta_calc();
+  if (rc) return rc;
+  rc = check_neutral_fault("step");
   if (rc) return rc;
   if (!isfinite(S.dt) || (S.dt <= 0.0 && !S.p.restart))
     return fail(LARE_F_NONFINITE, "step %d: invalid dt %g", step, S.dt);
@@ -1718,6 +2007,8 @@ int api_sur_grad(const float *X, const float *t, int32_t rows, float *grad, floa
 
 int api_sur_enable(int32_t on, double tol) {
   if (!(tol > 0.0)) return fail(LARE_E_ARG, "surrogate tolerance must be positive");
+  if (on && S.pi_eos) return fail(LARE_E_ARG, "the conduction surrogate is trained for EOS_IDEAL only");
+  if (on && S.any_open) return fail(LARE_E_ARG, "the conduction surrogate does not handle BC_OPEN sides");
   if (on && (!SU.have_params || !SU.have_norm)) return fail(LARE_E_STATE, "surrogate enable: no trained parameters");
   SU.enabled = on != 0;
   SU.tol = tol;
@@ -1798,6 +2089,7 @@ int api_potential_field(const lare_potential *pf, int32_t *iterations, double *r
   if (!pf || !pf->flux0) return fail(LARE_E_ARG, "potential_field: NULL argument");
   const int N0 = nx(), N1 = ny(), N2 = nz();
   if (S.per[2]) return fail(LARE_E_ARG, "potential_field: the z axis must be BC_USER (flux patch below)");
+  if (S.any_open) return fail(LARE_E_ARG, "potential_field: BC_OPEN sides are not supported");
   if (N2 > N1 + 2)
     return fail(LARE_E_ARG, "potential_field: nz = %d > ny + 2 = %d (the reference reads bz(1:nx, 1:nz, 0))", N2,
                 N1 + 2);

@@ -7,6 +7,8 @@ the MI355X through the C-ABI library (include/lare3d_dev.h).
     dev.eta_calc()                         # lagran.F90:1176
     dev.conduct_heat()                     # conduct.f90:39
     dev.boundary_conditions()              # boundary.f90:48
+    dev.neutral_fraction()                 # neutral.f90:158 (EOS_ION / EOS_PI)
+    dev.perpendicular_resistivity()        # neutral.f90:68 (Cowling)
     dev.energy_account()                   # diagnostics.F90:517
     dev.step(step)                         # lare3d.f90:84-87
 
@@ -94,6 +96,24 @@ class Device:
 
     def boundary_conditions(self):
         check(self.lib, self.lib.lare_dev_boundary_conditions(), "boundary_conditions")
+
+    def neutral_fraction(self):
+        """neutral.f90:158-195 over the cell box (EOS_ION / EOS_PI)."""
+        check(self.lib, self.lib.lare_dev_neutral_fraction(), "neutral_fraction")
+
+    def perpendicular_resistivity(self):
+        """neutral.f90:68-136 at nodes 0..n (cowling_resistivity)."""
+        check(self.lib, self.lib.lare_dev_perpendicular_resistivity(), "perpendicular_resistivity")
+
+    def neutral(self, xi=True, eta_perp=False):
+        """(xi_n, eta_perp) in the (-1:n+2)^3 layout; None where not asked."""
+        shape = tuple(n + 4 for n in self.n)
+        a = np.zeros(shape, order="F") if xi else None
+        b = np.zeros(shape, order="F") if eta_perp else None
+        dp = C.POINTER(C.c_double)
+        check(self.lib, self.lib.lare_dev_get_neutral(None if a is None else a.ctypes.data_as(dp),
+                                                      None if b is None else b.ctypes.data_as(dp)), "get_neutral")
+        return a, b
 
     def energy_account(self, with_divb=False):
         en = (C.c_double * 3)()

@@ -26,6 +26,8 @@ MODULE lare_gpu_bridge
     INTEGER(C_INT32_T) :: use_viscous_damping, restart, precision
     REAL(C_DOUBLE) :: alpha_av, va_max
     INTEGER(C_INT32_T) :: cooling_term, boris, shock_limiter, shock_compression, rke, cauchy, fourth_order
+    INTEGER(C_INT32_T) :: eos_number, cowling_resistivity, pad2_
+    REAL(C_DOUBLE) :: ionise_pot, f_bar, t_bar, r_bar, eta_bar, t_rad
   END TYPE lare_params
 
   TYPE, BIND(C) :: lare_fields
@@ -79,6 +81,17 @@ MODULE lare_gpu_bridge
     END FUNCTION
     INTEGER(C_INT) FUNCTION lare_dev_boundary_conditions() BIND(C, NAME='lare_dev_boundary_conditions')
       IMPORT :: C_INT
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_neutral_fraction() BIND(C, NAME='lare_dev_neutral_fraction')
+      IMPORT :: C_INT
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_perpendicular_resistivity() &
+        BIND(C, NAME='lare_dev_perpendicular_resistivity')
+      IMPORT :: C_INT
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION lare_dev_get_neutral(xi_n, eta_perp) BIND(C, NAME='lare_dev_get_neutral')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: xi_n, eta_perp
     END FUNCTION
     INTEGER(C_INT) FUNCTION lare_dev_energy_account(en, divb) BIND(C, NAME='lare_dev_energy_account')
       IMPORT :: C_INT, C_DOUBLE

@@ -93,6 +93,9 @@ PROGRAM ot_driver
   params%alpha_av = 0.05_C_DOUBLE; params%va_max = 10.0_C_DOUBLE
   params%cooling_term = 0; params%boris = 0; params%shock_limiter = 0; params%shock_compression = 0
   params%rke = 0; params%cauchy = 0; params%fourth_order = 0
+  params%eos_number = 1; params%cowling_resistivity = 0; params%pad2_ = 0
+  params%ionise_pot = 0.0_C_DOUBLE; params%f_bar = 0.0_C_DOUBLE; params%t_bar = 0.0_C_DOUBLE
+  params%r_bar = 0.0_C_DOUBLE; params%eta_bar = 0.0_C_DOUBLE; params%t_rad = 0.0_C_DOUBLE
 
   fields%rho = C_LOC(rho); fields%energy = C_LOC(energy)
   fields%vx = C_LOC(vx); fields%vy = C_LOC(vy); fields%vz = C_LOC(vz)

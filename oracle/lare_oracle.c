@@ -8,11 +8,14 @@
  * order of floating-point operations follow the reference; compile with
  * -ffp-contract=off so that no multiply-add is fused.
  *
- * Scope (as the device implementation): EOS_IDEAL with uniform xi_n and zero
- * ionisation potential, boris off, cooling_term off, rke off, no open
- * boundaries, no CAUCHY/SHOCKLIMITER/SHOCKCOMPRESSION/FOURTHORDER; each axis
- * is either periodic on both sides or BC_USER on both sides; one MPI rank
- * (so every MPI_SENDRECV is the periodic self-exchange of mpiboundary.f90).
+ * Scope (as the device implementation): the three equations of state of
+ * control.f90:187-199 (EOS_IDEAL with a uniform xi_n; EOS_ION and EOS_PI
+ * with the per-cell neutral fraction and the Cowling resistivity of
+ * neutral.f90), the run-time options of lare_params (boris, cooling_term,
+ * rke, the CAUCHY, SHOCKLIMITER, SHOCKCOMPRESSION and FOURTHORDER
+ * variants), no open boundaries; each axis is either periodic on both
+ * sides or BC_USER on both sides; one MPI rank (so every MPI_SENDRECV is
+ * the periodic self-exchange of mpiboundary.f90).
  */
 #include "lare_oracle.h"
 
@@ -72,6 +75,12 @@ struct orc_state {
   double *gamma_boris;     /* gamma_boris(-1:n+2)^3, mpi_routines.F90:311 */
   double *cool_term_v, *cool_term_b; /* (1:n)^3, mpi_routines.F90:315-319 */
   double *delta_ke;        /* delta_ke(-1:n+2)^3 when rke, mpi_routines.F90:313 */
+  double *xi_n;            /* xi_n(-1:n+2)^3 with EOS_ION / EOS_PI (neutral.f90:37) */
+  double *eta_perp;        /* eta_perp, set at 0..n, with cowling (neutral.f90:41) */
+  double ip;               /* ionise_pot: 0 unless EOS_ION (normalise.f90:48) */
+  int pi_eos;              /* EOS_ION or EOS_PI */
+  int any_open;            /* some side is BC_OPEN (boundary.f90:35-38) */
+  int fault;               /* LARE_F_TBOUNDS once neutral_fraction's bracket failed */
   /* lagran module arrays live only within one lagrangian_step */
   double *bx1, *by1, *bz1, *pressure, *rho_vl, *cv_vl, *visc_heat;
   double *alpha1, *alpha2, *alpha3, *fx_visc, *fy_visc, *fz_visc;
@@ -725,11 +734,143 @@ void orc_eulerian_remap(orc_state *o, int step) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* Partially ionised plasma (neutral.f90)                                    */
+/* ------------------------------------------------------------------------ */
+
+/* exp(x): Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2 (ln2 split in a
+ * 32-bit head, so k*LN2_HI is exact, and a tail), the Taylor polynomial of
+ * degree 13 in r by Horner's rule with fused multiply-adds (truncation below
+ * 0.05 ulp), and the exact scaling by 2^k.  Within an ulp of the libm exp
+ * that the reference's EXP calls (neutral.f90:151-152); written out so that
+ * the device computes the same operations and hence the same bits. */
+#define LN2_HI 0x1.62e42feep-1
+#define LN2_LO 0x1.a39ef35793c76p-33
+#define INV_LN2 0x1.71547652b82fep+0
+double orc_exp(double x) {
+  if (x != x) return x;
+  if (x > 710.0) return INFINITY;
+  if (x < -746.0) return 0.0;
+  const double k = rint(x * INV_LN2);
+  double r = fma(-k, LN2_HI, x);
+  r = fma(-k, LN2_LO, r);
+  double p = 0x1.6124613a86d09p-33;       /* 1/13! */
+  p = fma(p, r, 0x1.1eed8eff8d898p-29);   /* 1/12! */
+  p = fma(p, r, 0x1.ae64567f544e4p-26);   /* 1/11! */
+  p = fma(p, r, 0x1.27e4fb7789f5cp-22);   /* 1/10! */
+  p = fma(p, r, 0x1.71de3a556c734p-19);   /* 1/9! */
+  p = fma(p, r, 0x1.a01a01a01a01ap-16);   /* 1/8! */
+  p = fma(p, r, 0x1.a01a01a01a01ap-13);   /* 1/7! */
+  p = fma(p, r, 0x1.6c16c16c16c17p-10);   /* 1/6! */
+  p = fma(p, r, 0x1.1111111111111p-7);    /* 1/5! */
+  p = fma(p, r, 0x1.5555555555555p-5);    /* 1/4! */
+  p = fma(p, r, 0x1.5555555555555p-3);    /* 1/3! */
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return ldexp(p, (int)k);
+}
+
+/* neutral.f90:140-154 get_neutral: the Saha-type neutral fraction at
+ * temperature t_v and density rho_v, with the dilute photospheric radiation
+ * field (t_rad = tr, dilution 1/2) */
+double orc_saha_neutral(const lare_params *p, double t_v, double rho_v) {
+  const double t_rad = p->t_rad, dilution = 0.5;
+  const double bof = 1.0 / (dilution * p->f_bar * t_rad * sqrt(t_v)) *
+                     orc_exp((0.25 * (t_v / t_rad - 1.0) + 1.0) * p->t_bar / t_v);
+  const double r = 0.5 * (-1.0 + sqrt(1.0 + p->r_bar * rho_v * bof));
+  return r / (1.0 + r);
+}
+
+/* neutral.f90:158-195 neutral_fraction over the cell box -1..n+2: the
+ * temperature by bisection of T = (gamma-1)(e - (1-xi(T)) ionise_pot) /
+ * (2 - xi(T)) between (gamma-1) max((e - ionise_pot)/2, tiny) and
+ * (gamma-1) e, at most 100 halvings, stopping when the step or the residual
+ * falls below 1e-8; xi_n from the last probe.  An inverted bracket (the
+ * reference's STOP) sets the fault and leaves that cell's bisection to run
+ * its course. */
+static void neutral_fraction(orc_state *o) {
+  const double gm1 = o->p.gamma - 1.0, ip = o->ip;
+  for (int iz = -1; iz <= o->nz + 2; ++iz)
+    for (int iy = -1; iy <= o->ny + 2; ++iy)
+      for (int ix = -1; ix <= o->nx + 2; ++ix) {
+        const double rho0 = A(o->rho, ix, iy, iz), e0 = A(o->energy, ix, iy, iz);
+        const double ta1 = gm1 * dmax((e0 - ip) / 2.0, NONE_ZERO), ta2 = gm1 * e0;
+        if (ta1 > ta2) o->fault = LARE_F_TBOUNDS;
+        double dx = ta2 - ta1, t = ta1, x = t;
+        for (int loop = 1; loop <= 100; ++loop) {
+          dx = dx / 2.0;
+          x = t + dx;
+          const double xi_a = orc_saha_neutral(&o->p, x, rho0);
+          const double fa = x - gm1 * (e0 - (1.0 - xi_a) * ip) / (2.0 - xi_a);
+          if (fa <= 0.0) t = x;
+          if (fabs(dx) < 1.0e-8 || fabs(fa) < 1.0e-8) break;
+        }
+        A(o->xi_n, ix, iy, iz) = orc_saha_neutral(&o->p, x, rho0);
+      }
+}
+
+/* neutral.f90:68-136 perpendicular_resistivity at nodes 0..n: the volume
+ * averaged density, the temperature averaged over the eight cells, the
+ * neutral fraction there, and eta_perp = eta_bar xi/(1-xi) |B|^2 / rho^2 /
+ * sqrt(T), capped at 100.  The vertex field is the reference's: bxv and
+ * byv take bz at the upper faces (:97-98) where bx and by would be the
+ * symmetric choice. */
+static void perpendicular_resistivity(orc_state *o) {
+  const double gm1 = o->p.gamma - 1.0, ip = o->ip;
+  for (int iz = 0; iz <= o->nz; ++iz)
+    for (int iy = 0; iy <= o->ny; ++iy)
+      for (int ix = 0; ix <= o->nx; ++ix) {
+        const int ixp = ix + 1, iyp = iy + 1, izp = iz + 1;
+        const double *r = o->rho, *c = o->cv;
+        double rho_v = A(r, ix, iy, iz) * A(c, ix, iy, iz) + A(r, ixp, iy, iz) * A(c, ixp, iy, iz) +
+                       A(r, ix, iyp, iz) * A(c, ix, iyp, iz) + A(r, ixp, iyp, iz) * A(c, ixp, iyp, iz) +
+                       A(r, ix, iy, izp) * A(c, ix, iy, izp) + A(r, ixp, iy, izp) * A(c, ixp, iy, izp) +
+                       A(r, ix, iyp, izp) * A(c, ix, iyp, izp) + A(r, ixp, iyp, izp) * A(c, ixp, iyp, izp);
+        rho_v = rho_v / sum8(o, c, ix, iy, iz);
+        double bxv = A(o->bx, ix, iy, iz) + A(o->bx, ix, iyp, iz) + A(o->bz, ix, iy, izp) + A(o->bz, ix, iyp, izp);
+        double byv = A(o->by, ix, iy, iz) + A(o->by, ixp, iy, iz) + A(o->bz, ix, iy, izp) + A(o->bz, ixp, iy, izp);
+        double bzv = A(o->bz, ix, iy, iz) + A(o->bz, ixp, iy, iz) + A(o->bz, ix, iyp, iz) + A(o->bz, ixp, iyp, iz);
+        bxv = 0.25 * bxv;
+        byv = 0.25 * byv;
+        bzv = 0.25 * bzv;
+        const double bfieldsq = bxv * bxv + byv * byv + bzv * bzv;
+        double t_v = 0.0;
+        for (int iz2 = iz; iz2 <= iz + 1; ++iz2)
+          for (int iy2 = iy; iy2 <= iy + 1; ++iy2)
+            for (int ix2 = ix; ix2 <= ix + 1; ++ix2) {
+              const double xi = A(o->xi_n, ix2, iy2, iz2);
+              t_v = t_v + (A(o->energy, ix2, iy2, iz2) - (1.0 - xi) * ip) / (2.0 - xi);
+            }
+        t_v = t_v * gm1 / 8.0;
+        const double xi_v = orc_saha_neutral(&o->p, t_v, rho_v);
+        const double f = dmax(1.0 - xi_v, NONE_ZERO);
+        double ep = 0.0;
+        if (f > 0) ep = o->p.eta_bar * xi_v / f * bfieldsq / (rho_v * rho_v) / sqrt(t_v);
+        A(o->eta_perp, ix, iy, iz) = dmin(ep, 100.0);
+      }
+}
+
+int orc_neutral_fraction(orc_state *o) {
+  if (!o->xi_n) return LARE_E_STATE;
+  neutral_fraction(o);
+  return o->fault;
+}
+
+int orc_perpendicular_resistivity(orc_state *o) {
+  if (!o->eta_perp) return LARE_E_STATE;
+  perpendicular_resistivity(o);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Conduction (conduct.f90)                                                  */
 /* ------------------------------------------------------------------------ */
-static inline double temp_of(orc_state *o, double e) {
-  /* (gamma-1)/(2-xi)*(e - (1-xi)*ionise_pot), ionise_pot = 0 */
-  return (o->p.gamma - 1.0) / (2.0 - o->p.xi_n) * (e - (1.0 - o->p.xi_n) * 0.0);
+/* temperature of conduct.f90:72-73, (gamma-1)/(2-xi)*(e - (1-xi)*ionise_pot),
+ * with the neutral fraction of cell q (uniform xi_n under EOS_IDEAL) */
+static inline double xi_at(orc_state *o, long q) { return o->xi_n ? o->xi_n[q] : o->p.xi_n; }
+static inline double temp_at(orc_state *o, double e, long q) {
+  const double xi = xi_at(o, q);
+  return (o->p.gamma - 1.0) / (2.0 - xi) * (e - (1.0 - xi) * o->ip);
 }
 
 /* conduct.f90:56-143 calc_s_stages */
@@ -738,7 +879,7 @@ static void calc_s_stages(orc_state *o) {
   double k0 = o->p.kappa_0;
   for (int iz = -1; iz <= nz + 2; ++iz)
     for (int iy = -1; iy <= ny + 2; ++iy)
-      for (int ix = -1; ix <= nx + 2; ++ix) A(o->temperature, ix, iy, iz) = temp_of(o, A(o->energy, ix, iy, iz));
+      for (int ix = -1; ix <= nx + 2; ++ix) A(o->temperature, ix, iy, iz) = temp_at(o, A(o->energy, ix, iy, iz), IDX(ix, iy, iz));
   for (int iz = 0; iz <= nz; ++iz)
     for (int iy = 0; iy <= ny; ++iy)
       for (int ix = 0; ix <= nx; ++ix) {
@@ -761,7 +902,7 @@ static void calc_s_stages(orc_state *o) {
     for (int iy = 1; iy <= ny; ++iy)
       for (int ix = 1; ix <= nx; ++ix) {
         double kappa1 = k0 * A(o->larsen, ix, iy, iz);
-        double temp = temp_of(o, A(o->energy, ix, iy, iz));
+        double temp = temp_at(o, A(o->energy, ix, iy, iz), IDX(ix, iy, iz));
         double r = A(o->rho, ix, iy, iz);
         double tp = orc_pow25(temp);
         double dt1 = r * SQ(DXB(ix)) / (2.0 * kappa1 * tp);
@@ -871,7 +1012,7 @@ static void heat_conduct_sts2(orc_state *o) {
   for (int j = 0; j < 4; ++j) copy_all(o, Y[j], o->energy);
   for (int iz = -1; iz <= nz + 2; ++iz)
     for (int iy = -1; iy <= ny + 2; ++iy)
-      for (int ix = -1; ix <= nx + 2; ++ix) A(o->temperature, ix, iy, iz) = temp_of(o, A(Y[0], ix, iy, iz));
+      for (int ix = -1; ix <= nx + 2; ++ix) A(o->temperature, ix, iy, iz) = temp_at(o, A(Y[0], ix, iy, iz), IDX(ix, iy, iz));
   heat_flux(o, o->temperature, flux);
   for (int iz = 1; iz <= nz; ++iz)
     for (int iy = 1; iy <= ny; ++iy)
@@ -889,7 +1030,7 @@ static void heat_conduct_sts2(orc_state *o) {
   for (int j = 2; j <= s; ++j) {
     for (int iz = -1; iz <= nz + 2; ++iz)
       for (int iy = -1; iy <= ny + 2; ++iy)
-        for (int ix = -1; ix <= nx + 2; ++ix) A(o->temperature, ix, iy, iz) = temp_of(o, A(Y[2], ix, iy, iz));
+        for (int ix = -1; ix <= nx + 2; ++ix) A(o->temperature, ix, iy, iz) = temp_at(o, A(Y[2], ix, iy, iz), IDX(ix, iy, iz));
     temperature_bcs(o);
     heat_flux(o, o->temperature, flux);
     for (int iz = 1; iz <= nz; ++iz)
@@ -905,6 +1046,7 @@ static void heat_conduct_sts2(orc_state *o) {
       copy_all(o, Y[1], Y[2]);
       copy_all(o, o->energy, Y[3]);
       cell_bcs(o, o->energy);
+      if (o->pi_eos) neutral_fraction(o);
       copy_all(o, Y[2], o->energy);
       for (int iz = 1; iz <= nz; ++iz)
         for (int iy = 1; iy <= ny; ++iy)
@@ -977,10 +1119,12 @@ static inline void vertex_j(orc_state *o, int ix, int iy, int iz, double *jx, do
   *jz = (jz1 + jz2) * 0.5;
 }
 
-/* lagran.F90:1239-1452 resistive_effects (first order, no cowling) with
- * rkstep :1460-1503 and bstep :1582-1635. */
-/* rkstep (:1472-1503, no Cowling resistivity): EMFs and curlb at nodes 0..n
- * from the current B and eta */
+/* lagran.F90:1239-1452 resistive_effects with rkstep :1460-1580 and bstep
+ * :1582-1635. */
+/* rkstep: EMFs and curlb at nodes 0..n from the current B and eta.  With
+ * Cowling resistivity (:1504-1577) the vertex current splits into the part
+ * along the vertex field and the rest; eta_perp adds to the resistivity of
+ * the perpendicular part. */
 static void rkstep(orc_state *o, double *fxr, double *fyr, double *fzr, double *curlb) {
   int nx = o->nx, ny = o->ny, nz = o->nz;
   for (int iz = 0; iz <= nz; ++iz)
@@ -989,10 +1133,32 @@ static void rkstep(orc_state *o, double *fxr, double *fyr, double *fzr, double *
         double jx, jy, jz;
         vertex_j(o, ix, iy, iz, &jx, &jy, &jz);
         double e = A(o->eta, ix, iy, iz);
-        A(fxr, ix, iy, iz) = -jx * e * DXC(ix) * 0.5;
-        A(fyr, ix, iy, iz) = -jy * e * DYC(iy) * 0.5;
-        A(fzr, ix, iy, iz) = -jz * e * DZC(iz) * 0.5;
-        A(curlb, ix, iy, iz) = e * (jx * jx + jy * jy + jz * jz);
+        if (!o->p.cowling_resistivity) {
+          A(fxr, ix, iy, iz) = -jx * e * DXC(ix) * 0.5;
+          A(fyr, ix, iy, iz) = -jy * e * DYC(iy) * 0.5;
+          A(fzr, ix, iy, iz) = -jz * e * DZC(iz) * 0.5;
+          A(curlb, ix, iy, iz) = e * (jx * jx + jy * jy + jz * jz);
+          continue;
+        }
+        const int ixp = ix + 1, iyp = iy + 1, izp = iz + 1;
+        const double bxv = (A(o->bx, ix, iy, iz) + A(o->bx, ix, iyp, iz) + A(o->bx, ix, iy, izp) + A(o->bx, ix, iyp, izp)) * 0.25;
+        const double byv = (A(o->by, ix, iy, iz) + A(o->by, ixp, iy, iz) + A(o->by, ix, iy, izp) + A(o->by, ixp, iy, izp)) * 0.25;
+        const double bzv = (A(o->bz, ix, iy, iz) + A(o->bz, ixp, iy, iz) + A(o->bz, ix, iyp, iz) + A(o->bz, ixp, iyp, iz)) * 0.25;
+        const double magn_b = bxv * bxv + byv * byv + bzv * bzv;
+        double jpx = 0.0, jpy = 0.0, jpz = 0.0;
+        if (magn_b > NONE_ZERO) {
+          jpx = (jx * bxv + jy * byv + jz * bzv) * bxv / magn_b;
+          jpy = (jx * bxv + jy * byv + jz * bzv) * byv / magn_b;
+          jpz = (jx * bxv + jy * byv + jz * bzv) * bzv / magn_b;
+        }
+        const double jqx = jx - jpx, jqy = jy - jpy, jqz = jz - jpz;
+        const double mpar = sqrt(jpx * jpx + jpy * jpy + jpz * jpz);
+        const double mperp = sqrt(jqx * jqx + jqy * jqy + jqz * jqz);
+        const double ep = A(o->eta_perp, ix, iy, iz);
+        A(curlb, ix, iy, iz) = e * (mpar * mpar) + (ep + e) * (mperp * mperp);
+        A(fxr, ix, iy, iz) = -((jpx + jqx) * e + jqx * ep) * DXC(ix) * 0.5;
+        A(fyr, ix, iy, iz) = -((jpy + jqy) * e + jqy * ep) * DYC(iy) * 0.5;
+        A(fzr, ix, iy, iz) = -((jpz + jqz) * e + jqz * ep) * DZC(iz) * 0.5;
       }
 }
 
@@ -1147,6 +1313,165 @@ static void resistive_effects(orc_state *o, double *fxr, double *fyr, double *fz
   cell_bcs(o, o->energy);
   ohmic_total(o, curlb, dt);
   /* jx_r/jy_r/jz_r and the final rkstep (:1417-1445) only feed output. */
+}
+
+/* ------------------------------------------------------------------------ */
+/* Open boundaries (openboundary.f90)                                        */
+/* ------------------------------------------------------------------------ */
+
+/* The state of open_bcs for one boundary point: index 1 the interior values
+ * carried out by the Riemann invariants, index 0 the boundary values the
+ * solver sets, and the far field behind the boundary (openboundary.f90:23-27,
+ * with the normal direction first: "x" is the normal component). */
+typedef struct {
+  double vxbc[2], vybc[2], vzbc[2], bxbc[2], bybc[2], bzbc[2], rbc[2], ebc[2];
+  double pfar, rhofar, uxfar, uyfar, uzfar, bxfar, byfar, bzfar, pbc, vnorm;
+} obc_t;
+
+/* open_bcs_fast (openboundary.f90:327-382): a perpendicular far field */
+static void open_bcs_fast(obc_t *b, double gamma) {
+  const double c0far = sqrt(gamma * b->pfar / b->rhofar);
+  const double ctfar = sqrt((b->byfar * b->byfar + b->bzfar * b->bzfar) / b->rhofar);
+  const double cffar = sqrt(c0far * c0far + ctfar * ctfar);
+  const double c0 = sqrt(gamma * (gamma - 1.0) * b->ebc[1]);
+  const double ct = sqrt((b->bybc[1] * b->bybc[1] + b->bzbc[1] * b->bzbc[1]) / b->rbc[1]);
+  const double cf = sqrt(c0 * c0 + ct * ct);
+  const double vtest[3] = {b->vnorm + cf, b->vnorm - cf, b->vnorm};
+  double pstar[3], pmagstar[3], vstar[3], rhostar[3], bystar[3], bzstar[3];
+  for (int i = 0; i < 3; ++i) {
+    if (vtest[i] >= 0.0) {
+      pstar[i] = b->pbc + 0.5 * (b->bybc[1] * b->bybc[1] + b->bzbc[1] * b->bzbc[1]);
+      pmagstar[i] = 0.5 * (b->bybc[1] * b->bybc[1] + b->bzbc[1] * b->bzbc[1]);
+      vstar[i] = b->vxbc[1];
+      rhostar[i] = b->rbc[1];
+      bystar[i] = b->bybc[1];
+      bzstar[i] = b->bzbc[1];
+    } else {
+      pstar[i] = b->pfar + 0.5 * (b->byfar * b->byfar + b->bzfar * b->bzfar);
+      pmagstar[i] = 0.5 * (b->byfar * b->byfar + b->bzfar * b->bzfar);
+      vstar[i] = b->uxfar;
+      rhostar[i] = b->rhofar;
+      bystar[i] = b->byfar;
+      bzstar[i] = b->bzfar;
+    }
+  }
+  b->bxbc[0] = 0.5 * (b->bxbc[1] + b->bxfar);
+  b->bybc[0] = 0.5 * (bystar[0] + bystar[1]);
+  b->bzbc[0] = 0.5 * (bzstar[0] + bzstar[1]);
+  double pg = 0.5 * (pstar[0] + pstar[1] + b->rhofar * cffar * (vstar[0] - vstar[1])) - 0.5 * (pmagstar[0] + pmagstar[1]);
+  pg = dmax(pg, 0.0);
+  const double rhog = rhostar[2] + (pg - (pstar[2] - pmagstar[2])) / (c0far * c0far);
+  b->rbc[0] = dmax(rhog, NONE_ZERO);
+  b->ebc[0] = pg / (gamma - 1.0) / b->rbc[0];
+}
+
+/* open_bcs_alfven (openboundary.f90:386-455): a far field along the normal */
+static void open_bcs_alfven(obc_t *b, double gamma) {
+  const double lambdayfar = -b->bxfar * b->byfar, lambdazfar = -b->bxfar * b->bzfar;
+  const double c0far = sqrt(gamma * b->pfar / b->rhofar);
+  const double cxfar = sqrt(b->bxfar * b->bxfar / b->rhofar);
+  const double c0 = sqrt(gamma * (gamma - 1.0) * b->ebc[1]);
+  const double cx = sqrt(b->bxbc[1] * b->bxbc[1] / b->rbc[1]);
+  const double vtest[5] = {b->vnorm + c0, b->vnorm - c0, b->vnorm + cx, b->vnorm - cx, b->vnorm};
+  double pstar[5], pmagstar[5], lys[5], lzs[5], uxs[5], uys[5], uzs[5], rhostar[5];
+  for (int i = 0; i < 5; ++i) {
+    if (vtest[i] >= 0.0) {
+      pstar[i] = b->pbc + 0.5 * (b->bybc[1] * b->bybc[1] + b->bzbc[1] * b->bzbc[1] - b->bxbc[1] * b->bxbc[1]);
+      pmagstar[i] = 0.5 * (b->bybc[1] * b->bybc[1] + b->bzbc[1] * b->bzbc[1] - b->bxbc[1] * b->bxbc[1]);
+      lys[i] = -b->bybc[1] * b->bxbc[1];
+      lzs[i] = -b->bzbc[1] * b->bxbc[1];
+      uxs[i] = b->vxbc[1];
+      uys[i] = b->vybc[1];
+      uzs[i] = b->vzbc[1];
+      rhostar[i] = b->rbc[1];
+    } else {
+      pstar[i] = b->pfar + 0.5 * (b->byfar * b->byfar + b->bzfar * b->bzfar - b->bxfar * b->bxfar);
+      pmagstar[i] = 0.5 * (b->byfar * b->byfar + b->bzfar * b->bzfar - b->bxfar * b->bxfar);
+      lys[i] = lambdayfar;
+      lzs[i] = lambdazfar;
+      uxs[i] = b->uxfar;
+      uys[i] = b->uyfar;
+      uzs[i] = b->uzfar;
+      rhostar[i] = b->rhofar;
+    }
+  }
+  b->bxbc[0] = 0.5 * (b->bxbc[1] + b->bxfar);
+  if (fabs(b->bxbc[0]) <= NONE_ZERO) b->bxbc[0] = NONE_ZERO;
+  double lambdag = 0.5 * (lys[2] + lys[3] + b->rhofar * cxfar * (uys[2] - uys[3]));
+  b->bybc[0] = -lambdag / b->bxbc[0];
+  lambdag = 0.5 * (lzs[2] + lzs[3] + b->rhofar * cxfar * (uzs[2] - uzs[3]));
+  b->bzbc[0] = -lambdag / b->bxbc[0];
+  double pg = 0.5 * (pstar[0] + pstar[1] + b->rhofar * c0far * (uxs[0] - uxs[1])) -
+              0.5 * (b->bybc[0] * b->bybc[0] + b->bzbc[0] * b->bzbc[0] - b->bxbc[0] * b->bxbc[0]);
+  pg = dmax(pg, NONE_ZERO);
+  const double rhog = rhostar[4] + (pg - (pstar[4] - pmagstar[4])) / (c0far * c0far);
+  b->rbc[0] = dmax(rhog, NONE_ZERO);
+  b->ebc[0] = pg / (gamma - 1.0) / b->rbc[0];
+}
+
+/* element of arr at the transverse position q with layer l along axis a */
+static inline double *obc_at(orc_state *o, double *arr, int *q, int a, int l) {
+  q[a] = l;
+  return &A(arr, q[0], q[1], q[2]);
+}
+
+/* open_bcs (openboundary.f90:30-323): the six faces in the reference's
+ * order, each over the transverse points 0..n+1.  Face (a, side): the
+ * normal component is axis a, signed so that it points out of the domain
+ * (the low side flips it); the two transverse slots are the reference's:
+ * for x (y, z), for y (x, z), for z (y, x).  Layers along a: the boundary
+ * node and normal face (0 or n), the first interior cell (1 or n), the far
+ * field in the second ghost layer (-1 / -2 or n+2), and the values set in
+ * the first ghost layer (normal face -1 or n+1, cells 0 or n+1). */
+static void open_bcs(orc_state *o) {
+  const double gm = o->p.gamma;
+  double *v[3] = {o->vx, o->vy, o->vz}, *bf[3] = {o->bx, o->by, o->bz};
+  for (int a = 0; a < 3; ++a)
+    for (int hi = 0; hi <= 1; ++hi) {
+      if (o->bc[2 * a + hi] != LARE_BC_OPEN) continue;
+      const int n = nax(o, a);
+      const double sg = hi ? 1.0 : -1.0;
+      /* transverse slots (y, z) of the face */
+      const int ty = a == 1 ? 0 : 1, tz = a == 2 ? 0 : 2;
+      const int lb = hi ? n : 0, lc = hi ? n : 1, lfc = hi ? n + 2 : -1, lfn = hi ? n + 2 : -2;
+      const int lset_n = hi ? n + 1 : -1, lset_c = hi ? n + 1 : 0;
+      const int u = a == 0 ? 1 : 0, w = a == 2 ? 1 : 2;
+      const int nu = nax(o, u), nw = nax(o, w);
+      for (int jw = 0; jw <= nw + 1; ++jw)
+        for (int ju = 0; ju <= nu + 1; ++ju) {
+          int q[3];
+          q[u] = ju; q[w] = jw;
+#define AT(arr, l) (*obc_at(o, (arr), q, a, (l)))
+          obc_t b;
+          b.vxbc[1] = sg * AT(v[a], lb);
+          b.vybc[1] = AT(v[ty], lb);
+          b.vzbc[1] = AT(v[tz], lb);
+          b.bxbc[1] = sg * AT(bf[a], lb);
+          b.bybc[1] = AT(bf[ty], lc);
+          b.bzbc[1] = AT(bf[tz], lc);
+          b.rbc[1] = AT(o->rho, lc);
+          b.ebc[1] = AT(o->energy, lc);
+          b.pbc = (gm - 1.0) * AT(o->energy, lc) * AT(o->rho, lc);
+          b.pfar = (gm - 1.0) * AT(o->energy, lfc) * AT(o->rho, lfc);
+          b.uxfar = sg * AT(v[a], lfn);
+          b.uyfar = AT(v[ty], lfn);
+          b.uzfar = AT(v[tz], lfn);
+          b.bxfar = sg * AT(bf[a], lfn);
+          b.byfar = AT(bf[ty], lfc);
+          b.bzfar = AT(bf[tz], lfc);
+          b.rhofar = AT(o->rho, lfc);
+          b.vnorm = sg * AT(v[a], lb);
+          const double bperp = sqrt(b.byfar * b.byfar + b.bzfar * b.bzfar);
+          if (fabs(bperp) <= NONE_ZERO) open_bcs_alfven(&b, gm);
+          else open_bcs_fast(&b, gm);
+          AT(bf[a], lset_n) = sg * b.bxbc[0];
+          AT(bf[ty], lset_c) = b.bybc[0];
+          AT(bf[tz], lset_c) = b.bzbc[0];
+          AT(o->rho, lset_c) = b.rbc[0];
+          AT(o->energy, lset_c) = b.ebc[0];
+#undef AT
+        }
+    }
 }
 
 /* lagran.F90:483-524 viscous_damping */
@@ -1398,7 +1723,10 @@ static void set_dt(orc_state *o) {
                           DZB(iz) / dmax(fabs(A(o->vz, ix, iy, iz)), NONE_ZERO));
         dt_local = dmin(dt_local, dt2);
         double dxlocal = 1.0 / (1.0 / SQ(DXB(ix)) + 1.0 / SQ(DYB(iy)) + 1.0 / SQ(DZB(iz)));
-        if (o->p.resistive_mhd) dt3 = 0.2 * dxlocal / dmax(A(o->eta, ix, iy, iz), NONE_ZERO);
+        if (o->p.cowling_resistivity)
+          dt3 = 0.2 * dxlocal / dmax(dmax(A(o->eta, ix, iy, iz), A(o->eta_perp, ix, iy, iz)), NONE_ZERO);
+        else if (o->p.resistive_mhd)
+          dt3 = 0.2 * dxlocal / dmax(A(o->eta, ix, iy, iz), NONE_ZERO);
         dtr_local = dmin(dtr_local, dt3);
       }
   o->dt = DT_MULT * dt_local;
@@ -1512,7 +1840,7 @@ static void cauchy_jb(orc_state *o, int ix, int iy, int iz, double *jx, double *
 
 static void predictor_corrector_step(orc_state *o) {
   int nx = o->nx, ny = o->ny, nz = o->nz;
-  double gm1 = o->p.gamma - 1.0, xi = o->p.xi_n;
+  double gm1 = o->p.gamma - 1.0;
   const int cauchy = o->p.cauchy;
   double *bx0 = NULL, *by0 = NULL, *bz0 = NULL;
   if (!cauchy) {
@@ -1573,7 +1901,7 @@ static void predictor_corrector_step(orc_state *o) {
         double dv = A(o->cv1, ix, iy, iz) / A(o->cv, ix, iy, iz) - 1.0;
         double e1 = A(o->energy, ix, iy, iz) - A(o->pressure, ix, iy, iz) * dv / r;
         e1 = e1 + A(o->visc_heat, ix, iy, iz) * o->dt2 / r;
-        A(o->pressure, ix, iy, iz) = (e1 - (1.0 - xi) * 0.0) * gm1 * r * A(o->cv, ix, iy, iz) / A(o->cv1, ix, iy, iz);
+        A(o->pressure, ix, iy, iz) = (e1 - (1.0 - xi_at(o, IDX(ix, iy, iz))) * o->ip) * gm1 * r * A(o->cv, ix, iy, iz) / A(o->cv1, ix, iy, iz);
       }
 
   for (int iz = 0; iz <= nz; ++iz) {
@@ -1632,6 +1960,7 @@ static void predictor_corrector_step(orc_state *o) {
         A(o->vz, ix, iy, iz) = 2.0 * A(o->vz1, ix, iy, iz) - A(o->vz, ix, iy, iz);
       }
   velocity_bcs(o);
+  if (o->any_open) open_bcs(o);
 
   for (int iz = 1; iz <= nz; ++iz) {
     int izm = iz - 1;
@@ -1677,7 +2006,7 @@ static void predictor_corrector_step(orc_state *o) {
 /* lagran.F90:50-200 lagrangian_step */
 void orc_lagrangian_step(orc_state *o) {
   int nx = o->nx, ny = o->ny, nz = o->nz;
-  double gm1 = o->p.gamma - 1.0, xi = o->p.xi_n;
+  double gm1 = o->p.gamma - 1.0;
   o->bx1 = cellbox(o); o->by1 = cellbox(o); o->bz1 = cellbox(o);
   o->alpha1 = box(o, 0, nx + 1, 0, ny + 2, 0, nz + 2);
   o->alpha2 = box(o, -1, nx + 1, 0, ny + 1, 0, nz + 2);
@@ -1698,7 +2027,7 @@ void orc_lagrangian_step(orc_state *o) {
         A(o->bx1, ix, iy, iz) = (A(o->bx, ix, iy, iz) + A(o->bx, ix - 1, iy, iz)) * 0.5;
         A(o->by1, ix, iy, iz) = (A(o->by, ix, iy, iz) + A(o->by, ix, iy - 1, iz)) * 0.5;
         A(o->bz1, ix, iy, iz) = (A(o->bz, ix, iy, iz) + A(o->bz, ix, iy, iz - 1)) * 0.5;
-        A(o->pressure, ix, iy, iz) = gm1 * A(o->rho, ix, iy, iz) * (A(o->energy, ix, iy, iz) - (1.0 - xi) * 0.0);
+        A(o->pressure, ix, iy, iz) = gm1 * A(o->rho, ix, iy, iz) * (A(o->energy, ix, iy, iz) - (1.0 - xi_at(o, IDX(ix, iy, iz))) * o->ip);
       }
   for (int iz = -1; iz <= nz + 1; ++iz)
     for (int iy = -1; iy <= ny + 1; ++iy)
@@ -1728,6 +2057,8 @@ void orc_lagrangian_step(orc_state *o) {
     o->dt = o->dt / (double)substeps;
     for (int sc = 1; sc <= substeps; ++sc) {
       orc_eta_calc(o);
+      if (o->pi_eos) neutral_fraction(o);
+      if (o->p.cowling_resistivity) perpendicular_resistivity(o);
       resistive_effects(o, fxr, fyr, fzr, curlb);
     }
     /* bx1.. recomputed from the new B (:152-163); only the CAUCHY predictor
@@ -1845,9 +2176,10 @@ int orc_get_delta_ke(orc_state *o, double *out) {
   return 0;
 }
 
-/* lare3d.f90:82-87 loop body (EOS_IDEAL) */
+/* lare3d.f90:82-87 loop body */
 void orc_step(orc_state *o, int step, double *dt, double *time) {
   o->step = step;
+  if (o->pi_eos) neutral_fraction(o);
   orc_lagrangian_step(o);
   orc_eulerian_remap(o, step);
   if (o->p.rke) orc_energy_correction(o);
@@ -1989,6 +2321,11 @@ orc_state *orc_create(const lare_grid *g, const lare_params *p) {
   o->cool_term_v = box(o, 1, o->nx, 1, o->ny, 1, o->nz);
   o->cool_term_b = box(o, 1, o->nx, 1, o->ny, 1, o->nz);
   if (o->p.rke) o->delta_ke = cellbox(o);
+  o->pi_eos = o->p.eos_number == LARE_EOS_ION || o->p.eos_number == LARE_EOS_PI;
+  for (int q = 0; q < 6; ++q) o->any_open |= bc[q] == LARE_BC_OPEN;
+  o->ip = o->p.eos_number == LARE_EOS_ION ? o->p.ionise_pot : 0.0;
+  if (o->pi_eos) o->xi_n = cellbox(o);
+  if (o->p.cowling_resistivity) o->eta_perp = box(o, 0, o->nx, 0, o->ny, 0, o->nz);
   for (int iz = -1; iz <= o->nz + 2; ++iz)
     for (int iy = -1; iy <= o->ny + 2; ++iy)
       for (int ix = -1; ix <= o->nx + 2; ++ix) A(o->cv, ix, iy, iz) = DXB(ix) * DYB(iy) * DZB(iz);
@@ -2001,7 +2338,7 @@ void orc_destroy(orc_state *o) {
   if (!o) return;
   double **all[] = {&o->rho, &o->energy, &o->vx, &o->vy, &o->vz, &o->vx1, &o->vy1, &o->vz1, &o->bx, &o->by, &o->bz, &o->eta,
                     &o->cv, &o->cv1, &o->visc3, &o->visc_dep, &o->ohmic_dep, &o->p_visc, &o->temperature, &o->larsen,
-                    &o->gamma_boris, &o->cool_term_v, &o->cool_term_b, &o->delta_ke};
+                    &o->gamma_boris, &o->cool_term_v, &o->cool_term_b, &o->delta_ke, &o->xi_n, &o->eta_perp};
   for (unsigned q = 0; q < sizeof(all) / sizeof(all[0]); ++q) drop(all[q]);
   for (int q = 0; q < 7; ++q) free(o->m[q]);
   free(o);
@@ -2050,6 +2387,23 @@ int orc_download(orc_state *o, lare_fields *f) {
   if (f->ohmic_dep) xfer(o, o->ohmic_dep, f->ohmic_dep, -1, -1, -1, 0);
   return 0;
 }
+
+/* xi_n and eta_perp (0..n, zero elsewhere) in the cell layout */
+int orc_get_neutral(orc_state *o, double *xi, double *ep) {
+  if ((xi && !o->xi_n) || (ep && !o->eta_perp)) return LARE_E_STATE;
+  if (xi) xfer(o, o->xi_n, xi, -1, -1, -1, 0);
+  if (ep) {
+    int hx = o->nx + 2, hy = o->ny + 2;
+    long nxf = hx + 2, nyf = hy + 2;
+    for (long q = 0; q < nxf * nyf * (o->nz + 4); ++q) ep[q] = 0.0;
+    for (int k = 0; k <= o->nz; ++k)
+      for (int j = 0; j <= o->ny; ++j)
+        for (int i = 0; i <= o->nx; ++i) ep[((long)(k + 1) * nyf + (j + 1)) * nxf + (i + 1)] = A(o->eta_perp, i, j, k);
+  }
+  return 0;
+}
+
+int orc_get_fault(orc_state *o) { return o->fault; }
 
 int orc_get_scalars(orc_state *o, lare_scalars *s) {
   s->dt = o->dt; s->dtr = o->dtr; s->time = o->time;

@@ -49,6 +49,18 @@ void orc_step(orc_state *o, int step, double *dt, double *time);
 void orc_energy_correction(orc_state *o);
 /* delta_ke(-1:n+2)^3 (Fortran order) as it stands; -1 when rke is off */
 int orc_get_delta_ke(orc_state *o, double *out);
+/* neutral.f90:158-195 neutral_fraction and :68-136 perpendicular_resistivity;
+ * LARE_E_STATE when the option is off; neutral_fraction returns
+ * LARE_F_TBOUNDS once any cell's bracket was inverted */
+int orc_neutral_fraction(orc_state *o);
+int orc_perpendicular_resistivity(orc_state *o);
+/* xi_n and eta_perp (0..n, zero elsewhere) in the (-1:n+2)^3 layout */
+int orc_get_neutral(orc_state *o, double *xi_n, double *eta_perp);
+/* LARE_F_TBOUNDS after an inverted bracket in any neutral_fraction, else 0 */
+int orc_get_fault(orc_state *o);
+/* the exp and get_neutral of the restatement (neutral.f90:140-154) */
+double orc_exp(double x);
+double orc_saha_neutral(const lare_params *p, double t_v, double rho_v);
 /* initial_conditions.f90:114-296 potential_field (replaces bx, by, bz);
  * phi_out (may be NULL) receives phi(-1:nx+2, -1:ny+2, -1:nz+2) */
 int orc_potential_field(orc_state *o, const lare_potential *pf, int *iterations, double *residual, double *bz_max,

@@ -23,7 +23,7 @@ SRC = os.path.join(HERE, "lare_oracle.c")
 CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-shared"]
 PIN_TESTS = ["tests/test_pin_hydro.py", "tests/test_pin_transport.py", "tests/test_pin_oracle3d.py",
              "tests/test_pin_potential.py", "tests/test_pin_variants.py", "tests/test_pin_rke.py",
-             "tests/test_pin_cauchy.py"]
+             "tests/test_pin_cauchy.py", "tests/test_pin_neutral.py"]
 
 # (name, what it breaks, [(old, new), ...]); each `old` occurs exactly once
 MUTANTS = [
@@ -65,8 +65,8 @@ MUTANTS = [
     ("M15", "energy flux donor density not upwinded (xremap.f90:594)",
      [("double rhou = AT(o->rho1, c) * r.vad_p + AT(o->rho1, p) * r.vad_m;", "double rhou = AT(o->rho1, c);")]),
     ("M16", "shock-viscosity threshold on the current cell width of the other axis (lagran.F90:825)",
-     [("A(o->alpha1, ix, iy, iz) = edge_viscosity(o, cs_v, dvdots, DXB(ix), i1, j1, k1, i2, j2, k2);",
-       "A(o->alpha1, ix, iy, iz) = edge_viscosity(o, cs_v, dvdots, DYB(iy), i1, j1, k1, i2, j2, k2);")]),
+     [("A(o->alpha1, ix, iy, iz) = edge_viscosity(o, cs_v, dvdots, DXB(ix), DXB(ix - 1),",
+       "A(o->alpha1, ix, iy, iz) = edge_viscosity(o, cs_v, dvdots, DYB(iy), DXB(ix - 1),")]),
     ("M17", "potential_field: colouring in three dimensions (x1 restarted from z1 and iz, :141-143)",
      [("      for (int iz = 1; iz <= nz; ++iz) {\n        int x1 = z1;",
        "      for (int iz = 1; iz <= nz; ++iz) {\n        int x1 = (iz % 2) ? z1 : 3 - z1;")]),
@@ -115,6 +115,46 @@ MUTANTS = [
     ("M38", "FOURTHORDER: the heating with dt instead of dt/6 (lagran.F90:1388)",
      [("A(c1, ixm, iy, izm) + A(c1, ix, iym, izm) + A(c1, ixm, iym, izm)) * dt6 /",
        "A(c1, ixm, iy, izm) + A(c1, ix, iym, izm) + A(c1, ixm, iym, izm)) * dt /")]),
+    ("M39", "get_neutral: 0.5 for 0.25 in the exponent's radiation term (neutral.f90:152)",
+     [("orc_exp((0.25 * (t_v / t_rad - 1.0) + 1.0) * p->t_bar / t_v)",
+       "orc_exp((0.5 * (t_v / t_rad - 1.0) + 1.0) * p->t_bar / t_v)")]),
+    ("M40", "get_neutral: the dilution factor left out (neutral.f90:148, :151)",
+     [("const double t_rad = p->t_rad, dilution = 0.5;", "const double t_rad = p->t_rad, dilution = 1.0;")]),
+    ("M41", "neutral_fraction: bisection stopped at a 1e-4 step (neutral.f90:186)",
+     [("if (fabs(dx) < 1.0e-8 || fabs(fa) < 1.0e-8) break;", "if (fabs(dx) < 1.0e-4 || fabs(fa) < 1.0e-8) break;")]),
+    ("M42", "neutral_fraction: the ionisation energy left out of the EOS residual (neutral.f90:183-184)",
+     [("const double fa = x - gm1 * (e0 - (1.0 - xi_a) * ip) / (2.0 - xi_a);",
+       "const double fa = x - gm1 * e0 / (2.0 - xi_a);")]),
+    ("M43", "pressure of lagrangian_step without the ionisation energy (lagran.F90:93-94)",
+     [("(A(o->energy, ix, iy, iz) - (1.0 - xi_at(o, IDX(ix, iy, iz))) * o->ip);",
+       "(A(o->energy, ix, iy, iz) - (1.0 - xi_at(o, IDX(ix, iy, iz))) * 0.0);"),
+      ("(e1 - (1.0 - xi_at(o, IDX(ix, iy, iz))) * o->ip) * gm1",
+       "(e1 - (1.0 - xi_at(o, IDX(ix, iy, iz))) * 0.0) * gm1")]),
+    ("M44", "conduction temperature with 1 - xi for 2 - xi (conduct.f90:72-73)",
+     [("return (o->p.gamma - 1.0) / (2.0 - xi) * (e - (1.0 - xi) * o->ip);",
+       "return (o->p.gamma - 1.0) / (1.0 - xi + 1.0e-3) * (e - (1.0 - xi) * o->ip);")]),
+    ("M45", "no neutral_fraction between RKL2 stages (conduct.f90:404)",
+     [("      if (o->pi_eos) neutral_fraction(o);\n      copy_all(o, Y[2], o->energy);",
+       "      copy_all(o, Y[2], o->energy);")]),
+    ("M46", "perpendicular_resistivity's vertex field 'corrected' to the bx and by faces (neutral.f90:97-98)",
+     [("double bxv = A(o->bx, ix, iy, iz) + A(o->bx, ix, iyp, iz) + A(o->bz, ix, iy, izp) + A(o->bz, ix, iyp, izp);",
+       "double bxv = A(o->bx, ix, iy, iz) + A(o->bx, ix, iyp, iz) + A(o->bx, ix, iy, izp) + A(o->bx, ix, iyp, izp);"),
+      ("double byv = A(o->by, ix, iy, iz) + A(o->by, ixp, iy, iz) + A(o->bz, ix, iy, izp) + A(o->bz, ixp, iy, izp);",
+       "double byv = A(o->by, ix, iy, iz) + A(o->by, ixp, iy, iz) + A(o->by, ix, iy, izp) + A(o->by, ixp, iy, izp);")]),
+    ("M47", "rkstep (Cowling): eta_perp left out of the EMF (lagran.F90:1567-1572)",
+     [("A(fxr, ix, iy, iz) = -((jpx + jqx) * e + jqx * ep) * DXC(ix) * 0.5;",
+       "A(fxr, ix, iy, iz) = -((jpx + jqx) * e) * DXC(ix) * 0.5;"),
+      ("A(fyr, ix, iy, iz) = -((jpy + jqy) * e + jqy * ep) * DYC(iy) * 0.5;",
+       "A(fyr, ix, iy, iz) = -((jpy + jqy) * e) * DYC(iy) * 0.5;"),
+      ("A(fzr, ix, iy, iz) = -((jpz + jqz) * e + jqz * ep) * DZC(iz) * 0.5;",
+       "A(fzr, ix, iy, iz) = -((jpz + jqz) * e) * DZC(iz) * 0.5;")]),
+    ("M48", "set_dt: the Cowling limit without eta_perp (lagran.F90:1132-1134)",
+     [("dt3 = 0.2 * dxlocal / dmax(dmax(A(o->eta, ix, iy, iz), A(o->eta_perp, ix, iy, iz)), NONE_ZERO);",
+       "dt3 = 0.2 * dxlocal / dmax(A(o->eta, ix, iy, iz), NONE_ZERO);")]),
+    ("M49", "rkstep (Cowling): j_par normalised by |B| instead of |B|^2 (lagran.F90:1527-1529)",
+     [("jpx = (jx * bxv + jy * byv + jz * bzv) * bxv / magn_b;", "jpx = (jx * bxv + jy * byv + jz * bzv) * bxv / sqrt(magn_b);"),
+      ("jpy = (jx * bxv + jy * byv + jz * bzv) * byv / magn_b;", "jpy = (jx * bxv + jy * byv + jz * bzv) * byv / sqrt(magn_b);"),
+      ("jpz = (jx * bxv + jy * byv + jz * bzv) * bzv / magn_b;", "jpz = (jx * bxv + jy * byv + jz * bzv) * bzv / sqrt(magn_b);")]),
 ]
 
 

@@ -48,6 +48,12 @@ def _load():
         "orc_potential_field": ([vp, C.POINTER(LarePotential), C.POINTER(C.c_int), dp, dp, dp], C.c_int),
         "orc_energy_correction": ([vp], None),
         "orc_get_delta_ke": ([vp, dp], C.c_int),
+        "orc_neutral_fraction": ([vp], C.c_int),
+        "orc_perpendicular_resistivity": ([vp], C.c_int),
+        "orc_get_neutral": ([vp, dp, dp], C.c_int),
+        "orc_get_fault": ([vp], C.c_int),
+        "orc_exp": ([C.c_double], C.c_double),
+        "orc_saha_neutral": ([C.POINTER(LareParams), C.c_double, C.c_double], C.c_double),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -64,6 +70,22 @@ def lib():
     if _lib is None:
         _lib = _load()
     return _lib
+
+
+class OracleFault(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def exp(x):
+    """The restatement's exp (the one the neutral fraction uses)."""
+    return lib().orc_exp(x)
+
+
+def saha_neutral(params, t, rho):
+    """get_neutral of neutral.f90:140-154 for one (T, rho)."""
+    return lib().orc_saha_neutral(C.byref(params), t, rho)
 
 
 class Oracle:
@@ -103,6 +125,7 @@ class Oracle:
 
     def lagrangian_step(self):
         lib().orc_lagrangian_step(self.h)
+        self._check_fault("lagrangian_step")
 
     def eulerian_remap(self, step):
         lib().orc_eulerian_remap(self.h, step)
@@ -112,6 +135,7 @@ class Oracle:
 
     def conduct_heat(self):
         lib().orc_conduct_heat(self.h)
+        self._check_fault("conduct_heat")
 
     def boundary_conditions(self):
         lib().orc_boundary_conditions(self.h)
@@ -127,7 +151,36 @@ class Oracle:
     def step(self, step):
         dt, t = C.c_double(), C.c_double()
         lib().orc_step(self.h, step, C.byref(dt), C.byref(t))
+        self._check_fault("step")
         return dt.value, t.value
+
+    def _check_fault(self, what):
+        f = lib().orc_get_fault(self.h)
+        if f:
+            raise OracleFault(f, f"{what}: neutral_fraction temperature bracket inverted (fault {f})")
+
+    def neutral_fraction(self):
+        rc = lib().orc_neutral_fraction(self.h)
+        if rc < 0:
+            raise RuntimeError(f"neutral_fraction: {rc}")
+        self._check_fault("neutral_fraction")
+
+    def perpendicular_resistivity(self):
+        rc = lib().orc_perpendicular_resistivity(self.h)
+        if rc:
+            raise RuntimeError(f"perpendicular_resistivity: {rc}")
+
+    def neutral(self, xi=True, eta_perp=False):
+        """(xi_n, eta_perp) in the (-1:n+2)^3 layout; None where not asked."""
+        import numpy as np
+        shape = tuple(n + 4 for n in self.n)
+        a = np.zeros(shape, order="F") if xi else None
+        b = np.zeros(shape, order="F") if eta_perp else None
+        rc = lib().orc_get_neutral(self.h, None if a is None else a.ctypes.data_as(C.POINTER(C.c_double)),
+                                   None if b is None else b.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc:
+            raise RuntimeError(f"get_neutral: {rc}")
+        return a, b
 
     def energy_correction(self):
         lib().orc_energy_correction(self.h)

@@ -8,7 +8,8 @@ step and after the boundary fills.
 import numpy as np
 import pytest
 
-from lare3d_amd import BC_PERIODIC, BC_USER, configs
+from lare3d_amd import BC_OPEN, BC_PERIODIC, BC_USER, configs
+from lare3d_amd.abi import EOS_ION, EOS_PI
 from tests.util import STATE, field_errors, rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -21,13 +22,23 @@ def _pair(cfg):
     from oracle.oracle import Oracle
     grid, params, fields = cfg[:3]
     print(f"\n[case] n={grid.n} bcs={grid.bcs}", flush=True)
-    return Device(grid, params, fields), Oracle(grid, params, fields)
+    pair = Device(grid, params, fields), Oracle(grid, params, fields)
+    for d in pair:
+        d.pi = params.eos_number in (EOS_ION, EOS_PI)
+        d.cowling = bool(params.cowling_resistivity)
+    return pair
 
 
 def _init(dev, orc):
+    """lare3d.f90:65-74: boundary_conditions, eta_calc, and for a partially
+    ionised run neutral_fraction and perpendicular_resistivity."""
     for d in (dev, orc):
         d.boundary_conditions()
         d.eta_calc()
+        if d.pi:
+            d.neutral_fraction()
+        if d.cowling:
+            d.perpendicular_resistivity()
 
 
 # n = 32: rows wider than two 128-byte lines, several waves per y-z plane
@@ -49,6 +60,29 @@ def test_boundary_conditions_bit_exact(bcs, n):
     dev.close()
 
 
+@pytest.mark.parametrize("eos", [EOS_PI, EOS_ION])
+def test_neutral_fraction_bit_exact(eos):
+    """neutral_fraction over the whole cell box and perpendicular_resistivity
+    at nodes 0..n on a random state across the ionisation transition
+    (energy and rho random per cell, B random per face)."""
+    grid, params, f = configs.partially_ionised(configs.uniform(8, bcs=(BC_USER,) * 6), eos, 1)
+    rng = np.random.default_rng(11 + eos)
+    f.rho[...] = 10.0 ** rng.uniform(-1.0, 1.0, f.rho.shape)
+    f.energy[...] = rng.uniform(0.2, 6.0, f.energy.shape)
+    for name in ("bx", "by", "bz"):
+        getattr(f, name)[...] = rng.standard_normal(getattr(f, name).shape)
+    dev, orc = _pair((grid, params, f))
+    for d in (dev, orc):
+        d.neutral_fraction()
+        d.perpendicular_resistivity()
+    (xd, ed), (xo, eo) = dev.neutral(True, True), orc.neutral(True, True)
+    print(f"eos {eos}: xi {xo.min():.3f}..{xo.max():.3f}, eta_perp {eo.max():.3e}; differing xi "
+          f"{int(np.count_nonzero(xd != xo))}, eta_perp {int(np.count_nonzero(ed != eo))}", flush=True)
+    assert np.array_equal(xd, xo)
+    assert np.array_equal(ed, eo)
+    dev.close()
+
+
 def test_conduct_heat_arcade():
     grid, params, f, dt = configs.arcade_conduction(16, 16, 32, s_target=9)
     dev, orc = _pair((grid, params, f))
@@ -64,6 +98,35 @@ def test_conduct_heat_arcade():
         print(f"super-step {it}: s={sd.n_s_stages} rel L2(energy incl. ghosts) = {e:.3e}")
         assert e <= TOL
     dev.close()
+
+
+def _open(cfg, bcs):
+    from lare3d_amd.grid import Grid
+    g = cfg[0]
+    grid = Grid(*g.n, g.extent, bcs, g.stretch, g.widths)
+    return (grid,) + tuple(cfg[1:])
+
+
+def _alfven_box(n=(14, 11, 9)):
+    """x and y sides open, z periodic; B along x plus a transverse field and
+    velocity noise confined to the middle of the box, so that the x far
+    field has no perpendicular field; resistive, conducting, rke."""
+    from lare3d_amd.fields import Fields
+    from lare3d_amd.grid import Grid
+    grid = Grid(*n, ((0.0, 1.0), (0.0, 0.8), (0.0, 0.6)), (BC_OPEN, BC_OPEN, BC_OPEN, BC_OPEN, BC_PERIODIC,
+                                                          BC_PERIODIC))
+    params = configs.default_params(resistive_mhd=1, eta_background=2e-3, eta0=0.0, j_max=1e30, conduction=1,
+                                    kappa_0=2e-2, heat_flux_limiter=1, rke=1)
+    f = Fields(*n)
+    rng = np.random.default_rng(21)
+    f.rho[...] = 1.0
+    f.energy[...] = 0.6 / (params.gamma - 1.0)
+    f.bx[...] = 0.8
+    mid = lambda a, lo: np.abs(np.arange(a.shape[0])[:, None, None] + lo - n[0] / 2) < n[0] / 4
+    for name, lo in (("by", -1), ("bz", -1), ("vy", -2), ("vz", -2), ("vx", -2)):
+        a = getattr(f, name)
+        a[...] = np.where(mid(a, lo), 0.05 * rng.standard_normal(a.shape), 0.0)
+    return grid, params, f
 
 
 def _with(cfg, **kw):
@@ -110,6 +173,30 @@ CASES = {
     # FOURTHORDER resistive substeps, alone and with conduction
     "harris_rk4": lambda: _with(configs.harris(19, ny=13, nz=11), fourth_order=1),
     "kink_rk4": lambda: _with(configs.kink_tube(16), fourth_order=1),
+    # the partially ionised equation of state (neutral fraction per cell)
+    # with Cowling resistivity: across the transition on the sheet; with the
+    # ionisation energy in the pressure and conduction (nine RKL2 stages,
+    # the neutral fraction recomputed between them); on the flux tube with
+    # FOURTHORDER substeps; and the ionisation energy on the ideal vortex,
+    # through the predictor remap and through the CAUCHY predictor
+    "harris_pi_cowling": lambda: configs.partially_ionised(configs.harris(19, ny=13, nz=11), EOS_PI, 1),
+    "harris_ion_cond": lambda: configs.partially_ionised(
+        _with(configs.harris(16), conduction=1, kappa_0=100.0, heat_flux_limiter=1), EOS_ION, 1),
+    "kink_pi_rk4": lambda: configs.partially_ionised(_with(configs.kink_tube(16), fourth_order=1), EOS_PI, 1,
+                                                     temp_norm=4.0e5, L_norm=1.0e3),
+    "ot_ion": lambda: configs.partially_ionised(configs.orszag_tang(16), EOS_ION, 0, temp_norm=2.0e5),
+    "ot_ion_cauchy": lambda: configs.partially_ionised(_with(configs.orszag_tang(16), cauchy=1), EOS_ION, 0,
+                                                       temp_norm=2.0e5),
+    # open (far-field characteristic) boundaries: the sheet's y walls open
+    # (B along x, so the fast solver); the flux tube open in x and y (four
+    # open sides meeting at edges) with conduction; a box open in x with a
+    # field along x and no perpendicular far field (the Alfven solver there,
+    # the fast one on its open y sides), resistive with conduction and rke
+    "harris_open": lambda: _open(configs.harris(19, ny=13, nz=11), (BC_PERIODIC, BC_PERIODIC, BC_OPEN, BC_OPEN,
+                                                                    BC_PERIODIC, BC_PERIODIC)),
+    "kink_open": lambda: _open(configs.kink_tube(16), (BC_OPEN, BC_OPEN, BC_OPEN, BC_OPEN, BC_PERIODIC,
+                                                       BC_PERIODIC)),
+    "alfven_open": lambda: _alfven_box(),
 }
 
 
@@ -130,6 +217,12 @@ def test_full_steps(case):
         assert abs(dtd - dto) <= 1e-12 * abs(dto)
         assert worst <= TOL, errs
         assert sum(ndiff.values()) == 0, ndiff
+        if dev.pi:
+            # neutral fraction and Cowling resistivity, element for element
+            (xd, ed), (xo, eo) = dev.neutral(True, dev.cowling), orc.neutral(True, dev.cowling)
+            assert np.array_equal(xd, xo), np.abs(xd - xo).max()
+            if dev.cowling:
+                assert np.array_equal(ed, eo), np.abs(ed - eo).max()
     en_d = dev.energy_account()
     en_o = orc.energy_account()
     for a, b in zip(en_d, en_o):
