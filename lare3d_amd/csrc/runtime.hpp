@@ -1007,6 +1007,14 @@ int lagrangian_step() {
     if (rc) return rc;
   }
   if (S.any_open) {
+    // v1 = v over the whole array before the predictor remap
+    // (lagran.F90:1051-1053); the forces then set nodes 0..n and
+    // remap_v_bcs the filled sides, so a BC_OPEN side keeps v's ghosts
+    LAUNCH_BOX("copy_open", k_copy_open, full_box(), S.vx, S.vx1, 1);
+    LAUNCH_BOX("copy_open", k_copy_open, full_box(), S.vy, S.vy1, 1);
+    LAUNCH_BOX("copy_open", k_copy_open, full_box(), S.vz, S.vz1, 1);
+  }
+  if (S.any_open) {
     rc = open_save();
     if (rc) return rc;
   }
@@ -1170,9 +1178,14 @@ int api_init(const lare_grid *grid, const lare_params *params) {
     return fail(LARE_E_ARG, "cowling_resistivity needs a partially ionised eos_number (control.f90:158-161)");
   if (pi && sizeof(real) != sizeof(double))
     return fail(LARE_E_ARG, "the partially ionised equation of state runs in the FP64 mode only");
-  for (int q = 0; q < 6; ++q)
+  for (int q = 0; q < 6; ++q) {
     if (bc[q] == LARE_BC_OPEN && sizeof(real) != sizeof(double))
       return fail(LARE_E_ARG, "BC_OPEN boundaries run in the FP64 mode only");
+    // with CAUCHY nothing sets v1 on a BC_OPEN side's ghost nodes before the
+    // remap reads them (lagran.F90:1050-1059 copy v to v1 only without it)
+    if (bc[q] == LARE_BC_OPEN && params->cauchy)
+      return fail(LARE_E_ARG, "BC_OPEN boundaries with the CAUCHY predictor: v1 on the open ghosts is undefined");
+  }
   if (pi && !(params->f_bar > 0.0 && params->t_rad > 0.0 && params->r_bar > 0.0 && params->t_bar >= 0.0))
     return fail(LARE_E_ARG, "partially ionised: f_bar, t_rad and r_bar must be > 0, t_bar >= 0");
   if (pi && params->cowling_resistivity && !(params->eta_bar >= 0.0))

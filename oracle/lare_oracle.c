@@ -13,8 +13,8 @@
  * with the per-cell neutral fraction and the Cowling resistivity of
  * neutral.f90), the run-time options of lare_params (boris, cooling_term,
  * rke, the CAUCHY, SHOCKLIMITER, SHOCKCOMPRESSION and FOURTHORDER
- * variants), no open boundaries; each axis is either periodic on both
- * sides or BC_USER on both sides; one MPI rank (so every MPI_SENDRECV is
+ * variants); each axis is either periodic on both sides or has BC_USER
+ * or BC_OPEN on each side (openboundary.f90); one MPI rank (so every MPI_SENDRECV is
  * the periodic self-exchange of mpiboundary.f90).
  */
 #include "lare_oracle.h"

@@ -23,7 +23,7 @@ SRC = os.path.join(HERE, "lare_oracle.c")
 CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-shared"]
 PIN_TESTS = ["tests/test_pin_hydro.py", "tests/test_pin_transport.py", "tests/test_pin_oracle3d.py",
              "tests/test_pin_potential.py", "tests/test_pin_variants.py", "tests/test_pin_rke.py",
-             "tests/test_pin_cauchy.py", "tests/test_pin_neutral.py"]
+             "tests/test_pin_cauchy.py", "tests/test_pin_neutral.py", "tests/test_pin_open.py"]
 
 # (name, what it breaks, [(old, new), ...]); each `old` occurs exactly once
 MUTANTS = [
@@ -155,6 +155,19 @@ MUTANTS = [
      [("jpx = (jx * bxv + jy * byv + jz * bzv) * bxv / magn_b;", "jpx = (jx * bxv + jy * byv + jz * bzv) * bxv / sqrt(magn_b);"),
       ("jpy = (jx * bxv + jy * byv + jz * bzv) * byv / magn_b;", "jpy = (jx * bxv + jy * byv + jz * bzv) * byv / sqrt(magn_b);"),
       ("jpz = (jx * bxv + jy * byv + jz * bzv) * bzv / magn_b;", "jpz = (jx * bxv + jy * byv + jz * bzv) * bzv / sqrt(magn_b);")]),
+    ("M50", "open_bcs: the far field read from the first ghost cell layer (openboundary.f90:50-59)",
+     [("lfc = hi ? n + 2 : -1,", "lfc = hi ? n + 1 : 0,")]),
+    ("M51", "open_bcs: the normal components not turned outwards on the low side (openboundary.f90:38-43)",
+     [("const double sg = hi ? 1.0 : -1.0;", "const double sg = 1.0;")]),
+    ("M52", "open_bcs_fast: the incoming characteristic's velocity from the interior (openboundary.f90:363-366)",
+     [("      vstar[i] = b->uxfar;", "      vstar[i] = b->vxbc[1];")]),
+    ("M53", "open_bcs_alfven: the incoming Alfven wave's invariant from the interior (openboundary.f90:430-431)",
+     [("      lys[i] = lambdayfar;", "      lys[i] = -b->bybc[1] * b->bxbc[1];")]),
+    ("M54", "open_bcs: the entropy wave's density jump over the interior sound speed (openboundary.f90:378)",
+     [("const double rhog = rhostar[2] + (pg - (pstar[2] - pmagstar[2])) / (c0far * c0far);",
+       "const double rhog = rhostar[2] + (pg - (pstar[2] - pmagstar[2])) / (c0 * c0);")]),
+    ("M55", "open_bcs not called (lagran.F90:410-412)",
+     [("  if (o->any_open) open_bcs(o);", "  if (0) open_bcs(o);")]),
 ]
 
 
