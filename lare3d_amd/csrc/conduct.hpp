@@ -222,12 +222,13 @@ __device__ __forceinline__ acc flux_z_face(const Geo &g, const Tn &T, int lz, in
 
 // Ghost mapping of one coordinate under a read mode (the per-axis part of
 // ghost_read): the coordinate to read and whether to read E0 instead.
-template <int MODE>
+// OPEN: some side is BC_OPEN (its ghosts read E0 as stored).
+template <int MODE, bool OPEN>
 __device__ __forceinline__ void map_coord(int c, int n, int user, int olo, int ohi, int &m, bool &stale) {
   m = c;
   stale = false;
   if (MODE == TMODE_DIRECT || (c >= 1 && c <= n)) return;
-  if (c < 1 ? olo : ohi) stale = true;
+  if (OPEN && (c < 1 ? olo : ohi)) stale = true;
   else if (user) m = mirror_idx(c, n);
   else if (MODE == TMODE_STALE) stale = true;
   else m = c < 1 ? c + n : c - n;
@@ -244,7 +245,7 @@ __device__ __forceinline__ void map_coord(int c, int n, int user, int olo, int o
 // same values as the neighbour's own, and the six are summed in the order
 // of cell_heat_flux.
 #define RKL_OUT 63
-template <int MODE, int FIRST, bool PI>
+template <int MODE, int FIRST, bool PI, bool OPEN>
 __global__ __launch_bounds__(BX * BY) void k_rkl2_z(Geo g, Box b, const real *__restrict__ Y2, const real *__restrict__ Y1,
                                                     const real *__restrict__ E0, const real *__restrict__ rho,
                                                     const double *__restrict__ bx, const double *__restrict__ by,
@@ -267,8 +268,8 @@ __global__ __launch_bounds__(BX * BY) void k_rkl2_z(Geo g, Box b, const real *__
   bool xs[3], ys[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    map_coord<MODE>(i + q - 1, g.nx, g.user[0], g.open_lo[0], g.open_hi[0], xm[q], xs[q]);
-    map_coord<MODE>(j + q - 1, g.ny, g.user[1], g.open_lo[1], g.open_hi[1], ym[q], ys[q]);
+    map_coord<MODE, OPEN>(i + q - 1, g.nx, g.user[0], g.open_lo[0], g.open_hi[0], xm[q], xs[q]);
+    map_coord<MODE, OPEN>(j + q - 1, g.ny, g.user[1], g.open_lo[1], g.open_hi[1], ym[q], ys[q]);
   }
   const acc kap = (acc)a.kappa_0, tdt = (acc)a.dt;
   // T.t[p][q][r]: x offset p-1, y offset q-1, plane k-1+r
@@ -277,7 +278,7 @@ __global__ __launch_bounds__(BX * BY) void k_rkl2_z(Geo g, Box b, const real *__
   {                                                                              \
     int zm;                                                                      \
     bool zs;                                                                     \
-    map_coord<MODE>((kk), g.nz, g.user[2], g.open_lo[2], g.open_hi[2], zm, zs); \
+    map_coord<MODE, OPEN>((kk), g.nz, g.user[2], g.open_lo[2], g.open_hi[2], zm, zs); \
     _Pragma("unroll") for (int p = 0; p < 3; ++p)                                \
       _Pragma("unroll") for (int q = 0; q < 3; ++q) {                            \
         const int cq = g.idx(xm[p], ym[q], zm);                                  \

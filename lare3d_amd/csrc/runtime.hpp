@@ -373,27 +373,36 @@ dim3 rkl2_grid(const Box &in) {
 // stage 1 from E0 (ghosts as stored): Lc and Y1 = E0 + mu_t1 dt Lc
 int rkl2_stage1(const real *E0, real *Yout, CondArgs ca, double mu_t1) {
   const Box in = interior();
+  // the stage-1 read is direct: no ghost mapping, BC_OPEN or not
   if (ca.xi)
-    LAUNCH("rkl2_stage1", (k_rkl2_z<TMODE_DIRECT, 1, true>), rkl2_grid(in), box_block(), S.g, in, E0, E0, E0, S.rho,
-           S.bx, S.by, S.bz, S.larsen, S.Lc, Yout, ca, 0.0, 0.0, 0.0, mu_t1, 0.0);
+    LAUNCH("rkl2_stage1", (k_rkl2_z<TMODE_DIRECT, 1, true, false>), rkl2_grid(in), box_block(), S.g, in, E0, E0, E0,
+           S.rho, S.bx, S.by, S.bz, S.larsen, S.Lc, Yout, ca, 0.0, 0.0, 0.0, mu_t1, 0.0);
   else
-    LAUNCH("rkl2_stage1", (k_rkl2_z<TMODE_DIRECT, 1, false>), rkl2_grid(in), box_block(), S.g, in, E0, E0, E0, S.rho,
-           S.bx, S.by, S.bz, S.larsen, S.Lc, Yout, ca, 0.0, 0.0, 0.0, mu_t1, 0.0);
+    LAUNCH("rkl2_stage1", (k_rkl2_z<TMODE_DIRECT, 1, false, false>), rkl2_grid(in), box_block(), S.g, in, E0, E0, E0,
+           S.rho, S.bx, S.by, S.bz, S.larsen, S.Lc, Yout, ca, 0.0, 0.0, 0.0, mu_t1, 0.0);
   return 0;
 }
 // stage j >= 2 from Y2 (and Y1, E0, Lc) under a ghost-read mode
 int rkl2_stage(int mode, const real *Y2, const real *Y1, const real *E0, real *Yout, CondArgs ca, double mu, double nu,
                double one_m_mu_nu, double mu_t, double gam_t) {
   const Box in = interior();
-#define RKL2_STAGE(M, P)                                                                                         \
-  LAUNCH("rkl2_stage", (k_rkl2_z<M, 0, P>), rkl2_grid(in), box_block(), S.g, in, Y2, Y1, E0, S.rho, S.bx, S.by, \
+#define RKL2_STAGE(M, P, O)                                                                                       \
+  LAUNCH("rkl2_stage", (k_rkl2_z<M, 0, P, O>), rkl2_grid(in), box_block(), S.g, in, Y2, Y1, E0, S.rho, S.bx, S.by, \
          S.bz, S.larsen, S.Lc, Yout, ca, mu, nu, one_m_mu_nu, mu_t, gam_t)
-  if (mode == TMODE_STALE) {
-    if (ca.xi) RKL2_STAGE(TMODE_STALE, true);
-    else RKL2_STAGE(TMODE_STALE, false);
+  if (S.any_open) {
+    if (mode == TMODE_STALE) {
+      if (ca.xi) RKL2_STAGE(TMODE_STALE, true, true);
+      else RKL2_STAGE(TMODE_STALE, false, true);
+    } else {
+      if (ca.xi) RKL2_STAGE(TMODE_WRAP, true, true);
+      else RKL2_STAGE(TMODE_WRAP, false, true);
+    }
+  } else if (mode == TMODE_STALE) {
+    if (ca.xi) RKL2_STAGE(TMODE_STALE, true, false);
+    else RKL2_STAGE(TMODE_STALE, false, false);
   } else {
-    if (ca.xi) RKL2_STAGE(TMODE_WRAP, true);
-    else RKL2_STAGE(TMODE_WRAP, false);
+    if (ca.xi) RKL2_STAGE(TMODE_WRAP, true, false);
+    else RKL2_STAGE(TMODE_WRAP, false, false);
   }
 #undef RKL2_STAGE
   return 0;
