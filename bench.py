@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # box.  Counting: each distinct field read once, each written field once.
 KERNEL_BYTES_PER_POINT = {
     "mom_remap": 8 * (3 + 6 + 3),                   # v x3, v1, db1, rho_v, cv1v, cv2v, dmv -> v x3 (new buffer)
-    "ct_flux": 8 * (2 + 1 + 1 + 2),                 # B_t1, B_t2, v_sweep, db1 -> F1, F2
+    "ct_sweep": 8 * (3 + 3),                        # t1, t2, a -> t1, t2, a (new buffers); + v_sweep, db1 below
     "geom": 8 * (3 + 3),                            # v1 x3 -> cv1, cv2, db1 (full sweeps)
     "vertex": 8 * (4 + 4),                          # rho1, cv1, cv2, dm -> rho_v, cv1v, cv2v, dmv
     "alpha": 8 * (3 + 2 + 3),                       # v x3, rho_v, cs_v -> alpha1..3
@@ -49,7 +49,6 @@ KERNEL_BYTES_PER_POINT = {
     "force": 8 * (3 + 1 + 3 + 1 + 3 + 3),           # v x3, pp, Bpred x3, rho_v, alpha x3 -> v1 x3
     "shock_corr": 8 * (6 + 3 + 1 + 3 + 3 + 3),      # v, v1, alpha, pp, e, rho, visc_dep (rw) -> v_new x3
     "rho_energy": 8 * (6 + 3),                      # e, rho1, v_sweep, db1, cv1, cv2 -> dm, rho_new, e_new
-    "ct_update": 8 * (2 + 2 + 2 + 2),               # F1, F2, t1 (rw), t2 (rw), a (rw)
     "rkl2_stage": 8 * (9 + 1),                      # Y2,Y1,E0,rho,B x3,larsen,Lc -> Y3
 }
 
@@ -59,13 +58,22 @@ def box_points(name, n):
     sizes = {
         # mom_remap: the updated nodes 0..n (the copies of the rest of the
         # padded box, 4.7% more points at 48 B, are not counted)
-        "mom_remap": (n + 1) ** 3, "ct_flux": (n + 3) ** 3, "geom": (n + 4) ** 3,
+        "mom_remap": (n + 1) ** 3, "ct_sweep": (n + 5) ** 3, "geom": (n + 4) ** 3,
         "vertex": (n + 3) * (n + 1) ** 2, "alpha": (n + 3) * (n + 4) * (n + 3),
         "visc_dt": (n + 2) ** 3, "vertex_prep": (n + 3) ** 3, "force": (n + 1) ** 3,
-        "shock_corr": n ** 3, "rho_energy": (n + 1) * (n + 2) ** 2, "ct_update": (n + 2) ** 3,
+        "shock_corr": n ** 3, "rho_energy": (n + 1) * (n + 2) ** 2,
         "rkl2_stage": n ** 3,
     }
     return sizes.get(name)
+
+
+def launch_bytes(name, n):
+    """Algorithmic bytes of one launch of the named kernel (n^3 grid)."""
+    b = KERNEL_BYTES_PER_POINT[name] * box_points(name, n)
+    if name == "ct_sweep":
+        # the sweep velocity and db1, read over the flux box
+        b += 8 * 2 * (n + 3) ** 3
+    return b
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "harris256_fp64_hbm_traffic.json")
@@ -418,7 +426,7 @@ def main():
     dom = next((k for k, _ in ranked if k in KERNEL_BYTES_PER_POINT), ranked[0][0])
     cnt, ms = stats[dom]
     avg_s = ms / cnt / 1e3
-    bytes_launch = KERNEL_BYTES_PER_POINT[dom] * box_points(dom, n)
+    bytes_launch = launch_bytes(dom, n)
     achieved = bytes_launch / avg_s / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),

@@ -157,85 +157,320 @@ __device__ __forceinline__ double ct_flux_q(double v_advect, double fm, double f
   return (fu + (double)(Di * ((acc)1.0 - phi))) * v_advect * dt;
 }
 
-// Face B component Q at (i, j, k) as the face flux that the remap moves,
-// and back (remap.f90:46-48 and 104-106, same operation order), with the
-// index range each component is defined on (bx(-2:nx+2, -1:ny+2, -1:nz+2)
-// and its analogues, remap.f90:44-67).
+// The index range of face B component Q (bx(-2:nx+2, -1:ny+2, -1:nz+2) and
+// its analogues, remap.f90:44-67): the points that the scaling to face
+// fluxes and back touches.
 template <int Q>
 __device__ __forceinline__ bool bface_ok(int i, int j, int k) {
   return Q == 0 ? (j >= -1 && k >= -1) : (Q == 1 ? (i >= -1 && k >= -1) : (i >= -1 && j >= -1));
 }
+// The widths dxb, dyb, dzb at one point, held in registers (the fused
+// kernels load the two that are fixed along their march once).  Face B
+// component Q becomes the face flux that the remap moves by multiplying by
+// the other two widths, and goes back by dividing by them, in the
+// reference's operation order (remap.f90:46-48 and 104-106).
+struct Wid {
+  double w[3];
+};
 template <int Q>
-__device__ __forceinline__ double to_flux(const Geo &g, double v, int i, int j, int k) {
-  if (Q == 0) return v * g.dyb[j] * g.dzb[k];
-  if (Q == 1) return v * g.dxb[i] * g.dzb[k];
-  return v * g.dxb[i] * g.dyb[j];
+__device__ __forceinline__ double to_flux_w(double v, const Wid &m) {
+  if (Q == 0) return v * m.w[1] * m.w[2];
+  if (Q == 1) return v * m.w[0] * m.w[2];
+  return v * m.w[0] * m.w[1];
 }
 template <int Q>
-__device__ __forceinline__ double from_flux(const Geo &g, double v, int i, int j, int k) {
-  if (Q == 0) return v / g.dyb[j] / g.dzb[k];
-  if (Q == 1) return v / g.dxb[i] / g.dzb[k];
-  return v / g.dxb[i] / g.dyb[j];
+__device__ __forceinline__ double from_flux_w(double v, const Wid &m) {
+  if (Q == 0) return v / m.w[1] / m.w[2];
+  if (Q == 1) return v / m.w[0] / m.w[2];
+  return v / m.w[0] / m.w[1];
 }
+
 // The transverse components (t1, t2) of a sweep along AX and its CT update
 // pairs: x -> (by, bz), y -> (bx, bz), z -> (bx, by).
 template <int AX> struct CtComp {
   static constexpr int T1 = AX == 0 ? 1 : 0;
   static constexpr int T2 = AX == 2 ? 1 : 2;
 };
-// Stencil value of component Q at the point whose sweep coordinate is s
-// (other coordinates those of (i, j, k)); SC = 1 reads the field as B and
-// converts it to the face flux on the fly (first sweep of a remap).
-template <int AX, int Q, int SC>
-__device__ __forceinline__ double ct_load(const Geo &g, const double *__restrict__ f, int c, int s, int i, int j,
-                                          int k) {
-  const double v = f[c];
-  if (!SC) return v;
-  return to_flux<Q>(g, v, AX == 0 ? s : i, AX == 1 ? s : j, AX == 2 ? s : k);
+// ---------------------------------------------------------------------------
+// Fused CT sweeps: the fluxes and the three face updates of a sweep in one
+// pass, out of place (inputs t1i, t2i, ai; outputs t1o, t2o, ao), so that
+// the fluxes never go to memory.  Each update takes a flux at the point and
+// at one neighbour (xremap.f90:121-157 and analogues):
+//   t1 = t1 - F1 + F1(s-1), t2 = t2 - F2 + F2(s-1) along the sweep;
+//   a = a + F1 - F1(c - AS1) + F2 - F2(c - AS2) across the transverse axes.
+// A block is a tile of the plane across the march axis, rows of lanes along
+// x, marching for CTS_CHUNK points; it forms the fluxes from the point
+// before its first one, so the neighbour along the march axis is always the
+// previous iteration's, the neighbour along x comes from the lane below by
+// a shuffle and the neighbour along the row axis from the row below through
+// LDS.  Lane 0 of each row segment and row 0 of the block are halos: they
+// form fluxes but write nothing.  Every point of the padded box -2..n+2 is
+// written (copied where no update applies).  The fluxes are rounded to the
+// storage type (FP32 in the FP32 mode) and each is added to one face and
+// subtracted from its neighbour as the same value, while B accumulates in
+// FP64, so div B stays at FP64 round-off.
+#ifndef CTS_CHUNK
+#define CTS_CHUNK 16
+#endif
+#define CTS_THREADS 512
+
+__device__ __forceinline__ bool in_box(const Box &b, int i, int j, int k) {
+  return i >= b.x0 && i <= b.x1 && j >= b.y0 && j <= b.y1 && k >= b.z0 && k <= b.z1;
 }
 
-// CT fluxes of the two transverse face fields of the x sweep (vx_by_flux and
-// vx_bz_flux, xremap.f90:340-470) in one pass, the stencil running along the
-// lanes.
-// Each wave covers 64 consecutive points of the (x, y) plane flattened over
-// x = x0..n+2 (one column beyond the flux box), successive waves overlapping
-// by three points.  Every lane forms the quotients field / width of both
-// fields at its own point, and lanes 1..CTX_OUT take the values at s-1, s+1
-// and s+2 from their neighbours by wavefront shuffles, so each quotient is
-// divided once.  The kernel marches in z with v of plane k-1 and db1 of
-// plane k+1 in registers.  GS1 and VS2 are the y stride (by's widths, bz's
-// velocity average); by's velocity and bz's widths are averaged along z.
-#define CTX_OUT 61
+// Output of point (i, j, k) at index c from the input values r1, r2, ra of
+// t1, t2 and a there: F1 and F2 are the fluxes at the point, F1s and F2s
+// those at s-1 along the sweep, F1a and F2a those at c - AS1 and c - AS2
+// (the fluxes as stored values: rounded to the storage type).  SC = +1
+// converts the inputs to face fluxes and SC = -1 the results back, on each
+// component's index range (remap.f90:44-67 and 102-125); points outside that
+// range are copied as they are.
+template <int AX, int SC>
+__device__ __forceinline__ void ct_store(const Wid &wd, int c, int i, int j, int k, double r1, double r2, double ra,
+                                         double *__restrict__ t1o, double *__restrict__ t2o,
+                                         double *__restrict__ ao, real F1, real F1s, real F1a, real F2, real F2s,
+                                         real F2a, const Box &bt1, const Box &ba1, const Box &bt2, const Box &ba2) {
+  constexpr int T1 = CtComp<AX>::T1, T2 = CtComp<AX>::T2;
+  {
+    const bool ok = SC == 0 || bface_ok<T1>(i, j, k);
+    double v = r1;
+    if (SC > 0 && ok) v = to_flux_w<T1>(v, wd);
+    if (in_box(bt1, i, j, k)) v = v - (double)F1 + (double)F1s;
+    if (SC < 0 && ok) v = from_flux_w<T1>(v, wd);
+    t1o[c] = v;
+  }
+  {
+    const bool ok = SC == 0 || bface_ok<T2>(i, j, k);
+    double v = r2;
+    if (SC > 0 && ok) v = to_flux_w<T2>(v, wd);
+    if (in_box(bt2, i, j, k)) v = v - (double)F2 + (double)F2s;
+    if (SC < 0 && ok) v = from_flux_w<T2>(v, wd);
+    t2o[c] = v;
+  }
+  {
+    const bool ok = SC == 0 || bface_ok<AX>(i, j, k);
+    double v = ra;
+    if (SC > 0 && ok) v = to_flux_w<AX>(v, wd);
+    if (in_box(ba1, i, j, k)) v = v + (double)F1 - (double)F1a;
+    if (in_box(ba2, i, j, k)) v = v + (double)F2 - (double)F2a;
+    if (SC < 0 && ok) v = from_flux_w<AX>(v, wd);
+    ao[c] = v;
+  }
+}
+
+// y and z sweeps (M = 1, 2).  Rows of CTM_XW lanes along x, CTM_ROWS rows
+// along the other transverse axis P (which is t2's axis, so F2a comes from
+// the row below; t1 is bx, so F1a comes from the lane below).  Each thread
+// holds the four-point stencils of db1, of both fields' widths and of their
+// quotients field / width along the sweep in registers; a quotient is
+// formed once per point and serves as the s+2, s+1, s and s-1 value of four
+// successive fluxes, with the same operations as forming it at each use.
+// The loop is software-pipelined around the one barrier per point: the inputs of
+// the point's output are loaded first, and the stencil and velocities of
+// the next point are loaded before the barrier.
+#define CTM_XW 32
+#ifndef CTM_THREADS
+#define CTM_THREADS 256
+#endif
+#define CTM_ROWS (CTM_THREADS / CTM_XW)
+template <int M, int SC>
+__global__ __launch_bounds__(CTM_THREADS) void k_ct_sweep_m(Geo g, const double *__restrict__ t1i,
+                                                            const double *__restrict__ t2i,
+                                                            const double *__restrict__ ai, double *__restrict__ t1o,
+                                                            double *__restrict__ t2o, double *__restrict__ ao,
+                                                            const real *__restrict__ vel,
+                                                            const real *__restrict__ db1, int VS1, int GS1, int VS2,
+                                                            int GS2, double dt, int ntx, Box bt1, Box ba1, Box bt2,
+                                                            Box ba2) {
+  constexpr int T1 = CtComp<M>::T1, T2 = CtComp<M>::T2;
+  constexpr int P = M == 2 ? 1 : 2;
+  __shared__ real xch[2][CTM_ROWS][CTM_XW];
+  const int xl = (int)threadIdx.x % CTM_XW, row = (int)threadIdx.x / CTM_XW;
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = (int)(tile % (unsigned)ntx), ty = (int)(tile / (unsigned)ntx);
+  const int i = -3 + tx * (CTM_XW - 1) + xl;
+  const int p = -3 + ty * (CTM_ROWS - 1) + row;
+  const int n = g.n(M), nx = g.nx, nP = g.n(P);
+  const int m0 = -2 + (int)blockIdx.z * CTS_CHUNK;
+  const int m1 = m0 + CTS_CHUNK - 1 < n + 2 ? m0 + CTS_CHUNK - 1 : n + 2;
+  // a flux column lies in the flux box across the sweep; an output column
+  // in the padded box, off the halo lane and row
+  const bool col = i >= -1 && i <= nx + 1 && p >= -1 && p <= nP + 1;
+  const bool outc = xl >= 1 && row >= 1 && i >= -2 && i <= nx + 2 && p >= -2 && p <= nP + 2;
+  auto at = [&](int m) { return M == 2 ? g.idx(i, p, m) : g.idx(i, m, p); };
+  // widths at the point: x and P fixed along the march, the sweep axis's per point
+  const double *__restrict__ dbm = M == 1 ? g.dyb : g.dzb;
+  Wid wd;
+  wd.w[0] = i >= -2 && i <= nx + 2 ? g.dxb[i] : 1.0;
+  wd.w[P] = p >= -2 && p <= nP + 2 ? (P == 1 ? g.dyb[p] : g.dzb[p]) : 1.0;
+  wd.w[M] = 1.0;
+  double q1[4], q2[4];
+  acc w1[4], w2[4];
+  real d[4];
+  auto load = [&](int slot, int m) {
+    const int c = at(m);
+    d[slot] = db1[c];
+    w1[slot] = (d[slot] + db1[c + GS1]) * (acc)0.5;
+    w2[slot] = (d[slot] + db1[c + GS2]) * (acc)0.5;
+    double v1 = t1i[c], v2 = t2i[c];
+    if (SC > 0) {
+      Wid wm = wd;
+      wm.w[M] = dbm[m];
+      v1 = to_flux_w<T1>(v1, wm);
+      v2 = to_flux_w<T2>(v2, wm);
+    }
+    q1[slot] = v1 / (double)w1[slot];
+    q2[slot] = v2 / (double)w2[slot];
+  };
+  // sweep velocity at the flux point and its two averaging neighbours
+  real vc = 0, va = 0, vb = 0;
+  auto loadv = [&](int m) {
+    if (m >= -1 && m <= n + 1) {
+      const int c = at(m);
+      vc = vel[c];
+      va = vel[c - VS1];
+      vb = vel[c - VS2];
+    }
+  };
+  auto clampm = [&](int m) { return m < -1 ? -1 : (m > n + 2 ? n + 2 : m); };
+  // window over m-1, m, m+1, m+2 of the first flux point m0 - 1
+  if (col) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) load(q, clampm(m0 - 2 + q));
+    loadv(m0 - 1);
+  }
+  real F1s = 0, F2s = 0;
+  int buf = 0;
+  for (int m = m0 - 1; m <= m1; ++m) {
+    // the inputs of this point's output
+    const bool o = outc && m >= m0;
+    double r1 = 0, r2 = 0, ra = 0;
+    if (o) {
+      const int c = at(m);
+      r1 = t1i[c];
+      r2 = t2i[c];
+      ra = ai[c];
+      if (SC != 0) wd.w[M] = dbm[m];
+    }
+    real F1 = 0, F2 = 0;
+    if (col && m >= -1 && m <= n + 1) {
+      const acc dcm = (acc)0.5 * (d[0] + (m > -1 ? d[1] : d[2]));
+      const acc dci = (acc)0.5 * (d[1] + d[2]);
+      const acc dcp = (acc)0.5 * (d[2] + d[3]);
+      F1 = (real)ct_flux_q((vc + va) * 0.5, q1[0], q1[1], q1[2], q1[3], w1[1], w1[2], d[1], d[2], dcm, dci, dcp, dt);
+      F2 = (real)ct_flux_q((vc + vb) * 0.5, q2[0], q2[1], q2[2], q2[3], w2[1], w2[2], d[1], d[2], dcm, dci, dcp, dt);
+    }
+    // the next point's stencil and velocities
+    if (col && m < m1) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        q1[q] = q1[q + 1];
+        q2[q] = q2[q + 1];
+        w1[q] = w1[q + 1];
+        w2[q] = w2[q + 1];
+        d[q] = d[q + 1];
+      }
+      load(3, clampm(m + 3));
+      loadv(m + 1);
+    }
+    xch[buf][row][xl] = F2;
+    __syncthreads();
+    const real F2a = xch[buf][row > 0 ? row - 1 : 0][xl];
+    const real F1a = __shfl_up(F1, 1, CTM_XW);
+    if (o) {
+      const int jj = M == 1 ? m : p, kk = M == 2 ? m : p;
+      ct_store<M, SC>(wd, at(m), i, jj, kk, r1, r2, ra, t1o, t2o, ao, F1, F1s, F1a, F2, F2s, F2a, bt1, ba1, bt2, ba2);
+    }
+    F1s = F1;
+    F2s = F2;
+    buf ^= 1;
+  }
+}
+
+// x sweep.  Rows of 64 lanes along x (the sweep axis: every lane forms the
+// quotients of both fields at its own point, and the stencil values at s-1,
+// s+1, s+2 and the neighbour flux along the sweep come from other lanes by
+// wavefront shuffles, so each quotient is divided once), CTX_ROWS rows along y (t1 is by, so F1a comes from the
+// row below), marching in z (t2 is bz, so F2a is the previous plane's).
+// Lanes 1..61 form fluxes from the quotients of lanes 0..63; lanes 2..61
+// write, CTX_W = 60 points per row.  The inputs of each plane are loaded
+// one iteration ahead, before the barrier.
+#define CTX_W 60
+#define CTX_ROWS (CTS_THREADS / 64)
 template <int SC>
-__global__ __launch_bounds__(BX * BY) void k_ct_flux_x(Geo g, Box b, const double *__restrict__ fld1,
-                                                       const double *__restrict__ fld2,
-                                                       const real *__restrict__ vel, const real *__restrict__ db1,
-                                                       real *__restrict__ F1, real *__restrict__ F2, int GS1,
-                                                       int VS2, double dt) {
+__global__ __launch_bounds__(CTS_THREADS) void k_ct_sweep_x(Geo g, const double *__restrict__ t1i,
+                                                            const double *__restrict__ t2i,
+                                                            const double *__restrict__ ai, double *__restrict__ t1o,
+                                                            double *__restrict__ t2o, double *__restrict__ ao,
+                                                            const real *__restrict__ vel,
+                                                            const real *__restrict__ db1, int GS1, int VS2, double dt,
+                                                            int ntx, Box bt1, Box ba1, Box bt2, Box ba2) {
   constexpr int T1 = CtComp<0>::T1, T2 = CtComp<0>::T2;
-  const int lane = (int)threadIdx.x;  // BX = 64: threadIdx.y is the wave
-  const int wq = g.nx + 2 - b.x0 + 1;  // row of quotient points x0..n+2
-  const int npts = wq * (b.y1 - b.y0 + 1);
-  const int f = (int)(xcd_tile(blockIdx.x, gridDim.x) * BY + threadIdx.y) * CTX_OUT + lane - 1;
-  const bool valid = f >= 0 && f < npts;
-  const int ff = valid ? f : 0;  // lanes past the plane read point 0, unused
-  const int jr = ff / wq;
-  const int i = b.x0 + (ff - jr * wq);
-  const int j = b.y0 + jr;
-  const bool out = valid && lane >= 1 && lane <= CTX_OUT && i <= b.x1;
-  ZRANGE(b, k0, k1)
+  __shared__ real xch[2][CTX_ROWS][64];
+  const int lane = (int)threadIdx.x & 63, row = (int)threadIdx.x >> 6;
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = (int)(tile % (unsigned)ntx), ty = (int)(tile / (unsigned)ntx);
+  const int i = -4 + tx * CTX_W + lane;
+  const int j = -3 + ty * (CTX_ROWS - 1) + row;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int Z = g.sz;
-  const int nx = g.nx;
-  int c = g.idx(i, j, k0);
-  real vm = vel[c - Z];
-  real d = db1[c];
-  for (int k = k0; k <= k1; ++k, c += Z) {
-    const real dzo = db1[c + Z];
-    const acc w1 = (d + db1[c + GS1]) * (acc)0.5;
-    const acc w2 = (d + dzo) * (acc)0.5;
-    const double q1 = ct_load<0, T1, SC>(g, fld1, c, i, i, j, k) / (double)w1;
-    const double q2 = ct_load<0, T2, SC>(g, fld2, c, i, i, j, k) / (double)w2;
-    const acc dd = d;
+  const int k0 = -2 + (int)blockIdx.z * CTS_CHUNK;
+  const int k1 = k0 + CTS_CHUNK - 1 < nz + 2 ? k0 + CTS_CHUNK - 1 : nz + 2;
+  // quotient points x = -1..n+2 and flux points x = -1..n+1 of flux rows
+  // y = -1..n+1; output points of the padded box off the halo lanes and row
+  const bool qrow = j >= -1 && j <= ny + 1 && i >= -1 && i <= nx + 2;
+  const bool frow = qrow && lane >= 1 && lane <= CTX_W + 1 && i <= nx + 1;
+  const bool outc = lane >= 2 && lane <= CTX_W + 1 && row >= 1 && i >= -2 && i <= nx + 2 && j >= -2 && j <= ny + 2;
+  const int c0 = (qrow || outc) ? g.idx(i, j, 0) : 0;
+  // widths at the point: x and y fixed along the march, z's per plane
+  Wid wd;
+  wd.w[0] = i >= -2 && i <= nx + 2 ? g.dxb[i] : 1.0;
+  wd.w[1] = j >= -2 && j <= ny + 2 ? g.dyb[j] : 1.0;
+  wd.w[2] = 1.0;
+  // inputs of plane k, fetched one iteration ahead
+  real fd = 0, fg = 0, fz = 0, fv = 0, fv2 = 0, fvz = 0;
+  double f1 = 0, f2 = 0, fa = 0;
+  auto fetch = [&](int k) {
+    const int c = c0 + k * Z;
+    if (qrow && k >= -1 && k <= nz + 1) {
+      fd = db1[c];
+      fg = db1[c + GS1];
+      fz = db1[c + Z];
+      fv = vel[c];
+      fv2 = vel[c - VS2];
+      if (k == -1 || k == k0 - 1) fvz = vel[c - Z];
+      f1 = t1i[c];
+      f2 = t2i[c];
+    } else if (outc && k >= k0) {
+      f1 = t1i[c];
+      f2 = t2i[c];
+    }
+    if (outc && k >= k0) fa = ai[c];
+  };
+  fetch(k0 - 1);
+  real F2z = 0;  // F2 of the previous plane
+  real vm = 0;
+  int buf = 0;
+  for (int k = k0 - 1; k <= k1; ++k) {
+    const bool qk = qrow && k >= -1 && k <= nz + 1;
+    const int c = c0 + k * Z;
+    // this plane's inputs; then the next plane's loads go out
+    const real d = fd, dg = fg, dz = fz, vc0 = fv, v2 = fv2, vz = fvz;
+    const double r1 = f1, r2 = f2, ra = fa;
+    if (k < k1) fetch(k + 1);
+    if (SC != 0 && k >= -2) wd.w[2] = g.dzb[k];
+    double q1 = 0, q2 = 0;
+    acc w1 = 0, w2 = 0, dd = 0;
+    real vc = 0;
+    if (qk) {
+      w1 = (d + dg) * (acc)0.5;
+      w2 = (d + dz) * (acc)0.5;
+      q1 = (SC > 0 ? to_flux_w<T1>(r1, wd) : r1) / (double)w1;
+      q2 = (SC > 0 ? to_flux_w<T2>(r2, wd) : r2) / (double)w2;
+      dd = d;
+      vc = vc0;
+      if (k == -1 || k == k0 - 1) vm = vz;
+    }
     // neighbours along x; s-1 is clamped to -1 and s+2 to n+2
     double q1m = __shfl_up(q1, 1, 64), q2m = __shfl_up(q2, 1, 64);
     acc dm = __shfl_up(dd, 1, 64);
@@ -254,142 +489,23 @@ __global__ __launch_bounds__(BX * BY) void k_ct_flux_x(Geo g, Box b, const doubl
       q2q = q2p;
       dq = dp;
     }
-    const real vc = vel[c];
-    if (out) {
-      // dxc1 at s-1 (upper point s, or s+1 when s = -1), s and s+1
+    real F1 = 0, F2 = 0;
+    if (frow && qk) {
       const acc dcm = (acc)0.5 * (dm + (i > -1 ? dd : dp));
       const acc dci = (acc)0.5 * (dd + dp);
       const acc dcp = (acc)0.5 * (dp + dq);
-      F1[c] = ct_flux_q((vc + vm) * 0.5, q1m, q1, q1p, q1q, w1, w1p, dd, dp, dcm, dci, dcp, dt);
-      F2[c] = ct_flux_q((vc + vel[c - VS2]) * 0.5, q2m, q2, q2p, q2q, w2, w2p, dd, dp, dcm, dci, dcp, dt);
+      F1 = (real)ct_flux_q((vc + vm) * 0.5, q1m, q1, q1p, q1q, w1, w1p, dd, dp, dcm, dci, dcp, dt);
+      F2 = (real)ct_flux_q((vc + v2) * 0.5, q2m, q2, q2p, q2q, w2, w2p, dd, dp, dcm, dci, dcp, dt);
     }
     vm = vc;
-    d = dzo;
-  }
-}
-
-// The same fluxes for the y and z sweeps (M = 1, 2).  Each thread keeps its
-// (x, other transverse) column and marches along the sweep axis for ZCHUNK
-// points, holding the four-point stencils of db1, of both fields' widths
-// and of their quotients field / width in registers.  A quotient is formed
-// once per point and serves as the s+2, s+1, s and s-1 value of four
-// successive fluxes, with the same operations as forming it at each use.
-template <int M, int SC>
-__global__ __launch_bounds__(BX * BY) void k_ct_flux_m(Geo g, Box b, const double *__restrict__ fld1,
-                                                       const double *__restrict__ fld2,
-                                                       const real *__restrict__ vel, const real *__restrict__ db1,
-                                                       real *__restrict__ F1, real *__restrict__ F2, int VS1,
-                                                       int GS1, int VS2, int GS2, double dt) {
-  constexpr int T1 = CtComp<M>::T1, T2 = CtComp<M>::T2;
-  // the launch box with the march axis last: (x, P, M), P the other axis
-  const Box pb = M == 2 ? b : Box{b.x0, b.x1, b.z0, b.z1, b.y0, b.y1};
-  PLANE_POS(pb, i, p)
-  ZRANGE(pb, m0, m1)
-  if (p > pb.y1) return;
-  const int n = g.n(M);
-  // index, and quotients of both fields, at march coordinate m
-  auto at = [&](int m) { return M == 2 ? g.idx(i, p, m) : g.idx(i, m, p); };
-  double q1[4], q2[4];
-  acc w1[4], w2[4];
-  real d[4];
-  auto load = [&](int slot, int m) {
-    const int c = at(m);
-    const int jj = M == 1 ? m : p, kk = M == 2 ? m : p;
-    d[slot] = db1[c];
-    w1[slot] = (d[slot] + db1[c + GS1]) * (acc)0.5;
-    w2[slot] = (d[slot] + db1[c + GS2]) * (acc)0.5;
-    q1[slot] = ct_load<M, T1, SC>(g, fld1, c, m, i, jj, kk) / (double)w1[slot];
-    q2[slot] = ct_load<M, T2, SC>(g, fld2, c, m, i, jj, kk) / (double)w2[slot];
-  };
-  // window over m-1, m, m+1, m+2, each clamped to [-1, n+2]
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    int mm = m0 - 1 + q;
-    mm = mm < -1 ? -1 : (mm > n + 2 ? n + 2 : mm);
-    load(q, mm);
-  }
-  for (int m = m0; m <= m1; ++m) {
-    const int c = at(m);
-    // dxc1 at the clamped s-1 (its upper point is s, or s+1 when s = -1),
-    // at s, and at s+1 (upper point clamped to n+2, as d[3] already is)
-    const acc dcm = (acc)0.5 * (d[0] + (m > -1 ? d[1] : d[2]));
-    const acc dci = (acc)0.5 * (d[1] + d[2]);
-    const acc dcp = (acc)0.5 * (d[2] + d[3]);
-    const real vc = vel[c];
-    F1[c] = ct_flux_q((vc + vel[c - VS1]) * 0.5, q1[0], q1[1], q1[2], q1[3], w1[1], w1[2], d[1], d[2], dcm, dci, dcp,
-                      dt);
-    F2[c] = ct_flux_q((vc + vel[c - VS2]) * 0.5, q2[0], q2[1], q2[2], q2[3], w2[1], w2[2], d[1], d[2], dcm, dci, dcp,
-                      dt);
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      q1[q] = q1[q + 1];
-      q2[q] = q2[q + 1];
-      w1[q] = w1[q + 1];
-      w2[q] = w2[q + 1];
-      d[q] = d[q + 1];
-    }
-    if (m < m1) load(3, m + 3 > n + 2 ? n + 2 : m + 3);
-  }
-}
-
-// CT updates of a sweep along AX (xremap.f90:121-157 and analogues), both
-// pairs in the reference's order:
-//   t1 = t1 - F1 + F1(c - TS) on bt1;  a = a + F1 - F1(c - AS1) on ba1;
-//   t2 = t2 - F2 + F2(c - TS) on bt2;  a = a + F2 - F2(c - AS2) on ba2.
-// SC = 0: in place on the face fluxes (t1i == t1 etc.), over the box b.
-// SC = +1 (first sweep of a remap): the inputs are B; every point of each
-// component's range in b is converted to the face flux (remap.f90:44-67),
-// updated where its box says so, and stored to the outputs.
-// SC = -1 (last sweep of the predictor remap): the updated face flux of
-// every such point is converted back to B (remap.f90:102-125).
-// The fluxes F1, F2 are stored in the storage type (FP32 in the FP32 mode);
-// each is added to one face and subtracted from its neighbour as the same
-// stored value, and B accumulates in FP64, so div B stays at FP64 round-off.
-template <int AX, int SC>
-__global__ __launch_bounds__(BX * BY) void k_ct_update(Geo g, Box b, const real *__restrict__ F1,
-                                                       const real *__restrict__ F2, const double *t1i,
-                                                       const double *t2i, const double *ai, double *t1, double *t2,
-                                                       double *a, int TS, int AS1, int AS2, Box bt1, Box ba1,
-                                                       Box bt2, Box ba2) {
-  BOX_INDEX(b, i, j, k);
-  constexpr int T1 = CtComp<AX>::T1, T2 = CtComp<AX>::T2;
-  const int c = g.idx(i, j, k);
-  const bool u1 = i >= bt1.x0 && i <= bt1.x1 && j >= bt1.y0 && j <= bt1.y1 && k >= bt1.z0 && k <= bt1.z1;
-  const bool w1 = i >= ba1.x0 && i <= ba1.x1 && j >= ba1.y0 && j <= ba1.y1 && k >= ba1.z0 && k <= ba1.z1;
-  const bool u2 = i >= bt2.x0 && i <= bt2.x1 && j >= bt2.y0 && j <= bt2.y1 && k >= bt2.z0 && k <= bt2.z1;
-  const bool w2 = i >= ba2.x0 && i <= ba2.x1 && j >= ba2.y0 && j <= ba2.y1 && k >= ba2.z0 && k <= ba2.z1;
-  if (SC == 0) {
-    if (u1) t1[c] = t1[c] - F1[c] + F1[c - TS];
-    if (u2) t2[c] = t2[c] - F2[c] + F2[c - TS];
-    if (w1 || w2) {
-      double va = a[c];
-      if (w1) va = va + F1[c] - F1[c - AS1];
-      if (w2) va = va + F2[c] - F2[c - AS2];
-      a[c] = va;
-    }
-    return;
-  }
-  if (bface_ok<T1>(i, j, k)) {
-    double v = t1i[c];
-    if (SC > 0) v = to_flux<T1>(g, v, i, j, k);
-    if (u1) v = v - F1[c] + F1[c - TS];
-    if (SC < 0) v = from_flux<T1>(g, v, i, j, k);
-    t1[c] = v;
-  }
-  if (bface_ok<T2>(i, j, k)) {
-    double v = t2i[c];
-    if (SC > 0) v = to_flux<T2>(g, v, i, j, k);
-    if (u2) v = v - F2[c] + F2[c - TS];
-    if (SC < 0) v = from_flux<T2>(g, v, i, j, k);
-    t2[c] = v;
-  }
-  if (bface_ok<AX>(i, j, k)) {
-    double va = ai[c];
-    if (SC > 0) va = to_flux<AX>(g, va, i, j, k);
-    if (w1) va = va + F1[c] - F1[c - AS1];
-    if (w2) va = va + F2[c] - F2[c - AS2];
-    if (SC < 0) va = from_flux<AX>(g, va, i, j, k);
-    a[c] = va;
+    xch[buf][row][lane] = F1;
+    __syncthreads();
+    const real F1a = xch[buf][row > 0 ? row - 1 : 0][lane];
+    const real F1x = __shfl_up(F1, 1, 64), F2x = __shfl_up(F2, 1, 64);
+    if (outc && k >= k0)
+      ct_store<0, SC>(wd, c, i, j, k, r1, r2, ra, t1o, t2o, ao, F1, F1x, F1a, F2, F2x, F2z, bt1, ba1, bt2, ba2);
+    F2z = F2;
+    buf ^= 1;
   }
 }
 

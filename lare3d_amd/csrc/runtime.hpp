@@ -466,66 +466,74 @@ int heat_conduct_sts2(int delta_mode) {
 // ----------------------------------------------------------------------------
 int strides(int ax) { return ax == 0 ? 1 : (ax == 1 ? S.sy : S.sz); }
 
-// CT fluxes and updates of one sweep on the face arrays B.  sc = 0: B holds
-// face fluxes and is updated in place.  sc = +1 (first sweep of a remap): Bin
-// holds B; the fluxes convert it on the fly and the update writes the face
-// fluxes of every point to B (the scaling pass of remap.f90:44-67 folded in).
-// sc = -1 (last predictor sweep): the update converts every point back to B
-// (remap.f90:102-125 folded in).
+// CT fluxes and updates of one sweep in one pass, from the face arrays in
+// to the face arrays out (never the same: the stencil reads neighbours that
+// other blocks write).  sc = 0: both hold face fluxes.  sc = +1 (first sweep
+// of a remap): in holds B, converted on the fly (the scaling pass of
+// remap.f90:44-67 folded in).  sc = -1 (last predictor sweep): out gets B
+// back (remap.f90:102-125 folded in).
 template <int AX>
-int ct_pair(double *const Bin[3], double *const B[3], const real *vel, double dt, int sc) {
-  const int N0 = nx(), N1 = ny(), N2 = nz();
-  Box fb = mkbox(-1, N0 + 1, -1, N1 + 1, -1, N2 + 1);
-  Box ub = mkbox(0, N0 + 1, 0, N1 + 1, 0, N2 + 1);
-  const double *const *Bf = sc > 0 ? Bin : B;  // what the fluxes read
+int ct_sweep(const double *const in[3], double *const out[3], const real *vel, double dt, int sc) {
+  const int N[3] = {nx(), ny(), nz()};
+  const Box ub = mkbox(0, N[0] + 1, 0, N[1] + 1, 0, N[2] + 1);
   // (transverse field t, velocity-average axis, geometry-average axis, box_t, box_a) x 2
   int t1, va1, ga1, t2, va2, ga2;
   Box bt1 = ub, ba1 = ub, bt2 = ub, ba2 = ub;
   if (AX == 0) {
     t1 = 1; va1 = 2; ga1 = 1;  // vx_by_flux
     t2 = 2; va2 = 1; ga2 = 2;  // vx_bz_flux
-    bt2 = mkbox(1, N0, 1, N1, 0, N2);
-    ba2 = mkbox(0, N0, 0, N1 + 1, 0, N2 + 1);
+    bt2 = mkbox(1, N[0], 1, N[1], 0, N[2]);
+    ba2 = mkbox(0, N[0], 0, N[1] + 1, 0, N[2] + 1);
   } else if (AX == 1) {
     t1 = 0; va1 = 2; ga1 = 0;  // vy_bx_flux
     t2 = 2; va2 = 0; ga2 = 2;  // vy_bz_flux
-    ba2 = mkbox(1, N0, 0, N1, 1, N2);
+    ba2 = mkbox(1, N[0], 0, N[1], 1, N[2]);
   } else {
     t1 = 0; va1 = 1; ga1 = 0;  // vz_bx_flux
     t2 = 1; va2 = 0; ga2 = 1;  // vz_by_flux
   }
+  const dim3 blk(CTS_THREADS, 1, 1);
   if constexpr (AX == 0) {
-    // x sweep: va1 = z and ga2 = z (see the flux table above); waves of 64
-    // quotient points over x0..n+2, CTX_OUT outputs each
-    const long npts = (long)(N0 + 2 - fb.x0 + 1) * (fb.y1 - fb.y0 + 1);
-    const long waves = (npts + CTX_OUT - 1) / CTX_OUT;
-    const dim3 gx((unsigned)((waves + BY - 1) / BY), 1, (unsigned)((fb.z1 - fb.z0 + 1 + ZCHUNK - 1) / ZCHUNK));
+    // x sweep: va1 = z and ga2 = z (see the flux table above); tiles of
+    // CTX_W points in x by CTX_ROWS - 1 rows in y, marching in z
+    const int ntx = (N[0] + 5 + CTX_W - 1) / CTX_W, nty = (N[1] + 5 + CTX_ROWS - 2) / (CTX_ROWS - 1);
+    const dim3 grid((unsigned)(ntx * nty), 1, (unsigned)((N[2] + 5 + CTS_CHUNK - 1) / CTS_CHUNK));
     if (sc > 0)
-      LAUNCH("ct_flux", k_ct_flux_x<1>, gx, box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1, (real *)S.F, (real *)S.F2,
-             strides(ga1), strides(va2), dt);
+      LAUNCH("ct_sweep", k_ct_sweep_x<1>, grid, blk, S.g, in[t1], in[t2], in[AX], out[t1], out[t2], out[AX], vel,
+             S.db1, strides(ga1), strides(va2), dt, ntx, bt1, ba1, bt2, ba2);
+    else if (sc < 0)
+      LAUNCH("ct_sweep", k_ct_sweep_x<-1>, grid, blk, S.g, in[t1], in[t2], in[AX], out[t1], out[t2], out[AX], vel,
+             S.db1, strides(ga1), strides(va2), dt, ntx, bt1, ba1, bt2, ba2);
     else
-      LAUNCH("ct_flux", k_ct_flux_x<0>, gx, box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1, (real *)S.F, (real *)S.F2,
-             strides(ga1), strides(va2), dt);
+      LAUNCH("ct_sweep", k_ct_sweep_x<0>, grid, blk, S.g, in[t1], in[t2], in[AX], out[t1], out[t2], out[AX], vel,
+             S.db1, strides(ga1), strides(va2), dt, ntx, bt1, ba1, bt2, ba2);
   } else {
-    // y and z sweeps march along the sweep axis; the grid covers the other two
-    const Box gb = march_box(fb, AX);
+    // y and z sweeps march along the sweep axis; tiles of CTM_XW - 1 points
+    // in x by CTM_ROWS - 1 rows along the other transverse axis
+    const int P = AX == 2 ? 1 : 2;
+    const int ntx = (N[0] + 5 + CTM_XW - 2) / (CTM_XW - 1), nty = (N[P] + 5 + CTM_ROWS - 2) / (CTM_ROWS - 1);
+    const dim3 grid((unsigned)(ntx * nty), 1, (unsigned)((N[AX] + 5 + CTS_CHUNK - 1) / CTS_CHUNK));
+    const dim3 blkm(CTM_THREADS, 1, 1);
     if (sc > 0)
-      LAUNCH("ct_flux", (k_ct_flux_m<AX, 1>), zchunk_grid(gb), box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1,
-             (real *)S.F, (real *)S.F2, strides(va1), strides(ga1), strides(va2), strides(ga2), dt);
+      LAUNCH("ct_sweep", (k_ct_sweep_m<AX, 1>), grid, blkm, S.g, in[t1], in[t2], in[AX], out[t1], out[t2], out[AX],
+             vel, S.db1, strides(va1), strides(ga1), strides(va2), strides(ga2), dt, ntx, bt1, ba1, bt2, ba2);
+    else if (sc < 0)
+      LAUNCH("ct_sweep", (k_ct_sweep_m<AX, -1>), grid, blkm, S.g, in[t1], in[t2], in[AX], out[t1], out[t2], out[AX],
+             vel, S.db1, strides(va1), strides(ga1), strides(va2), strides(ga2), dt, ntx, bt1, ba1, bt2, ba2);
     else
-      LAUNCH("ct_flux", (k_ct_flux_m<AX, 0>), zchunk_grid(gb), box_block(), S.g, fb, Bf[t1], Bf[t2], vel, S.db1,
-             (real *)S.F, (real *)S.F2, strides(va1), strides(ga1), strides(va2), strides(ga2), dt);
+      LAUNCH("ct_sweep", (k_ct_sweep_m<AX, 0>), grid, blkm, S.g, in[t1], in[t2], in[AX], out[t1], out[t2], out[AX],
+             vel, S.db1, strides(va1), strides(ga1), strides(va2), strides(ga2), dt, ntx, bt1, ba1, bt2, ba2);
   }
-  if (sc == 0)
-    LAUNCH_BOX("ct_update", (k_ct_update<AX, 0>), ub, (real *)S.F, (real *)S.F2, B[t1], B[t2], B[AX], B[t1], B[t2], B[AX],
-               strides(AX), strides(t1), strides(t2), bt1, ba1, bt2, ba2);
-  else if (sc > 0)
-    LAUNCH_BOX("ct_update", (k_ct_update<AX, 1>), full_box(), (real *)S.F, (real *)S.F2, Bin[t1], Bin[t2], Bin[AX], B[t1], B[t2],
-               B[AX], strides(AX), strides(t1), strides(t2), bt1, ba1, bt2, ba2);
-  else
-    LAUNCH_BOX("ct_update", (k_ct_update<AX, -1>), full_box(), (real *)S.F, (real *)S.F2, B[t1], B[t2], B[AX], B[t1], B[t2], B[AX],
-               strides(AX), strides(t1), strides(t2), bt1, ba1, bt2, ba2);
   return 0;
+}
+
+// the face arrays a and the spare EMF arrays swap roles (the CT output went
+// to the spares)
+void swap_spare(double *&ax, double *&ay, double *&az) {
+  double *t;
+  t = ax; ax = S.fxr; S.fxr = t;
+  t = ay; ay = S.fyr; S.fyr = t;
+  t = az; az = S.fzr; S.fzr = t;
 }
 
 // one predictor-mode sweep (B only, no flag reset, no halo refresh) of the
@@ -541,9 +549,18 @@ int sweep_predictor(double dt, bool first, bool last) {
     LAUNCH("geom", (k_geom<AX, 0>), zchunk_grid(cell_box()), box_block(), S.g, cell_box(), S.vx, S.vy, S.vz, S.cv1,
            S.cv2, S.db1, dt, 1.0, 1.0, 1.0, pr);
   const real *vel = AX == 0 ? S.vx : (AX == 1 ? S.vy : S.vz);
-  double *const Bin[3] = {S.bx, S.by, S.bz};
-  double *const Bp[3] = {S.bpx, S.bpy, S.bpz};
-  return ct_pair<AX>(Bin, Bp, vel, dt, first ? 1 : (last ? -1 : 0));
+  if (first) {
+    // B itself stays untouched: the first sweep writes the copy
+    const double *const Bin[3] = {S.bx, S.by, S.bz};
+    double *const Bp[3] = {S.bpx, S.bpy, S.bpz};
+    return ct_sweep<AX>(Bin, Bp, vel, dt, 1);
+  }
+  const double *const Bp[3] = {S.bpx, S.bpy, S.bpz};
+  double *const spare[3] = {S.fxr, S.fyr, S.fzr};
+  int rc = ct_sweep<AX>(Bp, spare, vel, dt, last ? -1 : 0);
+  if (rc) return rc;
+  swap_spare(S.bpx, S.bpy, S.bpz);
+  return 0;
 }
 
 template <int AX>
@@ -552,9 +569,14 @@ int sweep_full(double dt, double pass[3], bool first, bool last) {
   const real *vel = AX == 0 ? S.vx1 : (AX == 1 ? S.vy1 : S.vz1);
   LAUNCH("geom", (k_geom<AX, 2>), zchunk_grid(cell_box()), box_block(), S.g, cell_box(), S.vx1, S.vy1, S.vz1, S.cv1,
          S.cv2, S.db1, dt, pass[0], pass[1], pass[2], PredP{});
-  double *const B[3] = {S.bx, S.by, S.bz};
-  int rc = ct_pair<AX>(B, B, vel, dt, first ? 1 : (last ? -1 : 0));
-  if (rc) return rc;
+  {
+    const double *const B[3] = {S.bx, S.by, S.bz};
+    double *const spare[3] = {S.fxr, S.fyr, S.fzr};
+    int rc = ct_sweep<AX>(B, spare, vel, dt, first ? 1 : (last ? -1 : 0));
+    if (rc) return rc;
+    swap_spare(S.bx, S.by, S.bz);
+  }
+  int rc = 0;
   int hi[3] = {N[0] + 1, N[1] + 1, N[2] + 1};
   hi[AX] = N[AX];
   Box mb = mkbox(0, hi[0], 0, hi[1], 0, hi[2]);
