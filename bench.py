@@ -454,7 +454,7 @@ def main():
                          f"{args.cpu_steps} steps after one untimed step, {secs:.1f} s"}
 
     out = {
-        "metric": "cell-updates/s at 256^3 (FP64 plain vs FP32+surrogate), % of HBM peak",
+        "metric": "cell-updates/s at 256³ (FP64 plain vs FP32+surrogate), % of HBM peak",
         "value": cups,
         "unit": "cell-updates/s",
         "n_gpus": 1,

@@ -212,7 +212,6 @@ template <int AX> struct CtComp {
 #ifndef CTS_CHUNK
 #define CTS_CHUNK 16
 #endif
-#define CTS_THREADS 512
 
 __device__ __forceinline__ bool in_box(const Box &b, int i, int j, int k) {
   return i >= b.x0 && i <= b.x1 && j >= b.y0 && j <= b.y1 && k >= b.z0 && k <= b.z1;
@@ -396,9 +395,12 @@ __global__ __launch_bounds__(CTM_THREADS) void k_ct_sweep_m(Geo g, const double 
 // write, CTX_W = 60 points per row.  The inputs of each plane are loaded
 // one iteration ahead, before the barrier.
 #define CTX_W 60
-#define CTX_ROWS (CTS_THREADS / 64)
+#ifndef CTX_THREADS
+#define CTX_THREADS 512
+#endif
+#define CTX_ROWS (CTX_THREADS / 64)
 template <int SC>
-__global__ __launch_bounds__(CTS_THREADS) void k_ct_sweep_x(Geo g, const double *__restrict__ t1i,
+__global__ __launch_bounds__(CTX_THREADS) void k_ct_sweep_x(Geo g, const double *__restrict__ t1i,
                                                             const double *__restrict__ t2i,
                                                             const double *__restrict__ ai, double *__restrict__ t1o,
                                                             double *__restrict__ t2o, double *__restrict__ ao,

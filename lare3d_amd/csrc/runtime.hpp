@@ -492,7 +492,7 @@ int ct_sweep(const double *const in[3], double *const out[3], const real *vel, d
     t1 = 0; va1 = 1; ga1 = 0;  // vz_bx_flux
     t2 = 1; va2 = 0; ga2 = 1;  // vz_by_flux
   }
-  const dim3 blk(CTS_THREADS, 1, 1);
+  const dim3 blk(CTX_THREADS, 1, 1);
   if constexpr (AX == 0) {
     // x sweep: va1 = z and ga2 = z (see the flux table above); tiles of
     // CTX_W points in x by CTX_ROWS - 1 rows in y, marching in z
