@@ -210,7 +210,7 @@ template <int AX> struct CtComp {
 // subtracted from its neighbour as the same value, while B accumulates in
 // FP64, so div B stays at FP64 round-off.
 #ifndef CTS_CHUNK
-#define CTS_CHUNK 16
+#define CTS_CHUNK 8
 #endif
 
 __device__ __forceinline__ bool in_box(const Box &b, int i, int j, int k) {
@@ -259,14 +259,17 @@ __device__ __forceinline__ void ct_store(const Wid &wd, int c, int i, int j, int
 
 // y and z sweeps (M = 1, 2).  Rows of CTM_XW lanes along x, CTM_ROWS rows
 // along the other transverse axis P (which is t2's axis, so F2a comes from
-// the row below; t1 is bx, so F1a comes from the lane below).  Each thread
-// holds the four-point stencils of db1, of both fields' widths and of their
-// quotients field / width along the sweep in registers; a quotient is
-// formed once per point and serves as the s+2, s+1, s and s-1 value of four
-// successive fluxes, with the same operations as forming it at each use.
-// The loop is software-pipelined around the one barrier per point: the inputs of
-// the point's output are loaded first, and the stencil and velocities of
-// the next point are loaded before the barrier.
+// the row below; t1 is bx, so F1a comes from the lane below).  Two phases
+// per chunk of CTS_CHUNK points along the sweep:
+//  1. each thread marches its column and forms the fluxes of points m0-1 ..
+//     m1 into LDS, holding the four-point stencils of db1, of both fields'
+//     widths and of their quotients field / width in registers (a quotient
+//     is formed once per point and serves as the s+2, s+1, s and s-1 value of
+//     four successive fluxes, with the same operations as forming it at
+//     each use);
+//  2. after one barrier, each output thread writes its points m0 .. m1,
+//     taking the fluxes at the point, at s-1, at the lane below and at the
+//     row below from LDS.
 #define CTM_XW 32
 #ifndef CTM_THREADS
 #define CTM_THREADS 256
@@ -283,7 +286,8 @@ __global__ __launch_bounds__(CTM_THREADS) void k_ct_sweep_m(Geo g, const double 
                                                             Box ba2) {
   constexpr int T1 = CtComp<M>::T1, T2 = CtComp<M>::T2;
   constexpr int P = M == 2 ? 1 : 2;
-  __shared__ real xch[2][CTM_ROWS][CTM_XW];
+  // fluxes F1, F2 of points m0-1 .. m1 (slot 0 = m0-1)
+  __shared__ real fl[CTS_CHUNK + 1][2][CTM_ROWS][CTM_XW];
   const int xl = (int)threadIdx.x % CTM_XW, row = (int)threadIdx.x / CTM_XW;
   const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = (int)(tile % (unsigned)ntx), ty = (int)(tile / (unsigned)ntx);
@@ -303,97 +307,92 @@ __global__ __launch_bounds__(CTM_THREADS) void k_ct_sweep_m(Geo g, const double 
   wd.w[0] = i >= -2 && i <= nx + 2 ? g.dxb[i] : 1.0;
   wd.w[P] = p >= -2 && p <= nP + 2 ? (P == 1 ? g.dyb[p] : g.dzb[p]) : 1.0;
   wd.w[M] = 1.0;
-  double q1[4], q2[4];
-  acc w1[4], w2[4];
-  real d[4];
-  auto load = [&](int slot, int m) {
-    const int c = at(m);
-    d[slot] = db1[c];
-    w1[slot] = (d[slot] + db1[c + GS1]) * (acc)0.5;
-    w2[slot] = (d[slot] + db1[c + GS2]) * (acc)0.5;
-    double v1 = t1i[c], v2 = t2i[c];
-    if (SC > 0) {
-      Wid wm = wd;
-      wm.w[M] = dbm[m];
-      v1 = to_flux_w<T1>(v1, wm);
-      v2 = to_flux_w<T2>(v2, wm);
-    }
-    q1[slot] = v1 / (double)w1[slot];
-    q2[slot] = v2 / (double)w2[slot];
-  };
-  // sweep velocity at the flux point and its two averaging neighbours
-  real vc = 0, va = 0, vb = 0;
-  auto loadv = [&](int m) {
-    if (m >= -1 && m <= n + 1) {
+  {
+    double q1[4], q2[4];
+    acc w1[4], w2[4];
+    real d[4];
+    auto load = [&](int slot, int m) {
       const int c = at(m);
-      vc = vel[c];
-      va = vel[c - VS1];
-      vb = vel[c - VS2];
-    }
-  };
-  auto clampm = [&](int m) { return m < -1 ? -1 : (m > n + 2 ? n + 2 : m); };
-  // window over m-1, m, m+1, m+2 of the first flux point m0 - 1
-  if (col) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) load(q, clampm(m0 - 2 + q));
-    loadv(m0 - 1);
-  }
-  real F1s = 0, F2s = 0;
-  int buf = 0;
-  for (int m = m0 - 1; m <= m1; ++m) {
-    // the inputs of this point's output
-    const bool o = outc && m >= m0;
-    double r1 = 0, r2 = 0, ra = 0;
-    if (o) {
-      const int c = at(m);
-      r1 = t1i[c];
-      r2 = t2i[c];
-      ra = ai[c];
-      if (SC != 0) wd.w[M] = dbm[m];
-    }
-    real F1 = 0, F2 = 0;
-    if (col && m >= -1 && m <= n + 1) {
-      const acc dcm = (acc)0.5 * (d[0] + (m > -1 ? d[1] : d[2]));
-      const acc dci = (acc)0.5 * (d[1] + d[2]);
-      const acc dcp = (acc)0.5 * (d[2] + d[3]);
-      F1 = (real)ct_flux_q((vc + va) * 0.5, q1[0], q1[1], q1[2], q1[3], w1[1], w1[2], d[1], d[2], dcm, dci, dcp, dt);
-      F2 = (real)ct_flux_q((vc + vb) * 0.5, q2[0], q2[1], q2[2], q2[3], w2[1], w2[2], d[1], d[2], dcm, dci, dcp, dt);
-    }
-    // the next point's stencil and velocities
-    if (col && m < m1) {
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        q1[q] = q1[q + 1];
-        q2[q] = q2[q + 1];
-        w1[q] = w1[q + 1];
-        w2[q] = w2[q + 1];
-        d[q] = d[q + 1];
+      d[slot] = db1[c];
+      w1[slot] = (d[slot] + db1[c + GS1]) * (acc)0.5;
+      w2[slot] = (d[slot] + db1[c + GS2]) * (acc)0.5;
+      double v1 = t1i[c], v2 = t2i[c];
+      if (SC > 0) {
+        Wid wm = wd;
+        wm.w[M] = dbm[m];
+        v1 = to_flux_w<T1>(v1, wm);
+        v2 = to_flux_w<T2>(v2, wm);
       }
-      load(3, clampm(m + 3));
-      loadv(m + 1);
+      q1[slot] = v1 / (double)w1[slot];
+      q2[slot] = v2 / (double)w2[slot];
+    };
+    // sweep velocity at the flux point and its two averaging neighbours
+    real vc = 0, va = 0, vb = 0;
+    auto loadv = [&](int m) {
+      if (m >= -1 && m <= n + 1) {
+        const int c = at(m);
+        vc = vel[c];
+        va = vel[c - VS1];
+        vb = vel[c - VS2];
+      }
+    };
+    auto clampm = [&](int m) { return m < -1 ? -1 : (m > n + 2 ? n + 2 : m); };
+    // window over m-1, m, m+1, m+2 of the first flux point m0 - 1
+    if (col) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) load(q, clampm(m0 - 2 + q));
+      loadv(m0 - 1);
     }
-    xch[buf][row][xl] = F2;
-    __syncthreads();
-    const real F2a = xch[buf][row > 0 ? row - 1 : 0][xl];
-    const real F1a = __shfl_up(F1, 1, CTM_XW);
-    if (o) {
-      const int jj = M == 1 ? m : p, kk = M == 2 ? m : p;
-      ct_store<M, SC>(wd, at(m), i, jj, kk, r1, r2, ra, t1o, t2o, ao, F1, F1s, F1a, F2, F2s, F2a, bt1, ba1, bt2, ba2);
+    for (int m = m0 - 1; m <= m1; ++m) {
+      real F1 = 0, F2 = 0;
+      if (col && m >= -1 && m <= n + 1) {
+        const acc dcm = (acc)0.5 * (d[0] + (m > -1 ? d[1] : d[2]));
+        const acc dci = (acc)0.5 * (d[1] + d[2]);
+        const acc dcp = (acc)0.5 * (d[2] + d[3]);
+        F1 = (real)ct_flux_q((vc + va) * 0.5, q1[0], q1[1], q1[2], q1[3], w1[1], w1[2], d[1], d[2], dcm, dci, dcp,
+                             dt);
+        F2 = (real)ct_flux_q((vc + vb) * 0.5, q2[0], q2[1], q2[2], q2[3], w2[1], w2[2], d[1], d[2], dcm, dci, dcp,
+                             dt);
+      }
+      fl[m - m0 + 1][0][row][xl] = F1;
+      fl[m - m0 + 1][1][row][xl] = F2;
+      if (col && m < m1) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          q1[q] = q1[q + 1];
+          q2[q] = q2[q + 1];
+          w1[q] = w1[q + 1];
+          w2[q] = w2[q + 1];
+          d[q] = d[q + 1];
+        }
+        load(3, clampm(m + 3));
+        loadv(m + 1);
+      }
     }
-    F1s = F1;
-    F2s = F2;
-    buf ^= 1;
+  }
+  __syncthreads();
+  if (!outc) return;
+#pragma unroll 4
+  for (int m = m0; m <= m1; ++m) {
+    const int c = at(m);
+    const int s = m - m0 + 1;
+    const double r1 = t1i[c], r2 = t2i[c], ra = ai[c];
+    if (SC != 0) wd.w[M] = dbm[m];
+    const int jj = M == 1 ? m : p, kk = M == 2 ? m : p;
+    ct_store<M, SC>(wd, c, i, jj, kk, r1, r2, ra, t1o, t2o, ao, fl[s][0][row][xl], fl[s - 1][0][row][xl],
+                    fl[s][0][row][xl - 1], fl[s][1][row][xl], fl[s - 1][1][row][xl], fl[s][1][row - 1][xl], bt1, ba1,
+                    bt2, ba2);
   }
 }
 
 // x sweep.  Rows of 64 lanes along x (the sweep axis: every lane forms the
 // quotients of both fields at its own point, and the stencil values at s-1,
-// s+1, s+2 and the neighbour flux along the sweep come from other lanes by
-// wavefront shuffles, so each quotient is divided once), CTX_ROWS rows along y (t1 is by, so F1a comes from the
-// row below), marching in z (t2 is bz, so F2a is the previous plane's).
-// Lanes 1..61 form fluxes from the quotients of lanes 0..63; lanes 2..61
-// write, CTX_W = 60 points per row.  The inputs of each plane are loaded
-// one iteration ahead, before the barrier.
+// s+1, s+2 come from other lanes by wavefront shuffles, so each quotient is
+// divided once), CTX_ROWS rows along y (t1 is by, so F1a comes from the row
+// below), marching in z (t2 is bz, so F2a is the previous plane's).  Lanes
+// 1..61 form fluxes from the quotients of lanes 0..63; lanes 2..61 write,
+// CTX_W = 60 points per row.  The same two phases as the y and z sweeps:
+// the fluxes of planes k0-1 .. k1 into LDS, one barrier, the outputs.
 #define CTX_W 60
 #ifndef CTX_THREADS
 #define CTX_THREADS 512
@@ -408,7 +407,7 @@ __global__ __launch_bounds__(CTX_THREADS) void k_ct_sweep_x(Geo g, const double 
                                                             const real *__restrict__ db1, int GS1, int VS2, double dt,
                                                             int ntx, Box bt1, Box ba1, Box bt2, Box ba2) {
   constexpr int T1 = CtComp<0>::T1, T2 = CtComp<0>::T2;
-  __shared__ real xch[2][CTX_ROWS][64];
+  __shared__ real fl[CTS_CHUNK + 1][2][CTX_ROWS][64];
   const int lane = (int)threadIdx.x & 63, row = (int)threadIdx.x >> 6;
   const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = (int)(tile % (unsigned)ntx), ty = (int)(tile / (unsigned)ntx);
@@ -429,85 +428,86 @@ __global__ __launch_bounds__(CTX_THREADS) void k_ct_sweep_x(Geo g, const double 
   wd.w[0] = i >= -2 && i <= nx + 2 ? g.dxb[i] : 1.0;
   wd.w[1] = j >= -2 && j <= ny + 2 ? g.dyb[j] : 1.0;
   wd.w[2] = 1.0;
-  // inputs of plane k, fetched one iteration ahead
-  real fd = 0, fg = 0, fz = 0, fv = 0, fv2 = 0, fvz = 0;
-  double f1 = 0, f2 = 0, fa = 0;
-  auto fetch = [&](int k) {
+  {
+    // inputs of plane k, fetched one iteration ahead
+    real fd = 0, fg = 0, fz = 0, fv = 0, fv2 = 0, fvz = 0;
+    double f1 = 0, f2 = 0;
+    auto fetch = [&](int k) {
+      const int c = c0 + k * Z;
+      if (qrow && k >= -1 && k <= nz + 1) {
+        fd = db1[c];
+        fg = db1[c + GS1];
+        fz = db1[c + Z];
+        fv = vel[c];
+        fv2 = vel[c - VS2];
+        if (k == -1 || k == k0 - 1) fvz = vel[c - Z];
+        f1 = t1i[c];
+        f2 = t2i[c];
+      }
+    };
+    fetch(k0 - 1);
+    real vm = 0;
+    for (int k = k0 - 1; k <= k1; ++k) {
+      const bool qk = qrow && k >= -1 && k <= nz + 1;
+      // this plane's inputs; then the next plane's loads go out
+      const real d = fd, dg = fg, dz = fz, vc0 = fv, v2 = fv2, vz = fvz;
+      const double r1 = f1, r2 = f2;
+      if (k < k1) fetch(k + 1);
+      if (SC > 0 && k >= -2) wd.w[2] = g.dzb[k];
+      double q1 = 0, q2 = 0;
+      acc w1 = 0, w2 = 0, dd = 0;
+      real vc = 0;
+      if (qk) {
+        w1 = (d + dg) * (acc)0.5;
+        w2 = (d + dz) * (acc)0.5;
+        q1 = (SC > 0 ? to_flux_w<T1>(r1, wd) : r1) / (double)w1;
+        q2 = (SC > 0 ? to_flux_w<T2>(r2, wd) : r2) / (double)w2;
+        dd = d;
+        vc = vc0;
+        if (k == -1 || k == k0 - 1) vm = vz;
+      }
+      // neighbours along x; s-1 is clamped to -1 and s+2 to n+2
+      double q1m = __shfl_up(q1, 1, 64), q2m = __shfl_up(q2, 1, 64);
+      acc dm = __shfl_up(dd, 1, 64);
+      const double q1p = __shfl_down(q1, 1, 64), q2p = __shfl_down(q2, 1, 64);
+      const acc dp = __shfl_down(dd, 1, 64);
+      double q1q = __shfl_down(q1, 2, 64), q2q = __shfl_down(q2, 2, 64);
+      acc dq = __shfl_down(dd, 2, 64);
+      const acc w1p = __shfl_down(w1, 1, 64), w2p = __shfl_down(w2, 1, 64);
+      if (i - 1 < -1) {
+        q1m = q1;
+        q2m = q2;
+        dm = dd;
+      }
+      if (i + 2 > nx + 2) {
+        q1q = q1p;
+        q2q = q2p;
+        dq = dp;
+      }
+      real F1 = 0, F2 = 0;
+      if (frow && qk) {
+        const acc dcm = (acc)0.5 * (dm + (i > -1 ? dd : dp));
+        const acc dci = (acc)0.5 * (dd + dp);
+        const acc dcp = (acc)0.5 * (dp + dq);
+        F1 = (real)ct_flux_q((vc + vm) * 0.5, q1m, q1, q1p, q1q, w1, w1p, dd, dp, dcm, dci, dcp, dt);
+        F2 = (real)ct_flux_q((vc + v2) * 0.5, q2m, q2, q2p, q2q, w2, w2p, dd, dp, dcm, dci, dcp, dt);
+      }
+      vm = vc;
+      fl[k - k0 + 1][0][row][lane] = F1;
+      fl[k - k0 + 1][1][row][lane] = F2;
+    }
+  }
+  __syncthreads();
+  if (!outc) return;
+#pragma unroll 4
+  for (int k = k0; k <= k1; ++k) {
     const int c = c0 + k * Z;
-    if (qrow && k >= -1 && k <= nz + 1) {
-      fd = db1[c];
-      fg = db1[c + GS1];
-      fz = db1[c + Z];
-      fv = vel[c];
-      fv2 = vel[c - VS2];
-      if (k == -1 || k == k0 - 1) fvz = vel[c - Z];
-      f1 = t1i[c];
-      f2 = t2i[c];
-    } else if (outc && k >= k0) {
-      f1 = t1i[c];
-      f2 = t2i[c];
-    }
-    if (outc && k >= k0) fa = ai[c];
-  };
-  fetch(k0 - 1);
-  real F2z = 0;  // F2 of the previous plane
-  real vm = 0;
-  int buf = 0;
-  for (int k = k0 - 1; k <= k1; ++k) {
-    const bool qk = qrow && k >= -1 && k <= nz + 1;
-    const int c = c0 + k * Z;
-    // this plane's inputs; then the next plane's loads go out
-    const real d = fd, dg = fg, dz = fz, vc0 = fv, v2 = fv2, vz = fvz;
-    const double r1 = f1, r2 = f2, ra = fa;
-    if (k < k1) fetch(k + 1);
-    if (SC != 0 && k >= -2) wd.w[2] = g.dzb[k];
-    double q1 = 0, q2 = 0;
-    acc w1 = 0, w2 = 0, dd = 0;
-    real vc = 0;
-    if (qk) {
-      w1 = (d + dg) * (acc)0.5;
-      w2 = (d + dz) * (acc)0.5;
-      q1 = (SC > 0 ? to_flux_w<T1>(r1, wd) : r1) / (double)w1;
-      q2 = (SC > 0 ? to_flux_w<T2>(r2, wd) : r2) / (double)w2;
-      dd = d;
-      vc = vc0;
-      if (k == -1 || k == k0 - 1) vm = vz;
-    }
-    // neighbours along x; s-1 is clamped to -1 and s+2 to n+2
-    double q1m = __shfl_up(q1, 1, 64), q2m = __shfl_up(q2, 1, 64);
-    acc dm = __shfl_up(dd, 1, 64);
-    const double q1p = __shfl_down(q1, 1, 64), q2p = __shfl_down(q2, 1, 64);
-    const acc dp = __shfl_down(dd, 1, 64);
-    double q1q = __shfl_down(q1, 2, 64), q2q = __shfl_down(q2, 2, 64);
-    acc dq = __shfl_down(dd, 2, 64);
-    const acc w1p = __shfl_down(w1, 1, 64), w2p = __shfl_down(w2, 1, 64);
-    if (i - 1 < -1) {
-      q1m = q1;
-      q2m = q2;
-      dm = dd;
-    }
-    if (i + 2 > nx + 2) {
-      q1q = q1p;
-      q2q = q2p;
-      dq = dp;
-    }
-    real F1 = 0, F2 = 0;
-    if (frow && qk) {
-      const acc dcm = (acc)0.5 * (dm + (i > -1 ? dd : dp));
-      const acc dci = (acc)0.5 * (dd + dp);
-      const acc dcp = (acc)0.5 * (dp + dq);
-      F1 = (real)ct_flux_q((vc + vm) * 0.5, q1m, q1, q1p, q1q, w1, w1p, dd, dp, dcm, dci, dcp, dt);
-      F2 = (real)ct_flux_q((vc + v2) * 0.5, q2m, q2, q2p, q2q, w2, w2p, dd, dp, dcm, dci, dcp, dt);
-    }
-    vm = vc;
-    xch[buf][row][lane] = F1;
-    __syncthreads();
-    const real F1a = xch[buf][row > 0 ? row - 1 : 0][lane];
-    const real F1x = __shfl_up(F1, 1, 64), F2x = __shfl_up(F2, 1, 64);
-    if (outc && k >= k0)
-      ct_store<0, SC>(wd, c, i, j, k, r1, r2, ra, t1o, t2o, ao, F1, F1x, F1a, F2, F2x, F2z, bt1, ba1, bt2, ba2);
-    F2z = F2;
-    buf ^= 1;
+    const int s = k - k0 + 1;
+    const double r1 = t1i[c], r2 = t2i[c], ra = ai[c];
+    if (SC != 0) wd.w[2] = g.dzb[k];
+    ct_store<0, SC>(wd, c, i, j, k, r1, r2, ra, t1o, t2o, ao, fl[s][0][row][lane], fl[s][0][row][lane - 1],
+                    fl[s][0][row - 1][lane], fl[s][1][row][lane], fl[s][1][row][lane - 1], fl[s - 1][1][row][lane],
+                    bt1, ba1, bt2, ba2);
   }
 }
 
