@@ -199,12 +199,12 @@ template <int AX> struct CtComp {
 //   t1 = t1 - F1 + F1(s-1), t2 = t2 - F2 + F2(s-1) along the sweep;
 //   a = a + F1 - F1(c - AS1) + F2 - F2(c - AS2) across the transverse axes.
 // A block is a tile of the plane across the march axis, rows of lanes along
-// x, marching for CTS_CHUNK points; it forms the fluxes from the point
-// before its first one, so the neighbour along the march axis is always the
-// previous iteration's, the neighbour along x comes from the lane below by
-// a shuffle and the neighbour along the row axis from the row below through
-// LDS.  Lane 0 of each row segment and row 0 of the block are halos: they
-// form fluxes but write nothing.  Every point of the padded box -2..n+2 is
+// x, covering a chunk of CTS_CHUNK points along the march.  It first forms
+// the fluxes of the chunk and of the point before it into LDS, then, after
+// one barrier, writes its points, taking each flux and its neighbours along
+// the march (the previous point), along x (the lane below) and along the
+// row axis (the row below) from LDS.  Lane 0 of each row segment and row 0
+// of the block are halos: they form fluxes but write nothing.  Every point of the padded box -2..n+2 is
 // written (copied where no update applies).  The fluxes are rounded to the
 // storage type (FP32 in the FP32 mode) and each is added to one face and
 // subtracted from its neighbour as the same value, while B accumulates in
