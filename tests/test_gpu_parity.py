@@ -149,6 +149,12 @@ CASES = {
     # waves per row of the x-sweep kernels, so neighbours cross tile and wave
     # edges inside the row
     "harris_wide": lambda: configs.harris(64, ny=9, nz=6),
+    # the CT sweep's tiles: 58 + 5 = 2 x 31 + 1 points across x in the y and
+    # z sweeps, 10 + 5 = 2 x 7 + 1 rows and 12 + 5 = 2 x 8 + 1 points along
+    # the march (a last tile or chunk of one point); 56 + 5 = 60 + 1 points
+    # across x in the x sweep
+    "harris_tile_edges": lambda: configs.harris(58, ny=10, nz=12),
+    "harris_tile_edges_x": lambda: configs.harris(56, ny=16, nz=20),
     "arcade": lambda: configs.arcade_conduction(16, 16, 24, s_target=7),
     "arcade_odd": lambda: configs.arcade_conduction(13, 11, 19, s_target=5),
     # the twisted flux tube (walls in x and y, periodic z; resistive with
